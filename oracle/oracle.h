@@ -1,0 +1,100 @@
+/*
+ * oracle.h -- CPU restatement of the lsdtt3 LEM-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (lsdtt3_amd/, include/)
+ * links, loads or calls this code.  It is the checker the GPU path is compared
+ * against: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg are
+ * its only users.
+ *
+ * Every function names the reference file:line whose behaviour it restates.
+ * Layout conventions are the reference's: row-major flat index row*n_cols+col,
+ * int64 receivers with -1 = nodata and -2 = outlet, uint8 direction codes
+ * 0..7 clockwise from N with 255 = none, boundary string N,E,S,W of o/n/p/b
+ * (src/core/lsdtt_types.hpp:142-144, src/core/lsdtt_flow_base.hpp:265-271).
+ *
+ * Parity status per function is listed in the header comment of each .c file.
+ */
+#ifndef LSDTT_ORACLE_H
+#define LSDTT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_OK = 0, ORC_ERR_ARG = 1, ORC_ERR_UNRESOLVED = 2, ORC_ERR_CYCLIC = 3,
+       ORC_ERR_NO_OUTLETS = 4, ORC_ERR_BC = 5 };
+
+/* ---- flow routing (oracle/orc_flow.c) --------------------------------- */
+
+/* D8 receivers.  z is double (float inputs are widened exactly by the caller,
+ * which reproduces the reference's double(z) - double(z_nb)).  forced may be
+ * NULL (no caller-nominated outlets).  On ORC_ERR_UNRESOLVED *fail_rc holds
+ * row*n_cols+col of the first failing cell in row-major order. */
+int orc_d8_receivers(int nr, int nc, double dx, const double* z,
+                     const uint8_t* mask, const char* bc,
+                     int interior_nodata_is_outlet, int allow_pits_as_outlets,
+                     const uint8_t* forced, int64_t* receiver, uint8_t* dir,
+                     int64_t* fail_rc);
+
+/* donor counts and row-major outlet list; returns number of outlets or <0 */
+int64_t orc_donors_outlets(int nr, int nc, const uint8_t* mask,
+                           const int64_t* receiver, uint8_t* donors,
+                           int64_t* outlets);
+
+/* Kahn FIFO topological order; returns length (== n_valid when acyclic) */
+int64_t orc_topo_kahn(int nr, int nc, const uint8_t* mask,
+                      const int64_t* receiver, const uint8_t* donors,
+                      int64_t* topo);
+
+/* s-vector by preorder DFS; s_start (-1 = none) and s_size (0 = none) are
+ * full-grid; returns s-vector length or <0 on error */
+int64_t orc_s_vector(int nr, int nc, const uint8_t* mask, const char* bc,
+                     const int64_t* receiver, const int64_t* outlets,
+                     int64_t n_outlets, int64_t* s_vector, int64_t* s_start,
+                     uint64_t* s_size);
+
+/* double accumulation in topo order (weights NULL -> dx^2 per valid cell) */
+void orc_accumulate(int64_t n, const uint8_t* mask, const int64_t* receiver,
+                    const int64_t* topo, int64_t n_topo, const double* weights,
+                    double dx, double* accum);
+
+/* Kahn-free int64 upstream cell counts (subtree sizes incl. self) */
+void orc_upstream_counts(int64_t n, const uint8_t* mask,
+                         const int64_t* receiver, const int64_t* topo,
+                         int64_t n_topo, int64_t* counts);
+
+/* 64-bit FNV-1a continuation over n bytes */
+uint64_t orc_fnv1a(uint64_t h, const void* p, int64_t n);
+
+/* ---- depression filling (oracle/orc_fill.c) ---------------------------- */
+
+/* epsilon-free priority-flood fill.  valid[i] = mask!=0 && z!=nodata.
+ * Output: filled (nodata value for invalid cells). */
+int orc_fill_priority_flood(int nr, int nc, const float* z,
+                            const uint8_t* mask, double nodata,
+                            const char* bc, float* filled);
+
+/* ---- stream power (oracle/orc_stream_power.c) -------------------------- */
+
+typedef struct {
+    double K, m, n, dt, dx;
+    double threshold;          /* channel threshold on area/dx (0 = all) */
+    int    divide_area_by_dx;  /* default 1 */
+    double slope_clamp;        /* default 1e-5 */
+} orc_sp_params;
+
+/* double-state D8 implicit stream power, in place on z (z_old is the
+ * pre-incision surface, area in m^2 per cell). Returns cells eroded. */
+int64_t orc_stream_power_d8(int64_t n, const uint8_t* mask,
+                            const int64_t* receiver, const uint8_t* dir,
+                            const int64_t* topo, int64_t n_topo,
+                            const double* area, const double* K_field,
+                            const orc_sp_params* p, const double* z_old,
+                            double* z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,159 @@
+"""ctypes front end of the CPU oracle (oracle/liblsdtt_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the lsdtt3_amd package.  Each wrapper
+names the C function it calls; the C sources cite the reference lines they
+restate.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblsdtt_oracle.so")
+_lib = None
+
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_d8_receivers.argtypes = [C.c_int, C.c_int, C.c_double, f64p, u8p,
+                                       C.c_char_p, C.c_int, C.c_int, C.c_void_p,
+                                       i64p, u8p, C.POINTER(C.c_int64)]
+        L.orc_d8_receivers.restype = C.c_int
+        L.orc_donors_outlets.argtypes = [C.c_int, C.c_int, u8p, i64p, u8p, i64p]
+        L.orc_donors_outlets.restype = C.c_int64
+        L.orc_topo_kahn.argtypes = [C.c_int, C.c_int, u8p, i64p, u8p, i64p]
+        L.orc_topo_kahn.restype = C.c_int64
+        L.orc_s_vector.argtypes = [C.c_int, C.c_int, u8p, C.c_char_p, i64p, i64p,
+                                   C.c_int64, i64p, i64p, u64p]
+        L.orc_s_vector.restype = C.c_int64
+        L.orc_accumulate.argtypes = [C.c_int64, u8p, i64p, i64p, C.c_int64,
+                                     C.c_void_p, C.c_double, f64p]
+        L.orc_accumulate.restype = None
+        L.orc_upstream_counts.argtypes = [C.c_int64, u8p, i64p, i64p, C.c_int64, i64p]
+        L.orc_upstream_counts.restype = None
+        L.orc_fill_priority_flood.argtypes = [C.c_int, C.c_int, f32p, u8p,
+                                              C.c_double, C.c_char_p, f32p]
+        L.orc_fill_priority_flood.restype = C.c_int
+        L.orc_fnv1a.argtypes = [C.c_uint64, C.c_void_p, C.c_int64]
+        L.orc_fnv1a.restype = C.c_uint64
+        _lib = L
+    return _lib
+
+
+def fnv1a(h: int, data: bytes) -> int:
+    """continue a 64-bit FNV-1a hash over `data`"""
+    return lib().orc_fnv1a(h, data, len(data))
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def d8_receivers(z, mask, dx, bc="oooo", interior_nodata_is_outlet=True,
+                 allow_pits_as_outlets=False, forced=None):
+    """orc_d8_receivers; z is float32 or float64, widened exactly to double."""
+    nr, nc = z.shape
+    zd = np.ascontiguousarray(z, dtype=np.float64)
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    rcv = np.empty(nr * nc, np.int64)
+    d = np.empty(nr * nc, np.uint8)
+    fail = C.c_int64(-1)
+    fplane = None
+    fptr = None
+    if forced is not None and len(forced):
+        fplane = np.zeros(nr * nc, np.uint8)
+        fplane[np.asarray(forced, np.int64)] = 1
+        fptr = fplane.ctypes.data_as(C.c_void_p)
+    st = lib().orc_d8_receivers(nr, nc, float(dx), zd.ravel(), m.ravel(),
+                                bc.encode(), int(interior_nodata_is_outlet),
+                                int(allow_pits_as_outlets), fptr, rcv, d,
+                                C.byref(fail))
+    if st != 0:
+        f = fail.value
+        raise OracleError(f"could not assign a receiver at row {f // nc}, col {f % nc}")
+    return rcv, d
+
+
+def flow_build(z, mask, dx, bc="oooo", interior_nodata_is_outlet=True,
+               allow_pits_as_outlets=False, forced=None, topo=True, svector=True):
+    """Full D8 flow object: dict of receiver, dir, donors, outlets, topo,
+    s_vector, s_start, s_size (absent parts are None)."""
+    nr, nc = z.shape
+    n = nr * nc
+    m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+    rcv, d = d8_receivers(z, mask, dx, bc, interior_nodata_is_outlet,
+                          allow_pits_as_outlets, forced)
+    donors = np.empty(n, np.uint8)
+    outlets = np.empty(max(n, 1), np.int64)
+    no = lib().orc_donors_outlets(nr, nc, m, rcv, donors, outlets)
+    if no < 0:
+        raise OracleError("donor count failure")
+    out = dict(receiver=rcv, dir=d, donors=donors, outlets=outlets[:no].copy(),
+               topo=None, s_vector=None, s_start=None, s_size=None)
+    if topo or svector:
+        t = np.empty(max(n, 1), np.int64)
+        nt = lib().orc_topo_kahn(nr, nc, m, rcv, donors, t)
+        if nt != int(m.sum()):
+            raise OracleError("routing graph is cyclic or incomplete")
+        out["topo"] = t[:nt].copy()
+    if svector:
+        sv = np.empty(max(n, 1), np.int64)
+        ss = np.empty(n, np.int64)
+        sz = np.empty(n, np.uint64)
+        ns = lib().orc_s_vector(nr, nc, m, bc.encode(), rcv, out["outlets"],
+                                len(out["outlets"]), sv, ss, sz)
+        if ns < 0:
+            raise OracleError("s-vector failure")
+        out["s_vector"], out["s_start"], out["s_size"] = sv[:ns].copy(), ss, sz
+    return out
+
+
+def accumulate(mask, receiver, topo, dx, weights=None):
+    m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+    n = m.size
+    acc = np.empty(n, np.float64)
+    wptr = None
+    if weights is not None:
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        wptr = w.ctypes.data_as(C.c_void_p)
+    lib().orc_accumulate(n, m, receiver, topo, len(topo), wptr, float(dx), acc)
+    return acc
+
+
+def upstream_counts(mask, receiver, topo):
+    m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+    cnt = np.empty(m.size, np.int64)
+    lib().orc_upstream_counts(m.size, m, receiver, topo, len(topo), cnt)
+    return cnt
+
+
+def fill(z, mask, nodata=-9999.0, bc="oooo"):
+    """orc_fill_priority_flood: epsilon-free spill-level fill (float32)."""
+    nr, nc = z.shape
+    zf = np.ascontiguousarray(z, dtype=np.float32)
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    out = np.empty_like(zf)
+    st = lib().orc_fill_priority_flood(nr, nc, zf.ravel(), m.ravel(),
+                                       float(nodata), bc.encode(), out.ravel())
+    if st != 0:
+        raise OracleError(f"fill failed (status {st}, bc {bc!r})")
+    return out
