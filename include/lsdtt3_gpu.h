@@ -1,0 +1,209 @@
+/*
+ * lsdtt3_gpu.h -- C ABI of the MI355X LEM-timestep library (liblsdtt3_gpu.so).
+ *
+ * Plain pointers and sizes only.  Layout conventions are the reference's:
+ * row-major flat index row*n_cols+col (int64 at this boundary), receiver -1 =
+ * nodata and -2 = outlet, D8 direction codes 0..7 clockwise from N with 255 =
+ * none, boundary string N,E,S,W of 'o' (open), 'n' (no flux), 'p'
+ * (periodic), 'b' (base level) -- src/core/lsdtt_types.hpp:142-144,
+ * src/core/lsdtt_flow_base.hpp:79-111 and :265-271.
+ *
+ * Error behaviour follows the reference's GPU dispatch rule
+ * (docs/developer_notes/gpu_compute.md:51-62, src/gpu/lsdtt_surface_gpu.hpp:
+ * 19-29): every call returns a status; on a non-zero status the outputs are
+ * unspecified and the caller falls back to its CPU path.  No C++ exception
+ * crosses this boundary; lsdtt_gpu_last_error() gives the message.
+ *
+ * Host-pointer entry points copy in and out.  *_device entry points take
+ * device pointers and a hipStream_t (as void*) and leave results resident.
+ */
+#ifndef LSDTT3_GPU_H
+#define LSDTT3_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSDTT_GPU_OK 0
+#define LSDTT_GPU_ERR_ARG 1          /* std::invalid_argument in the reference */
+#define LSDTT_GPU_ERR_RUNTIME 2      /* HIP or allocation failure */
+#define LSDTT_GPU_ERR_UNRESOLVED 3   /* "could not assign a receiver" */
+#define LSDTT_GPU_ERR_NO_DEVICE 4
+#define LSDTT_GPU_ERR_NOT_CONVERGED 5
+
+/* 1 when a usable GPU is present.  Replaces lsdtt::gpu_available()
+ * (src/core/lsdtt_backend_factory.cpp:33). */
+int lsdtt_gpu_available(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* lsdtt_gpu_last_error(void);
+
+/* ---------------------------------------------------------------------
+ * D8 flow object.  Replaces constructing lsdtt_flowinfo
+ * (src/cpu/lsdtt_flowinfo_cpu.cpp:183-253, options
+ * src/core/lsdtt_flow_base.hpp:79-111) and its array queries
+ * (receiver_of_unchecked, receiver_dir_of_unchecked,
+ * donor_count_of_unchecked, outlets, topological_order, s-vector and
+ * s_vector_start/size_unchecked, accumulate, get_drainage_area_raster;
+ * src/core/lsdtt_flow_base.hpp:126-491).
+ * ------------------------------------------------------------------- */
+typedef struct lsdtt_gpu_flow lsdtt_gpu_flow;
+
+typedef struct {
+    int32_t n_rows;
+    int32_t n_cols;
+    double cell_size;
+    char boundary_conditions[4]; /* N E S W */
+    int32_t interior_nodata_is_outlet;  /* default 1 */
+    int32_t allow_pits_as_outlets;      /* default 0; the LEM uses 1 */
+    int32_t build_topological_order;    /* default 1 */
+    int32_t build_s_vector;             /* default 1 */
+} lsdtt_gpu_flow_options;
+
+/* Sets the reference's defaults (oooo, 1, 0, 1, 1). */
+void lsdtt_gpu_flow_options_init(lsdtt_gpu_flow_options* o, int32_t n_rows, int32_t n_cols,
+                                 double cell_size);
+
+lsdtt_gpu_flow* lsdtt_gpu_flow_create(const lsdtt_gpu_flow_options* o);
+void lsdtt_gpu_flow_destroy(lsdtt_gpu_flow* f);
+
+/* Build from host arrays.  z is float32 (the reference's raster type) or
+ * float64 (the LEM's state); mask is uint8 (0 = nodata); forced outlets are
+ * flat indices (may be NULL / 0).  Out-of-range or masked forced indices
+ * give LSDTT_GPU_ERR_ARG, an unroutable cell LSDTT_GPU_ERR_UNRESOLVED. */
+int lsdtt_gpu_flow_build_f32(lsdtt_gpu_flow* f, const float* z, const uint8_t* mask,
+                             const int64_t* forced_outlets, int64_t n_forced);
+int lsdtt_gpu_flow_build_f64(lsdtt_gpu_flow* f, const double* z, const uint8_t* mask,
+                             const int64_t* forced_outlets, int64_t n_forced);
+
+/* Build from device arrays already resident in HBM (no host copies); the
+ * work is enqueued on `stream` (a hipStream_t, NULL = default stream). */
+int lsdtt_gpu_flow_build_device(lsdtt_gpu_flow* f, const void* d_z, int32_t z_is_f64,
+                                const uint8_t* d_mask, void* stream);
+
+/* Sizes of the built object. */
+int lsdtt_gpu_flow_sizes(const lsdtt_gpu_flow* f, int64_t* n_valid, int64_t* n_outlets,
+                         int64_t* max_depth);
+
+/* Copy results to host arrays (any pointer may be NULL to skip it):
+ * receiver/start int64[n], dir/donors uint8[n], size uint64[n],
+ * outlets int64[n_outlets], topo and s_vector int64[n_valid]. */
+int lsdtt_gpu_flow_get(lsdtt_gpu_flow* f, int64_t* receiver, uint8_t* dir, uint8_t* donors,
+                       int64_t* outlets, int64_t* topo, int64_t* s_vector, int64_t* s_start,
+                       uint64_t* s_size);
+
+/* Drainage area raster: float(n_contributing * cell_size^2) on valid cells,
+ * nodata elsewhere (get_drainage_area_raster, lsdtt_flowinfo_cpu.cpp:
+ * 2095-2150); int64 upstream counts may be requested as well (NULL skips). */
+int lsdtt_gpu_flow_area(lsdtt_gpu_flow* f, double nodata, float* area, uint8_t* area_mask,
+                        int64_t* counts);
+
+/* accumulate(weights) in double, summed in topological order exactly as
+ * lsdtt_flowinfo::accumulate (lsdtt_flowinfo_cpu.cpp:1979-2046); weights
+ * NULL means cell_size^2 per valid cell. Output double[n]. */
+int lsdtt_gpu_flow_accumulate(lsdtt_gpu_flow* f, const double* weights, double* accum);
+
+/* ---------------------------------------------------------------------
+ * Epsilon-free depression fill.  Replaces lsdtt::fill_pits(raster,
+ * min_slope <= 0, boundary_conditions) (src/cpu/
+ * lsdtt_raster_flow_preprocessing_cpu.cpp:707-782 -> reconstruction
+ * :1203-1394 / priority flood :893-920).  valid = mask != 0 && z != nodata
+ * (NaN-aware); invalid cells are written as nodata.  Periodic pairs must
+ * match and may not both be periodic (LSDTT_GPU_ERR_ARG).
+ * ------------------------------------------------------------------- */
+int lsdtt_gpu_fill(const float* z, const uint8_t* mask, int32_t n_rows, int32_t n_cols,
+                   double nodata, const char* boundary_conditions, float* filled);
+int lsdtt_gpu_fill_device(const float* d_z, const uint8_t* d_mask, int32_t n_rows,
+                          int32_t n_cols, double nodata, const char* boundary_conditions,
+                          float* d_filled, void* stream);
+
+/* ---------------------------------------------------------------------
+ * LEM timestep.  Replaces lsdtt::run_lem_step (src/cpu/lsdtt_lem_cpu.hpp:
+ * 220-232, .cpp:285-732) for the D8 fluvial + roering_implicit hillslope
+ * configuration, with a device-resident FP64 elevation state.  Options
+ * mirror lsdtt_lem_options (lsdtt_lem_cpu.hpp:97-182) and the Newton
+ * options (lsdtt_hillslope_newton_cpu.hpp:143-193); statistics mirror
+ * lsdtt_lem_step_stats (lsdtt_lem_cpu.hpp:185-211).
+ * ------------------------------------------------------------------- */
+typedef struct lsdtt_gpu_lem lsdtt_gpu_lem;
+
+typedef struct {
+    double timestep;                 /* dt (yr) */
+    double erodibility;              /* K */
+    double m_exponent, n_exponent;
+    double incision_threshold_area;  /* on area/dx; 0 = every cell */
+    int32_t divide_area_by_dx;       /* default 1 */
+    double slope_clamp;              /* default 1e-5 */
+    double hillslope_diffusivity;    /* D (m^2/yr) */
+    int32_t enable_nonlinear_hillslopes;
+    double critical_hillslope_slope; /* Sc */
+    double uplift_rate;              /* U (m/yr) */
+    double baselevel_fall_rate;
+    int32_t enable_d8_fluvial_erosion;
+    int32_t enable_hillslope_transport;  /* roering_implicit (Newton) */
+    int32_t enable_uplift;
+    char boundary_conditions[4];
+    /* Newton (lsdtt_hillslope_newton_options) and its linear solves */
+    double newton_abs_tol;           /* 1e-12 m/yr: the reference criterion */
+    double newton_rel_tol;           /* on ||F(z^n)||inf, default 1e-8 */
+    int32_t newton_max_iterations;   /* 50 */
+    int32_t newton_max_line_search;  /* 30 */
+    double slope_ratio_cap;          /* 0.9999 */
+    double krylov_rel_tol;           /* BiCGSTAB (FP32), default 1e-4 */
+    int32_t krylov_max_iterations;   /* 500 */
+} lsdtt_gpu_lem_options;
+
+typedef struct {
+    double delta_max_elevation;
+    double delta_mean_elevation;
+    double max_local_elevation_change;
+    double p999_local_elevation_change;
+    double mean_erosion_rate;
+    double max_erosion_rate;
+    int64_t n_valid;
+    int64_t cells_eroded;             /* stream-power cells updated */
+    int32_t newton_iterations;
+    int32_t krylov_iterations;
+    int32_t hillslope_substeps;
+    int32_t hillslope_rejected;
+    int32_t hillslope_converged;
+    double hillslope_initial_residual; /* ||F(z^n)||inf, m/yr */
+    double hillslope_residual;         /* final ||F||inf, m/yr */
+    double hillslope_tolerance;
+    double max_slope_ratio;
+} lsdtt_gpu_lem_stats;
+
+/* Defaults of lsdtt_lem_options, plus the Newton defaults above. */
+void lsdtt_gpu_lem_options_init(lsdtt_gpu_lem_options* o);
+
+lsdtt_gpu_lem* lsdtt_gpu_lem_create(int32_t n_rows, int32_t n_cols, double cell_size,
+                                    const uint8_t* mask, const lsdtt_gpu_lem_options* o);
+void lsdtt_gpu_lem_destroy(lsdtt_gpu_lem* lem);
+
+/* State transfer (host arrays of n_rows*n_cols). */
+int lsdtt_gpu_lem_set_elevation(lsdtt_gpu_lem* lem, const double* z);
+int lsdtt_gpu_lem_get_elevation(lsdtt_gpu_lem* lem, double* z);
+/* Per-pixel erodibility (check_lem_erodibility_field: finite, >= 0);
+ * NULL returns to the uniform K. */
+int lsdtt_gpu_lem_set_erodibility_field(lsdtt_gpu_lem* lem, const double* K);
+/* Device address of the resident FP64 elevation (for device-side I/O). */
+double* lsdtt_gpu_lem_elevation_device(lsdtt_gpu_lem* lem);
+
+/* One step; boundary_flats as from build_boundary_flats (uploaded when it
+ * changes).  LSDTT_GPU_ERR_NOT_CONVERGED: the hillslope Newton failed at
+ * the minimum sub-step (the reference would fall back to Picard). */
+int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary,
+                       lsdtt_gpu_lem_stats* stats);
+
+/* Valid cells with an 8-neighbour off a fixed ('o'/'b') edge or masked in
+ * a fixed direction (build_boundary_flats, lsdtt_lem_cpu.cpp:735-808);
+ * writes up to n_rows*n_cols flats, count in *n_out. */
+int lsdtt_gpu_build_boundary_flats(int32_t n_rows, int32_t n_cols, const uint8_t* mask,
+                                   const char* boundary_conditions, int64_t* out, int64_t* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

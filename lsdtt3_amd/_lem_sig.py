@@ -1,0 +1,50 @@
+"""ctypes signatures and structs of the LEM entry points of
+include/lsdtt3_gpu.h."""
+import ctypes as C
+
+
+class LemOptions(C.Structure):
+    _fields_ = [
+        ("timestep", C.c_double), ("erodibility", C.c_double),
+        ("m_exponent", C.c_double), ("n_exponent", C.c_double),
+        ("incision_threshold_area", C.c_double), ("divide_area_by_dx", C.c_int32),
+        ("slope_clamp", C.c_double), ("hillslope_diffusivity", C.c_double),
+        ("enable_nonlinear_hillslopes", C.c_int32), ("critical_hillslope_slope", C.c_double),
+        ("uplift_rate", C.c_double), ("baselevel_fall_rate", C.c_double),
+        ("enable_d8_fluvial_erosion", C.c_int32), ("enable_hillslope_transport", C.c_int32),
+        ("enable_uplift", C.c_int32), ("boundary_conditions", C.c_char * 4),
+        ("newton_abs_tol", C.c_double), ("newton_rel_tol", C.c_double),
+        ("newton_max_iterations", C.c_int32), ("newton_max_line_search", C.c_int32),
+        ("slope_ratio_cap", C.c_double), ("krylov_rel_tol", C.c_double),
+        ("krylov_max_iterations", C.c_int32),
+    ]
+
+
+class LemStats(C.Structure):
+    _fields_ = [
+        ("delta_max_elevation", C.c_double), ("delta_mean_elevation", C.c_double),
+        ("max_local_elevation_change", C.c_double), ("p999_local_elevation_change", C.c_double),
+        ("mean_erosion_rate", C.c_double), ("max_erosion_rate", C.c_double),
+        ("n_valid", C.c_int64), ("cells_eroded", C.c_int64),
+        ("newton_iterations", C.c_int32), ("krylov_iterations", C.c_int32),
+        ("hillslope_substeps", C.c_int32), ("hillslope_rejected", C.c_int32),
+        ("hillslope_converged", C.c_int32),
+        ("hillslope_initial_residual", C.c_double), ("hillslope_residual", C.c_double),
+        ("hillslope_tolerance", C.c_double), ("max_slope_ratio", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def declare(L, sig):
+    vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    sig(L, "lsdtt_gpu_lem_options_init", None, [C.POINTER(LemOptions)])
+    sig(L, "lsdtt_gpu_lem_create", vp, [i32, i32, dbl, vp, C.POINTER(LemOptions)])
+    sig(L, "lsdtt_gpu_lem_destroy", None, [vp])
+    sig(L, "lsdtt_gpu_lem_set_elevation", C.c_int, [vp, vp])
+    sig(L, "lsdtt_gpu_lem_get_elevation", C.c_int, [vp, vp])
+    sig(L, "lsdtt_gpu_lem_set_erodibility_field", C.c_int, [vp, vp])
+    sig(L, "lsdtt_gpu_lem_elevation_device", C.c_void_p, [vp])
+    sig(L, "lsdtt_gpu_lem_step", C.c_int, [vp, vp, i64, C.POINTER(LemStats)])
+    sig(L, "lsdtt_gpu_build_boundary_flats", C.c_int, [i32, i32, vp, C.c_char_p, vp, vp])

@@ -1,0 +1,229 @@
+// capi_lem.hip -- C ABI of the LEM timestep (lsdtt3_gpu.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../include/lsdtt3_gpu.h"
+#include "capi_util.hpp"
+#include "lem.hpp"
+
+using namespace lsdtt_gpu;
+
+struct lsdtt_gpu_lem {
+    std::unique_ptr<LemEngine> eng;
+    std::vector<int64_t> bf_cache;
+    bool bf_set = false;
+    bool z_set = false;
+};
+
+namespace {
+
+bool bc_ok(const char* s)
+{
+    for (int i = 0; i < 4; ++i)
+        if (s[i] != 'o' && s[i] != 'n' && s[i] != 'p' && s[i] != 'b') return false;
+    return true;
+}
+
+LemParams to_params(const lsdtt_gpu_lem_options& o)
+{
+    LemParams p;
+    p.dt = o.timestep;
+    p.K = o.erodibility;
+    p.m = o.m_exponent;
+    p.n = o.n_exponent;
+    p.threshold = o.incision_threshold_area;
+    p.divide_area_by_dx = o.divide_area_by_dx;
+    p.slope_clamp = o.slope_clamp;
+    p.D = o.hillslope_diffusivity;
+    p.nonlinear = o.enable_nonlinear_hillslopes;
+    p.Sc = o.critical_hillslope_slope;
+    p.U = o.uplift_rate;
+    p.baselevel_fall = o.baselevel_fall_rate;
+    p.enable_fluvial = o.enable_d8_fluvial_erosion;
+    p.enable_hillslope = o.enable_hillslope_transport;
+    p.enable_uplift = o.enable_uplift;
+    p.bc = BC{o.boundary_conditions[0], o.boundary_conditions[1], o.boundary_conditions[2],
+              o.boundary_conditions[3]};
+    p.newton_abs_tol = o.newton_abs_tol;
+    p.newton_rel_tol = o.newton_rel_tol;
+    p.newton_max_iterations = o.newton_max_iterations;
+    p.newton_max_line_search = o.newton_max_line_search;
+    p.slope_ratio_cap = o.slope_ratio_cap;
+    p.krylov_rel_tol = o.krylov_rel_tol;
+    p.krylov_max_iterations = o.krylov_max_iterations;
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+void lsdtt_gpu_lem_options_init(lsdtt_gpu_lem_options* o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    // lsdtt_lem_options defaults (lsdtt_lem_cpu.hpp:97-182)
+    o->timestep = 100.0;
+    o->erodibility = 1e-3;
+    o->m_exponent = 0.45;
+    o->n_exponent = 1.0;
+    o->incision_threshold_area = 0.0;
+    o->divide_area_by_dx = 1;
+    o->slope_clamp = 1e-5;
+    o->hillslope_diffusivity = 1e-3;
+    o->enable_nonlinear_hillslopes = 0;
+    o->critical_hillslope_slope = 0.6;
+    o->uplift_rate = 1e-4;
+    o->baselevel_fall_rate = 0.0;
+    o->enable_d8_fluvial_erosion = 1;
+    o->enable_hillslope_transport = 0;
+    o->enable_uplift = 1;
+    std::memcpy(o->boundary_conditions, "oooo", 4);
+    // lsdtt_hillslope_newton_options defaults (newton .hpp:143-165)
+    o->newton_abs_tol = 1e-12;
+    o->newton_rel_tol = 1e-8;
+    o->newton_max_iterations = 50;
+    o->newton_max_line_search = 30;
+    o->slope_ratio_cap = 0.9999;
+    o->krylov_rel_tol = 1e-4;
+    o->krylov_max_iterations = 500;
+}
+
+lsdtt_gpu_lem* lsdtt_gpu_lem_create(int32_t n_rows, int32_t n_cols, double cell_size, const uint8_t* mask,
+                                    const lsdtt_gpu_lem_options* o)
+{
+    return guarded_ptr<lsdtt_gpu_lem>([&]() -> lsdtt_gpu_lem* {
+        if (!o || !mask) throw std::invalid_argument("null argument");
+        if (n_rows <= 0 || n_cols <= 0) throw std::invalid_argument("grid must have at least one row and column");
+        if (!(cell_size > 0.0) || !std::isfinite(cell_size))
+            throw std::invalid_argument("cell_size must be finite and positive");
+        if (!bc_ok(o->boundary_conditions)) throw std::invalid_argument("boundary conditions must be four of o/n/p/b");
+        if (!(o->timestep > 0.0)) throw std::invalid_argument("timestep must be positive");
+        if (o->enable_hillslope_transport && o->enable_nonlinear_hillslopes && !(o->critical_hillslope_slope > 0.0))
+            throw std::invalid_argument("critical_hillslope_slope must be positive");
+        auto L = std::make_unique<lsdtt_gpu_lem>();
+        L->eng = std::make_unique<LemEngine>(n_rows, n_cols, cell_size, mask, to_params(*o));
+        return L.release();
+    });
+}
+
+void lsdtt_gpu_lem_destroy(lsdtt_gpu_lem* lem) { delete lem; }
+
+int lsdtt_gpu_lem_set_elevation(lsdtt_gpu_lem* lem, const double* z)
+{
+    return guarded([&] {
+        if (!lem || !z) return fail_arg("null argument");
+        lem->eng->set_elevation(z);
+        lem->z_set = true;
+        return LSDTT_GPU_OK;
+    });
+}
+
+int lsdtt_gpu_lem_get_elevation(lsdtt_gpu_lem* lem, double* z)
+{
+    return guarded([&] {
+        if (!lem || !z) return fail_arg("null argument");
+        lem->eng->get_elevation(z);
+        return LSDTT_GPU_OK;
+    });
+}
+
+int lsdtt_gpu_lem_set_erodibility_field(lsdtt_gpu_lem* lem, const double* K)
+{
+    return guarded([&] {
+        if (!lem) return fail_arg("null argument");
+        if (K) {
+            const long long n = lem->eng->n;
+            for (long long i = 0; i < n; ++i)
+                if (!std::isfinite(K[i]) || K[i] < 0.0)
+                    return fail_arg("run_lem_step: erodibility field must be finite and non-negative");
+        }
+        lem->eng->set_erodibility(K);
+        return LSDTT_GPU_OK;
+    });
+}
+
+double* lsdtt_gpu_lem_elevation_device(lsdtt_gpu_lem* lem)
+{
+    if (!lem) return nullptr;
+    lem->z_set = true;
+    return lem->eng->elevation_device();
+}
+
+int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary,
+                       lsdtt_gpu_lem_stats* out)
+{
+    return guarded([&] {
+        if (!lem || !out || (n_boundary > 0 && !boundary_flats)) return fail_arg("null argument");
+        if (!lem->z_set) return fail_arg("elevation not set");
+        if (!lem->bf_set || lem->bf_cache.size() != static_cast<size_t>(n_boundary) ||
+            (n_boundary && std::memcmp(lem->bf_cache.data(), boundary_flats, sizeof(int64_t) * n_boundary) != 0)) {
+            lem->bf_cache.assign(boundary_flats, boundary_flats + n_boundary);
+            std::vector<long long> v(lem->bf_cache.begin(), lem->bf_cache.end());
+            lem->eng->set_boundary_flats(v.data(), n_boundary);
+            lem->bf_set = true;
+        }
+        LemStats st;
+        const int rc = lem->eng->step(&st, nullptr);
+        out->delta_max_elevation = st.delta_max_elevation;
+        out->delta_mean_elevation = st.delta_mean_elevation;
+        out->max_local_elevation_change = st.max_local_elevation_change;
+        out->p999_local_elevation_change = st.p999_local_elevation_change;
+        out->mean_erosion_rate = st.mean_erosion_rate;
+        out->max_erosion_rate = st.max_erosion_rate;
+        out->n_valid = st.n_valid;
+        out->cells_eroded = st.cells_eroded;
+        out->newton_iterations = st.newton_iterations;
+        out->krylov_iterations = st.krylov_iterations;
+        out->hillslope_substeps = st.substeps;
+        out->hillslope_rejected = st.rejected;
+        out->hillslope_converged = st.hillslope_converged;
+        out->hillslope_initial_residual = st.hillslope_initial_residual;
+        out->hillslope_residual = st.hillslope_residual;
+        out->hillslope_tolerance = st.hillslope_tolerance;
+        out->max_slope_ratio = st.max_slope_ratio;
+        if (rc != 0) {
+            set_error("hillslope Newton did not converge at the minimum sub-step");
+            return LSDTT_GPU_ERR_NOT_CONVERGED;
+        }
+        return LSDTT_GPU_OK;
+    });
+}
+
+int lsdtt_gpu_build_boundary_flats(int32_t n_rows, int32_t n_cols, const uint8_t* mask,
+                                   const char* boundary_conditions, int64_t* out, int64_t* n_out)
+{
+    return guarded([&] {
+        if (!mask || !boundary_conditions || !out || !n_out) return fail_arg("null argument");
+        const char* bc = boundary_conditions;
+        const bool fn = bc[0] == 'o' || bc[0] == 'b', fe = bc[1] == 'o' || bc[1] == 'b';
+        const bool fs = bc[2] == 'o' || bc[2] == 'b', fw = bc[3] == 'o' || bc[3] == 'b';
+        int64_t k = 0;
+        for (int r = 0; r < n_rows; ++r)
+            for (int c = 0; c < n_cols; ++c) {
+                const int64_t f = static_cast<int64_t>(r) * n_cols + c;
+                if (!mask[f]) continue;
+                bool edge = false;
+                for (int dr = -1; dr <= 1 && !edge; ++dr)
+                    for (int dc = -1; dc <= 1; ++dc) {
+                        if (dr == 0 && dc == 0) continue;
+                        const int r2 = r + dr, c2 = c + dc;
+                        if (r2 < 0) edge |= fn;
+                        else if (r2 >= n_rows) edge |= fs;
+                        else if (c2 < 0) edge |= fw;
+                        else if (c2 >= n_cols) edge |= fe;
+                        else if (!mask[static_cast<int64_t>(r2) * n_cols + c2])
+                            edge |= (dr == -1 && fn) || (dr == 1 && fs) || (dc == -1 && fw) || (dc == 1 && fe);
+                    }
+                if (edge) out[k++] = f;
+            }
+        *n_out = k;
+        return LSDTT_GPU_OK;
+    });
+}
+
+}  // extern "C"

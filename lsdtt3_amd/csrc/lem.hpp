@@ -1,0 +1,102 @@
+// lem.hpp -- the LEM timestep on the device, FP64 state.
+//
+// One step (src/cpu/lsdtt_lem_cpu.cpp:285-732, order is the contract):
+//   1. D8 routing of the *input* surface (pits and interior nodata are
+//      outlets), upstream cell counts -> area = count * dx^2
+//   2. implicit stream power, level by level from the outlets up
+//      (src/cpu/lsdtt_stream_power_cpu.cpp:159-326)
+//   3. implicit hillslope diffusion (linear or Roering) by Newton with the
+//      reference's line search and sub-stepping
+//      (src/cpu/lsdtt_hillslope_newton_cpu.cpp:196-788); the Newton residual
+//      and state are FP64, the linear solves run BiCGSTAB on an assembled
+//      9-point Jacobian in FP32 (iterative refinement by the FP64 Newton loop)
+//   4. uplift U*dt on valid cells, then boundary cells restored to their
+//      pre-step value minus base-level fall
+//   5. step statistics, p99.9 by exact radix selection
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "dbuf.hpp"
+#include "flow.hpp"
+
+namespace lsdtt_gpu {
+
+struct LemParams {
+    double dt = 100.0;
+    double K = 1e-3, m = 0.45, n = 1.0, threshold = 0.0, slope_clamp = 1e-5;
+    int divide_area_by_dx = 1;
+    double D = 1e-3, Sc = 0.6, slope_ratio_cap = 0.9999;
+    int nonlinear = 0;
+    double U = 1e-4, baselevel_fall = 0.0;
+    int enable_fluvial = 1, enable_hillslope = 0, enable_uplift = 1;
+    BC bc{'o', 'o', 'o', 'o'};
+    double newton_abs_tol = 1e-12, newton_rel_tol = 1e-8;
+    int newton_max_iterations = 50, newton_max_line_search = 30;
+    double krylov_rel_tol = 1e-4;
+    int krylov_max_iterations = 500;
+};
+
+struct LemStats {
+    double delta_max_elevation = 0, delta_mean_elevation = 0, max_local_elevation_change = 0;
+    double p999_local_elevation_change = 0, mean_erosion_rate = 0, max_erosion_rate = 0;
+    long long n_valid = 0, cells_eroded = 0;
+    int newton_iterations = 0, krylov_iterations = 0, substeps = 0, rejected = 0;
+    int hillslope_converged = 1;
+    double hillslope_initial_residual = 0, hillslope_residual = 0, hillslope_tolerance = 0;
+    double max_slope_ratio = 0;
+};
+
+class LemEngine {
+public:
+    LemEngine(int nr, int nc, double dx, const uint8_t* mask_host, const LemParams& p);
+
+    void set_elevation(const double* z_host);
+    void get_elevation(double* z_host);
+    void set_erodibility(const double* K_host);  // null clears the field
+    void set_boundary_flats(const long long* flats, long long n);
+    double* elevation_device() { return z.p; }
+
+    // one timestep on stream `s`; returns 0, or LSDTT_GPU_ERR_NOT_CONVERGED
+    // when the hillslope solve failed at the minimum sub-step (the state is
+    // then the one after stream power + uplift, as the reference would have
+    // to fall back to Picard)
+    int step(LemStats* st, hipStream_t s);
+
+    int nr, nc;
+    long long n;
+    double dx;
+    LemParams p;
+    FlowEngine flow;
+
+    dbuf<uint8_t> mask, dof;
+    dbuf<double> z, z_before, K_field, bel;
+    dbuf<int> bflats;
+    long long n_bflats = 0;
+    bool has_K = false;
+
+    // hillslope work
+    dbuf<double> zn_hs, z_save;
+    dbuf<float> coef;            // 9 planes of n
+    dbuf<float> dinv, b, x, r, rhat, pv, phat, v, sv, shat, t;
+    dbuf<double> partials, scal;  // reduction partials; scalars
+    dbuf<unsigned> tickets;
+    dbuf<int> flags;
+    dbuf<float> absdz;
+    dbuf<unsigned> hist;
+
+    int hs_grid = 1024;
+    double cur_inv_dt = 0.0;
+
+private:
+    void stream_power(hipStream_t s);
+    int hillslope(LemStats* st, hipStream_t s);
+    int newton_solve(double inv_dt, LemStats* st, hipStream_t s);
+    int bicgstab(hipStream_t s);
+    void trial(double lambda, double* fnorm, double* ratio, hipStream_t s);
+    void stats(LemStats* st, hipStream_t s);
+};
+
+}  // namespace lsdtt_gpu

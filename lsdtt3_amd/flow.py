@@ -1,0 +1,164 @@
+"""FlowInfo and fill_pits on the GPU, mirroring the reference's Python
+``lsdtt3.FlowInfo`` (python_bindings/bindings/flow_info.cpp:315) and the C++
+``lsdtt_flowinfo`` query surface (src/core/lsdtt_flow_base.hpp:126-491).
+
+Arrays are numpy; every computation runs in liblsdtt3_gpu.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+class FlowInfo:
+    """D8 flow object built on the GPU.
+
+    Parameters mirror ``lsdtt_flowinfo_options`` (src/core/lsdtt_flow_base.hpp:
+    79-111): boundary_conditions (N,E,S,W of o/n/p/b), interior_nodata_is_outlet,
+    allow_pits_as_outlets, build_topological_order, build_s_vector and
+    forced_outlet_flats.  The DEM may be float32 (the reference raster type) or
+    float64 (the LEM state); slopes are taken in double either way.
+    """
+
+    def __init__(self, dem, mask, cell_size, boundary_conditions="oooo",
+                 interior_nodata_is_outlet=True, allow_pits_as_outlets=False,
+                 build_topological_order=True, build_s_vector=True,
+                 forced_outlet_flats=None):
+        dem = np.asarray(dem)
+        if dem.ndim != 2:
+            raise ValueError("dem must be 2-D")
+        self.n_rows, self.n_cols = dem.shape
+        self.n = self.n_rows * self.n_cols
+        self.cell_size = float(cell_size)
+        L = N.lib()
+        o = N.FlowOptions()
+        L.lsdtt_gpu_flow_options_init(C.byref(o), self.n_rows, self.n_cols, self.cell_size)
+        o.boundary_conditions = boundary_conditions.encode()[:4]
+        o.interior_nodata_is_outlet = int(interior_nodata_is_outlet)
+        o.allow_pits_as_outlets = int(allow_pits_as_outlets)
+        o.build_topological_order = int(build_topological_order)
+        o.build_s_vector = int(build_s_vector)
+        self._has_topo = bool(build_topological_order)
+        self._has_sv = bool(build_s_vector)
+        h = L.lsdtt_gpu_flow_create(C.byref(o))
+        if not h:
+            raise N.GpuError(N.ERR_ARG, (L.lsdtt_gpu_last_error() or b"").decode())
+        self._h = h
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        forced = None if forced_outlet_flats is None else np.ascontiguousarray(forced_outlet_flats, np.int64)
+        nf = 0 if forced is None else forced.size
+        if dem.dtype == np.float64:
+            z = np.ascontiguousarray(dem)
+            N.check(L.lsdtt_gpu_flow_build_f64(h, N.ptr(z), N.ptr(m), N.ptr(forced), nf))
+        else:
+            z = np.ascontiguousarray(dem, dtype=np.float32)
+            N.check(L.lsdtt_gpu_flow_build_f32(h, N.ptr(z), N.ptr(m), N.ptr(forced), nf))
+        nv, no, md = C.c_int64(), C.c_int64(), C.c_int64()
+        N.check(L.lsdtt_gpu_flow_sizes(h, C.byref(nv), C.byref(no), C.byref(md)))
+        self.n_valid, self.n_outlets, self.max_depth = nv.value, no.value, md.value
+        self._cache = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().lsdtt_gpu_flow_destroy(h)
+            self._h = None
+
+    def _get(self):
+        if "receiver" not in self._cache:
+            n = self.n
+            rcv = np.empty(n, np.int64)
+            d = np.empty(n, np.uint8)
+            dn = np.empty(n, np.uint8)
+            outl = np.empty(max(self.n_outlets, 1), np.int64)
+            topo = np.empty(max(self.n_valid, 1), np.int64) if self._has_topo else None
+            sv = np.empty(max(self.n_valid, 1), np.int64) if self._has_sv else None
+            ss = np.empty(n, np.int64) if self._has_sv else None
+            sz = np.empty(n, np.uint64) if self._has_sv else None
+            N.check(N.lib().lsdtt_gpu_flow_get(self._h, N.ptr(rcv), N.ptr(d), N.ptr(dn), N.ptr(outl),
+                                               N.ptr(topo), N.ptr(sv), N.ptr(ss), N.ptr(sz)))
+            self._cache.update(receiver=rcv, dir=d, donors=dn, outlets=outl[:self.n_outlets],
+                               topo=None if topo is None else topo[:self.n_valid],
+                               s_vector=None if sv is None else sv[:self.n_valid],
+                               s_start=ss, s_size=sz)
+        return self._cache
+
+    # ---- the lsdtt_flow_base array queries ---------------------------------
+    @property
+    def receivers(self):
+        return self._get()["receiver"]
+
+    @property
+    def receiver_directions(self):
+        return self._get()["dir"]
+
+    @property
+    def donor_counts(self):
+        return self._get()["donors"]
+
+    def outlets(self):
+        return self._get()["outlets"]
+
+    def topological_order(self):
+        if not self._has_topo:
+            raise RuntimeError("topological_order not built - set build_topological_order=True")
+        return self._get()["topo"]
+
+    @property
+    def s_vector(self):
+        if not self._has_sv:
+            raise RuntimeError("s_vector not built - set build_s_vector=True")
+        return self._get()["s_vector"]
+
+    @property
+    def s_vector_start(self):
+        return self._get()["s_start"]
+
+    @property
+    def s_vector_size(self):
+        return self._get()["s_size"]
+
+    def upslope_nodes(self, flat):
+        c = self._get()
+        st, sz = int(c["s_start"][flat]), int(c["s_size"][flat])
+        return c["s_vector"][st:st + sz]
+
+    def get_drainage_area_raster(self, no_data_value=-9999.0):
+        """(area float32[n_rows, n_cols], mask uint8) with area =
+        float(n_contributing * cell_size^2)."""
+        a = np.empty(self.n, np.float32)
+        m = np.empty(self.n, np.uint8)
+        N.check(N.lib().lsdtt_gpu_flow_area(self._h, float(no_data_value), N.ptr(a), N.ptr(m), None))
+        return a.reshape(self.n_rows, self.n_cols), m.reshape(self.n_rows, self.n_cols)
+
+    def n_contributing_pixels(self):
+        c = np.empty(self.n, np.int64)
+        N.check(N.lib().lsdtt_gpu_flow_area(self._h, 0.0, None, None, N.ptr(c)))
+        return c
+
+    def accumulate(self, cell_weights=None):
+        """double accumulation in topological order (cell_size^2 by default)."""
+        out = np.empty(self.n, np.float64)
+        w = None if cell_weights is None else np.ascontiguousarray(cell_weights, np.float64).ravel()
+        if w is not None and w.size != self.n:
+            raise ValueError("cell_weights size must equal n_rows * n_cols")
+        N.check(N.lib().lsdtt_gpu_flow_accumulate(self._h, N.ptr(w), N.ptr(out)))
+        return out
+
+
+def fill_pits(dem, mask, no_data_value=-9999.0, boundary_conditions="oooo"):
+    """Epsilon-free fill (lsdtt::fill_pits with min_slope <= 0), float32."""
+    z = np.ascontiguousarray(dem, dtype=np.float32)
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    out = np.empty_like(z)
+    nr, nc = z.shape
+    N.check(N.lib().lsdtt_gpu_fill(N.ptr(z), N.ptr(m), nr, nc, float(no_data_value),
+                                   boundary_conditions.encode()[:4], N.ptr(out)))
+    return out
+
+
+def gpu_available() -> bool:
+    return bool(N.lib().lsdtt_gpu_available())

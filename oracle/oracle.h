@@ -94,6 +94,57 @@ int64_t orc_stream_power_d8(int64_t n, const uint8_t* mask,
                             const orc_sp_params* p, const double* z_old,
                             double* z);
 
+/* ---- hillslope Newton and the LEM step (oracle/orc_lem.c) -------------- */
+
+typedef struct {
+    double D, U, Sc;
+    int    nonlinear;
+    double slope_ratio_cap;    /* 0.9999 */
+    double abs_tol;            /* 1e-12 m/yr (the reference's criterion) */
+    double rel_tol;            /* on ||F(z^n)||inf */
+    int    max_iterations;     /* 50 */
+    int    max_line_search;    /* 30 */
+} orc_hs_params;
+
+typedef struct {
+    int converged, iterations, line_search_steps, substeps, rejected;
+    double initial_residual, final_residual, tolerance, final_max_dz, max_slope_ratio;
+} orc_hs_result;
+
+int orc_hillslope_newton(int nr, int nc, double h, const uint8_t* mask,
+                         const char* bc, const uint8_t* is_channel,
+                         const orc_hs_params* p, double inv_dt,
+                         const double* zn, double* z, orc_hs_result* res);
+
+int orc_hillslope_transient(int nr, int nc, double h, const uint8_t* mask,
+                            const char* bc, const uint8_t* is_channel,
+                            const orc_hs_params* p, double dt_total,
+                            double dt_init, double* z, orc_hs_result* total);
+
+typedef struct {
+    double dt, K, m, n, threshold, slope_clamp, U, baselevel_fall;
+    int    divide_area_by_dx;
+    int    enable_fluvial, enable_hillslope, enable_uplift;
+    char   bc[4];
+    orc_hs_params hs;
+} orc_lem_params;
+
+typedef struct {
+    double delta_max_elevation, delta_mean_elevation, max_local_elevation_change;
+    double p999_local_elevation_change, mean_erosion_rate, max_erosion_rate;
+    int64_t n_valid, cells_eroded;
+    int newton_iterations, newton_substeps;
+    double hillslope_residual, hillslope_initial_residual;
+} orc_lem_stats;
+
+int64_t orc_boundary_flats(int nr, int nc, const uint8_t* mask, const char* bc,
+                           int64_t* out);
+
+int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
+                 const orc_lem_params* p, const int64_t* boundary_flats,
+                 int64_t n_bf, const double* K_field, double* z,
+                 orc_lem_stats* st);
+
 #ifdef __cplusplus
 }
 #endif

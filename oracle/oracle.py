@@ -55,6 +55,22 @@ def lib():
         L.orc_fill_priority_flood.restype = C.c_int
         L.orc_fnv1a.argtypes = [C.c_uint64, C.c_void_p, C.c_int64]
         L.orc_fnv1a.restype = C.c_uint64
+        L.orc_stream_power_d8.argtypes = [C.c_int64, u8p, i64p, u8p, i64p, C.c_int64, f64p,
+                                          C.c_void_p, C.POINTER(SpParams), f64p, f64p]
+        L.orc_stream_power_d8.restype = C.c_int64
+        L.orc_hillslope_newton.argtypes = [C.c_int, C.c_int, C.c_double, u8p, C.c_char_p, C.c_void_p,
+                                           C.POINTER(HsParams), C.c_double, f64p, f64p,
+                                           C.POINTER(HsResult)]
+        L.orc_hillslope_newton.restype = C.c_int
+        L.orc_hillslope_transient.argtypes = [C.c_int, C.c_int, C.c_double, u8p, C.c_char_p,
+                                              C.c_void_p, C.POINTER(HsParams), C.c_double,
+                                              C.c_double, f64p, C.POINTER(HsResult)]
+        L.orc_hillslope_transient.restype = C.c_int
+        L.orc_boundary_flats.argtypes = [C.c_int, C.c_int, u8p, C.c_char_p, i64p]
+        L.orc_boundary_flats.restype = C.c_int64
+        L.orc_lem_step.argtypes = [C.c_int, C.c_int, C.c_double, u8p, C.POINTER(LemParams), i64p,
+                                   C.c_int64, C.c_void_p, f64p, C.POINTER(LemStats)]
+        L.orc_lem_step.restype = C.c_int
         _lib = L
     return _lib
 
@@ -157,3 +173,122 @@ def fill(z, mask, nodata=-9999.0, bc="oooo"):
     if st != 0:
         raise OracleError(f"fill failed (status {st}, bc {bc!r})")
     return out
+
+
+# ---------------------------------------------------------------------------
+# LEM pieces (oracle/orc_lem.c)
+# ---------------------------------------------------------------------------
+class SpParams(C.Structure):
+    _fields_ = [("K", C.c_double), ("m", C.c_double), ("n", C.c_double), ("dt", C.c_double),
+                ("dx", C.c_double), ("threshold", C.c_double), ("divide_area_by_dx", C.c_int),
+                ("slope_clamp", C.c_double)]
+
+
+class HsParams(C.Structure):
+    _fields_ = [("D", C.c_double), ("U", C.c_double), ("Sc", C.c_double), ("nonlinear", C.c_int),
+                ("slope_ratio_cap", C.c_double), ("abs_tol", C.c_double), ("rel_tol", C.c_double),
+                ("max_iterations", C.c_int), ("max_line_search", C.c_int)]
+
+
+class HsResult(C.Structure):
+    _fields_ = [("converged", C.c_int), ("iterations", C.c_int), ("line_search_steps", C.c_int),
+                ("substeps", C.c_int), ("rejected", C.c_int), ("initial_residual", C.c_double),
+                ("final_residual", C.c_double), ("tolerance", C.c_double),
+                ("final_max_dz", C.c_double), ("max_slope_ratio", C.c_double)]
+
+
+class LemParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("K", C.c_double), ("m", C.c_double), ("n", C.c_double),
+                ("threshold", C.c_double), ("slope_clamp", C.c_double), ("U", C.c_double),
+                ("baselevel_fall", C.c_double), ("divide_area_by_dx", C.c_int),
+                ("enable_fluvial", C.c_int), ("enable_hillslope", C.c_int),
+                ("enable_uplift", C.c_int), ("bc", C.c_char * 4), ("hs", HsParams)]
+
+
+class LemStats(C.Structure):
+    _fields_ = [("delta_max_elevation", C.c_double), ("delta_mean_elevation", C.c_double),
+                ("max_local_elevation_change", C.c_double),
+                ("p999_local_elevation_change", C.c_double), ("mean_erosion_rate", C.c_double),
+                ("max_erosion_rate", C.c_double), ("n_valid", C.c_int64),
+                ("cells_eroded", C.c_int64), ("newton_iterations", C.c_int),
+                ("newton_substeps", C.c_int), ("hillslope_residual", C.c_double),
+                ("hillslope_initial_residual", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def hs_params(D, Sc=0.8, nonlinear=False, U=0.0, rel_tol=1e-8, abs_tol=1e-12, cap=0.9999,
+              max_iterations=50, max_line_search=30):
+    return HsParams(D, U, Sc, int(nonlinear), cap, abs_tol, rel_tol, max_iterations, max_line_search)
+
+
+def stream_power(z_old, z, mask, receiver, dirs, topo, area, K, m, n, dt, dx, threshold=0.0,
+                 divide_area_by_dx=True, slope_clamp=1e-5, K_field=None):
+    """orc_stream_power_d8 on a copy of z; returns (z_new, cells eroded)"""
+    zz = np.ascontiguousarray(z, dtype=np.float64).ravel().copy()
+    zo = np.ascontiguousarray(z_old, dtype=np.float64).ravel()
+    m8 = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+    p = SpParams(K, m, n, dt, dx, threshold, int(divide_area_by_dx), slope_clamp)
+    kf = None
+    kptr = None
+    if K_field is not None:
+        kf = np.ascontiguousarray(K_field, dtype=np.float64).ravel()
+        kptr = kf.ctypes.data_as(C.c_void_p)
+    e = lib().orc_stream_power_d8(zz.size, m8, receiver, dirs, topo, len(topo),
+                                  np.ascontiguousarray(area, np.float64).ravel(), kptr, C.byref(p), zo, zz)
+    return zz.reshape(np.shape(z)), e
+
+
+def hillslope_newton(z, zn, mask, dx, bc, params, inv_dt):
+    """one backward-Euler Newton solve; returns (z_new, HsResult)"""
+    zz = np.ascontiguousarray(z, dtype=np.float64).copy()
+    nr, nc = zz.shape
+    r = HsResult()
+    lib().orc_hillslope_newton(nr, nc, float(dx), np.ascontiguousarray(mask, np.uint8).ravel(),
+                               bc.encode(), None, C.byref(params), float(inv_dt),
+                               np.ascontiguousarray(zn, np.float64).ravel(), zz.ravel(), C.byref(r))
+    return zz, r
+
+
+def hillslope_transient(z, mask, dx, bc, params, dt_total, dt_init=0.0):
+    zz = np.ascontiguousarray(z, dtype=np.float64).copy()
+    nr, nc = zz.shape
+    r = HsResult()
+    lib().orc_hillslope_transient(nr, nc, float(dx), np.ascontiguousarray(mask, np.uint8).ravel(),
+                                  bc.encode(), None, C.byref(params), float(dt_total), float(dt_init),
+                                  zz.ravel(), C.byref(r))
+    return zz, r
+
+
+def boundary_flats(mask, bc):
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    nr, nc = m.shape
+    out = np.empty(nr * nc, np.int64)
+    k = lib().orc_boundary_flats(nr, nc, m.ravel(), bc.encode(), out)
+    return out[:k].copy()
+
+
+def lem_params(dt, K, m, n, U, D, Sc=0.8, nonlinear=False, bc="oooo", threshold=0.0,
+               slope_clamp=1e-5, baselevel_fall=0.0, fluvial=True, hillslope=True, uplift=True,
+               rel_tol=1e-8, abs_tol=1e-12):
+    return LemParams(dt, K, m, n, threshold, slope_clamp, U, baselevel_fall, 1, int(fluvial),
+                     int(hillslope), int(uplift), bc.encode(),
+                     hs_params(D, Sc, nonlinear, 0.0, rel_tol, abs_tol))
+
+
+def lem_step(z, mask, dx, params, boundary=None, K_field=None):
+    """orc_lem_step on a copy of z; returns (z_new, stats dict, status)"""
+    zz = np.ascontiguousarray(z, dtype=np.float64).copy()
+    nr, nc = zz.shape
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    bf = boundary_flats(m, params.bc.decode()) if boundary is None else np.ascontiguousarray(boundary, np.int64)
+    st = LemStats()
+    kptr = None
+    kf = None
+    if K_field is not None:
+        kf = np.ascontiguousarray(K_field, np.float64).ravel()
+        kptr = kf.ctypes.data_as(C.c_void_p)
+    rc = lib().orc_lem_step(nr, nc, float(dx), m.ravel(), C.byref(params), bf, len(bf), kptr,
+                            zz.ravel(), C.byref(st))
+    return zz, st.as_dict(), rc

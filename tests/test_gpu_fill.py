@@ -1,0 +1,76 @@
+"""GPU epsilon-free fill against the oracle: bit-exact float32."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import lsdtt3_amd
+    assert lsdtt3_amd.gpu_available()
+    return lsdtt3_amd
+
+
+def same(a, b):
+    return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("bc", ["oooo", "popo", "opop", "nnnn"])
+def test_gpu_fill_matches_oracle(gpu, seed, bc):
+    print(f"[gpu fill] seed={seed} bc={bc}", flush=True)
+    from lsdtt3_amd.synthetic import routing_surface
+    z = routing_surface(301, 203, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    mask = np.ones(z.shape, np.uint8)
+    if seed == 2:
+        mask[rng.random(z.shape) < 0.01] = 0
+        z[5:9, 5:9] = -9999.0          # nodata by value with mask set
+    assert same(gpu.fill_pits(z, mask, -9999.0, bc), orc.fill(z, mask, -9999.0, bc))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 7), (7, 1), (2, 2), (3, 3), (65, 1), (130, 129)])
+def test_gpu_fill_small_shapes(gpu, shape):
+    rng = np.random.default_rng(sum(shape))
+    z = rng.integers(0, 9, shape).astype(np.float32)
+    mask = np.ones(shape, np.uint8)
+    assert same(gpu.fill_pits(z, mask), orc.fill(z, mask))
+
+
+def test_gpu_fill_nan_nodata(gpu):
+    rng = np.random.default_rng(5)
+    z = (rng.random((70, 90)) * 20).astype(np.float32)
+    z[30:33, 40:44] = np.nan
+    mask = np.ones(z.shape, np.uint8)
+    got = gpu.fill_pits(z, mask, float("nan"))
+    want = orc.fill(z, mask, float("nan"))
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    assert np.array_equal(got[ok], want[ok])
+
+
+def test_gpu_fill_deep_maze(gpu):
+    """a spiral trench: the spill path winds through many tiles"""
+    n = 257
+    z = np.full((n, n), 50.0, np.float32)
+    r0, r1, c0, c1 = 2, n - 3, 2, n - 3
+    k = 0
+    while r0 < r1 and c0 < c1:
+        z[r0, c0:c1 + 1] = 10.0 - 1e-3 * k
+        z[r0:r1 + 1, c1] = 10.0 - 1e-3 * k
+        z[r1, c0:c1 + 1] = 10.0 - 1e-3 * k
+        z[r0 + 2:r1 + 1, c0] = 10.0 - 1e-3 * k
+        r0 += 2; r1 -= 2; c0 += 2; c1 -= 2; k += 1
+    z[0, 2] = 0.0
+    mask = np.ones(z.shape, np.uint8)
+    assert same(gpu.fill_pits(z, mask), orc.fill(z, mask))
+
+
+def test_gpu_fill_rejects_bad_bc(gpu):
+    import lsdtt3_amd._native as N
+    z = np.zeros((4, 4), np.float32)
+    with pytest.raises(N.GpuError):
+        gpu.fill_pits(z, np.ones_like(z, np.uint8), boundary_conditions="pppp")

@@ -1,0 +1,114 @@
+"""GPU LEM timestep (through the C ABI) against the oracle's FP64 step.
+
+Tolerances, stated per output:
+  * fluvial only: |dz| <= 1e-9 m.  The device and host evaluate the same
+    per-node expressions in the same order with no FMA contraction; only
+    pow() differs (ocml vs glibc, <= 1 ulp), and the n != 1 Newton stops at
+    slope_clamp * dx_eff, so a 1-ulp difference can at most shift one
+    iteration.
+  * with hillslope: both solve F(z) = 0 to ||F||inf <= rel_tol ||F(z^n)||inf,
+    so each result is within dt * tol / (1 - rho) of the exact root, rho < 1
+    the Jacobi contraction of I + dt J_L; the test allows 4 * dt * tol.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import lsdtt3_amd
+    assert lsdtt3_amd.gpu_available()
+    return lsdtt3_amd
+
+
+def run_both(gpu, z, dx, steps, kw, okw, mask=None):
+    mask = np.ones(z.shape, np.uint8) if mask is None else mask
+    lem = gpu.LandscapeModel(z, dx, mask=mask, **kw)
+    p = orc.lem_params(**okw)
+    zo = z.copy()
+    for i in range(steps):
+        st_g = lem.step()
+        zo, st_o, rc = orc.lem_step(zo, mask, dx, p)
+        assert rc == 0
+    return lem.elevation, zo, st_g, st_o
+
+
+@pytest.mark.parametrize("nexp", [1.0, 1.5])
+def test_gpu_fluvial_only_matches_oracle(gpu, nexp):
+    print(f"[gpu lem fluvial] n={nexp}", flush=True)
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(150, 170, dx=30.0, seed=5)
+    kw = dict(timestep_years=1000.0, erodibility=1e-5, m_exponent=0.5, n_exponent=nexp, uplift_rate=1e-3,
+              hillslope_model="none", boundary_conditions="nnon")
+    okw = dict(dt=1000.0, K=1e-5, m=0.5, n=nexp, U=1e-3, D=0.0, bc="nnon", hillslope=False)
+    zg, zo, sg, so = run_both(gpu, z, 30.0, 3, kw, okw)
+    assert np.abs(zg - zo).max() <= 1e-9
+    assert sg["cells_eroded"] == so["cells_eroded"]
+    assert sg["n_valid"] == so["n_valid"]
+    assert abs(sg["max_local_elevation_change"] - so["max_local_elevation_change"]) <= 1e-9
+    assert abs(sg["p999_local_elevation_change"] - so["p999_local_elevation_change"]) <= 1e-6
+
+
+def test_gpu_fluvial_erodibility_field(gpu):
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(64, 80, dx=30.0, seed=6)
+    K = np.random.default_rng(1).uniform(1e-6, 1e-4, z.shape)
+    mask = np.ones(z.shape, np.uint8)
+    lem = gpu.LandscapeModel(z, 30.0, timestep_years=500.0, m_exponent=0.5, n_exponent=1.5,
+                             hillslope_model="none", boundary_conditions="nnon", erodibility_field=K)
+    lem.step()
+    zo, _, _ = orc.lem_step(z, mask, 30.0, orc.lem_params(500.0, 1e-3, 0.5, 1.5, 1e-4, 0.0, bc="nnon",
+                                                          hillslope=False), K_field=K)
+    assert np.abs(lem.elevation - zo).max() <= 1e-9
+
+
+def test_gpu_linear_lem_matches_oracle(gpu):
+    """config-0 physics (linear D, n = 1) on a 128 x 160 grid, 4 steps"""
+    print("[gpu lem linear]", flush=True)
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(128, 160, dx=30.0, seed=7)
+    kw = dict(timestep_years=1000.0, erodibility=1e-5, m_exponent=0.5, n_exponent=1.0, uplift_rate=1e-3,
+              hillslope_diffusivity=0.01, boundary_conditions="nnon")
+    okw = dict(dt=1000.0, K=1e-5, m=0.5, n=1.0, U=1e-3, D=0.01, bc="nnon")
+    zg, zo, sg, so = run_both(gpu, z, 30.0, 4, kw, okw)
+    tol = 4 * 1000.0 * max(sg["hillslope_tolerance"], 1e-12)
+    assert sg["hillslope_converged"] == 1
+    assert sg["hillslope_residual"] <= sg["hillslope_tolerance"]
+    assert np.abs(zg - zo).max() <= max(tol, 1e-8), (np.abs(zg - zo).max(), tol)
+
+
+def test_gpu_roering_lem_matches_oracle(gpu):
+    """config-2 physics (Roering, Sc = 0.8, n = 1.5) on a sub-critical start"""
+    print("[gpu lem roering]", flush=True)
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(96, 112, dx=30.0, seed=8, max_slope=0.6)
+    kw = dict(timestep_years=500.0, erodibility=3e-5, m_exponent=0.5, n_exponent=1.5, uplift_rate=5e-4,
+              hillslope_diffusivity=0.005, nonlinear_hillslopes=True, critical_hillslope_slope=0.8,
+              boundary_conditions="nnon")
+    okw = dict(dt=500.0, K=3e-5, m=0.5, n=1.5, U=5e-4, D=0.005, Sc=0.8, nonlinear=True, bc="nnon")
+    zg, zo, sg, so = run_both(gpu, z, 30.0, 3, kw, okw)
+    assert sg["hillslope_converged"] == 1
+    assert sg["hillslope_residual"] <= sg["hillslope_tolerance"]
+    tol = 4 * 500.0 * sg["hillslope_tolerance"]
+    assert np.abs(zg - zo).max() <= max(tol, 1e-8), (np.abs(zg - zo).max(), tol)
+
+
+def test_gpu_boundary_restore_and_mask(gpu):
+    rng = np.random.default_rng(3)
+    z = rng.random((40, 50)) * 5 + 100
+    mask = np.ones(z.shape, np.uint8)
+    mask[10:14, 20:26] = 0
+    lem = gpu.LandscapeModel(z, 10.0, mask=mask, timestep_years=100.0, uplift_rate=1e-3,
+                             hillslope_diffusivity=0.02, boundary_conditions="onon", baselevel_fall_rate=1e-4)
+    lem.step()
+    zg = lem.elevation
+    bf = gpu.build_boundary_flats(mask, "onon")
+    assert np.array_equal(bf, orc.boundary_flats(mask, "onon"))
+    np.testing.assert_allclose(zg.ravel()[bf], z.ravel()[bf] - 0.01, rtol=0, atol=1e-12)
+    zo, so, rc = orc.lem_step(z, mask, 10.0, orc.lem_params(100.0, 1e-3, 0.45, 1.0, 1e-3, 0.02, bc="onon",
+                                                            baselevel_fall=1e-4))
+    assert np.abs(zg - zo)[mask != 0].max() <= 1e-6
