@@ -197,6 +197,17 @@ double* lsdtt_gpu_lem_elevation_device(lsdtt_gpu_lem* lem);
 int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary,
                        lsdtt_gpu_lem_stats* stats);
 
+/* Per-kernel timing with HIP events recorded on the step's stream (no
+ * counterpart in the reference; for roofline measurement).  enable != 0
+ * starts recording (and clears previous totals); _get returns, for kernel
+ * group `index` < _count, its name, total milliseconds, launches and the
+ * algorithmic bytes those launches move (each array element a kernel must
+ * read or write counted once). */
+int lsdtt_gpu_lem_profile(lsdtt_gpu_lem* lem, int32_t enable);
+int32_t lsdtt_gpu_lem_profile_count(lsdtt_gpu_lem* lem);
+int lsdtt_gpu_lem_profile_get(lsdtt_gpu_lem* lem, int32_t index, char* name, int32_t name_len,
+                              double* ms, int64_t* launches, double* bytes);
+
 /* Valid cells with an 8-neighbour off a fixed ('o'/'b') edge or masked in
  * a fixed direction (build_boundary_flats, lsdtt_lem_cpu.cpp:735-808);
  * writes up to n_rows*n_cols flats, count in *n_out. */

@@ -48,3 +48,6 @@ def declare(L, sig):
     sig(L, "lsdtt_gpu_lem_elevation_device", C.c_void_p, [vp])
     sig(L, "lsdtt_gpu_lem_step", C.c_int, [vp, vp, i64, C.POINTER(LemStats)])
     sig(L, "lsdtt_gpu_build_boundary_flats", C.c_int, [i32, i32, vp, C.c_char_p, vp, vp])
+    sig(L, "lsdtt_gpu_lem_profile", C.c_int, [vp, i32])
+    sig(L, "lsdtt_gpu_lem_profile_count", C.c_int32, [vp])
+    sig(L, "lsdtt_gpu_lem_profile_get", C.c_int, [vp, i32, C.c_char_p, i32, vp, vp, vp])

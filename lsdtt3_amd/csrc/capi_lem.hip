@@ -194,6 +194,51 @@ int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_
     });
 }
 
+int lsdtt_gpu_lem_profile(lsdtt_gpu_lem* lem, int32_t enable)
+{
+    return guarded([&] {
+        if (!lem) return fail_arg("null argument");
+        LSDTT_HIP(hipDeviceSynchronize());
+        lem->eng->timer.reset();
+        lem->eng->timer.enabled = enable != 0;
+        return LSDTT_GPU_OK;
+    });
+}
+
+int32_t lsdtt_gpu_lem_profile_count(lsdtt_gpu_lem* lem)
+{
+    if (!lem) return 0;
+    try {
+        LSDTT_HIP(hipDeviceSynchronize());
+        lem->eng->timer.collect();
+        return static_cast<int32_t>(lem->eng->timer.totals().size());
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 0;
+    }
+}
+
+int lsdtt_gpu_lem_profile_get(lsdtt_gpu_lem* lem, int32_t index, char* name, int32_t name_len, double* ms,
+                              int64_t* launches, double* bytes)
+{
+    return guarded([&] {
+        if (!lem) return fail_arg("null argument");
+        lem->eng->timer.collect();
+        const auto& t = lem->eng->timer.totals();
+        if (index < 0 || index >= static_cast<int32_t>(t.size())) return fail_arg("profile index out of range");
+        auto it = t.begin();
+        std::advance(it, index);
+        if (name && name_len > 0) {
+            std::strncpy(name, it->first.c_str(), static_cast<size_t>(name_len - 1));
+            name[name_len - 1] = '\0';
+        }
+        if (ms) *ms = it->second.ms;
+        if (launches) *launches = it->second.launches;
+        if (bytes) *bytes = it->second.bytes;
+        return LSDTT_GPU_OK;
+    });
+}
+
 int lsdtt_gpu_build_boundary_flats(int32_t n_rows, int32_t n_cols, const uint8_t* mask,
                                    const char* boundary_conditions, int64_t* out, int64_t* n_out)
 {

@@ -26,6 +26,20 @@ namespace lsdtt_gpu {
 namespace {
 
 constexpr int kBlock = 256;
+
+struct TScope {
+    KernelTimer* t;
+    hipStream_t s;
+    double bytes;
+    TScope(KernelTimer* t_, const char* name, hipStream_t s_, double b) : t(t_), s(s_), bytes(b)
+    {
+        if (t) t->begin(name, s);
+    }
+    ~TScope()
+    {
+        if (t) t->end(s, bytes);
+    }
+};
 constexpr int kLdsLevels = 2048;
 
 __device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long long* p)
@@ -513,10 +527,16 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
     int init[3] = {0x7fffffff, -1, 0};
     LSDTT_HIP(hipMemcpyAsync(misc.p, init, sizeof(init), hipMemcpyHostToDevice, s));
     dim3 b2(64, 4), g2(div_up(nc, 64), div_up(nr, 4));
-    k_receivers<T><<<g2, b2, 0, s>>>(z, msk, forced, nr, nc, opt.dx, opt.dx * kSqrt2, opt.bc,
+    {
+        TScope ts_(timer, "receivers", s, (sizeof(T) + 6.0) * n);
+        k_receivers<T><<<g2, b2, 0, s>>>(z, msk, forced, nr, nc, opt.dx, opt.dx * kSqrt2, opt.bc,
                                       opt.interior_nodata_is_outlet, opt.allow_pits_as_outlets, rcv, dir, misc.p);
+    }
     LSDTT_LAUNCH_CHECK();
-    k_donors<<<g1, kBlock, 0, s>>>(rcv, msk, nr, nc, opt.bc, donors, flags);
+    {
+        TScope ts_(timer, "donors", s, 9.0 * n);
+        k_donors<<<g1, kBlock, 0, s>>>(rcv, msk, nr, nc, opt.bc, donors, flags);
+    }
     LSDTT_LAUNCH_CHECK();
     size_t tb = 0;
     LSDTT_HIP(rocprim::exclusive_scan(nullptr, tb, flags.p, scan_out.p, 0, (size_t)n, rocprim::plus<int>(), s));
@@ -550,13 +570,19 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
     LSDTT_HIP(hipMemsetAsync(level_count.p, 0, sizeof(int) * (nl + 1), s));
     LSDTT_HIP(hipMemsetAsync(cursor.p, 0, sizeof(int) * (nl + 1), s));
     const int g1k = div_up(n, 1024 * kCellsPerThread);
-    k_level_hist<<<g1k, 1024, 0, s>>>(depth, n, level_count);
+    {
+        TScope ts_(timer, "level_hist", s, 4.0 * n);
+        k_level_hist<<<g1k, 1024, 0, s>>>(depth, n, level_count);
+    }
     LSDTT_LAUNCH_CHECK();
     LSDTT_HIP(rocprim::exclusive_scan(nullptr, tb, level_count.p, level_start.p, 0, (size_t)(nl + 1),
                                       rocprim::plus<int>(), s));
     LSDTT_HIP(rocprim::exclusive_scan(tmp.need(tb), tb, level_count.p, level_start.p, 0, (size_t)(nl + 1),
                                       rocprim::plus<int>(), s));
-    k_level_scatter<<<g1k, 1024, 0, s>>>(depth, n, level_start, cursor, order);
+    {
+        TScope ts_(timer, "level_scatter", s, 8.0 * n);
+        k_level_scatter<<<g1k, 1024, 0, s>>>(depth, n, level_start, cursor, order);
+    }
     LSDTT_LAUNCH_CHECK();
     LSDTT_HIP(hipMemcpyAsync(level_start_h.data(), level_start.p, sizeof(int) * (nl + 1), hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipStreamSynchronize(s));
@@ -582,7 +608,10 @@ void FlowEngine::pointer_jump(const int* weight, int* value_out, int* root_out, 
     const int one = 1;
     LSDTT_HIP(hipMemcpyAsync(chg, &one, sizeof(int), hipMemcpyHostToDevice, s));
     for (int r = 0; r < kRounds; ++r) {
-        k_pj_round<<<g1, kBlock, 0, s>>>(pj, n, chg + r, chg + r + 1);
+        {
+            TScope ts_(timer, "pointer_jump", s, 24.0 * n);
+            k_pj_round<<<g1, kBlock, 0, s>>>(pj, n, chg + r, chg + r + 1);
+        }
         LSDTT_LAUNCH_CHECK();
     }
     k_pj_extract<<<std::min(g1, 2048), kBlock, 0, s>>>(pj, mask, n, value_out, root_out,
@@ -624,13 +653,19 @@ void FlowEngine::upstream_counts_fast(hipStream_t s)
     // level 0 holds the roots; they have no receiver to add to
     const int lo = std::max(tail_from, 1);
     if (lo < nl) {
-        k_cnt_tail<<<1, 1024, 0, s>>>(order, level_start, lo, nl - 1, rcv, cnt);
+        {
+            TScope ts_(timer, "upstream_counts_tail", s, 16.0 * (level_start_h[nl] - level_start_h[lo]));
+            k_cnt_tail<<<1, 1024, 0, s>>>(order, level_start, lo, nl - 1, rcv, cnt);
+        }
         LSDTT_LAUNCH_CHECK();
     }
     for (int L = lo - 1; L >= 1; --L) {
         const int b = level_start_h[L], e = level_start_h[L + 1];
         if (e > b) {
-            k_cnt_level<<<div_up(e - b, kBlock), kBlock, 0, s>>>(order, b, e, rcv, cnt);
+            {
+                TScope ts_(timer, "upstream_counts", s, 16.0 * (e - b));
+                k_cnt_level<<<div_up(e - b, kBlock), kBlock, 0, s>>>(order, b, e, rcv, cnt);
+            }
             LSDTT_LAUNCH_CHECK();
         }
     }

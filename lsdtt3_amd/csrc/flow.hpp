@@ -18,6 +18,7 @@
 
 #include "common.hpp"
 #include "dbuf.hpp"
+#include "ktimer.hpp"
 
 namespace lsdtt_gpu {
 
@@ -59,6 +60,7 @@ public:
     dbuf<unsigned long long> pj, key, sort_keys_a, sort_keys_b;
     dbuf<int> flags, scan_out, misc, pjflags;
     dscratch tmp;
+    KernelTimer* timer = nullptr;  // optional, owned by the caller
 
     // per-level processing split: levels [0, tail_from) get one launch each
     // on the upstream/downstream sweeps, levels [tail_from, max_depth] run in

@@ -611,6 +611,7 @@ static FlowOptions lem_flow_options(int nr, int nc, double dx, BC bc)
 LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, const LemParams& p_)
     : nr(nr_), nc(nc_), n((long long)nr_ * nc_), dx(dx_), p(p_), flow(lem_flow_options(nr_, nc_, dx_, p_.bc))
 {
+    flow.timer = &timer;
     mask.reserve(n);
     dof.reserve(n);
     z.reserve(n);
@@ -700,12 +701,16 @@ void LemEngine::stream_power(hipStream_t s)
     for (int L = 1; L < split && L < nl; ++L) {
         const int b = flow.level_start_h[L], e = flow.level_start_h[L + 1];
         if (e > b) {
+            timer.begin("stream_power", s);
             k_sp_level<<<div_up(e - b, kB), kB, 0, s>>>(flow.order.p, b, e, a);
+            timer.end(s, 37.0 * (e - b));
             LSDTT_LAUNCH_CHECK();
         }
     }
     if (split < nl) {
+        timer.begin("stream_power_tail", s);
         k_sp_tail<<<1, 1024, 0, s>>>(flow.order.p, flow.level_start.p, split, nl - 1, a);
+        timer.end(s, 37.0 * (flow.level_start_h[nl] - flow.level_start_h[split]));
         LSDTT_LAUNCH_CHECK();
     }
 }
@@ -740,11 +745,21 @@ int LemEngine::bicgstab(hipStream_t s)
     int hf[3] = {0, 0, 0};
     for (int done = 0; !done;) {
         for (int k = 0; k < kChunk; ++k) {
+            timer.begin("bicgstab_p", s);
             k_bi_p<<<g, kB, 0, s>>>(a);
+            timer.end(s, 24.0 * n);
+            timer.begin("bicgstab_matvec", s);
             k_bi_v<<<g, kB, 0, s>>>(a);
+            timer.end(s, 48.0 * n);
+            timer.begin("bicgstab_s", s);
             k_bi_s<<<g, kB, 0, s>>>(a);
+            timer.end(s, 20.0 * n);
+            timer.begin("bicgstab_matvec", s);
             k_bi_t<<<g, kB, 0, s>>>(a);
+            timer.end(s, 48.0 * n);
+            timer.begin("bicgstab_x", s);
             k_bi_x<<<g, kB, 0, s>>>(a);
+            timer.end(s, 32.0 * n);
             LSDTT_LAUNCH_CHECK();
         }
         LSDTT_HIP(hipMemcpyAsync(hf, flags.p, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
@@ -757,7 +772,9 @@ int LemEngine::bicgstab(hipStream_t s)
 void LemEngine::trial(double lambda, double* fnorm, double* ratio, hipStream_t s)
 {
     HsDom d{nr, nc, p.bc, mask.p, dof.p, dx, p.D, 0.0, p.Sc, p.slope_ratio_cap, cur_inv_dt, p.nonlinear, zn_hs.p};
+    timer.begin("hillslope_residual", s);
     k_hs_trial<<<hs_grid, kB, 0, s>>>(d, z.p, x.p, lambda, n, partials.p, tickets.p, scal.p + S_TMP0);
+    timer.end(s, 22.0 * n);
     LSDTT_LAUNCH_CHECK();
     double h[2];
     LSDTT_HIP(hipMemcpyAsync(h, scal.p + S_TMP0, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
@@ -787,7 +804,9 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
             converged = true;
             break;
         }
+        timer.begin("hillslope_assemble", s);
         k_hs_assemble<<<hs_grid, kB, 0, s>>>(d, z.p, n, coef.p, dinv.p, b.p);
+        timer.end(s, 62.0 * n);
         LSDTT_LAUNCH_CHECK();
         st->krylov_iterations += bicgstab(s);
         // line search (newton .cpp:553-633)
@@ -817,7 +836,9 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
             lambda *= 0.5;
         }
         if (!accepted) break;
+        timer.begin("hillslope_update", s);
         k_hs_apply<<<hs_grid, kB, 0, s>>>(dof.p, z.p, x.p, lambda, n);
+        timer.end(s, 21.0 * n);
         LSDTT_LAUNCH_CHECK();
         fnorm = fnorm_try;
         ratio_now = ratio_try;
@@ -879,7 +900,9 @@ void LemEngine::stats(LemStats* st, hipStream_t s)
     absdz.reserve(n);
     hist.reserve(kRadixBins + 8);
     const double udz = p.enable_uplift ? p.U * p.dt : 0.0;
+    timer.begin("step_stats", s);
     k_stats<<<hs_grid, kB, 0, s>>>(mask.p, z_before.p, z.p, n, udz, absdz.p, partials.p, tickets.p, scal.p + S_SUM_B);
+    timer.end(s, 21.0 * n);
     LSDTT_LAUNCH_CHECK();
     double h[8];
     LSDTT_HIP(hipMemcpyAsync(h, scal.p + S_SUM_B, sizeof(h), hipMemcpyDeviceToHost, s));

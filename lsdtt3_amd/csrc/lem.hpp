@@ -21,6 +21,7 @@
 #include "common.hpp"
 #include "dbuf.hpp"
 #include "flow.hpp"
+#include "ktimer.hpp"
 
 namespace lsdtt_gpu {
 
@@ -88,6 +89,7 @@ public:
     dbuf<unsigned> hist;
 
     int hs_grid = 1024;
+    KernelTimer timer;
     double cur_inv_dt = 0.0;
 
 private:

@@ -130,3 +130,19 @@ class LandscapeModel:
         for _ in range(int(n_steps)):
             self.step()
         return self.last_stats
+
+    # ---- per-kernel event timing (roofline measurement) -------------------
+    def profile(self, enable: bool = True) -> None:
+        """start (clearing totals) or stop HIP-event timing of the kernels"""
+        N.check(N.lib().lsdtt_gpu_lem_profile(self._h, int(enable)))
+
+    def profile_totals(self) -> dict:
+        """{kernel group: (ms, launches, algorithmic bytes)}"""
+        L = N.lib()
+        out = {}
+        for i in range(L.lsdtt_gpu_lem_profile_count(self._h)):
+            name = C.create_string_buffer(64)
+            ms, nl, b = C.c_double(), C.c_int64(), C.c_double()
+            N.check(L.lsdtt_gpu_lem_profile_get(self._h, i, name, 64, C.byref(ms), C.byref(nl), C.byref(b)))
+            out[name.value.decode()] = (ms.value, nl.value, b.value)
+        return out
