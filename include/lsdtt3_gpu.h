@@ -173,6 +173,9 @@ typedef struct {
     double hillslope_residual;         /* final ||F||inf, m/yr */
     double hillslope_tolerance;
     double max_slope_ratio;
+    int32_t flow_levels;              /* depth-from-outlet levels this step */
+    int32_t flow_tail_from;           /* first level run by one workgroup */
+    int64_t n_outlets;
 } lsdtt_gpu_lem_stats;
 
 /* Defaults of lsdtt_lem_options, plus the Newton defaults above. */

@@ -1,0 +1,631 @@
+// hillslope.hip -- hillslope operator kernels (see hillslope.hpp).
+//
+// Reference (src/cpu/): lsdtt_hillslope_operator_cpu.hpp:146-159 (neighbour
+// resolution), :290-321 (face alphas); lsdtt_hillslope_newton_cpu.cpp:
+// 137-186 (dalpha_dz), :283-308 (residual), :415-471 (Jacobian entries),
+// :313-328 (max centred-gradient ratio).
+//
+// Every kernel has two paths with identical arithmetic: a fast path for a
+// DOF whose 3x3 block is interior and fully valid (all neighbour
+// resolutions are plain offsets, values come from nine registers), and the
+// general path that resolves every neighbour by the boundary rules.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "hillslope.hpp"
+#include "ktimer.hpp"
+#include "reduce.hpp"
+
+namespace lsdtt_gpu {
+
+namespace {
+
+constexpr int kB = 256;
+
+// ---------------------------------------------------------------------------
+// general path (any cell): the reference's neighbour resolution
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int nbr(const HsDom& d, int r, int c, int dr, int dc)
+{
+    int rr = r + dr, cc = c + dc;
+    if (rr < 0) rr = (d.bc.n == 'p') ? d.nr - 1 : r;
+    else if (rr >= d.nr) rr = (d.bc.s == 'p') ? 0 : r;
+    if (cc < 0) cc = (d.bc.w == 'p') ? d.nc - 1 : c;
+    else if (cc >= d.nc) cc = (d.bc.e == 'p') ? 0 : c;
+    int f = rr * d.nc + cc;
+    if (!d.mask[f]) f = r * d.nc + c;
+    return f;
+}
+
+// elevation z, or z + lam * dz for a line-search trial
+struct ZView {
+    const double* z;
+    const float* dz;
+    double lam;
+    __device__ __forceinline__ double operator()(int i) const
+    {
+        return dz ? z[i] + lam * static_cast<double>(dz[i]) : z[i];
+    }
+};
+
+__device__ __forceinline__ double sx_g(const HsDom& d, const ZView& Z, int r, int c)
+{
+    return (Z(nbr(d, r, c, 0, 1)) - Z(nbr(d, r, c, 0, -1))) / (2.0 * d.h);
+}
+__device__ __forceinline__ double sy_g(const HsDom& d, const ZView& Z, int r, int c)
+{
+    return (Z(nbr(d, r, c, 1, 0)) - Z(nbr(d, r, c, -1, 0))) / (2.0 * d.h);
+}
+
+constexpr int nesw_r[4] = {-1, 0, 1, 0};
+constexpr int nesw_c[4] = {0, 1, 0, -1};
+
+// c9[slot] += w with a runtime slot, as predicated register updates (a
+// runtime index into a register array would move it to scratch memory)
+__device__ __forceinline__ void add_slot(double c9[9], int slot, double w)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c9[k] += (k == slot) ? w : 0.0;
+}
+
+// slot of a resolved target in the 3x3 block of (r, c)
+__device__ __forceinline__ int slot_of(const HsDom& d, int r, int c, int t)
+{
+    const int tr = t / d.nc, tc = t - tr * d.nc;
+    int dr = tr - r, dc = tc - c;
+    if (dr > 1) dr -= d.nr;
+    else if (dr < -1) dr += d.nr;
+    if (dc > 1) dc -= d.nc;
+    else if (dc < -1) dc += d.nc;
+    return (dr + 1) * 3 + (dc + 1);
+}
+
+// cell of slot k of (r, c) for a product: periodic axes wrap, others clamp
+// (a clamped slot always carries a zero coefficient)
+__device__ __forceinline__ int slot_cell(const HsDom& d, int r, int c, int k)
+{
+    int rr = r + k / 3 - 1, cc = c + k % 3 - 1;
+    if (rr < 0) rr = (d.bc.n == 'p') ? d.nr - 1 : 0;
+    else if (rr >= d.nr) rr = (d.bc.s == 'p') ? 0 : d.nr - 1;
+    if (cc < 0) cc = (d.bc.w == 'p') ? d.nc - 1 : 0;
+    else if (cc >= d.nc) cc = (d.bc.e == 'p') ? 0 : d.nc - 1;
+    return rr * d.nc + cc;
+}
+
+// F (FP64) and the 9 Jacobian coefficients (FP64, into c9 when non-null) of
+// DOF (r, c) by the general rules.  Only cells on the grid edge or next to
+// a masked cell take it, in companion kernels of their own, so the hot
+// kernels do not carry its registers.
+__device__ double cell_general(const HsDom& d, const ZView& Z, int r, int c, double* c9)
+{
+    const double h = d.h, DA = d.D / (h * h), Sc2 = d.Sc * d.Sc;
+    const int self = r * d.nc + c;
+    const double zc = Z(self);
+    const double sxc = d.nonlinear ? sx_g(d, Z, r, c) : 0.0;
+    const double syc = d.nonlinear ? sy_g(d, Z, r, c) : 0.0;
+    double a[4], sn[4], st[4];
+    int nb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        nb[k] = nbr(d, r, c, nesw_r[k], nesw_c[k]);
+        sn[k] = (Z(nb[k]) - zc) / h;
+        st[k] = 0.0;
+        if (!d.nonlinear) {
+            a[k] = 1.0;
+            continue;
+        }
+        const int rr = nb[k] / d.nc, cc = nb[k] - rr * d.nc;
+        st[k] = (k == 0 || k == 2) ? 0.5 * (sxc + sx_g(d, Z, rr, cc)) : 0.5 * (syc + sy_g(d, Z, rr, cc));
+        double s2 = (sn[k] * sn[k] + st[k] * st[k]) / Sc2;
+        if (s2 > d.cap) s2 = d.cap;
+        a[k] = 1.0 / (1.0 - s2);
+    }
+    double flux = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) flux += a[k] * sn[k] * h;
+    double F = -d.U - DA * flux;
+    if (d.inv_dt > 0.0) F += d.inv_dt * (zc - d.zn[self]);
+    if (!c9) return F;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c9[k] = 0.0;
+    auto add = [&](int t, double w) {
+        if (w == 0.0 || !d.dof[t]) return;
+        add_slot(c9, slot_of(d, r, c, t), w);
+    };
+    if (d.inv_dt > 0.0) add(self, d.inv_dt);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        add(nb[q], -DA * a[q]);
+        add(self, +DA * a[q]);
+        if (!d.nonlinear) continue;
+        const double rraw = (sn[q] * sn[q] + st[q] * st[q]) / Sc2;
+        if (rraw >= d.cap) continue;
+        const int rr = nb[q] / d.nc, cc = nb[q] - rr * d.nc;
+        const double pref = a[q] * a[q] * 2.0 / Sc2;
+        const double dzf = sn[q] * h;
+        const double wt = pref * st[q] * (1.0 / (4.0 * h));
+        add(self, -DA * dzf * (pref * sn[q] * (-1.0 / h)));
+        add(nb[q], -DA * dzf * (pref * sn[q] * (+1.0 / h)));
+        if (q == 0 || q == 2) {
+            add(nbr(d, r, c, 0, 1), -DA * dzf * (+wt));
+            add(nbr(d, r, c, 0, -1), -DA * dzf * (-wt));
+            add(nbr(d, rr, cc, 0, 1), -DA * dzf * (+wt));
+            add(nbr(d, rr, cc, 0, -1), -DA * dzf * (-wt));
+        } else {
+            add(nbr(d, r, c, 1, 0), -DA * dzf * (+wt));
+            add(nbr(d, r, c, -1, 0), -DA * dzf * (-wt));
+            add(nbr(d, rr, cc, 1, 0), -DA * dzf * (+wt));
+            add(nbr(d, rr, cc, -1, 0), -DA * dzf * (-wt));
+        }
+    }
+    return F;
+}
+
+__device__ __forceinline__ double ratio_general(const HsDom& d, const ZView& Z, int r, int c)
+{
+    const double sx = sx_g(d, Z, r, c), sy = sy_g(d, Z, r, c);
+    return sqrt(sx * sx + sy * sy) / d.Sc;
+}
+
+// ---------------------------------------------------------------------------
+// fast path: the 3x3 block z[0..8] (row-major, centre 4) is interior and
+// valid, so every resolution is a plain offset
+// ---------------------------------------------------------------------------
+// NESW neighbour slots, and per face the tangential slots (own +/-, then
+// the neighbour's +/-)
+// (constexpr, so that unrolled loops index registers, never scratch)
+constexpr int f_nb[4] = {1, 5, 7, 3};
+constexpr int f_tp[4] = {5, 7, 5, 7};
+constexpr int f_tm[4] = {3, 1, 3, 1};
+constexpr int f_np[4] = {2, 8, 8, 6};
+constexpr int f_nm[4] = {0, 2, 6, 0};
+
+template <class T>
+__device__ __forceinline__ void faces_fast(const HsDom& d, const double z[9], T sn[4], T st[4], T a[4], bool capped[4])
+{
+    const T inv_h = static_cast<T>(1.0 / d.h), inv_2h = static_cast<T>(0.5 / d.h);
+    const T inv_Sc2 = static_cast<T>(1.0 / (d.Sc * d.Sc)), cap = static_cast<T>(d.cap);
+    // differences in FP64 (elevations are ~1e3 m, slopes need ~1e-6), then T
+    const T sxc = static_cast<T>(z[5] - z[3]) * inv_2h;
+    const T syc = static_cast<T>(z[7] - z[1]) * inv_2h;
+    const T sxn[4] = {static_cast<T>(z[2] - z[0]) * inv_2h, static_cast<T>(z[8] - z[2]) * inv_2h,
+                      static_cast<T>(z[8] - z[6]) * inv_2h, static_cast<T>(z[6] - z[0]) * inv_2h};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        sn[k] = static_cast<T>(z[f_nb[k]] - z[4]) * inv_h;
+        if (!d.nonlinear) {
+            st[k] = T(0);
+            a[k] = T(1);
+            capped[k] = true;
+            continue;
+        }
+        st[k] = T(0.5) * (((k & 1) ? syc : sxc) + sxn[k]);
+        const T raw = (sn[k] * sn[k] + st[k] * st[k]) * inv_Sc2;
+        capped[k] = raw >= cap;
+        a[k] = T(1) / (T(1) - (raw > cap ? cap : raw));
+    }
+}
+
+// FP64 residual on the fast path (same expressions as cell_general)
+__device__ __forceinline__ double F_fast(const HsDom& d, const double z[9], double zn_c)
+{
+    const double h = d.h, DA = d.D / (h * h);
+    double a[4], sn[4], st[4];
+    bool cp[4];
+    // same order of operations as cell_general: (zb - zc) / h etc.
+    const double sxc = d.nonlinear ? (z[5] - z[3]) / (2.0 * h) : 0.0;
+    const double syc = d.nonlinear ? (z[7] - z[1]) / (2.0 * h) : 0.0;
+    const double sxn[4] = {(z[2] - z[0]) / (2.0 * h), (z[8] - z[2]) / (2.0 * h), (z[8] - z[6]) / (2.0 * h),
+                           (z[6] - z[0]) / (2.0 * h)};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        sn[k] = (z[f_nb[k]] - z[4]) / h;
+        if (!d.nonlinear) {
+            a[k] = 1.0;
+            continue;
+        }
+        st[k] = 0.5 * (((k & 1) ? syc : sxc) + sxn[k]);
+        double s2 = (sn[k] * sn[k] + st[k] * st[k]) / (d.Sc * d.Sc);
+        if (s2 > d.cap) s2 = d.cap;
+        a[k] = 1.0 / (1.0 - s2);
+    }
+    (void)cp;
+    double flux = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) flux += a[k] * sn[k] * h;
+    double F = -d.U - DA * flux;
+    if (d.inv_dt > 0.0) F += d.inv_dt * (z[4] - zn_c);
+    return F;
+}
+
+// Jacobian row on the fast path in precision T
+template <class T>
+__device__ __forceinline__ void J_fast(const HsDom& d, const double z[9], T c9[9])
+{
+    T sn[4], st[4], a[4];
+    bool capped[4];
+    faces_fast<T>(d, z, sn, st, a, capped);
+    const T h = static_cast<T>(d.h), DA = static_cast<T>(d.D / (d.h * d.h));
+    const T two_inv_Sc2 = static_cast<T>(2.0 / (d.Sc * d.Sc)), inv_h = static_cast<T>(1.0 / d.h);
+    const T inv_4h = static_cast<T>(0.25 / d.h);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c9[k] = T(0);
+    c9[4] = static_cast<T>(d.inv_dt);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        c9[f_nb[q]] -= DA * a[q];
+        c9[4] += DA * a[q];
+        if (capped[q]) continue;
+        const T pref = a[q] * a[q] * two_inv_Sc2;
+        const T g = -DA * (sn[q] * h);
+        const T wt = pref * st[q] * inv_4h;
+        c9[4] -= g * (pref * sn[q] * inv_h);
+        c9[f_nb[q]] += g * (pref * sn[q] * inv_h);
+        c9[f_tp[q]] += g * wt;
+        c9[f_tm[q]] -= g * wt;
+        c9[f_np[q]] += g * wt;
+        c9[f_nm[q]] -= g * wt;
+    }
+}
+
+__device__ __forceinline__ void load9(const double* __restrict__ z, int f, int nc, double out[9])
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) out[k] = z[f + (k / 3 - 1) * nc + (k % 3 - 1)];
+}
+
+// ---------------------------------------------------------------------------
+// kernels.  Each hot kernel covers the fast cells (and writes zeros on
+// non-DOF cells); a companion kernel over the list of general DOF cells
+// runs first and leaves its block partials after the hot kernel's, which
+// the hot kernel's last block folds in (grid_reduce `extra`).
+// ---------------------------------------------------------------------------
+constexpr int kGenBlocks = 64;
+
+__host__ __device__ inline int gen_grid(int n_gen)
+{
+    return n_gen > 0 ? ((n_gen + kB - 1) / kB < kGenBlocks ? (n_gen + kB - 1) / kB : kGenBlocks) : 0;
+}
+
+__global__ void __launch_bounds__(256) k_fast_flags(HsDom d, uint8_t* __restrict__ fast)
+{
+    const long long n = (long long)d.nr * d.nc;
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
+         f += (long long)gridDim.x * blockDim.x) {
+        const int r = static_cast<int>(f / d.nc), c = static_cast<int>(f - (long long)r * d.nc);
+        uint8_t ok = d.dof[f] && r > 0 && r < d.nr - 1 && c > 0 && c < d.nc - 1;
+        if (ok)
+            for (int k = 0; k < 9; ++k) ok &= d.mask[f + (k / 3 - 1) * d.nc + (k % 3 - 1)] ? 1 : 0;
+        fast[f] = ok;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_prepare(HsDom d, const double* __restrict__ z, long long n,
+                                                 float* __restrict__ dinv, float* __restrict__ b)
+{
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
+         f += (long long)gridDim.x * blockDim.x) {
+        if (d.fast[f]) {
+            double zz[9];
+            load9(z, static_cast<int>(f), d.nc, zz);
+            b[f] = static_cast<float>(-F_fast(d, zz, d.zn[f]));
+            float c9[9];
+            J_fast<float>(d, zz, c9);
+            dinv[f] = c9[4] != 0.0f ? 1.0f / c9[4] : 0.0f;
+        } else if (!d.dof[f]) {
+            b[f] = 0.0f;
+            dinv[f] = 0.0f;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_prepare_gen(HsDom d, const double* __restrict__ z, float* __restrict__ dinv,
+                                                     float* __restrict__ b)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_gen; i += gridDim.x * blockDim.x) {
+        const int f = d.gen[i];
+        const int r = f / d.nc, c = f - r * d.nc;
+        double c9[9];
+        b[f] = static_cast<float>(-cell_general(d, ZView{z, nullptr, 0.0}, r, c, c9));
+        dinv[f] = c9[4] != 0.0 ? static_cast<float>(1.0 / c9[4]) : 0.0f;
+    }
+}
+
+// y = J x (FP32) on a fast row
+__device__ __forceinline__ float jx_fast(const HsDom& d, const double* __restrict__ z, const float* __restrict__ x,
+                                         long long f)
+{
+    double zz[9];
+    load9(z, static_cast<int>(f), d.nc, zz);
+    float c9[9];
+    J_fast<float>(d, zz, c9);
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc += c9[k] * x[f + (k / 3 - 1) * d.nc + (k % 3 - 1)];
+    return acc;
+}
+
+// y = J x on a general row (x is zero on non-DOF cells)
+__device__ __forceinline__ float jx_gen(const HsDom& d, const double* __restrict__ z, const float* __restrict__ x,
+                                        int f)
+{
+    const int r = f / d.nc, c = f - r * d.nc;
+    double c9[9];
+    cell_general(d, ZView{z, nullptr, 0.0}, r, c, c9);
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc += static_cast<float>(c9[k]) * x[slot_cell(d, r, c, k)];
+    return acc;
+}
+
+__global__ void __launch_bounds__(256) k_bi_init(BiArgs a)
+{
+    double acc[2] = {0.0, 0.0};
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
+         f += (long long)gridDim.x * blockDim.x) {
+        const float bv = a.b[f];
+        a.r[f] = bv;
+        a.rhat[f] = bv;
+        a.p[f] = 0.0f;
+        a.v[f] = 0.0f;
+        a.x[f] = 0.0f;
+        acc[0] += static_cast<double>(bv) * bv;
+    }
+    acc[1] = acc[0];
+    const bool mx[2] = {false, false};
+    double out[2];
+    if (grid_reduce<2>(acc, mx, a.partials, a.ticket, out)) {
+        a.scal[HS_RHO] = out[0];
+        a.scal[HS_BNORM2] = out[1];
+        a.scal[HS_RHO_OLD] = 1.0;
+        a.scal[HS_ALPHA] = 1.0;
+        a.scal[HS_OMEGA] = 1.0;
+        a.flags[0] = (out[1] == 0.0) ? 1 : 0;
+        a.flags[1] = 0;
+        a.flags[2] = 0;
+    }
+}
+
+// p = r + beta (p - omega v); phat = M^-1 p
+__global__ void __launch_bounds__(256) k_bi_p(BiArgs a)
+{
+    if (a.flags[0]) return;
+    const double beta = (a.scal[HS_RHO] / a.scal[HS_RHO_OLD]) * (a.scal[HS_ALPHA] / a.scal[HS_OMEGA]);
+    const float bt = static_cast<float>(beta), om = static_cast<float>(a.scal[HS_OMEGA]);
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
+         f += (long long)gridDim.x * blockDim.x) {
+        const float pv = a.r[f] + bt * (a.p[f] - om * a.v[f]);
+        a.p[f] = pv;
+        a.phat[f] = a.dinv[f] * pv;
+    }
+}
+
+// companion: out = J in on the general rows, block partials of the dots
+// (w1 . out) and (w2 . out) (w2 may be null) into slots base + block
+__global__ void __launch_bounds__(256) k_bi_mv_gen(BiArgs a, const float* __restrict__ in, float* __restrict__ out,
+                                                   const float* __restrict__ w1, const float* __restrict__ w2,
+                                                   int nv, int base)
+{
+    if (a.flags[0]) return;
+    double acc[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.d.n_gen; i += gridDim.x * blockDim.x) {
+        const int f = a.d.gen[i];
+        const float y = jx_gen(a.d, a.z, in, f);
+        out[f] = y;
+        acc[0] += static_cast<double>(w1[f]) * y;
+        if (w2) acc[1] += static_cast<double>(w2[f]) * y;
+    }
+    const bool mx[2] = {false, false};
+    if (nv == 1) {
+        double v1[1] = {acc[0]};
+        const bool m1[1] = {false};
+        block_partial<1>(v1, m1, a.partials, base + blockIdx.x);
+    } else {
+        block_partial<2>(acc, mx, a.partials, base + blockIdx.x);
+    }
+}
+
+// v = J phat; alpha = rho / (rhat . v)
+__global__ void __launch_bounds__(256) k_bi_v(BiArgs a)
+{
+    if (a.flags[0]) return;
+    double acc[1] = {0.0};
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
+         f += (long long)gridDim.x * blockDim.x) {
+        if (a.d.fast[f]) {
+            const float vv = jx_fast(a.d, a.z, a.phat, f);
+            a.v[f] = vv;
+            acc[0] += static_cast<double>(a.rhat[f]) * vv;
+        } else if (!a.d.dof[f]) {
+            a.v[f] = 0.0f;
+        }
+    }
+    const bool mx[1] = {false};
+    double out[1];
+    if (grid_reduce<1>(acc, mx, a.partials, a.ticket, out, gen_grid(a.d.n_gen))) {
+        if (out[0] == 0.0) {
+            a.flags[0] = 1;
+            a.flags[2] = 1;
+        } else {
+            a.scal[HS_ALPHA] = a.scal[HS_RHO] / out[0];
+        }
+    }
+}
+
+// s = r - alpha v; shat = M^-1 s
+__global__ void __launch_bounds__(256) k_bi_s(BiArgs a)
+{
+    if (a.flags[0]) return;
+    const float al = static_cast<float>(a.scal[HS_ALPHA]);
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
+         f += (long long)gridDim.x * blockDim.x) {
+        const float sv = a.r[f] - al * a.v[f];
+        a.s[f] = sv;
+        a.shat[f] = a.dinv[f] * sv;
+    }
+}
+
+// t = J shat; omega = (t . s) / (t . t)
+__global__ void __launch_bounds__(256) k_bi_t(BiArgs a)
+{
+    if (a.flags[0]) return;
+    double acc[2] = {0.0, 0.0};
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
+         f += (long long)gridDim.x * blockDim.x) {
+        if (a.d.fast[f]) {
+            const float tv = jx_fast(a.d, a.z, a.shat, f);
+            a.t[f] = tv;
+            acc[0] += static_cast<double>(tv) * a.s[f];
+            acc[1] += static_cast<double>(tv) * tv;
+        } else if (!a.d.dof[f]) {
+            a.t[f] = 0.0f;
+        }
+    }
+    const bool mx[2] = {false, false};
+    double out[2];
+    if (grid_reduce<2>(acc, mx, a.partials, a.ticket, out, gen_grid(a.d.n_gen)))
+        a.scal[HS_OMEGA] = out[1] > 0.0 ? out[0] / out[1] : 0.0;
+}
+
+// x += alpha phat + omega shat; r = s - omega t; convergence on ||r||
+__global__ void __launch_bounds__(256) k_bi_x(BiArgs a)
+{
+    if (a.flags[0]) return;
+    const float al = static_cast<float>(a.scal[HS_ALPHA]), om = static_cast<float>(a.scal[HS_OMEGA]);
+    double acc[2] = {0.0, 0.0};
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
+         f += (long long)gridDim.x * blockDim.x) {
+        a.x[f] += al * a.phat[f] + om * a.shat[f];
+        const float rv = a.s[f] - om * a.t[f];
+        a.r[f] = rv;
+        acc[0] += static_cast<double>(a.rhat[f]) * rv;
+        acc[1] += static_cast<double>(rv) * rv;
+    }
+    const bool mx[2] = {false, false};
+    double out[2];
+    if (grid_reduce<2>(acc, mx, a.partials, a.ticket, out)) {
+        a.scal[HS_RHO_OLD] = a.scal[HS_RHO];
+        a.scal[HS_RHO] = out[0];
+        a.scal[HS_RNORM2] = out[1];
+        const int it = a.flags[1] + 1;
+        a.flags[1] = it;
+        const bool conv = out[1] <= a.eta2 * a.scal[HS_BNORM2];
+        const bool stall = (a.scal[HS_OMEGA] == 0.0) || (out[0] == 0.0);
+        if (conv || it >= a.max_it || stall) a.flags[0] = 1;
+        if (stall && !conv) a.flags[2] = 1;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_trial_gen(HsDom d, const double* __restrict__ z, const float* __restrict__ dz,
+                                                   double lam, double* partials, int base)
+{
+    const ZView Z{z, lam != 0.0 ? dz : nullptr, lam};
+    double acc[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_gen; i += gridDim.x * blockDim.x) {
+        const int f = d.gen[i];
+        const int r = f / d.nc, c = f - r * d.nc;
+        acc[0] = fmax(acc[0], fabs(cell_general(d, Z, r, c, nullptr)));
+        if (d.nonlinear) acc[1] = fmax(acc[1], ratio_general(d, Z, r, c));
+    }
+    const bool mx[2] = {true, true};
+    block_partial<2>(acc, mx, partials, base + blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_trial(HsDom d, const double* __restrict__ z, const float* __restrict__ dz,
+                                               double lam, long long n, double* partials, unsigned* ticket,
+                                               double* out2)
+{
+    const ZView Z{z, lam != 0.0 ? dz : nullptr, lam};
+    double acc[2] = {0.0, 0.0};
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
+         f += (long long)gridDim.x * blockDim.x) {
+        if (!d.fast[f]) continue;
+        double zz[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) zz[k] = Z(static_cast<int>(f) + (k / 3 - 1) * d.nc + (k % 3 - 1));
+        acc[0] = fmax(acc[0], fabs(F_fast(d, zz, d.zn[f])));
+        if (d.nonlinear) {
+            const double sx = (zz[5] - zz[3]) / (2.0 * d.h), sy = (zz[7] - zz[1]) / (2.0 * d.h);
+            acc[1] = fmax(acc[1], sqrt(sx * sx + sy * sy) / d.Sc);
+        }
+    }
+    const bool mx[2] = {true, true};
+    double out[2];
+    if (grid_reduce<2>(acc, mx, partials, ticket, out, gen_grid(d.n_gen))) {
+        out2[0] = out[0];
+        out2[1] = out[1];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_apply(const uint8_t* __restrict__ dof, double* __restrict__ z,
+                                               const float* __restrict__ dz, double lam, long long n)
+{
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
+         f += (long long)gridDim.x * blockDim.x)
+        if (dof[f]) z[f] = z[f] + lam * static_cast<double>(dz[f]);
+}
+
+}  // namespace
+
+int hs_general_blocks(int n_gen) { return gen_grid(n_gen); }
+
+void hs_fast_flags(const HsDom& d, uint8_t* fast, hipStream_t s)
+{
+    const long long n = (long long)d.nr * d.nc;
+    k_fast_flags<<<std::max(1, std::min(4096, div_up(n, kB))), kB, 0, s>>>(d, fast);
+    LSDTT_LAUNCH_CHECK();
+}
+
+void hs_prepare(const HsDom& d, const double* z, long long n, float* dinv, float* b, int grid, hipStream_t s)
+{
+    if (d.n_gen > 0) k_prepare_gen<<<gen_grid(d.n_gen), kB, 0, s>>>(d, z, dinv, b);
+    k_prepare<<<grid, kB, 0, s>>>(d, z, n, dinv, b);
+    LSDTT_LAUNCH_CHECK();
+}
+
+void hs_bicgstab_init(const BiArgs& a, int grid, hipStream_t s)
+{
+    k_bi_init<<<grid, kB, 0, s>>>(a);
+    LSDTT_LAUNCH_CHECK();
+}
+
+void hs_bicgstab_iteration(const BiArgs& a, int grid, hipStream_t s, KernelTimer* t)
+{
+    const double n = static_cast<double>(a.n);
+    const int gg = gen_grid(a.d.n_gen);
+    if (t) t->begin("bicgstab_p", s);
+    k_bi_p<<<grid, kB, 0, s>>>(a);
+    if (t) t->end(s, 24.0 * n);  // r, p, v, dinv in; p, phat out
+    if (gg) k_bi_mv_gen<<<gg, kB, 0, s>>>(a, a.phat, a.v, a.rhat, nullptr, 1, grid);
+    if (t) t->begin("bicgstab_matvec", s);
+    k_bi_v<<<grid, kB, 0, s>>>(a);
+    if (t) t->end(s, 22.0 * n);  // z 8, phat 4, fast 1, rhat 4 in; v 4 out (+ dof 1)
+    if (t) t->begin("bicgstab_s", s);
+    k_bi_s<<<grid, kB, 0, s>>>(a);
+    if (t) t->end(s, 20.0 * n);
+    if (gg) k_bi_mv_gen<<<gg, kB, 0, s>>>(a, a.shat, a.t, a.s, a.t, 2, grid);
+    if (t) t->begin("bicgstab_matvec", s);
+    k_bi_t<<<grid, kB, 0, s>>>(a);
+    if (t) t->end(s, 22.0 * n);
+    if (t) t->begin("bicgstab_x", s);
+    k_bi_x<<<grid, kB, 0, s>>>(a);
+    if (t) t->end(s, 32.0 * n);
+    LSDTT_LAUNCH_CHECK();
+}
+
+void hs_trial(const HsDom& d, const double* z, const float* dz, double lam, long long n, double* partials,
+              unsigned* ticket, double* out2, int grid, hipStream_t s)
+{
+    if (d.n_gen > 0) k_trial_gen<<<gen_grid(d.n_gen), kB, 0, s>>>(d, z, dz, lam, partials, grid);
+    k_trial<<<grid, kB, 0, s>>>(d, z, dz, lam, n, partials, ticket, out2);
+    LSDTT_LAUNCH_CHECK();
+}
+
+void hs_apply(const uint8_t* dof, double* z, const float* dz, double lam, long long n, int grid, hipStream_t s)
+{
+    k_apply<<<grid, kB, 0, s>>>(dof, z, dz, lam, n);
+    LSDTT_LAUNCH_CHECK();
+}
+
+}  // namespace lsdtt_gpu

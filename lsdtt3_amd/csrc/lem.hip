@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "hillslope.hpp"
 #include "lem.hpp"
 #include "reduce.hpp"
 
@@ -35,10 +36,9 @@ namespace {
 constexpr int kB = 256;
 
 // scalar slots (double) of the hillslope solver
-enum {
-    S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_BNORM2, S_RNORM2, S_FNORM, S_RATIO, S_TMP0, S_TMP1,
-    S_SUM_B, S_SUM_A, S_MAX_B, S_MAX_A, S_MAX_L, S_SUM_E, S_MAX_E, S_NV, S_COUNT
-};
+// statistics occupy 8 slots after the hillslope solver's scalars
+constexpr int kStatSlot = HS_NSCAL;
+constexpr int kScalCount = HS_NSCAL + 8;
 
 inline int grid_for(long long n, int cap) { return std::max(1, std::min(cap, div_up(n, kB))); }
 
@@ -96,8 +96,11 @@ __device__ __forceinline__ bool sp_node(const SpArgs& a, int v)
             if (slope <= 0.0) {
                 eps = 0.0;
             } else {
-                const double fz = zn - zo + kam * a.dt * pow(slope, a.nexp);
-                const double fd = 1.0 + kam * a.dt * (a.nexp / dx_eff) * pow(slope, a.nexp - 1.0);
+                // one pow per iteration: S^n = S^(n-1) * S (the reference
+                // evaluates both powers; the results agree to an ulp)
+                const double pn1 = pow(slope, a.nexp - 1.0);
+                const double fz = zn - zo + kam * a.dt * (pn1 * slope);
+                const double fd = 1.0 + kam * a.dt * (a.nexp / dx_eff) * pn1;
                 eps = fz / fd;
             }
             zn -= eps;
@@ -140,366 +143,6 @@ __global__ void __launch_bounds__(1024) k_sp_tail(const int* __restrict__ order,
         }
         __syncthreads();
     }
-}
-
-// ---------------------------------------------------------------------------
-// hillslope operator
-// ---------------------------------------------------------------------------
-struct HsDom {
-    int nr, nc;
-    BC bc;
-    const uint8_t* mask;
-    const uint8_t* dof;
-    double h, D, U, Sc, cap, inv_dt;
-    int nonlinear;
-    const double* zn;
-};
-
-// lsdtt_hillslope_domain::neighbour (operator .hpp:146-159)
-__device__ __forceinline__ int hs_nbr(const HsDom& d, int r, int c, int dr, int dc)
-{
-    int rr = r + dr, cc = c + dc;
-    if (rr < 0) rr = (d.bc.n == 'p') ? d.nr - 1 : r;
-    else if (rr >= d.nr) rr = (d.bc.s == 'p') ? 0 : r;
-    if (cc < 0) cc = (d.bc.w == 'p') ? d.nc - 1 : c;
-    else if (cc >= d.nc) cc = (d.bc.e == 'p') ? 0 : c;
-    int f = rr * d.nc + cc;
-    if (!d.mask[f]) f = r * d.nc + c;
-    return f;
-}
-
-// elevation view: z, or z + lambda * dz for a line-search trial
-struct ZView {
-    const double* z;
-    const float* dz;
-    double lam;
-    __device__ __forceinline__ double operator()(int i) const
-    {
-        return dz ? z[i] + lam * static_cast<double>(dz[i]) : z[i];
-    }
-};
-
-__device__ __forceinline__ double hs_sx(const HsDom& d, const ZView& Z, int r, int c)
-{
-    return (Z(hs_nbr(d, r, c, 0, 1)) - Z(hs_nbr(d, r, c, 0, -1))) / (2.0 * d.h);
-}
-__device__ __forceinline__ double hs_sy(const HsDom& d, const ZView& Z, int r, int c)
-{
-    return (Z(hs_nbr(d, r, c, 1, 0)) - Z(hs_nbr(d, r, c, -1, 0))) / (2.0 * d.h);
-}
-
-__constant__ static const int nesw_r[4] = {-1, 0, 1, 0};
-__constant__ static const int nesw_c[4] = {0, 1, 0, -1};
-
-// face alphas and normal slopes (operator .hpp:290-321)
-__device__ __forceinline__ void hs_faces(const HsDom& d, const ZView& Z, int r, int c, double a[4], double sn[4],
-                                         int nb[4])
-{
-    const double zc = Z(r * d.nc + c);
-    const double sxc = d.nonlinear ? hs_sx(d, Z, r, c) : 0.0;
-    const double syc = d.nonlinear ? hs_sy(d, Z, r, c) : 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        nb[k] = hs_nbr(d, r, c, nesw_r[k], nesw_c[k]);
-        sn[k] = (Z(nb[k]) - zc) / d.h;
-        if (!d.nonlinear) {
-            a[k] = 1.0;
-            continue;
-        }
-        const int rr = nb[k] / d.nc, cc = nb[k] - rr * d.nc;
-        const double st = (k == 0 || k == 2) ? 0.5 * (sxc + hs_sx(d, Z, rr, cc)) : 0.5 * (syc + hs_sy(d, Z, rr, cc));
-        double s2 = (sn[k] * sn[k] + st * st) / (d.Sc * d.Sc);
-        if (s2 > d.cap) s2 = d.cap;
-        a[k] = 1.0 / (1.0 - s2);
-    }
-}
-
-// residual (newton .cpp:283-308)
-__device__ __forceinline__ double hs_residual(const HsDom& d, const ZView& Z, int r, int c)
-{
-    const double DA = d.D / (d.h * d.h);
-    double a[4], sn[4];
-    int nb[4];
-    hs_faces(d, Z, r, c, a, sn, nb);
-    double flux = 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) flux += a[k] * sn[k] * d.h;
-    double f = -d.U - DA * flux;
-    if (d.inv_dt > 0.0) {
-        const int i = r * d.nc + c;
-        f += d.inv_dt * (Z(i) - d.zn[i]);
-    }
-    return f;
-}
-
-__device__ __forceinline__ double hs_ratio(const HsDom& d, const ZView& Z, int r, int c)
-{
-    const double sx = hs_sx(d, Z, r, c), sy = hs_sy(d, Z, r, c);
-    return sqrt(sx * sx + sy * sy) / d.Sc;
-}
-
-// map a resolved target cell to its slot in the 3x3 stencil of (r, c)
-__device__ __forceinline__ int hs_slot(const HsDom& d, int r, int c, int t)
-{
-    const int tr = t / d.nc, tc = t - tr * d.nc;
-    int dr = tr - r, dc = tc - c;
-    if (dr > 1) dr -= d.nr;
-    else if (dr < -1) dr += d.nr;
-    if (dc > 1) dc -= d.nc;
-    else if (dc < -1) dc += d.nc;
-    return (dr + 1) * 3 + (dc + 1);
-}
-
-// stencil target of slot k for the matrix-vector product: periodic axes
-// wrap, others clamp (a clamped slot always carries a zero coefficient)
-__device__ __forceinline__ int hs_target(const HsDom& d, int r, int c, int k)
-{
-    int rr = r + k / 3 - 1, cc = c + k % 3 - 1;
-    if (rr < 0) rr = (d.bc.n == 'p') ? d.nr - 1 : 0;
-    else if (rr >= d.nr) rr = (d.bc.s == 'p') ? 0 : d.nr - 1;
-    if (cc < 0) cc = (d.bc.w == 'p') ? d.nc - 1 : 0;
-    else if (cc >= d.nc) cc = (d.bc.e == 'p') ? 0 : d.nc - 1;
-    return rr * d.nc + cc;
-}
-
-// Jacobian row and right-hand side (newton .cpp:415-471): 9 FP32
-// coefficients per DOF (accumulated in FP64), b = -F, Jacobi inverse
-__global__ void k_hs_assemble(HsDom d, const double* __restrict__ z, long long n, float* __restrict__ coef,
-                              float* __restrict__ dinv, float* __restrict__ b)
-{
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
-         f += (long long)gridDim.x * blockDim.x) {
-        const int r = static_cast<int>(f / d.nc), c = static_cast<int>(f - (long long)r * d.nc);
-        double c9[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) c9[k] = 0.0;
-        float rhs = 0.0f;
-        if (d.dof[f]) {
-            const ZView Z{z, nullptr, 0.0};
-            const double DA = d.D / (d.h * d.h), h = d.h, Sc2 = d.Sc * d.Sc;
-            double a[4], sn[4];
-            int nb[4];
-            hs_faces(d, Z, r, c, a, sn, nb);
-            double flux = 0.0;
-            for (int k = 0; k < 4; ++k) flux += a[k] * sn[k] * h;
-            double F = -d.U - DA * flux;
-            if (d.inv_dt > 0.0) F += d.inv_dt * (z[f] - d.zn[f]);
-            rhs = static_cast<float>(-F);
-            auto add = [&](int t, double w) {
-                if (w == 0.0 || !d.dof[t]) return;
-                c9[hs_slot(d, r, c, t)] += w;
-            };
-            const int self = static_cast<int>(f);
-            if (d.inv_dt > 0.0) add(self, d.inv_dt);
-            for (int q = 0; q < 4; ++q) {
-                add(nb[q], -DA * a[q]);
-                add(self, +DA * a[q]);
-                if (!d.nonlinear) continue;
-                const int rr = nb[q] / d.nc, cc = nb[q] - rr * d.nc;
-                const bool tx = (q == 0 || q == 2);
-                const double st = tx ? 0.5 * (hs_sx(d, Z, r, c) + hs_sx(d, Z, rr, cc))
-                                     : 0.5 * (hs_sy(d, Z, r, c) + hs_sy(d, Z, rr, cc));
-                const double rraw = (sn[q] * sn[q] + st * st) / Sc2;
-                if (rraw >= d.cap) continue;
-                const double pref = a[q] * a[q] * 2.0 / Sc2;
-                const double dzf = sn[q] * h;
-                const double wt = pref * st * (1.0 / (4.0 * h));
-                add(self, -DA * dzf * (pref * sn[q] * (-1.0 / h)));
-                add(nb[q], -DA * dzf * (pref * sn[q] * (+1.0 / h)));
-                if (tx) {
-                    add(hs_nbr(d, r, c, 0, 1), -DA * dzf * (+wt));
-                    add(hs_nbr(d, r, c, 0, -1), -DA * dzf * (-wt));
-                    add(hs_nbr(d, rr, cc, 0, 1), -DA * dzf * (+wt));
-                    add(hs_nbr(d, rr, cc, 0, -1), -DA * dzf * (-wt));
-                } else {
-                    add(hs_nbr(d, r, c, 1, 0), -DA * dzf * (+wt));
-                    add(hs_nbr(d, r, c, -1, 0), -DA * dzf * (-wt));
-                    add(hs_nbr(d, rr, cc, 1, 0), -DA * dzf * (+wt));
-                    add(hs_nbr(d, rr, cc, -1, 0), -DA * dzf * (-wt));
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) coef[k * n + f] = static_cast<float>(c9[k]);
-        dinv[f] = (d.dof[f] && c9[4] != 0.0) ? static_cast<float>(1.0 / c9[4]) : 0.0f;
-        b[f] = rhs;
-    }
-}
-
-__device__ __forceinline__ float hs_matvec_row(const HsDom& d, const float* __restrict__ coef, long long n,
-                                               const float* __restrict__ x, int f, int r, int c)
-{
-    float acc = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) acc += coef[k * n + f] * x[hs_target(d, r, c, k)];
-    return acc;
-}
-
-// ---- BiCGSTAB (right Jacobi preconditioning), FP32 vectors, FP64 sums ----
-struct BiArgs {
-    HsDom d;
-    long long n;
-    const float* coef;
-    const float* dinv;
-    const float* b;
-    float *x, *r, *rhat, *p, *phat, *v, *s, *shat, *t;
-    double* scal;
-    double* partials;
-    unsigned* ticket;
-    int* flags;  // [0] done, [1] iterations, [2] breakdown
-    double eta2; // (relative tolerance)^2
-    int max_it;
-};
-
-__global__ void k_bi_init(BiArgs a)
-{
-    double acc[2] = {0.0, 0.0};
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
-         f += (long long)gridDim.x * blockDim.x) {
-        const float bv = a.b[f];
-        a.r[f] = bv;
-        a.rhat[f] = bv;
-        a.p[f] = 0.0f;
-        a.v[f] = 0.0f;
-        a.x[f] = 0.0f;
-        acc[0] += static_cast<double>(bv) * bv;
-    }
-    acc[1] = acc[0];
-    const bool mx[2] = {false, false};
-    double out[2];
-    if (grid_reduce<2>(acc, mx, a.partials, a.ticket, out)) {
-        a.scal[S_RHO] = out[0];
-        a.scal[S_BNORM2] = out[1];
-        a.scal[S_RHO_OLD] = 1.0;
-        a.scal[S_ALPHA] = 1.0;
-        a.scal[S_OMEGA] = 1.0;
-        a.flags[0] = (out[1] == 0.0) ? 1 : 0;
-        a.flags[1] = 0;
-        a.flags[2] = 0;
-    }
-}
-
-__global__ void k_bi_p(BiArgs a)
-{
-    if (a.flags[0]) return;
-    const double beta = (a.scal[S_RHO] / a.scal[S_RHO_OLD]) * (a.scal[S_ALPHA] / a.scal[S_OMEGA]);
-    const float bt = static_cast<float>(beta), om = static_cast<float>(a.scal[S_OMEGA]);
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
-         f += (long long)gridDim.x * blockDim.x) {
-        const float pv = a.r[f] + bt * (a.p[f] - om * a.v[f]);
-        a.p[f] = pv;
-        a.phat[f] = a.dinv[f] * pv;
-    }
-}
-
-__global__ void k_bi_v(BiArgs a)
-{
-    if (a.flags[0]) return;
-    double acc[1] = {0.0};
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
-         f += (long long)gridDim.x * blockDim.x) {
-        const int r = static_cast<int>(f / a.d.nc), c = static_cast<int>(f - (long long)r * a.d.nc);
-        const float vv = hs_matvec_row(a.d, a.coef, a.n, a.phat, static_cast<int>(f), r, c);
-        a.v[f] = vv;
-        acc[0] += static_cast<double>(a.rhat[f]) * vv;
-    }
-    const bool mx[1] = {false};
-    double out[1];
-    if (grid_reduce<1>(acc, mx, a.partials, a.ticket, out)) {
-        if (out[0] == 0.0) {
-            a.flags[0] = 1;
-            a.flags[2] = 1;
-        } else {
-            a.scal[S_ALPHA] = a.scal[S_RHO] / out[0];
-        }
-    }
-}
-
-__global__ void k_bi_s(BiArgs a)
-{
-    if (a.flags[0]) return;
-    const float al = static_cast<float>(a.scal[S_ALPHA]);
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
-         f += (long long)gridDim.x * blockDim.x) {
-        const float sv = a.r[f] - al * a.v[f];
-        a.s[f] = sv;
-        a.shat[f] = a.dinv[f] * sv;
-    }
-}
-
-__global__ void k_bi_t(BiArgs a)
-{
-    if (a.flags[0]) return;
-    double acc[2] = {0.0, 0.0};
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
-         f += (long long)gridDim.x * blockDim.x) {
-        const int r = static_cast<int>(f / a.d.nc), c = static_cast<int>(f - (long long)r * a.d.nc);
-        const float tv = hs_matvec_row(a.d, a.coef, a.n, a.shat, static_cast<int>(f), r, c);
-        a.t[f] = tv;
-        acc[0] += static_cast<double>(tv) * a.s[f];
-        acc[1] += static_cast<double>(tv) * tv;
-    }
-    const bool mx[2] = {false, false};
-    double out[2];
-    if (grid_reduce<2>(acc, mx, a.partials, a.ticket, out)) a.scal[S_OMEGA] = out[1] > 0.0 ? out[0] / out[1] : 0.0;
-}
-
-__global__ void k_bi_x(BiArgs a)
-{
-    if (a.flags[0]) return;
-    const float al = static_cast<float>(a.scal[S_ALPHA]), om = static_cast<float>(a.scal[S_OMEGA]);
-    double acc[2] = {0.0, 0.0};
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
-         f += (long long)gridDim.x * blockDim.x) {
-        a.x[f] += al * a.phat[f] + om * a.shat[f];
-        const float rv = a.s[f] - om * a.t[f];
-        a.r[f] = rv;
-        acc[0] += static_cast<double>(a.rhat[f]) * rv;
-        acc[1] += static_cast<double>(rv) * rv;
-    }
-    const bool mx[2] = {false, false};
-    double out[2];
-    if (grid_reduce<2>(acc, mx, a.partials, a.ticket, out)) {
-        a.scal[S_RHO_OLD] = a.scal[S_RHO];
-        a.scal[S_RHO] = out[0];
-        a.scal[S_RNORM2] = out[1];
-        const int it = a.flags[1] + 1;
-        a.flags[1] = it;
-        const bool conv = out[1] <= a.eta2 * a.scal[S_BNORM2];
-        const bool stall = (a.scal[S_OMEGA] == 0.0) || (out[0] == 0.0);
-        if (conv || it >= a.max_it || stall) a.flags[0] = 1;
-        if (stall && !conv) a.flags[2] = 1;
-    }
-}
-
-// line-search trial: max |F| and max centred-gradient ratio of z + lam*dz
-// over the DOFs (lam = 0: the current iterate)
-__global__ void k_hs_trial(HsDom d, const double* __restrict__ z, const float* __restrict__ dz, double lam,
-                           long long n, double* partials, unsigned* ticket, double* out_scal)
-{
-    const ZView Z{z, lam != 0.0 ? dz : nullptr, lam};
-    double acc[2] = {0.0, 0.0};
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
-         f += (long long)gridDim.x * blockDim.x) {
-        if (!d.dof[f]) continue;
-        const int r = static_cast<int>(f / d.nc), c = static_cast<int>(f - (long long)r * d.nc);
-        acc[0] = fmax(acc[0], fabs(hs_residual(d, Z, r, c)));
-        if (d.nonlinear) acc[1] = fmax(acc[1], hs_ratio(d, Z, r, c));
-    }
-    const bool mx[2] = {true, true};
-    double out[2];
-    if (grid_reduce<2>(acc, mx, partials, ticket, out)) {
-        out_scal[0] = out[0];
-        out_scal[1] = out[1];
-    }
-}
-
-__global__ void k_hs_apply(const uint8_t* __restrict__ dof, double* __restrict__ z, const float* __restrict__ dz,
-                           double lam, long long n)
-{
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
-         f += (long long)gridDim.x * blockDim.x)
-        if (dof[f]) z[f] = z[f] + lam * static_cast<double>(dz[f]);
 }
 
 // ---------------------------------------------------------------------------
@@ -629,13 +272,31 @@ LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, con
             d[static_cast<size_t>(f)] = (mask_host[f] && !edge) ? 1 : 0;
         }
     LSDTT_HIP(hipMemcpy(dof.p, d.data(), n, hipMemcpyHostToDevice));
+    // fast DOFs: interior, whole 3x3 block valid; the rest go on the
+    // general list handled by the companion kernels
+    std::vector<uint8_t> fh(static_cast<size_t>(n), 0);
+    std::vector<int> gh;
+    for (int r = 0; r < nr; ++r)
+        for (int c = 0; c < nc; ++c) {
+            const long long f = (long long)r * nc + c;
+            if (!d[static_cast<size_t>(f)]) continue;
+            bool ok = r > 0 && r < nr - 1 && c > 0 && c < nc - 1;
+            for (int k = 0; ok && k < 9; ++k) ok = mask_host[f + (k / 3 - 1) * nc + (k % 3 - 1)] != 0;
+            if (ok) fh[static_cast<size_t>(f)] = 1;
+            else gh.push_back(static_cast<int>(f));
+        }
+    fast.reserve(n);
+    LSDTT_HIP(hipMemcpy(fast.p, fh.data(), n, hipMemcpyHostToDevice));
+    n_gen = static_cast<int>(gh.size());
+    gen.reserve(gh.size() ? gh.size() : 1);
+    if (n_gen) LSDTT_HIP(hipMemcpy(gen.p, gh.data(), sizeof(int) * gh.size(), hipMemcpyHostToDevice));
     hs_grid = grid_for(n, 2048);
-    partials.reserve(static_cast<size_t>(hs_grid) * 8 + 8);
-    scal.reserve(S_COUNT + 8);
+    partials.reserve(static_cast<size_t>(hs_grid + hs_general_blocks(n_gen) + 1) * 8);
+    scal.reserve(kScalCount + 8);
     tickets.reserve(4);
     flags.reserve(8);
     LSDTT_HIP(hipMemset(tickets.p, 0, sizeof(unsigned) * 4));
-    LSDTT_HIP(hipMemset(scal.p, 0, sizeof(double) * (S_COUNT + 8)));
+    LSDTT_HIP(hipMemset(scal.p, 0, sizeof(double) * (kScalCount + 8)));
 }
 
 void LemEngine::set_elevation(const double* zh)
@@ -715,12 +376,18 @@ void LemEngine::stream_power(hipStream_t s)
     }
 }
 
+HsDom LemEngine::hs_domain(double inv_dt) const
+{
+    return HsDom{nr, nc, p.bc, mask.p, dof.p, fast.p, gen.p, n_gen, dx, p.D, 0.0, p.Sc, p.slope_ratio_cap, inv_dt,
+                 p.nonlinear, zn_hs.p};
+}
+
 int LemEngine::bicgstab(hipStream_t s)
 {
     BiArgs a;
-    a.d = HsDom{nr, nc, p.bc, mask.p, dof.p, dx, p.D, 0.0, p.Sc, p.slope_ratio_cap, 0.0, p.nonlinear, zn_hs.p};
+    a.d = hs_domain(cur_inv_dt);
     a.n = n;
-    a.coef = coef.p;
+    a.z = z.p;
     a.dinv = dinv.p;
     a.b = b.p;
     a.x = x.p;
@@ -738,46 +405,29 @@ int LemEngine::bicgstab(hipStream_t s)
     a.flags = flags.p;
     a.eta2 = p.krylov_rel_tol * p.krylov_rel_tol;
     a.max_it = p.krylov_max_iterations;
-    const int g = hs_grid;
-    k_bi_init<<<g, kB, 0, s>>>(a);
-    LSDTT_LAUNCH_CHECK();
-    constexpr int kChunk = 4;
+    hs_bicgstab_init(a, hs_grid, s);
+    // iterations are enqueued in chunks between flag read-backs; an
+    // iteration after convergence returns at once.  The chunk starts at the
+    // usual iteration count of these well-conditioned systems and grows.
+    int chunk = std::max(1, last_krylov_iterations);
     int hf[3] = {0, 0, 0};
-    for (int done = 0; !done;) {
-        for (int k = 0; k < kChunk; ++k) {
-            timer.begin("bicgstab_p", s);
-            k_bi_p<<<g, kB, 0, s>>>(a);
-            timer.end(s, 24.0 * n);
-            timer.begin("bicgstab_matvec", s);
-            k_bi_v<<<g, kB, 0, s>>>(a);
-            timer.end(s, 48.0 * n);
-            timer.begin("bicgstab_s", s);
-            k_bi_s<<<g, kB, 0, s>>>(a);
-            timer.end(s, 20.0 * n);
-            timer.begin("bicgstab_matvec", s);
-            k_bi_t<<<g, kB, 0, s>>>(a);
-            timer.end(s, 48.0 * n);
-            timer.begin("bicgstab_x", s);
-            k_bi_x<<<g, kB, 0, s>>>(a);
-            timer.end(s, 32.0 * n);
-            LSDTT_LAUNCH_CHECK();
-        }
+    for (int done = 0; !done; chunk = std::min(2 * chunk, 16)) {
+        for (int k = 0; k < chunk; ++k) hs_bicgstab_iteration(a, hs_grid, s, timer.enabled ? &timer : nullptr);
         LSDTT_HIP(hipMemcpyAsync(hf, flags.p, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
         LSDTT_HIP(hipStreamSynchronize(s));
         done = hf[0];
     }
+    last_krylov_iterations = hf[1];
     return hf[1];
 }
 
 void LemEngine::trial(double lambda, double* fnorm, double* ratio, hipStream_t s)
 {
-    HsDom d{nr, nc, p.bc, mask.p, dof.p, dx, p.D, 0.0, p.Sc, p.slope_ratio_cap, cur_inv_dt, p.nonlinear, zn_hs.p};
     timer.begin("hillslope_residual", s);
-    k_hs_trial<<<hs_grid, kB, 0, s>>>(d, z.p, x.p, lambda, n, partials.p, tickets.p, scal.p + S_TMP0);
-    timer.end(s, 22.0 * n);
-    LSDTT_LAUNCH_CHECK();
+    hs_trial(hs_domain(cur_inv_dt), z.p, x.p, lambda, n, partials.p, tickets.p, scal.p + HS_TRIAL_F, hs_grid, s);
+    timer.end(s, (lambda != 0.0 ? 22.0 : 18.0) * n);
     double h[2];
-    LSDTT_HIP(hipMemcpyAsync(h, scal.p + S_TMP0, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    LSDTT_HIP(hipMemcpyAsync(h, scal.p + HS_TRIAL_F, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipStreamSynchronize(s));
     *fnorm = h[0];
     *ratio = h[1];
@@ -796,7 +446,7 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
     if (st->substeps == 0 && st->rejected == 0) st->hillslope_initial_residual = fnorm;
     st->hillslope_tolerance = tol;
     const double feasible = std::sqrt(p.slope_ratio_cap);
-    HsDom d{nr, nc, p.bc, mask.p, dof.p, dx, p.D, 0.0, p.Sc, p.slope_ratio_cap, inv_dt, p.nonlinear, zn_hs.p};
+    const HsDom d = hs_domain(inv_dt);
     int it = 0;
     bool converged = false;
     for (; it < p.newton_max_iterations; ++it) {
@@ -804,9 +454,9 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
             converged = true;
             break;
         }
-        timer.begin("hillslope_assemble", s);
-        k_hs_assemble<<<hs_grid, kB, 0, s>>>(d, z.p, n, coef.p, dinv.p, b.p);
-        timer.end(s, 62.0 * n);
+        timer.begin("hillslope_prepare", s);
+        hs_prepare(d, z.p, n, dinv.p, b.p, hs_grid, s);
+        timer.end(s, 26.0 * n);
         LSDTT_LAUNCH_CHECK();
         st->krylov_iterations += bicgstab(s);
         // line search (newton .cpp:553-633)
@@ -837,7 +487,7 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
         }
         if (!accepted) break;
         timer.begin("hillslope_update", s);
-        k_hs_apply<<<hs_grid, kB, 0, s>>>(dof.p, z.p, x.p, lambda, n);
+        hs_apply(dof.p, z.p, x.p, lambda, n, hs_grid, s);
         timer.end(s, 21.0 * n);
         LSDTT_LAUNCH_CHECK();
         fnorm = fnorm_try;
@@ -858,7 +508,6 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
 int LemEngine::hillslope(LemStats* st, hipStream_t s)
 {
     const size_t nn = static_cast<size_t>(n);
-    coef.reserve(9 * nn);
     for (auto* bf : {&dinv, &b, &x, &r, &rhat, &pv, &phat, &v, &sv, &shat, &t}) bf->reserve(nn);
     zn_hs.reserve(nn);
     z_save.reserve(nn);
@@ -901,11 +550,11 @@ void LemEngine::stats(LemStats* st, hipStream_t s)
     hist.reserve(kRadixBins + 8);
     const double udz = p.enable_uplift ? p.U * p.dt : 0.0;
     timer.begin("step_stats", s);
-    k_stats<<<hs_grid, kB, 0, s>>>(mask.p, z_before.p, z.p, n, udz, absdz.p, partials.p, tickets.p, scal.p + S_SUM_B);
+    k_stats<<<hs_grid, kB, 0, s>>>(mask.p, z_before.p, z.p, n, udz, absdz.p, partials.p, tickets.p, scal.p + kStatSlot);
     timer.end(s, 21.0 * n);
     LSDTT_LAUNCH_CHECK();
     double h[8];
-    LSDTT_HIP(hipMemcpyAsync(h, scal.p + S_SUM_B, sizeof(h), hipMemcpyDeviceToHost, s));
+    LSDTT_HIP(hipMemcpyAsync(h, scal.p + kStatSlot, sizeof(h), hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipStreamSynchronize(s));
     const long long nv = static_cast<long long>(h[7]);
     const double inv_dt = 1.0 / p.dt, inv_n = nv > 0 ? 1.0 / static_cast<double>(nv) : 0.0;
@@ -947,7 +596,12 @@ int LemEngine::step(LemStats* st, hipStream_t s)
         LSDTT_LAUNCH_CHECK();
     }
     LSDTT_HIP(hipMemsetAsync(flags.p + 4, 0, sizeof(unsigned long long), s));
-    if (p.enable_fluvial) stream_power(s);
+    if (p.enable_fluvial) {
+        stream_power(s);
+        st->flow_levels = flow.max_depth + 1;
+        st->flow_tail_from = flow.tail_from;
+        st->n_outlets = flow.n_outlets;
+    }
     int rc = 0;
     if (p.enable_hillslope) rc = hillslope(st, s);
     if (p.enable_uplift) {

@@ -21,6 +21,7 @@
 #include "common.hpp"
 #include "dbuf.hpp"
 #include "flow.hpp"
+#include "hillslope.hpp"
 #include "ktimer.hpp"
 
 namespace lsdtt_gpu {
@@ -48,6 +49,8 @@ struct LemStats {
     int hillslope_converged = 1;
     double hillslope_initial_residual = 0, hillslope_residual = 0, hillslope_tolerance = 0;
     double max_slope_ratio = 0;
+    int flow_levels = 0, flow_tail_from = 0;
+    long long n_outlets = 0;
 };
 
 class LemEngine {
@@ -72,15 +75,15 @@ public:
     LemParams p;
     FlowEngine flow;
 
-    dbuf<uint8_t> mask, dof;
+    dbuf<uint8_t> mask, dof, fast;
     dbuf<double> z, z_before, K_field, bel;
-    dbuf<int> bflats;
+    dbuf<int> bflats, gen;
+    int n_gen = 0;
     long long n_bflats = 0;
     bool has_K = false;
 
     // hillslope work
     dbuf<double> zn_hs, z_save;
-    dbuf<float> coef;            // 9 planes of n
     dbuf<float> dinv, b, x, r, rhat, pv, phat, v, sv, shat, t;
     dbuf<double> partials, scal;  // reduction partials; scalars
     dbuf<unsigned> tickets;
@@ -91,6 +94,7 @@ public:
     int hs_grid = 1024;
     KernelTimer timer;
     double cur_inv_dt = 0.0;
+    int last_krylov_iterations = 2;
 
 private:
     void stream_power(hipStream_t s);
@@ -98,6 +102,7 @@ private:
     int newton_solve(double inv_dt, LemStats* st, hipStream_t s);
     int bicgstab(hipStream_t s);
     void trial(double lambda, double* fnorm, double* ratio, hipStream_t s);
+    HsDom hs_domain(double inv_dt) const;
     void stats(LemStats* st, hipStream_t s);
 };
 

@@ -49,12 +49,23 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], const bool (&is_ma
     __syncthreads();
 }
 
+// Block reduction whose result is stored as partial slot `slot` (for a
+// companion kernel whose partials a later grid_reduce folds in).
+template <int NV>
+__device__ __forceinline__ void block_partial(double (&v)[NV], const bool (&is_max)[NV], double* partials, int slot)
+{
+    block_reduce<NV>(v, is_max);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NV; ++i) partials[(size_t)slot * NV + i] = v[i];
+}
+
 // Writes the block's partials, and in the last block combines all partials
-// (in block order) into out[0..NV).  `ticket` must be zero on entry and is
-// left zero.  Returns true in thread 0 of the last block.
+// (in block order, then `extra` companion slots stored after them) into
+// out[0..NV).  `ticket` must be zero on entry and is left zero.  Returns
+// true in thread 0 of the last block.
 template <int NV>
 __device__ __forceinline__ bool grid_reduce(double (&v)[NV], const bool (&is_max)[NV], double* partials,
-                                            unsigned* ticket, double* out)
+                                            unsigned* ticket, double* out, unsigned extra = 0)
 {
     block_reduce<NV>(v, is_max);
     __shared__ bool last;
@@ -66,17 +77,22 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[NV], const bool (&is_max
     }
     __syncthreads();
     if (!last) return false;
-    if (threadIdx.x == 0) {
-        __threadfence();
+    __threadfence();
+    // every thread folds a fixed stride of the partials, then the block
+    // reduces in its fixed order: deterministic and parallel
+    double w[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) w[i] = is_max[i] ? -1.7976931348623157e308 : 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x + extra; b += blockDim.x)
+#pragma unroll
         for (int i = 0; i < NV; ++i) {
-            double a = __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            for (unsigned b = 1; b < gridDim.x; ++b) {
-                const double x =
-                    __hip_atomic_load(&partials[(size_t)b * NV + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                a = is_max[i] ? (a > x ? a : x) : a + x;
-            }
-            out[i] = a;
+            const double x = __hip_atomic_load(&partials[(size_t)b * NV + i], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+            w[i] = is_max[i] ? (w[i] > x ? w[i] : x) : w[i] + x;
         }
+    block_reduce<NV>(w, is_max);
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < NV; ++i) out[i] = w[i];
         *ticket = 0u;
         return true;
     }
