@@ -137,9 +137,10 @@ def main():
                    "newton_iters_per_step": round(newton / args.steps, 2),
                    "krylov_iters_per_step": round(krylov / args.steps, 2),
                    "worst_relative_residual": worst_r,
-                   "flow_levels_last_step": int(st["flow_levels"]),
-                   "flow_tail_from_last_step": int(st["flow_tail_from"]),
-                   "outlets_last_step": int(st["n_outlets"])},
+                   "count_passes_last_step": int(st["count_passes"]),
+                   "stream_power_passes_last_step": int(st["stream_power_passes"]),
+                   "count_tile_runs_last_step": int(st["count_tile_runs"]),
+                   "stream_power_tile_runs_last_step": int(st["stream_power_tile_runs"])},
         "roofline": roofline,
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
     }

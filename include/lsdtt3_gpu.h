@@ -173,9 +173,10 @@ typedef struct {
     double hillslope_residual;         /* final ||F||inf, m/yr */
     double hillslope_tolerance;
     double max_slope_ratio;
-    int32_t flow_levels;              /* depth-from-outlet levels this step */
-    int32_t flow_tail_from;           /* first level run by one workgroup */
-    int64_t n_outlets;
+    int32_t count_passes;             /* tile passes of the upstream counts */
+    int32_t stream_power_passes;      /* tile passes of the stream power */
+    int64_t count_tile_runs;          /* 32 x 32 tiles processed by those passes */
+    int64_t stream_power_tile_runs;
 } lsdtt_gpu_lem_stats;
 
 /* Defaults of lsdtt_lem_options, plus the Newton defaults above. */

@@ -31,7 +31,8 @@ class LemStats(C.Structure):
         ("hillslope_converged", C.c_int32),
         ("hillslope_initial_residual", C.c_double), ("hillslope_residual", C.c_double),
         ("hillslope_tolerance", C.c_double), ("max_slope_ratio", C.c_double),
-        ("flow_levels", C.c_int32), ("flow_tail_from", C.c_int32), ("n_outlets", C.c_int64),
+        ("count_passes", C.c_int32), ("stream_power_passes", C.c_int32),
+        ("count_tile_runs", C.c_int64), ("stream_power_tile_runs", C.c_int64),
     ]
 
     def as_dict(self):

@@ -186,9 +186,10 @@ int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_
         out->hillslope_residual = st.hillslope_residual;
         out->hillslope_tolerance = st.hillslope_tolerance;
         out->max_slope_ratio = st.max_slope_ratio;
-        out->flow_levels = st.flow_levels;
-        out->flow_tail_from = st.flow_tail_from;
-        out->n_outlets = st.n_outlets;
+        out->count_passes = st.count_passes;
+        out->count_tile_runs = st.count_tile_runs;
+        out->stream_power_tile_runs = st.stream_power_tile_runs;
+        out->stream_power_passes = st.stream_power_passes;
         if (rc != 0) {
             set_error("hillslope Newton did not converge at the minimum sub-step");
             return LSDTT_GPU_ERR_NOT_CONVERGED;

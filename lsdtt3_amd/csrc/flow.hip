@@ -592,6 +592,20 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
     return 0;
 }
 
+template <class T>
+void FlowEngine::receivers_only(const T* z, const uint8_t* msk, hipStream_t s)
+{
+    mask = msk;
+    dim3 b2(64, 4), g2(div_up(opt.nc, 64), div_up(opt.nr, 4));
+    TScope ts_(timer, "receivers", s, (sizeof(T) + 6.0) * n);
+    k_receivers<T><<<g2, b2, 0, s>>>(z, msk, nullptr, opt.nr, opt.nc, opt.dx, opt.dx * kSqrt2, opt.bc,
+                                      opt.interior_nodata_is_outlet, opt.allow_pits_as_outlets, rcv, dir, misc.p);
+    LSDTT_LAUNCH_CHECK();
+}
+
+template void FlowEngine::receivers_only<double>(const double*, const uint8_t*, hipStream_t);
+template void FlowEngine::receivers_only<float>(const float*, const uint8_t*, hipStream_t);
+
 template long long FlowEngine::route<float>(const float*, const uint8_t*, const uint8_t*, hipStream_t);
 template long long FlowEngine::route<double>(const double*, const uint8_t*, const uint8_t*, hipStream_t);
 

@@ -40,6 +40,11 @@ public:
     template <class T>
     long long route(const T* z, const uint8_t* mask, const uint8_t* forced, hipStream_t s);
 
+    // receivers and directions only (no failure check: for
+    // allow_pits_as_outlets routing, where every cell resolves)
+    template <class T>
+    void receivers_only(const T* z, const uint8_t* mask, hipStream_t s);
+
     // Upstream sweep (requires route()).  keys: Kahn keys too; w/acc: optional
     // weighted double accumulation (w null -> no accumulation).
     void upstream(bool keys, const double* w, double* acc, hipStream_t s);

@@ -1,0 +1,655 @@
+// fluvial.hip -- tile-local upstream counts and implicit stream power (see
+// fluvial.hpp).  Reference: src/cpu/lsdtt_flowinfo_cpu.cpp:1979-2046 (counts
+// = accumulate with unit weights), src/cpu/lsdtt_stream_power_cpu.cpp:
+// 159-326 (stream power), applied in an order that respects every
+// receiver-before-donor (stream power) or donor-before-receiver (counts)
+// dependency, which is all the serial reference order guarantees.
+//
+// One wavefront owns one 32 x 32 tile.  Loads and stores touch cells
+// i = lane + 64 k (k = 0..15): lane l covers column l % 32 of tile rows
+// 2k + l / 32, so every HBM access is a coalesced 32-cell row segment, and
+// each lane issues all of its loads before it uses any of them.  Inside the
+// tile the inputs are staged in LDS and each level's cells are spread over
+// the 64 lanes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <utility>
+
+#include "fluvial.hpp"
+
+namespace lsdtt_gpu {
+
+namespace {
+
+constexpr int TW = FluvialTiles::TW, TH = FluvialTiles::TH, TN = FluvialTiles::TN;
+constexpr int WL = 64;            // lanes: one wavefront per tile
+constexpr int PER = TN / WL;      // cells per lane
+constexpr int kJumpRounds = 10;   // 2^10 >= TN
+
+static_assert(TW == 32 && TN % WL == 0, "lane layout assumes 32-wide tiles");
+static_assert((1 << kJumpRounds) >= TN, "pointer jumping rounds too few");
+
+// D8 offsets of common.hpp's direction order (N, NE, E, SE, S, SW, W, NW),
+// two bits per direction, biased by one
+constexpr unsigned kRowBits = 0x1A90u, kColBits = 0x01A9u;
+
+__device__ __forceinline__ int local_index(int g, int nc, int r0, int c0)
+{
+    const int gr = g / nc, gc = g - gr * nc;
+    return (gr >= r0 && gr < r0 + TH && gc >= c0 && gc < c0 + TW) ? (gr - r0) * TW + (gc - c0) : -1;
+}
+
+__device__ __forceinline__ int tile_of(int g, int nc, int ntx)
+{
+    const int gr = g / nc;
+    return (gr / TH) * ntx + (g - gr * nc) / TW;
+}
+
+// neighbour of cell (r, c) in D8 direction d under the boundary
+// conditions, -1 if there is none
+__device__ __forceinline__ int neighbour(int r, int c, int d, int nr, int nc, BC bc)
+{
+    const int rr = r + static_cast<int>((kRowBits >> (2 * d)) & 3u) - 1;
+    const int cc = c + static_cast<int>((kColBits >> (2 * d)) & 3u) - 1;
+    if (rr >= 0 && rr < nr && cc >= 0 && cc < nc) return rr * nc + cc;
+    const Step st = resolve_step(r, c, d, nr, nc, bc);
+    return st.kind == 1 ? st.r * nc + st.c : -1;
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ int wave_exclusive_scan(int v, int* total)
+{
+    const int lane = threadIdx.x & 63;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    *total = __shfl(x, 63);
+    return x - v;
+}
+
+// this lane's cell k of the tile at (r0, c0): flat index (0 outside the
+// grid, so that loads stay in bounds) and whether it is in the grid
+struct LaneCell {
+    int f, r, c;
+    bool in;
+};
+
+__device__ __forceinline__ LaneCell lane_cell(int k, int r0, int c0, int nr, int nc)
+{
+    const int lane = threadIdx.x;
+    LaneCell q;
+    q.r = r0 + 2 * k + (lane >> 5);
+    q.c = c0 + (lane & 31);
+    q.in = q.r < nr && q.c < nc;
+    q.f = q.in ? q.r * nc + q.c : 0;
+    return q;
+}
+
+// ---------------------------------------------------------------------------
+// once per set of receivers: each tile's cells sorted by in-tile hop count
+// to the tile's exit (pointer jumping in LDS, then a counting sort), stored
+// tile-major: ord[t*TN + p] = local cell, lstart[t*(TN+1) + L] = first p of
+// level L.  Also, per cell, the directions of its valid donors that lie in
+// another tile (only cells on the ring of the tile's part of the grid can
+// have them; a periodic wrap on a narrow grid can reach one cell by two
+// directions, which then counts once).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(WL) k_tile_order(const int* __restrict__ rcv, const uint8_t* __restrict__ mask,
+                                                   int nr, int nc, BC bc, short* __restrict__ ordg,
+                                                   short* __restrict__ lstartg, int* __restrict__ tile_maxdep,
+                                                   uint8_t* __restrict__ xdon)
+{
+    __shared__ int jmp[TN];
+    __shared__ int dep[TN];
+    __shared__ int hist[TN + 1];
+    __shared__ uint8_t xd[TN];
+    const int tx = blockIdx.x, ty = blockIdx.y, lane = threadIdx.x;
+    const int tile = ty * gridDim.x + tx;
+    const int r0 = ty * TH, c0 = tx * TW;
+    const int h = min(TH, nr - r0), w = min(TW, nc - c0);
+    bool valid[PER];
+    {
+        uint8_t m[PER];
+        int g[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+            m[k] = mask[q.f];
+            g[k] = rcv[q.f];
+            valid[k] = q.in && m[k];
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = lane + WL * k;
+            const int l = (valid[k] && g[k] >= 0) ? local_index(g[k], nc, r0, c0) : -1;
+            jmp[i] = l;
+            dep[i] = l >= 0 ? 1 : 0;
+            hist[i] = 0;
+            xd[i] = 0;
+        }
+    }
+    if (lane == 0) hist[TN] = 0;
+    __syncthreads();
+    // ring slots: 0..31 top row, 32..63 bottom row, 64..95 left column,
+    // 96..127 right column (corners belong to the rows)
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        const int s = lane + WL * half;
+        int lr, lc;
+        bool ok;
+        if (s < 32) {
+            lr = 0, lc = s, ok = lc < w;
+        } else if (s < 64) {
+            lr = h - 1, lc = s - 32, ok = lc < w && h > 1;
+        } else if (s < 96) {
+            lr = s - 64, lc = 0, ok = lr >= 1 && lr <= h - 2;
+        } else {
+            lr = s - 96, lc = w - 1, ok = lr >= 1 && lr <= h - 2 && w > 1;
+        }
+        const int r = r0 + lr, c = c0 + lc;
+        const int f = ok ? r * nc + c : 0;
+        int gs[8], rv[8];
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            int g = ok ? neighbour(r, c, d, nr, nc, bc) : -1;
+            if (g == f || (g >= 0 && local_index(g, nc, r0, c0) >= 0)) g = -1;
+            gs[d] = g;
+        }
+#pragma unroll
+        for (int d = 0; d < 8; ++d) rv[d] = rcv[gs[d] >= 0 ? gs[d] : 0];
+        unsigned bits = 0;
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            bool dup = false;
+#pragma unroll
+            for (int e = 0; e < d; ++e) dup |= gs[e] == gs[d];
+            // masked cells hold a negative receiver, never == f
+            if (gs[d] >= 0 && !dup && rv[d] == f) bits |= 1u << d;
+        }
+        if (ok && mask[f]) xd[lr * TW + lc] = static_cast<uint8_t>(bits);
+    }
+    for (int round = 0; round < kJumpRounds; ++round) {
+        int nj[PER], nd[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = lane + WL * k;
+            const int j = jmp[i];
+            nd[k] = dep[i];
+            nj[k] = j;
+            if (j >= 0) {
+                nd[k] += dep[j];
+                nj[k] = jmp[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = lane + WL * k;
+            jmp[i] = nj[k];
+            dep[i] = nd[k];
+        }
+        __syncthreads();
+    }
+    int mx = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = lane + WL * k;
+        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+        if (q.in) xdon[q.f] = xd[i];
+        if (valid[k]) {
+            const int d = dep[i];
+            atomicAdd(&hist[d], 1);
+            mx = max(mx, d);
+        }
+    }
+    mx = wave_max(mx);
+    __syncthreads();
+    // exclusive scan of hist[0..TN), PER consecutive bins per lane
+    int loc[PER], sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        loc[k] = hist[lane * PER + k];
+        sum += loc[k];
+    }
+    int total;
+    int base = wave_exclusive_scan(sum, &total);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        hist[lane * PER + k] = base;
+        lstartg[(size_t)tile * (TN + 1) + lane * PER + k] = static_cast<short>(base);
+        base += loc[k];
+        jmp[lane * PER + k] = 0;  // per-level cursors
+    }
+    if (lane == 0) {
+        hist[TN] = total;
+        lstartg[(size_t)tile * (TN + 1) + TN] = static_cast<short>(total);
+        tile_maxdep[tile] = mx;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+        if (valid[k]) {
+            const int i = lane + WL * k;
+            const int d = dep[i];
+            ordg[(size_t)tile * TN + hist[d] + atomicAdd(&jmp[d], 1)] = static_cast<short>(i);
+        }
+}
+
+// Pass bookkeeping shared by both sweeps.  Pass p reads act[p % 3], marks
+// act[(p + 1) % 3] and clears act[(p + 2) % 3] for pass p + 1; it does
+// nothing when pass p - 1 marked no tile (the sweep has settled).
+struct PassArgs {
+    const int* act;
+    int* act_next;
+    int* act_clear;
+    const int* prev_marked;  // null for the first pass
+    int* marked;
+    int* runs;
+};
+
+// level starts of levels 0..maxdep + 1 into LDS
+__device__ __forceinline__ int load_levels(const short* __restrict__ lstartg, const int* __restrict__ tile_maxdep,
+                                           int tile, short* ls)
+{
+    const int maxdep = tile_maxdep[tile];
+    for (int L = threadIdx.x; L <= maxdep + 1; L += WL) ls[L] = lstartg[(size_t)tile * (TN + 1) + L];
+    return maxdep;
+}
+
+// ---------------------------------------------------------------------------
+// counts: deep levels first; count = 1 + in-tile donors (LDS) + donors in
+// other tiles (HBM, their current values)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(WL) k_tile_counts(const int* __restrict__ rcv, const uint8_t* __restrict__ mask,
+                                                    const short* __restrict__ ordg, const short* __restrict__ lstartg,
+                                                    const int* __restrict__ tile_maxdep,
+                                                    const uint8_t* __restrict__ xdon, int nr, int nc, BC bc,
+                                                    int* __restrict__ cnt, PassArgs P)
+{
+    if (P.prev_marked && *P.prev_marked == 0) return;
+    const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x;
+    const int tile = ty * ntx + tx;
+    const int lane = threadIdx.x;
+    if (lane == 0) P.act_clear[tile] = 0;
+    if (!P.act[tile]) return;
+    __shared__ int acc[TN];
+    __shared__ short lrcv[TN];
+    __shared__ short ord[TN];
+    __shared__ short ls[TN + 1];
+    const int r0 = ty * TH, c0 = tx * TW;
+    int old[PER], g[PER];
+    uint8_t x[PER];
+    bool valid[PER];
+    {
+        uint8_t m[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+            m[k] = mask[q.f];
+            g[k] = rcv[q.f];
+            x[k] = xdon[q.f];
+            old[k] = cnt[q.f];  // only this tile writes its cells
+            valid[k] = q.in && m[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = lane + WL * k;
+        acc[i] = valid[k] ? 1 : 0;
+        lrcv[i] = static_cast<short>((valid[k] && g[k] >= 0) ? local_index(g[k], nc, r0, c0) : -1);
+        ord[i] = ordg[(size_t)tile * TN + i];
+    }
+    const int maxdep = load_levels(lstartg, tile_maxdep, tile, ls);
+    // inflow from other tiles (ring cells only)
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (!valid[k] || !x[k]) continue;
+        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+        int a = 0;
+        for (unsigned b = x[k]; b; b &= b - 1) {
+            const int g2 = neighbour(q.r, q.c, __builtin_ctz(b), nr, nc, bc);
+            a += __hip_atomic_load(&cnt[g2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        acc[lane + WL * k] += a;
+    }
+    __syncthreads();
+    for (int L = maxdep; L >= 0; --L) {
+        const int b = ls[L], e = ls[L + 1];
+        for (int p = b + lane; p < e; p += WL) {
+            const int i = ord[p];
+            const int l = lrcv[i];
+            if (l >= 0) atomicAdd(&acc[l], acc[i]);
+        }
+        __syncthreads();
+    }
+    // a changed count whose receiver lies in another tile re-runs that tile
+    bool marked = false;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (!valid[k]) continue;
+        const int i = lane + WL * k;
+        const int v = acc[i];
+        if (v != old[k]) {
+            const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+            __hip_atomic_store(&cnt[q.f], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lrcv[i] < 0 && g[k] >= 0) {
+                P.act_next[tile_of(g[k], nc, ntx)] = 1;
+                marked = true;
+            }
+        }
+    }
+    const bool wmarked = __ballot(marked) != 0;
+    if (lane == 0) {
+        if (wmarked) *P.marked = 1;
+        atomicAdd(P.runs, 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stream power: outlet side first; a cell's new elevation needs its
+// receiver's new elevation (LDS in the tile, HBM across the edge)
+// ---------------------------------------------------------------------------
+struct SpTileArgs {
+    const int* rcv;
+    const uint8_t* dir;
+    const uint8_t* mask;
+    const int* cnt;
+    const short* ord;
+    const short* lstart;
+    const int* tile_maxdep;
+    const uint8_t* xdon;
+    const double* z_old;
+    const double* K;
+    double* z;
+    int* tile_eroded;
+    SpParams p;
+};
+
+// Newton / closed form of lsdtt_stream_power_cpu.cpp:257-316 for a cell
+// that is not skipped
+__device__ __forceinline__ double sp_solve(const SpParams& a, double zo, double zr, double kam, double dx_eff)
+{
+    const double tol = a.slope_clamp * dx_eff;
+    double zn = zo;
+    if (a.n_is_one) {
+        const double spf = kam * (a.dt / dx_eff);
+        zn = (zo + zr * spf) / (1.0 + spf);
+        if ((zn - zo) > tol || isnan(zn)) zn = zo;
+    } else {
+        double eps = tol + 1.0;
+        int it = 0;
+        do {
+            const double slope = (zn - zr) / dx_eff;
+            if (slope <= 0.0) {
+                eps = 0.0;
+            } else {
+                // S^n = S^(n-1) * S: one pow per iteration; sqrt is the
+                // correctly rounded pow(S, 0.5)
+                const double pn1 = a.n1_half ? sqrt(slope) : pow(slope, a.nexp - 1.0);
+                const double fz = zn - zo + kam * a.dt * (pn1 * slope);
+                const double fd = 1.0 + kam * a.dt * (a.nexp / dx_eff) * pn1;
+                eps = fz / fd;
+            }
+            zn -= eps;
+            ++it;
+            if (it > 100) eps = tol / 2.0;
+        } while (fabs(eps) > tol);
+        if ((zn - zo) > tol || isnan(zn)) zn = zo;
+        if (zn < zr) zn = zr + a.slope_clamp * dx_eff;
+    }
+    return zn;
+}
+
+constexpr short kNoRcv = -1, kOutRcv = -2;  // lrcv codes besides a local cell
+
+__global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC bc, PassArgs P)
+{
+    if (P.prev_marked && *P.prev_marked == 0) return;
+    const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x;
+    const int tile = ty * ntx + tx;
+    const int lane = threadIdx.x;
+    if (lane == 0) P.act_clear[tile] = 0;
+    if (!P.act[tile]) return;
+    __shared__ double val[TN];   // z_old until the cell's level, then its new z
+    __shared__ double kam[TN];   // K A^m; < 0: the reference skips the cell
+    __shared__ short lrcv[TN];   // local receiver, kNoRcv, kOutRcv
+    __shared__ short ord[TN];
+    __shared__ short ls[TN + 1];
+    __shared__ uint8_t dcode[TN];
+    const int r0 = ty * TH, c0 = tx * TW;
+    const SpParams& p = A.p;
+    bool valid[PER];
+    uint8_t x[PER];
+#pragma unroll
+    for (int k0 = 0; k0 < PER; k0 += 8) {
+        uint8_t m[8], dr[8];
+        int g[8], ct[8];
+        double zo[8], Kf[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const LaneCell q = lane_cell(k0 + j, r0, c0, nr, nc);
+            m[j] = A.mask[q.f];
+            g[j] = A.rcv[q.f];
+            dr[j] = A.dir[q.f];
+            ct[j] = A.cnt[q.f];
+            zo[j] = A.z_old[q.f];
+            Kf[j] = A.K ? A.K[q.f] : p.K0;
+            x[k0 + j] = A.xdon[q.f];
+            valid[k0 + j] = q.in && m[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = k0 + j, i = lane + WL * k;
+            double km = -1.0;
+            short l = kNoRcv;
+            if (valid[k] && g[j] >= 0) {
+                double area = static_cast<double>(ct[j]) * p.pixel_area;
+                if (p.divide) area /= p.dx;
+                if (!(p.threshold > 0.0 && area < p.threshold)) {
+                    km = Kf[j] * (p.m_half ? sqrt(area) : pow(area, p.m));
+                    const int li = local_index(g[j], nc, r0, c0);
+                    l = li >= 0 ? static_cast<short>(li) : kOutRcv;
+                }
+            }
+            val[i] = zo[j];
+            kam[i] = km;
+            lrcv[i] = l;
+            dcode[i] = dr[j];
+            ord[i] = A.ord[(size_t)tile * TN + i];
+        }
+    }
+    const int maxdep = load_levels(A.lstart, A.tile_maxdep, tile, ls);
+    __syncthreads();
+    int eroded = 0;
+    for (int L = 0; L <= maxdep; ++L) {
+        const int b = ls[L], e = ls[L + 1];
+        for (int q = b + lane; q < e; q += WL) {
+            const int i = ord[q];
+            const short l = lrcv[i];
+            if (l == kNoRcv) continue;
+            const double z0 = val[i];
+            double zr;
+            if (l >= 0) {
+                zr = val[l];
+            } else {
+                const int g = neighbour(r0 + i / TW, c0 + i % TW, dcode[i], nr, nc, bc);
+                zr = __hip_atomic_load(&A.z[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (zr < z0) {
+                val[i] = sp_solve(p, z0, zr, kam[i], (dcode[i] & 1u) ? p.dx * kSqrt2 : p.dx);
+                ++eroded;
+            }
+        }
+        __syncthreads();
+    }
+    // a changed elevation with donors in other tiles re-runs those tiles
+    bool marked = false;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (!valid[k]) continue;
+        const int i = lane + WL * k;
+        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+        const double v = val[i];
+        const double o = A.z[q.f];  // only this tile writes its cells
+        if (__double_as_longlong(o) != __double_as_longlong(v)) {
+            __hip_atomic_store(&A.z[q.f], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (unsigned b = x[k]; b; b &= b - 1) {
+                P.act_next[tile_of(neighbour(q.r, q.c, __builtin_ctz(b), nr, nc, bc), nc, ntx)] = 1;
+                marked = true;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) eroded += __shfl_xor(eroded, o);
+    const bool wmarked = __ballot(marked) != 0;
+    if (lane == 0) {
+        A.tile_eroded[tile] = eroded;
+        if (wmarked) *P.marked = 1;
+        atomicAdd(P.runs, 1);
+    }
+}
+
+__global__ void k_fill_i32(int* __restrict__ a, long long n, int v)
+{
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void k_sum_i32(const int* __restrict__ a, int n, long long* __restrict__ out)
+{
+    __shared__ long long s[256];
+    long long acc = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += a[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+}  // namespace
+
+void FluvialTiles::resize(int nr_, int nc_, BC bc)
+{
+    bc_ = bc;
+    nr = nr_;
+    nc = nc_;
+    ntx = div_up(nc, TW);
+    nty = div_up(nr, TH);
+    ntiles = ntx * nty;
+    max_passes = ntiles + 2;
+    for (auto& a : act) a.reserve(ntiles);
+    tile_eroded.reserve(ntiles);
+    tile_maxdep.reserve(ntiles);
+    ord.reserve((size_t)ntiles * TN);
+    xdon.reserve((size_t)nr * nc);
+    lstart.reserve((size_t)ntiles * (TN + 1));
+    // per pass: marked flag, tiles run; then the eroded-cell sum (8-aligned)
+    flags.reserve(2 * (size_t)max_passes + 4);
+    flags_h.resize(2 * (size_t)max_passes + 4);
+}
+
+void FluvialTiles::depths(const int* rcv, const uint8_t* mask, hipStream_t s)
+{
+    k_tile_order<<<dim3(ntx, nty), WL, 0, s>>>(rcv, mask, nr, nc, bc_, ord.p, lstart.p, tile_maxdep.p, xdon.p);
+    LSDTT_LAUNCH_CHECK();
+}
+
+// Runs passes in batches with one host read-back per batch: a pass after
+// the settling one exits at once, so a batch that overshoots costs only
+// empty launches.  `after_batch` may enqueue work whose result is read back
+// with the flags (the eroded-cell sum).
+template <class Pass, class After>
+int FluvialTiles::settle(Pass&& pass, After&& after_batch, hipStream_t s)
+{
+    const size_t nflags = 2 * (size_t)max_passes + 4;
+    k_fill_i32<<<div_up(nflags, 256), 256, 0, s>>>(flags.p, nflags, 0);
+    LSDTT_LAUNCH_CHECK();
+    k_fill_i32<<<div_up(ntiles, 256), 256, 0, s>>>(act[0].p, ntiles, 1);
+    LSDTT_LAUNCH_CHECK();
+    k_fill_i32<<<div_up(ntiles, 256), 256, 0, s>>>(act[1].p, ntiles, 0);
+    LSDTT_LAUNCH_CHECK();
+    int launched = 0;
+    int batch = std::max(2, last_passes + 1);
+    for (;;) {
+        const int end = std::min(max_passes, launched + batch);
+        for (int q = launched; q < end; ++q) {
+            PassArgs P{act[q % 3].p, act[(q + 1) % 3].p, act[(q + 2) % 3].p, q > 0 ? flags.p + q - 1 : nullptr,
+                       flags.p + q, flags.p + max_passes + q};
+            pass(P);
+        }
+        launched = end;
+        after_batch();
+        LSDTT_HIP(hipMemcpyAsync(flags_h.data(), flags.p, nflags * sizeof(int), hipMemcpyDeviceToHost, s));
+        LSDTT_HIP(hipStreamSynchronize(s));
+        int settled = -1;
+        for (int q = 0; q < launched; ++q)
+            if (!flags_h[q]) {
+                settled = q;
+                break;
+            }
+        if (settled >= 0) {
+            tile_runs = 0;
+            for (int q = 0; q <= settled; ++q) tile_runs += flags_h[max_passes + q];
+            last_passes = settled + 1;
+            return last_passes;
+        }
+        if (launched >= max_passes) throw std::runtime_error("tile passes did not settle");
+        batch *= 2;
+    }
+}
+
+int FluvialTiles::counts(const int* rcv, const uint8_t* mask, int* cnt, hipStream_t s)
+{
+    const long long n = (long long)nr * nc;
+    k_fill_i32<<<div_up(n, 256), 256, 0, s>>>(cnt, n, 0);
+    LSDTT_LAUNCH_CHECK();
+    last_passes = last_count_passes;
+    const int passes = settle(
+        [&](const PassArgs& P) {
+            k_tile_counts<<<dim3(ntx, nty), WL, 0, s>>>(rcv, mask, ord.p, lstart.p, tile_maxdep.p, xdon.p, nr, nc,
+                                                        bc_, cnt, P);
+            LSDTT_LAUNCH_CHECK();
+        },
+        [] {}, s);
+    last_count_passes = passes;
+    return passes;
+}
+
+int FluvialTiles::stream_power(const int* rcv, const uint8_t* dir, const uint8_t* mask, const int* cnt,
+                               const double* z_old, const double* K, const SpParams& sp, double* z, long long* eroded,
+                               hipStream_t s)
+{
+    SpTileArgs A{rcv, dir, mask, cnt, ord.p, lstart.p, tile_maxdep.p, xdon.p, z_old, K, z, tile_eroded.p, sp};
+    long long* d_sum = reinterpret_cast<long long*>(flags.p + 2 * max_passes);
+    last_passes = last_sp_passes;
+    const int passes = settle(
+        [&](const PassArgs& P) {
+            k_tile_sp<<<dim3(ntx, nty), WL, 0, s>>>(A, nr, nc, bc_, P);
+            LSDTT_LAUNCH_CHECK();
+        },
+        [&] {
+            k_sum_i32<<<1, 256, 0, s>>>(tile_eroded.p, ntiles, d_sum);
+            LSDTT_LAUNCH_CHECK();
+        },
+        s);
+    last_sp_passes = passes;
+    std::memcpy(eroded, flags_h.data() + 2 * max_passes, sizeof(long long));
+    return passes;
+}
+
+}  // namespace lsdtt_gpu

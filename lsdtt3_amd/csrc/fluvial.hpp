@@ -1,0 +1,60 @@
+// fluvial.hpp -- upstream counts and implicit stream power by tiles.
+//
+// Both sweeps follow the D8 receiver graph: counts from the headwaters down
+// (count(v) = 1 + sum of its donors' counts), stream power from the outlets
+// up (z(v) is a function of its receiver's new elevation,
+// src/cpu/lsdtt_stream_power_cpu.cpp:218-319).  Instead of global depth
+// levels (random gathers), the grid is cut into 32 x 32 tiles held in LDS:
+// inside a tile the cells are ordered by their hop count to the tile's exit
+// (pointer jumping in LDS, once per step) and processed level by level by
+// the tile's one wavefront; values that cross a tile edge are read from HBM.
+// Passes repeat over the tiles that read a value another tile changed since
+// (the tile of a changed cell's receiver for counts, of its donors for
+// stream power) until a pass marks no tile.  Every cell's final value is the same function of
+// its receiver's (or donors') final values as in the serial reference order,
+// so the result does not depend on the pass schedule.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+#include "dbuf.hpp"
+
+namespace lsdtt_gpu {
+
+struct SpParams {
+    double K0, m, nexp, dt, dx, pixel_area, threshold, slope_clamp;
+    int divide, n_is_one;
+    int m_half, n1_half;  // m == 0.5, n - 1 == 0.5: sqrt in place of pow
+};
+
+class FluvialTiles {
+public:
+    static constexpr int TW = 32, TH = 32, TN = TW * TH;
+
+    void resize(int nr, int nc, BC bc);
+    // each tile's cells in level order, once per set of receivers
+    void depths(const int* rcv, const uint8_t* mask, hipStream_t s);
+    // counts (int32) of valid cells from receivers; returns passes used
+    int counts(const int* rcv, const uint8_t* mask, int* cnt, hipStream_t s);
+    // stream power in place on z (z == z_old on entry); returns passes used
+    int stream_power(const int* rcv, const uint8_t* dir, const uint8_t* mask, const int* cnt, const double* z_old,
+                     const double* K, const SpParams& sp, double* z, long long* eroded, hipStream_t s);
+
+    int nr = 0, nc = 0, ntx = 0, nty = 0, ntiles = 0, max_passes = 0;
+    long long tile_runs = 0;  // tiles processed by the last counts / stream_power
+    BC bc_{'o', 'o', 'o', 'o'};
+    dbuf<int> act[3], flags, tile_eroded, tile_maxdep;
+    std::vector<int> flags_h;
+    dbuf<short> ord, lstart;  // tile-major level order
+    dbuf<uint8_t> xdon;       // per cell: directions of donors in other tiles
+
+private:
+    // passes of the last call of each sweep: the next call's first batch
+    int last_passes = 0, last_count_passes = 0, last_sp_passes = 0;
+    // runs `pass(PassArgs)` until a pass marks no tile
+    template <class Pass, class After>
+    int settle(Pass&& pass, After&& after_batch, hipStream_t s);
+};
+
+}  // namespace lsdtt_gpu

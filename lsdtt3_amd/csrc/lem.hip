@@ -56,96 +56,6 @@ __global__ void k_gather(const double* __restrict__ z, const int* __restrict__ i
 }
 
 // ---------------------------------------------------------------------------
-// stream power
-// ---------------------------------------------------------------------------
-struct SpArgs {
-    const int* rcv;
-    const uint8_t* dir;
-    const int* cnt;
-    const double* z_old;
-    double* z;
-    const double* K;  // per-cell field or null
-    double K0, m, nexp, dt, dx, pixel_area, threshold, slope_clamp;
-    int divide, n_is_one;
-    unsigned long long* eroded;
-};
-
-__device__ __forceinline__ bool sp_node(const SpArgs& a, int v)
-{
-    const int rc = a.rcv[v];
-    const double dx_eff = (a.dir[v] & 1u) ? a.dx * kSqrt2 : a.dx;
-    double area = static_cast<double>(a.cnt[v]) * a.pixel_area;
-    if (a.divide) area /= a.dx;
-    if (a.threshold > 0.0 && area < a.threshold) return false;
-    const double zo = a.z_old[v];
-    const double zr = a.z[rc];
-    if (zr >= zo) return false;
-    const double tol = a.slope_clamp * dx_eff;
-    const double Ki = a.K ? a.K[v] : a.K0;
-    const double kam = Ki * pow(area, a.m);
-    double zn = zo;
-    if (a.n_is_one) {
-        const double spf = kam * (a.dt / dx_eff);
-        zn = (zo + zr * spf) / (1.0 + spf);
-        if ((zn - zo) > tol || isnan(zn)) zn = zo;
-    } else {
-        double eps = tol + 1.0;
-        int it = 0;
-        do {
-            const double slope = (zn - zr) / dx_eff;
-            if (slope <= 0.0) {
-                eps = 0.0;
-            } else {
-                // one pow per iteration: S^n = S^(n-1) * S (the reference
-                // evaluates both powers; the results agree to an ulp)
-                const double pn1 = pow(slope, a.nexp - 1.0);
-                const double fz = zn - zo + kam * a.dt * (pn1 * slope);
-                const double fd = 1.0 + kam * a.dt * (a.nexp / dx_eff) * pn1;
-                eps = fz / fd;
-            }
-            zn -= eps;
-            ++it;
-            if (it > 100) eps = tol / 2.0;
-        } while (fabs(eps) > tol);
-        if ((zn - zo) > tol || isnan(zn)) zn = zo;
-        if (zn < zr) zn = zr + a.slope_clamp * dx_eff;
-    }
-    a.z[v] = zn;
-    return true;
-}
-
-__device__ __forceinline__ void count_eroded(unsigned long long* ctr, bool e)
-{
-    const unsigned long long m = __ballot(e);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(ctr, static_cast<unsigned long long>(__popcll(m)));
-}
-
-__global__ void k_sp_level(const int* __restrict__ order, int begin, int end, SpArgs a)
-{
-    const int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    bool e = false;
-    if (i < end) e = sp_node(a, order[i]);
-    count_eroded(a.eroded, e);
-}
-
-// one workgroup walks levels lo..hi (outlets up): a receiver is always one
-// level shallower, so it is final when its donors read it
-__global__ void __launch_bounds__(1024) k_sp_tail(const int* __restrict__ order, const int* __restrict__ lstart,
-                                                  int lo, int hi, SpArgs a)
-{
-    for (int L = lo; L <= hi; ++L) {
-        const int b = lstart[L], e = lstart[L + 1];
-        for (int i0 = b; i0 < e; i0 += blockDim.x) {
-            const int i = i0 + threadIdx.x;
-            bool er = false;
-            if (i < e) er = sp_node(a, order[i]);
-            count_eroded(a.eroded, er);
-        }
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
 // uplift, restore, statistics
 // ---------------------------------------------------------------------------
 __global__ void k_uplift(const uint8_t* __restrict__ mask, double* __restrict__ z, double udz, long long n)
@@ -255,6 +165,7 @@ LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, con
     : nr(nr_), nc(nc_), n((long long)nr_ * nc_), dx(dx_), p(p_), flow(lem_flow_options(nr_, nc_, dx_, p_.bc))
 {
     flow.timer = &timer;
+    tiles.resize(nr, nc, p.bc);
     mask.reserve(n);
     dof.reserve(n);
     z.reserve(n);
@@ -336,15 +247,18 @@ void LemEngine::set_boundary_flats(const long long* flats, long long m)
 
 void LemEngine::stream_power(hipStream_t s)
 {
-    flow.route<double>(z_before.p, mask.p, nullptr, s);
-    flow.upstream_counts_fast(s);
-    SpArgs a;
-    a.rcv = flow.rcv.p;
-    a.dir = flow.dir.p;
-    a.cnt = flow.cnt.p;
-    a.z_old = z_before.p;
-    a.z = z.p;
-    a.K = has_K ? K_field.p : nullptr;
+    // receivers of the pre-step surface, then counts and stream power by
+    // tile passes (fluvial.hip)
+    flow.receivers_only<double>(z_before.p, mask.p, s);
+    timer.begin("tile_depths", s);
+    tiles.depths(flow.rcv.p, mask.p, s);
+    timer.end(s, 9.0 * n);  // rcv 4, mask 1 in; order 2, level starts 2 out
+    flow.cnt.reserve(n);
+    timer.begin("upstream_counts", s);
+    last_count_passes = tiles.counts(flow.rcv.p, mask.p, flow.cnt.p, s);
+    last_count_tiles = tiles.tile_runs;
+    timer.end(s, 9.0 * n);  // rcv 4, mask 1 in; cnt 4 out
+    SpParams a;
     a.K0 = p.K;
     a.m = p.m;
     a.nexp = p.n;
@@ -355,25 +269,15 @@ void LemEngine::stream_power(hipStream_t s)
     a.slope_clamp = p.slope_clamp;
     a.divide = p.divide_area_by_dx;
     a.n_is_one = std::fabs(p.n - 1.0) < 1e-6;
-    a.eroded = reinterpret_cast<unsigned long long*>(flags.p + 4);
-    if (flow.n_valid == 0) return;
-    const int nl = flow.max_depth + 1;
-    const int split = std::max(flow.tail_from, 1);
-    for (int L = 1; L < split && L < nl; ++L) {
-        const int b = flow.level_start_h[L], e = flow.level_start_h[L + 1];
-        if (e > b) {
-            timer.begin("stream_power", s);
-            k_sp_level<<<div_up(e - b, kB), kB, 0, s>>>(flow.order.p, b, e, a);
-            timer.end(s, 37.0 * (e - b));
-            LSDTT_LAUNCH_CHECK();
-        }
-    }
-    if (split < nl) {
-        timer.begin("stream_power_tail", s);
-        k_sp_tail<<<1, 1024, 0, s>>>(flow.order.p, flow.level_start.p, split, nl - 1, a);
-        timer.end(s, 37.0 * (flow.level_start_h[nl] - flow.level_start_h[split]));
-        LSDTT_LAUNCH_CHECK();
-    }
+    a.m_half = p.m == 0.5;
+    a.n1_half = p.n - 1.0 == 0.5;
+    long long er = 0;
+    timer.begin("stream_power", s);
+    last_sp_passes = tiles.stream_power(flow.rcv.p, flow.dir.p, mask.p, flow.cnt.p, z_before.p,
+                                        has_K ? K_field.p : nullptr, a, z.p, &er, s);
+    timer.end(s, (has_K ? 42.0 : 34.0) * n);  // rcv 4, dir 1, mask 1, cnt 4, z_old 8, z 8+8 (, K 8)
+    last_sp_tiles = tiles.tile_runs;
+    last_eroded = er;
 }
 
 HsDom LemEngine::hs_domain(double inv_dt) const
@@ -595,12 +499,12 @@ int LemEngine::step(LemStats* st, hipStream_t s)
         k_gather<<<div_up(n_bflats, kB), kB, 0, s>>>(z.p, bflats.p, n_bflats, bel.p);
         LSDTT_LAUNCH_CHECK();
     }
-    LSDTT_HIP(hipMemsetAsync(flags.p + 4, 0, sizeof(unsigned long long), s));
     if (p.enable_fluvial) {
         stream_power(s);
-        st->flow_levels = flow.max_depth + 1;
-        st->flow_tail_from = flow.tail_from;
-        st->n_outlets = flow.n_outlets;
+        st->count_passes = last_count_passes;
+        st->stream_power_passes = last_sp_passes;
+        st->count_tile_runs = last_count_tiles;
+        st->stream_power_tile_runs = last_sp_tiles;
     }
     int rc = 0;
     if (p.enable_hillslope) rc = hillslope(st, s);
@@ -613,10 +517,7 @@ int LemEngine::step(LemStats* st, hipStream_t s)
         LSDTT_LAUNCH_CHECK();
     }
     stats(st, s);
-    unsigned long long er = 0;
-    LSDTT_HIP(hipMemcpyAsync(&er, flags.p + 4, sizeof(er), hipMemcpyDeviceToHost, s));
-    LSDTT_HIP(hipStreamSynchronize(s));
-    st->cells_eroded = static_cast<long long>(er);
+    st->cells_eroded = p.enable_fluvial ? last_eroded : 0;
     return rc;
 }
 

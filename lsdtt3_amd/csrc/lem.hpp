@@ -21,6 +21,7 @@
 #include "common.hpp"
 #include "dbuf.hpp"
 #include "flow.hpp"
+#include "fluvial.hpp"
 #include "hillslope.hpp"
 #include "ktimer.hpp"
 
@@ -49,8 +50,8 @@ struct LemStats {
     int hillslope_converged = 1;
     double hillslope_initial_residual = 0, hillslope_residual = 0, hillslope_tolerance = 0;
     double max_slope_ratio = 0;
-    int flow_levels = 0, flow_tail_from = 0;
-    long long n_outlets = 0;
+    int count_passes = 0, stream_power_passes = 0;
+    long long count_tile_runs = 0, stream_power_tile_runs = 0;
 };
 
 class LemEngine {
@@ -74,6 +75,10 @@ public:
     double dx;
     LemParams p;
     FlowEngine flow;
+    FluvialTiles tiles;
+    int last_count_passes = 0, last_sp_passes = 0;
+    long long last_count_tiles = 0, last_sp_tiles = 0;
+    long long last_eroded = 0;
 
     dbuf<uint8_t> mask, dof, fast;
     dbuf<double> z, z_before, K_field, bel;

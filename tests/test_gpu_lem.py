@@ -112,3 +112,34 @@ def test_gpu_boundary_restore_and_mask(gpu):
     zo, so, rc = orc.lem_step(z, mask, 10.0, orc.lem_params(100.0, 1e-3, 0.45, 1.0, 1e-3, 0.02, bc="onon",
                                                             baselevel_fall=1e-4))
     assert np.abs(zg - zo)[mask != 0].max() <= 1e-6
+
+
+@pytest.mark.parametrize("bc,holes", [("popo", False), ("pnpn", True), ("nnnn", True), ("bnbn", False)])
+def test_gpu_fluvial_boundaries_and_holes(gpu, bc, holes):
+    """tile passes across periodic wraps, masked holes and base-level edges;
+    grid sizes not multiples of the 64 x 32 tile"""
+    print(f"[gpu lem fluvial bc] {bc} holes={holes}", flush=True)
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(101, 150, dx=30.0, seed=11)
+    mask = np.ones(z.shape, np.uint8)
+    if holes:
+        mask[np.random.default_rng(2).random(z.shape) < 0.03] = 0
+    kw = dict(timestep_years=1000.0, erodibility=2e-5, m_exponent=0.5, n_exponent=1.5, uplift_rate=1e-3,
+              hillslope_model="none", boundary_conditions=bc)
+    okw = dict(dt=1000.0, K=2e-5, m=0.5, n=1.5, U=1e-3, D=0.0, bc=bc, hillslope=False)
+    zg, zo, sg, so = run_both(gpu, z, 30.0, 2, kw, okw, mask=mask)
+    assert np.abs(zg - zo)[mask != 0].max() <= 1e-9
+    assert sg["cells_eroded"] == so["cells_eroded"]
+
+
+def test_gpu_fluvial_long_channel(gpu):
+    """one 1-D channel of 3000 cells crosses ~47 tiles: many passes"""
+    z = (np.arange(3000, dtype=np.float64) * 0.3)[None, :].repeat(3, 0) + 100.0
+    z[0] += 50.0
+    z[2] += 50.0
+    kw = dict(timestep_years=100.0, erodibility=1e-4, m_exponent=0.5, n_exponent=1.0, uplift_rate=1e-3,
+              hillslope_model="none", boundary_conditions="nnno")
+    okw = dict(dt=100.0, K=1e-4, m=0.5, n=1.0, U=1e-3, D=0.0, bc="nnno", hillslope=False)
+    zg, zo, sg, so = run_both(gpu, z, 10.0, 2, kw, okw)
+    assert np.abs(zg - zo).max() <= 1e-9
+    assert sg["stream_power_passes"] >= 2
