@@ -105,6 +105,33 @@ int lsdtt_gpu_flow_area(lsdtt_gpu_flow* f, double nodata, float* area, uint8_t* 
  * NULL means cell_size^2 per valid cell. Output double[n]. */
 int lsdtt_gpu_flow_accumulate(lsdtt_gpu_flow* f, const double* weights, double* accum);
 
+/* Chi raster: get_chi_raster(flow, {m_over_n, A_0})
+ * (src/cpu/lsdtt_chi_extraction_cpu.cpp:739-765; recursion :283-365,
+ * increment :154-179): every outlet starts at 0, A = contributing pixels x
+ * cell_size^2, chi float on valid cells and `nodata` elsewhere.  Needs a flow
+ * object built with the s-vector (LSDTT_GPU_ERR_RUNTIME otherwise, the
+ * reference's runtime_error); non-finite m_over_n or A_0 <= 0 give
+ * LSDTT_GPU_ERR_ARG.  chi float[n], chi_mask uint8[n] (either may be NULL). */
+int lsdtt_gpu_flow_chi(lsdtt_gpu_flow* f, double m_over_n, double A_0, double nodata, float* chi,
+                       uint8_t* chi_mask);
+
+/* Same raster into a device buffer (float[n]) on `stream`. */
+int lsdtt_gpu_flow_chi_device(lsdtt_gpu_flow* f, double m_over_n, double A_0, double nodata, float* d_chi,
+                              void* stream);
+
+/* Channel chi of every basin large enough: get_all_basin_outlet_flats(flow,
+ * min_pixels) (src/cpu/lsdtt_flow_base.cpp:82-110) followed by
+ * get_chi_from_outlets_zero_base_threshold(flow, outlets, min_pixels)
+ * (lsdtt_chi_extraction_cpu.cpp:368-497, 954-990), concatenated: for each
+ * qualifying outlet in row-major order, the pixels of its s-vector slice
+ * with at least min_pixels contributing pixels, in s-vector order.
+ * receivers[i] is the pixel's receiver (-2 for the basin outlet);
+ * basin_start[b] is the index of basin b's outlet.  Pass NULL arrays to
+ * obtain *n_pixels and *n_basins first (arrays sized n_valid suffice). */
+int lsdtt_gpu_flow_chi_threshold(lsdtt_gpu_flow* f, double m_over_n, double A_0, int64_t min_pixels,
+                                 int64_t* n_pixels, int64_t* n_basins, int64_t* flats, float* chi,
+                                 int64_t* receivers, int64_t* basin_start);
+
 /* ---------------------------------------------------------------------
  * Epsilon-free depression fill.  Replaces lsdtt::fill_pits(raster,
  * min_slope <= 0, boundary_conditions) (src/cpu/

@@ -66,6 +66,9 @@ def lib():
     _sig(L, "lsdtt_gpu_flow_get", C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp])
     _sig(L, "lsdtt_gpu_flow_area", C.c_int, [vp, dbl, vp, vp, vp])
     _sig(L, "lsdtt_gpu_flow_accumulate", C.c_int, [vp, vp, vp])
+    _sig(L, "lsdtt_gpu_flow_chi", C.c_int, [vp, dbl, dbl, dbl, vp, vp])
+    _sig(L, "lsdtt_gpu_flow_chi_device", C.c_int, [vp, dbl, dbl, dbl, vp, vp])
+    _sig(L, "lsdtt_gpu_flow_chi_threshold", C.c_int, [vp, dbl, dbl, i64, vp, vp, vp, vp, vp, vp])
     _sig(L, "lsdtt_gpu_fill", C.c_int, [vp, vp, i32, i32, dbl, C.c_char_p, vp])
     _sig(L, "lsdtt_gpu_fill_device", C.c_int, [vp, vp, i32, i32, dbl, C.c_char_p, vp, vp])
     _extend(L)

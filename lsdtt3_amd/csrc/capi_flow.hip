@@ -10,6 +10,7 @@
 
 #include "../../include/lsdtt3_gpu.h"
 #include "capi_util.hpp"
+#include "chi.hpp"
 #include "flow.hpp"
 
 namespace lsdtt_gpu {
@@ -27,6 +28,8 @@ struct lsdtt_gpu_flow {
     dbuf<double> z64;
     dbuf<uint8_t> mask, forced;
     dbuf<double> wbuf, abuf;
+    dbuf<float> chi;
+    ChiEngine chi_eng;
     bool built = false;
     bool has_topo = false, has_sv = false;
 };
@@ -289,6 +292,92 @@ int lsdtt_gpu_flow_accumulate(lsdtt_gpu_flow* f, const double* weights, double* 
         e.upstream(true, f->wbuf.p, f->abuf.p, s);
         LSDTT_HIP(hipMemcpyAsync(accum, f->abuf.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         LSDTT_HIP(hipStreamSynchronize(s));
+        return LSDTT_GPU_OK;
+    });
+}
+
+namespace {
+
+// validate_flow_for_chi / validate_chi_options
+// (lsdtt_chi_extraction_cpu.cpp:57-92), then the chi raster on the device
+int chi_prepare(lsdtt_gpu_flow* f, double m_over_n, double A_0, float nodata, float* d_out, hipStream_t s)
+{
+    if (!f || !f->built) return fail_arg("flow object not built");
+    if (!f->has_sv) {
+        set_error("lsdtt_chi_extraction_cpu: chi extraction requires a flow object with s_vector built.");
+        return LSDTT_GPU_ERR_RUNTIME;
+    }
+    if (!std::isfinite(m_over_n)) return fail_arg("lsdtt_chi_extraction_cpu: m_over_n must be finite.");
+    if (!std::isfinite(A_0) || A_0 <= 0.0) return fail_arg("lsdtt_chi_extraction_cpu: A_0 must be finite and > 0.");
+    if (!d_out) {
+        f->chi.reserve(f->eng->n);
+        d_out = f->chi.p;
+    }
+    f->chi_eng.raster(*f->eng, m_over_n, A_0, nodata, d_out, s);
+    return LSDTT_GPU_OK;
+}
+
+}  // namespace
+
+int lsdtt_gpu_flow_chi(lsdtt_gpu_flow* f, double m_over_n, double A_0, double nodata, float* chi, uint8_t* chi_mask)
+{
+    return guarded([&] {
+        hipStream_t s = nullptr;
+        const int rc = chi_prepare(f, m_over_n, A_0, static_cast<float>(nodata), nullptr, s);
+        if (rc != LSDTT_GPU_OK) return rc;
+        FlowEngine& e = *f->eng;
+        if (chi) LSDTT_HIP(hipMemcpyAsync(chi, f->chi.p, sizeof(float) * e.n, hipMemcpyDeviceToHost, s));
+        if (chi_mask) LSDTT_HIP(hipMemcpyAsync(chi_mask, e.mask, e.n, hipMemcpyDeviceToHost, s));
+        LSDTT_HIP(hipStreamSynchronize(s));
+        if (chi_mask)
+            for (long long i = 0; i < e.n; ++i) chi_mask[i] = chi_mask[i] ? 1 : 0;
+        return LSDTT_GPU_OK;
+    });
+}
+
+int lsdtt_gpu_flow_chi_device(lsdtt_gpu_flow* f, double m_over_n, double A_0, double nodata, float* d_chi,
+                              void* stream)
+{
+    return guarded([&] {
+        if (!d_chi) return fail_arg("null argument");
+        return chi_prepare(f, m_over_n, A_0, static_cast<float>(nodata), d_chi, static_cast<hipStream_t>(stream));
+    });
+}
+
+int lsdtt_gpu_flow_chi_threshold(lsdtt_gpu_flow* f, double m_over_n, double A_0, int64_t min_pixels,
+                                 int64_t* n_pixels, int64_t* n_basins, int64_t* flats, float* chi,
+                                 int64_t* receivers, int64_t* basin_start)
+{
+    return guarded([&] {
+        if (min_pixels <= 0)
+            return fail_arg("lsdtt_chi_extraction_cpu: minimum_contributing_pixels must be > 0.");
+        hipStream_t s = nullptr;
+        const int rc = chi_prepare(f, m_over_n, A_0, 0.0f, nullptr, s);
+        if (rc != LSDTT_GPU_OK) return rc;
+        FlowEngine& e = *f->eng;
+        ChiEngine& c = f->chi_eng;
+        const long long m = c.threshold(e, f->chi.p, min_pixels, s);
+        std::vector<int> fl(static_cast<size_t>(m)), rv(static_cast<size_t>(m));
+        if (m > 0) {
+            LSDTT_HIP(hipMemcpyAsync(fl.data(), c.sel_flat.p, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+            LSDTT_HIP(hipMemcpyAsync(rv.data(), c.sel_rcv.p, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+            if (chi) LSDTT_HIP(hipMemcpyAsync(chi, c.sel_chi.p, sizeof(float) * m, hipMemcpyDeviceToHost, s));
+        }
+        LSDTT_HIP(hipStreamSynchronize(s));
+        // a selected pixel whose receiver is an outlet code starts a basin
+        // (outlets meeting the threshold, row-major: the s-vector's order)
+        long long nb = 0;
+        for (long long i = 0; i < m; ++i) {
+            const int r = rv[static_cast<size_t>(i)];
+            if (r < 0) {
+                if (basin_start) basin_start[nb] = i;
+                ++nb;
+            }
+            if (flats) flats[i] = fl[static_cast<size_t>(i)];
+            if (receivers) receivers[i] = r;
+        }
+        if (n_pixels) *n_pixels = m;
+        if (n_basins) *n_basins = nb;
         return LSDTT_GPU_OK;
     });
 }

@@ -613,11 +613,18 @@ int FluvialTiles::settle(Pass&& pass, After&& after_batch, hipStream_t s)
     }
 }
 
-int FluvialTiles::counts(const int* rcv, const uint8_t* mask, int* cnt, hipStream_t s)
+int FluvialTiles::counts(const int* rcv, const uint8_t* mask, int* cnt, bool warm, hipStream_t s)
 {
+    // The counts are the unique solution of cnt(v) = 1 + sum of the donors'
+    // counts, and a pass re-runs every tile whose inputs differ from what it
+    // read, so the passes reach that solution from any starting values.
+    // Starting from the previous receivers' counts, only the tiles on routes
+    // that changed need a second pass.
     const long long n = (long long)nr * nc;
-    k_fill_i32<<<div_up(n, 256), 256, 0, s>>>(cnt, n, 0);
-    LSDTT_LAUNCH_CHECK();
+    if (!warm) {
+        k_fill_i32<<<div_up(n, 256), 256, 0, s>>>(cnt, n, 0);
+        LSDTT_LAUNCH_CHECK();
+    }
     last_passes = last_count_passes;
     const int passes = settle(
         [&](const PassArgs& P) {

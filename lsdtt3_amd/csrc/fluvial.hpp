@@ -35,8 +35,10 @@ public:
     void resize(int nr, int nc, BC bc);
     // each tile's cells in level order, once per set of receivers
     void depths(const int* rcv, const uint8_t* mask, hipStream_t s);
-    // counts (int32) of valid cells from receivers; returns passes used
-    int counts(const int* rcv, const uint8_t* mask, int* cnt, hipStream_t s);
+    // counts (int32) of valid cells from receivers; warm: cnt holds counts
+    // of earlier receivers on the same mask (the starting values); returns
+    // passes used
+    int counts(const int* rcv, const uint8_t* mask, int* cnt, bool warm, hipStream_t s);
     // stream power in place on z (z == z_old on entry); returns passes used
     int stream_power(const int* rcv, const uint8_t* dir, const uint8_t* mask, const int* cnt, const double* z_old,
                      const double* K, const SpParams& sp, double* z, long long* eroded, hipStream_t s);

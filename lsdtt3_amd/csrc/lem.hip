@@ -255,7 +255,8 @@ void LemEngine::stream_power(hipStream_t s)
     timer.end(s, 9.0 * n);  // rcv 4, mask 1 in; order 2, level starts 2 out
     flow.cnt.reserve(n);
     timer.begin("upstream_counts", s);
-    last_count_passes = tiles.counts(flow.rcv.p, mask.p, flow.cnt.p, s);
+    last_count_passes = tiles.counts(flow.rcv.p, mask.p, flow.cnt.p, counts_warm, s);
+    counts_warm = true;
     last_count_tiles = tiles.tile_runs;
     timer.end(s, 9.0 * n);  // rcv 4, mask 1 in; cnt 4 out
     SpParams a;

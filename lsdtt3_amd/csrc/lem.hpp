@@ -77,6 +77,7 @@ public:
     FlowEngine flow;
     FluvialTiles tiles;
     int last_count_passes = 0, last_sp_passes = 0;
+    bool counts_warm = false;  // flow.cnt holds the previous step's counts
     long long last_count_tiles = 0, last_sp_tiles = 0;
     long long last_eroded = 0;
 

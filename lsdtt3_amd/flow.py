@@ -149,6 +149,39 @@ class FlowInfo:
         return out
 
 
+    def get_chi_raster(self, m_over_n=0.45, A_0=1.0, no_data_value=-9999.0):
+        """lsdtt::get_chi_raster(flow, {m_over_n, A_0})
+        (src/cpu/lsdtt_chi_extraction_cpu.cpp:739-765): (chi float32
+        [n_rows, n_cols], mask uint8), every outlet at chi = 0."""
+        chi = np.empty(self.n, np.float32)
+        m = np.empty(self.n, np.uint8)
+        N.check(N.lib().lsdtt_gpu_flow_chi(self._h, float(m_over_n), float(A_0), float(no_data_value),
+                                           N.ptr(chi), N.ptr(m)))
+        return chi.reshape(self.n_rows, self.n_cols), m.reshape(self.n_rows, self.n_cols)
+
+    def get_chi_from_all_outlets_threshold(self, minimum_contributing_pixels, m_over_n=0.45, A_0=1.0):
+        """get_all_basin_outlet_flats(flow, min) followed by
+        get_chi_from_outlets_zero_base_threshold (lsdtt_chi_extraction_cpu.cpp:
+        368-497): list of basins, each a dict of outlet_flat, flats, chi and
+        receiver_flats (the outlet's receiver is -2)."""
+        npx, nb = C.c_int64(0), C.c_int64(0)
+        N.check(N.lib().lsdtt_gpu_flow_chi_threshold(self._h, float(m_over_n), float(A_0),
+                                                     int(minimum_contributing_pixels), C.byref(npx), C.byref(nb),
+                                                     None, None, None, None))
+        k, b = npx.value, nb.value
+        flats = np.empty(max(k, 1), np.int64)
+        chi = np.empty(max(k, 1), np.float32)
+        rec = np.empty(max(k, 1), np.int64)
+        bs = np.empty(max(b, 1), np.int64)
+        N.check(N.lib().lsdtt_gpu_flow_chi_threshold(self._h, float(m_over_n), float(A_0),
+                                                     int(minimum_contributing_pixels), C.byref(npx), C.byref(nb),
+                                                     N.ptr(flats), N.ptr(chi), N.ptr(rec), N.ptr(bs)))
+        bounds = list(bs[:b]) + [k]
+        return [dict(outlet_flat=int(flats[bounds[i]]), flats=flats[bounds[i]:bounds[i + 1]].copy(),
+                     chi=chi[bounds[i]:bounds[i + 1]].copy(), receiver_flats=rec[bounds[i]:bounds[i + 1]].copy())
+                for i in range(b)]
+
+
 def fill_pits(dem, mask, no_data_value=-9999.0, boundary_conditions="oooo"):
     """Epsilon-free fill (lsdtt::fill_pits with min_slope <= 0), float32."""
     z = np.ascontiguousarray(dem, dtype=np.float32)

@@ -68,6 +68,28 @@ void orc_upstream_counts(int64_t n, const uint8_t* mask,
 /* 64-bit FNV-1a continuation over n bytes */
 uint64_t orc_fnv1a(uint64_t h, const void* p, int64_t n);
 
+/* ---- chi (oracle/orc_chi.c) ------------------------------------------ */
+
+/* get_chi_raster: chi from every outlet (seed 0), A = s_size * dx^2; chi is
+ * nodata and chi_mask 0 off the basins. */
+int orc_chi_raster(int nr, int nc, double dx, const uint8_t* mask,
+                   const int64_t* receiver, const uint8_t* dir,
+                   const int64_t* s_vector, const int64_t* s_start,
+                   const uint64_t* s_size, double m_over_n, double A0,
+                   float nodata, float* chi, uint8_t* chi_mask);
+
+/* get_all_basin_outlet_flats(min_pixels) + get_chi_from_outlets_zero_base_
+ * threshold: per qualifying outlet (row-major) the slice pixels with
+ * s_size >= min_pixels in s-vector order.  Arrays sized n_valid; returns
+ * the number of pixels (or < 0), *n_basins the number of basins. */
+int64_t orc_chi_threshold(int nr, int nc, double dx, const uint8_t* mask,
+                          const int64_t* receiver, const uint8_t* dir,
+                          const int64_t* s_vector, const int64_t* s_start,
+                          const uint64_t* s_size, double m_over_n, double A0,
+                          int64_t min_pixels, int64_t* flats, float* chi,
+                          int64_t* receivers, int64_t* basin_start,
+                          int64_t* n_basins);
+
 /* ---- depression filling (oracle/orc_fill.c) ---------------------------- */
 
 /* epsilon-free priority-flood fill.  valid[i] = mask!=0 && z!=nodata.

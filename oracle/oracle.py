@@ -71,6 +71,13 @@ def lib():
         L.orc_lem_step.argtypes = [C.c_int, C.c_int, C.c_double, u8p, C.POINTER(LemParams), i64p,
                                    C.c_int64, C.c_void_p, f64p, C.POINTER(LemStats)]
         L.orc_lem_step.restype = C.c_int
+        L.orc_chi_raster.argtypes = [C.c_int, C.c_int, C.c_double, u8p, i64p, u8p, i64p, i64p, u64p,
+                                     C.c_double, C.c_double, C.c_float, f32p, u8p]
+        L.orc_chi_raster.restype = C.c_int
+        L.orc_chi_threshold.argtypes = [C.c_int, C.c_int, C.c_double, u8p, i64p, u8p, i64p, i64p, u64p,
+                                        C.c_double, C.c_double, C.c_int64, i64p, f32p, i64p, i64p,
+                                        C.POINTER(C.c_int64)]
+        L.orc_chi_threshold.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -141,6 +148,39 @@ def flow_build(z, mask, dx, bc="oooo", interior_nodata_is_outlet=True,
             raise OracleError("s-vector failure")
         out["s_vector"], out["s_start"], out["s_size"] = sv[:ns].copy(), ss, sz
     return out
+
+
+def chi_raster(flow, mask, dx, m_over_n=0.45, A0=1.0, nodata=-9999.0):
+    """orc_chi_raster on a flow_build() dict: (float32 chi, uint8 chi_mask)."""
+    nr, nc = mask.shape
+    m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+    chi = np.empty(nr * nc, np.float32)
+    cm = np.empty(nr * nc, np.uint8)
+    st = lib().orc_chi_raster(nr, nc, float(dx), m, flow["receiver"], flow["dir"], flow["s_vector"],
+                              flow["s_start"], flow["s_size"], float(m_over_n), float(A0), float(nodata),
+                              chi, cm)
+    if st != 0:
+        raise OracleError("chi extraction failed")
+    return chi.reshape(nr, nc), cm.reshape(nr, nc)
+
+
+def chi_threshold(flow, mask, dx, min_pixels, m_over_n=0.45, A0=1.0):
+    """orc_chi_threshold: dict of flats, chi, receivers, basin_start."""
+    nr, nc = mask.shape
+    m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+    nv = max(int(m.sum()), 1)
+    flats = np.empty(nv, np.int64)
+    chi = np.empty(nv, np.float32)
+    rec = np.empty(nv, np.int64)
+    bs = np.empty(nv, np.int64)
+    nb = C.c_int64(0)
+    k = lib().orc_chi_threshold(nr, nc, float(dx), m, flow["receiver"], flow["dir"], flow["s_vector"],
+                                flow["s_start"], flow["s_size"], float(m_over_n), float(A0),
+                                int(min_pixels), flats, chi, rec, bs, C.byref(nb))
+    if k < 0:
+        raise OracleError("thresholded chi extraction failed")
+    return dict(flats=flats[:k].copy(), chi=chi[:k].copy(), receivers=rec[:k].copy(),
+                basin_start=bs[:nb.value].copy())
 
 
 def accumulate(mask, receiver, topo, dx, weights=None):

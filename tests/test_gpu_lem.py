@@ -117,7 +117,7 @@ def test_gpu_boundary_restore_and_mask(gpu):
 @pytest.mark.parametrize("bc,holes", [("popo", False), ("pnpn", True), ("nnnn", True), ("bnbn", False)])
 def test_gpu_fluvial_boundaries_and_holes(gpu, bc, holes):
     """tile passes across periodic wraps, masked holes and base-level edges;
-    grid sizes not multiples of the 64 x 32 tile"""
+    grid sizes not multiples of the 32 x 32 tile"""
     print(f"[gpu lem fluvial bc] {bc} holes={holes}", flush=True)
     from lsdtt3_amd.synthetic import lem_surface
     z = lem_surface(101, 150, dx=30.0, seed=11)
@@ -133,13 +133,64 @@ def test_gpu_fluvial_boundaries_and_holes(gpu, bc, holes):
 
 
 def test_gpu_fluvial_long_channel(gpu):
-    """one 1-D channel of 3000 cells crosses ~47 tiles: many passes"""
-    z = (np.arange(3000, dtype=np.float64) * 0.3)[None, :].repeat(3, 0) + 100.0
-    z[0] += 50.0
-    z[2] += 50.0
+    """one 1-D channel of 3000 cells crosses ~94 tiles; the ridges beside it
+    fall eastward faster than the channel rises, so the tiles' mean
+    elevations run against the flow and every tile crossing is visited in
+    the wrong order: the sweeps need many passes and must still agree"""
+    x = np.arange(3000, dtype=np.float64)
+    z = np.empty((3, 3000))
+    z[1] = 100.0 + 0.3 * x
+    z[0] = z[2] = 5000.0 - 1.0 * x
     kw = dict(timestep_years=100.0, erodibility=1e-4, m_exponent=0.5, n_exponent=1.0, uplift_rate=1e-3,
               hillslope_model="none", boundary_conditions="nnno")
     okw = dict(dt=100.0, K=1e-4, m=0.5, n=1.0, U=1e-3, D=0.0, bc="nnno", hillslope=False)
+    mask = np.ones(z.shape, np.uint8)
+    lem = gpu.LandscapeModel(z, 10.0, mask=mask, **kw)
+    p = orc.lem_params(**okw)
+    zo = z.copy()
+    for step in range(2):
+        sg = lem.step()
+        zo, so, rc = orc.lem_step(zo, mask, 10.0, p)
+        assert rc == 0
+        assert np.abs(lem.elevation - zo).max() <= 1e-9
+        assert sg["cells_eroded"] == so["cells_eroded"]
+        assert sg["stream_power_passes"] >= 2
+        if step == 0:  # counts from zero: one pass per tile crossing
+            assert sg["count_passes"] >= 2 and sg["count_tile_runs"] > 94
+        else:  # same receivers: the previous counts are already the answer
+            assert sg["count_passes"] == 1
+
+
+def test_gpu_fluvial_long_channel_in_order(gpu):
+    """the same channel with tile means falling along the flow settles in
+    one pass of each sweep (tiles wait for their receivers' tiles)"""
+    z = (np.arange(3000, dtype=np.float64) * 0.3)[None, :].repeat(3, 0) + 100.0
+    z[0] += 50.0
+    z[2] += 50.0
+    kw = dict(timestep_years=100.0, erodibility=1e-4, m_exponent=0.5, n_exponent=1.5, uplift_rate=1e-3,
+              hillslope_model="none", boundary_conditions="nnno")
+    okw = dict(dt=100.0, K=1e-4, m=0.5, n=1.5, U=1e-3, D=0.0, bc="nnno", hillslope=False)
     zg, zo, sg, so = run_both(gpu, z, 10.0, 2, kw, okw)
     assert np.abs(zg - zo).max() <= 1e-9
-    assert sg["stream_power_passes"] >= 2
+    assert sg["stream_power_passes"] >= 1
+
+
+def test_gpu_counts_start_from_unrelated_counts(gpu):
+    """each step's counts start from the previous step's; after the
+    elevation is replaced by an unrelated surface the step still matches the
+    oracle's step on that surface"""
+    from lsdtt3_amd.synthetic import lem_surface
+    za = lem_surface(120, 140, dx=30.0, seed=31)
+    zb = lem_surface(120, 140, dx=30.0, seed=32)[:, ::-1].copy()
+    kw = dict(timestep_years=1000.0, erodibility=2e-5, m_exponent=0.5, n_exponent=1.5, uplift_rate=1e-3,
+              hillslope_model="none", boundary_conditions="oooo")
+    lem = gpu.LandscapeModel(za, 30.0, **kw)
+    lem.step()
+    lem.elevation = zb
+    st = lem.step()
+    mask = np.ones(zb.shape, np.uint8)
+    zo, so, rc = orc.lem_step(zb, mask, 30.0, orc.lem_params(1000.0, 2e-5, 0.5, 1.5, 1e-3, 0.0, bc="oooo",
+                                                             hillslope=False))
+    assert rc == 0
+    assert np.abs(lem.elevation - zo).max() <= 1e-9
+    assert st["cells_eroded"] == so["cells_eroded"]
