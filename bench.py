@@ -17,6 +17,14 @@ baseline is the oracle's single-threaded FP64 step on a 512 x 512 crop of
 the same recipe.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
+
+Profile summaries (text, for profiles/) from rocprofv3 CSV output:
+  python bench.py --summarize-trace DIR OUT   (--kernel-trace --stats run)
+  python bench.py --summarize-pmc FETCH_DIR WRITE_DIR OUT
+                                              (--pmc FETCH_SIZE / WRITE_SIZE runs)
+The PMC summary scales each counter by what it reads for the step's plain
+copy kernel (k_copy: n doubles read and written) and records the HBM bytes
+per dispatch of each kernel; the bench line's roofline.traffic comes from it.
 """
 from __future__ import annotations
 
@@ -68,7 +76,117 @@ def cpu_baseline(K, budget_s=15.0, size=512):
                       f"same recipe, {steps} steps in {el:.1f} s, 1 thread"}
 
 
+# timer group -> the kernel whose dispatches it times
+GROUP_KERNEL = {"stream_power": "k_tile_sp", "upstream_counts": "k_tile_counts", "tile_depths": "k_tile_order",
+                "hillslope_residual": "k_trial", "hillslope_prepare": "k_prepare", "receivers": "k_receivers"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lem_step_4096_hbm_traffic.txt")
+
+
+def _short(name):
+    n = name.replace("lsdtt_gpu::(anonymous namespace)::", "").replace("lsdtt_gpu::", "")
+    n = n.replace("void ", "")
+    return n.split("(")[0].split("<")[0]
+
+
+def _csv(path):
+    import csv
+    with open(path) as f:
+        return list(csv.DictReader(f))
+
+
+def _find(d, suffix):
+    for f in sorted(os.listdir(d)):
+        if f.endswith(suffix):
+            return os.path.join(d, f)
+    raise FileNotFoundError(f"no *{suffix} in {d}")
+
+
+def summarize_trace(d, out):
+    rows = _csv(_find(d, "kernel_stats.csv"))
+    lines = ["# rocprofv3 kernel statistics, LEM step at 4096^2 (bench.py workload)", "#",
+             "# made by: cd /tmp && rocprofv3 --kernel-trace --stats --output-format csv -d <dir> -o run -- "
+             "python3 bench.py --steps 3 --warmup 1 --no-cpu-baseline",
+             "# summarized by: python bench.py --summarize-trace <dir> <this file>",
+             "# totals cover the 4 LEM steps of the run (1 warm-up + 3 timed) plus the context set-up", "#",
+             f"{'kernel':34s} {'calls':>6s} {'total_ms':>10s} {'avg_us':>10s} {'min_us':>9s} {'max_us':>9s} {'pct':>6s}"]
+    agg = {}
+    for r in rows:
+        k = _short(r["Name"])
+        a = agg.setdefault(k, [0, 0.0, float("inf"), 0.0])
+        a[0] += int(r["Calls"])
+        a[1] += float(r["TotalDurationNs"])
+        a[2] = min(a[2], float(r["MinNs"]))
+        a[3] = max(a[3], float(r["MaxNs"]))
+    tot = sum(a[1] for a in agg.values())
+    for k, (c, t, mn, mx) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"{k[:34]:34s} {c:6d} {t / 1e6:10.3f} {t / c / 1e3:10.2f} {mn / 1e3:9.2f} {mx / 1e3:9.2f} "
+                     f"{100 * t / tot:6.2f}")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def summarize_pmc(fetch_dir, write_dir, out, n_cells=4096 * 4096):
+    per = {}
+    for key, d in (("fetch", fetch_dir), ("write", write_dir)):
+        for r in _csv(_find(d, "counter_collection.csv")):
+            k = _short(r["Kernel_Name"])
+            a = per.setdefault(k, {"fetch": [0, 0.0], "write": [0, 0.0]})[key]
+            a[0] += 1
+            a[1] += float(r["Counter_Value"])
+    copy_kib = 8.0 * n_cells / 1024.0
+    cf = per["k_copy"]["fetch"]
+    cw = per["k_copy"]["write"]
+    ff = copy_kib / (cf[1] / cf[0])
+    fw = copy_kib / (cw[1] / cw[0])
+    lines = ["# HBM traffic per dispatch from rocprofv3 PMC counters, LEM step at 4096^2 (bench.py workload)", "#",
+             "# made by: cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch_dir> -o run -- "
+             "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline",
+             "#          cd /tmp && rocprofv3 --pmc WRITE_SIZE --output-format csv -d <write_dir> -o run -- "
+             "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline",
+             "# summarized by: python bench.py --summarize-pmc <fetch_dir> <write_dir> <this file>",
+             f"# calibration: k_copy reads and writes {copy_kib:.0f} KiB per dispatch; FETCH_SIZE reads "
+             f"{cf[1] / cf[0]:.1f}, WRITE_SIZE {cw[1] / cw[0]:.1f} -> scale factors fetch {ff:.4f}, write {fw:.4f}",
+             "#",
+             f"{'kernel':34s} {'disp':>5s} {'fetch_KiB':>12s} {'write_KiB':>12s} {'HBM_MB/disp':>12s}"]
+    for k, v in sorted(per.items(), key=lambda kv: -(kv[1]["fetch"][1] * ff + kv[1]["write"][1] * fw)):
+        nf, sf = v["fetch"]
+        nw, sw = v["write"]
+        mf = sf / nf if nf else 0.0
+        mw = sw / nw if nw else 0.0
+        mb = (mf * ff + mw * fw) * 1024 / 1e6
+        lines.append(f"{k[:34]:34s} {max(nf, nw):5d} {mf:12.1f} {mw:12.1f} {mb:12.2f}")
+    lines.append("#")
+    for k, v in sorted(per.items()):
+        nf, sf = v["fetch"]
+        nw, sw = v["write"]
+        if nf and nw:
+            b = (sf / nf * ff + sw / nw * fw) * 1024
+            lines.append(f"traffic_bytes_per_dispatch {k} {b:.0f}")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def traffic_for(group, size):
+    """HBM bytes per dispatch of the group's kernel from the committed PMC
+    summary (same workload), or None."""
+    k = GROUP_KERNEL.get(group)
+    if k is None or size != 4096 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 3 and p[0] == "traffic_bytes_per_dispatch" and p[1] == k:
+                return float(p[2])
+    return None
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-trace":
+        summarize_trace(sys.argv[2], sys.argv[3])
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-pmc":
+        summarize_pmc(sys.argv[2], sys.argv[3], sys.argv[4])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
@@ -121,8 +239,9 @@ def main():
     dname, (dms, dlaunch, dbytes) = dom
     avg_s = dms / 1e3 / max(dlaunch, 1)
     achieved = dbytes / max(dlaunch, 1) / avg_s / 1e9 if avg_s > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+    roofline = {"bound": "hbm", "kernel": dname, "hip_kernel": GROUP_KERNEL.get(dname),
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(dname, args.size),
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": dlaunch,
                 "share_of_timed_kernels": round(dms / total_ms, 3) if total_ms else None}
     out = {

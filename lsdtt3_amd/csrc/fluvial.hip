@@ -574,7 +574,7 @@ void FluvialTiles::depths(const int* rcv, const uint8_t* mask, hipStream_t s)
 // empty launches.  `after_batch` may enqueue work whose result is read back
 // with the flags (the eroded-cell sum).
 template <class Pass, class After>
-int FluvialTiles::settle(Pass&& pass, After&& after_batch, hipStream_t s)
+int FluvialTiles::settle(const char* name, double first_pass_bytes, Pass&& pass, After&& after_batch, hipStream_t s)
 {
     const size_t nflags = 2 * (size_t)max_passes + 4;
     k_fill_i32<<<div_up(nflags, 256), 256, 0, s>>>(flags.p, nflags, 0);
@@ -590,7 +590,9 @@ int FluvialTiles::settle(Pass&& pass, After&& after_batch, hipStream_t s)
         for (int q = launched; q < end; ++q) {
             PassArgs P{act[q % 3].p, act[(q + 1) % 3].p, act[(q + 2) % 3].p, q > 0 ? flags.p + q - 1 : nullptr,
                        flags.p + q, flags.p + max_passes + q};
+            if (timer) timer->begin(name, s);
             pass(P);
+            if (timer) timer->end(s, q == 0 ? first_pass_bytes : 0.0);
         }
         launched = end;
         after_batch();
@@ -626,7 +628,9 @@ int FluvialTiles::counts(const int* rcv, const uint8_t* mask, int* cnt, bool war
         LSDTT_LAUNCH_CHECK();
     }
     last_passes = last_count_passes;
+    // algorithmic bytes of the sweep, counted once: rcv 4, mask 1 in; cnt 4 out
     const int passes = settle(
+        "upstream_counts", 9.0 * n,
         [&](const PassArgs& P) {
             k_tile_counts<<<dim3(ntx, nty), WL, 0, s>>>(rcv, mask, ord.p, lstart.p, tile_maxdep.p, xdon.p, nr, nc,
                                                         bc_, cnt, P);
@@ -644,7 +648,11 @@ int FluvialTiles::stream_power(const int* rcv, const uint8_t* dir, const uint8_t
     SpTileArgs A{rcv, dir, mask, cnt, ord.p, lstart.p, tile_maxdep.p, xdon.p, z_old, K, z, tile_eroded.p, sp};
     long long* d_sum = reinterpret_cast<long long*>(flags.p + 2 * max_passes);
     last_passes = last_sp_passes;
+    // algorithmic bytes of the sweep, counted once: rcv 4, dir 1, mask 1,
+    // cnt 4, z_old 8 (, K 8) in; z 8 read and 8 written
+    const double bytes = (K ? 42.0 : 34.0) * (double)nr * nc;
     const int passes = settle(
+        "stream_power", bytes,
         [&](const PassArgs& P) {
             k_tile_sp<<<dim3(ntx, nty), WL, 0, s>>>(A, nr, nc, bc_, P);
             LSDTT_LAUNCH_CHECK();

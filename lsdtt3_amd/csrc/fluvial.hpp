@@ -19,6 +19,7 @@
 
 #include "common.hpp"
 #include "dbuf.hpp"
+#include "ktimer.hpp"
 
 namespace lsdtt_gpu {
 
@@ -48,6 +49,7 @@ public:
     BC bc_{'o', 'o', 'o', 'o'};
     dbuf<int> act[3], flags, tile_eroded, tile_maxdep;
     std::vector<int> flags_h;
+    KernelTimer* timer = nullptr;  // optional: each pass timed as one launch
     dbuf<short> ord, lstart;  // tile-major level order
     dbuf<uint8_t> xdon;       // per cell: directions of donors in other tiles
 
@@ -56,7 +58,7 @@ private:
     int last_passes = 0, last_count_passes = 0, last_sp_passes = 0;
     // runs `pass(PassArgs)` until a pass marks no tile
     template <class Pass, class After>
-    int settle(Pass&& pass, After&& after_batch, hipStream_t s);
+    int settle(const char* name, double first_pass_bytes, Pass&& pass, After&& after_batch, hipStream_t s);
 };
 
 }  // namespace lsdtt_gpu

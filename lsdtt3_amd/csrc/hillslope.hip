@@ -203,37 +203,41 @@ __device__ __forceinline__ void faces_fast(const HsDom& d, const double z[9], T 
         st[k] = T(0.5) * (((k & 1) ? syc : sxc) + sxn[k]);
         const T raw = (sn[k] * sn[k] + st[k] * st[k]) * inv_Sc2;
         capped[k] = raw >= cap;
-        a[k] = T(1) / (T(1) - (raw > cap ? cap : raw));
+        if constexpr (sizeof(T) == 4)
+            a[k] = __builtin_amdgcn_rcpf(T(1) - (raw > cap ? cap : raw));  // ~1 ulp: Krylov operator only
+        else
+            a[k] = T(1) / (T(1) - (raw > cap ? cap : raw));
     }
 }
 
-// FP64 residual on the fast path (same expressions as cell_general)
-__device__ __forceinline__ double F_fast(const HsDom& d, const double z[9], double zn_c)
+// FP64 residual on the fast path (cell_general's expressions, with the
+// constant divisions by h, 2h and Sc^2 as products by their reciprocals),
+// and the Jacobian diagonal J_PP in FP64 (the FP32 row's centre entry)
+__device__ __forceinline__ double eval_fast(const HsDom& d, const double z[9], double zn_c, double* diag)
 {
-    const double h = d.h, DA = d.D / (h * h);
-    double a[4], sn[4], st[4];
-    bool cp[4];
-    // same order of operations as cell_general: (zb - zc) / h etc.
-    const double sxc = d.nonlinear ? (z[5] - z[3]) / (2.0 * h) : 0.0;
-    const double syc = d.nonlinear ? (z[7] - z[1]) / (2.0 * h) : 0.0;
-    const double sxn[4] = {(z[2] - z[0]) / (2.0 * h), (z[8] - z[2]) / (2.0 * h), (z[8] - z[6]) / (2.0 * h),
-                           (z[6] - z[0]) / (2.0 * h)};
+    const double DA = d.D * d.inv_h * d.inv_h;
+    const double sxc = (z[5] - z[3]) * d.inv_2h;
+    const double syc = (z[7] - z[1]) * d.inv_2h;
+    const double sxn[4] = {(z[2] - z[0]) * d.inv_2h, (z[8] - z[2]) * d.inv_2h, (z[8] - z[6]) * d.inv_2h,
+                           (z[6] - z[0]) * d.inv_2h};
+    double flux = 0.0, jd = 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        sn[k] = (z[f_nb[k]] - z[4]) / h;
+        const double dz = z[f_nb[k]] - z[4];
         if (!d.nonlinear) {
-            a[k] = 1.0;
+            flux += dz;
+            jd += 1.0;
             continue;
         }
-        st[k] = 0.5 * (((k & 1) ? syc : sxc) + sxn[k]);
-        double s2 = (sn[k] * sn[k] + st[k] * st[k]) / (d.Sc * d.Sc);
-        if (s2 > d.cap) s2 = d.cap;
-        a[k] = 1.0 / (1.0 - s2);
+        const double sn = dz * d.inv_h;
+        const double st = 0.5 * (((k & 1) ? syc : sxc) + sxn[k]);
+        const double raw = (sn * sn + st * st) * d.inv_Sc2;
+        const double a = 1.0 / (1.0 - (raw > d.cap ? d.cap : raw));
+        flux += a * dz;
+        jd += a;
+        if (raw < d.cap) jd += 2.0 * a * a * sn * sn * d.inv_Sc2;
     }
-    (void)cp;
-    double flux = 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) flux += a[k] * sn[k] * h;
+    *diag = d.inv_dt + DA * jd;
     double F = -d.U - DA * flux;
     if (d.inv_dt > 0.0) F += d.inv_dt * (z[4] - zn_c);
     return F;
@@ -301,37 +305,6 @@ __global__ void __launch_bounds__(256) k_fast_flags(HsDom d, uint8_t* __restrict
     }
 }
 
-__global__ void __launch_bounds__(256) k_prepare(HsDom d, const double* __restrict__ z, long long n,
-                                                 float* __restrict__ dinv, float* __restrict__ b)
-{
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
-         f += (long long)gridDim.x * blockDim.x) {
-        if (d.fast[f]) {
-            double zz[9];
-            load9(z, static_cast<int>(f), d.nc, zz);
-            b[f] = static_cast<float>(-F_fast(d, zz, d.zn[f]));
-            float c9[9];
-            J_fast<float>(d, zz, c9);
-            dinv[f] = c9[4] != 0.0f ? 1.0f / c9[4] : 0.0f;
-        } else if (!d.dof[f]) {
-            b[f] = 0.0f;
-            dinv[f] = 0.0f;
-        }
-    }
-}
-
-__global__ void __launch_bounds__(256) k_prepare_gen(HsDom d, const double* __restrict__ z, float* __restrict__ dinv,
-                                                     float* __restrict__ b)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_gen; i += gridDim.x * blockDim.x) {
-        const int f = d.gen[i];
-        const int r = f / d.nc, c = f - r * d.nc;
-        double c9[9];
-        b[f] = static_cast<float>(-cell_general(d, ZView{z, nullptr, 0.0}, r, c, c9));
-        dinv[f] = c9[4] != 0.0 ? static_cast<float>(1.0 / c9[4]) : 0.0f;
-    }
-}
-
 // y = J x (FP32) on a fast row
 __device__ __forceinline__ float jx_fast(const HsDom& d, const double* __restrict__ z, const float* __restrict__ x,
                                          long long f)
@@ -359,35 +332,7 @@ __device__ __forceinline__ float jx_gen(const HsDom& d, const double* __restrict
     return acc;
 }
 
-__global__ void __launch_bounds__(256) k_bi_init(BiArgs a)
-{
-    double acc[2] = {0.0, 0.0};
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
-         f += (long long)gridDim.x * blockDim.x) {
-        const float bv = a.b[f];
-        a.r[f] = bv;
-        a.rhat[f] = bv;
-        a.p[f] = 0.0f;
-        a.v[f] = 0.0f;
-        a.x[f] = 0.0f;
-        acc[0] += static_cast<double>(bv) * bv;
-    }
-    acc[1] = acc[0];
-    const bool mx[2] = {false, false};
-    double out[2];
-    if (grid_reduce<2>(acc, mx, a.partials, a.ticket, out)) {
-        a.scal[HS_RHO] = out[0];
-        a.scal[HS_BNORM2] = out[1];
-        a.scal[HS_RHO_OLD] = 1.0;
-        a.scal[HS_ALPHA] = 1.0;
-        a.scal[HS_OMEGA] = 1.0;
-        a.flags[0] = (out[1] == 0.0) ? 1 : 0;
-        a.flags[1] = 0;
-        a.flags[2] = 0;
-    }
-}
-
-// p = r + beta (p - omega v); phat = M^-1 p
+// p = r + beta (p - omega v); phat = M^-1 p  (first iteration: p = b)
 __global__ void __launch_bounds__(256) k_bi_p(BiArgs a)
 {
     if (a.flags[0]) return;
@@ -395,7 +340,7 @@ __global__ void __launch_bounds__(256) k_bi_p(BiArgs a)
     const float bt = static_cast<float>(beta), om = static_cast<float>(a.scal[HS_OMEGA]);
     for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
          f += (long long)gridDim.x * blockDim.x) {
-        const float pv = a.r[f] + bt * (a.p[f] - om * a.v[f]);
+        const float pv = a.first ? a.b[f] : a.r[f] + bt * (a.p[f] - om * a.v[f]);
         a.p[f] = pv;
         a.phat[f] = a.dinv[f] * pv;
     }
@@ -458,9 +403,10 @@ __global__ void __launch_bounds__(256) k_bi_s(BiArgs a)
 {
     if (a.flags[0]) return;
     const float al = static_cast<float>(a.scal[HS_ALPHA]);
+    const float* __restrict__ r = a.first ? a.b : a.r;
     for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
          f += (long long)gridDim.x * blockDim.x) {
-        const float sv = a.r[f] - al * a.v[f];
+        const float sv = r[f] - al * a.v[f];
         a.s[f] = sv;
         a.shat[f] = a.dinv[f] * sv;
     }
@@ -496,7 +442,8 @@ __global__ void __launch_bounds__(256) k_bi_x(BiArgs a)
     double acc[2] = {0.0, 0.0};
     for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
          f += (long long)gridDim.x * blockDim.x) {
-        a.x[f] += al * a.phat[f] + om * a.shat[f];
+        const float dx = al * a.phat[f] + om * a.shat[f];
+        a.x[f] = a.first ? dx : a.x[f] + dx;
         const float rv = a.s[f] - om * a.t[f];
         a.r[f] = rv;
         acc[0] += static_cast<double>(a.rhat[f]) * rv;
@@ -517,53 +464,78 @@ __global__ void __launch_bounds__(256) k_bi_x(BiArgs a)
     }
 }
 
+// general DOFs of a trial: F, the ratio, b, dinv and the trial values; block
+// partials (max |F|, max ratio, sum b^2) into slots base + block
 __global__ void __launch_bounds__(256) k_trial_gen(HsDom d, const double* __restrict__ z, const float* __restrict__ dz,
-                                                   double lam, double* partials, int base)
+                                                   double lam, double* __restrict__ z_out, float* __restrict__ b,
+                                                   float* __restrict__ dinv, double* partials, int base)
 {
     const ZView Z{z, lam != 0.0 ? dz : nullptr, lam};
-    double acc[2] = {0.0, 0.0};
+    double acc[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_gen; i += gridDim.x * blockDim.x) {
         const int f = d.gen[i];
         const int r = f / d.nc, c = f - r * d.nc;
-        acc[0] = fmax(acc[0], fabs(cell_general(d, Z, r, c, nullptr)));
+        double c9[9];
+        const double F = cell_general(d, Z, r, c, c9);
+        acc[0] = fmax(acc[0], fabs(F));
         if (d.nonlinear) acc[1] = fmax(acc[1], ratio_general(d, Z, r, c));
+        const float bf = static_cast<float>(-F);
+        b[f] = bf;
+        dinv[f] = c9[4] != 0.0 ? static_cast<float>(1.0 / c9[4]) : 0.0f;
+        acc[2] += static_cast<double>(bf) * bf;
+        if (lam != 0.0) z_out[f] = Z(f);
     }
-    const bool mx[2] = {true, true};
-    block_partial<2>(acc, mx, partials, base + blockIdx.x);
+    const bool mx[3] = {true, true, false};
+    block_partial<3>(acc, mx, partials, base + blockIdx.x);
 }
 
 __global__ void __launch_bounds__(256) k_trial(HsDom d, const double* __restrict__ z, const float* __restrict__ dz,
-                                               double lam, long long n, double* partials, unsigned* ticket,
-                                               double* out2)
+                                               double lam, long long n, double* __restrict__ z_out,
+                                               float* __restrict__ b, float* __restrict__ dinv, double* partials,
+                                               unsigned* ticket, double* scal, int* flags)
 {
     const ZView Z{z, lam != 0.0 ? dz : nullptr, lam};
-    double acc[2] = {0.0, 0.0};
+    double acc[3] = {0.0, 0.0, 0.0};
     for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
          f += (long long)gridDim.x * blockDim.x) {
-        if (!d.fast[f]) continue;
+        if (!d.fast[f]) {
+            if (!d.dof[f]) {
+                b[f] = 0.0f;
+                dinv[f] = 0.0f;
+            }
+            continue;
+        }
         double zz[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) zz[k] = Z(static_cast<int>(f) + (k / 3 - 1) * d.nc + (k % 3 - 1));
-        acc[0] = fmax(acc[0], fabs(F_fast(d, zz, d.zn[f])));
+        double diag;
+        const double F = eval_fast(d, zz, d.zn[f], &diag);
+        acc[0] = fmax(acc[0], fabs(F));
         if (d.nonlinear) {
-            const double sx = (zz[5] - zz[3]) / (2.0 * d.h), sy = (zz[7] - zz[1]) / (2.0 * d.h);
-            acc[1] = fmax(acc[1], sqrt(sx * sx + sy * sy) / d.Sc);
+            const double sx = (zz[5] - zz[3]) * d.inv_2h, sy = (zz[7] - zz[1]) * d.inv_2h;
+            acc[1] = fmax(acc[1], sqrt(sx * sx + sy * sy) * d.inv_Sc);
         }
+        const float bf = static_cast<float>(-F);
+        b[f] = bf;
+        dinv[f] = diag != 0.0 ? static_cast<float>(1.0 / diag) : 0.0f;
+        acc[2] += static_cast<double>(bf) * bf;
+        if (lam != 0.0) z_out[f] = zz[4];
     }
-    const bool mx[2] = {true, true};
-    double out[2];
-    if (grid_reduce<2>(acc, mx, partials, ticket, out, gen_grid(d.n_gen))) {
-        out2[0] = out[0];
-        out2[1] = out[1];
+    const bool mx[3] = {true, true, false};
+    double out[3];
+    if (grid_reduce<3>(acc, mx, partials, ticket, out, gen_grid(d.n_gen))) {
+        scal[HS_TRIAL_F] = out[0];
+        scal[HS_TRIAL_RATIO] = out[1];
+        // Krylov start for the system at this point: r = rhat = b, x = 0
+        scal[HS_RHO] = out[2];
+        scal[HS_BNORM2] = out[2];
+        scal[HS_RHO_OLD] = 1.0;
+        scal[HS_ALPHA] = 1.0;
+        scal[HS_OMEGA] = 1.0;
+        flags[0] = (out[2] == 0.0) ? 1 : 0;
+        flags[1] = 0;
+        flags[2] = 0;
     }
-}
-
-__global__ void __launch_bounds__(256) k_apply(const uint8_t* __restrict__ dof, double* __restrict__ z,
-                                               const float* __restrict__ dz, double lam, long long n)
-{
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
-         f += (long long)gridDim.x * blockDim.x)
-        if (dof[f]) z[f] = z[f] + lam * static_cast<double>(dz[f]);
 }
 
 }  // namespace
@@ -574,19 +546,6 @@ void hs_fast_flags(const HsDom& d, uint8_t* fast, hipStream_t s)
 {
     const long long n = (long long)d.nr * d.nc;
     k_fast_flags<<<std::max(1, std::min(4096, div_up(n, kB))), kB, 0, s>>>(d, fast);
-    LSDTT_LAUNCH_CHECK();
-}
-
-void hs_prepare(const HsDom& d, const double* z, long long n, float* dinv, float* b, int grid, hipStream_t s)
-{
-    if (d.n_gen > 0) k_prepare_gen<<<gen_grid(d.n_gen), kB, 0, s>>>(d, z, dinv, b);
-    k_prepare<<<grid, kB, 0, s>>>(d, z, n, dinv, b);
-    LSDTT_LAUNCH_CHECK();
-}
-
-void hs_bicgstab_init(const BiArgs& a, int grid, hipStream_t s)
-{
-    k_bi_init<<<grid, kB, 0, s>>>(a);
     LSDTT_LAUNCH_CHECK();
 }
 
@@ -614,17 +573,11 @@ void hs_bicgstab_iteration(const BiArgs& a, int grid, hipStream_t s, KernelTimer
     LSDTT_LAUNCH_CHECK();
 }
 
-void hs_trial(const HsDom& d, const double* z, const float* dz, double lam, long long n, double* partials,
-              unsigned* ticket, double* out2, int grid, hipStream_t s)
+void hs_trial(const HsDom& d, const double* z, const float* dz, double lam, long long n, double* z_out, float* b,
+              float* dinv, double* partials, unsigned* ticket, double* scal, int* flags, int grid, hipStream_t s)
 {
-    if (d.n_gen > 0) k_trial_gen<<<gen_grid(d.n_gen), kB, 0, s>>>(d, z, dz, lam, partials, grid);
-    k_trial<<<grid, kB, 0, s>>>(d, z, dz, lam, n, partials, ticket, out2);
-    LSDTT_LAUNCH_CHECK();
-}
-
-void hs_apply(const uint8_t* dof, double* z, const float* dz, double lam, long long n, int grid, hipStream_t s)
-{
-    k_apply<<<grid, kB, 0, s>>>(dof, z, dz, lam, n);
+    if (d.n_gen > 0) k_trial_gen<<<gen_grid(d.n_gen), kB, 0, s>>>(d, z, dz, lam, z_out, b, dinv, partials, grid);
+    k_trial<<<grid, kB, 0, s>>>(d, z, dz, lam, n, z_out, b, dinv, partials, ticket, scal, flags);
     LSDTT_LAUNCH_CHECK();
 }
 

@@ -30,7 +30,18 @@ struct HsDom {
     double h, D, U, Sc, cap, inv_dt;
     int nonlinear;
     const double* zn;
+    double inv_h, inv_2h, inv_Sc, inv_Sc2;  // 1/h, 1/(2h), 1/Sc, 1/Sc^2
 };
+
+// fills the reciprocal fields of a domain
+inline HsDom with_reciprocals(HsDom d)
+{
+    d.inv_h = 1.0 / d.h;
+    d.inv_2h = 0.5 / d.h;
+    d.inv_Sc = 1.0 / d.Sc;
+    d.inv_Sc2 = 1.0 / (d.Sc * d.Sc);
+    return d;
+}
 
 struct BiArgs {
     HsDom d;
@@ -45,6 +56,7 @@ struct BiArgs {
     int* flags;  // [0] done, [1] iterations, [2] breakdown
     double eta2;
     int max_it;
+    int first;  // first iteration: r = rhat = b, p = v = x = 0 (not read)
 };
 
 // scalar slots used by the solver (double array)
@@ -56,13 +68,14 @@ void hs_fast_flags(const HsDom& d, uint8_t* fast, hipStream_t s);
 // companion-kernel blocks for n_gen general cells (their partials follow the
 // hot kernel's `grid` partials, so `partials` holds grid + this many slots)
 int hs_general_blocks(int n_gen);
-// b = -F(z) (FP32), dinv = 1 / J_PP (FP32)
-void hs_prepare(const HsDom& d, const double* z, long long n, float* dinv, float* b, int grid, hipStream_t s);
-void hs_bicgstab_init(const BiArgs& a, int grid, hipStream_t s);
 void hs_bicgstab_iteration(const BiArgs& a, int grid, hipStream_t s, class KernelTimer* timer);
-// max |F(z + lam dz)| and max centred-gradient ratio over the DOFs
-void hs_trial(const HsDom& d, const double* z, const float* dz, double lam, long long n, double* partials,
-              unsigned* ticket, double* out2, int grid, hipStream_t s);
-void hs_apply(const uint8_t* dof, double* z, const float* dz, double lam, long long n, int grid, hipStream_t s);
+// One evaluation at the trial point zt = z + lam dz (dz unused when lam is
+// 0): scal[HS_TRIAL_F] = max |F(zt)|, scal[HS_TRIAL_RATIO] = max centred-
+// gradient ratio over the DOFs; and the next linear system at zt, should
+// the point be accepted: b = -F(zt), dinv = 1 / J_PP(zt) (FP32), the DOF
+// values of zt into z_out (when lam != 0) and the Krylov start (rho =
+// ||b||^2, flags cleared).
+void hs_trial(const HsDom& d, const double* z, const float* dz, double lam, long long n, double* z_out, float* b,
+              float* dinv, double* partials, unsigned* ticket, double* scal, int* flags, int grid, hipStream_t s);
 
 }  // namespace lsdtt_gpu

@@ -42,6 +42,15 @@ constexpr int kScalCount = HS_NSCAL + 8;
 
 inline int grid_for(long long n, int cap) { return std::max(1, std::min(cap, div_up(n, kB))); }
 
+__global__ void k_copy2(const double* __restrict__ a, double* __restrict__ b, double* __restrict__ c, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double v = a[i];
+        b[i] = v;
+        c[i] = v;
+    }
+}
+
 __global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, long long n)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -165,6 +174,7 @@ LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, con
     : nr(nr_), nc(nc_), n((long long)nr_ * nc_), dx(dx_), p(p_), flow(lem_flow_options(nr_, nc_, dx_, p_.bc))
 {
     flow.timer = &timer;
+    tiles.timer = &timer;
     tiles.resize(nr, nc, p.bc);
     mask.reserve(n);
     dof.reserve(n);
@@ -254,11 +264,9 @@ void LemEngine::stream_power(hipStream_t s)
     tiles.depths(flow.rcv.p, mask.p, s);
     timer.end(s, 9.0 * n);  // rcv 4, mask 1 in; order 2, level starts 2 out
     flow.cnt.reserve(n);
-    timer.begin("upstream_counts", s);
     last_count_passes = tiles.counts(flow.rcv.p, mask.p, flow.cnt.p, counts_warm, s);
     counts_warm = true;
     last_count_tiles = tiles.tile_runs;
-    timer.end(s, 9.0 * n);  // rcv 4, mask 1 in; cnt 4 out
     SpParams a;
     a.K0 = p.K;
     a.m = p.m;
@@ -273,18 +281,17 @@ void LemEngine::stream_power(hipStream_t s)
     a.m_half = p.m == 0.5;
     a.n1_half = p.n - 1.0 == 0.5;
     long long er = 0;
-    timer.begin("stream_power", s);
     last_sp_passes = tiles.stream_power(flow.rcv.p, flow.dir.p, mask.p, flow.cnt.p, z_before.p,
                                         has_K ? K_field.p : nullptr, a, z.p, &er, s);
-    timer.end(s, (has_K ? 42.0 : 34.0) * n);  // rcv 4, dir 1, mask 1, cnt 4, z_old 8, z 8+8 (, K 8)
     last_sp_tiles = tiles.tile_runs;
     last_eroded = er;
 }
 
 HsDom LemEngine::hs_domain(double inv_dt) const
 {
-    return HsDom{nr, nc, p.bc, mask.p, dof.p, fast.p, gen.p, n_gen, dx, p.D, 0.0, p.Sc, p.slope_ratio_cap, inv_dt,
-                 p.nonlinear, zn_hs.p};
+    HsDom d{nr, nc, p.bc, mask.p, dof.p, fast.p, gen.p, n_gen, dx, p.D, 0.0, p.Sc, p.slope_ratio_cap, inv_dt,
+            p.nonlinear, zn_hs.p};
+    return with_reciprocals(d);
 }
 
 int LemEngine::bicgstab(hipStream_t s)
@@ -292,12 +299,12 @@ int LemEngine::bicgstab(hipStream_t s)
     BiArgs a;
     a.d = hs_domain(cur_inv_dt);
     a.n = n;
-    a.z = z.p;
+    a.z = zc;
     a.dinv = dinv.p;
     a.b = b.p;
     a.x = x.p;
     a.r = r.p;
-    a.rhat = rhat.p;
+    a.rhat = b.p;  // rhat = r_0 = b, unchanged during the solve
     a.p = pv.p;
     a.phat = phat.p;
     a.v = v.p;
@@ -310,27 +317,37 @@ int LemEngine::bicgstab(hipStream_t s)
     a.flags = flags.p;
     a.eta2 = p.krylov_rel_tol * p.krylov_rel_tol;
     a.max_it = p.krylov_max_iterations;
-    hs_bicgstab_init(a, hs_grid, s);
+    a.first = 0;
+    // the start (rho = ||b||^2, flags) was set by the trial that produced b
     // iterations are enqueued in chunks between flag read-backs; an
     // iteration after convergence returns at once.  The chunk starts at the
     // usual iteration count of these well-conditioned systems and grows.
     int chunk = std::max(1, last_krylov_iterations);
     int hf[3] = {0, 0, 0};
+    int it_launched = 0;
     for (int done = 0; !done; chunk = std::min(2 * chunk, 16)) {
-        for (int k = 0; k < chunk; ++k) hs_bicgstab_iteration(a, hs_grid, s, timer.enabled ? &timer : nullptr);
+        for (int k = 0; k < chunk; ++k) {
+            BiArgs ai = a;
+            ai.first = (it_launched++ == 0) ? 1 : 0;
+            hs_bicgstab_iteration(ai, hs_grid, s, timer.enabled ? &timer : nullptr);
+        }
         LSDTT_HIP(hipMemcpyAsync(hf, flags.p, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
         LSDTT_HIP(hipStreamSynchronize(s));
         done = hf[0];
     }
     last_krylov_iterations = hf[1];
+    if (hf[1] == 0)  // no iteration ran (b = 0): the correction is zero
+        LSDTT_HIP(hipMemsetAsync(x.p, 0, sizeof(float) * n, s));
     return hf[1];
 }
 
 void LemEngine::trial(double lambda, double* fnorm, double* ratio, hipStream_t s)
 {
     timer.begin("hillslope_residual", s);
-    hs_trial(hs_domain(cur_inv_dt), z.p, x.p, lambda, n, partials.p, tickets.p, scal.p + HS_TRIAL_F, hs_grid, s);
-    timer.end(s, (lambda != 0.0 ? 22.0 : 18.0) * n);
+    hs_trial(hs_domain(cur_inv_dt), zc, x.p, lambda, n, zt, b.p, dinv.p, partials.p, tickets.p, scal.p, flags.p,
+             hs_grid, s);
+    // z 8, zn 8, fast 1, dof 1 in; b 4, dinv 4 out; a trial step adds dz 4 in, z_out 8 out
+    timer.end(s, (lambda != 0.0 ? 38.0 : 26.0) * n);
     double h[2];
     LSDTT_HIP(hipMemcpyAsync(h, scal.p + HS_TRIAL_F, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipStreamSynchronize(s));
@@ -351,7 +368,6 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
     if (st->substeps == 0 && st->rejected == 0) st->hillslope_initial_residual = fnorm;
     st->hillslope_tolerance = tol;
     const double feasible = std::sqrt(p.slope_ratio_cap);
-    const HsDom d = hs_domain(inv_dt);
     int it = 0;
     bool converged = false;
     for (; it < p.newton_max_iterations; ++it) {
@@ -359,10 +375,7 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
             converged = true;
             break;
         }
-        timer.begin("hillslope_prepare", s);
-        hs_prepare(d, z.p, n, dinv.p, b.p, hs_grid, s);
-        timer.end(s, 26.0 * n);
-        LSDTT_LAUNCH_CHECK();
+        // b and dinv at the iterate come from the trial that accepted it
         st->krylov_iterations += bicgstab(s);
         // line search (newton .cpp:553-633)
         const bool restoring = p.nonlinear && ratio_now > feasible;
@@ -391,10 +404,7 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
             lambda *= 0.5;
         }
         if (!accepted) break;
-        timer.begin("hillslope_update", s);
-        hs_apply(dof.p, z.p, x.p, lambda, n, hs_grid, s);
-        timer.end(s, 21.0 * n);
-        LSDTT_LAUNCH_CHECK();
+        std::swap(zc, zt);  // the accepted trial point is the new iterate
         fnorm = fnorm_try;
         ratio_now = ratio_try;
         if (fnorm <= tol) {
@@ -413,40 +423,52 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
 int LemEngine::hillslope(LemStats* st, hipStream_t s)
 {
     const size_t nn = static_cast<size_t>(n);
-    for (auto* bf : {&dinv, &b, &x, &r, &rhat, &pv, &phat, &v, &sv, &shat, &t}) bf->reserve(nn);
+    for (auto* bf : {&dinv, &b, &x, &r, &pv, &phat, &v, &sv, &shat, &t}) bf->reserve(nn);
     zn_hs.reserve(nn);
-    z_save.reserve(nn);
+    z_trial.reserve(nn);
     const double total = p.dt;
     double dt = total;
     const double dt_min = total * 1e-9, dt_max = total;
     double remaining = total;
     const int g = grid_for(n, 4096);
-    k_copy<<<g, kB, 0, s>>>(z.p, zn_hs.p, n);
+    // zn_hs: the state at the start of the sub-step (backward Euler's z^n,
+    // and what a rejected sub-step restores); z_trial: the line search's
+    // trial points, equal to z off the DOFs
+    k_copy2<<<g, kB, 0, s>>>(z.p, zn_hs.p, z_trial.p, n);
     LSDTT_LAUNCH_CHECK();
+    zc = z.p;
+    zt = z_trial.p;
+    int rc = 0;
     while (remaining > 0.0) {
         const double step = std::min(dt, remaining);
-        k_copy<<<g, kB, 0, s>>>(z.p, z_save.p, n);
-        LSDTT_LAUNCH_CHECK();
         const int its = newton_solve(1.0 / step, st, s);
         if (its < 0) {
-            k_copy<<<g, kB, 0, s>>>(z_save.p, z.p, n);
+            k_copy<<<g, kB, 0, s>>>(zn_hs.p, zc, n);
             LSDTT_LAUNCH_CHECK();
             ++st->rejected;
             if (st->rejected > 40 || step <= dt_min) {
                 st->hillslope_converged = 0;
-                return 1;
+                rc = 1;
+                break;
             }
             dt = std::max(dt_min, step * 0.5);
             continue;
         }
-        k_copy<<<g, kB, 0, s>>>(z.p, zn_hs.p, n);
-        LSDTT_LAUNCH_CHECK();
         remaining -= step;
+        if (remaining > 0.0) {
+            k_copy<<<g, kB, 0, s>>>(zc, zn_hs.p, n);
+            LSDTT_LAUNCH_CHECK();
+        }
         ++st->substeps;
         if (its <= 4) dt = std::min(dt_max, step * 1.7);
         else if (its >= 15) dt = std::max(dt_min, step * 0.5);
     }
-    return 0;
+    if (zc != z.p) {  // the result lives in the trial buffer
+        k_copy<<<g, kB, 0, s>>>(zc, z.p, n);
+        LSDTT_LAUNCH_CHECK();
+    }
+    zc = zt = nullptr;
+    return rc;
 }
 
 void LemEngine::stats(LemStats* st, hipStream_t s)

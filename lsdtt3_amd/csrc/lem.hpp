@@ -89,8 +89,11 @@ public:
     bool has_K = false;
 
     // hillslope work
-    dbuf<double> zn_hs, z_save;
-    dbuf<float> dinv, b, x, r, rhat, pv, phat, v, sv, shat, t;
+    dbuf<double> zn_hs, z_trial;
+    // hillslope iterate and trial point: z / z_trial, swapped on acceptance
+    double* zc = nullptr;
+    double* zt = nullptr;
+    dbuf<float> dinv, b, x, r, pv, phat, v, sv, shat, t;
     dbuf<double> partials, scal;  // reduction partials; scalars
     dbuf<unsigned> tickets;
     dbuf<int> flags;
