@@ -180,6 +180,19 @@ typedef struct {
     double slope_ratio_cap;          /* 0.9999 */
     double krylov_rel_tol;           /* BiCGSTAB (FP32), default 1e-4 */
     int32_t krylov_max_iterations;   /* 500 */
+    /* drainage_area_moving_average_kernel_size_pixels: odd box average of
+     * the drainage area before stream power (1 = none) */
+    int32_t drainage_area_kernel_size;
+    /* multiply_stream_power_term_by_w_over_dx, channel_width_w */
+    int32_t multiply_by_w_over_dx;
+    double channel_width_w;
+    /* hillslope_coupling_regime: 0 automatic, 1 coupled, 2
+     * channels_as_boundaries (lsdtt_lem_cpu.cpp:64-77).  The Dirichlet
+     * channel set is the mask given to lsdtt_gpu_lem_set_channel_mask; a
+     * set derived from incision_threshold_area needs the D-infinity flow
+     * graph of the reference (lsdtt_lem_cpu.cpp:137-154), which this
+     * library does not build: such a step returns LSDTT_GPU_ERR_ARG. */
+    int32_t hillslope_coupling_regime;
 } lsdtt_gpu_lem_options;
 
 typedef struct {
@@ -219,6 +232,10 @@ int lsdtt_gpu_lem_get_elevation(lsdtt_gpu_lem* lem, double* z);
 /* Per-pixel erodibility (check_lem_erodibility_field: finite, >= 0);
  * NULL returns to the uniform K. */
 int lsdtt_gpu_lem_set_erodibility_field(lsdtt_gpu_lem* lem, const double* K);
+/* explicit_channel_mask (lsdtt_lem_options, lsdtt_lem_cpu.cpp:93-103):
+ * cells != 0 are fixed-elevation channel pixels for the hillslope step
+ * (not unknowns), unless the regime is coupled.  NULL clears the mask. */
+int lsdtt_gpu_lem_set_channel_mask(lsdtt_gpu_lem* lem, const uint8_t* channel);
 /* Device address of the resident FP64 elevation (for device-side I/O). */
 double* lsdtt_gpu_lem_elevation_device(lsdtt_gpu_lem* lem);
 

@@ -17,6 +17,8 @@ class LemOptions(C.Structure):
         ("newton_max_iterations", C.c_int32), ("newton_max_line_search", C.c_int32),
         ("slope_ratio_cap", C.c_double), ("krylov_rel_tol", C.c_double),
         ("krylov_max_iterations", C.c_int32),
+        ("drainage_area_kernel_size", C.c_int32), ("multiply_by_w_over_dx", C.c_int32),
+        ("channel_width_w", C.c_double), ("hillslope_coupling_regime", C.c_int32),
     ]
 
 
@@ -47,6 +49,7 @@ def declare(L, sig):
     sig(L, "lsdtt_gpu_lem_set_elevation", C.c_int, [vp, vp])
     sig(L, "lsdtt_gpu_lem_get_elevation", C.c_int, [vp, vp])
     sig(L, "lsdtt_gpu_lem_set_erodibility_field", C.c_int, [vp, vp])
+    sig(L, "lsdtt_gpu_lem_set_channel_mask", C.c_int, [vp, vp])
     sig(L, "lsdtt_gpu_lem_elevation_device", C.c_void_p, [vp])
     sig(L, "lsdtt_gpu_lem_step", C.c_int, [vp, vp, i64, C.POINTER(LemStats)])
     sig(L, "lsdtt_gpu_build_boundary_flats", C.c_int, [i32, i32, vp, C.c_char_p, vp, vp])

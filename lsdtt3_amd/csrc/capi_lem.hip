@@ -28,7 +28,7 @@ bool bc_ok(const char* s)
     return true;
 }
 
-LemParams to_params(const lsdtt_gpu_lem_options& o)
+LemParams to_params(const lsdtt_gpu_lem_options& o, double o_cell_size)
 {
     LemParams p;
     p.dt = o.timestep;
@@ -55,6 +55,10 @@ LemParams to_params(const lsdtt_gpu_lem_options& o)
     p.slope_ratio_cap = o.slope_ratio_cap;
     p.krylov_rel_tol = o.krylov_rel_tol;
     p.krylov_max_iterations = o.krylov_max_iterations;
+    p.area_kernel = o.drainage_area_kernel_size;
+    // lsdtt_stream_power_cpu.cpp:181-183
+    p.width_factor = o.multiply_by_w_over_dx ? o.channel_width_w / o_cell_size : 1.0;
+    p.regime = o.hillslope_coupling_regime;
     return p;
 }
 
@@ -91,6 +95,10 @@ void lsdtt_gpu_lem_options_init(lsdtt_gpu_lem_options* o)
     o->slope_ratio_cap = 0.9999;
     o->krylov_rel_tol = 1e-4;
     o->krylov_max_iterations = 500;
+    o->drainage_area_kernel_size = 1;
+    o->multiply_by_w_over_dx = 0;
+    o->channel_width_w = 1.0;
+    o->hillslope_coupling_regime = 0;
 }
 
 lsdtt_gpu_lem* lsdtt_gpu_lem_create(int32_t n_rows, int32_t n_cols, double cell_size, const uint8_t* mask,
@@ -105,8 +113,16 @@ lsdtt_gpu_lem* lsdtt_gpu_lem_create(int32_t n_rows, int32_t n_cols, double cell_
         if (!(o->timestep > 0.0)) throw std::invalid_argument("timestep must be positive");
         if (o->enable_hillslope_transport && o->enable_nonlinear_hillslopes && !(o->critical_hillslope_slope > 0.0))
             throw std::invalid_argument("critical_hillslope_slope must be positive");
+        if (o->drainage_area_kernel_size < 1 || o->drainage_area_kernel_size % 2 == 0)
+            throw std::invalid_argument("drainage area moving-average kernel size must be a positive odd number");
+        if (o->multiply_by_w_over_dx && (!(o->channel_width_w > 0.0) || !std::isfinite(o->channel_width_w)))
+            throw std::invalid_argument(
+                "channel_width_w must be finite and positive when the width correction is enabled");
+        if (o->hillslope_coupling_regime < 0 || o->hillslope_coupling_regime > 2)
+            throw std::invalid_argument("hillslope_coupling_regime must be 0 (auto), 1 (coupled) or 2 "
+                                        "(channels_as_boundaries)");
         auto L = std::make_unique<lsdtt_gpu_lem>();
-        L->eng = std::make_unique<LemEngine>(n_rows, n_cols, cell_size, mask, to_params(*o));
+        L->eng = std::make_unique<LemEngine>(n_rows, n_cols, cell_size, mask, to_params(*o, cell_size));
         return L.release();
     });
 }
@@ -143,6 +159,15 @@ int lsdtt_gpu_lem_set_erodibility_field(lsdtt_gpu_lem* lem, const double* K)
                     return fail_arg("run_lem_step: erodibility field must be finite and non-negative");
         }
         lem->eng->set_erodibility(K);
+        return LSDTT_GPU_OK;
+    });
+}
+
+int lsdtt_gpu_lem_set_channel_mask(lsdtt_gpu_lem* lem, const uint8_t* channel)
+{
+    return guarded([&] {
+        if (!lem) return fail_arg("null argument");
+        lem->eng->set_channel_mask(channel);
         return LSDTT_GPU_OK;
     });
 }

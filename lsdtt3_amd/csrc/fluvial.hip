@@ -367,6 +367,7 @@ struct SpTileArgs {
     const uint8_t* dir;
     const uint8_t* mask;
     const int* cnt;
+    const double* area;  // null: count * pixel_area
     const short* ord;
     const short* lstart;
     const int* tile_maxdep;
@@ -437,7 +438,7 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
     for (int k0 = 0; k0 < PER; k0 += 8) {
         uint8_t m[8], dr[8];
         int g[8], ct[8];
-        double zo[8], Kf[8];
+        double zo[8], Kf[8], ar[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const LaneCell q = lane_cell(k0 + j, r0, c0, nr, nc);
@@ -447,6 +448,7 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
             ct[j] = A.cnt[q.f];
             zo[j] = A.z_old[q.f];
             Kf[j] = A.K ? A.K[q.f] : p.K0;
+            ar[j] = A.area ? A.area[q.f] : 0.0;
             x[k0 + j] = A.xdon[q.f];
             valid[k0 + j] = q.in && m[j];
         }
@@ -456,10 +458,11 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
             double km = -1.0;
             short l = kNoRcv;
             if (valid[k] && g[j] >= 0) {
-                double area = static_cast<double>(ct[j]) * p.pixel_area;
+                double area = A.area ? ar[j] : static_cast<double>(ct[j]) * p.pixel_area;
                 if (p.divide) area /= p.dx;
                 if (!(p.threshold > 0.0 && area < p.threshold)) {
-                    km = Kf[j] * (p.m_half ? sqrt(area) : pow(area, p.m));
+                    // K_i * width_factor * pow(area, m) (lsdtt_stream_power_cpu.cpp:258-260)
+                    km = Kf[j] * p.width * (p.m_half ? sqrt(area) : pow(area, p.m));
                     const int li = local_index(g[j], nc, r0, c0);
                     l = li >= 0 ? static_cast<short>(li) : kOutRcv;
                 }
@@ -521,6 +524,83 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
     }
 }
 
+// smooth_drainage_area (lsdtt_stream_power_cpu.cpp:34-139), row pass: one
+// thread per row keeps the reference's rolling sum (window -r..r, then per
+// column: record, subtract column c - r, add column c + r + 1)
+__device__ __forceinline__ int wrap_or_none(int i, int size, bool neg, bool pos)
+{
+    if (i < 0 && !neg) return -1;
+    if (i >= size && !pos) return -1;
+    return (i % size + size) % size;
+}
+
+__global__ void k_smooth_rows(const int* __restrict__ cnt, const uint8_t* __restrict__ mask, int nr, int nc,
+                              double pa, int rad, bool pw, bool pe, double* __restrict__ hsum, int* __restrict__ hcnt)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    const long long row = (long long)r * nc;
+    double sum = 0.0;
+    int count = 0;
+    for (int o = -rad; o <= rad; ++o) {
+        const int c = wrap_or_none(o, nc, pw, pe);
+        if (c >= 0 && mask[row + c]) {
+            sum += static_cast<double>(cnt[row + c]) * pa;
+            ++count;
+        }
+    }
+    for (int c = 0; c < nc; ++c) {
+        hsum[row + c] = sum;
+        hcnt[row + c] = count;
+        if (c + 1 < nc) {
+            const int cm = wrap_or_none(c - rad, nc, pw, pe);
+            if (cm >= 0 && mask[row + cm]) {
+                sum += -1.0 * (static_cast<double>(cnt[row + cm]) * pa);
+                --count;
+            }
+            const int cp = wrap_or_none(c + rad + 1, nc, pw, pe);
+            if (cp >= 0 && mask[row + cp]) {
+                sum += static_cast<double>(cnt[row + cp]) * pa;
+                ++count;
+            }
+        }
+    }
+}
+
+// column pass over the row sums: one thread per column (a wavefront reads
+// 64 consecutive columns of a row: coalesced)
+__global__ void k_smooth_cols(const uint8_t* __restrict__ mask, int nr, int nc, int rad, bool pn, bool ps,
+                              const double* __restrict__ hsum, const int* __restrict__ hcnt, double* __restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double sum = 0.0;
+    long long count = 0;
+    for (int o = -rad; o <= rad; ++o) {
+        const int r = wrap_or_none(o, nr, pn, ps);
+        if (r >= 0) {
+            sum += hsum[(long long)r * nc + c];
+            count += hcnt[(long long)r * nc + c];
+        }
+    }
+    for (int r = 0; r < nr; ++r) {
+        const long long f = (long long)r * nc + c;
+        out[f] = (mask[f] && count != 0) ? sum / static_cast<double>(count) : 0.0;
+        if (r + 1 < nr) {
+            const int rm = wrap_or_none(r - rad, nr, pn, ps);
+            if (rm >= 0) {
+                sum += -1.0 * hsum[(long long)rm * nc + c];
+                count -= hcnt[(long long)rm * nc + c];
+            }
+            const int rp = wrap_or_none(r + rad + 1, nr, pn, ps);
+            if (rp >= 0) {
+                sum += hsum[(long long)rp * nc + c];
+                count += hcnt[(long long)rp * nc + c];
+            }
+        }
+    }
+}
+
 __global__ void k_fill_i32(int* __restrict__ a, long long n, int v)
 {
     const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -561,6 +641,20 @@ void FluvialTiles::resize(int nr_, int nc_, BC bc)
     // per pass: marked flag, tiles run; then the eroded-cell sum (8-aligned)
     flags.reserve(2 * (size_t)max_passes + 4);
     flags_h.resize(2 * (size_t)max_passes + 4);
+}
+
+void FluvialTiles::smooth_area(const int* cnt, const uint8_t* mask, double pixel_area, int kernel_size,
+                               double* out, hipStream_t s)
+{
+    const long long n = (long long)nr * nc;
+    hsum.reserve(n);
+    hcnt.reserve(n);
+    const int rad = kernel_size / 2;
+    k_smooth_rows<<<div_up(nr, 64), 64, 0, s>>>(cnt, mask, nr, nc, pixel_area, rad, bc_.w == 'p', bc_.e == 'p',
+                                                hsum.p, hcnt.p);
+    LSDTT_LAUNCH_CHECK();
+    k_smooth_cols<<<div_up(nc, 64), 64, 0, s>>>(mask, nr, nc, rad, bc_.n == 'p', bc_.s == 'p', hsum.p, hcnt.p, out);
+    LSDTT_LAUNCH_CHECK();
 }
 
 void FluvialTiles::depths(const int* rcv, const uint8_t* mask, hipStream_t s)
@@ -642,10 +736,10 @@ int FluvialTiles::counts(const int* rcv, const uint8_t* mask, int* cnt, bool war
 }
 
 int FluvialTiles::stream_power(const int* rcv, const uint8_t* dir, const uint8_t* mask, const int* cnt,
-                               const double* z_old, const double* K, const SpParams& sp, double* z, long long* eroded,
-                               hipStream_t s)
+                               const double* area, const double* z_old, const double* K, const SpParams& sp,
+                               double* z, long long* eroded, hipStream_t s)
 {
-    SpTileArgs A{rcv, dir, mask, cnt, ord.p, lstart.p, tile_maxdep.p, xdon.p, z_old, K, z, tile_eroded.p, sp};
+    SpTileArgs A{rcv, dir, mask, cnt, area, ord.p, lstart.p, tile_maxdep.p, xdon.p, z_old, K, z, tile_eroded.p, sp};
     long long* d_sum = reinterpret_cast<long long*>(flags.p + 2 * max_passes);
     last_passes = last_sp_passes;
     // algorithmic bytes of the sweep, counted once: rcv 4, dir 1, mask 1,

@@ -27,6 +27,7 @@ struct SpParams {
     double K0, m, nexp, dt, dx, pixel_area, threshold, slope_clamp;
     int divide, n_is_one;
     int m_half, n1_half;  // m == 0.5, n - 1 == 0.5: sqrt in place of pow
+    double width;         // w / dx or 1 (multiplies K)
 };
 
 class FluvialTiles {
@@ -41,8 +42,15 @@ public:
     // passes used
     int counts(const int* rcv, const uint8_t* mask, int* cnt, bool warm, hipStream_t s);
     // stream power in place on z (z == z_old on entry); returns passes used
-    int stream_power(const int* rcv, const uint8_t* dir, const uint8_t* mask, const int* cnt, const double* z_old,
-                     const double* K, const SpParams& sp, double* z, long long* eroded, hipStream_t s);
+    // (area: per-cell drainage area in m^2, or null for count * dx^2)
+    int stream_power(const int* rcv, const uint8_t* dir, const uint8_t* mask, const int* cnt, const double* area,
+                     const double* z_old, const double* K, const SpParams& sp, double* z, long long* eroded,
+                     hipStream_t s);
+    // smooth_drainage_area of count * pixel_area: rolling row sums, then
+    // rolling column sums, in the reference's order (one thread per row,
+    // then per column), so the averages are bit-identical
+    void smooth_area(const int* cnt, const uint8_t* mask, double pixel_area, int kernel_size, double* out,
+                     hipStream_t s);
 
     int nr = 0, nc = 0, ntx = 0, nty = 0, ntiles = 0, max_passes = 0;
     long long tile_runs = 0;  // tiles processed by the last counts / stream_power
@@ -52,6 +60,8 @@ public:
     KernelTimer* timer = nullptr;  // optional: each pass timed as one launch
     dbuf<short> ord, lstart;  // tile-major level order
     dbuf<uint8_t> xdon;       // per cell: directions of donors in other tiles
+    dbuf<double> hsum;        // smoothing: row sums
+    dbuf<int> hcnt;           // smoothing: row counts
 
 private:
     // passes of the last call of each sweep: the next call's first batch

@@ -181,17 +181,42 @@ LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, con
     z.reserve(n);
     z_before.reserve(n);
     LSDTT_HIP(hipMemcpy(mask.p, mask_host, n, hipMemcpyHostToDevice));
+    mask_h.assign(mask_host, mask_host + n);
+    build_dofs();
+    hs_grid = grid_for(n, 2048);
+    scal.reserve(kScalCount + 8);
+    tickets.reserve(4);
+    flags.reserve(8);
+    LSDTT_HIP(hipMemset(tickets.p, 0, sizeof(unsigned) * 4));
+    LSDTT_HIP(hipMemset(scal.p, 0, sizeof(double) * (kScalCount + 8)));
+}
+
+// resolve_hillslope_coupling_regime (lsdtt_lem_cpu.cpp:64-77): a channel
+// set exists when the regime is channels_as_boundaries, or automatic with a
+// threshold or an explicit mask
+bool LemEngine::channels_active() const
+{
+    const bool has_mask = !channel_h.empty();
+    const int regime = p.regime != 0 ? p.regime : ((p.threshold > 0.0 || has_mask) ? 2 : 1);
+    return regime == 2;
+}
+
+void LemEngine::build_dofs()
+{
     // hillslope DOFs: valid, not on an 'o' (Dirichlet) edge
-    // (lsdtt_hillslope_operator_cpu.hpp:136-142; no channel set in the
-    // coupled regime, lsdtt_lem_cpu.cpp:64-77)
+    // (lsdtt_hillslope_operator_cpu.hpp:136-142), not a channel pixel
+    // (build_channel_mask, lsdtt_lem_cpu.cpp:85-134: explicit mask only)
+    const bool use_channels = channels_active() && !channel_h.empty();
     std::vector<uint8_t> d(static_cast<size_t>(n));
     for (int r = 0; r < nr; ++r)
         for (int c = 0; c < nc; ++c) {
             const long long f = (long long)r * nc + c;
             const bool edge = (r == 0 && p.bc.n == 'o') || (r == nr - 1 && p.bc.s == 'o') ||
                               (c == 0 && p.bc.w == 'o') || (c == nc - 1 && p.bc.e == 'o');
-            d[static_cast<size_t>(f)] = (mask_host[f] && !edge) ? 1 : 0;
+            const bool ch = use_channels && channel_h[static_cast<size_t>(f)] != 0;
+            d[static_cast<size_t>(f)] = (mask_h[static_cast<size_t>(f)] && !edge && !ch) ? 1 : 0;
         }
+    dof.reserve(n);
     LSDTT_HIP(hipMemcpy(dof.p, d.data(), n, hipMemcpyHostToDevice));
     // fast DOFs: interior, whole 3x3 block valid; the rest go on the
     // general list handled by the companion kernels
@@ -202,7 +227,7 @@ LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, con
             const long long f = (long long)r * nc + c;
             if (!d[static_cast<size_t>(f)]) continue;
             bool ok = r > 0 && r < nr - 1 && c > 0 && c < nc - 1;
-            for (int k = 0; ok && k < 9; ++k) ok = mask_host[f + (k / 3 - 1) * nc + (k % 3 - 1)] != 0;
+            for (int k = 0; ok && k < 9; ++k) ok = mask_h[static_cast<size_t>(f + (k / 3 - 1) * nc + (k % 3 - 1))] != 0;
             if (ok) fh[static_cast<size_t>(f)] = 1;
             else gh.push_back(static_cast<int>(f));
         }
@@ -211,13 +236,14 @@ LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, con
     n_gen = static_cast<int>(gh.size());
     gen.reserve(gh.size() ? gh.size() : 1);
     if (n_gen) LSDTT_HIP(hipMemcpy(gen.p, gh.data(), sizeof(int) * gh.size(), hipMemcpyHostToDevice));
-    hs_grid = grid_for(n, 2048);
-    partials.reserve(static_cast<size_t>(hs_grid + hs_general_blocks(n_gen) + 1) * 8);
-    scal.reserve(kScalCount + 8);
-    tickets.reserve(4);
-    flags.reserve(8);
-    LSDTT_HIP(hipMemset(tickets.p, 0, sizeof(unsigned) * 4));
-    LSDTT_HIP(hipMemset(scal.p, 0, sizeof(double) * (kScalCount + 8)));
+    partials.reserve(static_cast<size_t>(grid_for(n, 2048) + hs_general_blocks(n_gen) + 1) * 8);
+}
+
+void LemEngine::set_channel_mask(const uint8_t* ch)
+{
+    if (ch) channel_h.assign(ch, ch + n);
+    else channel_h.clear();
+    build_dofs();
 }
 
 void LemEngine::set_elevation(const double* zh)
@@ -280,8 +306,15 @@ void LemEngine::stream_power(hipStream_t s)
     a.n_is_one = std::fabs(p.n - 1.0) < 1e-6;
     a.m_half = p.m == 0.5;
     a.n1_half = p.n - 1.0 == 0.5;
+    a.width = p.width_factor;
+    const double* area = nullptr;
+    if (p.area_kernel > 1) {  // smooth_drainage_area (lsdtt_stream_power_cpu.cpp:34-139)
+        area_sm.reserve(n);
+        tiles.smooth_area(flow.cnt.p, mask.p, dx * dx, p.area_kernel, area_sm.p, s);
+        area = area_sm.p;
+    }
     long long er = 0;
-    last_sp_passes = tiles.stream_power(flow.rcv.p, flow.dir.p, mask.p, flow.cnt.p, z_before.p,
+    last_sp_passes = tiles.stream_power(flow.rcv.p, flow.dir.p, mask.p, flow.cnt.p, area, z_before.p,
                                         has_K ? K_field.p : nullptr, a, z.p, &er, s);
     last_sp_tiles = tiles.tile_runs;
     last_eroded = er;
@@ -515,6 +548,11 @@ void LemEngine::stats(LemStats* st, hipStream_t s)
 int LemEngine::step(LemStats* st, hipStream_t s)
 {
     *st = LemStats{};
+    if (p.enable_hillslope && channels_active() && channel_h.empty() && p.threshold > 0.0)
+        throw std::invalid_argument(
+            "run_lem_step: a channel set derived from incision_threshold_area needs the D-infinity flow graph "
+            "(lsdtt_lem_cpu.cpp:137-154), which this library does not build; pass an explicit channel mask or "
+            "the coupled regime");
     const int g = grid_for(n, 4096);
     k_copy<<<g, kB, 0, s>>>(z.p, z_before.p, n);
     LSDTT_LAUNCH_CHECK();

@@ -40,6 +40,9 @@ struct LemParams {
     int newton_max_iterations = 50, newton_max_line_search = 30;
     double krylov_rel_tol = 1e-4;
     int krylov_max_iterations = 500;
+    int area_kernel = 1;        // drainage-area moving average (odd; 1 = none)
+    double width_factor = 1.0;  // w / dx or 1
+    int regime = 0;             // 0 automatic, 1 coupled, 2 channels_as_boundaries
 };
 
 struct LemStats {
@@ -57,6 +60,8 @@ struct LemStats {
 class LemEngine {
 public:
     LemEngine(int nr, int nc, double dx, const uint8_t* mask_host, const LemParams& p);
+    // explicit Dirichlet channel mask for the hillslope (null clears it)
+    void set_channel_mask(const uint8_t* channel_host);
 
     void set_elevation(const double* z_host);
     void get_elevation(double* z_host);
@@ -82,7 +87,8 @@ public:
     long long last_eroded = 0;
 
     dbuf<uint8_t> mask, dof, fast;
-    dbuf<double> z, z_before, K_field, bel;
+    dbuf<double> z, z_before, K_field, bel, area_sm;
+    std::vector<uint8_t> mask_h, channel_h;  // host copies (DOF set rebuilds)
     dbuf<int> bflats, gen;
     int n_gen = 0;
     long long n_bflats = 0;
@@ -112,6 +118,10 @@ private:
     int bicgstab(hipStream_t s);
     void trial(double lambda, double* fnorm, double* ratio, hipStream_t s);
     HsDom hs_domain(double inv_dt) const;
+    // hillslope DOFs (valid, not on an 'o' edge, not a channel pixel of an
+    // explicit mask outside the coupled regime), fast flags, general list
+    void build_dofs();
+    bool channels_active() const;
     void stats(LemStats* st, hipStream_t s);
 };
 

@@ -113,6 +113,17 @@ class LandscapeModel:
         """device address of the resident FP64 elevation"""
         return int(N.lib().lsdtt_gpu_lem_elevation_device(self._h))
 
+    def set_channel_mask(self, channel):
+        """explicit_channel_mask: cells != 0 are fixed channel pixels for the
+        hillslope step (None clears it)."""
+        if channel is None:
+            N.check(N.lib().lsdtt_gpu_lem_set_channel_mask(self._h, None))
+            return
+        ch = np.ascontiguousarray(channel, dtype=np.uint8)
+        if ch.shape != (self.n_rows, self.n_cols):
+            raise ValueError("channel mask must have the DEM's shape")
+        N.check(N.lib().lsdtt_gpu_lem_set_channel_mask(self._h, N.ptr(ch)))
+
     def set_erodibility_field(self, K):
         Kd = None if K is None else np.ascontiguousarray(K, dtype=np.float64).ravel()
         N.check(N.lib().lsdtt_gpu_lem_set_erodibility_field(self._h, N.ptr(Kd)))

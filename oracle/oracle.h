@@ -105,7 +105,15 @@ typedef struct {
     double threshold;          /* channel threshold on area/dx (0 = all) */
     int    divide_area_by_dx;  /* default 1 */
     double slope_clamp;        /* default 1e-5 */
+    double width_factor;       /* w / dx when multiply_stream_power_term_by_w_over_dx, else 1 */
 } orc_sp_params;
+
+/* smooth_drainage_area (src/cpu/lsdtt_stream_power_cpu.cpp:34-139): odd box
+ * moving average of area over valid cells, rolling row sums then rolling
+ * column sums in the reference's order; periodic axes wrap.  Returns
+ * ORC_ERR_ARG for an even or non-positive kernel size. */
+int orc_smooth_area(int nr, int nc, const double* area, const uint8_t* mask,
+                    int kernel_size, const char* bc, double* out);
 
 /* double-state D8 implicit stream power, in place on z (z_old is the
  * pre-incision surface, area in m^2 per cell). Returns cells eroded. */
@@ -149,6 +157,9 @@ typedef struct {
     int    enable_fluvial, enable_hillslope, enable_uplift;
     char   bc[4];
     orc_hs_params hs;
+    int    area_kernel;            /* drainage_area_moving_average_kernel_size_pixels (1) */
+    double width_factor;           /* w / dx or 1 */
+    const uint8_t* channel;        /* explicit Dirichlet channel set for the hillslope, or NULL */
 } orc_lem_params;
 
 typedef struct {

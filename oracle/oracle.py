@@ -71,6 +71,8 @@ def lib():
         L.orc_lem_step.argtypes = [C.c_int, C.c_int, C.c_double, u8p, C.POINTER(LemParams), i64p,
                                    C.c_int64, C.c_void_p, f64p, C.POINTER(LemStats)]
         L.orc_lem_step.restype = C.c_int
+        L.orc_smooth_area.argtypes = [C.c_int, C.c_int, f64p, u8p, C.c_int, C.c_char_p, f64p]
+        L.orc_smooth_area.restype = C.c_int
         L.orc_chi_raster.argtypes = [C.c_int, C.c_int, C.c_double, u8p, i64p, u8p, i64p, i64p, u64p,
                                      C.c_double, C.c_double, C.c_float, f32p, u8p]
         L.orc_chi_raster.restype = C.c_int
@@ -221,7 +223,7 @@ def fill(z, mask, nodata=-9999.0, bc="oooo"):
 class SpParams(C.Structure):
     _fields_ = [("K", C.c_double), ("m", C.c_double), ("n", C.c_double), ("dt", C.c_double),
                 ("dx", C.c_double), ("threshold", C.c_double), ("divide_area_by_dx", C.c_int),
-                ("slope_clamp", C.c_double)]
+                ("slope_clamp", C.c_double), ("width_factor", C.c_double)]
 
 
 class HsParams(C.Structure):
@@ -242,7 +244,8 @@ class LemParams(C.Structure):
                 ("threshold", C.c_double), ("slope_clamp", C.c_double), ("U", C.c_double),
                 ("baselevel_fall", C.c_double), ("divide_area_by_dx", C.c_int),
                 ("enable_fluvial", C.c_int), ("enable_hillslope", C.c_int),
-                ("enable_uplift", C.c_int), ("bc", C.c_char * 4), ("hs", HsParams)]
+                ("enable_uplift", C.c_int), ("bc", C.c_char * 4), ("hs", HsParams),
+                ("area_kernel", C.c_int), ("width_factor", C.c_double), ("channel", C.c_void_p)]
 
 
 class LemStats(C.Structure):
@@ -269,7 +272,7 @@ def stream_power(z_old, z, mask, receiver, dirs, topo, area, K, m, n, dt, dx, th
     zz = np.ascontiguousarray(z, dtype=np.float64).ravel().copy()
     zo = np.ascontiguousarray(z_old, dtype=np.float64).ravel()
     m8 = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
-    p = SpParams(K, m, n, dt, dx, threshold, int(divide_area_by_dx), slope_clamp)
+    p = SpParams(K, m, n, dt, dx, threshold, int(divide_area_by_dx), slope_clamp, 1.0)
     kf = None
     kptr = None
     if K_field is not None:
@@ -311,10 +314,29 @@ def boundary_flats(mask, bc):
 
 def lem_params(dt, K, m, n, U, D, Sc=0.8, nonlinear=False, bc="oooo", threshold=0.0,
                slope_clamp=1e-5, baselevel_fall=0.0, fluvial=True, hillslope=True, uplift=True,
-               rel_tol=1e-8, abs_tol=1e-12):
-    return LemParams(dt, K, m, n, threshold, slope_clamp, U, baselevel_fall, 1, int(fluvial),
-                     int(hillslope), int(uplift), bc.encode(),
-                     hs_params(D, Sc, nonlinear, 0.0, rel_tol, abs_tol))
+               rel_tol=1e-8, abs_tol=1e-12, area_kernel=1, width_factor=1.0, channel=None,
+               divide_area_by_dx=True):
+    """channel: optional uint8 Dirichlet channel mask for the hillslope (kept
+    alive on the returned structure)."""
+    ch = None if channel is None else np.ascontiguousarray(channel, np.uint8).ravel()
+    p = LemParams(dt, K, m, n, threshold, slope_clamp, U, baselevel_fall, int(divide_area_by_dx), int(fluvial),
+                  int(hillslope), int(uplift), bc.encode(),
+                  hs_params(D, Sc, nonlinear, 0.0, rel_tol, abs_tol), int(area_kernel), float(width_factor),
+                  None if ch is None else ch.ctypes.data_as(C.c_void_p))
+    p._channel_ref = ch
+    return p
+
+
+def smooth_area(area, mask, kernel_size, bc="oooo"):
+    """orc_smooth_area: moving-average drainage area (float64)."""
+    a = np.ascontiguousarray(area, np.float64)
+    nr, nc = a.shape
+    out = np.empty_like(a)
+    st = lib().orc_smooth_area(nr, nc, a.ravel(), np.ascontiguousarray(mask, np.uint8).ravel(), int(kernel_size),
+                               bc.encode(), out.ravel())
+    if st != 0:
+        raise OracleError("drainage area moving-average kernel size must be a positive odd number")
+    return out
 
 
 def lem_step(z, mask, dx, params, boundary=None, K_field=None):

@@ -67,7 +67,7 @@ int64_t orc_stream_power_d8(int64_t n, const uint8_t* mask,
         if (zr >= zo) continue;
         const double tol = p->slope_clamp * dx_eff;
         const double Ki = K_field ? K_field[f] : p->K;
-        const double kam = Ki * pow(a, p->m);
+        const double kam = Ki * p->width_factor * pow(a, p->m);
         double zn = zo;
         if (n_is_one) {
             const double spf = kam * (p->dt / dx_eff);
@@ -555,6 +555,71 @@ static int cmp_float(const void* a, const void* b)
     return (x > y) - (x < y);
 }
 
+/* smooth_drainage_area, lsdtt_stream_power_cpu.cpp:34-139 */
+static int wrap_or_none(int i, int size, int periodic_neg, int periodic_pos)
+{
+    if (i < 0 && !periodic_neg) return -1;
+    if (i >= size && !periodic_pos) return -1;
+    return (i % size + size) % size;
+}
+
+int orc_smooth_area(int nr, int nc, const double* area, const uint8_t* mask,
+                    int kernel_size, const char* bc, double* out)
+{
+    if (kernel_size < 1 || kernel_size % 2 == 0) return ORC_ERR_ARG;
+    const int64_t n = (int64_t)nr * nc;
+    if (kernel_size == 1) {
+        memcpy(out, area, sizeof(double) * (size_t)n);
+        return ORC_OK;
+    }
+    const int rad = kernel_size / 2;
+    const int pn = bc[0] == 'p' || bc[0] == 'P', pe = bc[1] == 'p' || bc[1] == 'P';
+    const int ps = bc[2] == 'p' || bc[2] == 'P', pw = bc[3] == 'p' || bc[3] == 'P';
+    double* hs = (double*)calloc((size_t)n, sizeof(double));
+    int64_t* hc = (int64_t*)calloc((size_t)n, sizeof(int64_t));
+    for (int r = 0; r < nr; ++r) {
+        double sum = 0.0;
+        int64_t cnt = 0;
+        for (int o = -rad; o <= rad; ++o) {
+            const int c = wrap_or_none(o, nc, pw, pe);
+            if (c >= 0 && mask[(int64_t)r * nc + c]) { sum += area[(int64_t)r * nc + c]; ++cnt; }
+        }
+        for (int c = 0; c < nc; ++c) {
+            const int64_t f = (int64_t)r * nc + c;
+            hs[f] = sum;
+            hc[f] = cnt;
+            if (c + 1 < nc) {
+                const int cm = wrap_or_none(c - rad, nc, pw, pe);
+                if (cm >= 0 && mask[(int64_t)r * nc + cm]) { sum += -1.0 * area[(int64_t)r * nc + cm]; --cnt; }
+                const int cp = wrap_or_none(c + rad + 1, nc, pw, pe);
+                if (cp >= 0 && mask[(int64_t)r * nc + cp]) { sum += area[(int64_t)r * nc + cp]; ++cnt; }
+            }
+        }
+    }
+    for (int64_t i = 0; i < n; ++i) out[i] = 0.0;
+    for (int c = 0; c < nc; ++c) {
+        double sum = 0.0;
+        int64_t cnt = 0;
+        for (int o = -rad; o <= rad; ++o) {
+            const int r = wrap_or_none(o, nr, pn, ps);
+            if (r >= 0) { sum += hs[(int64_t)r * nc + c]; cnt += hc[(int64_t)r * nc + c]; }
+        }
+        for (int r = 0; r < nr; ++r) {
+            const int64_t f = (int64_t)r * nc + c;
+            if (mask[f] && cnt != 0) out[f] = sum / (double)cnt;
+            if (r + 1 < nr) {
+                const int rm = wrap_or_none(r - rad, nr, pn, ps);
+                if (rm >= 0) { sum += -1.0 * hs[(int64_t)rm * nc + c]; cnt -= hc[(int64_t)rm * nc + c]; }
+                const int rp = wrap_or_none(r + rad + 1, nr, pn, ps);
+                if (rp >= 0) { sum += hs[(int64_t)rp * nc + c]; cnt += hc[(int64_t)rp * nc + c]; }
+            }
+        }
+    }
+    free(hs);
+    free(hc);
+    return ORC_OK;
+}
+
 int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
                  const orc_lem_params* p, const int64_t* boundary_flats,
                  int64_t n_bf, const double* K_field, double* z,
@@ -581,15 +646,21 @@ int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
         const int64_t nt = orc_topo_kahn(nr, nc, mask, rcv, don, topo);
         orc_upstream_counts(n, mask, rcv, topo, nt, cnt);
         for (int64_t i = 0; i < n; ++i) area[i] = (double)cnt[i] * (dx * dx);
+        if (p->area_kernel > 1) {
+            double* sm = (double*)malloc(sizeof(double) * (size_t)n);
+            if (orc_smooth_area(nr, nc, area, mask, p->area_kernel, p->bc, sm) != ORC_OK) status = ORC_ERR_ARG;
+            memcpy(area, sm, sizeof(double) * (size_t)n);
+            free(sm);
+        }
         orc_sp_params sp = {p->K, p->m, p->n, p->dt, dx, p->threshold,
-                            p->divide_area_by_dx, p->slope_clamp};
+                            p->divide_area_by_dx, p->slope_clamp, p->width_factor};
         st->cells_eroded = orc_stream_power_d8(n, mask, rcv, dir, topo, nt, area, K_field, &sp, z_before, z);
         free(rcv); free(dir); free(don); free(outl); free(topo); free(cnt); free(area);
     }
 
     if (p->enable_hillslope) {
         orc_hs_result hr;
-        if (orc_hillslope_transient(nr, nc, dx, mask, p->bc, NULL, &p->hs, p->dt, p->dt, z, &hr) != 0)
+        if (orc_hillslope_transient(nr, nc, dx, mask, p->bc, p->channel, &p->hs, p->dt, p->dt, z, &hr) != 0)
             status = ORC_ERR_CYCLIC + 10;   /* not converged */
         st->newton_iterations = hr.iterations;
         st->newton_substeps = hr.substeps;
