@@ -193,6 +193,19 @@ typedef struct {
      * graph of the reference (lsdtt_lem_cpu.cpp:137-154), which this
      * library does not build: such a step returns LSDTT_GPU_ERR_ARG. */
     int32_t hillslope_coupling_regime;
+    /* hillslope_force_picard: skip Newton; otherwise Picard runs only when
+     * Newton fails.  roering_implicit_opts of the damped Picard fallback
+     * (lsdtt_hillslope_implicit_cpu.hpp:65-97; D, Sc and the law come from
+     * the fields above, U is 0): */
+    int32_t hillslope_force_picard;
+    double picard_tol;               /* 1e-4 m */
+    int32_t picard_max_iter;         /* 50 */
+    double relaxation_omega;         /* 1 */
+    double picard_slope_ratio_cap;   /* r_max, 0.99 */
+    double picard_dt_init, picard_dt_min, picard_dt_max;  /* 0 = auto */
+    double picard_grow_factor;       /* 1.5 */
+    double picard_shrink_factor;     /* 0.5 */
+    double picard_dz_target;         /* 0 = off */
 } lsdtt_gpu_lem_options;
 
 typedef struct {
@@ -217,6 +230,13 @@ typedef struct {
     int32_t stream_power_passes;      /* tile passes of the stream power */
     int64_t count_tile_runs;          /* 32 x 32 tiles processed by those passes */
     int64_t stream_power_tile_runs;
+    int32_t hillslope_method;         /* 0 Newton, 1 damped Picard */
+    int32_t picard_iterations;
+    int32_t picard_substeps;
+    int32_t picard_rejected;
+    int32_t picard_cg_iterations;
+    int32_t picard_limiter_fired;
+    double picard_max_slope_ratio;
 } lsdtt_gpu_lem_stats;
 
 /* Defaults of lsdtt_lem_options, plus the Newton defaults above. */
@@ -240,8 +260,9 @@ int lsdtt_gpu_lem_set_channel_mask(lsdtt_gpu_lem* lem, const uint8_t* channel);
 double* lsdtt_gpu_lem_elevation_device(lsdtt_gpu_lem* lem);
 
 /* One step; boundary_flats as from build_boundary_flats (uploaded when it
- * changes).  LSDTT_GPU_ERR_NOT_CONVERGED: the hillslope Newton failed at
- * the minimum sub-step (the reference would fall back to Picard). */
+ * changes).  When the hillslope Newton fails at its minimum sub-step the
+ * step falls back to the damped Picard solver, as the reference does
+ * (stats.hillslope_method = 1). */
 int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary,
                        lsdtt_gpu_lem_stats* stats);
 

@@ -19,6 +19,11 @@ class LemOptions(C.Structure):
         ("krylov_max_iterations", C.c_int32),
         ("drainage_area_kernel_size", C.c_int32), ("multiply_by_w_over_dx", C.c_int32),
         ("channel_width_w", C.c_double), ("hillslope_coupling_regime", C.c_int32),
+        ("hillslope_force_picard", C.c_int32), ("picard_tol", C.c_double), ("picard_max_iter", C.c_int32),
+        ("relaxation_omega", C.c_double), ("picard_slope_ratio_cap", C.c_double),
+        ("picard_dt_init", C.c_double), ("picard_dt_min", C.c_double), ("picard_dt_max", C.c_double),
+        ("picard_grow_factor", C.c_double), ("picard_shrink_factor", C.c_double),
+        ("picard_dz_target", C.c_double),
     ]
 
 
@@ -35,6 +40,9 @@ class LemStats(C.Structure):
         ("hillslope_tolerance", C.c_double), ("max_slope_ratio", C.c_double),
         ("count_passes", C.c_int32), ("stream_power_passes", C.c_int32),
         ("count_tile_runs", C.c_int64), ("stream_power_tile_runs", C.c_int64),
+        ("hillslope_method", C.c_int32), ("picard_iterations", C.c_int32), ("picard_substeps", C.c_int32),
+        ("picard_rejected", C.c_int32), ("picard_cg_iterations", C.c_int32),
+        ("picard_limiter_fired", C.c_int32), ("picard_max_slope_ratio", C.c_double),
     ]
 
     def as_dict(self):

@@ -59,6 +59,17 @@ LemParams to_params(const lsdtt_gpu_lem_options& o, double o_cell_size)
     // lsdtt_stream_power_cpu.cpp:181-183
     p.width_factor = o.multiply_by_w_over_dx ? o.channel_width_w / o_cell_size : 1.0;
     p.regime = o.hillslope_coupling_regime;
+    p.force_picard = o.hillslope_force_picard;
+    p.picard.picard_tol = o.picard_tol;
+    p.picard.picard_max_iter = o.picard_max_iter;
+    p.picard.omega = o.relaxation_omega;
+    p.picard.r_max = o.picard_slope_ratio_cap;
+    p.picard.dt_init = o.picard_dt_init;
+    p.picard.dt_min = o.picard_dt_min;
+    p.picard.dt_max = o.picard_dt_max;
+    p.picard.grow = o.picard_grow_factor;
+    p.picard.shrink = o.picard_shrink_factor;
+    p.picard.dz_target = o.picard_dz_target;
     return p;
 }
 
@@ -99,6 +110,18 @@ void lsdtt_gpu_lem_options_init(lsdtt_gpu_lem_options* o)
     o->multiply_by_w_over_dx = 0;
     o->channel_width_w = 1.0;
     o->hillslope_coupling_regime = 0;
+    // lsdtt_roering_implicit_options defaults (lsdtt_hillslope_implicit_cpu.hpp:65-97)
+    o->hillslope_force_picard = 0;
+    o->picard_tol = 1e-4;
+    o->picard_max_iter = 50;
+    o->relaxation_omega = 1.0;
+    o->picard_slope_ratio_cap = 0.99;
+    o->picard_dt_init = 0.0;
+    o->picard_dt_min = 0.0;
+    o->picard_dt_max = 0.0;
+    o->picard_grow_factor = 1.5;
+    o->picard_shrink_factor = 0.5;
+    o->picard_dz_target = 0.0;
 }
 
 lsdtt_gpu_lem* lsdtt_gpu_lem_create(int32_t n_rows, int32_t n_cols, double cell_size, const uint8_t* mask,
@@ -213,6 +236,13 @@ int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_
         out->max_slope_ratio = st.max_slope_ratio;
         out->count_passes = st.count_passes;
         out->count_tile_runs = st.count_tile_runs;
+        out->hillslope_method = st.hillslope_method;
+        out->picard_iterations = st.picard.picard_iters;
+        out->picard_substeps = st.picard.substeps;
+        out->picard_rejected = st.picard.rejected;
+        out->picard_cg_iterations = st.picard.cg_iters;
+        out->picard_limiter_fired = st.picard.limiter_fired;
+        out->picard_max_slope_ratio = st.picard.max_slope_ratio;
         out->stream_power_tile_runs = st.stream_power_tile_runs;
         out->stream_power_passes = st.stream_power_passes;
         if (rc != 0) {

@@ -42,12 +42,14 @@ constexpr int kScalCount = HS_NSCAL + 8;
 
 inline int grid_for(long long n, int cap) { return std::max(1, std::min(cap, div_up(n, kB))); }
 
-__global__ void k_copy2(const double* __restrict__ a, double* __restrict__ b, double* __restrict__ c, long long n)
+__global__ void k_copy3(const double* __restrict__ a, double* __restrict__ b, double* __restrict__ c,
+                        double* __restrict__ d, long long n)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         const double v = a[i];
         b[i] = v;
         c[i] = v;
+        d[i] = v;
     }
 }
 
@@ -459,6 +461,7 @@ int LemEngine::hillslope(LemStats* st, hipStream_t s)
     for (auto* bf : {&dinv, &b, &x, &r, &pv, &phat, &v, &sv, &shat, &t}) bf->reserve(nn);
     zn_hs.reserve(nn);
     z_trial.reserve(nn);
+    z_hs0.reserve(nn);
     const double total = p.dt;
     double dt = total;
     const double dt_min = total * 1e-9, dt_max = total;
@@ -466,8 +469,9 @@ int LemEngine::hillslope(LemStats* st, hipStream_t s)
     const int g = grid_for(n, 4096);
     // zn_hs: the state at the start of the sub-step (backward Euler's z^n,
     // and what a rejected sub-step restores); z_trial: the line search's
-    // trial points, equal to z off the DOFs
-    k_copy2<<<g, kB, 0, s>>>(z.p, zn_hs.p, z_trial.p, n);
+    // trial points, equal to z off the DOFs; z_hs0: the post-incision state
+    // the Picard fallback restarts from
+    k_copy3<<<g, kB, 0, s>>>(z.p, zn_hs.p, z_trial.p, z_hs0.p, n);
     LSDTT_LAUNCH_CHECK();
     zc = z.p;
     zt = z_trial.p;
@@ -568,7 +572,30 @@ int LemEngine::step(LemStats* st, hipStream_t s)
         st->stream_power_tile_runs = last_sp_tiles;
     }
     int rc = 0;
-    if (p.enable_hillslope) rc = hillslope(st, s);
+    if (p.enable_hillslope) {
+        // Newton first; damped Picard when it fails (from the post-incision
+        // surface, the Newton result discarded) or when it is forced
+        // (lsdtt_lem_cpu.cpp:531-582)
+        bool done = false;
+        if (!p.force_picard) {
+            done = hillslope(st, s) == 0;
+            if (!done) {
+                k_copy<<<g, kB, 0, s>>>(z_hs0.p, z.p, n);
+                LSDTT_LAUNCH_CHECK();
+            }
+        }
+        if (!done) {
+            PicardOptions po = p.picard;
+            po.D = p.D;
+            po.U = 0.0;
+            po.Sc = p.Sc;
+            po.nonlinear = p.nonlinear;
+            timer.begin("picard_hillslope", s);
+            picard.run(nr, nc, p.bc, dx, mask.p, dof.p, po, p.dt, z.p, &st->picard, s);
+            timer.end(s, 0.0);
+            st->hillslope_method = 1;
+        }
+    }
     if (p.enable_uplift) {
         k_uplift<<<g, kB, 0, s>>>(mask.p, z.p, p.U * p.dt, n);
         LSDTT_LAUNCH_CHECK();

@@ -24,6 +24,7 @@
 #include "fluvial.hpp"
 #include "hillslope.hpp"
 #include "ktimer.hpp"
+#include "picard.hpp"
 
 namespace lsdtt_gpu {
 
@@ -43,6 +44,8 @@ struct LemParams {
     int area_kernel = 1;        // drainage-area moving average (odd; 1 = none)
     double width_factor = 1.0;  // w / dx or 1
     int regime = 0;             // 0 automatic, 1 coupled, 2 channels_as_boundaries
+    int force_picard = 0;       // hillslope_force_picard
+    PicardOptions picard;       // fallback settings (D, Sc, nonlinear from above; U = 0)
 };
 
 struct LemStats {
@@ -55,6 +58,8 @@ struct LemStats {
     double max_slope_ratio = 0;
     int count_passes = 0, stream_power_passes = 0;
     long long count_tile_runs = 0, stream_power_tile_runs = 0;
+    int hillslope_method = 0;  // 0 Newton, 1 Picard
+    PicardStats picard;
 };
 
 class LemEngine {
@@ -95,7 +100,8 @@ public:
     bool has_K = false;
 
     // hillslope work
-    dbuf<double> zn_hs, z_trial;
+    dbuf<double> zn_hs, z_trial, z_hs0;  // z_hs0: post-incision state (Picard fallback)
+    PicardSolver picard;
     // hillslope iterate and trial point: z / z_trial, swapped on acceptance
     double* zc = nullptr;
     double* zt = nullptr;

@@ -151,6 +151,28 @@ int orc_hillslope_transient(int nr, int nc, double h, const uint8_t* mask,
                             const orc_hs_params* p, double dt_total,
                             double dt_init, double* z, orc_hs_result* total);
 
+/* ---- Picard fallback (oracle/orc_picard.c) ------------------------------ */
+
+typedef struct {
+    double D, U, Sc;
+    int    nonlinear;
+    double picard_tol;         /* 1e-4 m */
+    int    picard_max_iter;    /* 50 */
+    double relaxation_omega;   /* 1 */
+    double slope_ratio_cap;    /* r_max, 0.99 */
+    double dt_init, dt_min, dt_max;  /* 0 = auto */
+    double grow_factor, shrink_factor, dz_target;  /* 1.5, 0.5, 0 */
+} orc_picard_params;
+
+typedef struct {
+    int substeps, rejected, picard_iters, limiter_fired;
+    double final_dt, max_slope_ratio;
+} orc_picard_result;
+
+int orc_picard_transient(int nr, int nc, double h, const uint8_t* mask, const char* bc,
+                         const uint8_t* is_channel, const orc_picard_params* p, double dt_total, double* z,
+                         orc_picard_result* res);
+
 typedef struct {
     double dt, K, m, n, threshold, slope_clamp, U, baselevel_fall;
     int    divide_area_by_dx;
@@ -160,6 +182,8 @@ typedef struct {
     int    area_kernel;            /* drainage_area_moving_average_kernel_size_pixels (1) */
     double width_factor;           /* w / dx or 1 */
     const uint8_t* channel;        /* explicit Dirichlet channel set for the hillslope, or NULL */
+    int    force_picard;           /* hillslope_force_picard */
+    orc_picard_params picard;      /* the fallback's settings (D, Sc, nonlinear taken from hs) */
 } orc_lem_params;
 
 typedef struct {
@@ -168,6 +192,8 @@ typedef struct {
     int64_t n_valid, cells_eroded;
     int newton_iterations, newton_substeps;
     double hillslope_residual, hillslope_initial_residual;
+    int hillslope_method;          /* 0 Newton, 1 Picard */
+    int picard_iterations, picard_substeps;
 } orc_lem_stats;
 
 int64_t orc_boundary_flats(int nr, int nc, const uint8_t* mask, const char* bc,

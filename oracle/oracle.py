@@ -71,6 +71,9 @@ def lib():
         L.orc_lem_step.argtypes = [C.c_int, C.c_int, C.c_double, u8p, C.POINTER(LemParams), i64p,
                                    C.c_int64, C.c_void_p, f64p, C.POINTER(LemStats)]
         L.orc_lem_step.restype = C.c_int
+        L.orc_picard_transient.argtypes = [C.c_int, C.c_int, C.c_double, u8p, C.c_char_p, C.c_void_p,
+                                           C.POINTER(PicardParams), C.c_double, f64p, C.POINTER(PicardResult)]
+        L.orc_picard_transient.restype = C.c_int
         L.orc_smooth_area.argtypes = [C.c_int, C.c_int, f64p, u8p, C.c_int, C.c_char_p, f64p]
         L.orc_smooth_area.restype = C.c_int
         L.orc_chi_raster.argtypes = [C.c_int, C.c_int, C.c_double, u8p, i64p, u8p, i64p, i64p, u64p,
@@ -239,13 +242,48 @@ class HsResult(C.Structure):
                 ("final_max_dz", C.c_double), ("max_slope_ratio", C.c_double)]
 
 
+class PicardParams(C.Structure):
+    _fields_ = [("D", C.c_double), ("U", C.c_double), ("Sc", C.c_double), ("nonlinear", C.c_int),
+                ("picard_tol", C.c_double), ("picard_max_iter", C.c_int), ("relaxation_omega", C.c_double),
+                ("slope_ratio_cap", C.c_double), ("dt_init", C.c_double), ("dt_min", C.c_double),
+                ("dt_max", C.c_double), ("grow_factor", C.c_double), ("shrink_factor", C.c_double),
+                ("dz_target", C.c_double)]
+
+
+class PicardResult(C.Structure):
+    _fields_ = [("substeps", C.c_int), ("rejected", C.c_int), ("picard_iters", C.c_int),
+                ("limiter_fired", C.c_int), ("final_dt", C.c_double), ("max_slope_ratio", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def picard_params(D, Sc=0.8, nonlinear=False, U=0.0, picard_tol=1e-4, picard_max_iter=50, omega=1.0,
+                  r_max=0.99, dt_init=0.0, dt_min=0.0, dt_max=0.0, grow=1.5, shrink=0.5, dz_target=0.0):
+    return PicardParams(D, U, Sc, int(nonlinear), picard_tol, picard_max_iter, omega, r_max, dt_init, dt_min,
+                        dt_max, grow, shrink, dz_target)
+
+
+def picard_transient(z, mask, dx, bc, params, dt_total, channel=None):
+    """orc_picard_transient on a copy of z: (z_new, result dict)"""
+    zz = np.ascontiguousarray(z, np.float64).copy()
+    nr, nc = zz.shape
+    ch = None if channel is None else np.ascontiguousarray(channel, np.uint8).ravel()
+    res = PicardResult()
+    lib().orc_picard_transient(nr, nc, float(dx), np.ascontiguousarray(mask, np.uint8).ravel(), bc.encode(),
+                               None if ch is None else ch.ctypes.data_as(C.c_void_p), C.byref(params),
+                               float(dt_total), zz.ravel(), C.byref(res))
+    return zz, res.as_dict()
+
+
 class LemParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("K", C.c_double), ("m", C.c_double), ("n", C.c_double),
                 ("threshold", C.c_double), ("slope_clamp", C.c_double), ("U", C.c_double),
                 ("baselevel_fall", C.c_double), ("divide_area_by_dx", C.c_int),
                 ("enable_fluvial", C.c_int), ("enable_hillslope", C.c_int),
                 ("enable_uplift", C.c_int), ("bc", C.c_char * 4), ("hs", HsParams),
-                ("area_kernel", C.c_int), ("width_factor", C.c_double), ("channel", C.c_void_p)]
+                ("area_kernel", C.c_int), ("width_factor", C.c_double), ("channel", C.c_void_p),
+                ("force_picard", C.c_int), ("picard", PicardParams)]
 
 
 class LemStats(C.Structure):
@@ -255,7 +293,8 @@ class LemStats(C.Structure):
                 ("max_erosion_rate", C.c_double), ("n_valid", C.c_int64),
                 ("cells_eroded", C.c_int64), ("newton_iterations", C.c_int),
                 ("newton_substeps", C.c_int), ("hillslope_residual", C.c_double),
-                ("hillslope_initial_residual", C.c_double)]
+                ("hillslope_initial_residual", C.c_double), ("hillslope_method", C.c_int),
+                ("picard_iterations", C.c_int), ("picard_substeps", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -315,14 +354,16 @@ def boundary_flats(mask, bc):
 def lem_params(dt, K, m, n, U, D, Sc=0.8, nonlinear=False, bc="oooo", threshold=0.0,
                slope_clamp=1e-5, baselevel_fall=0.0, fluvial=True, hillslope=True, uplift=True,
                rel_tol=1e-8, abs_tol=1e-12, area_kernel=1, width_factor=1.0, channel=None,
-               divide_area_by_dx=True):
+               divide_area_by_dx=True, force_picard=False, picard=None, max_iterations=50):
     """channel: optional uint8 Dirichlet channel mask for the hillslope (kept
     alive on the returned structure)."""
     ch = None if channel is None else np.ascontiguousarray(channel, np.uint8).ravel()
     p = LemParams(dt, K, m, n, threshold, slope_clamp, U, baselevel_fall, int(divide_area_by_dx), int(fluvial),
                   int(hillslope), int(uplift), bc.encode(),
                   hs_params(D, Sc, nonlinear, 0.0, rel_tol, abs_tol), int(area_kernel), float(width_factor),
-                  None if ch is None else ch.ctypes.data_as(C.c_void_p))
+                  None if ch is None else ch.ctypes.data_as(C.c_void_p), int(force_picard),
+                  picard if picard is not None else picard_params(D, Sc, nonlinear))
+    p.hs.max_iterations = int(max_iterations)
     p._channel_ref = ch
     return p
 

@@ -659,13 +659,33 @@ int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
     }
 
     if (p->enable_hillslope) {
-        orc_hs_result hr;
-        if (orc_hillslope_transient(nr, nc, dx, mask, p->bc, p->channel, &p->hs, p->dt, p->dt, z, &hr) != 0)
-            status = ORC_ERR_CYCLIC + 10;   /* not converged */
-        st->newton_iterations = hr.iterations;
-        st->newton_substeps = hr.substeps;
-        st->hillslope_residual = hr.final_residual;
-        st->hillslope_initial_residual = hr.initial_residual;
+        /* Newton first, Picard when it fails or is forced (lsdtt_lem_cpu.cpp:531-582) */
+        int done = 0;
+        if (!p->force_picard) {
+            double* keep = (double*)malloc(sizeof(double) * (size_t)n);
+            memcpy(keep, z, sizeof(double) * (size_t)n);
+            orc_hs_result hr;
+            const int rc = orc_hillslope_transient(nr, nc, dx, mask, p->bc, p->channel, &p->hs, p->dt, p->dt, z, &hr);
+            st->newton_iterations = hr.iterations;
+            st->newton_substeps = hr.substeps;
+            st->hillslope_residual = hr.final_residual;
+            st->hillslope_initial_residual = hr.initial_residual;
+            if (rc == 0) done = 1;
+            else memcpy(z, keep, sizeof(double) * (size_t)n);   /* the Newton result is discarded */
+            free(keep);
+        }
+        if (!done) {
+            orc_picard_params pp = p->picard;
+            pp.D = p->hs.D;
+            pp.U = 0.0;
+            pp.Sc = p->hs.Sc;
+            pp.nonlinear = p->hs.nonlinear;
+            orc_picard_result pr;
+            orc_picard_transient(nr, nc, dx, mask, p->bc, p->channel, &pp, p->dt, z, &pr);
+            st->hillslope_method = 1;
+            st->picard_iterations = pr.picard_iters;
+            st->picard_substeps = pr.substeps;
+        }
     }
 
     if (p->enable_uplift) {

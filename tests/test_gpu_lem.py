@@ -267,3 +267,45 @@ def test_gpu_lem_option_errors(gpu):
     with pytest.raises(GpuError) as e:
         lem.step()
     assert e.value.status == 1 and "D-infinity" in str(e.value)
+
+
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_gpu_forced_picard_matches_oracle(gpu, nonlinear):
+    """hillslope_force_picard: damped Picard (FP64 CG solves to 1e-14) on
+    both sides; tolerance: both stop at max |dz| < picard_tol = 1e-9 m
+    between iterates of a contraction, so each is within ~1e-9 of the
+    same fixed point"""
+    print(f"[gpu lem picard] nonlinear={nonlinear}", flush=True)
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(72, 88, dx=30.0, seed=51, max_slope=0.6)
+    mask = np.ones(z.shape, np.uint8)
+    mask[30:33, 40:44] = 0
+    kw = dict(timestep_years=500.0, erodibility=3e-5, m_exponent=0.5, n_exponent=1.5, uplift_rate=5e-4,
+              hillslope_diffusivity=0.005, nonlinear_hillslopes=nonlinear, critical_hillslope_slope=0.8,
+              boundary_conditions="nnon", hillslope_force_picard=1, picard_tol=1e-9, picard_max_iter=200)
+    pp = orc.picard_params(0.005, Sc=0.8, nonlinear=nonlinear, picard_tol=1e-9, picard_max_iter=200)
+    okw = dict(dt=500.0, K=3e-5, m=0.5, n=1.5, U=5e-4, D=0.005, Sc=0.8, nonlinear=nonlinear, bc="nnon",
+               force_picard=True, picard=pp)
+    zg, zo, sg, so = run_both(gpu, z, 30.0, 2, kw, okw, mask=mask)
+    assert sg["hillslope_method"] == 1 and so["hillslope_method"] == 1
+    assert sg["picard_cg_iterations"] > 0
+    assert sg["picard_substeps"] == so["picard_substeps"]
+    assert np.abs(zg - zo)[mask != 0].max() <= 1e-7
+
+
+def test_gpu_newton_failure_falls_back_to_picard(gpu):
+    """one Newton iteration allowed: every Newton sub-step fails down to the
+    minimum sub-step, the step restarts from the post-incision surface with
+    Picard, on the device as in the oracle (lsdtt_lem_cpu.cpp:563-582)"""
+    print("[gpu lem newton->picard]", flush=True)
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(40, 48, dx=30.0, seed=52, max_slope=0.6)
+    kw = dict(timestep_years=500.0, erodibility=3e-5, m_exponent=0.5, n_exponent=1.5, uplift_rate=5e-4,
+              hillslope_diffusivity=0.005, nonlinear_hillslopes=True, critical_hillslope_slope=0.8,
+              boundary_conditions="nnon", newton_max_iterations=1, picard_tol=1e-9, picard_max_iter=200)
+    pp = orc.picard_params(0.005, Sc=0.8, nonlinear=True, picard_tol=1e-9, picard_max_iter=200)
+    okw = dict(dt=500.0, K=3e-5, m=0.5, n=1.5, U=5e-4, D=0.005, Sc=0.8, nonlinear=True, bc="nnon",
+               picard=pp, max_iterations=1)
+    zg, zo, sg, so = run_both(gpu, z, 30.0, 1, kw, okw)
+    assert sg["hillslope_method"] == 1 and so["hillslope_method"] == 1
+    assert np.abs(zg - zo).max() <= 1e-7
