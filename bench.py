@@ -180,6 +180,56 @@ def traffic_for(group, size):
     return None
 
 
+def ensemble_member(args):
+    """one instance of the 64-member ensemble: seeded size, surface and
+    parameters (K log-uniform 1e-6..1e-4, m/n uniform 0.3..0.6 with n = 1,
+    D log-uniform 1e-3..1e-1 (Roering, Sc = 0.8), U log-uniform 1e-4..1e-3)"""
+    i, seed = args
+    from lsdtt3_amd.synthetic import lem_surface
+    rng = np.random.default_rng(seed * 1000 + i)
+    cells = float(np.exp(rng.uniform(np.log(256.0 ** 2), np.log(2048.0 ** 2))))
+    aspect = float(rng.uniform(0.5, 2.0))
+    nr = max(32, int(round(np.sqrt(cells * aspect))))
+    nc = max(32, int(round(cells / nr)))
+    logu = lambda a, b: float(np.exp(rng.uniform(np.log(a), np.log(b))))  # noqa: E731
+    K, theta, D, U = logu(1e-6, 1e-4), float(rng.uniform(0.3, 0.6)), logu(1e-3, 1e-1), logu(1e-4, 1e-3)
+    z = lem_surface(nr, nc, dx=30.0, seed=seed * 1000 + i, max_slope=0.75 * 0.8)
+    return dict(dem=z, cell_size=30.0, timestep_years=500.0, erodibility=K, m_exponent=theta, n_exponent=1.0,
+                uplift_rate=U, hillslope_diffusivity=D, nonlinear_hillslopes=True, critical_hillslope_slope=0.8,
+                boundary_conditions="nnon", newton_rel_tol=1e-8)
+
+
+def bench_ensemble(args):
+    import lsdtt3_amd
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=8) as ex:
+        ms = list(ex.map(ensemble_member, [(i, 2024) for i in range(args.instances)]))
+    cells = [m["dem"].size for m in ms]
+    ens = lsdtt3_amd.Ensemble(ms, workers=args.workers)
+    ens.run(args.warmup)
+    stats, sec = ens.run(args.steps)
+    total = sum(ens.n_valid)
+    value = total * args.steps / sec
+    worst = max((s["hillslope_residual"] / s["hillslope_initial_residual"]) if s["hillslope_initial_residual"] > 0
+                else 0.0 for s in stats)
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(1e3 * sec / args.steps, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"batched ensemble of {len(ms)} LEM instances, cells log-uniform "
+                                  f"256^2..2048^2 (total {total}), per-instance K, m/n, D, U; Roering Newton "
+                                  f"r<=1e-8, stream power n=1; {args.workers} host workers, one stream each",
+                      "instances": len(ms), "total_cells": total, "min_cells": min(cells), "max_cells": max(cells),
+                      "worst_relative_residual": worst,
+                      "picard_fallbacks": sum(1 for s in stats if s["hillslope_method"] == 1)}}
+    if args.sequential:
+        seq = lsdtt3_amd.Ensemble(ms, workers=1)
+        seq.run(args.warmup)
+        _, s1 = seq.run(args.steps)
+        out["sequential_one_stream"] = {"value": round(total * args.steps / s1, 1), "unit": "cell-steps/s",
+                                        "seconds": round(s1, 3)}
+    print(json.dumps(out))
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-trace":
         summarize_trace(sys.argv[2], sys.argv[3])
@@ -194,6 +244,10 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--workload", choices=["lem4096", "ensemble"], default="lem4096")
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--sequential", action="store_true", help="ensemble: also time one instance at a time")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
@@ -204,6 +258,9 @@ def main():
     if not lsdtt3_amd.gpu_available():
         print("no GPU visible to liblsdtt3_gpu.so", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "ensemble":
+        bench_ensemble(args)
+        return
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)

@@ -266,6 +266,28 @@ double* lsdtt_gpu_lem_elevation_device(lsdtt_gpu_lem* lem);
 int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary,
                        lsdtt_gpu_lem_stats* stats);
 
+/* ---------------------------------------------------------------------
+ * Batched ensemble: n independent LEM instances (uneven grids, per-instance
+ * options) stepped concurrently on one GPU.  No counterpart in the
+ * reference, which runs one model per process (cli/lsdtt-lem.cpp:3553-3700);
+ * each instance's step is run_lem_step as above.  masks[i] and options[i]
+ * describe instance i; boundary flats come from its boundary codes;
+ * n_workers host threads drive the instances (<= 0: 16).
+ * ------------------------------------------------------------------- */
+typedef struct lsdtt_gpu_ensemble lsdtt_gpu_ensemble;
+
+lsdtt_gpu_ensemble* lsdtt_gpu_ensemble_create(int32_t n_instances, const int32_t* n_rows, const int32_t* n_cols,
+                                              const double* cell_size, const uint8_t* const* masks,
+                                              const lsdtt_gpu_lem_options* options, int32_t n_workers);
+void lsdtt_gpu_ensemble_destroy(lsdtt_gpu_ensemble* e);
+int lsdtt_gpu_ensemble_set_elevation(lsdtt_gpu_ensemble* e, int32_t instance, const double* z);
+int lsdtt_gpu_ensemble_get_elevation(lsdtt_gpu_ensemble* e, int32_t instance, double* z);
+/* n_steps steps of every instance; last_stats (n_instances entries, may be
+ * NULL) receives each instance's last step; *seconds the wall time from
+ * the first launch to the last instance's completion. */
+int lsdtt_gpu_ensemble_run(lsdtt_gpu_ensemble* e, int32_t n_steps, lsdtt_gpu_lem_stats* last_stats,
+                           double* seconds);
+
 /* Per-kernel timing with HIP events recorded on the step's stream (no
  * counterpart in the reference; for roofline measurement).  enable != 0
  * starts recording (and clears previous totals); _get returns, for kernel

@@ -5,6 +5,6 @@ lib/liblsdtt3_gpu.so); this package is a thin Python front end over that ABI
 that mirrors the reference's Python names (FlowInfo, LandscapeModel).
 """
 from .flow import FlowInfo, fill_pits, gpu_available  # noqa: F401
-from .lem import LandscapeModel, build_boundary_flats  # noqa: F401
+from .lem import Ensemble, LandscapeModel, build_boundary_flats, make_options  # noqa: F401
 
 __all__ = ["FlowInfo", "fill_pits", "gpu_available", "LandscapeModel", "build_boundary_flats"]

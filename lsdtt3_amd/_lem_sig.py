@@ -58,6 +58,11 @@ def declare(L, sig):
     sig(L, "lsdtt_gpu_lem_get_elevation", C.c_int, [vp, vp])
     sig(L, "lsdtt_gpu_lem_set_erodibility_field", C.c_int, [vp, vp])
     sig(L, "lsdtt_gpu_lem_set_channel_mask", C.c_int, [vp, vp])
+    sig(L, "lsdtt_gpu_ensemble_create", vp, [C.c_int32, vp, vp, vp, vp, vp, C.c_int32])
+    sig(L, "lsdtt_gpu_ensemble_destroy", None, [vp])
+    sig(L, "lsdtt_gpu_ensemble_set_elevation", C.c_int, [vp, C.c_int32, vp])
+    sig(L, "lsdtt_gpu_ensemble_get_elevation", C.c_int, [vp, C.c_int32, vp])
+    sig(L, "lsdtt_gpu_ensemble_run", C.c_int, [vp, C.c_int32, vp, vp])
     sig(L, "lsdtt_gpu_lem_elevation_device", C.c_void_p, [vp])
     sig(L, "lsdtt_gpu_lem_step", C.c_int, [vp, vp, i64, C.POINTER(LemStats)])
     sig(L, "lsdtt_gpu_build_boundary_flats", C.c_int, [i32, i32, vp, C.c_char_p, vp, vp])

@@ -8,6 +8,7 @@
 
 #include "../../include/lsdtt3_gpu.h"
 #include "capi_util.hpp"
+#include "capi_lem.hpp"
 #include "lem.hpp"
 
 using namespace lsdtt_gpu;
@@ -19,7 +20,7 @@ struct lsdtt_gpu_lem {
     bool z_set = false;
 };
 
-namespace {
+namespace lsdtt_gpu {
 
 bool bc_ok(const char* s)
 {
@@ -73,7 +74,56 @@ LemParams to_params(const lsdtt_gpu_lem_options& o, double o_cell_size)
     return p;
 }
 
-}  // namespace
+void validate_options(const lsdtt_gpu_lem_options& oo)
+{
+    const lsdtt_gpu_lem_options* o = &oo;
+    if (!bc_ok(o->boundary_conditions)) throw std::invalid_argument("boundary conditions must be four of o/n/p/b");
+    if (!(o->timestep > 0.0)) throw std::invalid_argument("timestep must be positive");
+    if (o->enable_hillslope_transport && o->enable_nonlinear_hillslopes && !(o->critical_hillslope_slope > 0.0))
+        throw std::invalid_argument("critical_hillslope_slope must be positive");
+    if (o->drainage_area_kernel_size < 1 || o->drainage_area_kernel_size % 2 == 0)
+        throw std::invalid_argument("drainage area moving-average kernel size must be a positive odd number");
+    if (o->multiply_by_w_over_dx && (!(o->channel_width_w > 0.0) || !std::isfinite(o->channel_width_w)))
+        throw std::invalid_argument(
+            "channel_width_w must be finite and positive when the width correction is enabled");
+    if (o->hillslope_coupling_regime < 0 || o->hillslope_coupling_regime > 2)
+        throw std::invalid_argument("hillslope_coupling_regime must be 0 (auto), 1 (coupled) or 2 "
+                                    "(channels_as_boundaries)");
+}
+
+void to_c_stats(const LemStats& st, lsdtt_gpu_lem_stats* out)
+{
+    out->delta_max_elevation = st.delta_max_elevation;
+    out->delta_mean_elevation = st.delta_mean_elevation;
+    out->max_local_elevation_change = st.max_local_elevation_change;
+    out->p999_local_elevation_change = st.p999_local_elevation_change;
+    out->mean_erosion_rate = st.mean_erosion_rate;
+    out->max_erosion_rate = st.max_erosion_rate;
+    out->n_valid = st.n_valid;
+    out->cells_eroded = st.cells_eroded;
+    out->newton_iterations = st.newton_iterations;
+    out->krylov_iterations = st.krylov_iterations;
+    out->hillslope_substeps = st.substeps;
+    out->hillslope_rejected = st.rejected;
+    out->hillslope_converged = st.hillslope_converged;
+    out->hillslope_initial_residual = st.hillslope_initial_residual;
+    out->hillslope_residual = st.hillslope_residual;
+    out->hillslope_tolerance = st.hillslope_tolerance;
+    out->max_slope_ratio = st.max_slope_ratio;
+    out->count_passes = st.count_passes;
+    out->count_tile_runs = st.count_tile_runs;
+    out->hillslope_method = st.hillslope_method;
+    out->picard_iterations = st.picard.picard_iters;
+    out->picard_substeps = st.picard.substeps;
+    out->picard_rejected = st.picard.rejected;
+    out->picard_cg_iterations = st.picard.cg_iters;
+    out->picard_limiter_fired = st.picard.limiter_fired;
+    out->picard_max_slope_ratio = st.picard.max_slope_ratio;
+    out->stream_power_tile_runs = st.stream_power_tile_runs;
+    out->stream_power_passes = st.stream_power_passes;
+}
+
+}  // namespace lsdtt_gpu
 
 extern "C" {
 
@@ -132,18 +182,7 @@ lsdtt_gpu_lem* lsdtt_gpu_lem_create(int32_t n_rows, int32_t n_cols, double cell_
         if (n_rows <= 0 || n_cols <= 0) throw std::invalid_argument("grid must have at least one row and column");
         if (!(cell_size > 0.0) || !std::isfinite(cell_size))
             throw std::invalid_argument("cell_size must be finite and positive");
-        if (!bc_ok(o->boundary_conditions)) throw std::invalid_argument("boundary conditions must be four of o/n/p/b");
-        if (!(o->timestep > 0.0)) throw std::invalid_argument("timestep must be positive");
-        if (o->enable_hillslope_transport && o->enable_nonlinear_hillslopes && !(o->critical_hillslope_slope > 0.0))
-            throw std::invalid_argument("critical_hillslope_slope must be positive");
-        if (o->drainage_area_kernel_size < 1 || o->drainage_area_kernel_size % 2 == 0)
-            throw std::invalid_argument("drainage area moving-average kernel size must be a positive odd number");
-        if (o->multiply_by_w_over_dx && (!(o->channel_width_w > 0.0) || !std::isfinite(o->channel_width_w)))
-            throw std::invalid_argument(
-                "channel_width_w must be finite and positive when the width correction is enabled");
-        if (o->hillslope_coupling_regime < 0 || o->hillslope_coupling_regime > 2)
-            throw std::invalid_argument("hillslope_coupling_regime must be 0 (auto), 1 (coupled) or 2 "
-                                        "(channels_as_boundaries)");
+        validate_options(*o);
         auto L = std::make_unique<lsdtt_gpu_lem>();
         L->eng = std::make_unique<LemEngine>(n_rows, n_cols, cell_size, mask, to_params(*o, cell_size));
         return L.release();
@@ -217,34 +256,7 @@ int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_
         }
         LemStats st;
         const int rc = lem->eng->step(&st, nullptr);
-        out->delta_max_elevation = st.delta_max_elevation;
-        out->delta_mean_elevation = st.delta_mean_elevation;
-        out->max_local_elevation_change = st.max_local_elevation_change;
-        out->p999_local_elevation_change = st.p999_local_elevation_change;
-        out->mean_erosion_rate = st.mean_erosion_rate;
-        out->max_erosion_rate = st.max_erosion_rate;
-        out->n_valid = st.n_valid;
-        out->cells_eroded = st.cells_eroded;
-        out->newton_iterations = st.newton_iterations;
-        out->krylov_iterations = st.krylov_iterations;
-        out->hillslope_substeps = st.substeps;
-        out->hillslope_rejected = st.rejected;
-        out->hillslope_converged = st.hillslope_converged;
-        out->hillslope_initial_residual = st.hillslope_initial_residual;
-        out->hillslope_residual = st.hillslope_residual;
-        out->hillslope_tolerance = st.hillslope_tolerance;
-        out->max_slope_ratio = st.max_slope_ratio;
-        out->count_passes = st.count_passes;
-        out->count_tile_runs = st.count_tile_runs;
-        out->hillslope_method = st.hillslope_method;
-        out->picard_iterations = st.picard.picard_iters;
-        out->picard_substeps = st.picard.substeps;
-        out->picard_rejected = st.picard.rejected;
-        out->picard_cg_iterations = st.picard.cg_iters;
-        out->picard_limiter_fired = st.picard.limiter_fired;
-        out->picard_max_slope_ratio = st.picard.max_slope_ratio;
-        out->stream_power_tile_runs = st.stream_power_tile_runs;
-        out->stream_power_passes = st.stream_power_passes;
+        to_c_stats(st, out);
         if (rc != 0) {
             set_error("hillslope Newton did not converge at the minimum sub-step");
             return LSDTT_GPU_ERR_NOT_CONVERGED;

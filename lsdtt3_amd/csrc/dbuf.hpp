@@ -33,6 +33,33 @@ struct dbuf {
     operator T*() const { return p; }
 };
 
+// owning pinned host buffer (grow-only): read-backs into it are true
+// asynchronous DMA copies, with no staging through a shared bounce buffer,
+// so engines driven from different host threads do not serialise on them
+template <class T>
+struct hbuf {
+    T* p = nullptr;
+    size_t n = 0;
+
+    hbuf() = default;
+    hbuf(const hbuf&) = delete;
+    hbuf& operator=(const hbuf&) = delete;
+    ~hbuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    void reserve(size_t k)
+    {
+        if (k <= n && p) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        LSDTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), (k ? k : 1) * sizeof(T), hipHostMallocDefault));
+        n = k;
+    }
+    T& operator[](size_t i) { return p[i]; }
+    T* data() { return p; }
+};
+
 // grow-only scratch for library temporaries (scan / sort storage)
 struct dscratch {
     void* p = nullptr;

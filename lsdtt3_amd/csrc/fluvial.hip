@@ -640,7 +640,7 @@ void FluvialTiles::resize(int nr_, int nc_, BC bc)
     lstart.reserve((size_t)ntiles * (TN + 1));
     // per pass: marked flag, tiles run; then the eroded-cell sum (8-aligned)
     flags.reserve(2 * (size_t)max_passes + 4);
-    flags_h.resize(2 * (size_t)max_passes + 4);
+    flags_h.reserve(2 * (size_t)max_passes + 4);
 }
 
 void FluvialTiles::smooth_area(const int* cnt, const uint8_t* mask, double pixel_area, int kernel_size,

@@ -56,7 +56,7 @@ public:
     long long tile_runs = 0;  // tiles processed by the last counts / stream_power
     BC bc_{'o', 'o', 'o', 'o'};
     dbuf<int> act[3], flags, tile_eroded, tile_maxdep;
-    std::vector<int> flags_h;
+    hbuf<int> flags_h;
     KernelTimer* timer = nullptr;  // optional: each pass timed as one launch
     dbuf<short> ord, lstart;  // tile-major level order
     dbuf<uint8_t> xdon;       // per cell: directions of donors in other tiles

@@ -319,16 +319,13 @@ __device__ __forceinline__ float jx_fast(const HsDom& d, const double* __restric
     return acc;
 }
 
-// y = J x on a general row (x is zero on non-DOF cells)
-__device__ __forceinline__ float jx_gen(const HsDom& d, const double* __restrict__ z, const float* __restrict__ x,
-                                        int f)
+// y = J x on general row i (x is zero on non-DOF cells), from the stored row
+__device__ __forceinline__ float jx_gen(const HsDom& d, const float* __restrict__ x, int i, int f)
 {
     const int r = f / d.nc, c = f - r * d.nc;
-    double c9[9];
-    cell_general(d, ZView{z, nullptr, 0.0}, r, c, c9);
     float acc = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) acc += static_cast<float>(c9[k]) * x[slot_cell(d, r, c, k)];
+    for (int k = 0; k < 9; ++k) acc += d.c9g[(size_t)k * d.n_gen + i] * x[slot_cell(d, r, c, k)];
     return acc;
 }
 
@@ -356,7 +353,7 @@ __global__ void __launch_bounds__(256) k_bi_mv_gen(BiArgs a, const float* __rest
     double acc[2] = {0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.d.n_gen; i += gridDim.x * blockDim.x) {
         const int f = a.d.gen[i];
-        const float y = jx_gen(a.d, a.z, in, f);
+        const float y = jx_gen(a.d, in, i, f);
         out[f] = y;
         acc[0] += static_cast<double>(w1[f]) * y;
         if (w2) acc[1] += static_cast<double>(w2[f]) * y;
@@ -484,6 +481,8 @@ __global__ void __launch_bounds__(256) k_trial_gen(HsDom d, const double* __rest
         dinv[f] = c9[4] != 0.0 ? static_cast<float>(1.0 / c9[4]) : 0.0f;
         acc[2] += static_cast<double>(bf) * bf;
         if (lam != 0.0) z_out[f] = Z(f);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) d.c9g[(size_t)k * d.n_gen + i] = static_cast<float>(c9[k]);
     }
     const bool mx[3] = {true, true, false};
     block_partial<3>(acc, mx, partials, base + blockIdx.x);

@@ -31,6 +31,9 @@ struct HsDom {
     int nonlinear;
     const double* zn;
     double inv_h, inv_2h, inv_Sc, inv_Sc2;  // 1/h, 1/(2h), 1/Sc, 1/Sc^2
+    // Jacobian rows of the general DOFs (FP32, 9 x n_gen, slot-major),
+    // written by the trial that produced the current iterate
+    float* c9g;
 };
 
 // fills the reciprocal fields of a domain

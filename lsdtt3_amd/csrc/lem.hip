@@ -128,31 +128,77 @@ __global__ void k_radix_hist(const float* __restrict__ v, long long n, const uns
     for (int i = threadIdx.x; i < kRadixBins; i += blockDim.x) h[i] = 0;
     __syncthreads();
     const unsigned prefix = sel[0], pmask = sel[1];
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
-         f += (long long)gridDim.x * blockDim.x) {
-        const unsigned u = __float_as_uint(v[f]);
-        if ((u & pmask) == prefix) atomicAdd(&h[(u >> shift) & (kRadixBins - 1)], 1u);
+    const int lane = threadIdx.x & 63;
+    // wavefront-uniform trip count; lanes that share a bin add once through
+    // their leader (|dz| values of one magnitude crowd into few bins)
+    for (long long base = blockIdx.x * (long long)blockDim.x + (threadIdx.x & ~63); base < n;
+         base += (long long)gridDim.x * blockDim.x) {
+        const long long f = base + lane;
+        unsigned bin = 0;
+        bool want = false;
+        if (f < n) {
+            const unsigned u = __float_as_uint(v[f]);
+            want = (u & pmask) == prefix;
+            bin = (u >> shift) & (kRadixBins - 1);
+        }
+        unsigned long long pending = __ballot(want);
+        while (pending) {
+            const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+            const unsigned lb = __shfl(bin, leader);
+            const unsigned long long same = __ballot(want && bin == lb);
+            if (lane == leader) atomicAdd(&h[lb], static_cast<unsigned>(__popcll(same)));
+            if (bin == lb) want = false;
+            pending &= ~same;
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kRadixBins; i += blockDim.x)
         if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
-// sel = {prefix, prefix mask, k}; picks the bin holding rank k
-__global__ void k_radix_pick(unsigned* __restrict__ hist, unsigned* __restrict__ sel, int shift, int bits)
+// sel = {prefix, prefix mask, k}; picks the bin holding rank k: one block
+// of 256 threads, 8 bins each, block-wide exclusive scan of the counts, the
+// bin whose cumulative range contains k wins; the histogram is cleared
+constexpr int kPickThreads = 256, kPickPer = kRadixBins / kPickThreads;
+__global__ void __launch_bounds__(kPickThreads) k_radix_pick(unsigned* __restrict__ hist, unsigned* __restrict__ sel,
+                                                             int shift, int bits)
 {
-    if (threadIdx.x != 0) return;
-    unsigned k = sel[2];
-    unsigned cum = 0;
-    int bin = 0;
-    for (; bin < (1 << bits); ++bin) {
-        if (cum + hist[bin] > k) break;
-        cum += hist[bin];
+    __shared__ unsigned warp_tot[kPickThreads / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int nb = 1 << bits;
+    unsigned c[kPickPer], sum = 0;
+#pragma unroll
+    for (int j = 0; j < kPickPer; ++j) {
+        const int b = t * kPickPer + j;
+        c[j] = b < nb ? hist[b] : 0u;
+        sum += c[j];
     }
-    sel[0] |= static_cast<unsigned>(bin) << shift;
-    sel[1] |= ((1u << bits) - 1u) << shift;
-    sel[2] = k - cum;
-    for (int i = 0; i < kRadixBins; ++i) hist[i] = 0;
+    // inclusive scan of the per-thread sums across the wavefront
+    unsigned x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) warp_tot[w] = x;
+    __syncthreads();
+    unsigned before = x - sum;
+    for (int i = 0; i < w; ++i) before += warp_tot[i];
+    const unsigned k = sel[2];
+    __syncthreads();
+    unsigned cum = before;
+#pragma unroll
+    for (int j = 0; j < kPickPer; ++j) {
+        if (cum <= k && k < cum + c[j]) {  // exactly one bin holds rank k
+            const int b = t * kPickPer + j;
+            sel[0] |= static_cast<unsigned>(b) << shift;
+            sel[1] |= ((1u << bits) - 1u) << shift;
+            sel[2] = k - cum;
+        }
+        cum += c[j];
+    }
+#pragma unroll
+    for (int j = 0; j < kPickPer; ++j) hist[t * kPickPer + j] = 0u;
 }
 
 }  // namespace
@@ -187,6 +233,7 @@ LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, con
     build_dofs();
     hs_grid = grid_for(n, 2048);
     scal.reserve(kScalCount + 8);
+    hpin.reserve(32);
     tickets.reserve(4);
     flags.reserve(8);
     LSDTT_HIP(hipMemset(tickets.p, 0, sizeof(unsigned) * 4));
@@ -239,6 +286,7 @@ void LemEngine::build_dofs()
     gen.reserve(gh.size() ? gh.size() : 1);
     if (n_gen) LSDTT_HIP(hipMemcpy(gen.p, gh.data(), sizeof(int) * gh.size(), hipMemcpyHostToDevice));
     partials.reserve(static_cast<size_t>(grid_for(n, 2048) + hs_general_blocks(n_gen) + 1) * 8);
+    c9g.reserve(static_cast<size_t>(9) * (n_gen ? n_gen : 1));
 }
 
 void LemEngine::set_channel_mask(const uint8_t* ch)
@@ -326,6 +374,7 @@ HsDom LemEngine::hs_domain(double inv_dt) const
 {
     HsDom d{nr, nc, p.bc, mask.p, dof.p, fast.p, gen.p, n_gen, dx, p.D, 0.0, p.Sc, p.slope_ratio_cap, inv_dt,
             p.nonlinear, zn_hs.p};
+    d.c9g = c9g.p;
     return with_reciprocals(d);
 }
 
@@ -358,7 +407,8 @@ int LemEngine::bicgstab(hipStream_t s)
     // iteration after convergence returns at once.  The chunk starts at the
     // usual iteration count of these well-conditioned systems and grows.
     int chunk = std::max(1, last_krylov_iterations);
-    int hf[3] = {0, 0, 0};
+    int* hf = reinterpret_cast<int*>(hpin.p + 16);
+    hf[0] = hf[1] = hf[2] = 0;
     int it_launched = 0;
     for (int done = 0; !done; chunk = std::min(2 * chunk, 16)) {
         for (int k = 0; k < chunk; ++k) {
@@ -383,7 +433,7 @@ void LemEngine::trial(double lambda, double* fnorm, double* ratio, hipStream_t s
              hs_grid, s);
     // z 8, zn 8, fast 1, dof 1 in; b 4, dinv 4 out; a trial step adds dz 4 in, z_out 8 out
     timer.end(s, (lambda != 0.0 ? 38.0 : 26.0) * n);
-    double h[2];
+    double* h = hpin.p;
     LSDTT_HIP(hipMemcpyAsync(h, scal.p + HS_TRIAL_F, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipStreamSynchronize(s));
     *fnorm = h[0];
@@ -517,8 +567,8 @@ void LemEngine::stats(LemStats* st, hipStream_t s)
     k_stats<<<hs_grid, kB, 0, s>>>(mask.p, z_before.p, z.p, n, udz, absdz.p, partials.p, tickets.p, scal.p + kStatSlot);
     timer.end(s, 21.0 * n);
     LSDTT_LAUNCH_CHECK();
-    double h[8];
-    LSDTT_HIP(hipMemcpyAsync(h, scal.p + kStatSlot, sizeof(h), hipMemcpyDeviceToHost, s));
+    double* h = hpin.p + 4;
+    LSDTT_HIP(hipMemcpyAsync(h, scal.p + kStatSlot, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipStreamSynchronize(s));
     const long long nv = static_cast<long long>(h[7]);
     const double inv_dt = 1.0 / p.dt, inv_n = nv > 0 ? 1.0 / static_cast<double>(nv) : 0.0;
@@ -531,17 +581,20 @@ void LemEngine::stats(LemStats* st, hipStream_t s)
     st->p999_local_elevation_change = 0.0;
     if (nv > 0) {
         const unsigned k = static_cast<unsigned>(std::floor(0.999 * static_cast<double>(nv - 1)));
-        unsigned sel[3] = {0u, 0u, k};
+        unsigned* sel = reinterpret_cast<unsigned*>(hpin.p + 24);
+        sel[0] = 0u;
+        sel[1] = 0u;
+        sel[2] = k;
         unsigned* dsel = hist.p + kRadixBins;
         LSDTT_HIP(hipMemsetAsync(hist.p, 0, sizeof(unsigned) * kRadixBins, s));
-        LSDTT_HIP(hipMemcpyAsync(dsel, sel, sizeof(sel), hipMemcpyHostToDevice, s));
+        LSDTT_HIP(hipMemcpyAsync(dsel, sel, sizeof(unsigned) * 3, hipMemcpyHostToDevice, s));
         const int shifts[3] = {21, 10, 0}, bits[3] = {11, 11, 10};
         for (int pass = 0; pass < 3; ++pass) {
             k_radix_hist<<<grid_for(n, 2048), kB, 0, s>>>(absdz.p, n, dsel, shifts[pass], hist.p);
-            k_radix_pick<<<1, 64, 0, s>>>(hist.p, dsel, shifts[pass], bits[pass]);
+            k_radix_pick<<<1, kPickThreads, 0, s>>>(hist.p, dsel, shifts[pass], bits[pass]);
             LSDTT_LAUNCH_CHECK();
         }
-        LSDTT_HIP(hipMemcpyAsync(sel, dsel, sizeof(sel), hipMemcpyDeviceToHost, s));
+        LSDTT_HIP(hipMemcpyAsync(sel, dsel, sizeof(unsigned) * 3, hipMemcpyDeviceToHost, s));
         LSDTT_HIP(hipStreamSynchronize(s));
         float pv;
         std::memcpy(&pv, &sel[0], sizeof(float));

@@ -106,7 +106,9 @@ public:
     double* zc = nullptr;
     double* zt = nullptr;
     dbuf<float> dinv, b, x, r, pv, phat, v, sv, shat, t;
+    dbuf<float> c9g;  // general DOFs' Jacobian rows (9 x n_gen)
     dbuf<double> partials, scal;  // reduction partials; scalars
+    hbuf<double> hpin;            // pinned host slots for read-backs
     dbuf<unsigned> tickets;
     dbuf<int> flags;
     dbuf<float> absdz;

@@ -361,10 +361,9 @@ __global__ void k_copy_d(const double* __restrict__ a, double* __restrict__ b, l
 
 double PicardSolver::read_scalar(int slot, hipStream_t s)
 {
-    double v = 0.0;
-    LSDTT_HIP(hipMemcpyAsync(&v, scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, s));
+    LSDTT_HIP(hipMemcpyAsync(hpin.p, scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipStreamSynchronize(s));
-    return v;
+    return hpin.p[0];
 }
 
 static PgArgs pg_args(int nr, int nc, BC bc, const uint8_t* mask, const uint8_t* dof, double h,
@@ -394,7 +393,8 @@ int PicardSolver::cg(hipStream_t s)
              scal.p, partials.p, ticket.p, flags.p, 10000};
     k_cg_init<<<grid_, kB, 0, s>>>(g);
     LSDTT_LAUNCH_CHECK();
-    int hf[2] = {0, 0};
+    int* hf = reinterpret_cast<int*>(hpin.p + 2);
+    hf[0] = hf[1] = 0;
     for (int chunk = 4;; chunk = std::min(2 * chunk, 64)) {
         for (int k = 0; k < chunk; ++k) {
             k_cg_Ap<<<grid_, kB, 0, s>>>(g);
@@ -402,7 +402,7 @@ int PicardSolver::cg(hipStream_t s)
             k_cg_p<<<grid_, kB, 0, s>>>(g);
         }
         LSDTT_LAUNCH_CHECK();
-        LSDTT_HIP(hipMemcpyAsync(hf, flags.p, sizeof(hf), hipMemcpyDeviceToHost, s));
+        LSDTT_HIP(hipMemcpyAsync(hf, flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
         LSDTT_HIP(hipStreamSynchronize(s));
         if (hf[0]) break;
     }
@@ -460,6 +460,7 @@ void PicardSolver::run(int nr, int nc, BC bc, double h, const uint8_t* mask, con
     scal.reserve(S_COUNT + 4);
     ticket.reserve(2);
     flags.reserve(4);
+    hpin.reserve(8);
     LSDTT_HIP(hipMemsetAsync(ticket.p, 0, sizeof(unsigned) * 2, s));
     *st = PicardStats{};
     const double omega = o.omega > 0.0 ? o.omega : 1.0;

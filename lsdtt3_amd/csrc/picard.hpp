@@ -55,6 +55,7 @@ private:
     dbuf<double> z_old, wE, wS, b, y, r, q, p, Ap, dinv, partials, scal, zp;
     dbuf<unsigned> ticket;
     dbuf<int> flags;
+    hbuf<double> hpin;  // pinned read-back slots
 
     double max_ratio2(const double* z, hipStream_t s);
     // one backward-Euler stage: damped Picard on [I - beta L] y = z_old + beta U

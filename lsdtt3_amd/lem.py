@@ -32,6 +32,37 @@ def build_boundary_flats(mask, boundary_conditions="oooo"):
     return out[:k.value].copy()
 
 
+def make_options(timestep_years=100.0, erodibility=1.0e-3, m_exponent=0.45, n_exponent=1.0, uplift_rate=1.0e-4,
+                 hillslope_diffusivity=1.0e-3, hillslope_model="roering_implicit", nonlinear_hillslopes=False,
+                 critical_hillslope_slope=0.8, incision_threshold_specific_area=0.0, boundary_conditions="oooo",
+                 fluvial_method="d8", baselevel_fall_rate=0.0, **solver) -> LemOptions:
+    """lsdtt_gpu_lem_options from the reference binding's parameter names;
+    extra keywords set fields of the struct by name."""
+    if hillslope_model not in ("roering_implicit", "none"):
+        raise ValueError(f"hillslope_model {hillslope_model!r} is not built on the GPU path")
+    if fluvial_method not in ("d8", "none"):
+        raise ValueError(f"fluvial_method {fluvial_method!r} is not built on the GPU path")
+    o = default_options()
+    o.timestep = float(timestep_years)
+    o.erodibility = float(erodibility)
+    o.m_exponent = float(m_exponent)
+    o.n_exponent = float(n_exponent)
+    o.uplift_rate = float(uplift_rate)
+    o.hillslope_diffusivity = float(hillslope_diffusivity)
+    o.enable_hillslope_transport = int(hillslope_model == "roering_implicit")
+    o.enable_nonlinear_hillslopes = int(bool(nonlinear_hillslopes))
+    o.critical_hillslope_slope = float(critical_hillslope_slope)
+    o.incision_threshold_area = float(incision_threshold_specific_area)
+    o.boundary_conditions = boundary_conditions.encode()[:4]
+    o.enable_d8_fluvial_erosion = int(fluvial_method == "d8")
+    o.baselevel_fall_rate = float(baselevel_fall_rate)
+    for k, v in solver.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown option {k!r}")
+        setattr(o, k, v)
+    return o
+
+
 class LandscapeModel:
     """GPU landscape evolution model.
 
@@ -51,32 +82,14 @@ class LandscapeModel:
         z = np.ascontiguousarray(dem, dtype=np.float64)
         if z.ndim != 2:
             raise ValueError("dem must be 2-D")
-        if hillslope_model not in ("roering_implicit", "none"):
-            raise ValueError(f"hillslope_model {hillslope_model!r} is not built on the GPU path")
-        if fluvial_method not in ("d8", "none"):
-            raise ValueError(f"fluvial_method {fluvial_method!r} is not built on the GPU path")
+        o = make_options(timestep_years, erodibility, m_exponent, n_exponent, uplift_rate, hillslope_diffusivity,
+                         hillslope_model, nonlinear_hillslopes, critical_hillslope_slope,
+                         incision_threshold_specific_area, boundary_conditions, fluvial_method, baselevel_fall_rate,
+                         **solver)
         self.n_rows, self.n_cols = z.shape
         self.cell_size = float(cell_size)
         self.mask = (np.ones(z.shape, np.uint8) if mask is None
                      else np.ascontiguousarray(mask, dtype=np.uint8))
-        o = default_options()
-        o.timestep = float(timestep_years)
-        o.erodibility = float(erodibility)
-        o.m_exponent = float(m_exponent)
-        o.n_exponent = float(n_exponent)
-        o.uplift_rate = float(uplift_rate)
-        o.hillslope_diffusivity = float(hillslope_diffusivity)
-        o.enable_hillslope_transport = int(hillslope_model == "roering_implicit")
-        o.enable_nonlinear_hillslopes = int(bool(nonlinear_hillslopes))
-        o.critical_hillslope_slope = float(critical_hillslope_slope)
-        o.incision_threshold_area = float(incision_threshold_specific_area)
-        o.boundary_conditions = boundary_conditions.encode()[:4]
-        o.enable_d8_fluvial_erosion = int(fluvial_method == "d8")
-        o.baselevel_fall_rate = float(baselevel_fall_rate)
-        for k, v in solver.items():
-            if not hasattr(o, k):
-                raise TypeError(f"unknown option {k!r}")
-            setattr(o, k, v)
         self.options = o
         L = N.lib()
         h = L.lsdtt_gpu_lem_create(self.n_rows, self.n_cols, self.cell_size, N.ptr(self.mask), C.byref(o))
@@ -157,3 +170,59 @@ class LandscapeModel:
             N.check(L.lsdtt_gpu_lem_profile_get(self._h, i, name, 64, C.byref(ms), C.byref(nl), C.byref(b)))
             out[name.value.decode()] = (ms.value, nl.value, b.value)
         return out
+
+
+class Ensemble:
+    """Independent LEM instances (uneven grids, per-instance parameters)
+    stepped concurrently on the GPU (lsdtt_gpu_ensemble_*).  ``models`` is a
+    list of dicts with ``dem``, ``cell_size``, optional ``mask`` and the
+    LandscapeModel keyword options."""
+
+    def __init__(self, models, workers=16):
+        L = N.lib()
+        self.shapes = []
+        self._keep = []
+        nr, nc, dx, masks = [], [], [], []
+        opts = (LemOptions * len(models))()
+        dems = []
+        for i, m in enumerate(models):
+            m = dict(m)
+            z = np.ascontiguousarray(m.pop("dem"), np.float64)
+            d = float(m.pop("cell_size"))
+            mk = m.pop("mask", None)
+            mk = np.ones(z.shape, np.uint8) if mk is None else np.ascontiguousarray(mk, np.uint8)
+            opts[i] = make_options(**m)
+            nr.append(z.shape[0])
+            nc.append(z.shape[1])
+            dx.append(d)
+            masks.append(mk)
+            dems.append(z)
+            self.shapes.append(z.shape)
+        self.n = len(models)
+        mptr = (C.c_void_p * self.n)(*[mk.ctypes.data for mk in masks])
+        h = L.lsdtt_gpu_ensemble_create(self.n, (C.c_int32 * self.n)(*nr), (C.c_int32 * self.n)(*nc),
+                                        (C.c_double * self.n)(*dx), mptr, opts, int(workers))
+        if not h:
+            raise N.GpuError(N.ERR_ARG, (L.lsdtt_gpu_last_error() or b"").decode())
+        self._h = h
+        for i, z in enumerate(dems):
+            N.check(L.lsdtt_gpu_ensemble_set_elevation(h, i, N.ptr(z)))
+        self.n_valid = [int(mk.sum()) for mk in masks]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().lsdtt_gpu_ensemble_destroy(h)
+            self._h = None
+
+    def run(self, n_steps):
+        """(list of last-step stats dicts, wall seconds)"""
+        st = (LemStats * self.n)()
+        sec = C.c_double(0.0)
+        N.check(N.lib().lsdtt_gpu_ensemble_run(self._h, int(n_steps), st, C.byref(sec)))
+        return [s.as_dict() for s in st], sec.value
+
+    def elevation(self, i):
+        z = np.empty(self.shapes[i], np.float64)
+        N.check(N.lib().lsdtt_gpu_ensemble_get_elevation(self._h, int(i), N.ptr(z)))
+        return z
