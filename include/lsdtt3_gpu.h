@@ -237,6 +237,13 @@ typedef struct {
     int32_t picard_cg_iterations;
     int32_t picard_limiter_fired;
     double picard_max_slope_ratio;
+    /* warm start of the first Newton solve (lsdtt_gpu_lem_set_warm_start;
+     * no counterpart in the reference): mode used, whether the predicted
+     * start was kept (it must lower ||F||inf below ||F(z^n)||inf, and keep
+     * the slope ratio feasible), and ||F||inf at the predicted start */
+    int32_t warm_start_mode;
+    int32_t warm_start_accepted;
+    double warm_start_residual;
 } lsdtt_gpu_lem_stats;
 
 /* Defaults of lsdtt_lem_options, plus the Newton defaults above. */
@@ -298,6 +305,91 @@ int lsdtt_gpu_lem_profile(lsdtt_gpu_lem* lem, int32_t enable);
 int32_t lsdtt_gpu_lem_profile_count(lsdtt_gpu_lem* lem);
 int lsdtt_gpu_lem_profile_get(lsdtt_gpu_lem* lem, int32_t index, char* name, int32_t name_len,
                               double* ms, int64_t* launches, double* bytes);
+
+/* ---------------------------------------------------------------------
+ * Learned hillslope warm start.  No counterpart in the reference, which
+ * always starts the hillslope Newton solve from z^n
+ * (lsdtt_hillslope_newton_cpu.cpp:196-260).  A 2-D U-Net (`levels`
+ * resolution levels, `base_channels` doubling per level, two 3x3
+ * convolutions + ReLU per level each way, 2x2 average pooling, nearest
+ * upsampling with skip concatenation, 1x1 head) with BF16 weights and
+ * activations and FP32 accumulation on MFMA; FP32 master weights, Adam.
+ * The LEM feeds it windows of tile + 2 halo cells with four features per
+ * pixel (z minus the window mean over z_scale, |grad z| / Sc, the h^2
+ * Laplacian over lap_scale, dt D / h^2 times kappa_scale) and takes its
+ * output times dt D / h^2 * lap_scale as the predicted hillslope change.
+ * ------------------------------------------------------------------- */
+typedef struct lsdtt_gpu_cnn lsdtt_gpu_cnn;
+typedef struct lsdtt_gpu_cnn_dataset lsdtt_gpu_cnn_dataset;
+
+/* in_channels 1..32; base_channels a multiple of 32 with
+ * base_channels * 2^(levels-1) <= 256; 2 <= levels <= 6.  Weights are
+ * He-normal from `seed`, biases zero.  tile/halo: the LEM windows (256/16). */
+lsdtt_gpu_cnn* lsdtt_gpu_cnn_create(int32_t in_channels, int32_t base_channels, int32_t levels, uint64_t seed,
+                                    int32_t tile, int32_t halo);
+void lsdtt_gpu_cnn_destroy(lsdtt_gpu_cnn* net);
+int64_t lsdtt_gpu_cnn_num_params(const lsdtt_gpu_cnn* net);
+/* FP32 parameter vector: per 3x3 layer W[cout][9][cin_pad] (tap = 3 (dy+1)
+ * + dx+1, cin_pad = cin rounded up to 8, padded channels zero) then
+ * bias[cout]; layers in the order: for each level from the finest, the
+ * encoder's two convolutions; then for each level below the coarsest, from
+ * the finest, the decoder's two convolutions (the first one's input is
+ * [upsampled coarser output, encoder skip]); then the head's
+ * weights[base_channels] and bias. */
+int lsdtt_gpu_cnn_get_params(const lsdtt_gpu_cnn* net, float* params);
+int lsdtt_gpu_cnn_set_params(lsdtt_gpu_cnn* net, const float* params);
+int lsdtt_gpu_cnn_get_grads(const lsdtt_gpu_cnn* net, float* grads);
+/* Host arrays: x FP32 NHWC [n][h][w][in_channels], y FP32 [n][h][w]; h and
+ * w divisible by 2^(levels-1), coarsest level's pixel count by 16. */
+int lsdtt_gpu_cnn_forward(lsdtt_gpu_cnn* net, const float* x, int32_t n, int32_t h, int32_t w, float* y);
+/* After _forward: MSE over pixels with mask != 0 (NULL = all) against
+ * target [n][h][w]; the parameter gradient is ADDED to the gradient buffer
+ * (cleared by _zero_grad); *loss the MSE. */
+int lsdtt_gpu_cnn_backward(lsdtt_gpu_cnn* net, const float* target, const uint8_t* mask, double* loss);
+int lsdtt_gpu_cnn_zero_grad(lsdtt_gpu_cnn* net);
+int lsdtt_gpu_cnn_adam_step(lsdtt_gpu_cnn* net, double lr, double beta1, double beta2, double eps);
+/* Throughput of the network on random inputs of n x h x w: mean
+ * milliseconds of a forward pass, of its 3x3 convolution stack alone, and
+ * of a training iteration (forward, backward, Adam); *flops the 3x3
+ * convolutions' FLOP of one forward pass (2 per multiply-add). */
+int lsdtt_gpu_cnn_benchmark(lsdtt_gpu_cnn* net, int32_t n, int32_t h, int32_t w, int32_t reps, double* forward_ms,
+                            double* conv_ms, double* train_ms, double* flops);
+
+/* One 3x3 convolution layer of the network on host arrays, through the
+ * network's own kernels (for checking them one layer at a time): x
+ * [n][h][w][cin] and w [cout][9][cin] are rounded to BF16; y
+ * [n][h][w][cout] receives the BF16 output (+ bias b, ReLU when relu != 0).
+ * With gy != NULL (FP32 output gradient): gw [cout][9][cin], gb [cout] and,
+ * if gx != NULL (cin a multiple of 32), gx [n][h][w][cin]; the masked
+ * output gradient is rounded to BF16 first, as in training.  cout a
+ * multiple of 32, n*h*w a multiple of 16. */
+int lsdtt_gpu_cnn_conv3x3(const float* x, const float* w, const float* b, int32_t n, int32_t h, int32_t w_px,
+                          int32_t cin, int32_t cout, int32_t relu, float* y, const float* gy, float* gx, float* gw,
+                          float* gb);
+
+/* Training pairs from LEM steps.  _capture makes every later step of `lem`
+ * keep its post-incision surface and hillslope change; _add copies the
+ * last captured pair into the dataset (device-resident; all pairs on the
+ * dataset's grid, whose mask and hillslope DOFs are taken from the first
+ * LEM added). */
+int lsdtt_gpu_lem_capture_hillslope(lsdtt_gpu_lem* lem, int32_t enable);
+lsdtt_gpu_cnn_dataset* lsdtt_gpu_cnn_dataset_create(void);
+void lsdtt_gpu_cnn_dataset_destroy(lsdtt_gpu_cnn_dataset* ds);
+int lsdtt_gpu_cnn_dataset_add(lsdtt_gpu_cnn_dataset* ds, lsdtt_gpu_lem* lem);
+int32_t lsdtt_gpu_cnn_dataset_size(const lsdtt_gpu_cnn_dataset* ds);
+/* n_iterations Adam steps (beta 0.9 / 0.999, eps 1e-8) on batches of
+ * random windows; losses (n_iterations, may be NULL) the batch MSEs. */
+int lsdtt_gpu_cnn_train(lsdtt_gpu_cnn* net, lsdtt_gpu_cnn_dataset* ds, int32_t n_iterations, int32_t batch,
+                        double lr, uint64_t seed, float* losses);
+/* Mean MSE of the network and of the zero prediction on n_batches random
+ * batches (parameters unchanged). */
+int lsdtt_gpu_cnn_evaluate(lsdtt_gpu_cnn* net, lsdtt_gpu_cnn_dataset* ds, int32_t n_batches, int32_t batch,
+                           uint64_t seed, double* loss, double* zero_loss);
+/* Warm start of each step's first Newton solve: mode 0 off (z^n), 1 the
+ * explicit Euler predictor z^n + dt (-F(z^n)), 2 the network `net` (which
+ * must outlive its use here).  The predicted start is kept only when it
+ * lowers ||F||inf; the converged residual is judged as without it. */
+int lsdtt_gpu_lem_set_warm_start(lsdtt_gpu_lem* lem, lsdtt_gpu_cnn* net, int32_t mode);
 
 /* Valid cells with an 8-neighbour off a fixed ('o'/'b') edge or masked in
  * a fixed direction (build_boundary_flats, lsdtt_lem_cpu.cpp:735-808);

@@ -43,6 +43,8 @@ class LemStats(C.Structure):
         ("hillslope_method", C.c_int32), ("picard_iterations", C.c_int32), ("picard_substeps", C.c_int32),
         ("picard_rejected", C.c_int32), ("picard_cg_iterations", C.c_int32),
         ("picard_limiter_fired", C.c_int32), ("picard_max_slope_ratio", C.c_double),
+        ("warm_start_mode", C.c_int32), ("warm_start_accepted", C.c_int32),
+        ("warm_start_residual", C.c_double),
     ]
 
     def as_dict(self):
@@ -69,3 +71,24 @@ def declare(L, sig):
     sig(L, "lsdtt_gpu_lem_profile", C.c_int, [vp, i32])
     sig(L, "lsdtt_gpu_lem_profile_count", C.c_int32, [vp])
     sig(L, "lsdtt_gpu_lem_profile_get", C.c_int, [vp, i32, C.c_char_p, i32, vp, vp, vp])
+    u64 = C.c_uint64
+    sig(L, "lsdtt_gpu_cnn_create", vp, [i32, i32, i32, u64, i32, i32])
+    sig(L, "lsdtt_gpu_cnn_destroy", None, [vp])
+    sig(L, "lsdtt_gpu_cnn_num_params", i64, [vp])
+    sig(L, "lsdtt_gpu_cnn_get_params", C.c_int, [vp, vp])
+    sig(L, "lsdtt_gpu_cnn_set_params", C.c_int, [vp, vp])
+    sig(L, "lsdtt_gpu_cnn_get_grads", C.c_int, [vp, vp])
+    sig(L, "lsdtt_gpu_cnn_forward", C.c_int, [vp, vp, i32, i32, i32, vp])
+    sig(L, "lsdtt_gpu_cnn_backward", C.c_int, [vp, vp, vp, vp])
+    sig(L, "lsdtt_gpu_cnn_zero_grad", C.c_int, [vp])
+    sig(L, "lsdtt_gpu_cnn_adam_step", C.c_int, [vp, dbl, dbl, dbl, dbl])
+    sig(L, "lsdtt_gpu_cnn_benchmark", C.c_int, [vp, i32, i32, i32, i32, vp, vp, vp, vp])
+    sig(L, "lsdtt_gpu_cnn_conv3x3", C.c_int, [vp, vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp])
+    sig(L, "lsdtt_gpu_lem_capture_hillslope", C.c_int, [vp, i32])
+    sig(L, "lsdtt_gpu_cnn_dataset_create", vp, [])
+    sig(L, "lsdtt_gpu_cnn_dataset_destroy", None, [vp])
+    sig(L, "lsdtt_gpu_cnn_dataset_add", C.c_int, [vp, vp])
+    sig(L, "lsdtt_gpu_cnn_dataset_size", i32, [vp])
+    sig(L, "lsdtt_gpu_cnn_train", C.c_int, [vp, vp, i32, i32, dbl, u64, vp])
+    sig(L, "lsdtt_gpu_cnn_evaluate", C.c_int, [vp, vp, i32, i32, u64, vp, vp])
+    sig(L, "lsdtt_gpu_lem_set_warm_start", C.c_int, [vp, vp, i32])

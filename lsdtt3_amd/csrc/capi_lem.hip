@@ -13,13 +13,6 @@
 
 using namespace lsdtt_gpu;
 
-struct lsdtt_gpu_lem {
-    std::unique_ptr<LemEngine> eng;
-    std::vector<int64_t> bf_cache;
-    bool bf_set = false;
-    bool z_set = false;
-};
-
 namespace lsdtt_gpu {
 
 bool bc_ok(const char* s)
@@ -119,6 +112,9 @@ void to_c_stats(const LemStats& st, lsdtt_gpu_lem_stats* out)
     out->picard_cg_iterations = st.picard.cg_iters;
     out->picard_limiter_fired = st.picard.limiter_fired;
     out->picard_max_slope_ratio = st.picard.max_slope_ratio;
+    out->warm_start_mode = st.warm_start_mode;
+    out->warm_start_accepted = st.warm_start_accepted;
+    out->warm_start_residual = st.warm_start_residual;
     out->stream_power_tile_runs = st.stream_power_tile_runs;
     out->stream_power_passes = st.stream_power_passes;
 }

@@ -2,8 +2,18 @@
 // engine's (shared by the single-model and ensemble entry points).
 #pragma once
 
+#include <memory>
+#include <vector>
+
 #include "../../include/lsdtt3_gpu.h"
 #include "lem.hpp"
+
+struct lsdtt_gpu_lem {
+    std::unique_ptr<lsdtt_gpu::LemEngine> eng;
+    std::vector<int64_t> bf_cache;
+    bool bf_set = false;
+    bool z_set = false;
+};
 
 namespace lsdtt_gpu {
 

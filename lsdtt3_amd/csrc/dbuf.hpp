@@ -13,6 +13,22 @@ struct dbuf {
     dbuf() = default;
     dbuf(const dbuf&) = delete;
     dbuf& operator=(const dbuf&) = delete;
+    dbuf(dbuf&& o) noexcept : p(o.p), n(o.n)
+    {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    dbuf& operator=(dbuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
     ~dbuf() { release(); }
 
     // grow to hold k elements; contents are not preserved

@@ -59,6 +59,21 @@ __global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, lon
         b[i] = a[i];
 }
 
+// out = float(a - b)
+__global__ void k_diff_f(const double* __restrict__ a, const double* __restrict__ b, float* __restrict__ out,
+                         long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = static_cast<float>(a[i] - b[i]);
+}
+
+// out = a * c (FP32)
+__global__ void k_scale_f(const float* __restrict__ a, float c, float* __restrict__ out, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = a[i] * c;
+}
+
 __global__ void k_gather(const double* __restrict__ z, const int* __restrict__ idx, long long m,
                          double* __restrict__ out)
 {
@@ -453,6 +468,31 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
     if (st->substeps == 0 && st->rejected == 0) st->hillslope_initial_residual = fnorm;
     st->hillslope_tolerance = tol;
     const double feasible = std::sqrt(p.slope_ratio_cap);
+    if (ws_mode != 0 && st->substeps == 0 && st->rejected == 0 && fnorm > tol) {
+        // predicted start z^n + dz; plain Newton from z^n stays the fallback
+        timer.begin("warm_start", s);
+        if (ws_mode == 1) {
+            // explicit Euler: b = -F(z^n) from the trial above
+            k_scale_f<<<grid_for(n, 4096), kB, 0, s>>>(b.p, static_cast<float>(1.0 / inv_dt), x.p, n);
+            LSDTT_LAUNCH_CHECK();
+        } else {
+            if (!ws) throw std::logic_error("learned warm start without a predictor");
+            ws->predict(zc, mask.p, dof.p, nr, nc, dx, p.D / (inv_dt * dx * dx), p.Sc, x.p, s);
+        }
+        timer.end(s, 0.0);
+        double fw = 0.0, rw = 0.0;
+        trial(1.0, &fw, &rw, s);
+        st->warm_start_mode = ws_mode;
+        st->warm_start_residual = fw;
+        if (fw < fnorm && (!p.nonlinear || rw <= feasible)) {
+            std::swap(zc, zt);
+            fnorm = fw;
+            ratio_now = rw;
+            st->warm_start_accepted = 1;
+        } else {
+            trial(0.0, &fnorm, &ratio_now, s);  // the system at z^n again
+        }
+    }
     int it = 0;
     bool converged = false;
     for (; it < p.newton_max_iterations; ++it) {
@@ -638,6 +678,11 @@ int LemEngine::step(LemStats* st, hipStream_t s)
             }
         }
         if (!done) {
+            if (p.force_picard && capture) {
+                z_hs0.reserve(static_cast<size_t>(n));
+                k_copy<<<g, kB, 0, s>>>(z.p, z_hs0.p, n);
+                LSDTT_LAUNCH_CHECK();
+            }
             PicardOptions po = p.picard;
             po.D = p.D;
             po.U = 0.0;
@@ -648,6 +693,13 @@ int LemEngine::step(LemStats* st, hipStream_t s)
             timer.end(s, 0.0);
             st->hillslope_method = 1;
         }
+    }
+    if (p.enable_hillslope && capture) {
+        cap_dz.reserve(static_cast<size_t>(n));
+        k_diff_f<<<g, kB, 0, s>>>(z.p, z_hs0.p, cap_dz.p, n);
+        LSDTT_LAUNCH_CHECK();
+        cap_kappa = p.D * p.dt / (dx * dx);
+        cap_valid = true;
     }
     if (p.enable_uplift) {
         k_uplift<<<g, kB, 0, s>>>(mask.p, z.p, p.U * p.dt, n);

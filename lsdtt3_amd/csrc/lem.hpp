@@ -25,6 +25,7 @@
 #include "hillslope.hpp"
 #include "ktimer.hpp"
 #include "picard.hpp"
+#include "warmstart.hpp"
 
 namespace lsdtt_gpu {
 
@@ -60,6 +61,10 @@ struct LemStats {
     long long count_tile_runs = 0, stream_power_tile_runs = 0;
     int hillslope_method = 0;  // 0 Newton, 1 Picard
     PicardStats picard;
+    // warm start of the first Newton solve (0 none, 1 explicit Euler
+    // predictor, 2 learned); accepted when it lowers ||F||inf below F(z^n)
+    int warm_start_mode = 0, warm_start_accepted = 0;
+    double warm_start_residual = 0;  // ||F||inf at the predicted start
 };
 
 class LemEngine {
@@ -115,6 +120,16 @@ public:
     dbuf<unsigned> hist;
 
     int hs_grid = 1024;
+    // warm start of the first Newton solve of a step: 0 off, 1 explicit
+    // Euler (dz = dt (-F(z^n))), 2 the learned predictor `ws`
+    int ws_mode = 0;
+    WarmStartPredictor* ws = nullptr;
+    // training capture: after each step, cap_dz = hillslope change of z
+    // (post-hillslope minus post-incision, FP32) and cap_kappa = dt D / h^2;
+    // z_hs0 holds the post-incision surface
+    bool capture = false, cap_valid = false;
+    dbuf<float> cap_dz;
+    double cap_kappa = 0.0;
     KernelTimer timer;
     double cur_inv_dt = 0.0;
     int last_krylov_iterations = 2;
