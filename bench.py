@@ -16,10 +16,20 @@ timed region, measured with HIP events on the step's stream; the CPU
 baseline is the oracle's single-threaded FP64 step on a 512 x 512 crop of
 the same recipe.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
+The line also carries learned_warm_start: the same step with the Newton
+solve started from the explicit-Euler prediction and from the U-Net's
+prediction (trained here first, on tile pairs captured from seeded 2048^2
+runs of the same physics), each to the same r <= 1e-8, with the network's
+training figures and its MFMA utilisation at the 4096^2 inference shape
+(256 windows of 288^2).  value stays the plain step, the fastest of the
+three.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-learned]
+       python bench.py --workload ensemble   (64-instance batched ensemble)
+       python bench.py --workload cnn        (the U-Net alone: MFMA utilisation)
 
 Profile summaries (text, for profiles/) from rocprofv3 CSV output:
-  python bench.py --summarize-trace DIR OUT   (--kernel-trace --stats run)
+  python bench.py --summarize-trace DIR OUT [TITLE CMD NOTE]   (--kernel-trace --stats run)
   python bench.py --summarize-pmc FETCH_DIR WRITE_DIR OUT
                                               (--pmc FETCH_SIZE / WRITE_SIZE runs)
 The PMC summary scales each counter by what it reads for the step's plain
@@ -101,13 +111,16 @@ def _find(d, suffix):
     raise FileNotFoundError(f"no *{suffix} in {d}")
 
 
-def summarize_trace(d, out):
+def summarize_trace(d, out, title=None, cmd=None, note=None):
     rows = _csv(_find(d, "kernel_stats.csv"))
-    lines = ["# rocprofv3 kernel statistics, LEM step at 4096^2 (bench.py workload)", "#",
-             "# made by: cd /tmp && rocprofv3 --kernel-trace --stats --output-format csv -d <dir> -o run -- "
-             "python3 bench.py --steps 3 --warmup 1 --no-cpu-baseline",
-             "# summarized by: python bench.py --summarize-trace <dir> <this file>",
-             "# totals cover the 4 LEM steps of the run (1 warm-up + 3 timed) plus the context set-up", "#",
+    title = title or "LEM step at 4096^2 (bench.py workload)"
+    cmd = cmd or "python3 bench.py --steps 3 --warmup 1 --no-cpu-baseline --no-learned"
+    note = note or "totals cover the 4 LEM steps of the run (1 warm-up + 3 timed) plus the context set-up"
+    lines = [f"# rocprofv3 kernel statistics, {title}", "#",
+             "# made by: cd /tmp && rocprofv3 --kernel-trace --stats --output-format csv -d <dir> -o run -- " + cmd,
+             "# summarized by: python bench.py --summarize-trace <dir> <this file>"
+             + ("" if cmd.endswith("--no-learned") and title.startswith("LEM") else " <title> <command> <note>"),
+             f"# {note}", "#",
              f"{'kernel':34s} {'calls':>6s} {'total_ms':>10s} {'avg_us':>10s} {'min_us':>9s} {'max_us':>9s} {'pct':>6s}"]
     agg = {}
     for r in rows:
@@ -140,9 +153,9 @@ def summarize_pmc(fetch_dir, write_dir, out, n_cells=4096 * 4096):
     fw = copy_kib / (cw[1] / cw[0])
     lines = ["# HBM traffic per dispatch from rocprofv3 PMC counters, LEM step at 4096^2 (bench.py workload)", "#",
              "# made by: cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch_dir> -o run -- "
-             "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline",
+             "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-learned",
              "#          cd /tmp && rocprofv3 --pmc WRITE_SIZE --output-format csv -d <write_dir> -o run -- "
-             "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline",
+             "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-learned",
              "# summarized by: python bench.py --summarize-pmc <fetch_dir> <write_dir> <this file>",
              f"# calibration: k_copy reads and writes {copy_kib:.0f} KiB per dispatch; FETCH_SIZE reads "
              f"{cf[1] / cf[0]:.1f}, WRITE_SIZE {cw[1] / cw[0]:.1f} -> scale factors fetch {ff:.4f}, write {fw:.4f}",
@@ -230,9 +243,125 @@ def bench_ensemble(args):
     print(json.dumps(out))
 
 
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def learned_comparison(args, z, K, sync, plain_value):
+    """Config 3: the learned warm start on the same 4096^2 workload.
+
+    Training data: seeded 2048^2 LEM runs of the same physics (K
+    log-uniform 1e-6..1e-4 per run) captured step by step; one snapshot
+    holds (2048 / 256)^2 = 64 tile pairs, so `pairs` tile pairs need
+    pairs / 64 snapshots.  Training: Adam lr 1e-3, batch 32 of 256^2 tiles
+    + 16-cell halos, one pass over the pairs.  Timed: the 4096^2 step with
+    the explicit-Euler start and with the learned start, each from the
+    same surface as the plain run, to the same r <= 1e-8."""
+    import lsdtt3_amd
+    from lsdtt3_amd import learned as Lm
+    from lsdtt3_amd.synthetic import lem_surface
+    t0 = time.perf_counter()
+    ds = Lm.Dataset()
+    runs = 4
+    per_run = max(1, int(np.ceil(args.pairs / 64 / runs)))
+    rng = np.random.default_rng(77)
+    for r in range(runs):
+        Kr = float(10.0 ** rng.uniform(-6.0, -4.0))
+        zr = lem_surface(args.train_size, args.train_size, dx=30.0, seed=100 + r, max_slope=0.75 * 0.8)
+        m = lsdtt3_amd.LandscapeModel(zr, 30.0, **lem_kwargs(Kr))
+        Lm.generate([m], ds, per_run)
+        del m
+    sync()
+    t_gen = time.perf_counter() - t0
+    net = Lm.UNet(4, 32, 4, seed=1, tile=256, halo=16)
+    zero_loss = net.evaluate(ds, 2, 32, seed=5)[1]
+    iters = max(1, int(np.ceil(len(ds) * 64 / 32)))
+    t0 = time.perf_counter()
+    losses = net.train(ds, iters, batch=32, lr=1e-3, seed=3)
+    sync()
+    t_train = time.perf_counter() - t0
+    loss = net.evaluate(ds, 2, 32, seed=5)[0]
+    win = 256 + 2 * 16
+    tiles = ((args.size + 255) // 256) ** 2
+    bm = net.benchmark(tiles, win, win, reps=2)
+    out = {"training": {"snapshots": len(ds), "tile_pairs": len(ds) * 64, "snapshot_grid": args.train_size,
+                        "runs": runs, "iterations": iters, "batch": 32, "lr": 1e-3,
+                        "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
+                        "ms_per_iteration": round(1e3 * t_train / iters, 2),
+                        "loss_zero_prediction": zero_loss, "loss_trained": loss,
+                        "first_batch_loss": float(losses[0]), "last_batch_loss": float(losses[-1])}}
+    flops = bm["conv_flops"]
+    tf = flops / (bm["conv_ms"] * 1e-3) / 1e12
+    out["inference"] = {"windows": tiles, "window": win, "conv_flops": flops, "forward_ms": round(bm["forward_ms"], 3),
+                        "conv_ms": round(bm["conv_ms"], 3),
+                        "mfma_roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": BF16_PEAK_TFLOPS,
+                                          "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 4),
+                                          "kernel": "k_conv_tile (3x3 conv stack, forward)"}}
+    for name, mode in (("explicit_start", Lm.WARM_EXPLICIT), ("learned_start", Lm.WARM_LEARNED)):
+        lem = lsdtt3_amd.LandscapeModel(z, 30.0, **lem_kwargs(K))
+        Lm.set_warm_start(lem, mode, net if mode == Lm.WARM_LEARNED else None)
+        for _ in range(args.warmup):
+            lem.step()
+        sync()
+        lem.profile(True)
+        newton = krylov = acc = 0
+        worst_r = ratio = 0.0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = lem.step()
+            newton += st["newton_iterations"]
+            krylov += st["krylov_iterations"]
+            acc += st["warm_start_accepted"]
+            if st["hillslope_initial_residual"] > 0:
+                worst_r = max(worst_r, st["hillslope_residual"] / st["hillslope_initial_residual"])
+                ratio = max(ratio, st["warm_start_residual"] / st["hillslope_initial_residual"])
+            if not st["hillslope_converged"]:
+                raise RuntimeError("hillslope Newton did not converge")
+        sync()
+        dt = time.perf_counter() - t0
+        prof = lem.profile_totals()
+        lem.profile(False)
+        v = int(st["n_valid"]) * args.steps / dt
+        out[name] = {"value": round(v, 1), "unit": "cell-steps/s", "vs_plain": round(v / plain_value, 4),
+                     "ms_per_step": round(1e3 * dt / args.steps, 3),
+                     "newton_iters_per_step": round(newton / args.steps, 2),
+                     "krylov_iters_per_step": round(krylov / args.steps, 2),
+                     "starts_accepted": acc, "worst_start_residual_ratio": ratio,
+                     "worst_relative_residual": worst_r,
+                     "predictor_ms_per_step": round(prof.get("warm_start", (0.0,))[0] / args.steps, 3)}
+        del lem
+    return out
+
+
+def bench_cnn(args):
+    """the warm-start U-Net alone (random weights and inputs): forward at the
+    4096^2 inference shape (256 windows of 288^2) and a training iteration
+    at batch 32; MFMA utilisation of the 3x3 convolution stack"""
+    from lsdtt3_amd import learned as Lm
+    net = Lm.UNet(4, 32, 4, seed=1, tile=256, halo=16)
+    win = 288
+    tiles = ((args.size + 255) // 256) ** 2
+    inf = net.benchmark(tiles, win, win, reps=max(1, args.steps))
+    tr = net.benchmark(32, win, win, reps=max(1, args.steps))
+    tf = inf["conv_flops"] / (inf["conv_ms"] * 1e-3) / 1e12
+    out = {"metric": "U-Net 3x3 convolution TFLOP/s (forward, BF16 MFMA)", "value": round(tf, 1),
+           "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+           "ms_per_step": round(inf["forward_ms"], 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+           "config": {"workload": f"warm-start U-Net (4 levels, 32 base channels, 3x3 convs) forward on {tiles} "
+                                  f"windows of {win}x{win}x4 (the {args.size}^2 inference), and training at batch 32",
+                      "params": net.num_params, "conv_flops_forward": inf["conv_flops"],
+                      "forward_ms": round(inf["forward_ms"], 3), "conv_ms": round(inf["conv_ms"], 3),
+                      "train_ms_batch32": round(tr["train_ms"], 3),
+                      "forward_ms_batch32": round(tr["forward_ms"], 3)},
+           "roofline": {"bound": "mfma", "kernel": "conv stack", "hip_kernel": "k_conv_tile",
+                        "achieved": round(tf, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tf / BF16_PEAK_TFLOPS, 4), "traffic": None}}
+    print(json.dumps(out))
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-trace":
-        summarize_trace(sys.argv[2], sys.argv[3])
+        summarize_trace(*sys.argv[2:7])
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-pmc":
         summarize_pmc(sys.argv[2], sys.argv[3], sys.argv[4])
@@ -244,10 +373,13 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["lem4096", "ensemble"], default="lem4096")
+    ap.add_argument("--workload", choices=["lem4096", "ensemble", "cnn"], default="lem4096")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--sequential", action="store_true", help="ensemble: also time one instance at a time")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
+    ap.add_argument("--pairs", type=int, default=20000, help="learned: training tile pairs")
+    ap.add_argument("--train-size", type=int, default=2048, help="learned: training LEM grid side")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
@@ -260,6 +392,9 @@ def main():
         sys.exit(1)
     if args.workload == "ensemble":
         bench_ensemble(args)
+        return
+    if args.workload == "cnn":
+        bench_cnn(args)
         return
     try:
         import torch
@@ -320,6 +455,8 @@ def main():
         "roofline": roofline,
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
     }
+    if not args.no_learned:
+        out["learned_warm_start"] = learned_comparison(args, z, K, sync, value)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(K, args.cpu_budget)
     print(json.dumps(out))
