@@ -95,19 +95,123 @@ __global__ void __launch_bounds__(kB) k_conv(const __bf16* __restrict__ X, const
 }
 
 // ---------------------------------------------------------------------------
+// 3x3 convolution for cin a multiple of 32: a workgroup computes a 16 x 16
+// pixel block of one image for 16 NCG output channels.  Per 32-channel chunk
+// of the input it stages the block plus a one-pixel halo (18 x 18 pixels x
+// 32 channels, pixel stride 40 elements so that 16-byte reads of consecutive
+// pixels fall in distinct LDS banks) and then, for each of the 9 taps, forms
+// the A operands of all 16 x 16 pixels from LDS (shifted reads, no im2col
+// buffer) while the B operands (weights of that tap and chunk) come from
+// global memory, shared by the wavefronts through L1.  Wavefront w owns
+// block rows 4w..4w+3: 4 pixel groups x NCG channel groups of MFMA tiles per
+// tap, so each A fragment feeds NCG MFMAs and each B fragment four.
+//
+// The input may be the decoder's concatenation [upsample2(U), X]: channels
+// below cs come from U (cs channels at half resolution, pixel (y/2, x/2)),
+// the rest from X (cin - cs channels); cs is a multiple of 32 (0: X only).
+// ---------------------------------------------------------------------------
+constexpr int kTile = 16;                // output block side (pixels)
+constexpr int kHalo = kTile + 2;         // staged side
+constexpr int kPixStride = 40;           // LDS elements per staged pixel (32 + 8 pad)
+
+template <bool RELU, bool OUT_BF16, int NCG>
+__global__ void __launch_bounds__(kB) k_conv_tile(const __bf16* __restrict__ U, int cs, const __bf16* __restrict__ X,
+                                                  const __bf16* __restrict__ Wt, const float* __restrict__ bias,
+                                                  void* __restrict__ Y, int N, int H, int W, int cin, int cout,
+                                                  int kpad, int tiles_x, int tiles_y)
+{
+    __shared__ __align__(16) __bf16 tile[kHalo * kHalo * kPixStride];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int b = blockIdx.x;
+    const int tx0 = (b % tiles_x) * kTile;
+    b /= tiles_x;
+    const int ty0 = (b % tiles_y) * kTile;
+    const int n = b / tiles_y;
+    const int co0 = blockIdx.y * (16 * NCG);
+    const int col = lane & 15, kg = lane >> 4;
+    const __bf16* wrow = Wt + (size_t)(co0 + col) * kpad + 8 * kg;
+    floatx4 acc[4][NCG];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int g = 0; g < NCG; ++g) acc[r][g] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int ck = cin - cs, Hh = H >> 1, Wh = W >> 1;
+    for (int ci0 = 0; ci0 < cin; ci0 += 32) {
+        __syncthreads();  // the previous chunk's reads are done
+        const bool from_u = ci0 < cs;
+        for (int i = threadIdx.x; i < kHalo * kHalo * 4; i += kB) {
+            const int pix = i >> 2, c8 = (i & 3) * 8;
+            const int yy = ty0 - 1 + pix / kHalo, xx = tx0 - 1 + pix % kHalo;
+            bf16x8 v = zero8();
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                const __bf16* src =
+                    from_u ? U + ((((size_t)n * Hh + (yy >> 1)) * Wh + (xx >> 1)) * cs + ci0 + c8)
+                           : X + ((((size_t)n * H + yy) * W + xx) * ck + (ci0 - cs) + c8);
+                v = *reinterpret_cast<const bf16x8*>(src);
+            }
+            *reinterpret_cast<bf16x8*>(&tile[pix * kPixStride + c8]) = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;  // offsets into the staged block (halo included)
+            bf16x8 bw[NCG];
+#pragma unroll
+            for (int g = 0; g < NCG; ++g)
+                bw[g] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)g * 16 * kpad + tap * cin + ci0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 4 * wave + r;
+                const bf16x8 a =
+                    *reinterpret_cast<const bf16x8*>(&tile[((row + dy) * kHalo + col + dx) * kPixStride + 8 * kg]);
+#pragma unroll
+                for (int g = 0; g < NCG; ++g)
+                    acc[r][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[g], acc[r][g], 0, 0, 0);
+            }
+        }
+    }
+    // lane (col, kg) holds, for block row `row`, the pixels 4 kg + i of that
+    // row and output channels co0 + 16 g + col
+    float bs[NCG];
+#pragma unroll
+    for (int g = 0; g < NCG; ++g) bs[g] = bias ? bias[co0 + 16 * g + col] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int yy = ty0 + 4 * wave + r;
+        if (yy >= H) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int xx = tx0 + 4 * kg + i;
+            if (xx >= W) continue;
+            const size_t p = ((size_t)n * H + yy) * W + xx;
+#pragma unroll
+            for (int g = 0; g < NCG; ++g) {
+                float v = acc[r][g][i] + bs[g];
+                if (RELU) v = fmaxf(v, 0.f);
+                if (OUT_BF16) static_cast<__bf16*>(Y)[p * cout + co0 + 16 * g + col] = __bf16(v);
+                else static_cast<float*>(Y)[p * cout + co0 + 16 * g + col] = v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient of a 3x3 convolution: dW[co][tap][ci] += sum over pixels p
 // of dY[p][co] X[p + off(tap)][ci].  A workgroup owns 32 output x 32 input
 // channels of one tap over a chunk of pixels.  Per iteration it stages 128
 // pixels of both operands transposed in LDS ([channel][pixel]); wavefront w
 // reduces pixels 32w..32w+31 into the whole 32 x 32 block (four MFMA tiles),
 // so the pixel index is the MFMA's k dimension.  The four wavefronts' sums
-// are combined in LDS and added to dW atomically (FP32).
+// are combined in LDS and added to dW atomically (FP32).  The input may be
+// the decoder's concatenation [upsample2(U), X] (cs channels from U, a
+// multiple of 32; cs = 0: X only).
 // ---------------------------------------------------------------------------
 constexpr int kWgPix = 128;
 constexpr int kWgChunk = 8192;
 
-__global__ void __launch_bounds__(kB) k_wgrad(const __bf16* __restrict__ dY, const __bf16* __restrict__ X,
-                                              float* __restrict__ dW, int N, int H, int W, int cin_pad, int cout)
+__global__ void __launch_bounds__(kB) k_wgrad(const __bf16* __restrict__ dY, const __bf16* __restrict__ U, int cs,
+                                              const __bf16* __restrict__ X, float* __restrict__ dW, int N, int H,
+                                              int W, int cin_pad, int cout)
 {
     __shared__ __align__(16) __bf16 At[32][kWgPix + 8];  // [co][pixel]
     __shared__ __align__(16) __bf16 Bt[32][kWgPix + 8];  // [ci][pixel]
@@ -140,7 +244,9 @@ __global__ void __launch_bounds__(kB) k_wgrad(const __bf16* __restrict__ dY, con
             const int yy = y + dy, xx = x + dx;
             const int c = ci0 + h16;
             if (yy >= 0 && yy < H && xx >= 0 && xx < W && c < cin_pad) {
-                const __bf16* xs = X + ((((size_t)n * H + yy) * W + xx) * cin_pad + c);
+                // channels below cs: the upsampled half-resolution input U
+                const __bf16* xs = c < cs ? U + ((((size_t)n * (H >> 1) + (yy >> 1)) * (W >> 1) + (xx >> 1)) * cs + c)
+                                          : X + ((((size_t)n * H + yy) * W + xx) * (cin_pad - cs) + (c - cs));
                 b0 = *reinterpret_cast<const bf16x8*>(xs);
                 if (c + 8 < cin_pad) b1 = *reinterpret_cast<const bf16x8*>(xs + 8);
             }
@@ -256,23 +362,6 @@ __global__ void k_pool_bwd(const float* __restrict__ gy, int N, int H, int W, in
     const int y = static_cast<int>(t % H);
     const int n = static_cast<int>(t / H);
     gx[i] += 0.25f * gy[(((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
-}
-
-// u = [nearest-upsampled coarse (Cs channels), skip (Ck channels)]
-__global__ void k_up_concat(const __bf16* __restrict__ coarse, const __bf16* __restrict__ skip, int N, int H, int W,
-                            int Cs, int Ck, __bf16* __restrict__ u)
-{
-    const int C = Cs + Ck;
-    const long long total = (long long)N * H * W * C;
-    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int c = static_cast<int>(i % C);
-    const long long p = i / C;
-    const int x = static_cast<int>(p % W);
-    const long long t = p / W;
-    const int y = static_cast<int>(t % H);
-    const int n = static_cast<int>(t / H);
-    u[i] = c < Cs ? coarse[(((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * Cs + c] : skip[p * Ck + (c - Cs)];
 }
 
 // concat backward, coarse part: g_c[y][x] = sum of the 4 fine gradients
@@ -416,13 +505,13 @@ __global__ void k_zero(float* __restrict__ a, long long n)
 }
 
 // stored activations: 0 the packed input; per level l the two encoder
-// outputs, the pooled encoder output, the concatenated decoder input and
-// the two decoder outputs
+// outputs, the pooled encoder output, (unused slot) and the two decoder
+// outputs; the decoder's concatenated input is never stored (its
+// convolutions read the upsampled coarse tensor and the skip in place)
 inline int S_X0() { return 0; }
 inline int S_EA(int l) { return 1 + 6 * l; }
 inline int S_EB(int l) { return 2 + 6 * l; }
 inline int S_P(int l) { return 3 + 6 * l; }
-inline int S_U(int l) { return 4 + 6 * l; }
 inline int S_DA(int l) { return 5 + 6 * l; }
 inline int S_DB(int l) { return 6 + 6 * l; }
 
@@ -551,12 +640,38 @@ void UNet::get_grads(float* host) const
     LSDTT_HIP(hipMemcpy(host, grads.p, sizeof(float) * nparam_, hipMemcpyDeviceToHost));
 }
 
-void UNet::conv(const ConvLayer& L, size_t wo, const __bf16* x, int N, int H, int W, __bf16* y, hipStream_t s)
+namespace {
+// y = conv3x3([upsample2(u), x]) (+ bias, ReLU), u with cs channels (cs = 0:
+// no u): the LDS block kernel when cin is a multiple of 32 (64-channel
+// output blocks when cout allows), the direct kernel otherwise (the
+// 8-channel input layer, which has no u)
+template <bool RELU, bool OUT_BF16>
+void launch_conv(const __bf16* u, int cs, const __bf16* x, const __bf16* w, const float* bias, void* y, int N, int H,
+                 int W, int cin, int cout, int kpad, hipStream_t s)
 {
-    const long long P = (long long)N * H * W;
-    dim3 g((unsigned)((P + 63) / 64), (unsigned)(L.cout / 32));
-    k_conv<true, true><<<g, kB, 0, s>>>(x, wbf.p + wo, params.p + L.boff, y, N, H, W, L.cin_pad, L.cout, L.kpad);
+    if (cin % 32 == 0 && cs % 32 == 0) {
+        const int tx = (W + kTile - 1) / kTile, ty = (H + kTile - 1) / kTile;
+        const unsigned nb = (unsigned)((long long)tx * ty * N);
+        if (cout % 64 == 0)
+            k_conv_tile<RELU, OUT_BF16, 4><<<dim3(nb, (unsigned)(cout / 64)), kB, 0, s>>>(
+                u, cs, x, w, bias, y, N, H, W, cin, cout, kpad, tx, ty);
+        else
+            k_conv_tile<RELU, OUT_BF16, 2><<<dim3(nb, (unsigned)(cout / 32)), kB, 0, s>>>(
+                u, cs, x, w, bias, y, N, H, W, cin, cout, kpad, tx, ty);
+    } else {
+        if (cs != 0) throw std::logic_error("conv3x3: an upsampled input needs cin and cs multiples of 32");
+        const long long P = (long long)N * H * W;
+        dim3 g((unsigned)((P + 63) / 64), (unsigned)(cout / 32));
+        k_conv<RELU, OUT_BF16><<<g, kB, 0, s>>>(x, w, bias, y, N, H, W, cin, cout, kpad);
+    }
     LSDTT_LAUNCH_CHECK();
+}
+}  // namespace
+
+void UNet::conv(const ConvLayer& L, size_t wo, const __bf16* u, int cs, const __bf16* x, int N, int H, int W,
+                __bf16* y, hipStream_t s)
+{
+    launch_conv<true, true>(u, cs, x, wbf.p + wo, params.p + L.boff, y, N, H, W, L.cin_pad, L.cout, L.kpad, s);
 }
 
 void UNet::forward(const float* x, int N, int H, int W, float* y, hipStream_t s)
@@ -579,7 +694,6 @@ void UNet::forward(const float* x, int N, int H, int W, float* y, hipStream_t s)
         act[S_EB(l)].reserve(px(l) * c);
         if (l < L - 1) {
             act[S_P(l)].reserve(px(l + 1) * c);
-            act[S_U(l)].reserve(px(l) * 3 * c);
             act[S_DA(l)].reserve(px(l) * c);
             act[S_DB(l)].reserve(px(l) * c);
         }
@@ -595,18 +709,18 @@ void UNet::forward(const float* x, int N, int H, int W, float* y, hipStream_t s)
     for (int l = 0; l < L; ++l) {
         const int Hl = H >> l, Wl = W >> l;
         const __bf16* in = l == 0 ? act[S_X0()].p : act[S_P(l - 1)].p;
-        conv(enc_a[l], wbf_off[l], in, N, Hl, Wl, act[S_EA(l)].p, s);
-        conv(enc_b[l], wbf_off[L + l], act[S_EA(l)].p, N, Hl, Wl, act[S_EB(l)].p, s);
+        conv(enc_a[l], wbf_off[l], nullptr, 0, in, N, Hl, Wl, act[S_EA(l)].p, s);
+        conv(enc_b[l], wbf_off[L + l], nullptr, 0, act[S_EA(l)].p, N, Hl, Wl, act[S_EB(l)].p, s);
         if (l < L - 1)
             k_pool<<<nblk(px(l + 1) * (base_ << l)), kB, 0, s>>>(act[S_EB(l)].p, N, Hl, Wl, base_ << l,
                                                                  act[S_P(l)].p);
     }
     for (int l = L - 2; l >= 0; --l) {
-        const int Hl = H >> l, Wl = W >> l, c = base_ << l, cs = base_ << (l + 1);
+        const int Hl = H >> l, Wl = W >> l, cs = base_ << (l + 1);
         const __bf16* coarse = (l == L - 2) ? act[S_EB(L - 1)].p : act[S_DB(l + 1)].p;
-        k_up_concat<<<nblk(px(l) * (cs + c)), kB, 0, s>>>(coarse, act[S_EB(l)].p, N, Hl, Wl, cs, c, act[S_U(l)].p);
-        conv(dec_a[l], wbf_off[2 * L + l], act[S_U(l)].p, N, Hl, Wl, act[S_DA(l)].p, s);
-        conv(dec_b[l], wbf_off[3 * L - 1 + l], act[S_DA(l)].p, N, Hl, Wl, act[S_DB(l)].p, s);
+        // input [upsample2(coarse), e_b[l]], read in place by the convolution
+        conv(dec_a[l], wbf_off[2 * L + l], coarse, cs, act[S_EB(l)].p, N, Hl, Wl, act[S_DA(l)].p, s);
+        conv(dec_b[l], wbf_off[3 * L - 1 + l], nullptr, 0, act[S_DA(l)].p, N, Hl, Wl, act[S_DB(l)].p, s);
     }
     if (time_convs) LSDTT_HIP(hipEventRecord(e1, s));
     k_head<<<nblk(px(0)), kB, 0, s>>>(act[S_DB(0)].p, px(0), base_, params.p + head_w, params.p + head_b, out.p);
@@ -630,21 +744,20 @@ void UNet::zero_grad(hipStream_t s)
 
 // one convolution's backward: g_out (FP32, gradient at the post-ReLU output
 // y) -> weight and bias gradients, and g_in (FP32, gradient at x) unless null
-void UNet::conv_backward(const ConvLayer& L, size_t wo, const __bf16* x, const __bf16* y, int N, int H, int W,
-                         const float* g_out, float* g_in, hipStream_t s)
+void UNet::conv_backward(const ConvLayer& L, size_t wo, const __bf16* u, int cs, const __bf16* x, const __bf16* y,
+                         int N, int H, int W, const float* g_out, float* g_in, hipStream_t s)
 {
     const long long P = (long long)N * H * W;
     k_mask_grad<<<nblk(P * L.cout), kB, 0, s>>>(g_out, y, P * L.cout, gb.p);
     dim3 gw((unsigned)(L.cout / 32), (unsigned)(9 * ((L.cin_pad + 31) / 32)),
             (unsigned)((P + kWgChunk - 1) / kWgChunk));
-    k_wgrad<<<gw, kB, 0, s>>>(gb.p, x, grads.p + L.woff, N, H, W, L.cin_pad, L.cout);
+    k_wgrad<<<gw, kB, 0, s>>>(gb.p, u, cs, x, grads.p + L.woff, N, H, W, L.cin_pad, L.cout);
     k_chan_sum<<<std::min(nblk(P * L.cout), 1024), kB, 0, s>>>(gb.p, nullptr, P, L.cout, grads.p + L.boff);
     if (g_in) {
         if (L.cin_pad % 32 != 0) throw std::logic_error("UNet: input gradient needs cin_pad % 32 == 0");
         k_flip<<<nblk((long long)L.cin_pad * 9 * L.cout), kB, 0, s>>>(wbf.p + wo, L.cout, L.cin_pad, L.kpad,
                                                                       wflip.p);
-        dim3 g((unsigned)((P + 63) / 64), (unsigned)(L.cin_pad / 32));
-        k_conv<false, false><<<g, kB, 0, s>>>(gb.p, wflip.p, nullptr, g_in, N, H, W, L.cout, L.cin_pad, 9 * L.cout);
+        launch_conv<false, false>(nullptr, 0, gb.p, wflip.p, nullptr, g_in, N, H, W, L.cout, L.cin_pad, 9 * L.cout, s);
     }
     LSDTT_LAUNCH_CHECK();
 }
@@ -674,8 +787,11 @@ double UNet::backward(const float* target, const uint8_t* loss_mask, hipStream_t
     // g0 holds the gradient at d_b[l]
     for (int l = 0; l <= L - 2; ++l) {
         const int Hl = H >> l, Wl = W >> l, c = base_ << l, cs = base_ << (l + 1);
-        conv_backward(dec_b[l], wbf_off[3 * L - 1 + l], act[S_DA(l)].p, act[S_DB(l)].p, N, Hl, Wl, g0.p, g1.p, s);
-        conv_backward(dec_a[l], wbf_off[2 * L + l], act[S_U(l)].p, act[S_DA(l)].p, N, Hl, Wl, g1.p, g2.p, s);
+        const __bf16* coarse = (l == L - 2) ? act[S_EB(L - 1)].p : act[S_DB(l + 1)].p;
+        conv_backward(dec_b[l], wbf_off[3 * L - 1 + l], nullptr, 0, act[S_DA(l)].p, act[S_DB(l)].p, N, Hl, Wl, g0.p,
+                      g1.p, s);
+        conv_backward(dec_a[l], wbf_off[2 * L + l], coarse, cs, act[S_EB(l)].p, act[S_DA(l)].p, N, Hl, Wl, g1.p,
+                      g2.p, s);
         k_up_bwd_skip<<<nblk(px(l) * c), kB, 0, s>>>(g2.p, px(l), cs, c, gskip[l].p);
         // the coarse part is the gradient at d_b[l + 1] (e_b[L - 1] at the bottom)
         k_up_bwd_coarse<<<nblk(px(l + 1) * cs), kB, 0, s>>>(g2.p, N, Hl, Wl, cs, c, g0.p);
@@ -685,12 +801,15 @@ double UNet::backward(const float* target, const uint8_t* loss_mask, hipStream_t
     const float* gcur = g0.p;
     for (int l = L - 1; l >= 0; --l) {
         const int Hl = H >> l, Wl = W >> l;
-        conv_backward(enc_b[l], wbf_off[L + l], act[S_EA(l)].p, act[S_EB(l)].p, N, Hl, Wl, gcur, g1.p, s);
+        conv_backward(enc_b[l], wbf_off[L + l], nullptr, 0, act[S_EA(l)].p, act[S_EB(l)].p, N, Hl, Wl, gcur, g1.p,
+                      s);
         if (l == 0) {
-            conv_backward(enc_a[0], wbf_off[0], act[S_X0()].p, act[S_EA(0)].p, N, Hl, Wl, g1.p, nullptr, s);
+            conv_backward(enc_a[0], wbf_off[0], nullptr, 0, act[S_X0()].p, act[S_EA(0)].p, N, Hl, Wl, g1.p, nullptr,
+                          s);
         } else {
             // gradient at the pooled e_b[l - 1], then added to its skip gradient
-            conv_backward(enc_a[l], wbf_off[l], act[S_P(l - 1)].p, act[S_EA(l)].p, N, Hl, Wl, g1.p, g2.p, s);
+            conv_backward(enc_a[l], wbf_off[l], nullptr, 0, act[S_P(l - 1)].p, act[S_EA(l)].p, N, Hl, Wl, g1.p, g2.p,
+                          s);
             const int c = base_ << (l - 1);
             k_pool_bwd<<<nblk(px(l - 1) * c), kB, 0, s>>>(g2.p, N, H >> (l - 1), W >> (l - 1), c, gskip[l - 1].p);
             LSDTT_LAUNCH_CHECK();
@@ -728,10 +847,8 @@ void conv3x3_layer(const float* x, const float* w, const float* b, int N, int H,
     LSDTT_HIP(hipMemcpy(fb.p, b, sizeof(float) * cout, hipMemcpyHostToDevice));
     k_pack_input<<<nblk(P * cin_pad), kB>>>(fx.p, P, cin, cin_pad, bx.p);
     k_to_bf16<<<nblk((long long)cout * kpad), kB>>>(fw.p, cout, 9 * cin_pad, kpad, bw.p);
-    dim3 g((unsigned)((P + 63) / 64), (unsigned)(cout / 32));
-    if (relu) k_conv<true, true><<<g, kB>>>(bx.p, bw.p, fb.p, by.p, N, H, W, cin_pad, cout, kpad);
-    else k_conv<false, true><<<g, kB>>>(bx.p, bw.p, fb.p, by.p, N, H, W, cin_pad, cout, kpad);
-    LSDTT_LAUNCH_CHECK();
+    if (relu) launch_conv<true, true>(nullptr, 0, bx.p, bw.p, fb.p, by.p, N, H, W, cin_pad, cout, kpad, nullptr);
+    else launch_conv<false, true>(nullptr, 0, bx.p, bw.p, fb.p, by.p, N, H, W, cin_pad, cout, kpad, nullptr);
     {
         std::vector<__bf16> hy((size_t)P * cout);
         LSDTT_HIP(hipMemcpy(hy.data(), by.p, sizeof(__bf16) * hy.size(), hipMemcpyDeviceToHost));
@@ -747,14 +864,14 @@ void conv3x3_layer(const float* x, const float* w, const float* b, int N, int H,
     if (relu) k_mask_grad<<<nblk(P * cout), kB>>>(fgy.p, by.p, P * cout, bgm.p);
     else k_pack_input<<<nblk(P * cout), kB>>>(fgy.p, P * cout, 1, 1, bgm.p);
     dim3 gw2((unsigned)(cout / 32), (unsigned)(9 * ((cin_pad + 31) / 32)), (unsigned)((P + kWgChunk - 1) / kWgChunk));
-    k_wgrad<<<gw2, kB>>>(bgm.p, bx.p, fg.p, N, H, W, cin_pad, cout);
+    k_wgrad<<<gw2, kB>>>(bgm.p, nullptr, 0, bx.p, fg.p, N, H, W, cin_pad, cout);
     k_chan_sum<<<std::min(nblk(P * cout), 1024), kB>>>(bgm.p, nullptr, P, cout, fg.p + (size_t)cout * 9 * cin_pad);
     if (gx) {
         fgx.reserve(P * cin_pad);
         bflip.reserve((size_t)cin_pad * 9 * cout);
         k_flip<<<nblk((long long)cin_pad * 9 * cout), kB>>>(bw.p, cout, cin_pad, kpad, bflip.p);
-        dim3 gg((unsigned)((P + 63) / 64), (unsigned)(cin_pad / 32));
-        k_conv<false, false><<<gg, kB>>>(bgm.p, bflip.p, nullptr, fgx.p, N, H, W, cout, cin_pad, 9 * cout);
+        launch_conv<false, false>(nullptr, 0, bgm.p, bflip.p, nullptr, fgx.p, N, H, W, cout, cin_pad, 9 * cout,
+                                  nullptr);
     }
     LSDTT_LAUNCH_CHECK();
     LSDTT_HIP(hipDeviceSynchronize());

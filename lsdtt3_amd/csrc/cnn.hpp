@@ -92,9 +92,11 @@ private:
 
     ConvLayer make_layer(int cin, int cout);
     void convert_weights(hipStream_t s);
-    void conv(const ConvLayer& L, size_t wo, const __bf16* x, int N, int H, int W, __bf16* y, hipStream_t s);
-    void conv_backward(const ConvLayer& L, size_t wo, const __bf16* x, const __bf16* y, int N, int H, int W,
-                       const float* g_out, float* g_in, hipStream_t s);
+    // input [upsample2(u) (cs channels), x] (u null, cs 0: x only)
+    void conv(const ConvLayer& L, size_t wo, const __bf16* u, int cs, const __bf16* x, int N, int H, int W, __bf16* y,
+              hipStream_t s);
+    void conv_backward(const ConvLayer& L, size_t wo, const __bf16* u, int cs, const __bf16* x, const __bf16* y,
+                       int N, int H, int W, const float* g_out, float* g_in, hipStream_t s);
 };
 
 // One 3x3 convolution of the network on host arrays, through the same

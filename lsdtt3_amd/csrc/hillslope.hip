@@ -273,17 +273,71 @@ __device__ __forceinline__ void J_fast(const HsDom& d, const double z[9], T c9[9
     }
 }
 
-__device__ __forceinline__ void load9(const double* __restrict__ z, int f, int nc, double out[9])
+// ---------------------------------------------------------------------------
+// LDS tiles for the stencil kernels: a workgroup walks 64 x 16 cell tiles
+// (persistent grid); per tile it stages z (or z + lam dz) and, for the
+// Jacobian products, the input vector over the tile plus a one-cell halo
+// with coalesced row loads, then each wavefront evaluates one tile row per
+// step from LDS.  Halo cells outside the grid are never read: a fast cell's
+// 3 x 3 block is interior.
+// ---------------------------------------------------------------------------
+constexpr int kTW = 64, kTH = 16, kSW = kTW + 2, kSH = kTH + 2;
+
+// Branch-free: every thread issues all of its loads (indices clamped into
+// the grid; a clamped value lands in a halo slot no fast cell reads) before
+// the first LDS store, so kStage loads are in flight at once (a load under
+// a branch would be followed by a full wait for it).
+template <bool HAS_DZ, bool HAS_X>
+__device__ __forceinline__ void stage_tile(const double* __restrict__ z, const float* __restrict__ dz, double lam,
+                                           const float* __restrict__ x, int nr, int nc, int r0, int c0,
+                                           double* __restrict__ zs, float* __restrict__ xs)
+{
+    constexpr int kStage = (kSH * kSW + kB - 1) / kB;
+    double zv[kStage];
+    float dv[kStage], xv[kStage];
+#pragma unroll
+    for (int j = 0; j < kStage; ++j) {
+        const int i = min(static_cast<int>(threadIdx.x) + j * kB, kSH * kSW - 1);
+        const int rr = min(max(r0 - 1 + i / kSW, 0), nr - 1), cc = min(max(c0 - 1 + i % kSW, 0), nc - 1);
+        const long long g = (long long)rr * nc + cc;
+        zv[j] = z[g];
+        if (HAS_DZ) dv[j] = dz[g];
+        if (HAS_X) xv[j] = x[g];
+    }
+#pragma unroll
+    for (int j = 0; j < kStage; ++j) {
+        const int i = threadIdx.x + j * kB;
+        if (i >= kSH * kSW) continue;
+        zs[i] = HAS_DZ ? zv[j] + lam * static_cast<double>(dv[j]) : zv[j];  // ZView's expression
+        if (HAS_X) xs[i] = xv[j];
+    }
+}
+
+__device__ __forceinline__ void tile_z9(const double* __restrict__ zs, int rl, int cl, double zz[9])
 {
 #pragma unroll
-    for (int k = 0; k < 9; ++k) out[k] = z[f + (k / 3 - 1) * nc + (k % 3 - 1)];
+    for (int q = 0; q < 9; ++q) zz[q] = zs[(rl + q / 3) * kSW + cl + q % 3];
+}
+
+// y = J x (FP32) on a fast row, from the staged tile (jx_fast's arithmetic)
+__device__ __forceinline__ float jx_tile(const HsDom& d, const double* __restrict__ zs, const float* __restrict__ xs,
+                                         int rl, int cl)
+{
+    double zz[9];
+    tile_z9(zs, rl, cl, zz);
+    float c9[9];
+    J_fast<float>(d, zz, c9);
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc += c9[k] * xs[(rl + k / 3) * kSW + cl + k % 3];
+    return acc;
 }
 
 // ---------------------------------------------------------------------------
 // kernels.  Each hot kernel covers the fast cells (and writes zeros on
 // non-DOF cells); a companion kernel over the list of general DOF cells
 // runs first and leaves its block partials after the hot kernel's, which
-// the hot kernel's last block folds in (grid_reduce `extra`).
+// the hot kernel's one-block finisher folds in after the hot kernel's own.
 // ---------------------------------------------------------------------------
 constexpr int kGenBlocks = 64;
 
@@ -305,20 +359,6 @@ __global__ void __launch_bounds__(256) k_fast_flags(HsDom d, uint8_t* __restrict
     }
 }
 
-// y = J x (FP32) on a fast row
-__device__ __forceinline__ float jx_fast(const HsDom& d, const double* __restrict__ z, const float* __restrict__ x,
-                                         long long f)
-{
-    double zz[9];
-    load9(z, static_cast<int>(f), d.nc, zz);
-    float c9[9];
-    J_fast<float>(d, zz, c9);
-    float acc = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) acc += c9[k] * x[f + (k / 3 - 1) * d.nc + (k % 3 - 1)];
-    return acc;
-}
-
 // y = J x on general row i (x is zero on non-DOF cells), from the stored row
 __device__ __forceinline__ float jx_gen(const HsDom& d, const float* __restrict__ x, int i, int f)
 {
@@ -329,17 +369,51 @@ __device__ __forceinline__ float jx_gen(const HsDom& d, const float* __restrict_
     return acc;
 }
 
+// The vector updates stream four cells per thread per load (float4: the
+// solver vectors are separate allocations, 16-byte aligned), with a scalar
+// tail when n is not a multiple of 4.
+__device__ __forceinline__ float4 ld4(const float* __restrict__ p, long long i)
+{
+    return reinterpret_cast<const float4*>(p)[i];
+}
+__device__ __forceinline__ void st4(float* __restrict__ p, long long i, float4 v)
+{
+    reinterpret_cast<float4*>(p)[i] = v;
+}
+
 // p = r + beta (p - omega v); phat = M^-1 p  (first iteration: p = b)
 __global__ void __launch_bounds__(256) k_bi_p(BiArgs a)
 {
     if (a.flags[0]) return;
     const double beta = (a.scal[HS_RHO] / a.scal[HS_RHO_OLD]) * (a.scal[HS_ALPHA] / a.scal[HS_OMEGA]);
     const float bt = static_cast<float>(beta), om = static_cast<float>(a.scal[HS_OMEGA]);
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
-         f += (long long)gridDim.x * blockDim.x) {
-        const float pv = a.first ? a.b[f] : a.r[f] + bt * (a.p[f] - om * a.v[f]);
-        a.p[f] = pv;
-        a.phat[f] = a.dinv[f] * pv;
+    const float* __restrict__ b = a.b;
+    const float* __restrict__ r = a.r;
+    const float* __restrict__ v = a.v;
+    const float* __restrict__ dinv = a.dinv;
+    float* __restrict__ p = a.p;
+    float* __restrict__ phat = a.phat;
+    const long long n4 = a.n >> 2, stride = (long long)gridDim.x * blockDim.x;
+    const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (long long i = t0; i < n4; i += stride) {
+        const float4 di = ld4(dinv, i);
+        float4 pv;
+        if (a.first) {
+            pv = ld4(b, i);
+        } else {
+            const float4 rv = ld4(r, i), po = ld4(p, i), vv = ld4(v, i);
+            pv.x = rv.x + bt * (po.x - om * vv.x);
+            pv.y = rv.y + bt * (po.y - om * vv.y);
+            pv.z = rv.z + bt * (po.z - om * vv.z);
+            pv.w = rv.w + bt * (po.w - om * vv.w);
+        }
+        st4(p, i, pv);
+        st4(phat, i, make_float4(di.x * pv.x, di.y * pv.y, di.z * pv.z, di.w * pv.w));
+    }
+    for (long long f = 4 * n4 + t0; f < a.n; f += stride) {
+        const float pv = a.first ? b[f] : r[f] + bt * (p[f] - om * v[f]);
+        p[f] = pv;
+        phat[f] = dinv[f] * pv;
     }
 }
 
@@ -372,26 +446,64 @@ __global__ void __launch_bounds__(256) k_bi_mv_gen(BiArgs a, const float* __rest
 __global__ void __launch_bounds__(256) k_bi_v(BiArgs a)
 {
     if (a.flags[0]) return;
+    __shared__ double zs[kSH * kSW];
+    __shared__ float xs[kSH * kSW];
+    const int nr = a.d.nr, nc = a.d.nc, tx_n = (nc + kTW - 1) / kTW, nt = tx_n * ((nr + kTH - 1) / kTH);
+    const int cl = threadIdx.x & 63;
+    const uint8_t* __restrict__ fast = a.d.fast;
+    const uint8_t* __restrict__ dof = a.d.dof;
+    const float* __restrict__ rhat = a.rhat;
+    float* __restrict__ vout = a.v;
     double acc[1] = {0.0};
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
-         f += (long long)gridDim.x * blockDim.x) {
-        if (a.d.fast[f]) {
-            const float vv = jx_fast(a.d, a.z, a.phat, f);
-            a.v[f] = vv;
-            acc[0] += static_cast<double>(a.rhat[f]) * vv;
-        } else if (!a.d.dof[f]) {
-            a.v[f] = 0.0f;
+    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+        const int r0 = (tile / tx_n) * kTH, c0 = (tile % tx_n) * kTW;
+        __syncthreads();
+        stage_tile<false, true>(a.z, nullptr, 0.0, a.phat, nr, nc, r0, c0, zs, xs);
+        // per-cell inputs of this thread's four rows, loaded before the
+        // barrier (they do not depend on the tile)
+        long long f[kTH / 4];
+        bool in[kTH / 4];
+        uint8_t fl[kTH / 4], dl[kTH / 4];
+        float rh[kTH / 4];
+#pragma unroll
+        for (int k = 0; k < kTH / 4; ++k) {
+            const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
+            in[k] = rr < nr && cc < nc;
+            f[k] = (long long)min(rr, nr - 1) * nc + min(cc, nc - 1);
+            fl[k] = fast[f[k]];
+            dl[k] = dof[f[k]];
+            rh[k] = rhat[f[k]];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTH / 4; ++k) {
+            if (!in[k]) continue;
+            if (fl[k]) {
+                const float vv = jx_tile(a.d, zs, xs, (threadIdx.x >> 6) + 4 * k, cl);
+                vout[f[k]] = vv;
+                acc[0] += static_cast<double>(rh[k]) * vv;
+            } else if (!dl[k]) {
+                vout[f[k]] = 0.0f;
+            }
         }
     }
     const bool mx[1] = {false};
+    block_partial<1>(acc, mx, a.partials, blockIdx.x);
+}
+
+// alpha = rho / (rhat . v) from the partials of k_bi_v and its companion
+__global__ void __launch_bounds__(256) k_bi_v_fin(BiArgs a, unsigned count)
+{
+    if (a.flags[0]) return;
+    const bool mx[1] = {false};
     double out[1];
-    if (grid_reduce<1>(acc, mx, a.partials, a.ticket, out, gen_grid(a.d.n_gen))) {
-        if (out[0] == 0.0) {
-            a.flags[0] = 1;
-            a.flags[2] = 1;
-        } else {
-            a.scal[HS_ALPHA] = a.scal[HS_RHO] / out[0];
-        }
+    fold_partials<1>(a.partials, count, mx, out);
+    if (threadIdx.x != 0) return;
+    if (out[0] == 0.0) {
+        a.flags[0] = 1;
+        a.flags[2] = 1;
+    } else {
+        a.scal[HS_ALPHA] = a.scal[HS_RHO] / out[0];
     }
 }
 
@@ -401,11 +513,22 @@ __global__ void __launch_bounds__(256) k_bi_s(BiArgs a)
     if (a.flags[0]) return;
     const float al = static_cast<float>(a.scal[HS_ALPHA]);
     const float* __restrict__ r = a.first ? a.b : a.r;
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
-         f += (long long)gridDim.x * blockDim.x) {
-        const float sv = r[f] - al * a.v[f];
-        a.s[f] = sv;
-        a.shat[f] = a.dinv[f] * sv;
+    const float* __restrict__ v = a.v;
+    const float* __restrict__ dinv = a.dinv;
+    float* __restrict__ sv = a.s;
+    float* __restrict__ shat = a.shat;
+    const long long n4 = a.n >> 2, stride = (long long)gridDim.x * blockDim.x;
+    const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (long long i = t0; i < n4; i += stride) {
+        const float4 rv = ld4(r, i), vv = ld4(v, i), di = ld4(dinv, i);
+        const float4 s4 = make_float4(rv.x - al * vv.x, rv.y - al * vv.y, rv.z - al * vv.z, rv.w - al * vv.w);
+        st4(sv, i, s4);
+        st4(shat, i, make_float4(di.x * s4.x, di.y * s4.y, di.z * s4.z, di.w * s4.w));
+    }
+    for (long long f = 4 * n4 + t0; f < a.n; f += stride) {
+        const float s1 = r[f] - al * v[f];
+        sv[f] = s1;
+        shat[f] = dinv[f] * s1;
     }
 }
 
@@ -413,22 +536,58 @@ __global__ void __launch_bounds__(256) k_bi_s(BiArgs a)
 __global__ void __launch_bounds__(256) k_bi_t(BiArgs a)
 {
     if (a.flags[0]) return;
+    __shared__ double zs[kSH * kSW];
+    __shared__ float xs[kSH * kSW];
+    const int nr = a.d.nr, nc = a.d.nc, tx_n = (nc + kTW - 1) / kTW, nt = tx_n * ((nr + kTH - 1) / kTH);
+    const int cl = threadIdx.x & 63;
+    const uint8_t* __restrict__ fast = a.d.fast;
+    const uint8_t* __restrict__ dof = a.d.dof;
+    const float* __restrict__ sv = a.s;
+    float* __restrict__ tout = a.t;
     double acc[2] = {0.0, 0.0};
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
-         f += (long long)gridDim.x * blockDim.x) {
-        if (a.d.fast[f]) {
-            const float tv = jx_fast(a.d, a.z, a.shat, f);
-            a.t[f] = tv;
-            acc[0] += static_cast<double>(tv) * a.s[f];
-            acc[1] += static_cast<double>(tv) * tv;
-        } else if (!a.d.dof[f]) {
-            a.t[f] = 0.0f;
+    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+        const int r0 = (tile / tx_n) * kTH, c0 = (tile % tx_n) * kTW;
+        __syncthreads();
+        stage_tile<false, true>(a.z, nullptr, 0.0, a.shat, nr, nc, r0, c0, zs, xs);
+        long long f[kTH / 4];
+        bool in[kTH / 4];
+        uint8_t fl[kTH / 4], dl[kTH / 4];
+        float sl[kTH / 4];
+#pragma unroll
+        for (int k = 0; k < kTH / 4; ++k) {
+            const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
+            in[k] = rr < nr && cc < nc;
+            f[k] = (long long)min(rr, nr - 1) * nc + min(cc, nc - 1);
+            fl[k] = fast[f[k]];
+            dl[k] = dof[f[k]];
+            sl[k] = sv[f[k]];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTH / 4; ++k) {
+            if (!in[k]) continue;
+            if (fl[k]) {
+                const float tv = jx_tile(a.d, zs, xs, (threadIdx.x >> 6) + 4 * k, cl);
+                tout[f[k]] = tv;
+                acc[0] += static_cast<double>(tv) * sl[k];
+                acc[1] += static_cast<double>(tv) * tv;
+            } else if (!dl[k]) {
+                tout[f[k]] = 0.0f;
+            }
         }
     }
     const bool mx[2] = {false, false};
+    block_partial<2>(acc, mx, a.partials, blockIdx.x);
+}
+
+// omega = (t . s) / (t . t)
+__global__ void __launch_bounds__(256) k_bi_t_fin(BiArgs a, unsigned count)
+{
+    if (a.flags[0]) return;
+    const bool mx[2] = {false, false};
     double out[2];
-    if (grid_reduce<2>(acc, mx, a.partials, a.ticket, out, gen_grid(a.d.n_gen)))
-        a.scal[HS_OMEGA] = out[1] > 0.0 ? out[0] / out[1] : 0.0;
+    fold_partials<2>(a.partials, count, mx, out);
+    if (threadIdx.x == 0) a.scal[HS_OMEGA] = out[1] > 0.0 ? out[0] / out[1] : 0.0;
 }
 
 // x += alpha phat + omega shat; r = s - omega t; convergence on ||r||
@@ -436,19 +595,53 @@ __global__ void __launch_bounds__(256) k_bi_x(BiArgs a)
 {
     if (a.flags[0]) return;
     const float al = static_cast<float>(a.scal[HS_ALPHA]), om = static_cast<float>(a.scal[HS_OMEGA]);
+    const float* __restrict__ phat = a.phat;
+    const float* __restrict__ shat = a.shat;
+    const float* __restrict__ sv = a.s;
+    const float* __restrict__ tv = a.t;
+    const float* __restrict__ rhat = a.rhat;
+    float* __restrict__ x = a.x;
+    float* __restrict__ r = a.r;
     double acc[2] = {0.0, 0.0};
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < a.n;
-         f += (long long)gridDim.x * blockDim.x) {
-        const float dx = al * a.phat[f] + om * a.shat[f];
-        a.x[f] = a.first ? dx : a.x[f] + dx;
-        const float rv = a.s[f] - om * a.t[f];
-        a.r[f] = rv;
-        acc[0] += static_cast<double>(a.rhat[f]) * rv;
+    const long long n4 = a.n >> 2, stride = (long long)gridDim.x * blockDim.x;
+    const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (long long i = t0; i < n4; i += stride) {
+        const float4 ph = ld4(phat, i), sh = ld4(shat, i), s4 = ld4(sv, i), t4 = ld4(tv, i), rh = ld4(rhat, i);
+        const float4 xo = a.first ? make_float4(0.f, 0.f, 0.f, 0.f) : ld4(x, i);
+        const float4 dx = make_float4(al * ph.x + om * sh.x, al * ph.y + om * sh.y, al * ph.z + om * sh.z,
+                                      al * ph.w + om * sh.w);
+        st4(x, i, a.first ? dx : make_float4(xo.x + dx.x, xo.y + dx.y, xo.z + dx.z, xo.w + dx.w));
+        const float4 rv = make_float4(s4.x - om * t4.x, s4.y - om * t4.y, s4.z - om * t4.z, s4.w - om * t4.w);
+        st4(r, i, rv);
+        acc[0] += static_cast<double>(rh.x) * rv.x;
+        acc[0] += static_cast<double>(rh.y) * rv.y;
+        acc[0] += static_cast<double>(rh.z) * rv.z;
+        acc[0] += static_cast<double>(rh.w) * rv.w;
+        acc[1] += static_cast<double>(rv.x) * rv.x;
+        acc[1] += static_cast<double>(rv.y) * rv.y;
+        acc[1] += static_cast<double>(rv.z) * rv.z;
+        acc[1] += static_cast<double>(rv.w) * rv.w;
+    }
+    for (long long f = 4 * n4 + t0; f < a.n; f += stride) {
+        const float dx = al * phat[f] + om * shat[f];
+        x[f] = a.first ? dx : x[f] + dx;
+        const float rv = sv[f] - om * tv[f];
+        r[f] = rv;
+        acc[0] += static_cast<double>(rhat[f]) * rv;
         acc[1] += static_cast<double>(rv) * rv;
     }
     const bool mx[2] = {false, false};
+    block_partial<2>(acc, mx, a.partials, blockIdx.x);
+}
+
+// rho, ||r||^2 and the convergence / breakdown flags
+__global__ void __launch_bounds__(256) k_bi_x_fin(BiArgs a, unsigned count)
+{
+    if (a.flags[0]) return;
+    const bool mx[2] = {false, false};
     double out[2];
-    if (grid_reduce<2>(acc, mx, a.partials, a.ticket, out)) {
+    fold_partials<2>(a.partials, count, mx, out);
+    if (threadIdx.x == 0) {
         a.scal[HS_RHO_OLD] = a.scal[HS_RHO];
         a.scal[HS_RHO] = out[0];
         a.scal[HS_RNORM2] = out[1];
@@ -493,36 +686,67 @@ __global__ void __launch_bounds__(256) k_trial(HsDom d, const double* __restrict
                                                float* __restrict__ b, float* __restrict__ dinv, double* partials,
                                                unsigned* ticket, double* scal, int* flags)
 {
-    const ZView Z{z, lam != 0.0 ? dz : nullptr, lam};
+    (void)n;
+    __shared__ double zs[kSH * kSW];
+    const int nr = d.nr, nc = d.nc, tx_n = (nc + kTW - 1) / kTW, nt = tx_n * ((nr + kTH - 1) / kTH);
+    const int cl = threadIdx.x & 63;
     double acc[3] = {0.0, 0.0, 0.0};
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
-         f += (long long)gridDim.x * blockDim.x) {
-        if (!d.fast[f]) {
-            if (!d.dof[f]) {
-                b[f] = 0.0f;
-                dinv[f] = 0.0f;
-            }
-            continue;
-        }
-        double zz[9];
+    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+        const int r0 = (tile / tx_n) * kTH, c0 = (tile % tx_n) * kTW;
+        __syncthreads();
+        if (lam != 0.0) stage_tile<true, false>(z, dz, lam, nullptr, nr, nc, r0, c0, zs, nullptr);
+        else stage_tile<false, false>(z, nullptr, 0.0, nullptr, nr, nc, r0, c0, zs, nullptr);
+        long long f[kTH / 4];
+        bool in[kTH / 4];
+        uint8_t fl[kTH / 4], dl[kTH / 4];
+        double znl[kTH / 4];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) zz[k] = Z(static_cast<int>(f) + (k / 3 - 1) * d.nc + (k % 3 - 1));
-        double diag;
-        const double F = eval_fast(d, zz, d.zn[f], &diag);
-        acc[0] = fmax(acc[0], fabs(F));
-        if (d.nonlinear) {
-            const double sx = (zz[5] - zz[3]) * d.inv_2h, sy = (zz[7] - zz[1]) * d.inv_2h;
-            acc[1] = fmax(acc[1], sqrt(sx * sx + sy * sy) * d.inv_Sc);
+        for (int k = 0; k < kTH / 4; ++k) {
+            const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
+            in[k] = rr < nr && cc < nc;
+            f[k] = (long long)min(rr, nr - 1) * nc + min(cc, nc - 1);
+            fl[k] = d.fast[f[k]];
+            dl[k] = d.dof[f[k]];
+            znl[k] = d.zn[f[k]];
         }
-        const float bf = static_cast<float>(-F);
-        b[f] = bf;
-        dinv[f] = diag != 0.0 ? static_cast<float>(1.0 / diag) : 0.0f;
-        acc[2] += static_cast<double>(bf) * bf;
-        if (lam != 0.0) z_out[f] = zz[4];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTH / 4; ++k) {
+            if (!in[k]) continue;
+            if (!fl[k]) {
+                if (!dl[k]) {
+                    b[f[k]] = 0.0f;
+                    dinv[f[k]] = 0.0f;
+                }
+                continue;
+            }
+            double zz[9];
+            tile_z9(zs, (threadIdx.x >> 6) + 4 * k, cl, zz);
+            double diag;
+            const double F = eval_fast(d, zz, znl[k], &diag);
+            acc[0] = fmax(acc[0], fabs(F));
+            if (d.nonlinear) {
+                const double sx = (zz[5] - zz[3]) * d.inv_2h, sy = (zz[7] - zz[1]) * d.inv_2h;
+                acc[1] = fmax(acc[1], sqrt(sx * sx + sy * sy) * d.inv_Sc);
+            }
+            const float bf = static_cast<float>(-F);
+            b[f[k]] = bf;
+            dinv[f[k]] = diag != 0.0 ? static_cast<float>(1.0 / diag) : 0.0f;
+            acc[2] += static_cast<double>(bf) * bf;
+            if (lam != 0.0) z_out[f[k]] = zz[4];
+        }
     }
     const bool mx[3] = {true, true, false};
+    block_partial<3>(acc, mx, partials, blockIdx.x);
+}
+
+// max |F|, max slope ratio and the Krylov start of the trial's system
+__global__ void __launch_bounds__(256) k_trial_fin(double* partials, unsigned count, double* scal, int* flags)
+{
+    const bool mx[3] = {true, true, false};
     double out[3];
-    if (grid_reduce<3>(acc, mx, partials, ticket, out, gen_grid(d.n_gen))) {
+    fold_partials<3>(partials, count, mx, out);
+    if (threadIdx.x == 0) {
         scal[HS_TRIAL_F] = out[0];
         scal[HS_TRIAL_RATIO] = out[1];
         // Krylov start for the system at this point: r = rhat = b, x = 0
@@ -558,6 +782,7 @@ void hs_bicgstab_iteration(const BiArgs& a, int grid, hipStream_t s, KernelTimer
     if (gg) k_bi_mv_gen<<<gg, kB, 0, s>>>(a, a.phat, a.v, a.rhat, nullptr, 1, grid);
     if (t) t->begin("bicgstab_matvec", s);
     k_bi_v<<<grid, kB, 0, s>>>(a);
+    k_bi_v_fin<<<1, kB, 0, s>>>(a, (unsigned)(grid + gg));
     if (t) t->end(s, 22.0 * n);  // z 8, phat 4, fast 1, rhat 4 in; v 4 out (+ dof 1)
     if (t) t->begin("bicgstab_s", s);
     k_bi_s<<<grid, kB, 0, s>>>(a);
@@ -565,9 +790,11 @@ void hs_bicgstab_iteration(const BiArgs& a, int grid, hipStream_t s, KernelTimer
     if (gg) k_bi_mv_gen<<<gg, kB, 0, s>>>(a, a.shat, a.t, a.s, a.t, 2, grid);
     if (t) t->begin("bicgstab_matvec", s);
     k_bi_t<<<grid, kB, 0, s>>>(a);
+    k_bi_t_fin<<<1, kB, 0, s>>>(a, (unsigned)(grid + gg));
     if (t) t->end(s, 22.0 * n);
     if (t) t->begin("bicgstab_x", s);
     k_bi_x<<<grid, kB, 0, s>>>(a);
+    k_bi_x_fin<<<1, kB, 0, s>>>(a, (unsigned)grid);
     if (t) t->end(s, 32.0 * n);
     LSDTT_LAUNCH_CHECK();
 }
@@ -577,6 +804,7 @@ void hs_trial(const HsDom& d, const double* z, const float* dz, double lam, long
 {
     if (d.n_gen > 0) k_trial_gen<<<gen_grid(d.n_gen), kB, 0, s>>>(d, z, dz, lam, z_out, b, dinv, partials, grid);
     k_trial<<<grid, kB, 0, s>>>(d, z, dz, lam, n, z_out, b, dinv, partials, ticket, scal, flags);
+    k_trial_fin<<<1, kB, 0, s>>>(partials, (unsigned)(grid + gen_grid(d.n_gen)), scal, flags);
     LSDTT_LAUNCH_CHECK();
 }
 

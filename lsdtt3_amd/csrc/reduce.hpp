@@ -99,4 +99,26 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[NV], const bool (&is_max
     return false;
 }
 
+// Folds `count` partial slots in slot order (thread-strided, then the block
+// reduction's fixed order) into w[0..NV), valid in thread 0.  For a
+// one-block finisher kernel launched after the kernel that wrote the
+// partials: the kernel boundary makes them visible, which costs far less
+// than grid_reduce's per-block device-scope fence (on a multi-XCD GPU each
+// such fence writes back the XCD's L2: ~20 ns per block, 44 us for a
+// 2048-block grid).
+template <int NV>
+__device__ __forceinline__ void fold_partials(const double* partials, unsigned count, const bool (&is_max)[NV],
+                                              double (&w)[NV])
+{
+#pragma unroll
+    for (int i = 0; i < NV; ++i) w[i] = is_max[i] ? -1.7976931348623157e308 : 0.0;
+    for (unsigned b = threadIdx.x; b < count; b += blockDim.x)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const double x = partials[(size_t)b * NV + i];
+            w[i] = is_max[i] ? (w[i] > x ? w[i] : x) : w[i] + x;
+        }
+    block_reduce<NV>(w, is_max);
+}
+
 }  // namespace lsdtt_gpu
