@@ -31,6 +31,20 @@ struct BC {
     char n, e, s, w;
 };
 
+// BC as one 32-bit kernel argument: a struct of chars passed by value can
+// be read from the argument segment with a vector load and a full wait;
+// an integer argument stays in scalar registers
+inline uint32_t bc_pack(BC b)
+{
+    return (uint32_t)(uint8_t)b.n | ((uint32_t)(uint8_t)b.e << 8) | ((uint32_t)(uint8_t)b.s << 16) |
+           ((uint32_t)(uint8_t)b.w << 24);
+}
+__host__ __device__ __forceinline__ BC bc_unpack(uint32_t w)
+{
+    return BC{static_cast<char>(w & 0xff), static_cast<char>((w >> 8) & 0xff), static_cast<char>((w >> 16) & 0xff),
+              static_cast<char>(w >> 24)};
+}
+
 // Result of stepping off a cell under the routing boundary rules
 // (lsdtt_flowinfo_cpu.cpp:101-158): kind 0 = blocked (no-flux edge),
 // 1 = lands inside the grid at (r, c), 2 = leaves through an 'o'/'b' edge.

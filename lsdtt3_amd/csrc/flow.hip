@@ -58,44 +58,62 @@ __device__ __forceinline__ void st_relaxed(unsigned long long* p, unsigned long 
 template <class T>
 __global__ void __launch_bounds__(256) k_receivers(const T* __restrict__ z, const uint8_t* __restrict__ mask,
                                                    const uint8_t* __restrict__ forced, int nr, int nc,
-                                                   double card, double diag, BC bc, int nodata_outlet,
+                                                   double card, double diag, uint32_t bcw, int nodata_outlet,
                                                    int pits_outlet, int* __restrict__ rcv,
                                                    uint8_t* __restrict__ dir, int* __restrict__ fail)
 {
+    const BC bc = bc_unpack(bcw);
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = blockIdx.y * blockDim.y + threadIdx.y;
     if (row >= nr || col >= nc) return;
     const int f = row * nc + col;
-    if (!mask[f]) {
+    // every load of the cell is issued before the first branch (neighbour
+    // indices clamped into the grid; edge cells take the general path
+    // below and ignore them), so their latencies overlap instead of each
+    // branch waiting for the load it tests
+    const bool interior = row > 0 && row < nr - 1 && col > 0 && col < nc - 1;
+    uint8_t mk[8];
+    T zn[8];
+    {
+        const int rm = row > 0 ? row - 1 : row, rp = row < nr - 1 ? row + 1 : row;
+        const int cm = col > 0 ? col - 1 : col, cp = col < nc - 1 ? col + 1 : col;
+        // the D8 offsets as compile-time constants (the __constant__
+        // tables would be read from memory)
+        constexpr int kdr[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
+        constexpr int kdc[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            const int rr = kdr[d] < 0 ? rm : (kdr[d] > 0 ? rp : row);
+            const int cc = kdc[d] < 0 ? cm : (kdc[d] > 0 ? cp : col);
+            const int g = rr * nc + cc;
+            mk[d] = mask[g];
+            zn[d] = z[g];
+        }
+    }
+    const uint8_t mc = mask[f];
+    const uint8_t fc = forced ? forced[f] : 0;
+    const double zc = static_cast<double>(z[f]);
+    if (!mc) {
         rcv[f] = kNodataReceiver;
         dir[f] = kNoDirection;
         return;
     }
     const bool bedge = (row == 0 && bc.n == 'b') || (col == nc - 1 && bc.e == 'b') ||
                        (row == nr - 1 && bc.s == 'b') || (col == 0 && bc.w == 'b');
-    if (bedge || (forced && forced[f])) {
+    if (bedge || fc) {
         rcv[f] = kOutletReceiver;
         dir[f] = kNoDirection;
         return;
     }
-    const double zc = static_cast<double>(z[f]);
     double best = 0.0;
     int brcv = kOutletReceiver, bdir = kNoDirection;
     bool allowed = false;
-    if (row > 0 && row < nr - 1 && col > 0 && col < nc - 1) {
-        // all 16 neighbour loads are issued before any is used, so their
-        // latencies overlap instead of serialising on the branches below
-        uint8_t mk[8];
-        T zn[8];
+    if (interior) {
+        constexpr int kdr[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
+        constexpr int kdc[8] = {0, 1, 1, 1, 0, -1, -1, -1};
 #pragma unroll
         for (int d = 0; d < 8; ++d) {
-            const int g = f + d8_row[d] * nc + d8_col[d];
-            mk[d] = mask[g];
-            zn[d] = z[g];
-        }
-#pragma unroll
-        for (int d = 0; d < 8; ++d) {
-            const int g = f + d8_row[d] * nc + d8_col[d];
+            const int g = f + kdr[d] * nc + kdc[d];
             if (!mk[d]) {
                 allowed |= (nodata_outlet != 0);
                 continue;
@@ -529,7 +547,7 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
     dim3 b2(64, 4), g2(div_up(nc, 64), div_up(nr, 4));
     {
         TScope ts_(timer, "receivers", s, (sizeof(T) + 6.0) * n);
-        k_receivers<T><<<g2, b2, 0, s>>>(z, msk, forced, nr, nc, opt.dx, opt.dx * kSqrt2, opt.bc,
+        k_receivers<T><<<g2, b2, 0, s>>>(z, msk, forced, nr, nc, opt.dx, opt.dx * kSqrt2, bc_pack(opt.bc),
                                       opt.interior_nodata_is_outlet, opt.allow_pits_as_outlets, rcv, dir, misc.p);
     }
     LSDTT_LAUNCH_CHECK();
@@ -598,7 +616,7 @@ void FlowEngine::receivers_only(const T* z, const uint8_t* msk, hipStream_t s)
     mask = msk;
     dim3 b2(64, 4), g2(div_up(opt.nc, 64), div_up(opt.nr, 4));
     TScope ts_(timer, "receivers", s, (sizeof(T) + 6.0) * n);
-    k_receivers<T><<<g2, b2, 0, s>>>(z, msk, nullptr, opt.nr, opt.nc, opt.dx, opt.dx * kSqrt2, opt.bc,
+    k_receivers<T><<<g2, b2, 0, s>>>(z, msk, nullptr, opt.nr, opt.nc, opt.dx, opt.dx * kSqrt2, bc_pack(opt.bc),
                                       opt.interior_nodata_is_outlet, opt.allow_pits_as_outlets, rcv, dir, misc.p);
     LSDTT_LAUNCH_CHECK();
 }

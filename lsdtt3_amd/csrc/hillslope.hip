@@ -162,10 +162,12 @@ __device__ double cell_general(const HsDom& d, const ZView& Z, int r, int c, dou
     return F;
 }
 
-__device__ __forceinline__ double ratio_general(const HsDom& d, const ZView& Z, int r, int c)
+// squared centred slope; the trial's largest slope ratio is
+// sqrt(max) / Sc, formed once in its finisher
+__device__ __forceinline__ double slope2_general(const HsDom& d, const ZView& Z, int r, int c)
 {
     const double sx = sx_g(d, Z, r, c), sy = sy_g(d, Z, r, c);
-    return sqrt(sx * sx + sy * sy) / d.Sc;
+    return sx * sx + sy * sy;
 }
 
 // ---------------------------------------------------------------------------
@@ -668,7 +670,7 @@ __global__ void __launch_bounds__(256) k_trial_gen(HsDom d, const double* __rest
         double c9[9];
         const double F = cell_general(d, Z, r, c, c9);
         acc[0] = fmax(acc[0], fabs(F));
-        if (d.nonlinear) acc[1] = fmax(acc[1], ratio_general(d, Z, r, c));
+        if (d.nonlinear) acc[1] = fmax(acc[1], slope2_general(d, Z, r, c));
         const float bf = static_cast<float>(-F);
         b[f] = bf;
         dinv[f] = c9[4] != 0.0 ? static_cast<float>(1.0 / c9[4]) : 0.0f;
@@ -727,11 +729,12 @@ __global__ void __launch_bounds__(256) k_trial(HsDom d, const double* __restrict
             acc[0] = fmax(acc[0], fabs(F));
             if (d.nonlinear) {
                 const double sx = (zz[5] - zz[3]) * d.inv_2h, sy = (zz[7] - zz[1]) * d.inv_2h;
-                acc[1] = fmax(acc[1], sqrt(sx * sx + sy * sy) * d.inv_Sc);
+                acc[1] = fmax(acc[1], sx * sx + sy * sy);  // squared slope; ratio in the finisher
             }
             const float bf = static_cast<float>(-F);
             b[f[k]] = bf;
-            dinv[f[k]] = diag != 0.0 ? static_cast<float>(1.0 / diag) : 0.0f;
+            // Jacobi preconditioner only: the hardware reciprocal suffices
+            dinv[f[k]] = diag != 0.0 ? __builtin_amdgcn_rcpf(static_cast<float>(diag)) : 0.0f;
             acc[2] += static_cast<double>(bf) * bf;
             if (lam != 0.0) z_out[f[k]] = zz[4];
         }
@@ -741,14 +744,15 @@ __global__ void __launch_bounds__(256) k_trial(HsDom d, const double* __restrict
 }
 
 // max |F|, max slope ratio and the Krylov start of the trial's system
-__global__ void __launch_bounds__(256) k_trial_fin(double* partials, unsigned count, double* scal, int* flags)
+__global__ void __launch_bounds__(256) k_trial_fin(double* partials, unsigned count, double inv_Sc, double* scal,
+                                                   int* flags)
 {
     const bool mx[3] = {true, true, false};
     double out[3];
     fold_partials<3>(partials, count, mx, out);
     if (threadIdx.x == 0) {
         scal[HS_TRIAL_F] = out[0];
-        scal[HS_TRIAL_RATIO] = out[1];
+        scal[HS_TRIAL_RATIO] = sqrt(out[1]) * inv_Sc;
         // Krylov start for the system at this point: r = rhat = b, x = 0
         scal[HS_RHO] = out[2];
         scal[HS_BNORM2] = out[2];
@@ -804,7 +808,7 @@ void hs_trial(const HsDom& d, const double* z, const float* dz, double lam, long
 {
     if (d.n_gen > 0) k_trial_gen<<<gen_grid(d.n_gen), kB, 0, s>>>(d, z, dz, lam, z_out, b, dinv, partials, grid);
     k_trial<<<grid, kB, 0, s>>>(d, z, dz, lam, n, z_out, b, dinv, partials, ticket, scal, flags);
-    k_trial_fin<<<1, kB, 0, s>>>(partials, (unsigned)(grid + gen_grid(d.n_gen)), scal, flags);
+    k_trial_fin<<<1, kB, 0, s>>>(partials, (unsigned)(grid + gen_grid(d.n_gen)), d.inv_Sc, scal, flags);
     LSDTT_LAUNCH_CHECK();
 }
 

@@ -100,18 +100,20 @@ __global__ void k_restore(double* __restrict__ z, const int* __restrict__ idx, c
 
 // lsdtt_lem_cpu.cpp:655-722 sums and maxima; |dz| as float for p99.9
 // (invalid cells get +inf, which sorts after every valid value)
-__global__ void k_stats(const uint8_t* __restrict__ mask, const double* __restrict__ zb, const double* __restrict__ za,
-                        long long n, double udz, float* __restrict__ absdz, double* partials, unsigned* ticket,
-                        double* out)
+__global__ void __launch_bounds__(256) k_stats(const uint8_t* __restrict__ mask, const double* __restrict__ zb,
+                                               const double* __restrict__ za, long long n, double udz,
+                                               float* __restrict__ absdz, double* partials)
 {
     double acc[8] = {0.0, 0.0, -1e30, -1e30, 0.0, 0.0, 0.0, 0.0};
     for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
          f += (long long)gridDim.x * blockDim.x) {
-        if (!mask[f]) {
+        // loads first, then the mask test (no wait between them)
+        const uint8_t m = mask[f];
+        const double b = zb[f], a = za[f];
+        if (!m) {
             absdz[f] = __int_as_float(0x7f800000);
             continue;
         }
-        const double b = zb[f], a = za[f];
         acc[0] += b;
         acc[1] += a;
         acc[2] = fmax(acc[2], b);
@@ -127,10 +129,18 @@ __global__ void k_stats(const uint8_t* __restrict__ mask, const double* __restri
         acc[7] += 1.0;
     }
     const bool mx[8] = {false, false, true, true, true, false, true, false};
+    block_partial<8>(acc, mx, partials, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_stats_fin(const double* partials, unsigned count, double* out)
+{
+    const bool mx[8] = {false, false, true, true, true, false, true, false};
     double res[8];
-    if (grid_reduce<8>(acc, mx, partials, ticket, res))
+    fold_partials<8>(partials, count, mx, res);
+    if (threadIdx.x == 0)
         for (int i = 0; i < 8; ++i) out[i] = res[i];
 }
+
 
 // exact k-th smallest of the float bit patterns (all >= 0): three radix
 // passes of 11, 11 and 10 bits from the top
@@ -604,7 +614,8 @@ void LemEngine::stats(LemStats* st, hipStream_t s)
     hist.reserve(kRadixBins + 8);
     const double udz = p.enable_uplift ? p.U * p.dt : 0.0;
     timer.begin("step_stats", s);
-    k_stats<<<hs_grid, kB, 0, s>>>(mask.p, z_before.p, z.p, n, udz, absdz.p, partials.p, tickets.p, scal.p + kStatSlot);
+    k_stats<<<hs_grid, kB, 0, s>>>(mask.p, z_before.p, z.p, n, udz, absdz.p, partials.p);
+    k_stats_fin<<<1, kB, 0, s>>>(partials.p, (unsigned)hs_grid, scal.p + kStatSlot);
     timer.end(s, 21.0 * n);
     LSDTT_LAUNCH_CHECK();
     double* h = hpin.p + 4;
