@@ -381,11 +381,14 @@ struct SpTileArgs {
 
 // Newton / closed form of lsdtt_stream_power_cpu.cpp:257-316 for a cell
 // that is not skipped
+// NM: 0 n == 1, 1 n - 1 == 0.5 (sqrt), 2 general (pow): a kernel per mode so
+// that the common ones carry no pow registers
+template <int NM>
 __device__ __forceinline__ double sp_solve(const SpParams& a, double zo, double zr, double kam, double dx_eff)
 {
     const double tol = a.slope_clamp * dx_eff;
     double zn = zo;
-    if (a.n_is_one) {
+    if (NM == 0) {
         const double spf = kam * (a.dt / dx_eff);
         zn = (zo + zr * spf) / (1.0 + spf);
         if ((zn - zo) > tol || isnan(zn)) zn = zo;
@@ -399,7 +402,7 @@ __device__ __forceinline__ double sp_solve(const SpParams& a, double zo, double 
             } else {
                 // S^n = S^(n-1) * S: one pow per iteration; sqrt is the
                 // correctly rounded pow(S, 0.5)
-                const double pn1 = a.n1_half ? sqrt(slope) : pow(slope, a.nexp - 1.0);
+                const double pn1 = NM == 1 ? sqrt(slope) : pow(slope, a.nexp - 1.0);
                 const double fz = zn - zo + kam * a.dt * (pn1 * slope);
                 const double fd = 1.0 + kam * a.dt * (a.nexp / dx_eff) * pn1;
                 eps = fz / fd;
@@ -415,7 +418,20 @@ __device__ __forceinline__ double sp_solve(const SpParams& a, double zo, double 
 }
 
 constexpr short kNoRcv = -1, kOutRcv = -2;  // lrcv codes besides a local cell
+// packed local receiver + D8 code of the stream power tiles
+constexpr int kNoRcvP = 0xFFF, kOutRcvP = 0xFFE;
+static_assert(TN <= kOutRcvP, "local indices must fit below the packed codes");
+__device__ __forceinline__ short pack_lrd(short l, uint8_t d)
+{
+    const int lr = l == kNoRcv ? kNoRcvP : (l == kOutRcv ? kOutRcvP : static_cast<int>(l));
+    return static_cast<short>(lr | ((d & 7) << 12));
+}
 
+// cells whose inputs are loaded together per lane (register pressure vs
+// loads in flight)
+constexpr int kSpBatch = 4;
+
+template <bool MH, int NM>
 __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC bc, PassArgs P)
 {
     if (P.prev_marked && *P.prev_marked == 0) return;
@@ -424,23 +440,24 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
     if (!P.act[tile]) return;
+    // 10 KB of LDS per tile (four tiles per SIMD): the level starts are read
+    // from global memory one level ahead, the D8 code shares a word with
+    // the local receiver
     __shared__ double val[TN];   // z_old until the cell's level, then its new z
     __shared__ double kam[TN];   // K A^m; < 0: the reference skips the cell
-    __shared__ short lrcv[TN];   // local receiver, kNoRcv, kOutRcv
+    __shared__ short lrd[TN];    // local receiver (low 12 bits: index, kNoRcvP, kOutRcvP) | D8 code << 12
     __shared__ short ord[TN];
-    __shared__ short ls[TN + 1];
-    __shared__ uint8_t dcode[TN];
     const int r0 = ty * TH, c0 = tx * TW;
     const SpParams& p = A.p;
     bool valid[PER];
     uint8_t x[PER];
 #pragma unroll
-    for (int k0 = 0; k0 < PER; k0 += 8) {
-        uint8_t m[8], dr[8];
-        int g[8], ct[8];
-        double zo[8], Kf[8], ar[8];
+    for (int k0 = 0; k0 < PER; k0 += kSpBatch) {
+        uint8_t m[kSpBatch], dr[kSpBatch];
+        int g[kSpBatch], ct[kSpBatch];
+        double zo[kSpBatch], Kf[kSpBatch], ar[kSpBatch];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int j = 0; j < kSpBatch; ++j) {
             const LaneCell q = lane_cell(k0 + j, r0, c0, nr, nc);
             m[j] = A.mask[q.f];
             g[j] = A.rcv[q.f];
@@ -453,7 +470,7 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
             valid[k0 + j] = q.in && m[j];
         }
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int j = 0; j < kSpBatch; ++j) {
             const int k = k0 + j, i = lane + WL * k;
             double km = -1.0;
             short l = kNoRcv;
@@ -462,40 +479,46 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
                 if (p.divide) area /= p.dx;
                 if (!(p.threshold > 0.0 && area < p.threshold)) {
                     // K_i * width_factor * pow(area, m) (lsdtt_stream_power_cpu.cpp:258-260)
-                    km = Kf[j] * p.width * (p.m_half ? sqrt(area) : pow(area, p.m));
+                    km = Kf[j] * p.width * (MH ? sqrt(area) : pow(area, p.m));
                     const int li = local_index(g[j], nc, r0, c0);
                     l = li >= 0 ? static_cast<short>(li) : kOutRcv;
                 }
             }
             val[i] = zo[j];
             kam[i] = km;
-            lrcv[i] = l;
-            dcode[i] = dr[j];
+            lrd[i] = pack_lrd(l, dr[j]);
             ord[i] = A.ord[(size_t)tile * TN + i];
         }
     }
-    const int maxdep = load_levels(A.lstart, A.tile_maxdep, tile, ls);
+    const short* __restrict__ lsg = A.lstart + (size_t)tile * (TN + 1);
+    const int maxdep = A.tile_maxdep[tile];
+    int lb = lsg[0], le = lsg[1];
     __syncthreads();
     int eroded = 0;
     for (int L = 0; L <= maxdep; ++L) {
-        const int b = ls[L], e = ls[L + 1];
-        for (int q = b + lane; q < e; q += WL) {
+        // next level's end, fetched while this level computes
+        const int lnext = L + 2 <= TN ? lsg[min(L + 2, TN)] : le;
+        for (int q = lb + lane; q < le; q += WL) {
             const int i = ord[q];
-            const short l = lrcv[i];
-            if (l == kNoRcv) continue;
+            const int w = lrd[i];
+            const int lr = w & 0xFFF;
+            if (lr == kNoRcvP) continue;
+            const int dcode = (w >> 12) & 7;
             const double z0 = val[i];
             double zr;
-            if (l >= 0) {
-                zr = val[l];
+            if (lr != kOutRcvP) {
+                zr = val[lr];
             } else {
-                const int g = neighbour(r0 + i / TW, c0 + i % TW, dcode[i], nr, nc, bc);
+                const int g = neighbour(r0 + i / TW, c0 + i % TW, dcode, nr, nc, bc);
                 zr = __hip_atomic_load(&A.z[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             if (zr < z0) {
-                val[i] = sp_solve(p, z0, zr, kam[i], (dcode[i] & 1u) ? p.dx * kSqrt2 : p.dx);
+                val[i] = sp_solve<NM>(p, z0, zr, kam[i], (dcode & 1) ? p.dx * kSqrt2 : p.dx);
                 ++eroded;
             }
         }
+        lb = le;
+        le = lnext;
         __syncthreads();
     }
     // a changed elevation with donors in other tiles re-runs those tiles
@@ -748,7 +771,17 @@ int FluvialTiles::stream_power(const int* rcv, const uint8_t* dir, const uint8_t
     const int passes = settle(
         "stream_power", bytes,
         [&](const PassArgs& P) {
-            k_tile_sp<<<dim3(ntx, nty), WL, 0, s>>>(A, nr, nc, bc_, P);
+            const dim3 g(ntx, nty);
+            const int nm = sp.n_is_one ? 0 : (sp.n1_half ? 1 : 2);
+            if (sp.m_half) {
+                if (nm == 0) k_tile_sp<true, 0><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+                else if (nm == 1) k_tile_sp<true, 1><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+                else k_tile_sp<true, 2><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+            } else {
+                if (nm == 0) k_tile_sp<false, 0><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+                else if (nm == 1) k_tile_sp<false, 1><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+                else k_tile_sp<false, 2><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+            }
             LSDTT_LAUNCH_CHECK();
         },
         [&] {

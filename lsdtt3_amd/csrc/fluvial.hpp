@@ -32,7 +32,7 @@ struct SpParams {
 
 class FluvialTiles {
 public:
-    static constexpr int TW = 32, TH = 32, TN = TW * TH;
+    static constexpr int TW = 32, TH = 16, TN = TW * TH;
 
     void resize(int nr, int nc, BC bc);
     // each tile's cells in level order, once per set of receivers

@@ -342,6 +342,10 @@ __device__ __forceinline__ float jx_tile(const HsDom& d, const double* __restric
 // the hot kernel's one-block finisher folds in after the hot kernel's own.
 // ---------------------------------------------------------------------------
 constexpr int kGenBlocks = 64;
+// partial-sum regions of one BiCGSTAB iteration (in doubles, times the grid
+// for the 2-value regions): v dots, t dots, x dots; a region is folded by
+// the next kernel while that kernel writes its own
+constexpr int kPartV = 0, kPartT = 2, kPartX = 4;
 
 __host__ __device__ inline int gen_grid(int n_gen)
 {
@@ -419,31 +423,6 @@ __global__ void __launch_bounds__(256) k_bi_p(BiArgs a)
     }
 }
 
-// companion: out = J in on the general rows, block partials of the dots
-// (w1 . out) and (w2 . out) (w2 may be null) into slots base + block
-__global__ void __launch_bounds__(256) k_bi_mv_gen(BiArgs a, const float* __restrict__ in, float* __restrict__ out,
-                                                   const float* __restrict__ w1, const float* __restrict__ w2,
-                                                   int nv, int base)
-{
-    if (a.flags[0]) return;
-    double acc[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.d.n_gen; i += gridDim.x * blockDim.x) {
-        const int f = a.d.gen[i];
-        const float y = jx_gen(a.d, in, i, f);
-        out[f] = y;
-        acc[0] += static_cast<double>(w1[f]) * y;
-        if (w2) acc[1] += static_cast<double>(w2[f]) * y;
-    }
-    const bool mx[2] = {false, false};
-    if (nv == 1) {
-        double v1[1] = {acc[0]};
-        const bool m1[1] = {false};
-        block_partial<1>(v1, m1, a.partials, base + blockIdx.x);
-    } else {
-        block_partial<2>(acc, mx, a.partials, base + blockIdx.x);
-    }
-}
-
 // v = J phat; alpha = rho / (rhat . v)
 __global__ void __launch_bounds__(256) k_bi_v(BiArgs a)
 {
@@ -489,31 +468,43 @@ __global__ void __launch_bounds__(256) k_bi_v(BiArgs a)
             }
         }
     }
-    const bool mx[1] = {false};
-    block_partial<1>(acc, mx, a.partials, blockIdx.x);
-}
-
-// alpha = rho / (rhat . v) from the partials of k_bi_v and its companion
-__global__ void __launch_bounds__(256) k_bi_v_fin(BiArgs a, unsigned count)
-{
-    if (a.flags[0]) return;
-    const bool mx[1] = {false};
-    double out[1];
-    fold_partials<1>(a.partials, count, mx, out);
-    if (threadIdx.x != 0) return;
-    if (out[0] == 0.0) {
-        a.flags[0] = 1;
-        a.flags[2] = 1;
-    } else {
-        a.scal[HS_ALPHA] = a.scal[HS_RHO] / out[0];
+    // the general rows (grid edge, next to a masked cell): stored Jacobian rows
+    for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < a.d.n_gen; gi += gridDim.x * blockDim.x) {
+        const int f = a.d.gen[gi];
+        const float y = jx_gen(a.d, a.phat, gi, f);
+        vout[f] = y;
+        acc[0] += static_cast<double>(rhat[f]) * y;
     }
+    const bool mx[1] = {false};
+    block_partial<1>(acc, mx, a.partials + kPartV, blockIdx.x);
 }
 
 // s = r - alpha v; shat = M^-1 s
 __global__ void __launch_bounds__(256) k_bi_s(BiArgs a)
 {
     if (a.flags[0]) return;
-    const float al = static_cast<float>(a.scal[HS_ALPHA]);
+    // alpha = rho / (rhat . v): every block folds k_bi_v's partials in the
+    // same order (so all agree), block 0 keeps it for k_bi_x
+    double rv1[1];
+    const bool mx1[1] = {false};
+    fold_partials<1>(a.partials + kPartV, gridDim.x, mx1, rv1);
+    __shared__ double alpha_s;
+    __shared__ int brk;
+    if (threadIdx.x == 0) {
+        brk = rv1[0] == 0.0;
+        alpha_s = brk ? 0.0 : a.scal[HS_RHO] / rv1[0];
+        if (blockIdx.x == 0) {
+            if (brk) {
+                a.flags[0] = 1;
+                a.flags[2] = 1;
+            } else {
+                a.scal[HS_ALPHA] = alpha_s;
+            }
+        }
+    }
+    __syncthreads();
+    if (brk) return;
+    const float al = static_cast<float>(alpha_s);
     const float* __restrict__ r = a.first ? a.b : a.r;
     const float* __restrict__ v = a.v;
     const float* __restrict__ dinv = a.dinv;
@@ -578,25 +569,32 @@ __global__ void __launch_bounds__(256) k_bi_t(BiArgs a)
             }
         }
     }
+    for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < a.d.n_gen; gi += gridDim.x * blockDim.x) {
+        const int f = a.d.gen[gi];
+        const float tv = jx_gen(a.d, a.shat, gi, f);
+        tout[f] = tv;
+        acc[0] += static_cast<double>(tv) * sv[f];
+        acc[1] += static_cast<double>(tv) * tv;
+    }
     const bool mx[2] = {false, false};
-    block_partial<2>(acc, mx, a.partials, blockIdx.x);
-}
-
-// omega = (t . s) / (t . t)
-__global__ void __launch_bounds__(256) k_bi_t_fin(BiArgs a, unsigned count)
-{
-    if (a.flags[0]) return;
-    const bool mx[2] = {false, false};
-    double out[2];
-    fold_partials<2>(a.partials, count, mx, out);
-    if (threadIdx.x == 0) a.scal[HS_OMEGA] = out[1] > 0.0 ? out[0] / out[1] : 0.0;
+    block_partial<2>(acc, mx, a.partials + kPartT * gridDim.x, blockIdx.x);
 }
 
 // x += alpha phat + omega shat; r = s - omega t; convergence on ||r||
 __global__ void __launch_bounds__(256) k_bi_x(BiArgs a)
 {
     if (a.flags[0]) return;
-    const float al = static_cast<float>(a.scal[HS_ALPHA]), om = static_cast<float>(a.scal[HS_OMEGA]);
+    // omega = (t . s) / (t . t) from k_bi_t's partials, folded by every block
+    double tt[2];
+    const bool mx2[2] = {false, false};
+    fold_partials<2>(a.partials + kPartT * gridDim.x, gridDim.x, mx2, tt);
+    __shared__ double omega_s;
+    if (threadIdx.x == 0) {
+        omega_s = tt[1] > 0.0 ? tt[0] / tt[1] : 0.0;
+        if (blockIdx.x == 0) a.scal[HS_OMEGA] = omega_s;
+    }
+    __syncthreads();
+    const float al = static_cast<float>(a.scal[HS_ALPHA]), om = static_cast<float>(omega_s);
     const float* __restrict__ phat = a.phat;
     const float* __restrict__ shat = a.shat;
     const float* __restrict__ sv = a.s;
@@ -633,7 +631,7 @@ __global__ void __launch_bounds__(256) k_bi_x(BiArgs a)
         acc[1] += static_cast<double>(rv) * rv;
     }
     const bool mx[2] = {false, false};
-    block_partial<2>(acc, mx, a.partials, blockIdx.x);
+    block_partial<2>(acc, mx, a.partials + kPartX * gridDim.x, blockIdx.x);
 }
 
 // rho, ||r||^2 and the convergence / breakdown flags
@@ -642,7 +640,7 @@ __global__ void __launch_bounds__(256) k_bi_x_fin(BiArgs a, unsigned count)
     if (a.flags[0]) return;
     const bool mx[2] = {false, false};
     double out[2];
-    fold_partials<2>(a.partials, count, mx, out);
+    fold_partials<2>(a.partials + kPartX * count, count, mx, out);
     if (threadIdx.x == 0) {
         a.scal[HS_RHO_OLD] = a.scal[HS_RHO];
         a.scal[HS_RHO] = out[0];
@@ -779,22 +777,17 @@ void hs_fast_flags(const HsDom& d, uint8_t* fast, hipStream_t s)
 void hs_bicgstab_iteration(const BiArgs& a, int grid, hipStream_t s, KernelTimer* t)
 {
     const double n = static_cast<double>(a.n);
-    const int gg = gen_grid(a.d.n_gen);
     if (t) t->begin("bicgstab_p", s);
     k_bi_p<<<grid, kB, 0, s>>>(a);
     if (t) t->end(s, 24.0 * n);  // r, p, v, dinv in; p, phat out
-    if (gg) k_bi_mv_gen<<<gg, kB, 0, s>>>(a, a.phat, a.v, a.rhat, nullptr, 1, grid);
     if (t) t->begin("bicgstab_matvec", s);
     k_bi_v<<<grid, kB, 0, s>>>(a);
-    k_bi_v_fin<<<1, kB, 0, s>>>(a, (unsigned)(grid + gg));
     if (t) t->end(s, 22.0 * n);  // z 8, phat 4, fast 1, rhat 4 in; v 4 out (+ dof 1)
     if (t) t->begin("bicgstab_s", s);
     k_bi_s<<<grid, kB, 0, s>>>(a);
     if (t) t->end(s, 20.0 * n);
-    if (gg) k_bi_mv_gen<<<gg, kB, 0, s>>>(a, a.shat, a.t, a.s, a.t, 2, grid);
     if (t) t->begin("bicgstab_matvec", s);
     k_bi_t<<<grid, kB, 0, s>>>(a);
-    k_bi_t_fin<<<1, kB, 0, s>>>(a, (unsigned)(grid + gg));
     if (t) t->end(s, 22.0 * n);
     if (t) t->begin("bicgstab_x", s);
     k_bi_x<<<grid, kB, 0, s>>>(a);
