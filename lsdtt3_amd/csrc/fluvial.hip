@@ -291,7 +291,6 @@ __global__ void __launch_bounds__(WL) k_tile_counts(const int* __restrict__ rcv,
     __shared__ short ls[TN + 1];
     const int r0 = ty * TH, c0 = tx * TW;
     int old[PER], g[PER];
-    uint8_t x[PER];
     bool valid[PER];
     {
         uint8_t m[PER];
@@ -300,7 +299,6 @@ __global__ void __launch_bounds__(WL) k_tile_counts(const int* __restrict__ rcv,
             const LaneCell q = lane_cell(k, r0, c0, nr, nc);
             m[k] = mask[q.f];
             g[k] = rcv[q.f];
-            x[k] = xdon[q.f];
             old[k] = cnt[q.f];  // only this tile writes its cells
             valid[k] = q.in && m[k];
         }
@@ -313,17 +311,41 @@ __global__ void __launch_bounds__(WL) k_tile_counts(const int* __restrict__ rcv,
         ord[i] = ordg[(size_t)tile * TN + i];
     }
     const int maxdep = load_levels(lstartg, tile_maxdep, tile, ls);
-    // inflow from other tiles (ring cells only)
+    __syncthreads();
+    // inflow from other tiles into the ring cells: ring slots as in
+    // k_tile_order (two per lane); every donor's count is loaded at once
+    // (addresses of all 8 neighbours, clamped to the cell itself when off
+    // the grid) and selected by the cross-tile donor bits afterwards
+    {
+        const int h = min(TH, nr - r0), w = min(TW, nc - c0);
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        if (!valid[k] || !x[k]) continue;
-        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
-        int a = 0;
-        for (unsigned b = x[k]; b; b &= b - 1) {
-            const int g2 = neighbour(q.r, q.c, __builtin_ctz(b), nr, nc, bc);
-            a += __hip_atomic_load(&cnt[g2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int half = 0; half < 2; ++half) {
+            const int sl = lane + WL * half;
+            int lr, lc;
+            bool ok;
+            if (sl < 32) {
+                lr = 0, lc = sl, ok = lc < w;
+            } else if (sl < 64) {
+                lr = h - 1, lc = sl - 32, ok = lc < w && h > 1;
+            } else if (sl < 96) {
+                lr = sl - 64, lc = 0, ok = lr >= 1 && lr <= h - 2;
+            } else {
+                lr = sl - 96, lc = w - 1, ok = lr >= 1 && lr <= h - 2 && w > 1;
+            }
+            const int r = r0 + lr, c = c0 + lc;
+            const int f = ok ? r * nc + c : 0;
+            const uint8_t bits = xdon[f];
+            int v[8];
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                const int g2 = ok ? neighbour(r, c, d, nr, nc, bc) : -1;
+                v[d] = __hip_atomic_load(&cnt[g2 >= 0 ? g2 : f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            int a = 0;
+#pragma unroll
+            for (int d = 0; d < 8; ++d) a += ((bits >> d) & 1u) ? v[d] : 0;
+            if (ok && a) acc[lr * TW + lc] += a;
         }
-        acc[lane + WL * k] += a;
     }
     __syncthreads();
     for (int L = maxdep; L >= 0; --L) {
@@ -451,6 +473,7 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
     const SpParams& p = A.p;
     bool valid[PER];
     uint8_t x[PER];
+    double zcur[PER];  // the cells' current z (only this tile writes them)
 #pragma unroll
     for (int k0 = 0; k0 < PER; k0 += kSpBatch) {
         uint8_t m[kSpBatch], dr[kSpBatch];
@@ -467,6 +490,7 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
             Kf[j] = A.K ? A.K[q.f] : p.K0;
             ar[j] = A.area ? A.area[q.f] : 0.0;
             x[k0 + j] = A.xdon[q.f];
+            zcur[k0 + j] = A.z[q.f];
             valid[k0 + j] = q.in && m[j];
         }
 #pragma unroll
@@ -529,7 +553,7 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
         const int i = lane + WL * k;
         const LaneCell q = lane_cell(k, r0, c0, nr, nc);
         const double v = val[i];
-        const double o = A.z[q.f];  // only this tile writes its cells
+        const double o = zcur[k];
         if (__double_as_longlong(o) != __double_as_longlong(v)) {
             __hip_atomic_store(&A.z[q.f], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             for (unsigned b = x[k]; b; b &= b - 1) {
