@@ -10,9 +10,12 @@ power K log-uniform in [1e-6, 1e-4] (seeded), m = 0.5, n = 1.5; U = 5e-4
 m/yr, dt = 500 yr; boundary "nnon" (south edge fixed).
 
 One step = one run_lem_step on the resident state.  value = valid cells x
-timed steps / wall seconds, inputs resident in HBM before the clock starts.
-The roofline object is for the kernel group with the largest share of the
-timed region, measured with HIP events on the step's stream; the CPU
+timed steps / wall seconds, inputs resident in HBM before the clock starts,
+with no instrumentation in the stream.  The per-kernel breakdown and the
+roofline object (the kernel group with the largest share) come from a
+separate run of --profile-steps steps timed with HIP events recorded on
+the step's stream (the events add gaps of their own, so that run is not the
+timed one); the CPU
 baseline is the oracle's single-threaded FP64 step on a 512 x 512 crop of
 the same recipe.
 
@@ -246,6 +249,38 @@ def bench_ensemble(args):
 BF16_PEAK_TFLOPS = 2500.0
 
 
+def run_steps(lem, n, sync, profile=False):
+    """n steps; wall seconds (inputs resident, synchronised on both sides),
+    Newton / Krylov totals, worst relative residual, warm-start figures and,
+    with profile, the per-kernel-group HIP-event totals.  The event records
+    between kernel groups add their own gaps to the stream, so the timed
+    throughput comes from a run without them and the breakdown from a
+    separate profiled run."""
+    if profile:
+        lem.profile(True)
+    acc = {"newton": 0, "krylov": 0, "worst_r": 0.0, "accepted": 0, "start_ratio": 0.0}
+    sync()
+    t0 = time.perf_counter()
+    st = None
+    for _ in range(n):
+        st = lem.step()
+        acc["newton"] += st["newton_iterations"]
+        acc["krylov"] += st["krylov_iterations"]
+        acc["accepted"] += st["warm_start_accepted"]
+        if st["hillslope_initial_residual"] > 0:
+            acc["worst_r"] = max(acc["worst_r"], st["hillslope_residual"] / st["hillslope_initial_residual"])
+            acc["start_ratio"] = max(acc["start_ratio"], st["warm_start_residual"] / st["hillslope_initial_residual"])
+        if not st["hillslope_converged"]:
+            raise RuntimeError("hillslope Newton did not converge")
+    sync()
+    dt = time.perf_counter() - t0
+    prof = None
+    if profile:
+        prof = lem.profile_totals()
+        lem.profile(False)
+    return dt, acc, st, prof
+
+
 def learned_comparison(args, z, K, sync, plain_value):
     """Config 3: the learned warm start on the same 4096^2 workload.
 
@@ -301,25 +336,10 @@ def learned_comparison(args, z, K, sync, plain_value):
         Lm.set_warm_start(lem, mode, net if mode == Lm.WARM_LEARNED else None)
         for _ in range(args.warmup):
             lem.step()
-        sync()
-        lem.profile(True)
-        newton = krylov = acc = 0
-        worst_r = ratio = 0.0
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            st = lem.step()
-            newton += st["newton_iterations"]
-            krylov += st["krylov_iterations"]
-            acc += st["warm_start_accepted"]
-            if st["hillslope_initial_residual"] > 0:
-                worst_r = max(worst_r, st["hillslope_residual"] / st["hillslope_initial_residual"])
-                ratio = max(ratio, st["warm_start_residual"] / st["hillslope_initial_residual"])
-            if not st["hillslope_converged"]:
-                raise RuntimeError("hillslope Newton did not converge")
-        sync()
-        dt = time.perf_counter() - t0
-        prof = lem.profile_totals()
-        lem.profile(False)
+        dt, a, st, _ = run_steps(lem, args.steps, sync)
+        _, _, _, prof = run_steps(lem, min(args.steps, args.profile_steps), sync, profile=True)
+        newton, krylov, acc = a["newton"], a["krylov"], a["accepted"]
+        worst_r, ratio = a["worst_r"], a["start_ratio"]
         v = int(st["n_valid"]) * args.steps / dt
         out[name] = {"value": round(v, 1), "unit": "cell-steps/s", "vs_plain": round(v / plain_value, 4),
                      "ms_per_step": round(1e3 * dt / args.steps, 3),
@@ -327,7 +347,8 @@ def learned_comparison(args, z, K, sync, plain_value):
                      "krylov_iters_per_step": round(krylov / args.steps, 2),
                      "starts_accepted": acc, "worst_start_residual_ratio": ratio,
                      "worst_relative_residual": worst_r,
-                     "predictor_ms_per_step": round(prof.get("warm_start", (0.0,))[0] / args.steps, 3)}
+                     "predictor_ms_per_step": round(prof.get("warm_start", (0.0,))[0]
+                                                    / min(args.steps, args.profile_steps), 3)}
         del lem
     return out
 
@@ -378,6 +399,8 @@ def main():
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--sequential", action="store_true", help="ensemble: also time one instance at a time")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
+    ap.add_argument("--profile-steps", type=int, default=10,
+                    help="steps of the separate HIP-event-profiled run (per-kernel breakdown, roofline)")
     ap.add_argument("--pairs", type=int, default=20000, help="learned: training tile pairs")
     ap.add_argument("--train-size", type=int, default=2048, help="learned: training LEM grid side")
     args = ap.parse_args()
@@ -406,23 +429,10 @@ def main():
     lem = lsdtt3_amd.LandscapeModel(z, 30.0, **lem_kwargs(K))
     for _ in range(args.warmup):
         lem.step()
-    sync()
-    lem.profile(True)
-    newton = krylov = 0
-    worst_r = 0.0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = lem.step()
-        newton += st["newton_iterations"]
-        krylov += st["krylov_iterations"]
-        if st["hillslope_initial_residual"] > 0:
-            worst_r = max(worst_r, st["hillslope_residual"] / st["hillslope_initial_residual"])
-        if not st["hillslope_converged"]:
-            raise RuntimeError("hillslope Newton did not converge")
-    sync()
-    dt = time.perf_counter() - t0
-    prof = lem.profile_totals()
-    lem.profile(False)
+    dt, a, st, _ = run_steps(lem, args.steps, sync)
+    newton, krylov, worst_r = a["newton"], a["krylov"], a["worst_r"]
+    psteps = min(args.steps, args.profile_steps)
+    _, _, _, prof = run_steps(lem, psteps, sync, profile=True)
 
     n_valid = int(st["n_valid"])
     value = n_valid * args.steps / dt
@@ -453,7 +463,8 @@ def main():
                    "count_tile_runs_last_step": int(st["count_tile_runs"]),
                    "stream_power_tile_runs_last_step": int(st["stream_power_tile_runs"])},
         "roofline": roofline,
-        "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
+        "kernels_ms_per_step": {k: round(v[0] / psteps, 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
+        "profiled_steps": psteps,
     }
     if not args.no_learned:
         out["learned_warm_start"] = learned_comparison(args, z, K, sync, value)
