@@ -797,10 +797,20 @@ void hs_bicgstab_iteration(const BiArgs& a, int grid, hipStream_t s, KernelTimer
 }
 
 void hs_trial(const HsDom& d, const double* z, const float* dz, double lam, long long n, double* z_out, float* b,
-              float* dinv, double* partials, unsigned* ticket, double* scal, int* flags, int grid, hipStream_t s)
+              float* dinv, double* partials, unsigned* ticket, double* scal, int* flags, int grid, hipStream_t s,
+              hipStream_t aux, hipEvent_t fork, hipEvent_t join)
 {
-    if (d.n_gen > 0) k_trial_gen<<<gen_grid(d.n_gen), kB, 0, s>>>(d, z, dz, lam, z_out, b, dinv, partials, grid);
+    const bool side = d.n_gen > 0 && aux && fork && join;
+    if (side) {
+        LSDTT_HIP(hipEventRecord(fork, s));
+        LSDTT_HIP(hipStreamWaitEvent(aux, fork, 0));
+        k_trial_gen<<<gen_grid(d.n_gen), kB, 0, aux>>>(d, z, dz, lam, z_out, b, dinv, partials, grid);
+        LSDTT_HIP(hipEventRecord(join, aux));
+    } else if (d.n_gen > 0) {
+        k_trial_gen<<<gen_grid(d.n_gen), kB, 0, s>>>(d, z, dz, lam, z_out, b, dinv, partials, grid);
+    }
     k_trial<<<grid, kB, 0, s>>>(d, z, dz, lam, n, z_out, b, dinv, partials, ticket, scal, flags);
+    if (side) LSDTT_HIP(hipStreamWaitEvent(s, join, 0));
     k_trial_fin<<<1, kB, 0, s>>>(partials, (unsigned)(grid + gen_grid(d.n_gen)), d.inv_Sc, scal, flags);
     LSDTT_LAUNCH_CHECK();
 }

@@ -77,8 +77,11 @@ void hs_bicgstab_iteration(const BiArgs& a, int grid, hipStream_t s, class Kerne
 // gradient ratio over the DOFs; and the next linear system at zt, should
 // the point be accepted: b = -F(zt), dinv = 1 / J_PP(zt) (FP32), the DOF
 // values of zt into z_out (when lam != 0) and the Krylov start (rho =
-// ||b||^2, flags cleared).
+// ||b||^2, flags cleared).  With `aux` (a second stream of the same device)
+// and two events, the general cells' companion kernel runs on aux beside
+// the main kernel instead of before it.
 void hs_trial(const HsDom& d, const double* z, const float* dz, double lam, long long n, double* z_out, float* b,
-              float* dinv, double* partials, unsigned* ticket, double* scal, int* flags, int grid, hipStream_t s);
+              float* dinv, double* partials, unsigned* ticket, double* scal, int* flags, int grid, hipStream_t s,
+              hipStream_t aux = nullptr, hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
 
 }  // namespace lsdtt_gpu

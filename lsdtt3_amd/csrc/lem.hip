@@ -265,6 +265,16 @@ LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, con
     LSDTT_HIP(hipMemset(scal.p, 0, sizeof(double) * (kScalCount + 8)));
 }
 
+LemEngine::~LemEngine()
+{
+    if (aux_stream) {
+        (void)hipStreamSynchronize(aux_stream);
+        (void)hipEventDestroy(ev_fork);
+        (void)hipEventDestroy(ev_join);
+        (void)hipStreamDestroy(aux_stream);
+    }
+}
+
 // resolve_hillslope_coupling_regime (lsdtt_lem_cpu.cpp:64-77): a channel
 // set exists when the regime is channels_as_boundaries, or automatic with a
 // threshold or an explicit mask
@@ -403,7 +413,7 @@ HsDom LemEngine::hs_domain(double inv_dt) const
     return with_reciprocals(d);
 }
 
-int LemEngine::bicgstab(hipStream_t s)
+int LemEngine::bicgstab(hipStream_t s, int newton_it)
 {
     BiArgs a;
     a.d = hs_domain(cur_inv_dt);
@@ -429,13 +439,16 @@ int LemEngine::bicgstab(hipStream_t s)
     a.first = 0;
     // the start (rho = ||b||^2, flags) was set by the trial that produced b
     // iterations are enqueued in chunks between flag read-backs; an
-    // iteration after convergence returns at once.  The chunk starts at the
-    // usual iteration count of these well-conditioned systems and grows.
-    int chunk = std::max(1, last_krylov_iterations);
+    // iteration after convergence returns at once (but still costs its
+    // launches).  The first chunk is what the same Newton iteration of the
+    // previous solve needed (these well-conditioned systems repeat their
+    // counts from step to step), then one more at a time, growing.
+    const int slot = std::min(newton_it, kKrylovHist - 1);
+    int chunk = std::max(1, krylov_hist[slot]);
     int* hf = reinterpret_cast<int*>(hpin.p + 16);
     hf[0] = hf[1] = hf[2] = 0;
     int it_launched = 0;
-    for (int done = 0; !done; chunk = std::min(2 * chunk, 16)) {
+    for (int done = 0, round = 0; !done; ++round, chunk = round == 1 ? 1 : std::min(2 * chunk, 16)) {
         for (int k = 0; k < chunk; ++k) {
             BiArgs ai = a;
             ai.first = (it_launched++ == 0) ? 1 : 0;
@@ -445,7 +458,7 @@ int LemEngine::bicgstab(hipStream_t s)
         LSDTT_HIP(hipStreamSynchronize(s));
         done = hf[0];
     }
-    last_krylov_iterations = hf[1];
+    krylov_hist[slot] = hf[1];
     if (hf[1] == 0)  // no iteration ran (b = 0): the correction is zero
         LSDTT_HIP(hipMemsetAsync(x.p, 0, sizeof(float) * n, s));
     return hf[1];
@@ -454,8 +467,13 @@ int LemEngine::bicgstab(hipStream_t s)
 void LemEngine::trial(double lambda, double* fnorm, double* ratio, hipStream_t s)
 {
     timer.begin("hillslope_residual", s);
+    if (!aux_stream) {
+        LSDTT_HIP(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
+        LSDTT_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        LSDTT_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
     hs_trial(hs_domain(cur_inv_dt), zc, x.p, lambda, n, zt, b.p, dinv.p, partials.p, tickets.p, scal.p, flags.p,
-             hs_grid, s);
+             hs_grid, s, aux_stream, ev_fork, ev_join);
     // z 8, zn 8, fast 1, dof 1 in; b 4, dinv 4 out; a trial step adds dz 4 in, z_out 8 out
     timer.end(s, (lambda != 0.0 ? 38.0 : 26.0) * n);
     double* h = hpin.p;
@@ -511,7 +529,7 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
             break;
         }
         // b and dinv at the iterate come from the trial that accepted it
-        st->krylov_iterations += bicgstab(s);
+        st->krylov_iterations += bicgstab(s, it);
         // line search (newton .cpp:553-633)
         const bool restoring = p.nonlinear && ratio_now > feasible;
         double lambda = 1.0, fnorm_try = fnorm, ratio_try = 0.0;

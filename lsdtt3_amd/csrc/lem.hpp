@@ -70,6 +70,9 @@ struct LemStats {
 class LemEngine {
 public:
     LemEngine(int nr, int nc, double dx, const uint8_t* mask_host, const LemParams& p);
+    ~LemEngine();
+    LemEngine(const LemEngine&) = delete;
+    LemEngine& operator=(const LemEngine&) = delete;
     // explicit Dirichlet channel mask for the hillslope (null clears it)
     void set_channel_mask(const uint8_t* channel_host);
 
@@ -120,6 +123,9 @@ public:
     dbuf<unsigned> hist;
 
     int hs_grid = 1024;
+    // side stream for the trial's general-cell kernel (default priority)
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // warm start of the first Newton solve of a step: 0 off, 1 explicit
     // Euler (dz = dt (-F(z^n))), 2 the learned predictor `ws`
     int ws_mode = 0;
@@ -132,13 +138,15 @@ public:
     double cap_kappa = 0.0;
     KernelTimer timer;
     double cur_inv_dt = 0.0;
-    int last_krylov_iterations = 2;
+    // BiCGSTAB iterations used by each Newton iteration of the last solve
+    static constexpr int kKrylovHist = 8;
+    int krylov_hist[kKrylovHist] = {2, 2, 2, 2, 2, 2, 2, 2};
 
 private:
     void stream_power(hipStream_t s);
     int hillslope(LemStats* st, hipStream_t s);
     int newton_solve(double inv_dt, LemStats* st, hipStream_t s);
-    int bicgstab(hipStream_t s);
+    int bicgstab(hipStream_t s, int newton_it);
     void trial(double lambda, double* fnorm, double* ratio, hipStream_t s);
     HsDom hs_domain(double inv_dt) const;
     // hillslope DOFs (valid, not on an 'o' edge, not a channel pixel of an
