@@ -257,7 +257,7 @@ struct PassArgs {
     int* act_clear;
     const int* prev_marked;  // null for the first pass
     int* marked;
-    int* runs;
+    int* runs;  // per tile: times run in this settle (one writer per pass)
 };
 
 // level starts of levels 0..maxdep + 1 into LDS
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(WL) k_tile_counts(const int* __restrict__ rcv,
     const bool wmarked = __ballot(marked) != 0;
     if (lane == 0) {
         if (wmarked) *P.marked = 1;
-        atomicAdd(P.runs, 1);
+        P.runs[tile] += 1;  // a single-address atomic from every tile would serialise
     }
 }
 
@@ -567,7 +567,7 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
     if (lane == 0) {
         A.tile_eroded[tile] = eroded;
         if (wmarked) *P.marked = 1;
-        atomicAdd(P.runs, 1);
+        P.runs[tile] += 1;  // a single-address atomic from every tile would serialise
     }
 }
 
@@ -654,14 +654,22 @@ __global__ void k_fill_i32(int* __restrict__ a, long long n, int v)
     if (i < n) a[i] = v;
 }
 
-__global__ void k_sum_i32(const int* __restrict__ a, int n, long long* __restrict__ out)
+// one-block sum (deterministic): kSumThreads threads, loads unrolled so that
+// each thread keeps several in flight
+constexpr int kSumThreads = 1024;
+__global__ void __launch_bounds__(kSumThreads) k_sum_i32(const int* __restrict__ a, int n, long long* __restrict__ out)
 {
-    __shared__ long long s[256];
+    __shared__ long long s[kSumThreads];
     long long acc = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += a[i];
+    int i = threadIdx.x;
+    for (; i + 3 * kSumThreads < n; i += 4 * kSumThreads) {
+        const int v0 = a[i], v1 = a[i + kSumThreads], v2 = a[i + 2 * kSumThreads], v3 = a[i + 3 * kSumThreads];
+        acc += (long long)v0 + v1 + v2 + v3;
+    }
+    for (; i < n; i += kSumThreads) acc += a[i];
     s[threadIdx.x] = acc;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = kSumThreads / 2; o > 0; o >>= 1) {
         if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
         __syncthreads();
     }
@@ -685,9 +693,11 @@ void FluvialTiles::resize(int nr_, int nc_, BC bc)
     ord.reserve((size_t)ntiles * TN);
     xdon.reserve((size_t)nr * nc);
     lstart.reserve((size_t)ntiles * (TN + 1));
-    // per pass: marked flag, tiles run; then the eroded-cell sum (8-aligned)
-    flags.reserve(2 * (size_t)max_passes + 4);
-    flags_h.reserve(2 * (size_t)max_passes + 4);
+    // [0, 2) eroded-cell sum and [2, 4) tile-run sum (long long each), then
+    // per pass its marked flag
+    flags.reserve(4 + (size_t)max_passes);
+    flags_h.reserve(4 + (size_t)max_passes);
+    tile_run.reserve(ntiles);
 }
 
 void FluvialTiles::smooth_area(const int* cnt, const uint8_t* mask, double pixel_area, int kernel_size,
@@ -717,8 +727,10 @@ void FluvialTiles::depths(const int* rcv, const uint8_t* mask, hipStream_t s)
 template <class Pass, class After>
 int FluvialTiles::settle(const char* name, double first_pass_bytes, Pass&& pass, After&& after_batch, hipStream_t s)
 {
-    const size_t nflags = 2 * (size_t)max_passes + 4;
+    const size_t nflags = 4 + (size_t)max_passes;
     k_fill_i32<<<div_up(nflags, 256), 256, 0, s>>>(flags.p, nflags, 0);
+    LSDTT_LAUNCH_CHECK();
+    k_fill_i32<<<div_up(ntiles, 256), 256, 0, s>>>(tile_run.p, ntiles, 0);
     LSDTT_LAUNCH_CHECK();
     k_fill_i32<<<div_up(ntiles, 256), 256, 0, s>>>(act[0].p, ntiles, 1);
     LSDTT_LAUNCH_CHECK();
@@ -729,25 +741,29 @@ int FluvialTiles::settle(const char* name, double first_pass_bytes, Pass&& pass,
     for (;;) {
         const int end = std::min(max_passes, launched + batch);
         for (int q = launched; q < end; ++q) {
-            PassArgs P{act[q % 3].p, act[(q + 1) % 3].p, act[(q + 2) % 3].p, q > 0 ? flags.p + q - 1 : nullptr,
-                       flags.p + q, flags.p + max_passes + q};
+            PassArgs P{act[q % 3].p, act[(q + 1) % 3].p, act[(q + 2) % 3].p, q > 0 ? flags.p + 4 + q - 1 : nullptr,
+                       flags.p + 4 + q, tile_run.p};
             if (timer) timer->begin(name, s);
             pass(P);
             if (timer) timer->end(s, q == 0 ? first_pass_bytes : 0.0);
         }
         launched = end;
         after_batch();
-        LSDTT_HIP(hipMemcpyAsync(flags_h.data(), flags.p, nflags * sizeof(int), hipMemcpyDeviceToHost, s));
+        k_sum_i32<<<1, kSumThreads, 0, s>>>(tile_run.p, ntiles, reinterpret_cast<long long*>(flags.p + 2));
+        LSDTT_LAUNCH_CHECK();
+        LSDTT_HIP(hipMemcpyAsync(flags_h.data(), flags.p, (4 + (size_t)launched) * sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
         LSDTT_HIP(hipStreamSynchronize(s));
         int settled = -1;
         for (int q = 0; q < launched; ++q)
-            if (!flags_h[q]) {
+            if (!flags_h[4 + q]) {
                 settled = q;
                 break;
             }
         if (settled >= 0) {
-            tile_runs = 0;
-            for (int q = 0; q <= settled; ++q) tile_runs += flags_h[max_passes + q];
+            long long runs = 0;
+            std::memcpy(&runs, flags_h.data() + 2, sizeof(long long));
+            tile_runs = runs;
             last_passes = settled + 1;
             return last_passes;
         }
@@ -787,7 +803,7 @@ int FluvialTiles::stream_power(const int* rcv, const uint8_t* dir, const uint8_t
                                double* z, long long* eroded, hipStream_t s)
 {
     SpTileArgs A{rcv, dir, mask, cnt, area, ord.p, lstart.p, tile_maxdep.p, xdon.p, z_old, K, z, tile_eroded.p, sp};
-    long long* d_sum = reinterpret_cast<long long*>(flags.p + 2 * max_passes);
+    long long* d_sum = reinterpret_cast<long long*>(flags.p);
     last_passes = last_sp_passes;
     // algorithmic bytes of the sweep, counted once: rcv 4, dir 1, mask 1,
     // cnt 4, z_old 8 (, K 8) in; z 8 read and 8 written
@@ -809,12 +825,12 @@ int FluvialTiles::stream_power(const int* rcv, const uint8_t* dir, const uint8_t
             LSDTT_LAUNCH_CHECK();
         },
         [&] {
-            k_sum_i32<<<1, 256, 0, s>>>(tile_eroded.p, ntiles, d_sum);
+            k_sum_i32<<<1, kSumThreads, 0, s>>>(tile_eroded.p, ntiles, d_sum);
             LSDTT_LAUNCH_CHECK();
         },
         s);
     last_sp_passes = passes;
-    std::memcpy(eroded, flags_h.data() + 2 * max_passes, sizeof(long long));
+    std::memcpy(eroded, flags_h.data(), sizeof(long long));
     return passes;
 }
 

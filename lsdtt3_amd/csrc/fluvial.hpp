@@ -55,7 +55,7 @@ public:
     int nr = 0, nc = 0, ntx = 0, nty = 0, ntiles = 0, max_passes = 0;
     long long tile_runs = 0;  // tiles processed by the last counts / stream_power
     BC bc_{'o', 'o', 'o', 'o'};
-    dbuf<int> act[3], flags, tile_eroded, tile_maxdep;
+    dbuf<int> act[3], flags, tile_eroded, tile_maxdep, tile_run;
     hbuf<int> flags_h;
     KernelTimer* timer = nullptr;  // optional: each pass timed as one launch
     dbuf<short> ord, lstart;  // tile-major level order
