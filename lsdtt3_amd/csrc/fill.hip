@@ -130,10 +130,12 @@ __global__ void k_fill_jump(int* __restrict__ ptr, long long n, const int* __res
     if (k >= n) return;
     const int p = __hip_atomic_load(&ptr[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int q = __hip_atomic_load(&ptr[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (q != p) {
-        __hip_atomic_store(&ptr[k], q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        *cur = 1;
-    }
+    const bool ch = q != p;
+    if (ch) __hip_atomic_store(&ptr[k], q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // one flag store per wavefront (stores of every thread to one address
+    // would queue at its memory channel)
+    const unsigned long long m = __ballot(ch);
+    if (ch && (threadIdx.x & 63) == __ffsll(m) - 1) *cur = 1;
 }
 
 // a cell whose descent path ends at an exit already has its spill level:

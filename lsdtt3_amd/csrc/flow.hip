@@ -232,9 +232,11 @@ __global__ void k_pj_round(unsigned long long* __restrict__ P, long long n, cons
     if (nx == f) return;
     const unsigned long long wn = ld_relaxed(&P[nx]);
     const unsigned nn = static_cast<unsigned>(wn);
-    if (nn == nx) return;
-    st_relaxed(&P[f], pack(static_cast<unsigned>(w >> 32) + static_cast<unsigned>(wn >> 32), nn));
-    *cur = 1;
+    const bool ch = nn != nx;
+    if (ch) st_relaxed(&P[f], pack(static_cast<unsigned>(w >> 32) + static_cast<unsigned>(wn >> 32), nn));
+    // one flag store per wavefront
+    const unsigned long long m = __ballot(ch);
+    if (ch && (threadIdx.x & 63) == __ffsll(m) - 1) *cur = 1;
 }
 
 // value(v) = acc(v) + acc(root) for non-roots; acc(v) for roots.  Also block
