@@ -30,6 +30,7 @@ three.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-learned]
        python bench.py --workload ensemble   (64-instance batched ensemble)
        python bench.py --workload cnn        (the U-Net alone: MFMA utilisation)
+       python bench.py --workload routing    (fill + D8 flow object + chi on a 4096^2 FP32 DEM)
 
 Profile summaries (text, for profiles/) from rocprofv3 CSV output:
   python bench.py --summarize-trace DIR OUT [TITLE CMD NOTE]   (--kernel-trace --stats run)
@@ -380,6 +381,89 @@ def bench_cnn(args):
     print(json.dumps(out))
 
 
+def bench_routing(args):
+    """The routing pipeline on a device-resident float32 DEM: epsilon-free
+    fill -> D8 flow object (pits as outlets; donors, Kahn order, s-vector,
+    int64 counts) -> chi raster (theta 0.45, A0 = 1 m^2) from the counts
+    (area = count x 100 m^2).  Surface: power-law noise (beta 2.2, 100 m std)
+    with closed bowls on ~1% of the cells (depth U(0, 20] m) and exactly flat
+    plateaus on ~5%, dx = 10 m, boundary "oooo"."""
+    import ctypes as C
+    from lsdtt3_amd import _native as N
+    from lsdtt3_amd.synthetic import routing_surface
+    L = N.lib()
+    n = args.size
+    z = routing_surface(n, n, seed=2024).astype(np.float32)
+    dz = N.DeviceArray.from_host(z)
+    dm = N.DeviceArray.from_host(np.ones((n, n), np.uint8))
+    dfill = N.DeviceArray((n, n), np.float32)
+    dchi = N.DeviceArray((n, n), np.float32)
+    o = N.FlowOptions()
+    L.lsdtt_gpu_flow_options_init(C.byref(o), n, n, 10.0)
+    o.boundary_conditions = b"oooo"
+    o.allow_pits_as_outlets = 1
+    fh = L.lsdtt_gpu_flow_create(C.byref(o))
+    if not fh:
+        raise RuntimeError((L.lsdtt_gpu_last_error() or b"").decode())
+
+    def stage_fill():
+        N.check(L.lsdtt_gpu_fill_device(dz.ptr, dm.ptr, n, n, -9999.0, b"oooo", dfill.ptr, None))
+
+    def stage_flow():
+        N.check(L.lsdtt_gpu_flow_build_device(fh, dfill.ptr, 0, dm.ptr, None))
+
+    def stage_chi():
+        N.check(L.lsdtt_gpu_flow_chi_device(fh, 0.45, 1.0, -9999.0, dchi.ptr, None))
+
+    stages = [("fill", stage_fill), ("flow_object", stage_flow), ("chi", stage_chi)]
+    for _ in range(max(1, args.warmup)):
+        for _, f in stages:
+            f()
+    N.synchronize()
+    # whole pipeline, no synchronisation between stages
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for _, f in stages:
+            f()
+    N.synchronize()
+    dt = time.perf_counter() - t0
+    # stage breakdown (each stage timed alone, synchronised)
+    per = {}
+    for k, f in stages:
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            f()
+        N.synchronize()
+        per[k] = 1e3 * (time.perf_counter() - t1) / args.steps
+    # the result is the oracle's (bit-exact parity is the routing tests' job);
+    # here only a sanity check that chi was written
+    chi = dchi.to_host()
+    assert np.isfinite(chi).any()
+    nv, no, md = C.c_int64(), C.c_int64(), C.c_int64()
+    N.check(L.lsdtt_gpu_flow_sizes(fh, C.byref(nv), C.byref(no), C.byref(md)))
+    L.lsdtt_gpu_flow_destroy(fh)
+    cells = n * n
+    value = cells * args.steps / dt
+    # algorithmic bytes per cell of the pipeline (each array element a
+    # kernel must read or write counted once): fill 9 (z 4, mask 1, filled
+    # 4), receivers 10, donors + outlets 5, Kahn order 13, counts 24,
+    # s-vector 28, chi on every valid cell 29
+    bpc = 9 + 10 + 5 + 13 + 24 + 28 + 29
+    achieved = bpc * cells / (dt / args.steps) / 1e9
+    out = {"metric": "routing pipeline cells/s at 4096^2 (fill + D8 flow object + chi, bit-exact)",
+           "value": round(value, 1), "unit": "cells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(1e3 * dt / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32 z, int64 counts", "data": "synthetic",
+           "config": {"workload": f"routing surface {n}x{n} dx=10 (beta 2.2, 100 m std, 1% closed bowls, 5% "
+                                  f"flat plateaus), bc oooo, pits as outlets, theta 0.45, A0 1",
+                      "n_valid": int(nv.value), "n_outlets": int(no.value), "max_depth": int(md.value),
+                      "stage_ms": {k: round(v, 3) for k, v in per.items()}},
+           "roofline": {"bound": "hbm", "kernel": "whole pipeline", "achieved": round(achieved, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": None, "algorithmic_bytes_per_cell": bpc}}
+    print(json.dumps(out))
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-trace":
         summarize_trace(*sys.argv[2:7])
@@ -394,7 +478,7 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["lem4096", "ensemble", "cnn"], default="lem4096")
+    ap.add_argument("--workload", choices=["lem4096", "ensemble", "cnn", "routing"], default="lem4096")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--sequential", action="store_true", help="ensemble: also time one instance at a time")
@@ -419,11 +503,11 @@ def main():
     if args.workload == "cnn":
         bench_cnn(args)
         return
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is plumbing only
-        sync = lambda: None  # noqa: E731
+    if args.workload == "routing":
+        bench_routing(args)
+        return
+    from lsdtt3_amd import _native as N
+    sync = N.synchronize
 
     z, K, passes = workload(args.size)
     lem = lsdtt3_amd.LandscapeModel(z, 30.0, **lem_kwargs(K))

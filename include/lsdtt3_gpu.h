@@ -40,6 +40,15 @@ int lsdtt_gpu_available(void);
 /* Message of the last failing call on this thread ("" if none). */
 const char* lsdtt_gpu_last_error(void);
 
+/* Device memory for the *_device entry points (no counterpart in the
+ * reference, whose arrays live on the host): plain hipMalloc / hipFree /
+ * hipMemcpy on the current device.  kind: 1 host to device, 2 device to
+ * host, 3 device to device.  _synchronize waits for all work on the device. */
+void* lsdtt_gpu_device_alloc(int64_t bytes);
+void lsdtt_gpu_device_free(void* p);
+int lsdtt_gpu_memcpy(void* dst, const void* src, int64_t bytes, int32_t kind);
+int lsdtt_gpu_synchronize(void);
+
 /* ---------------------------------------------------------------------
  * D8 flow object.  Replaces constructing lsdtt_flowinfo
  * (src/cpu/lsdtt_flowinfo_cpu.cpp:183-253, options

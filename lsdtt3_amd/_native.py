@@ -56,6 +56,10 @@ def lib():
     vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
     _sig(L, "lsdtt_gpu_available", C.c_int, [])
     _sig(L, "lsdtt_gpu_last_error", C.c_char_p, [])
+    _sig(L, "lsdtt_gpu_device_alloc", vp, [i64])
+    _sig(L, "lsdtt_gpu_device_free", None, [vp])
+    _sig(L, "lsdtt_gpu_memcpy", C.c_int, [vp, vp, i64, i32])
+    _sig(L, "lsdtt_gpu_synchronize", C.c_int, [])
     _sig(L, "lsdtt_gpu_flow_options_init", None, [C.POINTER(FlowOptions), i32, i32, dbl])
     _sig(L, "lsdtt_gpu_flow_create", vp, [C.POINTER(FlowOptions)])
     _sig(L, "lsdtt_gpu_flow_destroy", None, [vp])
@@ -93,3 +97,42 @@ def ptr(a) -> C.c_void_p:
     if a is None:
         return None
     return C.c_void_p(a.ctypes.data)
+
+
+class DeviceArray:
+    """Device memory of the library (for the *_device entry points): a
+    numpy-shaped buffer with explicit uploads and downloads."""
+
+    def __init__(self, shape, dtype):
+        import numpy as np
+        self.shape = tuple(shape) if hasattr(shape, "__len__") else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = lib().lsdtt_gpu_device_alloc(max(1, self.nbytes))
+        if not p:
+            raise GpuError(ERR_RUNTIME, (lib().lsdtt_gpu_last_error() or b"").decode())
+        self.ptr = C.c_void_p(p)
+
+    @classmethod
+    def from_host(cls, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        check(lib().lsdtt_gpu_memcpy(d.ptr, C.c_void_p(a.ctypes.data), d.nbytes, 1))
+        return d
+
+    def to_host(self):
+        import numpy as np
+        out = np.empty(self.shape, self.dtype)
+        check(lib().lsdtt_gpu_memcpy(C.c_void_p(out.ctypes.data), self.ptr, self.nbytes, 2))
+        return out
+
+    def __del__(self):
+        p = getattr(self, "ptr", None)
+        if p is not None and _lib is not None:
+            _lib.lsdtt_gpu_device_free(p)
+            self.ptr = None
+
+
+def synchronize() -> None:
+    check(lib().lsdtt_gpu_synchronize())

@@ -131,6 +131,41 @@ int lsdtt_gpu_available(void)
 
 const char* lsdtt_gpu_last_error(void) { return last_error_cstr(); }
 
+void* lsdtt_gpu_device_alloc(int64_t bytes)
+{
+    return guarded_ptr<void>([&]() -> void* {
+        if (bytes <= 0) throw std::invalid_argument("allocation size must be positive");
+        void* p = nullptr;
+        LSDTT_HIP(hipMalloc(&p, static_cast<size_t>(bytes)));
+        return p;
+    });
+}
+
+void lsdtt_gpu_device_free(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+int lsdtt_gpu_memcpy(void* dst, const void* src, int64_t bytes, int32_t kind)
+{
+    return guarded([&] {
+        if ((!dst || !src) && bytes > 0) return fail_arg("null argument");
+        if (bytes < 0 || kind < 1 || kind > 3) return fail_arg("bytes >= 0 and kind 1, 2 or 3");
+        const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                                          : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+        if (bytes) LSDTT_HIP(hipMemcpy(dst, src, static_cast<size_t>(bytes), k));
+        return LSDTT_GPU_OK;
+    });
+}
+
+int lsdtt_gpu_synchronize(void)
+{
+    return guarded([&] {
+        LSDTT_HIP(hipDeviceSynchronize());
+        return LSDTT_GPU_OK;
+    });
+}
+
 void lsdtt_gpu_flow_options_init(lsdtt_gpu_flow_options* o, int32_t n_rows, int32_t n_cols, double cell_size)
 {
     if (!o) return;
