@@ -18,15 +18,38 @@ __global__ void k_zero_u32(unsigned* __restrict__ a, long long n)
     if (i < n) a[i] = 0u;
 }
 
-// bit k of `bits` set when some valid cell has upstream count k
-__global__ void k_count_presence(const uint8_t* __restrict__ mask, const int* __restrict__ cnt, long long n,
-                                 unsigned* __restrict__ bits)
+// bit k of `bits` set when some valid cell has upstream count k.  Counts
+// below kLdsBits are gathered in a workgroup bitmap first (most cells have
+// small counts: one global atomic per word per workgroup instead of one per
+// cell on the same few words); larger counts test the word before the
+// atomic.
+constexpr int kLdsBits = 8192;
+__global__ void __launch_bounds__(kB) k_count_presence(const uint8_t* __restrict__ mask, const int* __restrict__ cnt,
+                                                       long long n, unsigned* __restrict__ bits)
 {
-    const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (f < n && mask[f]) {
+    __shared__ unsigned lb[kLdsBits / 32];
+    for (int i = threadIdx.x; i < kLdsBits / 32; i += kB) lb[i] = 0u;
+    __syncthreads();
+    for (long long f = blockIdx.x * (long long)kB + threadIdx.x; f < n; f += (long long)gridDim.x * kB) {
+        const uint8_t m = mask[f];
         const int k = cnt[f];
-        atomicOr(&bits[k >> 5], 1u << (k & 31));
+        if (!m) continue;
+        const unsigned bit = 1u << (k & 31);
+        if (k < kLdsBits) {
+            if (!(lb[k >> 5] & bit)) atomicOr(&lb[k >> 5], bit);
+        } else if (!(__hip_atomic_load(&bits[k >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit)) {
+            atomicOr(&bits[k >> 5], bit);
+        }
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kLdsBits / 32; i += kB)
+        if (lb[i]) atomicOr(&bits[i], lb[i]);
+}
+
+__global__ void k_word_popc(const unsigned* __restrict__ bits, long long nw, int* __restrict__ pc)
+{
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < nw) pc[i] = __popc(bits[i]);
 }
 
 __global__ void k_chi_init(const uint8_t* __restrict__ mask, const int* __restrict__ rcv, long long n, float nodata,
@@ -40,10 +63,19 @@ struct ChiArgs {
     const int* rcv;
     const uint8_t* dir;
     const int* cnt;
-    const double* tab;
+    const double* vals;      // f(k) of the counts present, in increasing k
+    const unsigned* bits;    // presence bitmap of the counts
+    const int* wrank;        // set bits in the words before each word
     double dx;
     float* chi;
 };
+
+// f(k): the rank of k among the counts present indexes the compact table
+__device__ __forceinline__ double f_of(const ChiArgs& a, int k)
+{
+    const int w = k >> 5;
+    return a.vals[a.wrank[w] + __popc(a.bits[w] & ((1u << (k & 31)) - 1u))];
+}
 
 // lsdtt_chi_extraction_cpu.cpp:355-361: (step * 0.5) * (f_v + f_r) added to
 // the receiver's float chi in double, rounded to float
@@ -51,7 +83,7 @@ __device__ __forceinline__ void chi_node(const ChiArgs& a, int v)
 {
     const int r = a.rcv[v];
     const double step = (a.dir[v] & 1u) ? a.dx * kSqrt2 : a.dx;
-    const double inc = step * 0.5 * (a.tab[a.cnt[v]] + a.tab[a.cnt[r]]);
+    const double inc = step * 0.5 * (f_of(a, a.cnt[v]) + f_of(a, a.cnt[r]));
     a.chi[v] = static_cast<float>(static_cast<double>(a.chi[r]) + inc);
 }
 
@@ -108,30 +140,33 @@ void ChiEngine::raster(FlowEngine& e, double m_over_n, double A0, float nodata, 
     bits.reserve(nw);
     k_zero_u32<<<div_up(nw, kB), kB, 0, s>>>(bits.p, nw);
     LSDTT_LAUNCH_CHECK();
-    k_count_presence<<<g, kB, 0, s>>>(e.mask, e.cnt, n, bits.p);
+    k_count_presence<<<std::min(g, 2048), kB, 0, s>>>(e.mask, e.cnt, n, bits.p);
     LSDTT_LAUNCH_CHECK();
+    // rank table: set bits before each word
+    wpc.reserve(nw + 1);
+    wrank.reserve(nw + 1);
+    k_word_popc<<<div_up(nw, kB), kB, 0, s>>>(bits.p, nw, wpc.p);
+    LSDTT_LAUNCH_CHECK();
+    size_t tb = 0;
+    LSDTT_HIP(rocprim::exclusive_scan(nullptr, tb, wpc.p, wrank.p, 0, (size_t)nw, rocprim::plus<int>(), s));
+    LSDTT_HIP(rocprim::exclusive_scan(tmp.need(tb), tb, wpc.p, wrank.p, 0, (size_t)nw, rocprim::plus<int>(), s));
     bits_h.resize(static_cast<size_t>(nw));
     LSDTT_HIP(hipMemcpyAsync(bits_h.data(), bits.p, sizeof(unsigned) * nw, hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipStreamSynchronize(s));
-    long long kmax = 0;
-    for (long long w = nw - 1; w >= 0; --w)
-        if (bits_h[w]) {
-            kmax = w * 32 + 31 - __builtin_clz(bits_h[w]);
-            break;
-        }
     // f(k) = pow(A0 / (k * dx^2), theta), the host libm's values
-    // (lsdtt_chi_extraction_cpu.cpp:154-163 with A from :748-751)
+    // (lsdtt_chi_extraction_cpu.cpp:154-163 with A from :748-751), for the
+    // counts present only, in increasing k
     const double pa = e.opt.dx * e.opt.dx;
-    table_h.assign(static_cast<size_t>(kmax) + 1, 0.0);
+    table_h.clear();
     for (long long w = 0; w < nw; ++w)
         for (unsigned b = bits_h[w]; b; b &= b - 1) {
             const long long k = w * 32 + __builtin_ctz(b);
             const double denom = static_cast<double>(k) * pa;
-            table_h[static_cast<size_t>(k)] = std::pow(A0 / denom, m_over_n);
+            table_h.push_back(std::pow(A0 / denom, m_over_n));
         }
-    table.reserve(kmax + 1);
-    LSDTT_HIP(hipMemcpyAsync(table.p, table_h.data(), sizeof(double) * (kmax + 1), hipMemcpyHostToDevice, s));
-    ChiArgs a{e.rcv.p, e.dir.p, e.cnt.p, table.p, e.opt.dx, chi};
+    table.reserve(std::max<size_t>(table_h.size(), 1));
+    LSDTT_HIP(hipMemcpyAsync(table.p, table_h.data(), sizeof(double) * table_h.size(), hipMemcpyHostToDevice, s));
+    ChiArgs a{e.rcv.p, e.dir.p, e.cnt.p, table.p, bits.p, wrank.p, e.opt.dx, chi};
     const int nl = e.max_depth + 1;
     const int split = std::max(e.tail_from, 1);
     for (int L = 1; L < split && L < nl; ++L) {

@@ -7,8 +7,9 @@
 // theta) depends only on the receiver's final value, so the depth-from-outlet
 // levels of the flow object give the reference's values exactly.  f is
 // evaluated on the host with the C library's pow for every count present in
-// the grid (a table indexed by count), which keeps chi bit-identical to a
-// host evaluation with the same libm.
+// the grid (a compact table indexed by the count's rank among the counts
+// present), which keeps chi bit-identical to a host evaluation with the same
+// libm.
 #pragma once
 
 #include <vector>
@@ -29,6 +30,7 @@ public:
     long long threshold(FlowEngine& e, const float* chi, long long min_pixels, hipStream_t s);
 
     dbuf<unsigned> bits;
+    dbuf<int> wpc, wrank;
     dbuf<double> table;
     dbuf<int> flag, pos, sel_flat, sel_rcv;
     dbuf<float> sel_chi;
