@@ -282,6 +282,20 @@ double* lsdtt_gpu_lem_elevation_device(lsdtt_gpu_lem* lem);
 int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary,
                        lsdtt_gpu_lem_stats* stats);
 
+/* Tile shape of the upstream-count and stream-power sweeps: tiles of 32
+ * columns by tile_height rows (8, 16 or 32; default 16).  No counterpart in
+ * the reference, whose serial sweeps have no tiles
+ * (lsdtt_stream_power_cpu.cpp:218-319); the results do not depend on it.
+ * _autotune_tiles times `steps` steps (after one untimed step) with each
+ * height from the current surface, restores the surface after each, keeps
+ * the fastest (the current height unless another is more than 3% faster)
+ * and returns it in *tile_height; ms_per_step (3 entries, may be NULL)
+ * receives the times of heights 8, 16 and 32. */
+int lsdtt_gpu_lem_set_tile_height(lsdtt_gpu_lem* lem, int32_t tile_height);
+int32_t lsdtt_gpu_lem_tile_height(lsdtt_gpu_lem* lem);
+int lsdtt_gpu_lem_autotune_tiles(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary,
+                                 int32_t steps, int32_t* tile_height, double* ms_per_step);
+
 /* ---------------------------------------------------------------------
  * Batched ensemble: n independent LEM instances (uneven grids, per-instance
  * options) stepped concurrently on one GPU.  No counterpart in the
@@ -303,6 +317,18 @@ int lsdtt_gpu_ensemble_get_elevation(lsdtt_gpu_ensemble* e, int32_t instance, do
  * the first launch to the last instance's completion. */
 int lsdtt_gpu_ensemble_run(lsdtt_gpu_ensemble* e, int32_t n_steps, lsdtt_gpu_lem_stats* last_stats,
                            double* seconds);
+/* Tile height of instance i's routing sweeps (as lsdtt_gpu_lem_set_tile_height). */
+int lsdtt_gpu_ensemble_set_tile_height(lsdtt_gpu_ensemble* e, int32_t instance, int32_t tile_height);
+/* Chooses every instance's tile height by measurement: per size class
+ * (octave of the cell count) the height that steps the class's largest
+ * instance fastest alone; then `steps` concurrent steps of the whole
+ * ensemble with the current heights, each uniform height and the per-class
+ * heights, from the same surfaces (restored afterwards); an assignment
+ * replaces the current one only when it is more than 3% faster.
+ * tile_heights (n_instances entries, may be NULL) receives it, *seconds
+ * (may be NULL) the time the tuning took. */
+int lsdtt_gpu_ensemble_autotune_tiles(lsdtt_gpu_ensemble* e, int32_t steps, int32_t* tile_heights,
+                                      double* seconds);
 
 /* Per-kernel timing with HIP events recorded on the step's stream (no
  * counterpart in the reference; for roofline measurement).  enable != 0

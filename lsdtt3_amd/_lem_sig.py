@@ -65,6 +65,11 @@ def declare(L, sig):
     sig(L, "lsdtt_gpu_ensemble_set_elevation", C.c_int, [vp, C.c_int32, vp])
     sig(L, "lsdtt_gpu_ensemble_get_elevation", C.c_int, [vp, C.c_int32, vp])
     sig(L, "lsdtt_gpu_ensemble_run", C.c_int, [vp, C.c_int32, vp, vp])
+    sig(L, "lsdtt_gpu_ensemble_set_tile_height", C.c_int, [vp, i32, i32])
+    sig(L, "lsdtt_gpu_ensemble_autotune_tiles", C.c_int, [vp, i32, vp, vp])
+    sig(L, "lsdtt_gpu_lem_set_tile_height", C.c_int, [vp, i32])
+    sig(L, "lsdtt_gpu_lem_tile_height", i32, [vp])
+    sig(L, "lsdtt_gpu_lem_autotune_tiles", C.c_int, [vp, vp, i64, i32, vp, vp])
     sig(L, "lsdtt_gpu_lem_elevation_device", C.c_void_p, [vp])
     sig(L, "lsdtt_gpu_lem_step", C.c_int, [vp, vp, i64, C.POINTER(LemStats)])
     sig(L, "lsdtt_gpu_build_boundary_flats", C.c_int, [i32, i32, vp, C.c_char_p, vp, vp])

@@ -6,13 +6,16 @@
 // runs all of an instance's steps back to back; the step's host decisions
 // (pass settling, Krylov and Newton convergence) only wait on that
 // instance's stream, so the kernels of different instances overlap on the
-// GPU and small grids that cannot fill it alone share it.
+// GPU and small grids that cannot fill it alone share it.  The routing
+// sweeps' tile height is chosen per instance by measurement
+// (lsdtt_gpu_ensemble_autotune_tiles).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -52,6 +55,45 @@ struct lsdtt_gpu_ensemble {
             i.eng.reset();
             if (i.stream) (void)hipStreamDestroy(i.stream);
         }
+    }
+
+    // n_steps steps of every instance, concurrently; wall seconds.  A pool
+    // of host workers takes instances largest first (the longest start at
+    // once) and runs all of an instance's steps back to back.
+    double run_all(int n_steps)
+    {
+        const int m = static_cast<int>(inst.size());
+        std::vector<int> order(static_cast<size_t>(m));
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return (long long)inst[a].nr * inst[a].nc > (long long)inst[b].nr * inst[b].nc;
+        });
+        std::atomic<int> next{0};
+        LSDTT_HIP(hipDeviceSynchronize());
+        const auto t0 = std::chrono::steady_clock::now();
+        auto work = [&] {
+            for (;;) {
+                const int k = next.fetch_add(1);
+                if (k >= m) return;
+                Instance& I = inst[static_cast<size_t>(order[static_cast<size_t>(k)])];
+                try {
+                    for (int s = 0; s < n_steps; ++s) I.rc |= I.eng->step(&I.last, I.stream);
+                    LSDTT_HIP(hipStreamSynchronize(I.stream));
+                } catch (const std::exception& ex) {
+                    I.rc = -1;
+                    I.err = ex.what();
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int w = 0; w < workers; ++w) pool.emplace_back(work);
+        for (auto& t : pool) t.join();
+        LSDTT_HIP(hipDeviceSynchronize());
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (int i = 0; i < m; ++i)
+            if (inst[static_cast<size_t>(i)].rc < 0)
+                throw std::runtime_error("instance " + std::to_string(i) + ": " + inst[static_cast<size_t>(i)].err);
+        return sec;
     }
 };
 
@@ -122,44 +164,108 @@ int lsdtt_gpu_ensemble_run(lsdtt_gpu_ensemble* e, int32_t n_steps, lsdtt_gpu_lem
         if (!e || n_steps < 0) return fail_arg("null ensemble or negative step count");
         for (auto& I : e->inst)
             if (!I.z_set) return fail_arg("elevation not set for every instance");
-        const int m = static_cast<int>(e->inst.size());
-        // largest first: the longest instances start at once
-        std::vector<int> order(static_cast<size_t>(m));
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-            return (long long)e->inst[a].nr * e->inst[a].nc > (long long)e->inst[b].nr * e->inst[b].nc;
-        });
-        std::atomic<int> next{0};
+        try {
+            e->last_seconds = e->run_all(n_steps);
+        } catch (const std::runtime_error& ex) {
+            set_error(ex.what());
+            return LSDTT_GPU_ERR_RUNTIME;
+        }
+        if (seconds) *seconds = e->last_seconds;
+        if (last_stats)
+            for (size_t i = 0; i < e->inst.size(); ++i) to_c_stats(e->inst[i].last, &last_stats[i]);
+        return LSDTT_GPU_OK;
+    });
+}
+
+int lsdtt_gpu_ensemble_set_tile_height(lsdtt_gpu_ensemble* e, int32_t i, int32_t tile_height)
+{
+    return guarded([&] {
+        if (!e || i < 0 || i >= static_cast<int>(e->inst.size())) return fail_arg("bad instance or null");
+        if (!FluvialTiles::valid_tile_height(tile_height)) return fail_arg("tile height must be 8, 16 or 32");
         LSDTT_HIP(hipDeviceSynchronize());
-        const auto t0 = std::chrono::steady_clock::now();
-        auto work = [&] {
-            for (;;) {
-                const int k = next.fetch_add(1);
-                if (k >= m) return;
-                Instance& I = e->inst[static_cast<size_t>(order[static_cast<size_t>(k)])];
-                try {
-                    for (int s = 0; s < n_steps; ++s) I.rc |= I.eng->step(&I.last, I.stream);
-                    LSDTT_HIP(hipStreamSynchronize(I.stream));
-                } catch (const std::exception& ex) {
-                    I.rc = -1;
-                    I.err = ex.what();
+        e->inst[static_cast<size_t>(i)].eng->tiles.set_tile_height(tile_height);
+        return LSDTT_GPU_OK;
+    });
+}
+
+int lsdtt_gpu_ensemble_autotune_tiles(lsdtt_gpu_ensemble* e, int32_t steps, int32_t* tile_heights,
+                                      double* seconds)
+{
+    return guarded([&] {
+        if (!e || steps < 1) return fail_arg("null ensemble or fewer than one step");
+        for (auto& I : e->inst)
+            if (!I.z_set) return fail_arg("elevation not set for every instance");
+        const size_t m = e->inst.size();
+        const auto t_begin = std::chrono::steady_clock::now();
+        // 1. per size class (octave of the cell count), the height that runs
+        //    the class's largest instance fastest on its own
+        auto cells = [&](size_t i) { return (long long)e->inst[i].nr * e->inst[i].nc; };
+        std::vector<int> cls(m), tuned(m, 16);
+        std::map<int, size_t> rep;  // size class -> its largest instance
+        for (size_t i = 0; i < m; ++i) {
+            cls[i] = 63 - __builtin_clzll(static_cast<unsigned long long>(cells(i)));
+            auto it = rep.find(cls[i]);
+            if (it == rep.end()) rep.emplace(cls[i], i);
+            else if (cells(i) > cells(it->second)) it->second = i;
+        }
+        for (const auto& [c, r] : rep) {
+            Instance& I = e->inst[r];
+            const int th = I.eng->autotune_tiles(steps, nullptr, I.stream);
+            for (size_t i = 0; i < m; ++i)
+                if (cls[i] == c) tuned[i] = th;
+        }
+        // 2. the instances share the GPU when they run together, which can
+        //    favour other heights than running alone: time the ensemble with
+        //    each uniform height and with the per-class heights, from the
+        //    same surfaces, and keep the fastest assignment
+        std::vector<dbuf<double>> saved(m);
+        for (size_t i = 0; i < m; ++i) {
+            Instance& I = e->inst[i];
+            saved[i].reserve(static_cast<size_t>(I.eng->n));
+            LSDTT_HIP(hipMemcpy(saved[i].p, I.eng->z.p, sizeof(double) * I.eng->n, hipMemcpyDeviceToDevice));
+        }
+        auto restore = [&] {
+            for (size_t i = 0; i < m; ++i)
+                LSDTT_HIP(hipMemcpy(e->inst[i].eng->z.p, saved[i].p, sizeof(double) * e->inst[i].eng->n,
+                                    hipMemcpyDeviceToDevice));
+        };
+        // candidate 0 is the current assignment; another replaces it only
+        // when it is faster by more than kMargin (trials of a few steps
+        // carry a few per cent of noise)
+        constexpr double kMargin = 0.03;
+        std::vector<std::vector<int>> cands(1, std::vector<int>(m));
+        for (size_t i = 0; i < m; ++i) cands[0][i] = e->inst[i].eng->tiles.tile_height();
+        for (int h : FluvialTiles::kTileHeights) cands.emplace_back(m, h);
+        cands.push_back(tuned);
+        double cur_sec = 0.0, best_sec = 0.0;
+        size_t best = 0;
+        try {
+            for (size_t c = 0; c < cands.size(); ++c) {
+                if (c > 0 && std::find(cands.begin(), cands.begin() + static_cast<long>(c), cands[c]) !=
+                                 cands.begin() + static_cast<long>(c))
+                    continue;  // timed already
+                for (size_t i = 0; i < m; ++i) e->inst[i].eng->tiles.set_tile_height(cands[c][i]);
+                e->run_all(1);  // settles the pass and Krylov heuristics
+                restore();
+                const double sec = e->run_all(steps);
+                restore();
+                if (c == 0) cur_sec = best_sec = sec;
+                else if (sec < best_sec && sec < (1.0 - kMargin) * cur_sec) {
+                    best_sec = sec;
+                    best = c;
                 }
             }
-        };
-        std::vector<std::thread> pool;
-        for (int w = 0; w < e->workers; ++w) pool.emplace_back(work);
-        for (auto& t : pool) t.join();
-        LSDTT_HIP(hipDeviceSynchronize());
-        e->last_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (seconds) *seconds = e->last_seconds;
-        for (int i = 0; i < m; ++i) {
-            Instance& I = e->inst[static_cast<size_t>(i)];
-            if (I.rc < 0) {
-                set_error("instance " + std::to_string(i) + ": " + I.err);
-                return LSDTT_GPU_ERR_RUNTIME;
-            }
-            if (last_stats) to_c_stats(I.last, &last_stats[i]);
+        } catch (const std::runtime_error& ex) {
+            restore();
+            set_error(ex.what());
+            return LSDTT_GPU_ERR_RUNTIME;
         }
+        for (size_t i = 0; i < m; ++i) {
+            e->inst[i].eng->tiles.set_tile_height(cands[best][i]);
+            if (tile_heights) tile_heights[i] = cands[best][i];
+        }
+        LSDTT_HIP(hipDeviceSynchronize());
+        if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
         return LSDTT_GPU_OK;
     });
 }

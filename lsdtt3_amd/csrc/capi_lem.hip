@@ -237,19 +237,33 @@ double* lsdtt_gpu_lem_elevation_device(lsdtt_gpu_lem* lem)
     return lem->eng->elevation_device();
 }
 
+}  // extern "C"
+
+namespace {
+
+// uploads the fixed-elevation set when it differs from the last one
+void use_boundary_flats(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary)
+{
+    if (!lem->bf_set || lem->bf_cache.size() != static_cast<size_t>(n_boundary) ||
+        (n_boundary && std::memcmp(lem->bf_cache.data(), boundary_flats, sizeof(int64_t) * n_boundary) != 0)) {
+        lem->bf_cache.assign(boundary_flats, boundary_flats + n_boundary);
+        std::vector<long long> v(lem->bf_cache.begin(), lem->bf_cache.end());
+        lem->eng->set_boundary_flats(v.data(), n_boundary);
+        lem->bf_set = true;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
 int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary,
                        lsdtt_gpu_lem_stats* out)
 {
     return guarded([&] {
         if (!lem || !out || (n_boundary > 0 && !boundary_flats)) return fail_arg("null argument");
         if (!lem->z_set) return fail_arg("elevation not set");
-        if (!lem->bf_set || lem->bf_cache.size() != static_cast<size_t>(n_boundary) ||
-            (n_boundary && std::memcmp(lem->bf_cache.data(), boundary_flats, sizeof(int64_t) * n_boundary) != 0)) {
-            lem->bf_cache.assign(boundary_flats, boundary_flats + n_boundary);
-            std::vector<long long> v(lem->bf_cache.begin(), lem->bf_cache.end());
-            lem->eng->set_boundary_flats(v.data(), n_boundary);
-            lem->bf_set = true;
-        }
+        use_boundary_flats(lem, boundary_flats, n_boundary);
         LemStats st;
         const int rc = lem->eng->step(&st, nullptr);
         to_c_stats(st, out);
@@ -257,6 +271,33 @@ int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_
             set_error("hillslope Newton did not converge at the minimum sub-step");
             return LSDTT_GPU_ERR_NOT_CONVERGED;
         }
+        return LSDTT_GPU_OK;
+    });
+}
+
+int lsdtt_gpu_lem_set_tile_height(lsdtt_gpu_lem* lem, int32_t tile_height)
+{
+    return guarded([&] {
+        if (!lem) return fail_arg("null argument");
+        if (!FluvialTiles::valid_tile_height(tile_height)) return fail_arg("tile height must be 8, 16 or 32");
+        LSDTT_HIP(hipDeviceSynchronize());
+        lem->eng->tiles.set_tile_height(tile_height);
+        return LSDTT_GPU_OK;
+    });
+}
+
+int32_t lsdtt_gpu_lem_tile_height(lsdtt_gpu_lem* lem) { return lem ? lem->eng->tiles.tile_height() : 0; }
+
+int lsdtt_gpu_lem_autotune_tiles(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary,
+                                 int32_t steps, int32_t* tile_height, double* ms_per_step)
+{
+    return guarded([&] {
+        if (!lem || (n_boundary > 0 && !boundary_flats)) return fail_arg("null argument");
+        if (!lem->z_set) return fail_arg("elevation not set");
+        if (steps < 1) return fail_arg("autotune needs at least one step per tile height");
+        use_boundary_flats(lem, boundary_flats, n_boundary);
+        const int th = lem->eng->autotune_tiles(steps, ms_per_step, nullptr);
+        if (tile_height) *tile_height = th;
         return LSDTT_GPU_OK;
     });
 }

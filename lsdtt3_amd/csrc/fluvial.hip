@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <type_traits>
 #include <utility>
 
 #include "fluvial.hpp"
@@ -25,24 +26,35 @@ namespace lsdtt_gpu {
 
 namespace {
 
-constexpr int TW = FluvialTiles::TW, TH = FluvialTiles::TH, TN = FluvialTiles::TN;
-constexpr int WL = 64;            // lanes: one wavefront per tile
-constexpr int PER = TN / WL;      // cells per lane
-constexpr int kJumpRounds = 10;   // 2^10 >= TN
+// Tiles are TW = 32 columns by TH rows, TH one of FluvialTiles::kTileHeights
+// (a template parameter of every tile kernel); a tile's TN = TW * TH cells
+// are spread over one wavefront, PER = TN / WL per lane.
+constexpr int TW = FluvialTiles::TW;
+constexpr int WL = 64;  // lanes: one wavefront per tile
 
-static_assert(TW == 32 && TN % WL == 0, "lane layout assumes 32-wide tiles");
-static_assert((1 << kJumpRounds) >= TN, "pointer jumping rounds too few");
+template <int TH>
+struct TileShape {
+    static constexpr int TN = TW * TH;
+    static constexpr int PER = TN / WL;
+    // pointer-jumping rounds: 2^rounds >= TN bounds any in-tile path
+    static constexpr int kJumpRounds = TN <= 256 ? 8 : (TN <= 512 ? 9 : 10);
+    static_assert(TW == 32 && TN % WL == 0 && TH >= 2, "lane layout assumes 32-wide tiles of even height");
+    static_assert(TH <= 32, "ring slots cover at most 32 rows");
+    static_assert((1 << kJumpRounds) >= TN, "pointer jumping rounds too few");
+};
 
 // D8 offsets of common.hpp's direction order (N, NE, E, SE, S, SW, W, NW),
 // two bits per direction, biased by one
 constexpr unsigned kRowBits = 0x1A90u, kColBits = 0x01A9u;
 
+template <int TH>
 __device__ __forceinline__ int local_index(int g, int nc, int r0, int c0)
 {
     const int gr = g / nc, gc = g - gr * nc;
     return (gr >= r0 && gr < r0 + TH && gc >= c0 && gc < c0 + TW) ? (gr - r0) * TW + (gc - c0) : -1;
 }
 
+template <int TH>
 __device__ __forceinline__ int tile_of(int g, int nc, int ntx)
 {
     const int gr = g / nc;
@@ -106,11 +118,13 @@ __device__ __forceinline__ LaneCell lane_cell(int k, int r0, int c0, int nr, int
 // have them; a periodic wrap on a narrow grid can reach one cell by two
 // directions, which then counts once).
 // ---------------------------------------------------------------------------
+template <int TH>
 __global__ void __launch_bounds__(WL) k_tile_order(const int* __restrict__ rcv, const uint8_t* __restrict__ mask,
                                                    int nr, int nc, BC bc, short* __restrict__ ordg,
                                                    short* __restrict__ lstartg, int* __restrict__ tile_maxdep,
                                                    uint8_t* __restrict__ xdon)
 {
+    constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
     __shared__ int jmp[TN];
     __shared__ int dep[TN];
     __shared__ int hist[TN + 1];
@@ -133,7 +147,7 @@ __global__ void __launch_bounds__(WL) k_tile_order(const int* __restrict__ rcv, 
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int i = lane + WL * k;
-            const int l = (valid[k] && g[k] >= 0) ? local_index(g[k], nc, r0, c0) : -1;
+            const int l = (valid[k] && g[k] >= 0) ? local_index<TH>(g[k], nc, r0, c0) : -1;
             jmp[i] = l;
             dep[i] = l >= 0 ? 1 : 0;
             hist[i] = 0;
@@ -164,7 +178,7 @@ __global__ void __launch_bounds__(WL) k_tile_order(const int* __restrict__ rcv, 
 #pragma unroll
         for (int d = 0; d < 8; ++d) {
             int g = ok ? neighbour(r, c, d, nr, nc, bc) : -1;
-            if (g == f || (g >= 0 && local_index(g, nc, r0, c0) >= 0)) g = -1;
+            if (g == f || (g >= 0 && local_index<TH>(g, nc, r0, c0) >= 0)) g = -1;
             gs[d] = g;
         }
 #pragma unroll
@@ -180,7 +194,7 @@ __global__ void __launch_bounds__(WL) k_tile_order(const int* __restrict__ rcv, 
         }
         if (ok && mask[f]) xd[lr * TW + lc] = static_cast<uint8_t>(bits);
     }
-    for (int round = 0; round < kJumpRounds; ++round) {
+    for (int round = 0; round < TileShape<TH>::kJumpRounds; ++round) {
         int nj[PER], nd[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
@@ -261,9 +275,11 @@ struct PassArgs {
 };
 
 // level starts of levels 0..maxdep + 1 into LDS
+template <int TH>
 __device__ __forceinline__ int load_levels(const short* __restrict__ lstartg, const int* __restrict__ tile_maxdep,
                                            int tile, short* ls)
 {
+    constexpr int TN = TileShape<TH>::TN;
     const int maxdep = tile_maxdep[tile];
     for (int L = threadIdx.x; L <= maxdep + 1; L += WL) ls[L] = lstartg[(size_t)tile * (TN + 1) + L];
     return maxdep;
@@ -273,12 +289,14 @@ __device__ __forceinline__ int load_levels(const short* __restrict__ lstartg, co
 // counts: deep levels first; count = 1 + in-tile donors (LDS) + donors in
 // other tiles (HBM, their current values)
 // ---------------------------------------------------------------------------
+template <int TH>
 __global__ void __launch_bounds__(WL) k_tile_counts(const int* __restrict__ rcv, const uint8_t* __restrict__ mask,
                                                     const short* __restrict__ ordg, const short* __restrict__ lstartg,
                                                     const int* __restrict__ tile_maxdep,
                                                     const uint8_t* __restrict__ xdon, int nr, int nc, BC bc,
                                                     int* __restrict__ cnt, PassArgs P)
 {
+    constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
     if (P.prev_marked && *P.prev_marked == 0) return;
     const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x;
     const int tile = ty * ntx + tx;
@@ -307,10 +325,10 @@ __global__ void __launch_bounds__(WL) k_tile_counts(const int* __restrict__ rcv,
     for (int k = 0; k < PER; ++k) {
         const int i = lane + WL * k;
         acc[i] = valid[k] ? 1 : 0;
-        lrcv[i] = static_cast<short>((valid[k] && g[k] >= 0) ? local_index(g[k], nc, r0, c0) : -1);
+        lrcv[i] = static_cast<short>((valid[k] && g[k] >= 0) ? local_index<TH>(g[k], nc, r0, c0) : -1);
         ord[i] = ordg[(size_t)tile * TN + i];
     }
-    const int maxdep = load_levels(lstartg, tile_maxdep, tile, ls);
+    const int maxdep = load_levels<TH>(lstartg, tile_maxdep, tile, ls);
     __syncthreads();
     // inflow from other tiles into the ring cells: ring slots as in
     // k_tile_order (two per lane); every donor's count is loaded at once
@@ -368,7 +386,7 @@ __global__ void __launch_bounds__(WL) k_tile_counts(const int* __restrict__ rcv,
             const LaneCell q = lane_cell(k, r0, c0, nr, nc);
             __hip_atomic_store(&cnt[q.f], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (lrcv[i] < 0 && g[k] >= 0) {
-                P.act_next[tile_of(g[k], nc, ntx)] = 1;
+                P.act_next[tile_of<TH>(g[k], nc, ntx)] = 1;
                 marked = true;
             }
         }
@@ -442,7 +460,7 @@ __device__ __forceinline__ double sp_solve(const SpParams& a, double zo, double 
 constexpr short kNoRcv = -1, kOutRcv = -2;  // lrcv codes besides a local cell
 // packed local receiver + D8 code of the stream power tiles
 constexpr int kNoRcvP = 0xFFF, kOutRcvP = 0xFFE;
-static_assert(TN <= kOutRcvP, "local indices must fit below the packed codes");
+static_assert(TW * 32 <= kOutRcvP, "local indices must fit below the packed codes");
 __device__ __forceinline__ short pack_lrd(short l, uint8_t d)
 {
     const int lr = l == kNoRcv ? kNoRcvP : (l == kOutRcv ? kOutRcvP : static_cast<int>(l));
@@ -453,18 +471,20 @@ __device__ __forceinline__ short pack_lrd(short l, uint8_t d)
 // loads in flight)
 constexpr int kSpBatch = 4;
 
-template <bool MH, int NM>
+template <int TH, bool MH, int NM>
 __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC bc, PassArgs P)
 {
+    constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
+    static_assert(PER % kSpBatch == 0, "cells per lane must be whole batches");
     if (P.prev_marked && *P.prev_marked == 0) return;
     const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x;
     const int tile = ty * ntx + tx;
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
     if (!P.act[tile]) return;
-    // 10 KB of LDS per tile (four tiles per SIMD): the level starts are read
-    // from global memory one level ahead, the D8 code shares a word with
-    // the local receiver
+    // 20 bytes of LDS per cell (10 KB for a 32 x 16 tile): the level starts
+    // are read from global memory one level ahead, the D8 code shares a word
+    // with the local receiver
     __shared__ double val[TN];   // z_old until the cell's level, then its new z
     __shared__ double kam[TN];   // K A^m; < 0: the reference skips the cell
     __shared__ short lrd[TN];    // local receiver (low 12 bits: index, kNoRcvP, kOutRcvP) | D8 code << 12
@@ -504,7 +524,7 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
                 if (!(p.threshold > 0.0 && area < p.threshold)) {
                     // K_i * width_factor * pow(area, m) (lsdtt_stream_power_cpu.cpp:258-260)
                     km = Kf[j] * p.width * (MH ? sqrt(area) : pow(area, p.m));
-                    const int li = local_index(g[j], nc, r0, c0);
+                    const int li = local_index<TH>(g[j], nc, r0, c0);
                     l = li >= 0 ? static_cast<short>(li) : kOutRcv;
                 }
             }
@@ -557,7 +577,7 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
         if (__double_as_longlong(o) != __double_as_longlong(v)) {
             __hip_atomic_store(&A.z[q.f], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             for (unsigned b = x[k]; b; b &= b - 1) {
-                P.act_next[tile_of(neighbour(q.r, q.c, __builtin_ctz(b), nr, nc, bc), nc, ntx)] = 1;
+                P.act_next[tile_of<TH>(neighbour(q.r, q.c, __builtin_ctz(b), nr, nc, bc), nc, ntx)] = 1;
                 marked = true;
             }
         }
@@ -676,15 +696,45 @@ __global__ void __launch_bounds__(kSumThreads) k_sum_i32(const int* __restrict__
     if (threadIdx.x == 0) *out = s[0];
 }
 
+// calls f(std::integral_constant<int, TH>) for the runtime tile height
+template <class F>
+void with_tile_height(int th, F&& f)
+{
+    switch (th) {
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    default: throw std::invalid_argument("tile height must be 8, 16 or 32");
+    }
+}
+
 }  // namespace
+
+bool FluvialTiles::valid_tile_height(int th)
+{
+    for (int h : kTileHeights)
+        if (h == th) return true;
+    return false;
+}
+
+void FluvialTiles::set_tile_height(int th_)
+{
+    if (!valid_tile_height(th_)) throw std::invalid_argument("tile height must be 8, 16 or 32");
+    if (th_ == th) return;
+    th = th_;
+    // the pass heuristics were learnt on the other tiling
+    last_passes = last_count_passes = last_sp_passes = 0;
+    if (nr > 0) resize(nr, nc, bc_);
+}
 
 void FluvialTiles::resize(int nr_, int nc_, BC bc)
 {
     bc_ = bc;
     nr = nr_;
     nc = nc_;
+    const int TN = TW * th;
     ntx = div_up(nc, TW);
-    nty = div_up(nr, TH);
+    nty = div_up(nr, th);
     ntiles = ntx * nty;
     max_passes = ntiles + 2;
     for (auto& a : act) a.reserve(ntiles);
@@ -716,7 +766,10 @@ void FluvialTiles::smooth_area(const int* cnt, const uint8_t* mask, double pixel
 
 void FluvialTiles::depths(const int* rcv, const uint8_t* mask, hipStream_t s)
 {
-    k_tile_order<<<dim3(ntx, nty), WL, 0, s>>>(rcv, mask, nr, nc, bc_, ord.p, lstart.p, tile_maxdep.p, xdon.p);
+    with_tile_height(th, [&](auto H) {
+        k_tile_order<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(rcv, mask, nr, nc, bc_, ord.p, lstart.p, tile_maxdep.p,
+                                                         xdon.p);
+    });
     LSDTT_LAUNCH_CHECK();
 }
 
@@ -789,8 +842,10 @@ int FluvialTiles::counts(const int* rcv, const uint8_t* mask, int* cnt, bool war
     const int passes = settle(
         "upstream_counts", 9.0 * n,
         [&](const PassArgs& P) {
-            k_tile_counts<<<dim3(ntx, nty), WL, 0, s>>>(rcv, mask, ord.p, lstart.p, tile_maxdep.p, xdon.p, nr, nc,
-                                                        bc_, cnt, P);
+            with_tile_height(th, [&](auto H) {
+                k_tile_counts<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(rcv, mask, ord.p, lstart.p, tile_maxdep.p, xdon.p,
+                                                                 nr, nc, bc_, cnt, P);
+            });
             LSDTT_LAUNCH_CHECK();
         },
         [] {}, s);
@@ -813,15 +868,18 @@ int FluvialTiles::stream_power(const int* rcv, const uint8_t* dir, const uint8_t
         [&](const PassArgs& P) {
             const dim3 g(ntx, nty);
             const int nm = sp.n_is_one ? 0 : (sp.n1_half ? 1 : 2);
-            if (sp.m_half) {
-                if (nm == 0) k_tile_sp<true, 0><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
-                else if (nm == 1) k_tile_sp<true, 1><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
-                else k_tile_sp<true, 2><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
-            } else {
-                if (nm == 0) k_tile_sp<false, 0><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
-                else if (nm == 1) k_tile_sp<false, 1><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
-                else k_tile_sp<false, 2><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
-            }
+            with_tile_height(th, [&](auto H) {
+                constexpr int h = decltype(H)::value;
+                if (sp.m_half) {
+                    if (nm == 0) k_tile_sp<h, true, 0><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+                    else if (nm == 1) k_tile_sp<h, true, 1><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+                    else k_tile_sp<h, true, 2><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+                } else {
+                    if (nm == 0) k_tile_sp<h, false, 0><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+                    else if (nm == 1) k_tile_sp<h, false, 1><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+                    else k_tile_sp<h, false, 2><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+                }
+            });
             LSDTT_LAUNCH_CHECK();
         },
         [&] {

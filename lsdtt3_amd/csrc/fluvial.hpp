@@ -4,7 +4,8 @@
 // (count(v) = 1 + sum of its donors' counts), stream power from the outlets
 // up (z(v) is a function of its receiver's new elevation,
 // src/cpu/lsdtt_stream_power_cpu.cpp:218-319).  Instead of global depth
-// levels (random gathers), the grid is cut into 32 x 32 tiles held in LDS:
+// levels (random gathers), the grid is cut into tiles of 32 columns by 8,
+// 16 (default) or 32 rows held in LDS:
 // inside a tile the cells are ordered by their hop count to the tile's exit
 // (pointer jumping in LDS, once per step) and processed level by level by
 // the tile's one wavefront; values that cross a tile edge are read from HBM.
@@ -32,9 +33,16 @@ struct SpParams {
 
 class FluvialTiles {
 public:
-    static constexpr int TW = 32, TH = 16, TN = TW * TH;
+    static constexpr int TW = 32;
+    static constexpr int kTileHeights[3] = {8, 16, 32};
+    static bool valid_tile_height(int th);
 
     void resize(int nr, int nc, BC bc);
+    // tile rows (one of kTileHeights); re-sizes the tile buffers.  Results
+    // do not depend on it, only the speed does.  The next sweep must be
+    // preceded by depths().
+    void set_tile_height(int th);
+    int tile_height() const { return th; }
     // each tile's cells in level order, once per set of receivers
     void depths(const int* rcv, const uint8_t* mask, hipStream_t s);
     // counts (int32) of valid cells from receivers; warm: cnt holds counts
@@ -64,6 +72,7 @@ public:
     dbuf<int> hcnt;           // smoothing: row counts
 
 private:
+    int th = 16;
     // passes of the last call of each sweep: the next call's first batch
     int last_passes = 0, last_count_passes = 0, last_sp_passes = 0;
     // runs `pass(PassArgs)` until a pass marks no tile

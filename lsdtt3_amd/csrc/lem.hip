@@ -743,4 +743,63 @@ int LemEngine::step(LemStats* st, hipStream_t s)
     return rc;
 }
 
+int LemEngine::autotune_tiles(int steps, double* ms_per_step, hipStream_t s)
+{
+    if (steps < 1) throw std::invalid_argument("autotune needs at least one step per tile height");
+    dbuf<double> saved;
+    saved.reserve(static_cast<size_t>(n));
+    LSDTT_HIP(hipMemcpyAsync(saved.p, z.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    const bool cap = capture;
+    capture = false;  // the trial steps are not training samples
+    hipEvent_t e0, e1;
+    LSDTT_HIP(hipEventCreate(&e0));
+    LSDTT_HIP(hipEventCreate(&e1));
+    // the current height first; another replaces it only when it is faster
+    // by more than kMargin (trials of a few steps carry a few per cent of
+    // noise)
+    constexpr double kMargin = 0.03;
+    const int cur = tiles.tile_height();
+    int order[3] = {cur, 0, 0};
+    for (int c = 0, k = 1; c < 3; ++c)
+        if (FluvialTiles::kTileHeights[c] != cur) order[k++] = FluvialTiles::kTileHeights[c];
+    int best = cur;
+    double best_ms = 0.0, cur_ms = 0.0;
+    LemStats st;
+    try {
+        for (int c = 0; c < 3; ++c) {
+            const int th = order[c];
+            tiles.set_tile_height(th);
+            // one untimed step settles the pass and Krylov heuristics
+            step(&st, s);
+            LSDTT_HIP(hipMemcpyAsync(z.p, saved.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            LSDTT_HIP(hipEventRecord(e0, s));
+            for (int k = 0; k < steps; ++k) step(&st, s);
+            LSDTT_HIP(hipEventRecord(e1, s));
+            LSDTT_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            LSDTT_HIP(hipEventElapsedTime(&ms, e0, e1));
+            const double per = static_cast<double>(ms) / steps;
+            if (ms_per_step) ms_per_step[th == 8 ? 0 : (th == 16 ? 1 : 2)] = per;
+            if (c == 0) {
+                cur_ms = best_ms = per;
+            } else if (per < best_ms && per < (1.0 - kMargin) * cur_ms) {
+                best_ms = per;
+                best = th;
+            }
+            LSDTT_HIP(hipMemcpyAsync(z.p, saved.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        }
+    } catch (...) {
+        capture = cap;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    capture = cap;
+    tiles.set_tile_height(best);
+    LSDTT_HIP(hipStreamSynchronize(s));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
 }  // namespace lsdtt_gpu

@@ -88,6 +88,14 @@ public:
     // to fall back to Picard)
     int step(LemStats* st, hipStream_t s);
 
+    // Times `steps` steps (after one untimed step) with each tile height of
+    // FluvialTiles::kTileHeights from the current surface, which is
+    // restored after each trial, and keeps the fastest (the current height
+    // unless another is more than 3% faster); returns it.
+    // ms_per_step (optional, 3 entries) receives each height's time.  The
+    // results of a step do not depend on the tile height.
+    int autotune_tiles(int steps, double* ms_per_step, hipStream_t s);
+
     int nr, nc;
     long long n;
     double dx;

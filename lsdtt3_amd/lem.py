@@ -155,6 +155,27 @@ class LandscapeModel:
             self.step()
         return self.last_stats
 
+    # ---- tile shape of the routing sweeps (speed only) ----------------------
+    @property
+    def tile_height(self) -> int:
+        """rows of the 32-column tiles of the count and stream-power sweeps"""
+        return int(N.lib().lsdtt_gpu_lem_tile_height(self._h))
+
+    @tile_height.setter
+    def tile_height(self, th: int) -> None:
+        N.check(N.lib().lsdtt_gpu_lem_set_tile_height(self._h, int(th)))
+
+    def autotune_tiles(self, steps: int = 2):
+        """times `steps` steps with each tile height (8, 16, 32) from the
+        current surface (restored after each trial) and keeps the fastest;
+        returns (height, {height: ms per step})"""
+        th = C.c_int32(0)
+        ms = (C.c_double * 3)()
+        fp = self.fixed_pixels
+        N.check(N.lib().lsdtt_gpu_lem_autotune_tiles(self._h, N.ptr(fp) if fp.size else None, fp.size,
+                                                     int(steps), C.byref(th), ms))
+        return th.value, {h: ms[i] for i, h in enumerate((8, 16, 32))}
+
     # ---- per-kernel event timing (roofline measurement) -------------------
     def profile(self, enable: bool = True) -> None:
         """start (clearing totals) or stop HIP-event timing of the kernels"""
@@ -221,6 +242,18 @@ class Ensemble:
         sec = C.c_double(0.0)
         N.check(N.lib().lsdtt_gpu_ensemble_run(self._h, int(n_steps), st, C.byref(sec)))
         return [s.as_dict() for s in st], sec.value
+
+    def set_tile_height(self, i, th):
+        N.check(N.lib().lsdtt_gpu_ensemble_set_tile_height(self._h, int(i), int(th)))
+
+    def autotune_tiles(self, steps=2):
+        """chooses every instance's tile height by measurement (per size
+        class alone, then the whole ensemble with each uniform height and
+        with the per-class heights); returns (heights, seconds taken)"""
+        th = (C.c_int32 * self.n)()
+        sec = C.c_double(0.0)
+        N.check(N.lib().lsdtt_gpu_ensemble_autotune_tiles(self._h, int(steps), th, C.byref(sec)))
+        return list(th), sec.value
 
     def elevation(self, i):
         z = np.empty(self.shapes[i], np.float64)
