@@ -197,10 +197,11 @@ typedef struct {
     double channel_width_w;
     /* hillslope_coupling_regime: 0 automatic, 1 coupled, 2
      * channels_as_boundaries (lsdtt_lem_cpu.cpp:64-77).  The Dirichlet
-     * channel set is the mask given to lsdtt_gpu_lem_set_channel_mask; a
-     * set derived from incision_threshold_area needs the D-infinity flow
-     * graph of the reference (lsdtt_lem_cpu.cpp:137-154), which this
-     * library does not build: such a step returns LSDTT_GPU_ERR_ARG. */
+     * channel set is the mask given to lsdtt_gpu_lem_set_channel_mask, or
+     * else, each step, the cells whose D-infinity drainage area of the
+     * post-incision surface (rounded to float, interior nodata as outlets,
+     * lsdtt_lem_cpu.cpp:137-154) passes incision_threshold_area
+     * (lsdtt_gpu_lem_get_channel_mask returns it). */
     int32_t hillslope_coupling_regime;
     /* hillslope_force_picard: skip Newton; otherwise Picard runs only when
      * Newton fails.  roering_implicit_opts of the damped Picard fallback
@@ -282,6 +283,12 @@ double* lsdtt_gpu_lem_elevation_device(lsdtt_gpu_lem* lem);
 int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary,
                        lsdtt_gpu_lem_stats* stats);
 
+/* The hillslope's Dirichlet channel set of the last step (uint8[n]): the
+ * explicit mask, or the cells whose D-infinity drainage area passed
+ * incision_threshold_area (build_channel_mask, lsdtt_lem_cpu.cpp:86-134,
+ * on the post-incision surface), or none in the coupled regime. */
+int lsdtt_gpu_lem_get_channel_mask(lsdtt_gpu_lem* lem, uint8_t* channel);
+
 /* Tile shape of the upstream-count and stream-power sweeps: tiles of 32
  * columns by tile_height rows (8, 16 or 32; default 16).  No counterpart in
  * the reference, whose serial sweeps have no tiles
@@ -295,6 +302,24 @@ int lsdtt_gpu_lem_set_tile_height(lsdtt_gpu_lem* lem, int32_t tile_height);
 int32_t lsdtt_gpu_lem_tile_height(lsdtt_gpu_lem* lem);
 int lsdtt_gpu_lem_autotune_tiles(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary,
                                  int32_t steps, int32_t* tile_height, double* ms_per_step);
+
+/* ---------------------------------------------------------------------
+ * D-infinity flow graph (lsdtt_flowgraph_cpu with method d_infinity,
+ * src/cpu/lsdtt_flowgraph_cpu.cpp:87-177, 423-663) and its accumulation
+ * (accumulate() with cell_size^2 per valid cell, :988-1033) from host
+ * arrays.  Valid cells: mask != 0, z finite and (when nodata is finite)
+ * z != nodata.  Per cell f: receivers int64[2f], [2f+1] (-1 = none; the
+ * reference's CSR rows in slot order) and weights float[2f], [2f+1];
+ * slope and bearing float rasters (nodata where there is no receiver);
+ * area double raster (0 off the valid cells).  Any output may be NULL.
+ * atan2 is the correctly rounded value; the reference's std::atan2(float,
+ * float) returns it or one ulp beside it, depending on the C library.  The
+ * accumulation adds a cell's donors in direction order (N, NE, ..., NW)
+ * where the reference follows its Kahn order: the areas agree to rounding.
+ * ------------------------------------------------------------------- */
+int lsdtt_gpu_dinf_graph(const float* z, const uint8_t* mask, int32_t n_rows, int32_t n_cols, double cell_size,
+                         const char* boundary_conditions, int32_t interior_nodata_is_outlet, double nodata,
+                         int64_t* receivers, float* weights, float* slope, float* bearing, double* area);
 
 /* ---------------------------------------------------------------------
  * Batched ensemble: n independent LEM instances (uneven grids, per-instance

@@ -69,6 +69,8 @@ def declare(L, sig):
     sig(L, "lsdtt_gpu_ensemble_autotune_tiles", C.c_int, [vp, i32, vp, vp])
     sig(L, "lsdtt_gpu_lem_set_tile_height", C.c_int, [vp, i32])
     sig(L, "lsdtt_gpu_lem_tile_height", i32, [vp])
+    sig(L, "lsdtt_gpu_lem_get_channel_mask", C.c_int, [vp, vp])
+    sig(L, "lsdtt_gpu_dinf_graph", C.c_int, [vp, vp, i32, i32, dbl, C.c_char_p, i32, dbl, vp, vp, vp, vp, vp])
     sig(L, "lsdtt_gpu_lem_autotune_tiles", C.c_int, [vp, vp, i64, i32, vp, vp])
     sig(L, "lsdtt_gpu_lem_elevation_device", C.c_void_p, [vp])
     sig(L, "lsdtt_gpu_lem_step", C.c_int, [vp, vp, i64, C.POINTER(LemStats)])

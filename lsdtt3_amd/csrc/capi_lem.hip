@@ -275,6 +275,26 @@ int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_
     });
 }
 
+int lsdtt_gpu_lem_get_channel_mask(lsdtt_gpu_lem* lem, uint8_t* channel)
+{
+    return guarded([&] {
+        if (!lem || !channel) return fail_arg("null argument");
+        LemEngine& e = *lem->eng;
+        const size_t n = static_cast<size_t>(e.n);
+        LSDTT_HIP(hipDeviceSynchronize());
+        if (!e.channels_active()) {
+            std::memset(channel, 0, n);
+        } else if (!e.channel_h.empty()) {
+            std::memcpy(channel, e.channel_h.data(), n);
+        } else if (e.chan.p && e.darea_warm) {
+            LSDTT_HIP(hipMemcpy(channel, e.chan.p, n, hipMemcpyDeviceToHost));
+        } else {
+            std::memset(channel, 0, n);
+        }
+        return LSDTT_GPU_OK;
+    });
+}
+
 int lsdtt_gpu_lem_set_tile_height(lsdtt_gpu_lem* lem, int32_t tile_height)
 {
     return guarded([&] {

@@ -1,0 +1,199 @@
+// dinf.hip -- D-infinity receivers and weights (see dinf.hpp).
+//
+// Reference semantics restated (src/cpu/lsdtt_flowgraph_cpu.cpp):
+//   facets (Tarboton 1997, Table 1) :62-67: facet a pairs a cardinal
+//       neighbour e1 and a diagonal neighbour e2
+//   validity :120-139: mask != 0, z finite and not the nodata value
+//   resolve_neighbor :364-396: periodic edges wrap, other edges end the grid
+//   build_dinf :423-663, per valid cell that is not on a 'b' edge and (with
+//       interior_nodata_is_outlet) touches no invalid in-grid cell: over the
+//       facets whose two cells exist, differ from the cell and are valid,
+//       with e0 above one of them: s1 = (e0 - e1)/dx, s2 = (e1 - e2)/dx in
+//       float; both lower: r = atan2(s2, s1), clamped to [0, pi/4] with
+//       s = s1 or (e0 - e2)/(dx sqrt2) at the clamps, else
+//       s = sqrt(s1^2 + s2^2); only e1 lower: r = 0, s = s1; only e2 lower:
+//       r = pi/4, s = (e0 - e2)/(dx sqrt2).  The first facet of largest
+//       s > 0 wins; r == 0 sends all flow to e1, r >= pi/4 to e2, otherwise
+//       e2 gets r/(pi/4) and e1 the rest (one receiver if they coincide).
+//
+// Every operation is the reference's float operation (no contraction, IEEE
+// division and square root).  atan2 is evaluated in double and rounded to
+// float: the correctly rounded value, which the reference's atan2f returns
+// or misses by one ulp depending on the C library.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "dinf.hpp"
+
+namespace lsdtt_gpu {
+
+namespace {
+
+__constant__ const int kE1R[8] = {0, -1, -1, 0, 0, 1, 1, 0};
+__constant__ const int kE1C[8] = {1, 0, 0, -1, -1, 0, 0, 1};
+__constant__ const int kE2R[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+__constant__ const int kE2C[8] = {1, 1, -1, -1, -1, -1, 1, 1};
+__constant__ const int kAC[8] = {0, 1, 1, 2, 2, 3, 3, 4};
+__constant__ const int kAF[8] = {1, -1, 1, -1, 1, -1, 1, -1};
+
+// resolve_neighbor: periodic wrap, else none (-1)
+__device__ __forceinline__ int wrap_nb(int r, int c, int dr, int dc, int nr, int nc, BC bc)
+{
+    int tr = r + dr, tc = c + dc;
+    if (tr < 0) {
+        if (bc.n != 'p') return -1;
+        tr = nr - 1;
+    } else if (tr >= nr) {
+        if (bc.s != 'p') return -1;
+        tr = 0;
+    }
+    if (tc < 0) {
+        if (bc.w != 'p') return -1;
+        tc = nc - 1;
+    } else if (tc >= nc) {
+        if (bc.e != 'p') return -1;
+        tc = 0;
+    }
+    return tr * nc + tc;
+}
+
+template <class T>
+__device__ __forceinline__ bool cell_valid(const T* z, const uint8_t* mask, int f, bool use_nd, float nd, float* zf)
+{
+    const float v = static_cast<float>(z[f]);
+    *zf = v;
+    return mask[f] && isfinite(v) && !(use_nd && v == nd);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_dinf_receivers(const T* __restrict__ z, const uint8_t* __restrict__ mask,
+                                                        int nr, int nc, float cs, uint32_t bcw, int nodata_outlet,
+                                                        int use_nd, float nd, int* __restrict__ rec,
+                                                        float* __restrict__ w, float* __restrict__ slope,
+                                                        float* __restrict__ bearing)
+{
+    const BC bc = bc_unpack(bcw);
+    const long long n = (long long)nr * nc;
+    for (long long ff = blockIdx.x * (long long)blockDim.x + threadIdx.x; ff < n;
+         ff += (long long)gridDim.x * blockDim.x) {
+        const int f = static_cast<int>(ff);
+        const int r = f / nc, c = f - r * nc;
+        int r0 = -1, r1 = -1;
+        float w0 = 0.0f, w1 = 0.0f, best_s = 0.0f, best_r = 0.0f;
+        int best = -1;
+        float e0;
+        bool ok = cell_valid(z, mask, f, use_nd, nd, &e0);
+        ok = ok && !((bc.n == 'b' && r == 0) || (bc.s == 'b' && r == nr - 1) || (bc.e == 'b' && c == nc - 1) ||
+                     (bc.w == 'b' && c == 0));
+        if (ok && nodata_outlet) {
+            for (int dr = -1; dr <= 1 && ok; ++dr)
+                for (int dc = -1; dc <= 1; ++dc) {
+                    if (!dr && !dc) continue;
+                    const int rr = r + dr, cc = c + dc;
+                    if (rr < 0 || rr >= nr || cc < 0 || cc >= nc) continue;
+                    float t;
+                    if (!cell_valid(z, mask, rr * nc + cc, use_nd, nd, &t)) {
+                        ok = false;
+                        break;
+                    }
+                }
+        }
+        if (ok) {
+            const float pi4 = static_cast<float>(M_PI / 4.0);
+            const float sqrt2f = static_cast<float>(1.4142135623730950488);
+            for (int a = 0; a < 8; ++a) {
+                const int g1 = wrap_nb(r, c, kE1R[a], kE1C[a], nr, nc, bc);
+                const int g2 = wrap_nb(r, c, kE2R[a], kE2C[a], nr, nc, bc);
+                if (g1 < 0 || g2 < 0 || g1 == f || g2 == f) continue;
+                float e1, e2;
+                if (!cell_valid(z, mask, g1, use_nd, nd, &e1) || !cell_valid(z, mask, g2, use_nd, nd, &e2)) continue;
+                if (e0 <= e1 && e0 <= e2) continue;
+                float rr, sl;
+                const float s1 = (e0 - e1) / cs;
+                const float s2 = (e1 - e2) / cs;
+                if (e0 > e1 && e0 > e2) {
+                    rr = static_cast<float>(atan2(static_cast<double>(s2), static_cast<double>(s1)));
+                    if (rr < 0.0f) {
+                        rr = 0.0f;
+                        sl = s1;
+                    } else if (rr > pi4) {
+                        rr = pi4;
+                        sl = (e0 - e2) / (cs * sqrt2f);
+                    } else {
+                        sl = sqrtf(s1 * s1 + s2 * s2);
+                    }
+                } else if (e0 > e1) {
+                    rr = 0.0f;
+                    sl = s1;
+                } else {
+                    rr = pi4;
+                    sl = (e0 - e2) / (cs * sqrt2f);
+                }
+                if (sl > best_s) {
+                    best_s = sl;
+                    best_r = rr;
+                    best = a;
+                    r0 = g1;
+                    r1 = g2;
+                }
+            }
+            if (best >= 0) {
+                if (best_r == 0.0f) {
+                    w0 = 1.0f;
+                    r1 = -1;
+                } else if (best_r >= pi4) {
+                    r0 = r1;
+                    w0 = 1.0f;
+                    r1 = -1;
+                } else if (r0 == r1) {
+                    w0 = 1.0f;
+                    r1 = -1;
+                } else {
+                    w1 = best_r / pi4;
+                    w0 = 1.0f - w1;
+                }
+            } else {
+                r0 = r1 = -1;
+            }
+        }
+        rec[2 * (size_t)f] = r0;
+        rec[2 * (size_t)f + 1] = r1;
+        w[2 * (size_t)f] = w0;
+        w[2 * (size_t)f + 1] = w1;
+        if (slope) slope[f] = best >= 0 ? best_s : nd;
+        if (bearing) {
+            float br = nd;
+            if (best >= 0) {
+                const float rad2deg = static_cast<float>(180.0 / M_PI);
+                const float ang =
+                    static_cast<float>(kAF[best]) * best_r + static_cast<float>(kAC[best]) * (static_cast<float>(M_PI) * 0.5f);
+                br = 360.0f - (ang * rad2deg) + 90.0f;
+                while (br >= 360.0f) br -= 360.0f;
+                while (br < 0.0f) br += 360.0f;
+            }
+            bearing[f] = br;
+        }
+    }
+}
+
+}  // namespace
+
+template <class T>
+void dinf_receivers(const T* z, const uint8_t* mask, int nr, int nc, const DinfOptions& o, int* rec, float* w,
+                    float* slope, float* bearing, hipStream_t s)
+{
+    const long long n = (long long)nr * nc;
+    const int grid = std::max(1, std::min(8192, div_up(n, 256)));
+    k_dinf_receivers<T><<<grid, 256, 0, s>>>(z, mask, nr, nc, static_cast<float>(o.dx), bc_pack(o.bc),
+                                             o.interior_nodata_is_outlet ? 1 : 0, o.use_nodata ? 1 : 0, o.nodata,
+                                             rec, w, slope, bearing);
+    LSDTT_LAUNCH_CHECK();
+}
+
+template void dinf_receivers<float>(const float*, const uint8_t*, int, int, const DinfOptions&, int*, float*, float*,
+                                    float*, hipStream_t);
+template void dinf_receivers<double>(const double*, const uint8_t*, int, int, const DinfOptions&, int*, float*,
+                                     float*, float*, hipStream_t);
+
+}  // namespace lsdtt_gpu

@@ -591,6 +591,238 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
     }
 }
 
+// ---------------------------------------------------------------------------
+// D-infinity accumulation (lsdtt_flowgraph_cpu.cpp:988-1033) by the same
+// tile passes.  A cell's value is its own area plus, in D8 direction order
+// (N, NE, ..., NW; a neighbour reached twice by a periodic wrap counts
+// once), each donor's value times the donor's weight toward it; a donor is
+// a neighbour one of whose (at most two) receivers is the cell.  The graph
+// is acyclic, so the values are the unique solution whatever the pass
+// schedule or tiling.  (The reference adds the donors in its Kahn order
+// instead; the two sums differ only by rounding.)
+// ---------------------------------------------------------------------------
+
+// once per graph: each tile's cells sorted by in-tile depth, depth(v) =
+// 1 + the largest depth of v's in-tile donors (0 without), found by
+// relaxation to the least fixed point (LDS atomic max, until a sweep changes
+// nothing); stored like k_tile_order's order
+template <int TH>
+__global__ void __launch_bounds__(WL) k_dinf_tile_order(const int* __restrict__ rec, const uint8_t* __restrict__ mask,
+                                                        int nr, int nc, short* __restrict__ ordg,
+                                                        short* __restrict__ lstartg, int* __restrict__ tile_maxdep)
+{
+    constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
+    __shared__ int dep[TN];
+    __shared__ int hist[TN + 1];
+    __shared__ int cur[TN];
+    __shared__ short lrec[2][TN];
+    const int tx = blockIdx.x, ty = blockIdx.y, lane = threadIdx.x;
+    const int tile = ty * gridDim.x + tx;
+    const int r0 = ty * TH, c0 = tx * TW;
+    bool valid[PER];
+    {
+        uint8_t m[PER];
+        int g0[PER], g1[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+            m[k] = mask[q.f];
+            g0[k] = rec[2 * (size_t)q.f];
+            g1[k] = rec[2 * (size_t)q.f + 1];
+            valid[k] = q.in && m[k];
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = lane + WL * k;
+            lrec[0][i] = static_cast<short>((valid[k] && g0[k] >= 0) ? local_index<TH>(g0[k], nc, r0, c0) : -1);
+            lrec[1][i] = static_cast<short>((valid[k] && g1[k] >= 0) ? local_index<TH>(g1[k], nc, r0, c0) : -1);
+            dep[i] = 0;
+            hist[i] = 0;
+        }
+    }
+    if (lane == 0) hist[TN] = 0;
+    __syncthreads();
+    // one wavefront per tile: the ballot is the whole tile's.  An acyclic
+    // graph settles within TN sweeps (depths < TN); the cap and the clamp
+    // below keep a malformed graph in bounds.
+    for (int sweep = 0; sweep < TN; ++sweep) {
+        bool ch = false;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (!valid[k]) continue;
+            const int i = lane + WL * k;
+            const int d1 = dep[i] + 1;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int l = lrec[j][i];
+                if (l >= 0 && atomicMax(&dep[l], d1) < d1) ch = true;
+            }
+        }
+        const bool any = __ballot(ch) != 0;
+        __syncthreads();
+        if (!any) break;
+    }
+    int mx = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+        if (valid[k]) {
+            const int i = lane + WL * k;
+            const int d = min(dep[i], TN - 1);
+            dep[i] = d;
+            atomicAdd(&hist[d], 1);
+            mx = max(mx, d);
+        }
+    mx = wave_max(mx);
+    __syncthreads();
+    int loc[PER], sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        loc[k] = hist[lane * PER + k];
+        sum += loc[k];
+    }
+    int total;
+    int base = wave_exclusive_scan(sum, &total);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        hist[lane * PER + k] = base;
+        lstartg[(size_t)tile * (TN + 1) + lane * PER + k] = static_cast<short>(base);
+        base += loc[k];
+        cur[lane * PER + k] = 0;
+    }
+    if (lane == 0) {
+        hist[TN] = total;
+        lstartg[(size_t)tile * (TN + 1) + TN] = static_cast<short>(total);
+        tile_maxdep[tile] = mx;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+        if (valid[k]) {
+            const int i = lane + WL * k;
+            const int d = dep[i];
+            ordg[(size_t)tile * TN + hist[d] + atomicAdd(&cur[d], 1)] = static_cast<short>(i);
+        }
+}
+
+struct DinfTileArgs {
+    const int* rec;      // 2 per cell, -1 none
+    const float* w;      // 2 per cell
+    const uint8_t* mask;
+    const short* ord;
+    const short* lstart;
+    const int* tile_maxdep;
+    double cell_area;    // each valid cell's own contribution (dx^2)
+    double* acc;
+};
+
+template <int TH>
+__global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, int nc, BC bc, PassArgs P)
+{
+    constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
+    if (P.prev_marked && *P.prev_marked == 0) return;
+    const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x;
+    const int tile = ty * ntx + tx;
+    const int lane = threadIdx.x;
+    if (lane == 0) P.act_clear[tile] = 0;
+    if (!P.act[tile]) return;
+    __shared__ double val[TN];
+    __shared__ float wl[2][TN];
+    __shared__ short lr[2][TN];  // local receivers (-1: none or in another tile)
+    __shared__ short ord[TN];
+    __shared__ short ls[TN + 1];
+    const int r0 = ty * TH, c0 = tx * TW;
+    int g[PER][2];
+    double old[PER];
+    bool valid[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+        const uint8_t m = A.mask[q.f];
+        g[k][0] = A.rec[2 * (size_t)q.f];
+        g[k][1] = A.rec[2 * (size_t)q.f + 1];
+        const float w0 = A.w[2 * (size_t)q.f], w1 = A.w[2 * (size_t)q.f + 1];
+        old[k] = A.acc[q.f];  // only this tile writes its cells
+        valid[k] = q.in && m;
+        const int i = lane + WL * k;
+        lr[0][i] = static_cast<short>((valid[k] && g[k][0] >= 0) ? local_index<TH>(g[k][0], nc, r0, c0) : -1);
+        lr[1][i] = static_cast<short>((valid[k] && g[k][1] >= 0) ? local_index<TH>(g[k][1], nc, r0, c0) : -1);
+        wl[0][i] = w0;
+        wl[1][i] = w1;
+        val[i] = 0.0;
+        ord[i] = A.ord[(size_t)tile * TN + i];
+    }
+    const int maxdep = load_levels<TH>(A.lstart, A.tile_maxdep, tile, ls);
+    __syncthreads();
+    const int h = min(TH, nr - r0), wd = min(TW, nc - c0);
+    for (int L = 0; L <= maxdep; ++L) {
+        const int b = ls[L], e = ls[L + 1];
+        for (int p = b + lane; p < e; p += WL) {
+            const int i = ord[p];
+            const int lrow = i / TW, lcol = i - lrow * TW;
+            double a = A.cell_area;
+            if (lrow >= 1 && lrow <= h - 2 && lcol >= 1 && lcol <= wd - 2) {
+                // all eight neighbours are distinct cells of this tile
+#pragma unroll
+                for (int d = 0; d < 8; ++d) {
+                    const int u = i + d8_row[d] * TW + d8_col[d];
+                    if (lr[0][u] == i) a += val[u] * static_cast<double>(wl[0][u]);
+                    else if (lr[1][u] == i) a += val[u] * static_cast<double>(wl[1][u]);
+                }
+            } else {
+                const int r = r0 + lrow, c = c0 + lcol, v = r * nc + c;
+                int seen[8];
+                int ns = 0;
+                for (int d = 0; d < 8; ++d) {
+                    const Step st = resolve_step(r, c, d, nr, nc, bc);
+                    if (st.kind != 1) continue;
+                    const int u = st.r * nc + st.c;
+                    bool dup = u == v;
+                    for (int j = 0; j < ns; ++j) dup |= seen[j] == u;
+                    if (dup) continue;
+                    seen[ns++] = u;
+                    const int lu = local_index<TH>(u, nc, r0, c0);
+                    if (lu >= 0) {
+                        if (lr[0][lu] == i) a += val[lu] * static_cast<double>(wl[0][lu]);
+                        else if (lr[1][lu] == i) a += val[lu] * static_cast<double>(wl[1][lu]);
+                    } else {
+                        const int h0 = A.rec[2 * (size_t)u], h1 = A.rec[2 * (size_t)u + 1];
+                        if (h0 == v || h1 == v) {
+                            const double au = __hip_atomic_load(&A.acc[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            a += au * static_cast<double>(A.w[2 * (size_t)u + (h0 == v ? 0 : 1)]);
+                        }
+                    }
+                }
+            }
+            val[i] = a;
+        }
+        __syncthreads();
+    }
+    // a changed value whose receiver lies in another tile re-runs that tile
+    bool marked = false;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (!valid[k]) continue;
+        const int i = lane + WL * k;
+        const double v = val[i];
+        if (__double_as_longlong(v) != __double_as_longlong(old[k])) {
+            const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+            __hip_atomic_store(&A.acc[q.f], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                if (g[k][j] >= 0 && lr[j][i] < 0) {
+                    P.act_next[tile_of<TH>(g[k][j], nc, ntx)] = 1;
+                    marked = true;
+                }
+        }
+    }
+    const bool wmarked = __ballot(marked) != 0;
+    if (lane == 0) {
+        if (wmarked) *P.marked = 1;
+        P.runs[tile] += 1;
+    }
+}
+
 // smooth_drainage_area (lsdtt_stream_power_cpu.cpp:34-139), row pass: one
 // thread per row keeps the reference's rolling sum (window -r..r, then per
 // column: record, subtract column c - r, add column c + r + 1)
@@ -723,7 +955,7 @@ void FluvialTiles::set_tile_height(int th_)
     if (th_ == th) return;
     th = th_;
     // the pass heuristics were learnt on the other tiling
-    last_passes = last_count_passes = last_sp_passes = 0;
+    last_passes = last_count_passes = last_sp_passes = last_dinf_passes = 0;
     if (nr > 0) resize(nr, nc, bc_);
 }
 
@@ -889,6 +1121,39 @@ int FluvialTiles::stream_power(const int* rcv, const uint8_t* dir, const uint8_t
         s);
     last_sp_passes = passes;
     std::memcpy(eroded, flags_h.data(), sizeof(long long));
+    return passes;
+}
+
+void FluvialTiles::dinf_order(const int* rec, const uint8_t* mask, hipStream_t s)
+{
+    with_tile_height(th, [&](auto H) {
+        k_dinf_tile_order<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(rec, mask, nr, nc, ord.p, lstart.p,
+                                                                            tile_maxdep.p);
+    });
+    LSDTT_LAUNCH_CHECK();
+}
+
+int FluvialTiles::dinf_accumulate(const int* rec, const float* w, const uint8_t* mask, double cell_area,
+                                  double* acc, bool warm, hipStream_t s)
+{
+    const long long n = (long long)nr * nc;
+    if (!warm) {
+        LSDTT_HIP(hipMemsetAsync(acc, 0, sizeof(double) * n, s));
+    }
+    DinfTileArgs A{rec, w, mask, ord.p, lstart.p, tile_maxdep.p, cell_area, acc};
+    last_passes = last_dinf_passes;
+    // algorithmic bytes of the sweep, counted once: rec 8, w 8, mask 1 in;
+    // acc 8 out
+    const int passes = settle(
+        "dinf_accumulate", 25.0 * n,
+        [&](const PassArgs& P) {
+            with_tile_height(th, [&](auto H) {
+                k_dinf_tile_accum<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(A, nr, nc, bc_, P);
+            });
+            LSDTT_LAUNCH_CHECK();
+        },
+        [] {}, s);
+    last_dinf_passes = passes;
     return passes;
 }
 

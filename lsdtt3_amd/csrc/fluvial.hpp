@@ -54,6 +54,15 @@ public:
     int stream_power(const int* rcv, const uint8_t* dir, const uint8_t* mask, const int* cnt, const double* area,
                      const double* z_old, const double* K, const SpParams& sp, double* z, long long* eroded,
                      hipStream_t s);
+    // D-infinity accumulation (rec / w: two receivers and weights per cell,
+    // -1 = none): dinf_order once per graph, then dinf_accumulate (acc =
+    // cell_area per valid cell plus its donors' weighted values; warm: acc
+    // holds any earlier values of the same mask, else it is cleared);
+    // returns passes used.  The order shares the buffers of depths(), which
+    // the D8 sweeps need again afterwards.
+    void dinf_order(const int* rec, const uint8_t* mask, hipStream_t s);
+    int dinf_accumulate(const int* rec, const float* w, const uint8_t* mask, double cell_area, double* acc,
+                        bool warm, hipStream_t s);
     // smooth_drainage_area of count * pixel_area: rolling row sums, then
     // rolling column sums, in the reference's order (one thread per row,
     // then per column), so the averages are bit-identical
@@ -74,7 +83,7 @@ public:
 private:
     int th = 16;
     // passes of the last call of each sweep: the next call's first batch
-    int last_passes = 0, last_count_passes = 0, last_sp_passes = 0;
+    int last_passes = 0, last_count_passes = 0, last_sp_passes = 0, last_dinf_passes = 0;
     // runs `pass(PassArgs)` until a pass marks no tile
     template <class Pass, class After>
     int settle(const char* name, double first_pass_bytes, Pass&& pass, After&& after_batch, hipStream_t s);

@@ -74,6 +74,61 @@ __global__ void k_scale_f(const float* __restrict__ a, float c, float* __restric
         out[i] = a[i] * c;
 }
 
+// channel set from the D-inf area (build_channel_mask, lsdtt_lem_cpu.cpp:
+// 117-131; channel_threshold_quantity, lsdtt_stream_power_cpu.cpp:613-619),
+// and the DOF and fast sets without it
+__global__ void k_channel_dofs(const uint8_t* __restrict__ mask, const double* __restrict__ area, double thr,
+                               double dx, int divide, const uint8_t* __restrict__ dof0,
+                               const uint8_t* __restrict__ fast0, uint8_t* __restrict__ chan,
+                               uint8_t* __restrict__ dof, uint8_t* __restrict__ fast, long long n)
+{
+    const double den = dx > 1e-12 ? dx : 1e-12;
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
+         f += (long long)gridDim.x * blockDim.x) {
+        const uint8_t m = mask[f], d = dof0[f], q0 = fast0[f];
+        const double a = area[f];
+        const double q = divide ? a / den : a;
+        const uint8_t c = (m && q >= thr) ? 1 : 0;
+        chan[f] = c;
+        dof[f] = c ? 0 : d;
+        fast[f] = c ? 0 : q0;
+    }
+}
+
+// the general DOF list without channel cells, in its original (row-major)
+// order: one block, a running offset over chunks of blockDim.x entries;
+// *count receives the new length
+__global__ void __launch_bounds__(1024) k_gen_filter(const int* __restrict__ gen0, int n0,
+                                                     const uint8_t* __restrict__ chan, int* __restrict__ gen,
+                                                     int* __restrict__ count)
+{
+    __shared__ int wsum[16];
+    __shared__ int base;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    for (int s0 = 0; s0 < n0; s0 += blockDim.x) {
+        const int i = s0 + threadIdx.x;
+        const int f = i < n0 ? gen0[i] : 0;
+        const bool keep = i < n0 && !chan[f];
+        const unsigned long long bal = __ballot(keep);
+        const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wv] = __popcll(bal);
+        __syncthreads();
+        int off = base;
+        for (int k = 0; k < wv; ++k) off += wsum[k];
+        if (keep) gen[off + pre] = f;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int k = 0; k < nw; ++k) t += wsum[k];
+            base += t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = base;
+}
+
 __global__ void k_gather(const double* __restrict__ z, const int* __restrict__ idx, long long m,
                          double* __restrict__ out)
 {
@@ -260,7 +315,7 @@ LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, con
     scal.reserve(kScalCount + 8);
     hpin.reserve(32);
     tickets.reserve(4);
-    flags.reserve(8);
+    flags.reserve(12);  // [8] the filtered general count
     LSDTT_HIP(hipMemset(tickets.p, 0, sizeof(unsigned) * 4));
     LSDTT_HIP(hipMemset(scal.p, 0, sizeof(double) * (kScalCount + 8)));
 }
@@ -285,11 +340,17 @@ bool LemEngine::channels_active() const
     return regime == 2;
 }
 
+bool LemEngine::threshold_channels() const
+{
+    return p.enable_hillslope && channels_active() && channel_h.empty() && p.threshold > 0.0;
+}
+
 void LemEngine::build_dofs()
 {
     // hillslope DOFs: valid, not on an 'o' (Dirichlet) edge
     // (lsdtt_hillslope_operator_cpu.hpp:136-142), not a channel pixel
-    // (build_channel_mask, lsdtt_lem_cpu.cpp:85-134: explicit mask only)
+    // (build_channel_mask, lsdtt_lem_cpu.cpp:85-134: an explicit mask here;
+    // a threshold-derived set is removed each step by derive_channels)
     const bool use_channels = channels_active() && !channel_h.empty();
     std::vector<uint8_t> d(static_cast<size_t>(n));
     for (int r = 0; r < nr; ++r)
@@ -322,6 +383,51 @@ void LemEngine::build_dofs()
     if (n_gen) LSDTT_HIP(hipMemcpy(gen.p, gh.data(), sizeof(int) * gh.size(), hipMemcpyHostToDevice));
     partials.reserve(static_cast<size_t>(grid_for(n, 2048) + hs_general_blocks(n_gen) + 1) * 8);
     c9g.reserve(static_cast<size_t>(9) * (n_gen ? n_gen : 1));
+    if (threshold_channels()) {
+        // the channel-free sets, from which each step removes its channels
+        dof_base.reserve(n);
+        fast_base.reserve(n);
+        LSDTT_HIP(hipMemcpy(dof_base.p, dof.p, n, hipMemcpyDeviceToDevice));
+        LSDTT_HIP(hipMemcpy(fast_base.p, fast.p, n, hipMemcpyDeviceToDevice));
+        n_gen_base = n_gen;
+        gen_base.reserve(gh.size() ? gh.size() : 1);
+        if (n_gen) LSDTT_HIP(hipMemcpy(gen_base.p, gen.p, sizeof(int) * gh.size(), hipMemcpyDeviceToDevice));
+        chan.reserve(n);
+        drec.reserve(2 * (size_t)n);
+        dw.reserve(2 * (size_t)n);
+        darea.reserve(n);
+        darea_warm = false;
+    }
+}
+
+void LemEngine::derive_channels(hipStream_t s)
+{
+    // D-inf graph of the post-incision surface as a float raster, interior
+    // nodata as outlets (build_hillslope_flowgraph, lsdtt_lem_cpu.cpp:137-154)
+    DinfOptions o;
+    o.dx = dx;
+    o.bc = p.bc;
+    timer.begin("dinf_receivers", s);
+    dinf_receivers<double>(z.p, mask.p, nr, nc, o, drec.p, dw.p, nullptr, nullptr, s);
+    // z 8, mask 1 in; rec 8, w 8 out
+    timer.end(s, 25.0 * n);
+    timer.begin("dinf_order", s);
+    tiles.dinf_order(drec.p, mask.p, s);
+    timer.end(s, 9.0 * n);
+    last_dinf_passes = tiles.dinf_accumulate(drec.p, dw.p, mask.p, dx * dx, darea.p, darea_warm, s);
+    darea_warm = true;
+    timer.begin("channel_dofs", s);
+    const int g = grid_for(n, 4096);
+    k_channel_dofs<<<g, kB, 0, s>>>(mask.p, darea.p, p.threshold, dx, p.divide_area_by_dx, dof_base.p,
+                                    fast_base.p, chan.p, dof.p, fast.p, n);
+    LSDTT_LAUNCH_CHECK();
+    int* hn = reinterpret_cast<int*>(hpin.p + 30);
+    k_gen_filter<<<1, 1024, 0, s>>>(gen_base.p, n_gen_base, chan.p, gen.p, flags.p + 8);
+    LSDTT_LAUNCH_CHECK();
+    LSDTT_HIP(hipMemcpyAsync(hn, flags.p + 8, sizeof(int), hipMemcpyDeviceToHost, s));
+    LSDTT_HIP(hipStreamSynchronize(s));
+    timer.end(s, 13.0 * n);
+    n_gen = *hn;
 }
 
 void LemEngine::set_channel_mask(const uint8_t* ch)
@@ -674,11 +780,6 @@ void LemEngine::stats(LemStats* st, hipStream_t s)
 int LemEngine::step(LemStats* st, hipStream_t s)
 {
     *st = LemStats{};
-    if (p.enable_hillslope && channels_active() && channel_h.empty() && p.threshold > 0.0)
-        throw std::invalid_argument(
-            "run_lem_step: a channel set derived from incision_threshold_area needs the D-infinity flow graph "
-            "(lsdtt_lem_cpu.cpp:137-154), which this library does not build; pass an explicit channel mask or "
-            "the coupled regime");
     const int g = grid_for(n, 4096);
     k_copy<<<g, kB, 0, s>>>(z.p, z_before.p, n);
     LSDTT_LAUNCH_CHECK();
@@ -694,6 +795,7 @@ int LemEngine::step(LemStats* st, hipStream_t s)
         st->stream_power_tile_runs = last_sp_tiles;
     }
     int rc = 0;
+    if (threshold_channels()) derive_channels(s);
     if (p.enable_hillslope) {
         // Newton first; damped Picard when it fails (from the post-incision
         // surface, the Newton result discarded) or when it is forced

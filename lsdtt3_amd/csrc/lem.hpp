@@ -20,6 +20,7 @@
 
 #include "common.hpp"
 #include "dbuf.hpp"
+#include "dinf.hpp"
 #include "flow.hpp"
 #include "fluvial.hpp"
 #include "hillslope.hpp"
@@ -108,6 +109,16 @@ public:
     long long last_eroded = 0;
 
     dbuf<uint8_t> mask, dof, fast;
+    // channel set derived each step from the D-infinity drainage area of the
+    // post-incision surface (lsdtt_lem_cpu.cpp:86-154): the DOF arrays
+    // without channels, the receivers, the area and the set itself
+    dbuf<uint8_t> dof_base, fast_base, chan;
+    dbuf<int> gen_base, drec;
+    dbuf<float> dw;
+    dbuf<double> darea;
+    int n_gen_base = 0;
+    bool darea_warm = false;
+    int last_dinf_passes = 0;
     dbuf<double> z, z_before, K_field, bel, area_sm;
     std::vector<uint8_t> mask_h, channel_h;  // host copies (DOF set rebuilds)
     dbuf<int> bflats, gen;
@@ -160,7 +171,15 @@ private:
     // hillslope DOFs (valid, not on an 'o' edge, not a channel pixel of an
     // explicit mask outside the coupled regime), fast flags, general list
     void build_dofs();
+    void derive_channels(hipStream_t s);
+
+public:
+    // the hillslope has a Dirichlet channel set (channels_as_boundaries)
     bool channels_active() const;
+    // channels from the drainage-area threshold (no explicit mask, not coupled)
+    bool threshold_channels() const;
+
+private:
     void stats(LemStats* st, hipStream_t s);
 };
 

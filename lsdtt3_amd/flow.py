@@ -182,6 +182,50 @@ class FlowInfo:
                 for i in range(b)]
 
 
+class FlowGraph:
+    """D-infinity flow graph on the GPU, mirroring the reference's Python
+    ``FlowGraph(dem, method="dinf")`` (python_bindings/bindings/flow_graph.cpp:
+    90-101) over ``lsdtt_flowgraph_cpu`` (src/cpu/lsdtt_flowgraph_cpu.cpp).
+
+    ``receivers`` (n, 2) int64 flat indices (-1 = none) and ``weights``
+    (n, 2) float32, in the reference's CSR slot order; ``slope`` and
+    ``bearing`` rasters; ``accumulate()`` the drainage area (cell_size^2 per
+    valid cell, float64).  Only method "dinf" is built here.
+    """
+
+    def __init__(self, dem, mask, cell_size, method="dinf", boundary_conditions="oooo",
+                 interior_nodata_is_outlet=True, no_data_value=-9999.0):
+        if method not in ("dinf", "d_infinity"):
+            raise ValueError("only the D-infinity method is built on the GPU")
+        z = np.ascontiguousarray(dem, dtype=np.float32)
+        if z.ndim != 2:
+            raise ValueError("dem must be 2-D")
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        self.n_rows, self.n_cols = z.shape
+        n = z.size
+        self.cell_size = float(cell_size)
+        self.receivers = np.empty((n, 2), np.int64)
+        self.weights = np.empty((n, 2), np.float32)
+        self.slope = np.empty(z.shape, np.float32)
+        self.bearing = np.empty(z.shape, np.float32)
+        self._area = np.empty(z.shape, np.float64)
+        N.check(N.lib().lsdtt_gpu_dinf_graph(N.ptr(z), N.ptr(m), self.n_rows, self.n_cols, self.cell_size,
+                                             boundary_conditions.encode()[:4], int(interior_nodata_is_outlet),
+                                             float(no_data_value), N.ptr(self.receivers), N.ptr(self.weights),
+                                             N.ptr(self.slope), N.ptr(self.bearing), N.ptr(self._area)))
+
+    def accumulate(self):
+        """drainage area (m^2) in float64, 0 off the valid cells"""
+        return self._area.copy()
+
+    def receivers_of(self, row, col):
+        f = int(row) * self.n_cols + int(col)
+        k = int((self.receivers[f] >= 0).sum())
+        rec = self.receivers[f, :k]
+        return dict(pixels=np.stack([rec // self.n_cols, rec % self.n_cols], axis=1),
+                    weights=self.weights[f, :k].copy())
+
+
 def fill_pits(dem, mask, no_data_value=-9999.0, boundary_conditions="oooo"):
     """Epsilon-free fill (lsdtt::fill_pits with min_slope <= 0), float32."""
     z = np.ascontiguousarray(dem, dtype=np.float32)

@@ -155,6 +155,14 @@ class LandscapeModel:
             self.step()
         return self.last_stats
 
+    @property
+    def channel_mask(self):
+        """the hillslope's Dirichlet channel set of the last step (uint8):
+        the explicit mask, the D-inf area threshold's cells, or none"""
+        out = np.empty((self.n_rows, self.n_cols), np.uint8)
+        N.check(N.lib().lsdtt_gpu_lem_get_channel_mask(self._h, N.ptr(out)))
+        return out
+
     # ---- tile shape of the routing sweeps (speed only) ----------------------
     @property
     def tile_height(self) -> int:

@@ -68,6 +68,37 @@ void orc_upstream_counts(int64_t n, const uint8_t* mask,
 /* 64-bit FNV-1a continuation over n bytes */
 uint64_t orc_fnv1a(uint64_t h, const void* p, int64_t n);
 
+/* ---- D-infinity flow graph (oracle/orc_dinf.c) ------------------------ */
+
+/* build_dinf: per cell up to two receivers rec[2f], rec[2f+1] (-1 = none)
+ * with float weights w[2f..2f+1], nrec[f] in {0,1,2}; slope and bearing
+ * (nodata off the receivers' cells; may be NULL).  libm_atan2: 1 calls
+ * atan2f like the reference, 0 uses the correctly rounded value.  Returns
+ * ORC_ERR_BC for a bad boundary string. */
+int orc_dinf_graph(int nr, int nc, double dx, const float* z, const uint8_t* mask, float nodata,
+                   const char* bc, int interior_nodata_is_outlet, int libm_atan2, int64_t* rec, float* w,
+                   uint8_t* nrec, float* slope, float* bearing);
+
+/* build_topological_order over the D-inf graph (unique donors, Kahn FIFO,
+ * sources in row-major order); returns the order's length, -1 on a bad
+ * receiver or weights that do not sum to 1 */
+int64_t orc_dinf_topo(int64_t n, const uint8_t* valid, const int64_t* rec, const float* w,
+                      const uint8_t* nrec, int64_t* topo);
+
+/* accumulate: dx^2 per cell of the order, pushed along weighted edges */
+void orc_dinf_accumulate(int64_t n, const int64_t* rec, const float* w, const uint8_t* nrec, const int64_t* topo,
+                         int64_t n_topo, double dx, double* accum);
+
+/* channel_threshold_quantity: area / max(1e-12, dx) or area */
+double orc_channel_threshold_quantity(double area, double dx, int divide_area_by_dx);
+
+/* The hillslope step's channel set from the D-inf graph of the float
+ * surface z: valid cells whose threshold quantity >= threshold.  area_out
+ * (may be NULL) receives the accumulated area (m^2). */
+int orc_dinf_channel_mask(int nr, int nc, double dx, const float* z, const uint8_t* mask, const char* bc,
+                          double threshold, int divide_area_by_dx, int libm_atan2, uint8_t* is_channel,
+                          double* area_out);
+
 /* ---- chi (oracle/orc_chi.c) ------------------------------------------ */
 
 /* get_chi_raster: chi from every outlet (seed 0), A = s_size * dx^2; chi is
@@ -184,6 +215,7 @@ typedef struct {
     const uint8_t* channel;        /* explicit Dirichlet channel set for the hillslope, or NULL */
     int    force_picard;           /* hillslope_force_picard */
     orc_picard_params picard;      /* the fallback's settings (D, Sc, nonlinear taken from hs) */
+    int    regime;                 /* hillslope coupling: 0 automatic, 1 coupled, 2 channels as boundaries */
 } orc_lem_params;
 
 typedef struct {

@@ -83,6 +83,18 @@ def lib():
                                         C.c_double, C.c_double, C.c_int64, i64p, f32p, i64p, i64p,
                                         C.POINTER(C.c_int64)]
         L.orc_chi_threshold.restype = C.c_int64
+        L.orc_dinf_graph.argtypes = [C.c_int, C.c_int, C.c_double, f32p, u8p, C.c_float, C.c_char_p, C.c_int,
+                                     C.c_int, i64p, f32p, u8p, f32p, f32p]
+        L.orc_dinf_graph.restype = C.c_int
+        L.orc_dinf_topo.argtypes = [C.c_int64, u8p, i64p, f32p, u8p, i64p]
+        L.orc_dinf_topo.restype = C.c_int64
+        L.orc_dinf_accumulate.argtypes = [C.c_int64, i64p, f32p, u8p, i64p, C.c_int64, C.c_double, f64p]
+        L.orc_dinf_accumulate.restype = None
+        L.orc_channel_threshold_quantity.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.orc_channel_threshold_quantity.restype = C.c_double
+        L.orc_dinf_channel_mask.argtypes = [C.c_int, C.c_int, C.c_double, f32p, u8p, C.c_char_p, C.c_double,
+                                            C.c_int, C.c_int, u8p, C.c_void_p]
+        L.orc_dinf_channel_mask.restype = C.c_int
         _lib = L
     return _lib
 
@@ -283,7 +295,7 @@ class LemParams(C.Structure):
                 ("enable_fluvial", C.c_int), ("enable_hillslope", C.c_int),
                 ("enable_uplift", C.c_int), ("bc", C.c_char * 4), ("hs", HsParams),
                 ("area_kernel", C.c_int), ("width_factor", C.c_double), ("channel", C.c_void_p),
-                ("force_picard", C.c_int), ("picard", PicardParams)]
+                ("force_picard", C.c_int), ("picard", PicardParams), ("regime", C.c_int)]
 
 
 class LemStats(C.Structure):
@@ -354,9 +366,12 @@ def boundary_flats(mask, bc):
 def lem_params(dt, K, m, n, U, D, Sc=0.8, nonlinear=False, bc="oooo", threshold=0.0,
                slope_clamp=1e-5, baselevel_fall=0.0, fluvial=True, hillslope=True, uplift=True,
                rel_tol=1e-8, abs_tol=1e-12, area_kernel=1, width_factor=1.0, channel=None,
-               divide_area_by_dx=True, force_picard=False, picard=None, max_iterations=50):
+               divide_area_by_dx=True, force_picard=False, picard=None, max_iterations=50, regime=0):
     """channel: optional uint8 Dirichlet channel mask for the hillslope (kept
-    alive on the returned structure)."""
+    alive on the returned structure).  regime: 0 automatic (channels as
+    boundaries when a threshold or a mask is given), 1 coupled, 2 channels
+    as boundaries; without a mask the channels are the cells whose D-inf
+    drainage area passes the threshold."""
     ch = None if channel is None else np.ascontiguousarray(channel, np.uint8).ravel()
     p = LemParams(dt, K, m, n, threshold, slope_clamp, U, baselevel_fall, int(divide_area_by_dx), int(fluvial),
                   int(hillslope), int(uplift), bc.encode(),
@@ -364,6 +379,7 @@ def lem_params(dt, K, m, n, U, D, Sc=0.8, nonlinear=False, bc="oooo", threshold=
                   None if ch is None else ch.ctypes.data_as(C.c_void_p), int(force_picard),
                   picard if picard is not None else picard_params(D, Sc, nonlinear))
     p.hs.max_iterations = int(max_iterations)
+    p.regime = int(regime)
     p._channel_ref = ch
     return p
 
@@ -395,3 +411,53 @@ def lem_step(z, mask, dx, params, boundary=None, K_field=None):
     rc = lib().orc_lem_step(nr, nc, float(dx), m.ravel(), C.byref(params), bf, len(bf), kptr,
                             zz.ravel(), C.byref(st))
     return zz, st.as_dict(), rc
+
+
+def dinf_graph(z, mask, dx, bc="oooo", nodata=-9999.0, interior_nodata_is_outlet=True, libm_atan2=False):
+    """orc_dinf_graph on a float32 surface: dict of rec (n, 2) int64 (-1 =
+    none), w (n, 2) float32, nrec, slope, bearing (float32 rasters)."""
+    zf = np.ascontiguousarray(z, np.float32)
+    nr, nc = zf.shape
+    n = nr * nc
+    rec = np.empty((n, 2), np.int64)
+    w = np.empty((n, 2), np.float32)
+    nrec = np.empty(n, np.uint8)
+    slope = np.empty((nr, nc), np.float32)
+    bearing = np.empty((nr, nc), np.float32)
+    st = lib().orc_dinf_graph(nr, nc, float(dx), zf.ravel(), np.ascontiguousarray(mask, np.uint8).ravel(),
+                              float(nodata), bc.lower().encode(), int(interior_nodata_is_outlet), int(libm_atan2),
+                              rec.ravel(), w.ravel(), nrec, slope.ravel(), bearing.ravel())
+    if st != 0:
+        raise OracleError(f"dinf graph failed ({st})")
+    return dict(rec=rec, w=w, nrec=nrec, slope=slope, bearing=bearing)
+
+
+def dinf_accumulate(g, valid, dx):
+    """Kahn order then accumulate (dx^2 per valid cell); (area, topo)"""
+    v = np.ascontiguousarray(valid, np.uint8).ravel()
+    n = v.size
+    topo = np.empty(n, np.int64)
+    nt = lib().orc_dinf_topo(n, v, g["rec"].ravel(), g["w"].ravel(), g["nrec"], topo)
+    if nt < 0:
+        raise OracleError("bad D-inf graph")
+    acc = np.empty(n, np.float64)
+    lib().orc_dinf_accumulate(n, g["rec"].ravel(), g["w"].ravel(), g["nrec"], topo, nt, float(dx), acc)
+    return acc, topo[:nt]
+
+
+def channel_threshold_quantity(area, dx, divide_area_by_dx=True):
+    return lib().orc_channel_threshold_quantity(float(area), float(dx), int(divide_area_by_dx))
+
+
+def dinf_channel_mask(z, mask, dx, threshold, bc="oooo", divide_area_by_dx=True, libm_atan2=False):
+    """orc_dinf_channel_mask: (is_channel uint8 raster, area float64 raster)"""
+    zf = np.ascontiguousarray(z, np.float32)
+    nr, nc = zf.shape
+    out = np.empty((nr, nc), np.uint8)
+    area = np.empty((nr, nc), np.float64)
+    st = lib().orc_dinf_channel_mask(nr, nc, float(dx), zf.ravel(), np.ascontiguousarray(mask, np.uint8).ravel(),
+                                     bc.lower().encode(), float(threshold), int(divide_area_by_dx), int(libm_atan2),
+                                     out.ravel(), area.ctypes.data_as(C.c_void_p))
+    if st != 0:
+        raise OracleError(f"dinf channel mask failed ({st})")
+    return out, area

@@ -658,6 +658,28 @@ int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
         free(rcv); free(dir); free(don); free(outl); free(topo); free(cnt); free(area);
     }
 
+    /* the hillslope's Dirichlet channel set (lsdtt_lem_cpu.cpp:86-134,
+     * 397-432): none in the coupled regime, else the explicit mask, else the
+     * drainage-area threshold on the D-inf graph of the post-incision
+     * surface (a float raster in the reference) */
+    const uint8_t* channel = NULL;
+    uint8_t* derived = NULL;
+    if (p->enable_hillslope) {
+        const int regime = p->regime != 0 ? p->regime : ((p->threshold > 0.0 || p->channel) ? 2 : 1);
+        if (regime == 2 && p->channel) {
+            channel = p->channel;
+        } else if (regime == 2 && p->threshold > 0.0) {
+            float* zf = (float*)malloc(sizeof(float) * (size_t)n);
+            for (int64_t i = 0; i < n; ++i) zf[i] = (float)z[i];
+            derived = (uint8_t*)malloc((size_t)n);
+            if (orc_dinf_channel_mask(nr, nc, dx, zf, mask, p->bc, p->threshold, p->divide_area_by_dx, 0, derived,
+                                      NULL) != ORC_OK)
+                status = ORC_ERR_ARG;
+            channel = derived;
+            free(zf);
+        }
+    }
+
     if (p->enable_hillslope) {
         /* Newton first, Picard when it fails or is forced (lsdtt_lem_cpu.cpp:531-582) */
         int done = 0;
@@ -665,7 +687,7 @@ int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
             double* keep = (double*)malloc(sizeof(double) * (size_t)n);
             memcpy(keep, z, sizeof(double) * (size_t)n);
             orc_hs_result hr;
-            const int rc = orc_hillslope_transient(nr, nc, dx, mask, p->bc, p->channel, &p->hs, p->dt, p->dt, z, &hr);
+            const int rc = orc_hillslope_transient(nr, nc, dx, mask, p->bc, channel, &p->hs, p->dt, p->dt, z, &hr);
             st->newton_iterations = hr.iterations;
             st->newton_substeps = hr.substeps;
             st->hillslope_residual = hr.final_residual;
@@ -681,7 +703,7 @@ int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
             pp.Sc = p->hs.Sc;
             pp.nonlinear = p->hs.nonlinear;
             orc_picard_result pr;
-            orc_picard_transient(nr, nc, dx, mask, p->bc, p->channel, &pp, p->dt, z, &pr);
+            orc_picard_transient(nr, nc, dx, mask, p->bc, channel, &pp, p->dt, z, &pr);
             st->hillslope_method = 1;
             st->picard_iterations = pr.picard_iters;
             st->picard_substeps = pr.substeps;
@@ -725,5 +747,6 @@ int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
     free(lc);
     free(z_before);
     free(bel);
+    free(derived);
     return status;
 }

@@ -262,11 +262,6 @@ def test_gpu_lem_option_errors(gpu):
                 dict(multiply_by_w_over_dx=1, channel_width_w=0.0), dict(hillslope_coupling_regime=3)):
         with pytest.raises(GpuError):
             gpu.LandscapeModel(z, 10.0, **bad)
-    # a threshold-derived channel set needs D-infinity routing: refused
-    lem = gpu.LandscapeModel(z, 10.0, incision_threshold_specific_area=50.0, hillslope_diffusivity=0.01)
-    with pytest.raises(GpuError) as e:
-        lem.step()
-    assert e.value.status == 1 and "D-infinity" in str(e.value)
 
 
 @pytest.mark.parametrize("nonlinear", [False, True])
