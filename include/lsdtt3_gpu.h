@@ -216,6 +216,11 @@ typedef struct {
     double picard_grow_factor;       /* 1.5 */
     double picard_shrink_factor;     /* 0.5 */
     double picard_dz_target;         /* 0 = off */
+    /* enable_dinf_fluvial_erosion (lsdtt_lem_options, lsdtt_lem_cpu.hpp:172;
+     * run_lem_step :357-391): D-infinity incision
+     * (compute_dinf_stream_power_incision, lsdtt_stream_power_cpu.cpp:
+     * 329-602) when D8 incision is off; default 0 */
+    int32_t enable_dinf_fluvial_erosion;
 } lsdtt_gpu_lem_options;
 
 typedef struct {

@@ -23,7 +23,7 @@ class LemOptions(C.Structure):
         ("relaxation_omega", C.c_double), ("picard_slope_ratio_cap", C.c_double),
         ("picard_dt_init", C.c_double), ("picard_dt_min", C.c_double), ("picard_dt_max", C.c_double),
         ("picard_grow_factor", C.c_double), ("picard_shrink_factor", C.c_double),
-        ("picard_dz_target", C.c_double),
+        ("picard_dz_target", C.c_double), ("enable_dinf_fluvial_erosion", C.c_int32),
     ]
 
 

@@ -37,7 +37,9 @@ LemParams to_params(const lsdtt_gpu_lem_options& o, double o_cell_size)
     p.Sc = o.critical_hillslope_slope;
     p.U = o.uplift_rate;
     p.baselevel_fall = o.baselevel_fall_rate;
-    p.enable_fluvial = o.enable_d8_fluvial_erosion;
+    // run_lem_step: D8 incision, else D-inf incision (lsdtt_lem_cpu.cpp:324-391)
+    p.enable_fluvial = o.enable_d8_fluvial_erosion || o.enable_dinf_fluvial_erosion;
+    p.fluvial_dinf = !o.enable_d8_fluvial_erosion && o.enable_dinf_fluvial_erosion;
     p.enable_hillslope = o.enable_hillslope_transport;
     p.enable_uplift = o.enable_uplift;
     p.bc = BC{o.boundary_conditions[0], o.boundary_conditions[1], o.boundary_conditions[2],
@@ -141,6 +143,7 @@ void lsdtt_gpu_lem_options_init(lsdtt_gpu_lem_options* o)
     o->uplift_rate = 1e-4;
     o->baselevel_fall_rate = 0.0;
     o->enable_d8_fluvial_erosion = 1;
+    o->enable_dinf_fluvial_erosion = 0;
     o->enable_hillslope_transport = 0;
     o->enable_uplift = 1;
     std::memcpy(o->boundary_conditions, "oooo", 4);

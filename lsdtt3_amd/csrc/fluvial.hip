@@ -823,6 +823,347 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// D-infinity stream power (lsdtt_stream_power_cpu.cpp:329-602) by the tile
+// passes: a cell's new elevation is a function of its (at most two)
+// receivers' new elevations, so a tile evaluates its cells in order of
+// in-tile height above the tile's exit, height(v) = 1 + the largest height
+// of v's in-tile receivers (0 without); receivers in other tiles are read
+// from HBM, and a changed cell re-runs the tiles of its donors in other
+// tiles (directions in xdon).
+// ---------------------------------------------------------------------------
+template <int TH>
+__global__ void __launch_bounds__(WL) k_dinf_tile_up_order(const int* __restrict__ rec,
+                                                           const uint8_t* __restrict__ mask, int nr, int nc,
+                                                           BC bc, short* __restrict__ ordg,
+                                                           short* __restrict__ lstartg, int* __restrict__ tile_maxdep,
+                                                           uint8_t* __restrict__ xdon)
+{
+    constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
+    __shared__ int hgt[TN];
+    __shared__ int hist[TN + 1];
+    __shared__ int cur[TN];
+    __shared__ short lrec[2][TN];
+    __shared__ uint8_t xd[TN];
+    const int tx = blockIdx.x, ty = blockIdx.y, lane = threadIdx.x;
+    const int tile = ty * gridDim.x + tx;
+    const int r0 = ty * TH, c0 = tx * TW;
+    bool valid[PER];
+    {
+        uint8_t m[PER];
+        int g0[PER], g1[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+            m[k] = mask[q.f];
+            g0[k] = rec[2 * (size_t)q.f];
+            g1[k] = rec[2 * (size_t)q.f + 1];
+            valid[k] = q.in && m[k];
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = lane + WL * k;
+            lrec[0][i] = static_cast<short>((valid[k] && g0[k] >= 0) ? local_index<TH>(g0[k], nc, r0, c0) : -1);
+            lrec[1][i] = static_cast<short>((valid[k] && g1[k] >= 0) ? local_index<TH>(g1[k], nc, r0, c0) : -1);
+            hgt[i] = 0;
+            hist[i] = 0;
+            xd[i] = 0;
+        }
+    }
+    if (lane == 0) hist[TN] = 0;
+    __syncthreads();
+    // donors in other tiles: only ring cells can have them (ring slots as in
+    // k_tile_order); a neighbour reached by two directions counts once
+    {
+        const int h = min(TH, nr - r0), w = min(TW, nc - c0);
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int sl = lane + WL * half;
+            int lr, lc;
+            bool ok;
+            if (sl < 32) {
+                lr = 0, lc = sl, ok = lc < w;
+            } else if (sl < 64) {
+                lr = h - 1, lc = sl - 32, ok = lc < w && h > 1;
+            } else if (sl < 96) {
+                lr = sl - 64, lc = 0, ok = lr >= 1 && lr <= h - 2;
+            } else {
+                lr = sl - 96, lc = w - 1, ok = lr >= 1 && lr <= h - 2 && w > 1;
+            }
+            const int r = r0 + lr, c = c0 + lc;
+            const int f = ok ? r * nc + c : 0;
+            int gs[8];
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                int g = ok ? neighbour(r, c, d, nr, nc, bc) : -1;
+                if (g == f || (g >= 0 && local_index<TH>(g, nc, r0, c0) >= 0)) g = -1;
+                gs[d] = g;
+            }
+            unsigned bits = 0;
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                bool dup = false;
+#pragma unroll
+                for (int e = 0; e < d; ++e) dup |= gs[e] == gs[d];
+                if (gs[d] >= 0 && !dup) {
+                    const int a0 = rec[2 * (size_t)gs[d]], a1 = rec[2 * (size_t)gs[d] + 1];
+                    if (a0 == f || a1 == f) bits |= 1u << d;
+                }
+            }
+            if (ok && mask[f]) xd[lr * TW + lc] = static_cast<uint8_t>(bits);
+        }
+    }
+    // heights by relaxation (each lane raises only its own cells)
+    for (int sweep = 0; sweep < TN; ++sweep) {
+        bool ch = false;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (!valid[k]) continue;
+            const int i = lane + WL * k;
+            int hv = hgt[i];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int l = lrec[j][i];
+                if (l >= 0) hv = max(hv, hgt[l] + 1);
+            }
+            if (hv > hgt[i]) {
+                hgt[i] = hv;
+                ch = true;
+            }
+        }
+        const bool any = __ballot(ch) != 0;
+        __syncthreads();
+        if (!any) break;
+    }
+    int mx = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = lane + WL * k;
+        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+        if (q.in) xdon[q.f] = xd[i];
+        if (valid[k]) {
+            const int d = min(hgt[i], TN - 1);
+            hgt[i] = d;
+            atomicAdd(&hist[d], 1);
+            mx = max(mx, d);
+        }
+    }
+    mx = wave_max(mx);
+    __syncthreads();
+    int loc[PER], sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        loc[k] = hist[lane * PER + k];
+        sum += loc[k];
+    }
+    int total;
+    int base = wave_exclusive_scan(sum, &total);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        hist[lane * PER + k] = base;
+        lstartg[(size_t)tile * (TN + 1) + lane * PER + k] = static_cast<short>(base);
+        base += loc[k];
+        cur[lane * PER + k] = 0;
+    }
+    if (lane == 0) {
+        hist[TN] = total;
+        lstartg[(size_t)tile * (TN + 1) + TN] = static_cast<short>(total);
+        tile_maxdep[tile] = mx;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+        if (valid[k]) {
+            const int i = lane + WL * k;
+            const int d = hgt[i];
+            ordg[(size_t)tile * TN + hist[d] + atomicAdd(&cur[d], 1)] = static_cast<short>(i);
+        }
+}
+
+struct DinfSpArgs {
+    const int* rec;
+    const float* w;
+    const uint8_t* mask;
+    const double* area;  // m^2 per cell (after any smoothing)
+    const short* ord;
+    const short* lstart;
+    const int* tile_maxdep;
+    const uint8_t* xdon;
+    const double* z_old;
+    const double* K;
+    double* z;
+    int* tile_eroded;
+    SpParams p;
+};
+
+// the reference's receiver classes (lsdtt_stream_power_cpu.cpp:435-441)
+constexpr uint8_t kDinfSkip = 0, kDinfCard = 1, kDinfDiag = 2, kDinfTwo = 3;
+
+template <int TH>
+__global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int nc, BC bc, PassArgs P)
+{
+    constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
+    if (P.prev_marked && *P.prev_marked == 0) return;
+    const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x;
+    const int tile = ty * ntx + tx;
+    const int lane = threadIdx.x;
+    if (lane == 0) P.act_clear[tile] = 0;
+    if (!P.act[tile]) return;
+    __shared__ double val[TN];  // z_old until the cell's level, then its new z
+    __shared__ double kam[TN];
+    __shared__ int gr[2][TN];   // global receivers
+    __shared__ short lr[2][TN]; // local receivers, -1 outside the tile or none
+    __shared__ uint8_t kind[TN];
+    __shared__ short ord[TN];
+    const int r0 = ty * TH, c0 = tx * TW;
+    const SpParams& p = A.p;
+    bool valid[PER];
+    uint8_t x[PER];
+    double zcur[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+        const uint8_t m = A.mask[q.f];
+        const int g0 = A.rec[2 * (size_t)q.f], g1 = A.rec[2 * (size_t)q.f + 1];
+        const float w0 = A.w[2 * (size_t)q.f], w1 = A.w[2 * (size_t)q.f + 1];
+        const double zo = A.z_old[q.f];
+        const double Kf = A.K ? A.K[q.f] : p.K0;
+        const double ar = A.area[q.f];
+        x[k] = A.xdon[q.f];
+        zcur[k] = A.z[q.f];
+        valid[k] = q.in && m;
+        const int i = lane + WL * k;
+        const int nrec = (g0 >= 0) + (g1 >= 0);
+        uint8_t kd = kDinfSkip;
+        double km = 0.0;
+        if (valid[k] && nrec > 0) {
+            double area = ar;
+            if (p.divide) area /= p.dx;
+            if (!(p.threshold > 0.0 && area < p.threshold)) {
+                const bool only_card = nrec == 1 || w0 >= 1.0f - 1e-6f;
+                const bool only_diag = nrec > 1 && w1 >= 1.0f - 1e-6f;
+                kd = only_card ? kDinfCard : (only_diag ? kDinfDiag : kDinfTwo);
+                km = Kf * p.width * pow(area, p.m);
+            }
+        }
+        val[i] = zo;
+        kam[i] = km;
+        kind[i] = kd;
+        gr[0][i] = g0;
+        gr[1][i] = g1;
+        lr[0][i] = static_cast<short>((valid[k] && g0 >= 0) ? local_index<TH>(g0, nc, r0, c0) : -1);
+        lr[1][i] = static_cast<short>((valid[k] && g1 >= 0) ? local_index<TH>(g1, nc, r0, c0) : -1);
+        ord[i] = A.ord[(size_t)tile * TN + i];
+    }
+    const short* __restrict__ lsg = A.lstart + (size_t)tile * (TN + 1);
+    const int maxdep = A.tile_maxdep[tile];
+    int lb = lsg[0], le = lsg[1];
+    __syncthreads();
+    const double dxc = p.dx, dxd = p.dx * kSqrt2, tol = p.slope_clamp * p.dx;
+    int eroded = 0;
+    for (int L = 0; L <= maxdep; ++L) {
+        const int lnext = L + 2 <= TN ? lsg[min(L + 2, TN)] : le;
+        for (int q = lb + lane; q < le; q += WL) {
+            const int i = ord[q];
+            const uint8_t kd = kind[i];
+            if (kd == kDinfSkip) continue;
+            const double zo = val[i];
+            // recv_elev (:443-448): the receivers' current elevations
+            double ze[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int l = lr[j][i], g = gr[j][i];
+                ze[j] = l >= 0 ? val[l]
+                               : (g >= 0 ? __hip_atomic_load(&A.z[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                         : zo);
+            }
+            const double km = kam[i];
+            double zn = zo;
+            if (kd == kDinfTwo) {
+                // :459-493
+                double eps = tol + 1.0;
+                int it = 0;
+                do {
+                    const double t1 = (ze[0] - zn) / dxc, t2 = (ze[1] - ze[0]) / dxc;
+                    const double sbt = t1 * t1 + t2 * t2;
+                    if (sbt <= 0.0) {
+                        eps = 0.0;
+                    } else {
+                        const double fz = zn - zo + km * pow(sbt, 0.5 * p.nexp) * p.dt;
+                        const double fd =
+                            1.0 - km * p.nexp * (ze[0] - zn) / (dxc * dxc) * pow(sbt, 0.5 * p.nexp - 1.0) * p.dt;
+                        eps = fz / fd;
+                    }
+                    zn -= eps;
+                    ++it;
+                    if (it > 100) eps = tol / 2.0;
+                } while (fabs(eps) > tol);
+                if ((zn - zo) > tol || isnan(zn)) zn = zo;
+                const double zmin = ze[0] < ze[1] ? ze[0] : ze[1];
+                if (zn < zmin) zn = zmin + p.slope_clamp * dxc;
+                ++eroded;
+            } else {
+                // :497-566
+                const double zr = kd == kDinfDiag ? ze[1] : ze[0];
+                const double dxe = kd == kDinfDiag ? dxd : dxc;
+                if (zr < zo) {
+                    if (p.n_is_one) {
+                        const double spf = km * (p.dt / dxe);
+                        zn = (zo + zr * spf) / (1.0 + spf);
+                        if ((zn - zo) > tol || isnan(zn)) zn = zo;
+                    } else {
+                        double eps = tol + 1.0;
+                        int it = 0;
+                        do {
+                            const double slope = (zn - zr) / dxe;
+                            if (slope <= 0.0) {
+                                eps = 0.0;
+                            } else {
+                                const double fz = zn - zo + km * p.dt * pow(slope, p.nexp);
+                                const double fd = 1.0 + km * p.dt * (p.nexp / dxe) * pow(slope, p.nexp - 1.0);
+                                eps = fz / fd;
+                            }
+                            zn -= eps;
+                            ++it;
+                            if (it > 100) eps = tol / 2.0;
+                        } while (fabs(eps) > tol);
+                        if ((zn - zo) > tol || isnan(zn)) zn = zo;
+                        if (zn < zr) zn = zr + p.slope_clamp * dxe;
+                    }
+                    ++eroded;
+                }
+            }
+            val[i] = zn;
+        }
+        lb = le;
+        le = lnext;
+        __syncthreads();
+    }
+    bool marked = false;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (!valid[k]) continue;
+        const int i = lane + WL * k;
+        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+        const double v = val[i];
+        if (__double_as_longlong(zcur[k]) != __double_as_longlong(v)) {
+            __hip_atomic_store(&A.z[q.f], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (unsigned b = x[k]; b; b &= b - 1) {
+                P.act_next[tile_of<TH>(neighbour(q.r, q.c, __builtin_ctz(b), nr, nc, bc), nc, ntx)] = 1;
+                marked = true;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) eroded += __shfl_xor(eroded, o);
+    const bool wmarked = __ballot(marked) != 0;
+    if (lane == 0) {
+        A.tile_eroded[tile] = eroded;
+        if (wmarked) *P.marked = 1;
+        P.runs[tile] += 1;
+    }
+}
+
 // smooth_drainage_area (lsdtt_stream_power_cpu.cpp:34-139), row pass: one
 // thread per row keeps the reference's rolling sum (window -r..r, then per
 // column: record, subtract column c - r, add column c + r + 1)
@@ -833,7 +1174,9 @@ __device__ __forceinline__ int wrap_or_none(int i, int size, bool neg, bool pos)
     return (i % size + size) % size;
 }
 
-__global__ void k_smooth_rows(const int* __restrict__ cnt, const uint8_t* __restrict__ mask, int nr, int nc,
+// (cnt: upstream counts times pa = pixel area, or areas in m^2 with pa = 1)
+template <class T>
+__global__ void k_smooth_rows(const T* __restrict__ cnt, const uint8_t* __restrict__ mask, int nr, int nc,
                               double pa, int rad, bool pw, bool pe, double* __restrict__ hsum, int* __restrict__ hcnt)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -982,18 +1325,30 @@ void FluvialTiles::resize(int nr_, int nc_, BC bc)
     tile_run.reserve(ntiles);
 }
 
-void FluvialTiles::smooth_area(const int* cnt, const uint8_t* mask, double pixel_area, int kernel_size,
-                               double* out, hipStream_t s)
+template <class T>
+void FluvialTiles::smooth(const T* cnt, const uint8_t* mask, double pixel_area, int kernel_size, double* out,
+                          hipStream_t s)
 {
     const long long n = (long long)nr * nc;
     hsum.reserve(n);
     hcnt.reserve(n);
     const int rad = kernel_size / 2;
-    k_smooth_rows<<<div_up(nr, 64), 64, 0, s>>>(cnt, mask, nr, nc, pixel_area, rad, bc_.w == 'p', bc_.e == 'p',
-                                                hsum.p, hcnt.p);
+    k_smooth_rows<T><<<div_up(nr, 64), 64, 0, s>>>(cnt, mask, nr, nc, pixel_area, rad, bc_.w == 'p', bc_.e == 'p',
+                                                   hsum.p, hcnt.p);
     LSDTT_LAUNCH_CHECK();
     k_smooth_cols<<<div_up(nc, 64), 64, 0, s>>>(mask, nr, nc, rad, bc_.n == 'p', bc_.s == 'p', hsum.p, hcnt.p, out);
     LSDTT_LAUNCH_CHECK();
+}
+
+void FluvialTiles::smooth_area(const int* cnt, const uint8_t* mask, double pixel_area, int kernel_size,
+                               double* out, hipStream_t s)
+{
+    smooth<int>(cnt, mask, pixel_area, kernel_size, out, s);
+}
+
+void FluvialTiles::smooth_area(const double* area, const uint8_t* mask, int kernel_size, double* out, hipStream_t s)
+{
+    smooth<double>(area, mask, 1.0, kernel_size, out, s);
 }
 
 void FluvialTiles::depths(const int* rcv, const uint8_t* mask, hipStream_t s)
@@ -1111,6 +1466,43 @@ int FluvialTiles::stream_power(const int* rcv, const uint8_t* dir, const uint8_t
                     else if (nm == 1) k_tile_sp<h, false, 1><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
                     else k_tile_sp<h, false, 2><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
                 }
+            });
+            LSDTT_LAUNCH_CHECK();
+        },
+        [&] {
+            k_sum_i32<<<1, kSumThreads, 0, s>>>(tile_eroded.p, ntiles, d_sum);
+            LSDTT_LAUNCH_CHECK();
+        },
+        s);
+    last_sp_passes = passes;
+    std::memcpy(eroded, flags_h.data(), sizeof(long long));
+    return passes;
+}
+
+void FluvialTiles::dinf_up_order(const int* rec, const uint8_t* mask, hipStream_t s)
+{
+    with_tile_height(th, [&](auto H) {
+        k_dinf_tile_up_order<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(rec, mask, nr, nc, bc_, ord.p,
+                                                                               lstart.p, tile_maxdep.p, xdon.p);
+    });
+    LSDTT_LAUNCH_CHECK();
+}
+
+int FluvialTiles::dinf_stream_power(const int* rec, const float* w, const uint8_t* mask, const double* area,
+                                    const double* z_old, const double* K, const SpParams& sp, double* z,
+                                    long long* eroded, hipStream_t s)
+{
+    DinfSpArgs A{rec, w, mask, area, ord.p, lstart.p, tile_maxdep.p, xdon.p, z_old, K, z, tile_eroded.p, sp};
+    long long* d_sum = reinterpret_cast<long long*>(flags.p);
+    last_passes = last_sp_passes;
+    // algorithmic bytes of the sweep, counted once: rec 8, w 8, mask 1,
+    // area 8, z_old 8 (, K 8) in; z 8 read and 8 written
+    const double bytes = (K ? 57.0 : 49.0) * (double)nr * nc;
+    const int passes = settle(
+        "stream_power", bytes,
+        [&](const PassArgs& P) {
+            with_tile_height(th, [&](auto H) {
+                k_dinf_tile_sp<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(A, nr, nc, bc_, P);
             });
             LSDTT_LAUNCH_CHECK();
         },

@@ -63,11 +63,20 @@ public:
     void dinf_order(const int* rec, const uint8_t* mask, hipStream_t s);
     int dinf_accumulate(const int* rec, const float* w, const uint8_t* mask, double cell_area, double* acc,
                         bool warm, hipStream_t s);
+    // D-infinity stream power in place on z (z == z_old on entry; area in
+    // m^2 per cell): dinf_up_order once per graph (shares the buffers of
+    // depths()), then dinf_stream_power; returns passes used
+    void dinf_up_order(const int* rec, const uint8_t* mask, hipStream_t s);
+    int dinf_stream_power(const int* rec, const float* w, const uint8_t* mask, const double* area,
+                          const double* z_old, const double* K, const SpParams& sp, double* z, long long* eroded,
+                          hipStream_t s);
     // smooth_drainage_area of count * pixel_area: rolling row sums, then
     // rolling column sums, in the reference's order (one thread per row,
     // then per column), so the averages are bit-identical
     void smooth_area(const int* cnt, const uint8_t* mask, double pixel_area, int kernel_size, double* out,
                      hipStream_t s);
+    // the same of areas given in m^2
+    void smooth_area(const double* area, const uint8_t* mask, int kernel_size, double* out, hipStream_t s);
 
     int nr = 0, nc = 0, ntx = 0, nty = 0, ntiles = 0, max_passes = 0;
     long long tile_runs = 0;  // tiles processed by the last counts / stream_power
@@ -87,6 +96,8 @@ private:
     // runs `pass(PassArgs)` until a pass marks no tile
     template <class Pass, class After>
     int settle(const char* name, double first_pass_bytes, Pass&& pass, After&& after_batch, hipStream_t s);
+    template <class T>
+    void smooth(const T* v, const uint8_t* mask, double pa, int kernel_size, double* out, hipStream_t s);
 };
 
 }  // namespace lsdtt_gpu

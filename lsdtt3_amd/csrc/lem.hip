@@ -383,6 +383,11 @@ void LemEngine::build_dofs()
     if (n_gen) LSDTT_HIP(hipMemcpy(gen.p, gh.data(), sizeof(int) * gh.size(), hipMemcpyHostToDevice));
     partials.reserve(static_cast<size_t>(grid_for(n, 2048) + hs_general_blocks(n_gen) + 1) * 8);
     c9g.reserve(static_cast<size_t>(9) * (n_gen ? n_gen : 1));
+    if (p.fluvial_dinf || threshold_channels()) {
+        drec.reserve(2 * (size_t)n);
+        dw.reserve(2 * (size_t)n);
+        darea.reserve(n);
+    }
     if (threshold_channels()) {
         // the channel-free sets, from which each step removes its channels
         dof_base.reserve(n);
@@ -393,10 +398,6 @@ void LemEngine::build_dofs()
         gen_base.reserve(gh.size() ? gh.size() : 1);
         if (n_gen) LSDTT_HIP(hipMemcpy(gen_base.p, gen.p, sizeof(int) * gh.size(), hipMemcpyDeviceToDevice));
         chan.reserve(n);
-        drec.reserve(2 * (size_t)n);
-        dw.reserve(2 * (size_t)n);
-        darea.reserve(n);
-        darea_warm = false;
     }
 }
 
@@ -472,8 +473,59 @@ void LemEngine::set_boundary_flats(const long long* flats, long long m)
     if (m) LSDTT_HIP(hipMemcpy(bflats.p, v.data(), sizeof(int) * m, hipMemcpyHostToDevice));
 }
 
+void LemEngine::dinf_stream_power(const SpParams& a, hipStream_t s)
+{
+    // run_lem_step's D-inf branch (lsdtt_lem_cpu.cpp:357-391): the D-inf
+    // graph of the pre-step surface as a float raster, its accumulation,
+    // the smoothing, then the sweep from the outlets up
+    DinfOptions o;
+    o.dx = dx;
+    o.bc = p.bc;
+    timer.begin("dinf_receivers", s);
+    dinf_receivers<double>(z_before.p, mask.p, nr, nc, o, drec.p, dw.p, nullptr, nullptr, s);
+    timer.end(s, 25.0 * n);
+    timer.begin("dinf_order", s);
+    tiles.dinf_order(drec.p, mask.p, s);
+    timer.end(s, 9.0 * n);
+    last_count_passes = tiles.dinf_accumulate(drec.p, dw.p, mask.p, dx * dx, darea.p, darea_warm, s);
+    last_count_tiles = tiles.tile_runs;
+    darea_warm = true;
+    const double* area = darea.p;
+    if (p.area_kernel > 1) {
+        area_sm.reserve(n);
+        tiles.smooth_area(darea.p, mask.p, p.area_kernel, area_sm.p, s);
+        area = area_sm.p;
+    }
+    timer.begin("tile_depths", s);
+    tiles.dinf_up_order(drec.p, mask.p, s);
+    timer.end(s, 9.0 * n);
+    long long er = 0;
+    last_sp_passes = tiles.dinf_stream_power(drec.p, dw.p, mask.p, area, z_before.p, has_K ? K_field.p : nullptr, a,
+                                             z.p, &er, s);
+    last_sp_tiles = tiles.tile_runs;
+    last_eroded = er;
+}
+
 void LemEngine::stream_power(hipStream_t s)
 {
+    if (p.fluvial_dinf) {
+        SpParams a;
+        a.K0 = p.K;
+        a.m = p.m;
+        a.nexp = p.n;
+        a.dt = p.dt;
+        a.dx = dx;
+        a.pixel_area = dx * dx;
+        a.threshold = p.threshold;
+        a.slope_clamp = p.slope_clamp;
+        a.divide = p.divide_area_by_dx;
+        a.n_is_one = std::fabs(p.n - 1.0) < 1e-6;
+        a.m_half = 0;
+        a.n1_half = 0;
+        a.width = p.width_factor;
+        dinf_stream_power(a, s);
+        return;
+    }
     // receivers of the pre-step surface, then counts and stream power by
     // tile passes (fluvial.hip)
     flow.receivers_only<double>(z_before.p, mask.p, s);

@@ -38,6 +38,7 @@ struct LemParams {
     int nonlinear = 0;
     double U = 1e-4, baselevel_fall = 0.0;
     int enable_fluvial = 1, enable_hillslope = 0, enable_uplift = 1;
+    int fluvial_dinf = 0;  // with enable_fluvial: D-infinity incision instead of D8
     BC bc{'o', 'o', 'o', 'o'};
     double newton_abs_tol = 1e-12, newton_rel_tol = 1e-8;
     int newton_max_iterations = 50, newton_max_line_search = 30;
@@ -163,6 +164,7 @@ public:
 
 private:
     void stream_power(hipStream_t s);
+    void dinf_stream_power(const SpParams& a, hipStream_t s);
     int hillslope(LemStats* st, hipStream_t s);
     int newton_solve(double inv_dt, LemStats* st, hipStream_t s);
     int bicgstab(hipStream_t s, int newton_it);

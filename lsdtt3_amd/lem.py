@@ -40,7 +40,7 @@ def make_options(timestep_years=100.0, erodibility=1.0e-3, m_exponent=0.45, n_ex
     extra keywords set fields of the struct by name."""
     if hillslope_model not in ("roering_implicit", "none"):
         raise ValueError(f"hillslope_model {hillslope_model!r} is not built on the GPU path")
-    if fluvial_method not in ("d8", "none"):
+    if fluvial_method not in ("d8", "dinf", "none"):
         raise ValueError(f"fluvial_method {fluvial_method!r} is not built on the GPU path")
     o = default_options()
     o.timestep = float(timestep_years)
@@ -55,6 +55,7 @@ def make_options(timestep_years=100.0, erodibility=1.0e-3, m_exponent=0.45, n_ex
     o.incision_threshold_area = float(incision_threshold_specific_area)
     o.boundary_conditions = boundary_conditions.encode()[:4]
     o.enable_d8_fluvial_erosion = int(fluvial_method == "d8")
+    o.enable_dinf_fluvial_erosion = int(fluvial_method == "dinf")
     o.baselevel_fall_rate = float(baselevel_fall_rate)
     for k, v in solver.items():
         if not hasattr(o, k):

@@ -139,6 +139,10 @@ typedef struct {
     double width_factor;       /* w / dx when multiply_stream_power_term_by_w_over_dx, else 1 */
 } orc_sp_params;
 
+int64_t orc_stream_power_dinf(int64_t n, const uint8_t* valid, const int64_t* rec, const float* w,
+                              const uint8_t* nrec, const int64_t* topo, int64_t n_topo, const double* area,
+                              const double* K_field, const orc_sp_params* p, const double* z_old, double* z);
+
 /* smooth_drainage_area (src/cpu/lsdtt_stream_power_cpu.cpp:34-139): odd box
  * moving average of area over valid cells, rolling row sums then rolling
  * column sums in the reference's order; periodic axes wrap.  Returns
@@ -216,6 +220,7 @@ typedef struct {
     int    force_picard;           /* hillslope_force_picard */
     orc_picard_params picard;      /* the fallback's settings (D, Sc, nonlinear taken from hs) */
     int    regime;                 /* hillslope coupling: 0 automatic, 1 coupled, 2 channels as boundaries */
+    int    fluvial_dinf;           /* with enable_fluvial: D-infinity incision instead of D8 */
 } orc_lem_params;
 
 typedef struct {

@@ -95,6 +95,9 @@ def lib():
         L.orc_dinf_channel_mask.argtypes = [C.c_int, C.c_int, C.c_double, f32p, u8p, C.c_char_p, C.c_double,
                                             C.c_int, C.c_int, u8p, C.c_void_p]
         L.orc_dinf_channel_mask.restype = C.c_int
+        L.orc_stream_power_dinf.argtypes = [C.c_int64, u8p, i64p, f32p, u8p, i64p, C.c_int64, f64p, C.c_void_p,
+                                            C.POINTER(SpParams), f64p, f64p]
+        L.orc_stream_power_dinf.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -295,7 +298,8 @@ class LemParams(C.Structure):
                 ("enable_fluvial", C.c_int), ("enable_hillslope", C.c_int),
                 ("enable_uplift", C.c_int), ("bc", C.c_char * 4), ("hs", HsParams),
                 ("area_kernel", C.c_int), ("width_factor", C.c_double), ("channel", C.c_void_p),
-                ("force_picard", C.c_int), ("picard", PicardParams), ("regime", C.c_int)]
+                ("force_picard", C.c_int), ("picard", PicardParams), ("regime", C.c_int),
+                ("fluvial_dinf", C.c_int)]
 
 
 class LemStats(C.Structure):
@@ -334,6 +338,22 @@ def stream_power(z_old, z, mask, receiver, dirs, topo, area, K, m, n, dt, dx, th
     return zz.reshape(np.shape(z)), e
 
 
+def stream_power_dinf(z_old, z, mask, graph, area, K, m, n, dt, dx, threshold=0.0, divide_area_by_dx=True,
+                      slope_clamp=1e-5, K_field=None, width_factor=1.0):
+    """orc_stream_power_dinf on a copy of z (graph from dinf_graph); returns
+    (z_new, cells whose solve ran)"""
+    zz = np.ascontiguousarray(z, dtype=np.float64).ravel().copy()
+    zo = np.ascontiguousarray(z_old, dtype=np.float64).ravel()
+    m8 = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+    _, topo = dinf_accumulate(graph, m8, dx)
+    p = SpParams(K, m, n, dt, dx, threshold, int(divide_area_by_dx), slope_clamp, float(width_factor))
+    kf = None if K_field is None else np.ascontiguousarray(K_field, dtype=np.float64).ravel()
+    e = lib().orc_stream_power_dinf(zz.size, m8, graph["rec"].ravel(), graph["w"].ravel(), graph["nrec"], topo,
+                                    len(topo), np.ascontiguousarray(area, np.float64).ravel(),
+                                    None if kf is None else kf.ctypes.data_as(C.c_void_p), C.byref(p), zo, zz)
+    return zz.reshape(np.shape(z)), e
+
+
 def hillslope_newton(z, zn, mask, dx, bc, params, inv_dt):
     """one backward-Euler Newton solve; returns (z_new, HsResult)"""
     zz = np.ascontiguousarray(z, dtype=np.float64).copy()
@@ -366,7 +386,8 @@ def boundary_flats(mask, bc):
 def lem_params(dt, K, m, n, U, D, Sc=0.8, nonlinear=False, bc="oooo", threshold=0.0,
                slope_clamp=1e-5, baselevel_fall=0.0, fluvial=True, hillslope=True, uplift=True,
                rel_tol=1e-8, abs_tol=1e-12, area_kernel=1, width_factor=1.0, channel=None,
-               divide_area_by_dx=True, force_picard=False, picard=None, max_iterations=50, regime=0):
+               divide_area_by_dx=True, force_picard=False, picard=None, max_iterations=50, regime=0,
+               fluvial_dinf=False):
     """channel: optional uint8 Dirichlet channel mask for the hillslope (kept
     alive on the returned structure).  regime: 0 automatic (channels as
     boundaries when a threshold or a mask is given), 1 coupled, 2 channels
@@ -380,6 +401,7 @@ def lem_params(dt, K, m, n, U, D, Sc=0.8, nonlinear=False, bc="oooo", threshold=
                   picard if picard is not None else picard_params(D, Sc, nonlinear))
     p.hs.max_iterations = int(max_iterations)
     p.regime = int(regime)
+    p.fluvial_dinf = int(fluvial_dinf)
     p._channel_ref = ch
     return p
 

@@ -260,3 +260,100 @@ int orc_dinf_channel_mask(int nr, int nc, double dx, const float* z, const uint8
     free(rec); free(w); free(nrec); free(valid); free(topo); free(area);
     return st;
 }
+
+/* compute_dinf_stream_power_incision (lsdtt_stream_power_cpu.cpp:329-602)
+ * in FP64 state: receivers first (reverse Kahn order: every receiver
+ * precedes its donors, as in the reference's reverse levels
+ * (lsdtt_flowgraph_cpu.cpp:1176-1223), and each cell reads only its
+ * receivers, so the result is the same).  Skips outlets (no receivers),
+ * invalid cells and cells below the area threshold (on area/dx when
+ * divide_area_by_dx).  Receiver classes (:435-441): one receiver, or w0 >=
+ * 1 - 1e-6 (float): "cardinal" (rec[0], dx); w1 >= 1 - 1e-6: "diagonal"
+ * (rec[1], dx sqrt2); else two receivers: Newton on
+ * sbt = ((z_e1 - z)/dx)^2 + ((z_e2 - z_e1)/dx)^2 (:459-493), then the rise
+ * guard and the clamp above the lower receiver.  pow(x, 2) is written x*x
+ * (the same correctly rounded value).  Returns the cells whose solve ran
+ * (two receivers, or one lower receiver). */
+int64_t orc_stream_power_dinf(int64_t n, const uint8_t* valid, const int64_t* rec, const float* w,
+                              const uint8_t* nrec, const int64_t* topo, int64_t n_topo, const double* area,
+                              const double* K_field, const orc_sp_params* p, const double* z_old, double* z)
+{
+    const int n_is_one = fabs(p->n - 1.0) < 1e-6;
+    const double dxc = p->dx, dxd = p->dx * 1.4142135623730950488;
+    const double tol = p->slope_clamp * dxc;
+    int64_t eroded = 0;
+    for (int64_t ti = n_topo - 1; ti >= 0; --ti) {
+        const int64_t f = topo[ti];
+        const int k = nrec[f];
+        if (!valid[f] || k == 0) continue;
+        double a = area[f];
+        if (p->divide_area_by_dx) a /= dxc;
+        if (p->threshold > 0.0 && a < p->threshold) continue;
+        const double zo = z_old[f];
+        double zn = zo;
+        const float w0 = w[2 * f], w1 = k > 1 ? w[2 * f + 1] : 0.0f;
+        const int only_card = k == 1 || w0 >= 1.0f - 1e-6f;
+        const int only_diag = k > 1 && w1 >= 1.0f - 1e-6f;
+        const int two = k > 1 && !only_card && !only_diag;
+        /* recv_elev (:443-448): a missing receiver reads the cell's own z_old */
+        const int64_t g0 = rec[2 * f], g1 = k > 1 ? rec[2 * f + 1] : -1;
+        const double ze1 = (g0 >= 0 && g0 < n) ? z[g0] : zo;
+        const double ze2 = (g1 >= 0 && g1 < n) ? z[g1] : zo;
+        const double Ki = K_field ? K_field[f] : p->K;
+        const double kam = Ki * p->width_factor * pow(a, p->m);
+        if (two) {
+            double eps = tol + 1.0;
+            int it = 0;
+            do {
+                const double t1 = (ze1 - zn) / dxc, t2 = (ze2 - ze1) / dxc;
+                const double sbt = t1 * t1 + t2 * t2;
+                if (sbt <= 0.0) {
+                    eps = 0.0;
+                } else {
+                    const double fz = zn - zo + kam * pow(sbt, 0.5 * p->n) * p->dt;
+                    const double fd = 1.0 - kam * p->n * (ze1 - zn) / (dxc * dxc) * pow(sbt, 0.5 * p->n - 1.0) * p->dt;
+                    eps = fz / fd;
+                }
+                zn -= eps;
+                ++it;
+                if (it > 100) eps = tol / 2.0;
+            } while (fabs(eps) > tol);
+            if ((zn - zo) > tol || isnan(zn)) zn = zo;
+            const double zmin = ze1 < ze2 ? ze1 : ze2;
+            if (zn < zmin) zn = zmin + p->slope_clamp * dxc;
+            ++eroded;
+        } else {
+            const double zr = only_diag ? ze2 : ze1;
+            const double dxe = only_diag ? dxd : dxc;
+            if (zr >= zo) {
+                zn = zo;
+            } else if (n_is_one) {
+                const double spf = kam * (p->dt / dxe);
+                zn = (zo + zr * spf) / (1.0 + spf);
+                if ((zn - zo) > tol || isnan(zn)) zn = zo;
+                ++eroded;
+            } else {
+                double eps = tol + 1.0;
+                int it = 0;
+                do {
+                    const double slope = (zn - zr) / dxe;
+                    if (slope <= 0.0) {
+                        eps = 0.0;
+                    } else {
+                        const double fz = zn - zo + kam * p->dt * pow(slope, p->n);
+                        const double fd = 1.0 + kam * p->dt * (p->n / dxe) * pow(slope, p->n - 1.0);
+                        eps = fz / fd;
+                    }
+                    zn -= eps;
+                    ++it;
+                    if (it > 100) eps = tol / 2.0;
+                } while (fabs(eps) > tol);
+                if ((zn - zo) > tol || isnan(zn)) zn = zo;
+                if (zn < zr) zn = zr + p->slope_clamp * dxe;
+                ++eroded;
+            }
+        }
+        z[f] = zn;
+    }
+    return eroded;
+}

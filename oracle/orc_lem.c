@@ -633,7 +633,34 @@ int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
     for (int64_t i = 0; i < n_bf; ++i) if (boundary_flats[i] < n) bel[i] = z[boundary_flats[i]];
     int status = ORC_OK;
 
-    if (p->enable_fluvial) {
+    if (p->enable_fluvial && p->fluvial_dinf) {
+        /* run_lem_step's D-inf branch (lsdtt_lem_cpu.cpp:357-391): the
+         * D-inf graph of the input surface (a float raster in the reference),
+         * its accumulation, the smoothing, then the incision */
+        float* zf = (float*)malloc(sizeof(float) * (size_t)n);
+        int64_t* rec = (int64_t*)malloc(sizeof(int64_t) * (size_t)(2 * n));
+        float* w = (float*)malloc(sizeof(float) * (size_t)(2 * n));
+        uint8_t* nrec = (uint8_t*)malloc((size_t)n);
+        int64_t* topo = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+        double* area = (double*)malloc(sizeof(double) * (size_t)n);
+        for (int64_t i = 0; i < n; ++i) zf[i] = (float)z_before[i];
+        if (orc_dinf_graph(nr, nc, dx, zf, mask, NAN, p->bc, 1, 0, rec, w, nrec, NULL, NULL) != ORC_OK) {
+            status = ORC_ERR_BC;
+        } else {
+            const int64_t nt = orc_dinf_topo(n, mask, rec, w, nrec, topo);
+            orc_dinf_accumulate(n, rec, w, nrec, topo, nt, dx, area);
+            if (p->area_kernel > 1) {
+                double* sm = (double*)malloc(sizeof(double) * (size_t)n);
+                if (orc_smooth_area(nr, nc, area, mask, p->area_kernel, p->bc, sm) != ORC_OK) status = ORC_ERR_ARG;
+                memcpy(area, sm, sizeof(double) * (size_t)n);
+                free(sm);
+            }
+            orc_sp_params sp = {p->K, p->m, p->n, p->dt, dx, p->threshold,
+                                p->divide_area_by_dx, p->slope_clamp, p->width_factor};
+            st->cells_eroded = orc_stream_power_dinf(n, mask, rec, w, nrec, topo, nt, area, K_field, &sp, z_before, z);
+        }
+        free(zf); free(rec); free(w); free(nrec); free(topo); free(area);
+    } else if (p->enable_fluvial) {
         int64_t* rcv = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
         uint8_t* dir = (uint8_t*)malloc((size_t)n);
         uint8_t* don = (uint8_t*)malloc((size_t)n);

@@ -135,3 +135,62 @@ def test_gpu_lem_derived_channels_are_dirichlet(gpu):
     b.step()
     assert a.channel_mask.sum() == 0
     assert np.array_equal(a.elevation, b.elevation)
+
+
+# ---- D-infinity incision in the LEM step ---------------------------------
+
+@pytest.mark.parametrize("nexp,bc,kernel,kfield", [(1.0, "nnon", 1, False), (1.5, "oooo", 1, True),
+                                                   (1.0, "popo", 3, False), (1.3, "nobn", 5, False)])
+def test_gpu_dinf_fluvial_matches_oracle(gpu, nexp, bc, kernel, kfield):
+    """fluvial only: |dz| <= 1e-9 m (the device and host pow may differ by
+    an ulp; the area sums differ by rounding, see above)"""
+    print(f"[gpu dinf] fluvial n={nexp} {bc} k={kernel}", flush=True)
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(130, 150, dx=30.0, seed=9, max_slope=0.6)
+    mask = np.ones(z.shape, np.uint8)
+    K = np.random.default_rng(2).uniform(1e-6, 1e-4, z.shape) if kfield else None
+    kw = dict(timestep_years=1000.0, erodibility=1e-5, m_exponent=0.5, n_exponent=nexp, uplift_rate=1e-3,
+              hillslope_model="none", boundary_conditions=bc, fluvial_method="dinf",
+              drainage_area_kernel_size=kernel)
+    lem = gpu.LandscapeModel(z, 30.0, mask=mask, erodibility_field=K, **kw)
+    p = orc.lem_params(1000.0, 1e-5, 0.5, nexp, 1e-3, 0.0, bc=bc, hillslope=False, fluvial_dinf=True,
+                       area_kernel=kernel)
+    zo = z.copy()
+    for step in range(3):
+        sg = lem.step()
+        zo, so, rc = orc.lem_step(zo, mask, 30.0, p, K_field=K)
+        assert rc == 0
+        assert np.abs(lem.elevation - zo).max() <= 1e-9, step
+        assert sg["cells_eroded"] == so["cells_eroded"], step
+
+
+def test_gpu_dinf_lem_full_step_matches_oracle(gpu):
+    """D-inf incision, Roering hillslope with threshold-derived channels"""
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(96, 112, dx=30.0, seed=43, max_slope=0.6)
+    mask = np.ones(z.shape, np.uint8)
+    lem = gpu.LandscapeModel(z, 30.0, **roering("nnon", fluvial_method="dinf"))
+    p = orc.lem_params(500.0, 2e-5, 0.5, 1.0, 5e-4, 0.02, Sc=0.8, nonlinear=True, bc="nnon", threshold=600.0,
+                       fluvial_dinf=True)
+    zo = z.copy()
+    for step in range(2):
+        st = lem.step()
+        zo, so, rc = orc.lem_step(zo, mask, 30.0, p)
+        assert rc == 0
+        tol = max(4 * 500.0 * max(so["hillslope_residual"], 1e-12), 1e-8)
+        assert np.abs(lem.elevation - zo).max() <= max(tol, 1e-6), step
+        assert st["cells_eroded"] == so["cells_eroded"]
+
+
+def test_gpu_dinf_fluvial_independent_of_tile_height(gpu):
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(97, 131, dx=30.0, seed=12, max_slope=0.6)
+    out = []
+    for th in (8, 16, 32):
+        lem = gpu.LandscapeModel(z, 30.0, **roering("popo", fluvial_method="dinf", n_exponent=1.5))
+        lem.tile_height = th
+        lem.run(2)
+        out.append((lem.elevation, lem.channel_mask))
+    for zt, ct in out[1:]:
+        assert np.array_equal(zt, out[0][0])
+        assert np.array_equal(ct, out[0][1])

@@ -194,3 +194,79 @@ def test_lem_derived_channels_are_dirichlet():
     zc, _, _ = orc.lem_step(z, mask, 10.0, pc)
     z0, _, _ = orc.lem_step(z, mask, 10.0, p0)
     assert np.array_equal(zc, z0)
+
+
+# ---- D-infinity stream power (lsdtt_stream_power_cpu.cpp:329-602) --------
+
+def east_plane(nr=12, nc=30, dx=10.0):
+    c = np.arange(nc, dtype=np.float64)
+    return np.repeat((200.0 - 0.5 * dx * 0.1 * c)[None, :], nr, axis=0)
+
+
+@pytest.mark.parametrize("nexp", [1.0, 1.5])
+def test_dinf_incision_equals_d8_on_a_cardinal_plane(nexp):
+    """every cell's single D-inf receiver is its D8 receiver (east, dx): the
+    two incision laws are the same computation"""
+    z = east_plane()
+    mask = np.ones(z.shape, np.uint8)
+    kw = dict(hillslope=False, uplift=False, bc="nonn")
+    pd = orc.lem_params(1000.0, 1e-4, 0.5, nexp, 0.0, 0.0, fluvial_dinf=True, **kw)
+    p8 = orc.lem_params(1000.0, 1e-4, 0.5, nexp, 0.0, 0.0, **kw)
+    zd, sd, rd = orc.lem_step(z, mask, 10.0, pd)
+    z8, s8, r8 = orc.lem_step(z, mask, 10.0, p8)
+    assert rd == 0 and r8 == 0
+    assert np.array_equal(zd, z8)
+    assert sd["cells_eroded"] == s8["cells_eroded"] > 0
+    assert (zd < z).any()
+
+
+def test_dinf_single_diagonal_receiver_uses_the_cardinal_length():
+    """a cell whose only lower neighbour is diagonal has one receiver, which
+    the reference classes as 'cardinal' (one receiver): the n = 1 closed
+    form runs with dx, not dx sqrt2 (lsdtt_stream_power_cpu.cpp:435-441,
+    505-528)"""
+    z = np.full((5, 5), 11.0)
+    z[2, 2] = 10.0
+    z[1, 3] = 9.0   # NE of the centre
+    z[0, 4] = 8.0   # NE of that: the diagonal cell drains on to the corner
+    mask = np.ones(z.shape, np.uint8)
+    g = orc.dinf_graph(z.astype(np.float32), mask, 10.0, bc="nnnn")
+    f = 2 * 5 + 2
+    assert g["nrec"][f] == 1 and g["rec"][f][0] == 1 * 5 + 3
+    area, _ = orc.dinf_accumulate(g, mask, 10.0)
+    K, m, dt, dx = 1e-3, 0.5, 100.0, 10.0
+    zn, _ = orc.stream_power_dinf(z, z, mask, g, area, K, m, 1.0, dt, dx)
+    a = area[f] / dx
+    spf = K * a ** m * (dt / dx)
+    zr = zn.ravel()[1 * 5 + 3]
+    expect = (10.0 + zr * spf) / (1.0 + spf)
+    assert abs(zn[2, 2] - expect) <= 1e-12 * abs(expect)
+
+
+def test_dinf_two_receiver_newton_root():
+    """cells with two receivers end at a root of
+    z - z_old + K A^m dt sbt^(n/2) = 0 (to the Newton stopping rule), unless
+    the rise guard or the clamp fired"""
+    rng = np.random.default_rng(4)
+    r, c = np.mgrid[0:40, 0:40]
+    z = 300.0 - 0.8 * r - 0.45 * c + rng.normal(0, 0.05, (40, 40))
+    mask = np.ones(z.shape, np.uint8)
+    dx, K, m, nexp, dt = 10.0, 2e-4, 0.5, 1.0, 200.0
+    g = orc.dinf_graph(z.astype(np.float32), mask, dx, bc="oooo")
+    area, _ = orc.dinf_accumulate(g, mask, dx)
+    zn, _ = orc.stream_power_dinf(z, z, mask, g, area, K, m, nexp, dt, dx)
+    zf, zo = zn.ravel(), z.ravel()
+    checked = 0
+    for f in range(z.size):
+        if g["nrec"][f] != 2 or g["w"][f][0] >= 1 - 1e-6 or g["w"][f][1] >= 1 - 1e-6:
+            continue
+        e1, e2 = zf[g["rec"][f][0]], zf[g["rec"][f][1]]
+        if zf[f] <= min(e1, e2) + 1e-5 * dx * 1.0001 or zf[f] == zo[f]:
+            continue  # clamp or guard
+        sbt = ((e1 - zf[f]) / dx) ** 2 + ((e2 - e1) / dx) ** 2
+        kam = K * (area[f] / dx) ** m
+        fz = zf[f] - zo[f] + kam * sbt ** (0.5 * nexp) * dt
+        fd = 1.0 - kam * nexp * (e1 - zf[f]) / dx ** 2 * sbt ** (0.5 * nexp - 1.0) * dt
+        assert abs(fz / fd) <= 1e-5 * dx
+        checked += 1
+    assert checked > 100
