@@ -65,10 +65,15 @@ def workload(size, seed=2024):
     return z, K, lem_surface.last_passes
 
 
+# variant of the step (the metric's workload is the default: D8 incision,
+# no channel threshold); set from the command line
+STEP_VARIANT = {"fluvial_method": "d8", "incision_threshold_specific_area": 0.0}
+
+
 def lem_kwargs(K):
     return dict(timestep_years=500.0, erodibility=K, m_exponent=0.5, n_exponent=1.5, uplift_rate=5e-4,
                 hillslope_diffusivity=0.005, nonlinear_hillslopes=True, critical_hillslope_slope=0.8,
-                boundary_conditions="nnon", newton_rel_tol=1e-8)
+                boundary_conditions="nnon", newton_rel_tol=1e-8, **STEP_VARIANT)
 
 
 def cpu_baseline(K, budget_s=15.0, size=512):
@@ -497,6 +502,11 @@ def main():
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--sequential", action="store_true", help="ensemble: also time one instance at a time")
+    ap.add_argument("--fluvial", choices=["d8", "dinf"], default="d8",
+                    help="lem4096: D8 (the metric's workload) or D-infinity incision")
+    ap.add_argument("--channel-threshold", type=float, default=0.0,
+                    help="lem4096: incision_threshold_area (m, on area/dx); > 0 derives the hillslope's "
+                         "channel set from the D-infinity area each step")
     ap.add_argument("--autotune-steps", type=int, default=2,
                     help="timed steps per tile height when autotuning the routing tiles (0: default height)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
@@ -526,6 +536,10 @@ def main():
     from lsdtt3_amd import _native as N
     sync = N.synchronize
 
+    STEP_VARIANT["fluvial_method"] = args.fluvial
+    STEP_VARIANT["incision_threshold_specific_area"] = args.channel_threshold
+    variant = ("" if args.fluvial == "d8" else ", D-infinity incision") + \
+        (f", channels from D-inf area/dx >= {args.channel_threshold:g} m" if args.channel_threshold > 0 else "")
     z, K, passes = workload(args.size)
     lem = lsdtt3_amd.LandscapeModel(z, 30.0, **lem_kwargs(K))
     for _ in range(args.warmup):
@@ -560,7 +574,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain GPU LEM {args.size}x{args.size} dx=30 FP64 state, Roering "
                                f"Newton (D=0.005 Sc=0.8, r<=1e-8) + stream power n=1.5 m=0.5 "
-                               f"K={K:.3e}, U=5e-4 dt=500 bc=nnon",
+                               f"K={K:.3e}, U=5e-4 dt=500 bc=nnon{variant}",
                    "n_valid": n_valid, "smoothing_passes": passes,
                    "newton_iters_per_step": round(newton / args.steps, 2),
                    "krylov_iters_per_step": round(krylov / args.steps, 2),

@@ -716,6 +716,36 @@ struct DinfTileArgs {
     double* acc;
 };
 
+// ring slot of a tile cell (as in k_tile_order: 0..31 top row, 32..63
+// bottom row, 64..95 left column, 96..127 right column), -1 inside
+__device__ __forceinline__ int ring_slot(int lrow, int lcol, int h, int w)
+{
+    if (lrow == 0) return lcol;
+    if (lrow == h - 1) return 32 + lcol;
+    if (lcol == 0) return 64 + lrow;
+    if (lcol == w - 1) return 96 + lrow;
+    return -1;
+}
+
+// (lr, lc) of ring slot sl and whether the slot is a cell of the tile
+__device__ __forceinline__ bool slot_cell(int sl, int h, int w, int* lr, int* lc)
+{
+    if (sl < 32) {
+        *lr = 0, *lc = sl;
+        return sl < w;
+    }
+    if (sl < 64) {
+        *lr = h - 1, *lc = sl - 32;
+        return *lc < w && h > 1;
+    }
+    if (sl < 96) {
+        *lr = sl - 64, *lc = 0;
+        return *lr >= 1 && *lr <= h - 2;
+    }
+    *lr = sl - 96, *lc = w - 1;
+    return *lr >= 1 && *lr <= h - 2 && w > 1;
+}
+
 template <int TH>
 __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, int nc, BC bc, PassArgs P)
 {
@@ -731,7 +761,14 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, 
     __shared__ short lr[2][TN];  // local receivers (-1: none or in another tile)
     __shared__ short ord[TN];
     __shared__ short ls[TN + 1];
+    // ring cells: per direction the neighbour's local index (-1: none, the
+    // cell itself, a repeat of an earlier direction, or another tile) and
+    // the contributions of donors in other tiles, loaded before the sweep
+    __shared__ double xv[128][8];
+    __shared__ short nbl[128][8];
+    __shared__ uint8_t xbits[128];
     const int r0 = ty * TH, c0 = tx * TW;
+    const int h = min(TH, nr - r0), wd = min(TW, nc - c0);
     int g[PER][2];
     double old[PER];
     bool valid[PER];
@@ -753,15 +790,61 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, 
         ord[i] = A.ord[(size_t)tile * TN + i];
     }
     const int maxdep = load_levels<TH>(A.lstart, A.tile_maxdep, tile, ls);
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        const int sl = lane + WL * half;
+        int lrw, lcl;
+        const bool ok = slot_cell(sl, h, wd, &lrw, &lcl);
+        const int r = r0 + lrw, c = c0 + lcl, v = ok ? r * nc + c : 0;
+        int gs[8];
+        bool ext[8];
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            int u = -1;
+            if (ok) {
+                const Step st = resolve_step(r, c, d, nr, nc, bc);
+                if (st.kind == 1) u = st.r * nc + st.c;
+            }
+            bool dup = u == v;
+#pragma unroll
+            for (int e = 0; e < d; ++e) dup |= gs[e] == u;
+            if (dup) u = -1;
+            gs[d] = u;
+            const int lu = u >= 0 ? local_index<TH>(u, nc, r0, c0) : -1;
+            ext[d] = u >= 0 && lu < 0;
+            nbl[sl][d] = static_cast<short>(lu);
+        }
+        int h0[8], h1[8];
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            const int u = ext[d] ? gs[d] : 0;
+            h0[d] = ext[d] ? A.rec[2 * (size_t)u] : -1;
+            h1[d] = ext[d] ? A.rec[2 * (size_t)u + 1] : -1;
+        }
+        unsigned bits = 0;
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            const bool don = ext[d] && (h0[d] == v || h1[d] == v);
+            double x = 0.0;
+            if (don) {
+                const int u = gs[d];
+                const double au = __hip_atomic_load(&A.acc[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                x = au * static_cast<double>(A.w[2 * (size_t)u + (h0[d] == v ? 0 : 1)]);
+                bits |= 1u << d;
+            }
+            xv[sl][d] = x;
+        }
+        xbits[sl] = static_cast<uint8_t>(bits);
+    }
     __syncthreads();
-    const int h = min(TH, nr - r0), wd = min(TW, nc - c0);
     for (int L = 0; L <= maxdep; ++L) {
         const int b = ls[L], e = ls[L + 1];
         for (int p = b + lane; p < e; p += WL) {
             const int i = ord[p];
             const int lrow = i / TW, lcol = i - lrow * TW;
             double a = A.cell_area;
-            if (lrow >= 1 && lrow <= h - 2 && lcol >= 1 && lcol <= wd - 2) {
+            const int sl = ring_slot(lrow, lcol, h, wd);
+            if (sl < 0) {
                 // all eight neighbours are distinct cells of this tile
 #pragma unroll
                 for (int d = 0; d < 8; ++d) {
@@ -770,26 +853,16 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, 
                     else if (lr[1][u] == i) a += val[u] * static_cast<double>(wl[1][u]);
                 }
             } else {
-                const int r = r0 + lrow, c = c0 + lcol, v = r * nc + c;
-                int seen[8];
-                int ns = 0;
+                const unsigned xb = xbits[sl];
+#pragma unroll
                 for (int d = 0; d < 8; ++d) {
-                    const Step st = resolve_step(r, c, d, nr, nc, bc);
-                    if (st.kind != 1) continue;
-                    const int u = st.r * nc + st.c;
-                    bool dup = u == v;
-                    for (int j = 0; j < ns; ++j) dup |= seen[j] == u;
-                    if (dup) continue;
-                    seen[ns++] = u;
-                    const int lu = local_index<TH>(u, nc, r0, c0);
-                    if (lu >= 0) {
-                        if (lr[0][lu] == i) a += val[lu] * static_cast<double>(wl[0][lu]);
-                        else if (lr[1][lu] == i) a += val[lu] * static_cast<double>(wl[1][lu]);
+                    if ((xb >> d) & 1u) {
+                        a += xv[sl][d];
                     } else {
-                        const int h0 = A.rec[2 * (size_t)u], h1 = A.rec[2 * (size_t)u + 1];
-                        if (h0 == v || h1 == v) {
-                            const double au = __hip_atomic_load(&A.acc[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            a += au * static_cast<double>(A.w[2 * (size_t)u + (h0 == v ? 0 : 1)]);
+                        const int lu = nbl[sl][d];
+                        if (lu >= 0) {
+                            if (lr[0][lu] == i) a += val[lu] * static_cast<double>(wl[0][lu]);
+                            else if (lr[1][lu] == i) a += val[lu] * static_cast<double>(wl[1][lu]);
                         }
                     }
                 }
@@ -1016,7 +1089,9 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
     __shared__ short lr[2][TN]; // local receivers, -1 outside the tile or none
     __shared__ uint8_t kind[TN];
     __shared__ short ord[TN];
+    __shared__ double zx[128][2];  // ring cells: receivers in other tiles, loaded before the sweep
     const int r0 = ty * TH, c0 = tx * TW;
+    const int h = min(TH, nr - r0), wd = min(TW, nc - c0);
     const SpParams& p = A.p;
     bool valid[PER];
     uint8_t x[PER];
@@ -1044,7 +1119,8 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
                 const bool only_card = nrec == 1 || w0 >= 1.0f - 1e-6f;
                 const bool only_diag = nrec > 1 && w1 >= 1.0f - 1e-6f;
                 kd = only_card ? kDinfCard : (only_diag ? kDinfDiag : kDinfTwo);
-                km = Kf * p.width * pow(area, p.m);
+                // pow(area, 0.5) is sqrt(area): both correctly rounded
+                km = Kf * p.width * (p.m_half ? sqrt(area) : pow(area, p.m));
             }
         }
         val[i] = zo;
@@ -1055,6 +1131,18 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
         lr[0][i] = static_cast<short>((valid[k] && g0 >= 0) ? local_index<TH>(g0, nc, r0, c0) : -1);
         lr[1][i] = static_cast<short>((valid[k] && g1 >= 0) ? local_index<TH>(g1, nc, r0, c0) : -1);
         ord[i] = A.ord[(size_t)tile * TN + i];
+    }
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        const int sl = lane + WL * half;
+        int lrw, lcl;
+        const bool ok = slot_cell(sl, h, wd, &lrw, &lcl);
+        const int f = ok ? (r0 + lrw) * nc + c0 + lcl : 0;
+        const int g0 = A.rec[2 * (size_t)f], g1 = A.rec[2 * (size_t)f + 1];
+        const bool x0 = ok && g0 >= 0 && local_index<TH>(g0, nc, r0, c0) < 0;
+        const bool x1 = ok && g1 >= 0 && local_index<TH>(g1, nc, r0, c0) < 0;
+        zx[sl][0] = x0 ? __hip_atomic_load(&A.z[g0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        zx[sl][1] = x1 ? __hip_atomic_load(&A.z[g1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
     }
     const short* __restrict__ lsg = A.lstart + (size_t)tile * (TN + 1);
     const int maxdep = A.tile_maxdep[tile];
@@ -1071,12 +1159,11 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
             const double zo = val[i];
             // recv_elev (:443-448): the receivers' current elevations
             double ze[2];
+            const int sl = ring_slot(i / TW, i % TW, h, wd);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 const int l = lr[j][i], g = gr[j][i];
-                ze[j] = l >= 0 ? val[l]
-                               : (g >= 0 ? __hip_atomic_load(&A.z[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                         : zo);
+                ze[j] = l >= 0 ? val[l] : ((g >= 0 && sl >= 0) ? zx[sl][j] : zo);
             }
             const double km = kam[i];
             double zn = zo;
@@ -1090,9 +1177,12 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
                     if (sbt <= 0.0) {
                         eps = 0.0;
                     } else {
-                        const double fz = zn - zo + km * pow(sbt, 0.5 * p.nexp) * p.dt;
-                        const double fd =
-                            1.0 - km * p.nexp * (ze[0] - zn) / (dxc * dxc) * pow(sbt, 0.5 * p.nexp - 1.0) * p.dt;
+                        // one pow per iteration: sbt^(n/2 - 1) = sbt^(n/2) / sbt
+                        // (the reference evaluates both powers; they agree to
+                        // an ulp or two)
+                        const double pw = pow(sbt, 0.5 * p.nexp);
+                        const double fz = zn - zo + km * pw * p.dt;
+                        const double fd = 1.0 - km * p.nexp * (ze[0] - zn) / (dxc * dxc) * (pw / sbt) * p.dt;
                         eps = fz / fd;
                     }
                     zn -= eps;
@@ -1120,8 +1210,10 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
                             if (slope <= 0.0) {
                                 eps = 0.0;
                             } else {
-                                const double fz = zn - zo + km * p.dt * pow(slope, p.nexp);
-                                const double fd = 1.0 + km * p.dt * (p.nexp / dxe) * pow(slope, p.nexp - 1.0);
+                                // S^n = S^(n-1) * S: one pow per iteration
+                                const double pn1 = pow(slope, p.nexp - 1.0);
+                                const double fz = zn - zo + km * p.dt * (pn1 * slope);
+                                const double fd = 1.0 + km * p.dt * (p.nexp / dxe) * pn1;
                                 eps = fz / fd;
                             }
                             zn -= eps;

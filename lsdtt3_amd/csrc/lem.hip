@@ -520,7 +520,7 @@ void LemEngine::stream_power(hipStream_t s)
         a.slope_clamp = p.slope_clamp;
         a.divide = p.divide_area_by_dx;
         a.n_is_one = std::fabs(p.n - 1.0) < 1e-6;
-        a.m_half = 0;
+        a.m_half = p.m == 0.5;
         a.n1_half = 0;
         a.width = p.width_factor;
         dinf_stream_power(a, s);
