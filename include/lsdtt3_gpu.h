@@ -309,6 +309,74 @@ int lsdtt_gpu_lem_autotune_tiles(lsdtt_gpu_lem* lem, const int64_t* boundary_fla
                                  int32_t steps, int32_t* tile_height, double* ms_per_step);
 
 /* ---------------------------------------------------------------------
+ * LEM timestep control (src/cpu/lsdtt_lem_timestep_cpu.hpp:62-292,
+ * lsdtt_lem_timestep_cpu.cpp:28-302).  The median interior gradient (valid
+ * cells whose four cardinal neighbours are valid; sqrt(gx^2 + gy^2) of
+ * central differences; the mean of the two middle values for an even count)
+ * is computed on the device from the LEM context's resident surface, exactly
+ * (radix selection of the FP64 values).  The controller and the monitor
+ * are the reference's host-side decisions.
+ * ------------------------------------------------------------------- */
+typedef struct {
+    double target_ratio;            /* 0.7 */
+    double dt_min_yr, dt_max_yr;    /* 0: dt_initial / 100, dt_initial * 100 */
+    double grow_factor;             /* 1.15 */
+    double shrink_factor;           /* 0.5 */
+    int32_t slope_refresh_interval; /* 25 */
+    double ladder_ratio;            /* 2 (1: continuous dt) */
+    double ratio_smoothing;         /* 0.15 */
+} lsdtt_gpu_timestep_options;
+
+typedef struct {
+    double reference_slope, cell_relief_m, uplift_per_step_m, ratio, suggested_dt_yr;
+    int32_t too_large;
+    char warning[512];
+} lsdtt_gpu_timestep_diagnosis;
+
+typedef struct {
+    double dz_tolerance_m;              /* 1e-4 */
+    double dz_rate_tolerance_m_per_yr;  /* 0: off */
+    int32_t patience_steps;             /* 5 */
+    double min_dt_fraction;             /* 0.25 */
+    int32_t stall_window;               /* 1000 */
+    double stall_decay_threshold;       /* 0.99 */
+    double oscillation_alternation_fraction;  /* 0.7 */
+} lsdtt_gpu_convergence_options;
+
+typedef struct lsdtt_gpu_timestep_controller lsdtt_gpu_timestep_controller;
+typedef struct lsdtt_gpu_convergence_monitor lsdtt_gpu_convergence_monitor;
+
+void lsdtt_gpu_timestep_options_init(lsdtt_gpu_timestep_options* o);
+void lsdtt_gpu_convergence_options_init(lsdtt_gpu_convergence_options* o);
+/* median_interior_gradient of the resident surface */
+int lsdtt_gpu_lem_median_interior_gradient(lsdtt_gpu_lem* lem, double* gradient);
+/* diagnose_lem_timestep (target_ratio <= 0: 0.7) */
+int lsdtt_gpu_lem_diagnose_timestep(lsdtt_gpu_lem* lem, double uplift_rate, double timestep_yr,
+                                    double target_ratio, lsdtt_gpu_timestep_diagnosis* out);
+/* the timestep of the following steps (lsdtt_lem_options::timestep) */
+int lsdtt_gpu_lem_set_timestep(lsdtt_gpu_lem* lem, double timestep_yr);
+/* lsdtt_lem_timestep_controller: _update reports the step just taken
+ * (max_local_elevation_change of its statistics) and returns the next dt;
+ * the reference gradient is refreshed from `lem` on the options' interval */
+lsdtt_gpu_timestep_controller* lsdtt_gpu_timestep_controller_create(double dt_initial_yr,
+                                                                    const lsdtt_gpu_timestep_options* o);
+void lsdtt_gpu_timestep_controller_destroy(lsdtt_gpu_timestep_controller* c);
+int lsdtt_gpu_timestep_controller_update(lsdtt_gpu_timestep_controller* c, lsdtt_gpu_lem* lem,
+                                         double max_local_dz_m, double* dt_next);
+int lsdtt_gpu_timestep_controller_get(const lsdtt_gpu_timestep_controller* c, double* dt, double* last_ratio,
+                                      double* reference_slope, int64_t* n_shrinks, int64_t* n_grows,
+                                      int64_t* n_changes);
+/* lsdtt_lem_convergence_monitor: state 0 running, 1 converged, 2 stalled */
+lsdtt_gpu_convergence_monitor* lsdtt_gpu_convergence_monitor_create(double dt_initial_yr,
+                                                                    const lsdtt_gpu_convergence_options* o);
+void lsdtt_gpu_convergence_monitor_destroy(lsdtt_gpu_convergence_monitor* m);
+int lsdtt_gpu_convergence_monitor_update(lsdtt_gpu_convergence_monitor* m, double max_local_dz_m,
+                                         double dt_used_yr, int32_t* state);
+int lsdtt_gpu_convergence_monitor_get(const lsdtt_gpu_convergence_monitor* m, int32_t* state,
+                                      int32_t* oscillating, int64_t* n_steps, double* last_dz,
+                                      double* window_decay, char* message, int32_t message_len);
+
+/* ---------------------------------------------------------------------
  * D-infinity flow graph (lsdtt_flowgraph_cpu with method d_infinity,
  * src/cpu/lsdtt_flowgraph_cpu.cpp:87-177, 423-663) and its accumulation
  * (accumulate() with cell_size^2 per valid cell, :988-1033) from host

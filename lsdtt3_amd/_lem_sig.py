@@ -3,6 +3,24 @@ include/lsdtt3_gpu.h."""
 import ctypes as C
 
 
+class TimestepOptions(C.Structure):
+    _fields_ = [("target_ratio", C.c_double), ("dt_min_yr", C.c_double), ("dt_max_yr", C.c_double),
+                ("grow_factor", C.c_double), ("shrink_factor", C.c_double), ("slope_refresh_interval", C.c_int32),
+                ("ladder_ratio", C.c_double), ("ratio_smoothing", C.c_double)]
+
+
+class TimestepDiagnosis(C.Structure):
+    _fields_ = [("reference_slope", C.c_double), ("cell_relief_m", C.c_double), ("uplift_per_step_m", C.c_double),
+                ("ratio", C.c_double), ("suggested_dt_yr", C.c_double), ("too_large", C.c_int32),
+                ("warning", C.c_char * 512)]
+
+
+class ConvergenceOptions(C.Structure):
+    _fields_ = [("dz_tolerance_m", C.c_double), ("dz_rate_tolerance_m_per_yr", C.c_double),
+                ("patience_steps", C.c_int32), ("min_dt_fraction", C.c_double), ("stall_window", C.c_int32),
+                ("stall_decay_threshold", C.c_double), ("oscillation_alternation_fraction", C.c_double)]
+
+
 class LemOptions(C.Structure):
     _fields_ = [
         ("timestep", C.c_double), ("erodibility", C.c_double),
@@ -70,6 +88,22 @@ def declare(L, sig):
     sig(L, "lsdtt_gpu_lem_set_tile_height", C.c_int, [vp, i32])
     sig(L, "lsdtt_gpu_lem_tile_height", i32, [vp])
     sig(L, "lsdtt_gpu_lem_get_channel_mask", C.c_int, [vp, vp])
+    pd = C.POINTER(C.c_double)
+    pi64 = C.POINTER(C.c_int64)
+    sig(L, "lsdtt_gpu_timestep_options_init", None, [C.POINTER(TimestepOptions)])
+    sig(L, "lsdtt_gpu_convergence_options_init", None, [C.POINTER(ConvergenceOptions)])
+    sig(L, "lsdtt_gpu_lem_median_interior_gradient", C.c_int, [vp, pd])
+    sig(L, "lsdtt_gpu_lem_diagnose_timestep", C.c_int, [vp, dbl, dbl, dbl, C.POINTER(TimestepDiagnosis)])
+    sig(L, "lsdtt_gpu_lem_set_timestep", C.c_int, [vp, dbl])
+    sig(L, "lsdtt_gpu_timestep_controller_create", vp, [dbl, C.POINTER(TimestepOptions)])
+    sig(L, "lsdtt_gpu_timestep_controller_destroy", None, [vp])
+    sig(L, "lsdtt_gpu_timestep_controller_update", C.c_int, [vp, vp, dbl, pd])
+    sig(L, "lsdtt_gpu_timestep_controller_get", C.c_int, [vp, pd, pd, pd, pi64, pi64, pi64])
+    sig(L, "lsdtt_gpu_convergence_monitor_create", vp, [dbl, C.POINTER(ConvergenceOptions)])
+    sig(L, "lsdtt_gpu_convergence_monitor_destroy", None, [vp])
+    sig(L, "lsdtt_gpu_convergence_monitor_update", C.c_int, [vp, dbl, dbl, C.POINTER(C.c_int32)])
+    sig(L, "lsdtt_gpu_convergence_monitor_get", C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), pi64,
+                                                           pd, pd, C.c_char_p, i32])
     sig(L, "lsdtt_gpu_dinf_graph", C.c_int, [vp, vp, i32, i32, dbl, C.c_char_p, i32, dbl, vp, vp, vp, vp, vp])
     sig(L, "lsdtt_gpu_lem_autotune_tiles", C.c_int, [vp, vp, i64, i32, vp, vp])
     sig(L, "lsdtt_gpu_lem_elevation_device", C.c_void_p, [vp])

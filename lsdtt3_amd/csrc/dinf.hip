@@ -102,6 +102,9 @@ __global__ void __launch_bounds__(256) k_dinf_receivers(const T* __restrict__ z,
         if (ok) {
             const float pi4 = static_cast<float>(M_PI / 4.0);
             const float sqrt2f = static_cast<float>(1.4142135623730950488);
+            // the winning facet's slopes, for its angle (one atan2 per cell)
+            float bs1 = 0.0f, bs2 = 0.0f;
+            bool interior = false;
             for (int a = 0; a < 8; ++a) {
                 const int g1 = wrap_nb(r, c, kE1R[a], kE1C[a], nr, nc, bc);
                 const int g2 = wrap_nb(r, c, kE2R[a], kE2C[a], nr, nc, bc);
@@ -110,17 +113,31 @@ __global__ void __launch_bounds__(256) k_dinf_receivers(const T* __restrict__ z,
                 if (!cell_valid(z, mask, g1, use_nd, nd, &e1) || !cell_valid(z, mask, g2, use_nd, nd, &e2)) continue;
                 if (e0 <= e1 && e0 <= e2) continue;
                 float rr, sl;
+                bool in = false;
                 const float s1 = (e0 - e1) / cs;
                 const float s2 = (e1 - e2) / cs;
                 if (e0 > e1 && e0 > e2) {
-                    rr = static_cast<float>(atan2(static_cast<double>(s2), static_cast<double>(s1)));
-                    if (rr < 0.0f) {
+                    // r = atan2(s2, s1) with s1 > 0 is < 0 exactly when s2 < 0;
+                    // it rounds above pi4 (the float above pi/4) when s2/s1 - 1
+                    // is above ~1.03e-7 and not when below; between the two
+                    // margins the rounded atan2 decides.  The interior
+                    // slope does not depend on r, so r is evaluated once,
+                    // for the winning facet.
+                    const double q = static_cast<double>(s2), p1 = static_cast<double>(s1);
+                    int cls;  // 0 below 0, 1 inside, 2 above pi/4
+                    if (s2 < 0.0f) cls = 0;
+                    else if (q < p1 * (1.0 + 0.9e-7)) cls = 1;
+                    else if (q > p1 * (1.0 + 1.2e-7)) cls = 2;
+                    else cls = static_cast<float>(atan2(q, p1)) > pi4 ? 2 : 1;
+                    if (cls == 0) {
                         rr = 0.0f;
                         sl = s1;
-                    } else if (rr > pi4) {
+                    } else if (cls == 2) {
                         rr = pi4;
                         sl = (e0 - e2) / (cs * sqrt2f);
                     } else {
+                        rr = 0.0f;  // set from atan2 below if this facet wins
+                        in = true;
                         sl = sqrtf(s1 * s1 + s2 * s2);
                     }
                 } else if (e0 > e1) {
@@ -136,8 +153,13 @@ __global__ void __launch_bounds__(256) k_dinf_receivers(const T* __restrict__ z,
                     best = a;
                     r0 = g1;
                     r1 = g2;
+                    interior = in;
+                    bs1 = s1;
+                    bs2 = s2;
                 }
             }
+            if (best >= 0 && interior)
+                best_r = static_cast<float>(atan2(static_cast<double>(bs2), static_cast<double>(bs1)));
             if (best >= 0) {
                 if (best_r == 0.0f) {
                     w0 = 1.0f;

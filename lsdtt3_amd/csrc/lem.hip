@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256) k_stats_fin(const double* partials, unsig
 constexpr int kRadixBins = 2048;
 
 __global__ void k_radix_hist(const float* __restrict__ v, long long n, const unsigned* __restrict__ sel, int shift,
-                             unsigned* __restrict__ hist)
+                             int bits, unsigned* __restrict__ hist)
 {
     __shared__ unsigned h[kRadixBins];
     for (int i = threadIdx.x; i < kRadixBins; i += blockDim.x) h[i] = 0;
@@ -219,7 +219,7 @@ __global__ void k_radix_hist(const float* __restrict__ v, long long n, const uns
         if (f < n) {
             const unsigned u = __float_as_uint(v[f]);
             want = (u & pmask) == prefix;
-            bin = (u >> shift) & (kRadixBins - 1);
+            bin = (u >> shift) & ((1u << bits) - 1u);  // this pass's digit only
         }
         unsigned long long pending = __ballot(want);
         while (pending) {
@@ -817,7 +817,7 @@ void LemEngine::stats(LemStats* st, hipStream_t s)
         LSDTT_HIP(hipMemcpyAsync(dsel, sel, sizeof(unsigned) * 3, hipMemcpyHostToDevice, s));
         const int shifts[3] = {21, 10, 0}, bits[3] = {11, 11, 10};
         for (int pass = 0; pass < 3; ++pass) {
-            k_radix_hist<<<grid_for(n, 2048), kB, 0, s>>>(absdz.p, n, dsel, shifts[pass], hist.p);
+            k_radix_hist<<<grid_for(n, 2048), kB, 0, s>>>(absdz.p, n, dsel, shifts[pass], bits[pass], hist.p);
             k_radix_pick<<<1, kPickThreads, 0, s>>>(hist.p, dsel, shifts[pass], bits[pass]);
             LSDTT_LAUNCH_CHECK();
         }

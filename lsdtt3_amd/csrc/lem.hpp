@@ -118,6 +118,7 @@ public:
     dbuf<float> dw;
     dbuf<double> darea;
     int n_gen_base = 0;
+    dbuf<unsigned long long> grad_keys;  // median interior gradient (timestep control)
     bool darea_warm = false;
     int last_dinf_passes = 0;
     dbuf<double> z, z_before, K_field, bel, area_sm;

@@ -12,7 +12,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from ._lem_sig import LemOptions, LemStats
+from ._lem_sig import ConvergenceOptions, LemOptions, LemStats, TimestepDiagnosis, TimestepOptions
 
 
 def default_options() -> LemOptions:
@@ -164,6 +164,34 @@ class LandscapeModel:
         N.check(N.lib().lsdtt_gpu_lem_get_channel_mask(self._h, N.ptr(out)))
         return out
 
+    # ---- timestep control (src/cpu/lsdtt_lem_timestep_cpu.hpp) --------------
+    def median_interior_gradient(self) -> float:
+        """median_interior_gradient of the current surface (device)"""
+        g = C.c_double(0.0)
+        N.check(N.lib().lsdtt_gpu_lem_median_interior_gradient(self._h, C.byref(g)))
+        return g.value
+
+    def diagnose_timestep(self, uplift_rate=None, timestep_years=None, target_ratio=0.7) -> dict:
+        """diagnose_lem_timestep for this model's (or the given) U and dt"""
+        d = TimestepDiagnosis()
+        U = self.options.uplift_rate if uplift_rate is None else uplift_rate
+        dt = self.options.timestep if timestep_years is None else timestep_years
+        N.check(N.lib().lsdtt_gpu_lem_diagnose_timestep(self._h, float(U), float(dt), float(target_ratio),
+                                                        C.byref(d)))
+        out = {k: getattr(d, k) for k, _ in TimestepDiagnosis._fields_}
+        out["too_large"] = bool(out["too_large"])
+        out["warning"] = d.warning.decode()
+        return out
+
+    @property
+    def timestep(self) -> float:
+        return self.options.timestep
+
+    @timestep.setter
+    def timestep(self, dt: float) -> None:
+        N.check(N.lib().lsdtt_gpu_lem_set_timestep(self._h, float(dt)))
+        self.options.timestep = float(dt)
+
     # ---- tile shape of the routing sweeps (speed only) ----------------------
     @property
     def tile_height(self) -> int:
@@ -200,6 +228,77 @@ class LandscapeModel:
             N.check(L.lsdtt_gpu_lem_profile_get(self._h, i, name, 64, C.byref(ms), C.byref(nl), C.byref(b)))
             out[name.value.decode()] = (ms.value, nl.value, b.value)
         return out
+
+
+class TimestepController:
+    """lsdtt_lem_timestep_controller on a LandscapeModel's resident surface:
+    ``update(model, max_local_dz)`` returns the next timestep."""
+
+    def __init__(self, dt_initial, **options):
+        o = TimestepOptions()
+        N.lib().lsdtt_gpu_timestep_options_init(C.byref(o))
+        for k, v in options.items():
+            if not hasattr(o, k):
+                raise TypeError(f"unknown option {k!r}")
+            setattr(o, k, v)
+        self._h = N.lib().lsdtt_gpu_timestep_controller_create(float(dt_initial), C.byref(o))
+        if not self._h:
+            raise N.GpuError(N.ERR_ARG, (N.lib().lsdtt_gpu_last_error() or b"").decode())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().lsdtt_gpu_timestep_controller_destroy(h)
+            self._h = None
+
+    def update(self, model, max_local_dz):
+        dt = C.c_double(0.0)
+        N.check(N.lib().lsdtt_gpu_timestep_controller_update(self._h, model._h, float(max_local_dz), C.byref(dt)))
+        return dt.value
+
+    def state(self) -> dict:
+        v = [C.c_double() for _ in range(3)] + [C.c_int64() for _ in range(3)]
+        N.check(N.lib().lsdtt_gpu_timestep_controller_get(self._h, *[C.byref(x) for x in v]))
+        names = ("dt", "last_ratio", "reference_slope", "n_shrinks", "n_grows", "n_changes")
+        return {k: x.value for k, x in zip(names, v)}
+
+
+class ConvergenceMonitor:
+    """lsdtt_lem_convergence_monitor: ``update(max_dz, dt_used)`` returns
+    "running", "converged" or "stalled"."""
+
+    STATES = ("running", "converged", "stalled")
+
+    def __init__(self, dt_initial, **options):
+        o = ConvergenceOptions()
+        N.lib().lsdtt_gpu_convergence_options_init(C.byref(o))
+        for k, v in options.items():
+            if not hasattr(o, k):
+                raise TypeError(f"unknown option {k!r}")
+            setattr(o, k, v)
+        self._h = N.lib().lsdtt_gpu_convergence_monitor_create(float(dt_initial), C.byref(o))
+        if not self._h:
+            raise N.GpuError(N.ERR_ARG, (N.lib().lsdtt_gpu_last_error() or b"").decode())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().lsdtt_gpu_convergence_monitor_destroy(h)
+            self._h = None
+
+    def update(self, max_dz, dt_used):
+        st = C.c_int32(0)
+        N.check(N.lib().lsdtt_gpu_convergence_monitor_update(self._h, float(max_dz), float(dt_used), C.byref(st)))
+        return self.STATES[st.value]
+
+    def state(self) -> dict:
+        st, osc, ns = C.c_int32(), C.c_int32(), C.c_int64()
+        ld, dec = C.c_double(), C.c_double()
+        msg = C.create_string_buffer(512)
+        N.check(N.lib().lsdtt_gpu_convergence_monitor_get(self._h, C.byref(st), C.byref(osc), C.byref(ns),
+                                                          C.byref(ld), C.byref(dec), msg, 512))
+        return dict(state=self.STATES[st.value], oscillating=bool(osc.value), n_steps=ns.value, last_dz=ld.value,
+                    window_decay=dec.value, message=msg.value.decode())
 
 
 class Ensemble:

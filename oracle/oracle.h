@@ -99,6 +99,59 @@ int orc_dinf_channel_mask(int nr, int nc, double dx, const float* z, const uint8
                           double threshold, int divide_area_by_dx, int libm_atan2, uint8_t* is_channel,
                           double* area_out);
 
+/* ---- LEM timestep control (oracle/orc_timestep.c) --------------------- */
+
+typedef struct {
+    double reference_slope, cell_relief_m, uplift_per_step_m, ratio, suggested_dt_yr;
+    int too_large;
+} orc_ts_diagnosis;
+
+typedef struct {
+    double target_ratio;            /* 0.7 */
+    double dt_min_yr, dt_max_yr;    /* 0 -> dt_initial / 100, * 100 */
+    double grow_factor;             /* 1.15 */
+    double shrink_factor;           /* 0.5 */
+    int    slope_refresh_interval;  /* 25 */
+    double ladder_ratio;            /* 2 (1 = no ladder) */
+    double ratio_smoothing;         /* 0.15 */
+} orc_ts_options;
+
+typedef struct {
+    orc_ts_options o;
+    double dt, dt_raw, dt_initial, dt_min, dt_max, slope, last_ratio, ratio_ema;
+    long n_steps, n_shrinks, n_grows, n_changes;
+    int rung, lowest_rung;
+} orc_ts_controller;
+
+typedef struct {
+    double dz_tolerance_m;              /* 1e-4 */
+    double dz_rate_tolerance_m_per_yr;  /* 0 */
+    int    patience_steps;              /* 5 */
+    double min_dt_fraction;             /* 0.25 */
+    int    stall_window;                /* 1000 */
+    double stall_decay_threshold;       /* 0.99 */
+    double oscillation_alternation_fraction;  /* 0.7 */
+} orc_ts_monitor_options;
+
+typedef struct {
+    orc_ts_monitor_options o;
+    double dt_initial, last_dz, decay;
+    long n_steps;
+    int n_below, oscillating, state, window, n_hist;
+    double* hist;
+} orc_ts_monitor;
+
+double orc_median_interior_gradient(int nr, int nc, double dx, const double* z, const uint8_t* mask);
+void orc_diagnose_timestep(int nr, int nc, double dx, const double* z, const uint8_t* mask, double U, double dt,
+                           double target, orc_ts_diagnosis* out);
+void orc_ts_controller_init(orc_ts_controller* c, double dt_initial, const orc_ts_options* o);
+/* slope_now: the median interior gradient of the surface after the step
+ * (used on the refresh interval) */
+double orc_ts_controller_update(orc_ts_controller* c, double max_dz, double slope_now, double dx);
+void orc_ts_monitor_init(orc_ts_monitor* m, double dt_initial, const orc_ts_monitor_options* o);
+void orc_ts_monitor_free(orc_ts_monitor* m);
+int orc_ts_monitor_update(orc_ts_monitor* m, double max_dz, double dt_used);
+
 /* ---- chi (oracle/orc_chi.c) ------------------------------------------ */
 
 /* get_chi_raster: chi from every outlet (seed 0), A = s_size * dx^2; chi is

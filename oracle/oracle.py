@@ -98,6 +98,21 @@ def lib():
         L.orc_stream_power_dinf.argtypes = [C.c_int64, u8p, i64p, f32p, u8p, i64p, C.c_int64, f64p, C.c_void_p,
                                             C.POINTER(SpParams), f64p, f64p]
         L.orc_stream_power_dinf.restype = C.c_int64
+        L.orc_median_interior_gradient.argtypes = [C.c_int, C.c_int, C.c_double, f64p, u8p]
+        L.orc_median_interior_gradient.restype = C.c_double
+        L.orc_diagnose_timestep.argtypes = [C.c_int, C.c_int, C.c_double, f64p, u8p, C.c_double, C.c_double,
+                                            C.c_double, C.POINTER(TsDiagnosis)]
+        L.orc_diagnose_timestep.restype = None
+        L.orc_ts_controller_init.argtypes = [C.POINTER(TsController), C.c_double, C.POINTER(TsOptions)]
+        L.orc_ts_controller_init.restype = None
+        L.orc_ts_controller_update.argtypes = [C.POINTER(TsController), C.c_double, C.c_double, C.c_double]
+        L.orc_ts_controller_update.restype = C.c_double
+        L.orc_ts_monitor_init.argtypes = [C.POINTER(TsMonitor), C.c_double, C.POINTER(TsMonitorOptions)]
+        L.orc_ts_monitor_init.restype = None
+        L.orc_ts_monitor_free.argtypes = [C.POINTER(TsMonitor)]
+        L.orc_ts_monitor_free.restype = None
+        L.orc_ts_monitor_update.argtypes = [C.POINTER(TsMonitor), C.c_double, C.c_double]
+        L.orc_ts_monitor_update.restype = C.c_int
         _lib = L
     return _lib
 
@@ -289,6 +304,53 @@ def picard_transient(z, mask, dx, bc, params, dt_total, channel=None):
                                None if ch is None else ch.ctypes.data_as(C.c_void_p), C.byref(params),
                                float(dt_total), zz.ravel(), C.byref(res))
     return zz, res.as_dict()
+
+
+class TsDiagnosis(C.Structure):
+    _fields_ = [("reference_slope", C.c_double), ("cell_relief_m", C.c_double), ("uplift_per_step_m", C.c_double),
+                ("ratio", C.c_double), ("suggested_dt_yr", C.c_double), ("too_large", C.c_int)]
+
+
+class TsOptions(C.Structure):
+    _fields_ = [("target_ratio", C.c_double), ("dt_min_yr", C.c_double), ("dt_max_yr", C.c_double),
+                ("grow_factor", C.c_double), ("shrink_factor", C.c_double), ("slope_refresh_interval", C.c_int),
+                ("ladder_ratio", C.c_double), ("ratio_smoothing", C.c_double)]
+
+
+def ts_options(**kw):
+    """lsdtt_lem_timestep_controller_options defaults
+    (src/cpu/lsdtt_lem_timestep_cpu.hpp:116-160) with overrides"""
+    o = TsOptions(0.7, 0.0, 0.0, 1.15, 0.5, 25, 2.0, 0.15)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class TsController(C.Structure):
+    _fields_ = [("o", TsOptions), ("dt", C.c_double), ("dt_raw", C.c_double), ("dt_initial", C.c_double),
+                ("dt_min", C.c_double), ("dt_max", C.c_double), ("slope", C.c_double), ("last_ratio", C.c_double),
+                ("ratio_ema", C.c_double), ("n_steps", C.c_long), ("n_shrinks", C.c_long), ("n_grows", C.c_long),
+                ("n_changes", C.c_long), ("rung", C.c_int), ("lowest_rung", C.c_int)]
+
+
+class TsMonitorOptions(C.Structure):
+    _fields_ = [("dz_tolerance_m", C.c_double), ("dz_rate_tolerance_m_per_yr", C.c_double),
+                ("patience_steps", C.c_int), ("min_dt_fraction", C.c_double), ("stall_window", C.c_int),
+                ("stall_decay_threshold", C.c_double), ("oscillation_alternation_fraction", C.c_double)]
+
+
+def ts_monitor_options(**kw):
+    """lsdtt_lem_convergence_options defaults (.hpp:236-292) with overrides"""
+    o = TsMonitorOptions(1e-4, 0.0, 5, 0.25, 1000, 0.99, 0.70)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class TsMonitor(C.Structure):
+    _fields_ = [("o", TsMonitorOptions), ("dt_initial", C.c_double), ("last_dz", C.c_double), ("decay", C.c_double),
+                ("n_steps", C.c_long), ("n_below", C.c_int), ("oscillating", C.c_int), ("state", C.c_int),
+                ("window", C.c_int), ("n_hist", C.c_int), ("hist", C.c_void_p)]
 
 
 class LemParams(C.Structure):
@@ -483,3 +545,48 @@ def dinf_channel_mask(z, mask, dx, threshold, bc="oooo", divide_area_by_dx=True,
     if st != 0:
         raise OracleError(f"dinf channel mask failed ({st})")
     return out, area
+
+
+def median_interior_gradient(z, mask, dx):
+    zz = np.ascontiguousarray(z, np.float64)
+    nr, nc = zz.shape
+    return lib().orc_median_interior_gradient(nr, nc, float(dx), zz.ravel(),
+                                              np.ascontiguousarray(mask, np.uint8).ravel())
+
+
+def diagnose_timestep(z, mask, dx, U, dt, target=0.7):
+    zz = np.ascontiguousarray(z, np.float64)
+    nr, nc = zz.shape
+    d = TsDiagnosis()
+    lib().orc_diagnose_timestep(nr, nc, float(dx), zz.ravel(), np.ascontiguousarray(mask, np.uint8).ravel(),
+                                float(U), float(dt), float(target), C.byref(d))
+    return {k: getattr(d, k) for k, _ in TsDiagnosis._fields_}
+
+
+class TimestepController:
+    """the oracle controller; update(max_dz, z, mask, dx) -> next dt"""
+
+    def __init__(self, dt_initial, **opts):
+        self.c = TsController()
+        self.o = ts_options(**opts)
+        lib().orc_ts_controller_init(C.byref(self.c), float(dt_initial), C.byref(self.o))
+
+    def update(self, max_dz, z, mask, dx):
+        slope = median_interior_gradient(z, mask, dx)
+        return lib().orc_ts_controller_update(C.byref(self.c), float(max_dz), slope, float(dx))
+
+
+class ConvergenceMonitor:
+    """the oracle monitor; update(max_dz, dt_used) -> 0 running, 1 converged, 2 stalled"""
+
+    def __init__(self, dt_initial, **opts):
+        self.m = TsMonitor()
+        self.o = ts_monitor_options(**opts)
+        lib().orc_ts_monitor_init(C.byref(self.m), float(dt_initial), C.byref(self.o))
+
+    def __del__(self):
+        if getattr(self, "m", None) is not None and self.m.hist:
+            lib().orc_ts_monitor_free(C.byref(self.m))
+
+    def update(self, max_dz, dt_used):
+        return lib().orc_ts_monitor_update(C.byref(self.m), float(max_dz), float(dt_used))

@@ -304,3 +304,24 @@ def test_gpu_newton_failure_falls_back_to_picard(gpu):
     zg, zo, sg, so = run_both(gpu, z, 30.0, 1, kw, okw)
     assert sg["hillslope_method"] == 1 and so["hillslope_method"] == 1
     assert np.abs(zg - zo).max() <= 1e-7
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_gpu_p999_is_the_exact_order_statistic(gpu, seed):
+    """the step's p99.9 |dz| (radix selection on the device) is element
+    floor(0.999 (n - 1)) of the sorted float |z_after - z_before| of the
+    device's own surfaces (lsdtt_statistics.cpp:420-429), bit for bit"""
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(97, 113, dx=30.0, seed=seed, max_slope=0.6)
+    mask = np.ones(z.shape, np.uint8)
+    mask[40:44, 50:55] = 0
+    lem = gpu.LandscapeModel(z, 30.0, mask=mask, timestep_years=800.0, erodibility=3e-5, m_exponent=0.5,
+                             n_exponent=1.0, uplift_rate=1e-3, hillslope_diffusivity=0.01,
+                             nonlinear_hillslopes=True, critical_hillslope_slope=0.8, boundary_conditions="nnon")
+    zb = lem.elevation
+    st = lem.step()
+    za = lem.elevation
+    v = np.sort(np.abs(za - zb)[mask != 0].astype(np.float32))
+    p = v[int(np.floor(0.999 * (v.size - 1)))]
+    assert np.float32(st["p999_local_elevation_change"]) == p
+    assert st["max_local_elevation_change"] == np.abs(za - zb)[mask != 0].max()
