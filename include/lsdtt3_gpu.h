@@ -114,6 +114,39 @@ int lsdtt_gpu_flow_area(lsdtt_gpu_flow* f, double nodata, float* area, uint8_t* 
  * NULL means cell_size^2 per valid cell. Output double[n]. */
 int lsdtt_gpu_flow_accumulate(lsdtt_gpu_flow* f, const double* weights, double* accum);
 
+/* Analytic fluvial steady state on the flow object:
+ * compute_d8_fluvial_steady_state(flow, opts)
+ * (src/cpu/lsdtt_steady_state_cpu.cpp:76-225; options :62-89, float in the
+ * reference).  chi = chi(receiver) + dx_eff * powf(A, -m/n) * w (float,
+ * outlets 0; A = accumulate() / dx when divide_area_by_dx); z = baselevel +
+ * (U/K)^(1/n) chi, or in relief mode target_relief * chi / max(chi) with
+ * K_used back-calculated; z_out float[n] (nodata off the valid cells).
+ * erodibility_field (float[n], may be NULL) gives the K-weighted chi.  powf
+ * is the correctly rounded value (the reference's C library returns it or
+ * one ulp beside it).  For lsdtt-lem's snap
+ * (snap_lem_surface_to_fluvial_steady_state, lsdtt_lem_cpu.cpp:813-845):
+ * lsdtt_gpu_fill, then a flow object on the filled surface, then this. */
+typedef struct {
+    double U, K, m, n;           /* 1e-4, 1e-4, 0.45, 1 */
+    int32_t relief_mode;         /* 0 */
+    double target_relief;        /* 1000 m */
+    double baselevel_elevation;  /* 0 */
+    int32_t divide_area_by_dx;   /* 1 */
+    double channel_threshold_area;  /* 0: chi_c not reported */
+} lsdtt_gpu_steady_state_options;
+
+typedef struct {
+    double K_used, max_unit_height, computed_relief, erodibility_field_scale;
+    double max_unit_height_at_channel_head;
+    int64_t n_channel_pixels;
+    int32_t chi_c_reported;
+} lsdtt_gpu_steady_state_result;
+
+void lsdtt_gpu_steady_state_options_init(lsdtt_gpu_steady_state_options* o);
+int lsdtt_gpu_flow_steady_state(lsdtt_gpu_flow* f, const lsdtt_gpu_steady_state_options* o,
+                                const float* erodibility_field, double nodata, float* z_out,
+                                lsdtt_gpu_steady_state_result* result);
+
 /* Chi raster: get_chi_raster(flow, {m_over_n, A_0})
  * (src/cpu/lsdtt_chi_extraction_cpu.cpp:739-765; recursion :283-365,
  * increment :154-179): every outlet starts at 0, A = contributing pixels x

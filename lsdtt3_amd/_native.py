@@ -45,6 +45,18 @@ def _sig(L, name, res, args):
     f.argtypes = args
 
 
+class SteadyOptions(C.Structure):
+    _fields_ = [("U", C.c_double), ("K", C.c_double), ("m", C.c_double), ("n", C.c_double),
+                ("relief_mode", C.c_int32), ("target_relief", C.c_double), ("baselevel_elevation", C.c_double),
+                ("divide_area_by_dx", C.c_int32), ("channel_threshold_area", C.c_double)]
+
+
+class SteadyResult(C.Structure):
+    _fields_ = [("K_used", C.c_double), ("max_unit_height", C.c_double), ("computed_relief", C.c_double),
+                ("erodibility_field_scale", C.c_double), ("max_unit_height_at_channel_head", C.c_double),
+                ("n_channel_pixels", C.c_int64), ("chi_c_reported", C.c_int32)]
+
+
 def lib():
     """The loaded C-ABI library (raises if it has not been built)."""
     global _lib
@@ -70,6 +82,9 @@ def lib():
     _sig(L, "lsdtt_gpu_flow_get", C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp])
     _sig(L, "lsdtt_gpu_flow_area", C.c_int, [vp, dbl, vp, vp, vp])
     _sig(L, "lsdtt_gpu_flow_accumulate", C.c_int, [vp, vp, vp])
+    _sig(L, "lsdtt_gpu_steady_state_options_init", None, [C.POINTER(SteadyOptions)])
+    _sig(L, "lsdtt_gpu_flow_steady_state", C.c_int, [vp, C.POINTER(SteadyOptions), vp, C.c_double, vp,
+                                                     C.POINTER(SteadyResult)])
     _sig(L, "lsdtt_gpu_flow_chi", C.c_int, [vp, dbl, dbl, dbl, vp, vp])
     _sig(L, "lsdtt_gpu_flow_chi_device", C.c_int, [vp, dbl, dbl, dbl, vp, vp])
     _sig(L, "lsdtt_gpu_flow_chi_threshold", C.c_int, [vp, dbl, dbl, i64, vp, vp, vp, vp, vp, vp])

@@ -11,6 +11,7 @@
 #include "../../include/lsdtt3_gpu.h"
 #include "capi_util.hpp"
 #include "chi.hpp"
+#include "steady.hpp"
 #include "flow.hpp"
 
 namespace lsdtt_gpu {
@@ -297,6 +298,74 @@ int lsdtt_gpu_flow_area(lsdtt_gpu_flow* f, double nodata, float* area, uint8_t* 
                                   : static_cast<float>(nodata);
             if (area_mask) area_mask[i] = v ? 1 : 0;
             if (counts) counts[i] = v ? c[static_cast<size_t>(i)] : 0;
+        }
+        return LSDTT_GPU_OK;
+    });
+}
+
+void lsdtt_gpu_steady_state_options_init(lsdtt_gpu_steady_state_options* o)
+{
+    if (!o) return;
+    const SteadyOptions d;
+    o->U = d.U;
+    o->K = d.K;
+    o->m = d.m;
+    o->n = d.n;
+    o->relief_mode = d.relief_mode ? 1 : 0;
+    o->target_relief = d.target_relief;
+    o->baselevel_elevation = d.baselevel_elevation;
+    o->divide_area_by_dx = d.divide_area_by_dx ? 1 : 0;
+    o->channel_threshold_area = d.channel_threshold_area;
+}
+
+int lsdtt_gpu_flow_steady_state(lsdtt_gpu_flow* f, const lsdtt_gpu_steady_state_options* o,
+                                const float* erodibility_field, double nodata, float* z_out,
+                                lsdtt_gpu_steady_state_result* res)
+{
+    return guarded([&] {
+        if (!f || !f->built || !o || !z_out) return fail_arg("flow object not built or null argument");
+        FlowEngine& e = *f->eng;
+        const long long n = e.n;
+        SteadyOptions so;
+        so.U = static_cast<float>(o->U);
+        so.K = static_cast<float>(o->K);
+        so.m = static_cast<float>(o->m);
+        so.n = static_cast<float>(o->n);
+        so.relief_mode = o->relief_mode != 0;
+        so.target_relief = static_cast<float>(o->target_relief);
+        so.baselevel_elevation = static_cast<float>(o->baselevel_elevation);
+        so.divide_area_by_dx = o->divide_area_by_dx != 0;
+        so.channel_threshold_area = static_cast<float>(o->channel_threshold_area);
+        hipStream_t s = nullptr;
+        dbuf<float> w, Kd, zd;
+        if (erodibility_field) {
+            // make_chi_weights' checks (lsdtt_steady_state_cpu.cpp:44-66)
+            if (!(so.K > 0.0f) || !std::isfinite(so.K))
+                return fail_arg("erodibility_field: the reference K must be > 0");
+            std::vector<uint8_t> m(static_cast<size_t>(n));
+            LSDTT_HIP(hipMemcpy(m.data(), e.mask, n, hipMemcpyDeviceToHost));
+            for (long long i = 0; i < n; ++i)
+                if (m[static_cast<size_t>(i)] && !(erodibility_field[i] > 0.0f && std::isfinite(erodibility_field[i])))
+                    return fail_arg("erodibility_field: K must be > 0; pixel (" + std::to_string(i / f->o.n_cols) +
+                                    ", " + std::to_string(i % f->o.n_cols) + ") has K = " +
+                                    std::to_string(erodibility_field[i]));
+            Kd.reserve(n);
+            w.reserve(n);
+            LSDTT_HIP(hipMemcpyAsync(Kd.p, erodibility_field, sizeof(float) * n, hipMemcpyHostToDevice, s));
+            chi_weights(e.mask, Kd.p, n, so.K, so.n, w.p, s);
+        }
+        zd.reserve(n);
+        const SteadyResult r =
+            steady_state_d8(e, so, erodibility_field ? w.p : nullptr, static_cast<float>(nodata), zd.p, s);
+        LSDTT_HIP(hipMemcpy(z_out, zd.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+        if (res) {
+            res->K_used = r.K_used;
+            res->max_unit_height = r.max_unit_height;
+            res->computed_relief = r.computed_relief;
+            res->erodibility_field_scale = r.erodibility_field_scale;
+            res->max_unit_height_at_channel_head = r.max_unit_height_at_channel_head;
+            res->n_channel_pixels = r.n_channel_pixels;
+            res->chi_c_reported = r.chi_c_reported ? 1 : 0;
         }
         return LSDTT_GPU_OK;
     });

@@ -30,10 +30,6 @@ namespace lsdtt_gpu {
 
 namespace {
 
-__constant__ const int kE1R[8] = {0, -1, -1, 0, 0, 1, 1, 0};
-__constant__ const int kE1C[8] = {1, 0, 0, -1, -1, 0, 0, 1};
-__constant__ const int kE2R[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
-__constant__ const int kE2C[8] = {1, 1, -1, -1, -1, -1, 1, 1};
 __constant__ const int kAC[8] = {0, 1, 1, 2, 2, 3, 3, 4};
 __constant__ const int kAF[8] = {1, -1, 1, -1, 1, -1, 1, -1};
 
@@ -58,13 +54,13 @@ __device__ __forceinline__ int wrap_nb(int r, int c, int dr, int dc, int nr, int
     return tr * nc + tc;
 }
 
-template <class T>
-__device__ __forceinline__ bool cell_valid(const T* z, const uint8_t* mask, int f, bool use_nd, float nd, float* zf)
-{
-    const float v = static_cast<float>(z[f]);
-    *zf = v;
-    return mask[f] && isfinite(v) && !(use_nd && v == nd);
-}
+
+// facet a's cardinal (e1) and diagonal (e2) neighbours as slots of the 3x3
+// neighbourhood, slot (dr + 1) * 3 + (dc + 1): e1 at (0,1) (-1,0) (-1,0)
+// (0,-1) (0,-1) (1,0) (1,0) (0,1), e2 at (-1,1) (-1,1) (-1,-1) (-1,-1)
+// (1,-1) (1,-1) (1,1) (1,1)
+__constant__ const int kE1S[8] = {5, 1, 1, 3, 3, 7, 7, 5};
+__constant__ const int kE2S[8] = {2, 2, 0, 0, 6, 6, 8, 8};
 
 template <class T>
 __global__ void __launch_bounds__(256) k_dinf_receivers(const T* __restrict__ z, const uint8_t* __restrict__ mask,
@@ -79,25 +75,34 @@ __global__ void __launch_bounds__(256) k_dinf_receivers(const T* __restrict__ z,
          ff += (long long)gridDim.x * blockDim.x) {
         const int f = static_cast<int>(ff);
         const int r = f / nc, c = f - r * nc;
+        // the 3x3 neighbourhood once: resolved (wrapped) cells, their float
+        // elevations and validity; whether the raw neighbour is in the grid
+        int nb[9];
+        float zz[9];
+        bool vv[9], ing[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int dr = k / 3 - 1, dc = k % 3 - 1;
+            nb[k] = (k == 4) ? f : wrap_nb(r, c, dr, dc, nr, nc, bc);
+            const int rr = r + dr, cc = c + dc;
+            ing[k] = rr >= 0 && rr < nr && cc >= 0 && cc < nc;
+            const int g = nb[k] >= 0 ? nb[k] : f;
+            const float v = static_cast<float>(z[g]);
+            zz[k] = v;
+            vv[k] = nb[k] >= 0 && mask[g] && isfinite(v) && !(use_nd && v == nd);
+        }
         int r0 = -1, r1 = -1;
         float w0 = 0.0f, w1 = 0.0f, best_s = 0.0f, best_r = 0.0f;
         int best = -1;
-        float e0;
-        bool ok = cell_valid(z, mask, f, use_nd, nd, &e0);
+        const float e0 = zz[4];
+        bool ok = vv[4];
         ok = ok && !((bc.n == 'b' && r == 0) || (bc.s == 'b' && r == nr - 1) || (bc.e == 'b' && c == nc - 1) ||
                      (bc.w == 'b' && c == 0));
         if (ok && nodata_outlet) {
-            for (int dr = -1; dr <= 1 && ok; ++dr)
-                for (int dc = -1; dc <= 1; ++dc) {
-                    if (!dr && !dc) continue;
-                    const int rr = r + dr, cc = c + dc;
-                    if (rr < 0 || rr >= nr || cc < 0 || cc >= nc) continue;
-                    float t;
-                    if (!cell_valid(z, mask, rr * nc + cc, use_nd, nd, &t)) {
-                        ok = false;
-                        break;
-                    }
-                }
+            // touches_interior_nodata: an invalid raw neighbour inside the grid
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                if (k != 4 && ing[k] && !vv[k]) ok = false;
         }
         if (ok) {
             const float pi4 = static_cast<float>(M_PI / 4.0);
@@ -106,11 +111,11 @@ __global__ void __launch_bounds__(256) k_dinf_receivers(const T* __restrict__ z,
             float bs1 = 0.0f, bs2 = 0.0f;
             bool interior = false;
             for (int a = 0; a < 8; ++a) {
-                const int g1 = wrap_nb(r, c, kE1R[a], kE1C[a], nr, nc, bc);
-                const int g2 = wrap_nb(r, c, kE2R[a], kE2C[a], nr, nc, bc);
+                const int k1 = kE1S[a], k2 = kE2S[a];
+                const int g1 = nb[k1], g2 = nb[k2];
                 if (g1 < 0 || g2 < 0 || g1 == f || g2 == f) continue;
-                float e1, e2;
-                if (!cell_valid(z, mask, g1, use_nd, nd, &e1) || !cell_valid(z, mask, g2, use_nd, nd, &e2)) continue;
+                if (!vv[k1] || !vv[k2]) continue;
+                const float e1 = zz[k1], e2 = zz[k2];
                 if (e0 <= e1 && e0 <= e2) continue;
                 float rr, sl;
                 bool in = false;

@@ -149,6 +149,31 @@ class FlowInfo:
         return out
 
 
+    def fluvial_steady_state(self, U=1e-4, K=1e-4, m=0.45, n=1.0, relief_mode=False, target_relief=1000.0,
+                             baselevel_elevation=0.0, divide_area_by_dx=True, channel_threshold_area=0.0,
+                             erodibility_field=None, no_data_value=-9999.0):
+        """compute_d8_fluvial_steady_state(flow, opts)
+        (src/cpu/lsdtt_steady_state_cpu.cpp:173-225): (float32 elevation
+        raster, result dict with K_used, max_unit_height, computed_relief,
+        erodibility_field_scale, chi_c fields)"""
+        L = N.lib()
+        o = N.SteadyOptions()
+        L.lsdtt_gpu_steady_state_options_init(C.byref(o))
+        o.U, o.K, o.m, o.n = float(U), float(K), float(m), float(n)
+        o.relief_mode = int(relief_mode)
+        o.target_relief = float(target_relief)
+        o.baselevel_elevation = float(baselevel_elevation)
+        o.divide_area_by_dx = int(divide_area_by_dx)
+        o.channel_threshold_area = float(channel_threshold_area)
+        kf = None if erodibility_field is None else np.ascontiguousarray(erodibility_field, np.float32)
+        out = np.empty((self.n_rows, self.n_cols), np.float32)
+        r = N.SteadyResult()
+        N.check(L.lsdtt_gpu_flow_steady_state(self._h, C.byref(o), N.ptr(kf), float(no_data_value), N.ptr(out),
+                                              C.byref(r)))
+        res = {k: getattr(r, k) for k, _ in N.SteadyResult._fields_}
+        res["chi_c_reported"] = bool(res["chi_c_reported"])
+        return out, res
+
     def get_chi_raster(self, m_over_n=0.45, A_0=1.0, no_data_value=-9999.0):
         """lsdtt::get_chi_raster(flow, {m_over_n, A_0})
         (src/cpu/lsdtt_chi_extraction_cpu.cpp:739-765): (chi float32

@@ -152,6 +152,29 @@ void orc_ts_monitor_init(orc_ts_monitor* m, double dt_initial, const orc_ts_moni
 void orc_ts_monitor_free(orc_ts_monitor* m);
 int orc_ts_monitor_update(orc_ts_monitor* m, double max_dz, double dt_used);
 
+/* ---- analytic fluvial steady state (oracle/orc_steady.c) --------------- */
+
+typedef struct {
+    float U, K, m, n;
+    int relief_mode;
+    float target_relief, baselevel_elevation;
+    int divide_area_by_dx;
+    float channel_threshold_area;
+} orc_ss_options;
+
+typedef struct {
+    float K_used, max_unit_height, computed_relief, erodibility_field_scale, max_unit_height_at_channel_head;
+    int64_t n_channel_pixels;
+    int chi_c_reported;
+} orc_ss_result;
+
+/* compute_d8_fluvial_steady_state on the oracle's D8 arrays (receiver -1
+ * nodata / -2 outlet, dir, Kahn order, accumulated area in m^2) */
+int orc_steady_state_d8(int nr, int nc, double cell_size, const uint8_t* valid, const int64_t* receiver,
+                        const uint8_t* dir, const int64_t* topo, int64_t n_topo, const double* area,
+                        const float* K_field, const orc_ss_options* o, int libm_pow, float nodata, float* z_out,
+                        orc_ss_result* res);
+
 /* ---- chi (oracle/orc_chi.c) ------------------------------------------ */
 
 /* get_chi_raster: chi from every outlet (seed 0), A = s_size * dx^2; chi is

@@ -98,6 +98,10 @@ def lib():
         L.orc_stream_power_dinf.argtypes = [C.c_int64, u8p, i64p, f32p, u8p, i64p, C.c_int64, f64p, C.c_void_p,
                                             C.POINTER(SpParams), f64p, f64p]
         L.orc_stream_power_dinf.restype = C.c_int64
+        L.orc_steady_state_d8.argtypes = [C.c_int, C.c_int, C.c_double, u8p, i64p, u8p, i64p, C.c_int64, f64p,
+                                          C.c_void_p, C.POINTER(SsOptions), C.c_int, C.c_float, f32p,
+                                          C.POINTER(SsResult)]
+        L.orc_steady_state_d8.restype = C.c_int
         L.orc_median_interior_gradient.argtypes = [C.c_int, C.c_int, C.c_double, f64p, u8p]
         L.orc_median_interior_gradient.restype = C.c_double
         L.orc_diagnose_timestep.argtypes = [C.c_int, C.c_int, C.c_double, f64p, u8p, C.c_double, C.c_double,
@@ -304,6 +308,18 @@ def picard_transient(z, mask, dx, bc, params, dt_total, channel=None):
                                None if ch is None else ch.ctypes.data_as(C.c_void_p), C.byref(params),
                                float(dt_total), zz.ravel(), C.byref(res))
     return zz, res.as_dict()
+
+
+class SsOptions(C.Structure):
+    _fields_ = [("U", C.c_float), ("K", C.c_float), ("m", C.c_float), ("n", C.c_float), ("relief_mode", C.c_int),
+                ("target_relief", C.c_float), ("baselevel_elevation", C.c_float), ("divide_area_by_dx", C.c_int),
+                ("channel_threshold_area", C.c_float)]
+
+
+class SsResult(C.Structure):
+    _fields_ = [("K_used", C.c_float), ("max_unit_height", C.c_float), ("computed_relief", C.c_float),
+                ("erodibility_field_scale", C.c_float), ("max_unit_height_at_channel_head", C.c_float),
+                ("n_channel_pixels", C.c_int64), ("chi_c_reported", C.c_int)]
 
 
 class TsDiagnosis(C.Structure):
@@ -590,3 +606,32 @@ class ConvergenceMonitor:
 
     def update(self, max_dz, dt_used):
         return lib().orc_ts_monitor_update(C.byref(self.m), float(max_dz), float(dt_used))
+
+
+def steady_state_d8(z, mask, dx, bc="oooo", U=1e-4, K=1e-4, m=0.45, n=1.0, relief_mode=False,
+                    target_relief=1000.0, baselevel=0.0, divide_area_by_dx=True, channel_threshold_area=0.0,
+                    K_field=None, libm_pow=False, nodata=-9999.0, interior_nodata_is_outlet=True,
+                    allow_pits_as_outlets=False):
+    """compute_d8_fluvial_steady_state on the oracle's flow object of z
+    (float32 surface): (z_out float32 raster, result dict)"""
+    zf = np.ascontiguousarray(z, np.float32)
+    nr, nc = zf.shape
+    mk = np.ascontiguousarray(mask, np.uint8)
+    fl = flow_build(zf.astype(np.float64), mk, dx, bc, interior_nodata_is_outlet, allow_pits_as_outlets,
+                    svector=False)
+    area = accumulate(mk, fl["receiver"], fl["topo"], dx)
+    o = SsOptions(U, K, m, n, int(relief_mode), target_relief, baselevel, int(divide_area_by_dx),
+                  channel_threshold_area)
+    r = SsResult()
+    out = np.empty((nr, nc), np.float32)
+    kf = None if K_field is None else np.ascontiguousarray(K_field, np.float32).ravel()
+    st = lib().orc_steady_state_d8(nr, nc, float(dx), mk.ravel(), fl["receiver"], fl["dir"], fl["topo"],
+                                   len(fl["topo"]), np.ascontiguousarray(area, np.float64).ravel(),
+                                   None if kf is None else kf.ctypes.data_as(C.c_void_p), C.byref(o),
+                                   int(libm_pow), float(nodata), out.ravel(), C.byref(r))
+    if st != 0:
+        raise OracleError("steady state: bad erodibility field")
+    res = {k: getattr(r, k) for k, _ in SsResult._fields_}
+    res["area"] = np.asarray(area).reshape(nr, nc)
+    res["flow"] = fl
+    return out, res
