@@ -427,6 +427,25 @@ int lsdtt_gpu_dinf_graph(const float* z, const uint8_t* mask, int32_t n_rows, in
                          const char* boundary_conditions, int32_t interior_nodata_is_outlet, double nodata,
                          int64_t* receivers, float* weights, float* slope, float* bearing, double* area);
 
+/* Analytic fluvial steady state on the D-infinity graph of z:
+ * compute_dinf_fluvial_steady_state(flow, opts)
+ * (src/cpu/lsdtt_steady_state_cpu.cpp:227-436) on the graph and area of
+ * lsdtt_gpu_dinf_graph.  Per valid cell with receivers, amn = powf(A, -m/n)
+ * w as on D8; two receivers (neither weight >= 1 - 1e-6): dchi =
+ * (chi(e2) - chi(e1)) / dx, disc = amn^2 - dchi^2, chi = chi(e1) +
+ * dx sqrt(disc), or chi(e1) + dx amn when disc < 0; one receiver:
+ * chi(r) + dx_eff amn with dx_eff = dx sqrt2 when r is one row and one
+ * column away.  Scalars, z_out and chi_c as lsdtt_gpu_flow_steady_state;
+ * nodata marks invalid input cells and fills z_out off them.  area (double
+ * raster, may be NULL) receives the accumulated area used.  For lsdtt-lem's
+ * snap with use_dinf (lsdtt_lem_cpu.cpp:830-836): lsdtt_gpu_fill, then
+ * this on the filled surface. */
+int lsdtt_gpu_dinf_steady_state(const float* z, const uint8_t* mask, int32_t n_rows, int32_t n_cols,
+                                double cell_size, const char* boundary_conditions, int32_t interior_nodata_is_outlet,
+                                double nodata, const lsdtt_gpu_steady_state_options* o,
+                                const float* erodibility_field, float* z_out, double* area,
+                                lsdtt_gpu_steady_state_result* result);
+
 /* ---------------------------------------------------------------------
  * Batched ensemble: n independent LEM instances (uneven grids, per-instance
  * options) stepped concurrently on one GPU.  No counterpart in the

@@ -105,6 +105,7 @@ def declare(L, sig):
     sig(L, "lsdtt_gpu_convergence_monitor_get", C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), pi64,
                                                            pd, pd, C.c_char_p, i32])
     sig(L, "lsdtt_gpu_dinf_graph", C.c_int, [vp, vp, i32, i32, dbl, C.c_char_p, i32, dbl, vp, vp, vp, vp, vp])
+    sig(L, "lsdtt_gpu_dinf_steady_state", C.c_int, [vp, vp, i32, i32, dbl, C.c_char_p, i32, dbl, vp, vp, vp, vp, vp])
     sig(L, "lsdtt_gpu_lem_autotune_tiles", C.c_int, [vp, vp, i64, i32, vp, vp])
     sig(L, "lsdtt_gpu_lem_elevation_device", C.c_void_p, [vp])
     sig(L, "lsdtt_gpu_lem_step", C.c_int, [vp, vp, i64, C.POINTER(LemStats)])

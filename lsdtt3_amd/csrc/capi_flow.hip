@@ -11,7 +11,7 @@
 #include "../../include/lsdtt3_gpu.h"
 #include "capi_util.hpp"
 #include "chi.hpp"
-#include "steady.hpp"
+#include "capi_steady.hpp"
 #include "flow.hpp"
 
 namespace lsdtt_gpu {
@@ -326,29 +326,14 @@ int lsdtt_gpu_flow_steady_state(lsdtt_gpu_flow* f, const lsdtt_gpu_steady_state_
         if (!f || !f->built || !o || !z_out) return fail_arg("flow object not built or null argument");
         FlowEngine& e = *f->eng;
         const long long n = e.n;
-        SteadyOptions so;
-        so.U = static_cast<float>(o->U);
-        so.K = static_cast<float>(o->K);
-        so.m = static_cast<float>(o->m);
-        so.n = static_cast<float>(o->n);
-        so.relief_mode = o->relief_mode != 0;
-        so.target_relief = static_cast<float>(o->target_relief);
-        so.baselevel_elevation = static_cast<float>(o->baselevel_elevation);
-        so.divide_area_by_dx = o->divide_area_by_dx != 0;
-        so.channel_threshold_area = static_cast<float>(o->channel_threshold_area);
+        const SteadyOptions so = steady_options(o);
         hipStream_t s = nullptr;
         dbuf<float> w, Kd, zd;
         if (erodibility_field) {
-            // make_chi_weights' checks (lsdtt_steady_state_cpu.cpp:44-66)
-            if (!(so.K > 0.0f) || !std::isfinite(so.K))
-                return fail_arg("erodibility_field: the reference K must be > 0");
             std::vector<uint8_t> m(static_cast<size_t>(n));
             LSDTT_HIP(hipMemcpy(m.data(), e.mask, n, hipMemcpyDeviceToHost));
-            for (long long i = 0; i < n; ++i)
-                if (m[static_cast<size_t>(i)] && !(erodibility_field[i] > 0.0f && std::isfinite(erodibility_field[i])))
-                    return fail_arg("erodibility_field: K must be > 0; pixel (" + std::to_string(i / f->o.n_cols) +
-                                    ", " + std::to_string(i % f->o.n_cols) + ") has K = " +
-                                    std::to_string(erodibility_field[i]));
+            const std::string err = erodibility_field_error(erodibility_field, m.data(), n, f->o.n_cols, so.K);
+            if (!err.empty()) return fail_arg(err);
             Kd.reserve(n);
             w.reserve(n);
             LSDTT_HIP(hipMemcpyAsync(Kd.p, erodibility_field, sizeof(float) * n, hipMemcpyHostToDevice, s));
@@ -358,15 +343,7 @@ int lsdtt_gpu_flow_steady_state(lsdtt_gpu_flow* f, const lsdtt_gpu_steady_state_
         const SteadyResult r =
             steady_state_d8(e, so, erodibility_field ? w.p : nullptr, static_cast<float>(nodata), zd.p, s);
         LSDTT_HIP(hipMemcpy(z_out, zd.p, sizeof(float) * n, hipMemcpyDeviceToHost));
-        if (res) {
-            res->K_used = r.K_used;
-            res->max_unit_height = r.max_unit_height;
-            res->computed_relief = r.computed_relief;
-            res->erodibility_field_scale = r.erodibility_field_scale;
-            res->max_unit_height_at_channel_head = r.max_unit_height_at_channel_head;
-            res->n_channel_pixels = r.n_channel_pixels;
-            res->chi_c_reported = r.chi_c_reported ? 1 : 0;
-        }
+        copy_steady_result(r, res);
         return LSDTT_GPU_OK;
     });
 }

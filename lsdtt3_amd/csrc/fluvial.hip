@@ -1256,6 +1256,152 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
     }
 }
 
+// ---------------------------------------------------------------------------
+// D-infinity steady-state chi (compute_dinf_fluvial_steady_state,
+// lsdtt_steady_state_cpu.cpp:282-355) by the same tile passes and up order
+// as the D-infinity stream power: a cell's chi is a function of its
+// receivers' chi, in float.  Two receivers (neither weight >= 1 - 1e-6):
+// dchi = (chi(e2) - chi(e1)) / dx, disc = amn^2 - dchi^2, chi = chi(e1) +
+// dx sqrt(disc), or chi(e1) + dx amn when disc < 0; one receiver (slot 1
+// when its weight is >= 1 - 1e-6, else slot 0): chi = chi(r) + dx_eff amn,
+// dx_eff = dx sqrt2 when r is one row and one column away (by index
+// difference, as the reference: a periodic wrap is not diagonal).
+// ---------------------------------------------------------------------------
+struct DinfChiArgs {
+    const int* rec;
+    const float* w;
+    const uint8_t* mask;
+    const float* amn;  // powf(A, -m/n) times the chi weight per valid cell
+    const short* ord;
+    const short* lstart;
+    const int* tile_maxdep;
+    const uint8_t* xdon;
+    float dx, dxd;
+    float* chi;
+};
+
+// 0: no receiver (chi stays 0); 1: two receivers; 2 / 3: one receiver in
+// slot 0 / 1, plus 4 when it is a diagonal step
+constexpr uint8_t kChiSkip = 0, kChiTwo = 1, kChiOne0 = 2, kChiOne1 = 3, kChiDiag = 4;
+
+template <int TH>
+__global__ void __launch_bounds__(WL) k_dinf_tile_chi(DinfChiArgs A, int nr, int nc, BC bc, PassArgs P)
+{
+    constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
+    if (P.prev_marked && *P.prev_marked == 0) return;
+    const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x;
+    const int tile = ty * ntx + tx;
+    const int lane = threadIdx.x;
+    if (lane == 0) P.act_clear[tile] = 0;
+    if (!P.act[tile]) return;
+    __shared__ float val[TN];
+    __shared__ float amn[TN];
+    __shared__ short lr[2][TN];  // local receivers, -1 outside the tile or none
+    __shared__ uint8_t kind[TN];
+    __shared__ short ord[TN];
+    __shared__ float cx[128][2];  // ring cells: chi of receivers in other tiles
+    const int r0 = ty * TH, c0 = tx * TW;
+    const int h = min(TH, nr - r0), wd = min(TW, nc - c0);
+    bool valid[PER];
+    uint8_t x[PER];
+    float ccur[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+        const uint8_t m = A.mask[q.f];
+        const int g0 = A.rec[2 * (size_t)q.f], g1 = A.rec[2 * (size_t)q.f + 1];
+        const float w0 = A.w[2 * (size_t)q.f], w1 = A.w[2 * (size_t)q.f + 1];
+        x[k] = A.xdon[q.f];
+        ccur[k] = A.chi[q.f];
+        valid[k] = q.in && m;
+        const int nrec = (g0 >= 0) + (g1 >= 0);
+        uint8_t kd = kChiSkip;
+        if (valid[k] && nrec > 0) {
+            const bool one1 = nrec > 1 && w1 >= 1.0f - 1e-6f;
+            if (nrec > 1 && !(w0 >= 1.0f - 1e-6f) && !one1) {
+                kd = kChiTwo;
+            } else {
+                const int g = one1 ? g1 : g0;
+                const int dr = abs(g / nc - q.r), dc = abs(g % nc - q.c);
+                kd = static_cast<uint8_t>((one1 ? kChiOne1 : kChiOne0) | ((dr == 1 && dc == 1) ? kChiDiag : 0));
+            }
+        }
+        const int i = lane + WL * k;
+        val[i] = ccur[k];
+        amn[i] = A.amn[q.f];
+        kind[i] = kd;
+        lr[0][i] = static_cast<short>((valid[k] && g0 >= 0) ? local_index<TH>(g0, nc, r0, c0) : -1);
+        lr[1][i] = static_cast<short>((valid[k] && g1 >= 0) ? local_index<TH>(g1, nc, r0, c0) : -1);
+        ord[i] = A.ord[(size_t)tile * TN + i];
+    }
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        const int sl = lane + WL * half;
+        int lrw, lcl;
+        const bool ok = slot_cell(sl, h, wd, &lrw, &lcl);
+        const int f = ok ? (r0 + lrw) * nc + c0 + lcl : 0;
+        const int g0 = A.rec[2 * (size_t)f], g1 = A.rec[2 * (size_t)f + 1];
+        const bool x0 = ok && g0 >= 0 && local_index<TH>(g0, nc, r0, c0) < 0;
+        const bool x1 = ok && g1 >= 0 && local_index<TH>(g1, nc, r0, c0) < 0;
+        cx[sl][0] = x0 ? __hip_atomic_load(&A.chi[g0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0f;
+        cx[sl][1] = x1 ? __hip_atomic_load(&A.chi[g1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0f;
+    }
+    const short* __restrict__ lsg = A.lstart + (size_t)tile * (TN + 1);
+    const int maxdep = A.tile_maxdep[tile];
+    int lb = lsg[0], le = lsg[1];
+    __syncthreads();
+    const float dx = A.dx, dxd = A.dxd;
+    for (int L = 0; L <= maxdep; ++L) {
+        const int lnext = L + 2 <= TN ? lsg[min(L + 2, TN)] : le;
+        for (int q = lb + lane; q < le; q += WL) {
+            const int i = ord[q];
+            const uint8_t kd = kind[i];
+            if (kd == kChiSkip) continue;
+            const int sl = ring_slot(i / TW, i % TW, h, wd);
+            float c[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int l = lr[j][i];
+                c[j] = l >= 0 ? val[l] : (sl >= 0 ? cx[sl][j] : 0.0f);
+            }
+            const float a = amn[i];
+            float cn;
+            if (kd == kChiTwo) {
+                const float dchi = (c[1] - c[0]) / dx;
+                const float disc = a * a - dchi * dchi;
+                cn = disc >= 0.0f ? c[0] + dx * sqrtf(disc) : c[0] + dx * a;
+            } else {
+                const float dxe = (kd & kChiDiag) ? dxd : dx;
+                cn = ((kd & 3) == kChiOne1 ? c[1] : c[0]) + dxe * a;
+            }
+            val[i] = cn;
+        }
+        lb = le;
+        le = lnext;
+        __syncthreads();
+    }
+    bool marked = false;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (!valid[k]) continue;
+        const int i = lane + WL * k;
+        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+        const float v = val[i];
+        if (__float_as_uint(ccur[k]) != __float_as_uint(v)) {
+            __hip_atomic_store(&A.chi[q.f], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (unsigned b = x[k]; b; b &= b - 1) {
+                P.act_next[tile_of<TH>(neighbour(q.r, q.c, __builtin_ctz(b), nr, nc, bc), nc, ntx)] = 1;
+                marked = true;
+            }
+        }
+    }
+    const bool wmarked = __ballot(marked) != 0;
+    if (lane == 0) {
+        if (wmarked) *P.marked = 1;
+        P.runs[tile] += 1;
+    }
+}
+
 // smooth_drainage_area (lsdtt_stream_power_cpu.cpp:34-139), row pass: one
 // thread per row keeps the reference's rolling sum (window -r..r, then per
 // column: record, subtract column c - r, add column c + r + 1)
@@ -1605,6 +1751,32 @@ int FluvialTiles::dinf_stream_power(const int* rec, const float* w, const uint8_
         s);
     last_sp_passes = passes;
     std::memcpy(eroded, flags_h.data(), sizeof(long long));
+    return passes;
+}
+
+int FluvialTiles::dinf_chi(const int* rec, const float* w, const uint8_t* mask, const float* amn, float dx,
+                           float* chi, hipStream_t s)
+{
+    // chi is the unique solution of the receiver recursion; starting from
+    // zeros, the passes reach it whatever their schedule
+    const long long n = (long long)nr * nc;
+    k_fill_i32<<<div_up(n, 256), 256, 0, s>>>(reinterpret_cast<int*>(chi), n, 0);
+    LSDTT_LAUNCH_CHECK();
+    DinfChiArgs A{rec, w, mask, amn, ord.p, lstart.p, tile_maxdep.p, xdon.p, dx, dx * static_cast<float>(kSqrt2),
+                  chi};
+    last_passes = last_chi_passes;
+    // algorithmic bytes of the sweep, counted once: rec 8, w 8, mask 1,
+    // amn 4 in; chi 4 out
+    const int passes = settle(
+        "dinf_chi", 25.0 * n,
+        [&](const PassArgs& P) {
+            with_tile_height(th, [&](auto H) {
+                k_dinf_tile_chi<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(A, nr, nc, bc_, P);
+            });
+            LSDTT_LAUNCH_CHECK();
+        },
+        [] {}, s);
+    last_chi_passes = passes;
     return passes;
 }
 

@@ -70,6 +70,11 @@ public:
     int dinf_stream_power(const int* rec, const float* w, const uint8_t* mask, const double* area,
                           const double* z_old, const double* K, const SpParams& sp, double* z, long long* eroded,
                           hipStream_t s);
+    // steady-state chi of the D-infinity graph after dinf_up_order (amn:
+    // powf(A, -m/n) times the chi weight per valid cell; chi float[n],
+    // 0 where no receiver leads out); returns passes used
+    int dinf_chi(const int* rec, const float* w, const uint8_t* mask, const float* amn, float dx, float* chi,
+                 hipStream_t s);
     // smooth_drainage_area of count * pixel_area: rolling row sums, then
     // rolling column sums, in the reference's order (one thread per row,
     // then per column), so the averages are bit-identical
@@ -92,7 +97,7 @@ public:
 private:
     int th = 16;
     // passes of the last call of each sweep: the next call's first batch
-    int last_passes = 0, last_count_passes = 0, last_sp_passes = 0, last_dinf_passes = 0;
+    int last_passes = 0, last_count_passes = 0, last_sp_passes = 0, last_dinf_passes = 0, last_chi_passes = 0;
     // runs `pass(PassArgs)` until a pass marks no tile
     template <class Pass, class After>
     int settle(const char* name, double first_pass_bytes, Pass&& pass, After&& after_batch, hipStream_t s);

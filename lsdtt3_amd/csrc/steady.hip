@@ -12,6 +12,9 @@
 //     U (MaxChi / target_relief)^n); z = baselevel + f * chi
 //   compute_d8_fluvial_steady_state :173-225  dx_eff = dx or dx * sqrt2f
 //     (float) by the D8 code's parity
+//   compute_dinf_fluvial_steady_state :227-436  the same amn on the
+//     D-infinity graph's accumulate(); chi from one or two receivers
+//     (FluvialTiles::dinf_chi)
 // powf is evaluated as the correctly rounded value (pow in double, rounded
 // to float); the reference's powf returns it or one ulp beside it,
 // depending on the C library.
@@ -138,6 +141,58 @@ __global__ void k_chi_weights(const uint8_t* __restrict__ mask, const float* __r
         w[i] = mask[i] ? static_cast<float>(pow(static_cast<double>(Kref / K[i]), static_cast<double>(inv_n))) : 1.0f;
 }
 
+// steps 2-4 of both paths (:106-170, :358-436): max chi over valid cells
+// (1 if not positive), f and K_used with the host's float pow, elevations,
+// chi_c over the channel pixels
+SteadyResult finish(const uint8_t* mask, const float* chi, const double* area, long long n, float dx,
+                    const SteadyOptions& o, const float* w, float nodata, float* z_out, hipStream_t s)
+{
+    const int g = div_up(n, kB);
+    dbuf<unsigned> mx;
+    dbuf<unsigned long long> nch;
+    mx.reserve(2);
+    nch.reserve(1);
+    LSDTT_HIP(hipMemsetAsync(mx.p, 0, sizeof(unsigned) * 2, s));
+    LSDTT_HIP(hipMemsetAsync(nch.p, 0, sizeof(unsigned long long), s));
+    k_chi_max<<<std::min(g, 2048), kB, 0, s>>>(mask, chi, area, n, static_cast<double>(dx),
+                                               o.divide_area_by_dx ? 1 : 0,
+                                               static_cast<double>(o.channel_threshold_area), mx.p, nch.p);
+    LSDTT_LAUNCH_CHECK();
+    unsigned hm[2];
+    unsigned long long hn = 0;
+    LSDTT_HIP(hipMemcpyAsync(hm, mx.p, sizeof hm, hipMemcpyDeviceToHost, s));
+    LSDTT_HIP(hipMemcpyAsync(&hn, nch.p, sizeof hn, hipMemcpyDeviceToHost, s));
+    LSDTT_HIP(hipStreamSynchronize(s));
+    float max_chi, chi_c;
+    std::memcpy(&max_chi, &hm[0], sizeof max_chi);
+    std::memcpy(&chi_c, &hm[1], sizeof chi_c);
+    if (max_chi <= 0.0f) max_chi = 1.0f;
+    // step 3 (:112-123): scalars, with the host's float pow as the reference
+    SteadyResult r;
+    float f;
+    if (o.relief_mode) {
+        r.K_used = o.U * std::pow(max_chi / o.target_relief, o.n);
+        f = o.target_relief / max_chi;
+    } else {
+        r.K_used = o.K;
+        f = std::pow(o.U / o.K, 1.0f / o.n);
+    }
+    k_steady_z<<<g, kB, 0, s>>>(mask, chi, n, o.baselevel_elevation, f, nodata, z_out);
+    LSDTT_LAUNCH_CHECK();
+    LSDTT_HIP(hipStreamSynchronize(s));
+    r.max_unit_height = max_chi;
+    r.computed_relief = f * max_chi;
+    if (w) r.erodibility_field_scale = r.K_used / o.K;
+    if (o.channel_threshold_area > 0.0f) {
+        r.n_channel_pixels = static_cast<long long>(hn);
+        if (hn > 0) {
+            r.max_unit_height_at_channel_head = chi_c;
+            r.chi_c_reported = true;
+        }
+    }
+    return r;
+}
+
 }  // namespace
 
 void chi_weights(const uint8_t* mask, const float* K, long long n, float Kref, float nexp, float* w, hipStream_t s)
@@ -179,49 +234,23 @@ SteadyResult steady_state_d8(FlowEngine& e, const SteadyOptions& o, const float*
         k_sweep_tail<<<1, 1024, 0, s>>>(e.order.p, e.level_start.p, split, nl - 1, a);
         LSDTT_LAUNCH_CHECK();
     }
-    dbuf<unsigned> mx;
-    dbuf<unsigned long long> nch;
-    mx.reserve(2);
-    nch.reserve(1);
-    LSDTT_HIP(hipMemsetAsync(mx.p, 0, sizeof(unsigned) * 2, s));
-    LSDTT_HIP(hipMemsetAsync(nch.p, 0, sizeof(unsigned long long), s));
-    k_chi_max<<<std::min(g, 2048), kB, 0, s>>>(e.mask, chi.p, area.p, n, static_cast<double>(dx),
-                                               o.divide_area_by_dx ? 1 : 0,
-                                               static_cast<double>(o.channel_threshold_area), mx.p, nch.p);
+    return finish(e.mask, chi.p, area.p, n, dx, o, w, nodata, z_out, s);
+}
+
+SteadyResult steady_state_dinf(FluvialTiles& t, const int* rec, const float* wr, const uint8_t* valid,
+                               const double* area, const SteadyOptions& o, const float* w, float dx, float nodata,
+                               float* z_out, hipStream_t s)
+{
+    const long long n = (long long)t.nr * t.nc;
+    dbuf<float> amn, chi;
+    amn.reserve(n);
+    chi.reserve(n);
+    const int g = div_up(n, kB);
+    k_amn<<<g, kB, 0, s>>>(valid, area, w, n, dx, o.divide_area_by_dx ? 1 : 0, -o.m / o.n, amn.p, chi.p);
     LSDTT_LAUNCH_CHECK();
-    unsigned hm[2];
-    unsigned long long hn = 0;
-    LSDTT_HIP(hipMemcpyAsync(hm, mx.p, sizeof hm, hipMemcpyDeviceToHost, s));
-    LSDTT_HIP(hipMemcpyAsync(&hn, nch.p, sizeof hn, hipMemcpyDeviceToHost, s));
-    LSDTT_HIP(hipStreamSynchronize(s));
-    float max_chi, chi_c;
-    std::memcpy(&max_chi, &hm[0], sizeof max_chi);
-    std::memcpy(&chi_c, &hm[1], sizeof chi_c);
-    if (max_chi <= 0.0f) max_chi = 1.0f;
-    // step 3 (:112-123): scalars, with the host's float pow as the reference
-    SteadyResult r;
-    float f;
-    if (o.relief_mode) {
-        r.K_used = o.U * std::pow(max_chi / o.target_relief, o.n);
-        f = o.target_relief / max_chi;
-    } else {
-        r.K_used = o.K;
-        f = std::pow(o.U / o.K, 1.0f / o.n);
-    }
-    k_steady_z<<<g, kB, 0, s>>>(e.mask, chi.p, n, o.baselevel_elevation, f, nodata, z_out);
-    LSDTT_LAUNCH_CHECK();
-    LSDTT_HIP(hipStreamSynchronize(s));
-    r.max_unit_height = max_chi;
-    r.computed_relief = f * max_chi;
-    if (w) r.erodibility_field_scale = r.K_used / o.K;
-    if (o.channel_threshold_area > 0.0f) {
-        r.n_channel_pixels = static_cast<long long>(hn);
-        if (hn > 0) {
-            r.max_unit_height_at_channel_head = chi_c;
-            r.chi_c_reported = true;
-        }
-    }
-    return r;
+    t.dinf_up_order(rec, valid, s);
+    t.dinf_chi(rec, wr, valid, amn.p, dx, chi.p, s);
+    return finish(valid, chi.p, area, n, dx, o, w, nodata, z_out, s);
 }
 
 }  // namespace lsdtt_gpu

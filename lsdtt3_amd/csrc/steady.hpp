@@ -1,9 +1,11 @@
 // steady.hpp -- analytic fluvial steady state on the D8 flow object
 // (compute_d8_fluvial_steady_state, src/cpu/lsdtt_steady_state_cpu.cpp:
-// 76-225): chi by a sweep from the outlets, z = baselevel + f chi.
+// 76-225) and on the D-infinity graph (compute_dinf_fluvial_steady_state,
+// :227-436): chi by a sweep from the outlets, z = baselevel + f chi.
 #pragma once
 
 #include "flow.hpp"
+#include "fluvial.hpp"
 
 namespace lsdtt_gpu {
 
@@ -33,5 +35,13 @@ void chi_weights(const uint8_t* mask, const float* K, long long n, float Kref, f
 // w: per-cell chi weights (K / K_i)^(1/n) on the device, or null.
 SteadyResult steady_state_d8(FlowEngine& e, const SteadyOptions& o, const float* w, float nodata, float* z_out,
                              hipStream_t s);
+
+// On a D-infinity graph (rec / wr: two receivers and weights per cell, -1 =
+// none) of the valid cells `valid` with its accumulated area (m^2 per
+// cell): chi by the tile passes of t (sized to the grid and its boundary
+// conditions; its order buffers are overwritten), then as steady_state_d8.
+SteadyResult steady_state_dinf(FluvialTiles& t, const int* rec, const float* wr, const uint8_t* valid,
+                               const double* area, const SteadyOptions& o, const float* w, float dx, float nodata,
+                               float* z_out, hipStream_t s);
 
 }  // namespace lsdtt_gpu

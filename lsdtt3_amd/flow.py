@@ -156,23 +156,14 @@ class FlowInfo:
         (src/cpu/lsdtt_steady_state_cpu.cpp:173-225): (float32 elevation
         raster, result dict with K_used, max_unit_height, computed_relief,
         erodibility_field_scale, chi_c fields)"""
-        L = N.lib()
-        o = N.SteadyOptions()
-        L.lsdtt_gpu_steady_state_options_init(C.byref(o))
-        o.U, o.K, o.m, o.n = float(U), float(K), float(m), float(n)
-        o.relief_mode = int(relief_mode)
-        o.target_relief = float(target_relief)
-        o.baselevel_elevation = float(baselevel_elevation)
-        o.divide_area_by_dx = int(divide_area_by_dx)
-        o.channel_threshold_area = float(channel_threshold_area)
-        kf = None if erodibility_field is None else np.ascontiguousarray(erodibility_field, np.float32)
+        o = _steady_options(U, K, m, n, relief_mode, target_relief, baselevel_elevation, divide_area_by_dx,
+                            channel_threshold_area)
+        kf = _erodibility(erodibility_field, self.n)
         out = np.empty((self.n_rows, self.n_cols), np.float32)
         r = N.SteadyResult()
-        N.check(L.lsdtt_gpu_flow_steady_state(self._h, C.byref(o), N.ptr(kf), float(no_data_value), N.ptr(out),
-                                              C.byref(r)))
-        res = {k: getattr(r, k) for k, _ in N.SteadyResult._fields_}
-        res["chi_c_reported"] = bool(res["chi_c_reported"])
-        return out, res
+        N.check(N.lib().lsdtt_gpu_flow_steady_state(self._h, C.byref(o), N.ptr(kf), float(no_data_value),
+                                                    N.ptr(out), C.byref(r)))
+        return out, _steady_result(r)
 
     def get_chi_raster(self, m_over_n=0.45, A_0=1.0, no_data_value=-9999.0):
         """lsdtt::get_chi_raster(flow, {m_over_n, A_0})
@@ -207,6 +198,34 @@ class FlowInfo:
                 for i in range(b)]
 
 
+def _steady_options(U, K, m, n, relief_mode, target_relief, baselevel_elevation, divide_area_by_dx,
+                    channel_threshold_area):
+    o = N.SteadyOptions()
+    N.lib().lsdtt_gpu_steady_state_options_init(C.byref(o))
+    o.U, o.K, o.m, o.n = float(U), float(K), float(m), float(n)
+    o.relief_mode = int(relief_mode)
+    o.target_relief = float(target_relief)
+    o.baselevel_elevation = float(baselevel_elevation)
+    o.divide_area_by_dx = int(divide_area_by_dx)
+    o.channel_threshold_area = float(channel_threshold_area)
+    return o
+
+
+def _erodibility(field, n):
+    if field is None:
+        return None
+    kf = np.ascontiguousarray(field, np.float32)
+    if kf.size != n:
+        raise ValueError("erodibility_field size must equal n_rows * n_cols")
+    return kf
+
+
+def _steady_result(r):
+    res = {k: getattr(r, k) for k, _ in N.SteadyResult._fields_}
+    res["chi_c_reported"] = bool(res["chi_c_reported"])
+    return res
+
+
 class FlowGraph:
     """D-infinity flow graph on the GPU, mirroring the reference's Python
     ``FlowGraph(dem, method="dinf")`` (python_bindings/bindings/flow_graph.cpp:
@@ -234,14 +253,39 @@ class FlowGraph:
         self.slope = np.empty(z.shape, np.float32)
         self.bearing = np.empty(z.shape, np.float32)
         self._area = np.empty(z.shape, np.float64)
+        self._z, self._m = z, m
+        self._bc = boundary_conditions.encode()[:4]
+        self._ino = int(interior_nodata_is_outlet)
+        self.no_data_value = float(no_data_value)
         N.check(N.lib().lsdtt_gpu_dinf_graph(N.ptr(z), N.ptr(m), self.n_rows, self.n_cols, self.cell_size,
-                                             boundary_conditions.encode()[:4], int(interior_nodata_is_outlet),
-                                             float(no_data_value), N.ptr(self.receivers), N.ptr(self.weights),
-                                             N.ptr(self.slope), N.ptr(self.bearing), N.ptr(self._area)))
+                                             self._bc, self._ino, self.no_data_value, N.ptr(self.receivers),
+                                             N.ptr(self.weights), N.ptr(self.slope), N.ptr(self.bearing),
+                                             N.ptr(self._area)))
 
     def accumulate(self):
         """drainage area (m^2) in float64, 0 off the valid cells"""
         return self._area.copy()
+
+    def fluvial_steady_state(self, U=1e-4, K=1e-4, m=0.45, n=1.0, relief_mode=False, target_relief=1000.0,
+                             baselevel_elevation=0.0, divide_area_by_dx=True, channel_threshold_area=0.0,
+                             erodibility_field=None):
+        """compute_dinf_fluvial_steady_state(flow, opts)
+        (src/cpu/lsdtt_steady_state_cpu.cpp:227-436) on this graph's
+        surface: (float32 elevation raster, result dict as
+        FlowInfo.fluvial_steady_state's; z is no_data_value off the valid
+        cells)"""
+        o = _steady_options(U, K, m, n, relief_mode, target_relief, baselevel_elevation, divide_area_by_dx,
+                            channel_threshold_area)
+        kf = _erodibility(erodibility_field, self.n_rows * self.n_cols)
+        out = np.empty((self.n_rows, self.n_cols), np.float32)
+        area = np.empty((self.n_rows, self.n_cols), np.float64)
+        r = N.SteadyResult()
+        N.check(N.lib().lsdtt_gpu_dinf_steady_state(N.ptr(self._z), N.ptr(self._m), self.n_rows, self.n_cols,
+                                                    self.cell_size, self._bc, self._ino, self.no_data_value,
+                                                    C.byref(o), N.ptr(kf), N.ptr(out), N.ptr(area), C.byref(r)))
+        res = _steady_result(r)
+        res["area"] = area
+        return out, res
 
     def receivers_of(self, row, col):
         f = int(row) * self.n_cols + int(col)

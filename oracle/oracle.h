@@ -175,6 +175,13 @@ int orc_steady_state_d8(int nr, int nc, double cell_size, const uint8_t* valid, 
                         const float* K_field, const orc_ss_options* o, int libm_pow, float nodata, float* z_out,
                         orc_ss_result* res);
 
+/* compute_dinf_fluvial_steady_state on the oracle's D-inf arrays (graph,
+ * Kahn order and accumulated area of orc_dinf.c) */
+int orc_steady_state_dinf(int nr, int nc, double cell_size, const uint8_t* valid, const int64_t* rec, const float* w,
+                          const uint8_t* nrec, const int64_t* topo, int64_t n_topo, const double* area,
+                          const float* K_field, const orc_ss_options* o, int libm_pow, float nodata, float* z_out,
+                          orc_ss_result* res);
+
 /* ---- chi (oracle/orc_chi.c) ------------------------------------------ */
 
 /* get_chi_raster: chi from every outlet (seed 0), A = s_size * dx^2; chi is

@@ -102,6 +102,10 @@ def lib():
                                           C.c_void_p, C.POINTER(SsOptions), C.c_int, C.c_float, f32p,
                                           C.POINTER(SsResult)]
         L.orc_steady_state_d8.restype = C.c_int
+        L.orc_steady_state_dinf.argtypes = [C.c_int, C.c_int, C.c_double, u8p, i64p, f32p, u8p, i64p, C.c_int64,
+                                            f64p, C.c_void_p, C.POINTER(SsOptions), C.c_int, C.c_float, f32p,
+                                            C.POINTER(SsResult)]
+        L.orc_steady_state_dinf.restype = C.c_int
         L.orc_median_interior_gradient.argtypes = [C.c_int, C.c_int, C.c_double, f64p, u8p]
         L.orc_median_interior_gradient.restype = C.c_double
         L.orc_diagnose_timestep.argtypes = [C.c_int, C.c_int, C.c_double, f64p, u8p, C.c_double, C.c_double,
@@ -634,4 +638,38 @@ def steady_state_d8(z, mask, dx, bc="oooo", U=1e-4, K=1e-4, m=0.45, n=1.0, relie
     res = {k: getattr(r, k) for k, _ in SsResult._fields_}
     res["area"] = np.asarray(area).reshape(nr, nc)
     res["flow"] = fl
+    return out, res
+
+
+def steady_state_dinf(z, mask, dx, bc="oooo", U=1e-4, K=1e-4, m=0.45, n=1.0, relief_mode=False,
+                      target_relief=1000.0, baselevel=0.0, divide_area_by_dx=True, channel_threshold_area=0.0,
+                      K_field=None, libm_pow=False, libm_atan2=False, nodata=-9999.0, interior_nodata_is_outlet=True,
+                      area=None):
+    """compute_dinf_fluvial_steady_state on the oracle's D-inf graph of z
+    (float32): (z_out float32 raster, result dict with graph and area).
+    area: accumulated areas to use in place of the oracle's own (the chi
+    sweep is then checked alone)"""
+    zf = np.ascontiguousarray(z, np.float32)
+    nr, nc = zf.shape
+    mk = np.ascontiguousarray(mask, np.uint8)
+    g = dinf_graph(zf, mk, dx, bc=bc, nodata=nodata, interior_nodata_is_outlet=interior_nodata_is_outlet,
+                   libm_atan2=libm_atan2)
+    valid = ((mk != 0) & np.isfinite(zf) & (zf != np.float32(nodata))).astype(np.uint8)
+    own, topo = dinf_accumulate(g, valid, dx)
+    area = own if area is None else np.ascontiguousarray(area, np.float64).ravel()
+    o = SsOptions(U, K, m, n, int(relief_mode), target_relief, baselevel, int(divide_area_by_dx),
+                  channel_threshold_area)
+    r = SsResult()
+    out = np.empty((nr, nc), np.float32)
+    kf = None if K_field is None else np.ascontiguousarray(K_field, np.float32).ravel()
+    st = lib().orc_steady_state_dinf(nr, nc, float(dx), valid.ravel(), g["rec"].ravel(), g["w"].ravel(), g["nrec"],
+                                     np.ascontiguousarray(topo, np.int64), len(topo), area,
+                                     None if kf is None else kf.ctypes.data_as(C.c_void_p), C.byref(o),
+                                     int(libm_pow), float(nodata), out.ravel(), C.byref(r))
+    if st != 0:
+        raise OracleError("steady state: bad erodibility field")
+    res = {k: getattr(r, k) for k, _ in SsResult._fields_}
+    res["area"] = area.reshape(nr, nc)
+    res["graph"] = g
+    res["valid"] = valid
     return out, res
