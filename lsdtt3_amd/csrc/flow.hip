@@ -52,113 +52,195 @@ __device__ __forceinline__ void st_relaxed(unsigned long long* p, unsigned long 
 }
 
 // ---------------------------------------------------------------------------
-// receivers: one thread per cell, rows along x so a wavefront reads 64
-// consecutive cells of a row
+// receivers.  Interior cells (not on the grid's outer rows or columns):
+// a thread walks kRecvRows consecutive rows of one column, rows along x so
+// a wavefront reads 64 consecutive cells of a row; the rows r0 - 1 ..
+// r0 + kRecvRows of the columns c - 1, c, c + 1 are all loaded before the
+// first cell is decided (3 z and 3 mask loads per cell instead of 9 + 9,
+// all in flight at once).  The outer ring, where steps wrap or leave the
+// grid, is a second small launch; keeping its general path out of the
+// interior kernel keeps that kernel's code small.
 // ---------------------------------------------------------------------------
+constexpr int kRecvRows = 4;
+
+__device__ __forceinline__ void write_receiver(int f, int r, int d, int* __restrict__ rcv, uint8_t* __restrict__ dir)
+{
+    rcv[f] = r;
+    dir[f] = static_cast<uint8_t>(d);
+}
+
+// the steepest descent of an interior cell from its 8 neighbours (D8 order
+// N, NE, E, SE, S, SW, W, NW), the first of equal slopes winning
+template <class T>
+__device__ __forceinline__ void decide_interior(const uint8_t* __restrict__ forced, int nc, int row, int col,
+                                                double card, double diag, int nodata_outlet, int pits_outlet,
+                                                uint8_t mc, T zcell, const uint8_t (&mk)[8], const T (&zn)[8],
+                                                int* __restrict__ rcv, uint8_t* __restrict__ dir,
+                                                int* __restrict__ fail)
+{
+    const int f = row * nc + col;
+    if (!mc) {
+        write_receiver(f, kNodataReceiver, kNoDirection, rcv, dir);
+        return;
+    }
+    if (forced && forced[f]) {
+        write_receiver(f, kOutletReceiver, kNoDirection, rcv, dir);
+        return;
+    }
+    constexpr int kdr[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
+    constexpr int kdc[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+    const double zc = static_cast<double>(zcell);
+    double best = 0.0;
+    int bdir = kNoDirection;
+    bool allowed = false;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        if (!mk[d]) {
+            allowed |= (nodata_outlet != 0);
+            continue;
+        }
+        const double dz = zc - static_cast<double>(zn[d]);
+        if (dz <= 0.0) continue;
+        const double sl = dz / ((d & 1) ? diag : card);
+        if (sl > best) {
+            best = sl;
+            bdir = d;
+        }
+    }
+    if (bdir != kNoDirection) {
+        write_receiver(f, f + kdr[bdir] * nc + kdc[bdir], bdir, rcv, dir);
+    } else {
+        write_receiver(f, kOutletReceiver, kNoDirection, rcv, dir);
+        if (!allowed && !pits_outlet) atomicMin(fail, f);
+    }
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) k_receivers(const T* __restrict__ z, const uint8_t* __restrict__ mask,
                                                    const uint8_t* __restrict__ forced, int nr, int nc,
-                                                   double card, double diag, uint32_t bcw, int nodata_outlet,
-                                                   int pits_outlet, int* __restrict__ rcv,
-                                                   uint8_t* __restrict__ dir, int* __restrict__ fail)
+                                                   double card, double diag, int nodata_outlet, int pits_outlet,
+                                                   int* __restrict__ rcv, uint8_t* __restrict__ dir,
+                                                   int* __restrict__ fail)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r0 = (blockIdx.y * blockDim.y + threadIdx.y) * kRecvRows;
+    if (col >= nc || r0 >= nr) return;
+    // the window: rows r0 - 1 .. r0 + kRecvRows, clamped into the grid (the
+    // clamped values belong to ring cells, which this kernel skips)
+    const int cm = col > 0 ? col - 1 : col, cp = col < nc - 1 ? col + 1 : col;
+    T zw[kRecvRows + 2][3];
+    uint8_t mw[kRecvRows + 2][3];
+#pragma unroll
+    for (int k = 0; k < kRecvRows + 2; ++k) {
+        const int rr = min(max(r0 - 1 + k, 0), nr - 1);
+        const size_t b = (size_t)rr * nc;
+        zw[k][0] = z[b + cm];
+        zw[k][1] = z[b + col];
+        zw[k][2] = z[b + cp];
+        mw[k][0] = mask[b + cm];
+        mw[k][1] = mask[b + col];
+        mw[k][2] = mask[b + cp];
+    }
+    if (col == 0 || col == nc - 1) return;
+#pragma unroll
+    for (int k = 1; k <= kRecvRows; ++k) {
+        const int row = r0 - 1 + k;
+        if (row >= nr - 1) break;
+        if (row == 0) continue;
+        const uint8_t mk[8] = {mw[k - 1][1], mw[k - 1][2], mw[k][2], mw[k + 1][2],
+                               mw[k + 1][1], mw[k + 1][0], mw[k][0], mw[k - 1][0]};
+        const T zn[8] = {zw[k - 1][1], zw[k - 1][2], zw[k][2], zw[k + 1][2],
+                         zw[k + 1][1], zw[k + 1][0], zw[k][0], zw[k - 1][0]};
+        decide_interior<T>(forced, nc, row, col, card, diag, nodata_outlet, pits_outlet, mw[k][1], zw[k][1], mk, zn,
+                           rcv, dir, fail);
+    }
+}
+
+// the outer ring (2 nc + 2 (nr - 2) cells, fewer on 1- and 2-wide grids):
+// every step is resolved against the boundary conditions
+template <class T>
+__global__ void __launch_bounds__(256) k_receivers_ring(const T* __restrict__ z, const uint8_t* __restrict__ mask,
+                                                        const uint8_t* __restrict__ forced, int nr, int nc,
+                                                        double card, double diag, uint32_t bcw, int nodata_outlet,
+                                                        int pits_outlet, int* __restrict__ rcv,
+                                                        uint8_t* __restrict__ dir, int* __restrict__ fail)
 {
     const BC bc = bc_unpack(bcw);
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    const int row = blockIdx.y * blockDim.y + threadIdx.y;
-    if (row >= nr || col >= nc) return;
-    const int f = row * nc + col;
-    // every load of the cell is issued before the first branch (neighbour
-    // indices clamped into the grid; edge cells take the general path
-    // below and ignore them), so their latencies overlap instead of each
-    // branch waiting for the load it tests
-    const bool interior = row > 0 && row < nr - 1 && col > 0 && col < nc - 1;
-    uint8_t mk[8];
-    T zn[8];
-    {
-        const int rm = row > 0 ? row - 1 : row, rp = row < nr - 1 ? row + 1 : row;
-        const int cm = col > 0 ? col - 1 : col, cp = col < nc - 1 ? col + 1 : col;
-        // the D8 offsets as compile-time constants (the __constant__
-        // tables would be read from memory)
-        constexpr int kdr[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
-        constexpr int kdc[8] = {0, 1, 1, 1, 0, -1, -1, -1};
-#pragma unroll
-        for (int d = 0; d < 8; ++d) {
-            const int rr = kdr[d] < 0 ? rm : (kdr[d] > 0 ? rp : row);
-            const int cc = kdc[d] < 0 ? cm : (kdc[d] > 0 ? cp : col);
-            const int g = rr * nc + cc;
-            mk[d] = mask[g];
-            zn[d] = z[g];
-        }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // top row, bottom row (when nr > 1), then the side columns of rows 1 .. nr - 2
+    const int nbottom = nr > 1 ? nc : 0;
+    const int nside = nr > 2 ? (nc > 1 ? 2 : 1) * (nr - 2) : 0;
+    int row, col;
+    if (i < nc) {
+        row = 0, col = i;
+    } else if (i < nc + nbottom) {
+        row = nr - 1, col = i - nc;
+    } else if (i < nc + nbottom + nside) {
+        const int k = i - nc - nbottom;
+        const int per = nc > 1 ? 2 : 1;
+        row = 1 + k / per;
+        col = (k % per) ? nc - 1 : 0;
+    } else {
+        return;
     }
-    const uint8_t mc = mask[f];
-    const uint8_t fc = forced ? forced[f] : 0;
-    const double zc = static_cast<double>(z[f]);
-    if (!mc) {
-        rcv[f] = kNodataReceiver;
-        dir[f] = kNoDirection;
+    const int f = row * nc + col;
+    if (!mask[f]) {
+        write_receiver(f, kNodataReceiver, kNoDirection, rcv, dir);
         return;
     }
     const bool bedge = (row == 0 && bc.n == 'b') || (col == nc - 1 && bc.e == 'b') ||
                        (row == nr - 1 && bc.s == 'b') || (col == 0 && bc.w == 'b');
-    if (bedge || fc) {
-        rcv[f] = kOutletReceiver;
-        dir[f] = kNoDirection;
+    if (bedge || (forced && forced[f])) {
+        write_receiver(f, kOutletReceiver, kNoDirection, rcv, dir);
         return;
     }
+    const double zc = static_cast<double>(z[f]);
     double best = 0.0;
     int brcv = kOutletReceiver, bdir = kNoDirection;
     bool allowed = false;
-    if (interior) {
-        constexpr int kdr[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
-        constexpr int kdc[8] = {0, 1, 1, 1, 0, -1, -1, -1};
-#pragma unroll
-        for (int d = 0; d < 8; ++d) {
-            const int g = f + kdr[d] * nc + kdc[d];
-            if (!mk[d]) {
-                allowed |= (nodata_outlet != 0);
-                continue;
-            }
-            const double dz = zc - static_cast<double>(zn[d]);
-            if (dz <= 0.0) continue;
-            const double s = dz / ((d & 1) ? diag : card);
-            if (s > best) {
-                best = s;
-                brcv = g;
-                bdir = d;
-            }
+    for (int d = 0; d < 8; ++d) {
+        const Step st = resolve_step(row, col, d, nr, nc, bc);
+        if (st.kind == 2) {
+            allowed = true;
+            continue;
         }
-    } else {
-        for (int d = 0; d < 8; ++d) {
-            const Step st = resolve_step(row, col, d, nr, nc, bc);
-            if (st.kind == 2) {
-                allowed = true;
-                continue;
-            }
-            if (st.kind == 0) continue;
-            const int g = st.r * nc + st.c;
-            if (!mask[g]) {
-                allowed |= (nodata_outlet != 0);
-                continue;
-            }
-            if (g == f) continue;
-            const double dz = zc - static_cast<double>(z[g]);
-            if (dz <= 0.0) continue;
-            const double s = dz / ((d & 1) ? diag : card);
-            if (s > best) {
-                best = s;
-                brcv = g;
-                bdir = d;
-            }
+        if (st.kind == 0) continue;
+        const int g = st.r * nc + st.c;
+        if (!mask[g]) {
+            allowed |= (nodata_outlet != 0);
+            continue;
+        }
+        if (g == f) continue;
+        const double dz = zc - static_cast<double>(z[g]);
+        if (dz <= 0.0) continue;
+        const double sl = dz / ((d & 1) ? diag : card);
+        if (sl > best) {
+            best = sl;
+            brcv = g;
+            bdir = d;
         }
     }
     if (bdir != kNoDirection) {
-        rcv[f] = brcv;
-        dir[f] = static_cast<uint8_t>(bdir);
+        write_receiver(f, brcv, bdir, rcv, dir);
     } else {
-        rcv[f] = kOutletReceiver;
-        dir[f] = kNoDirection;
+        write_receiver(f, kOutletReceiver, kNoDirection, rcv, dir);
         if (!allowed && !pits_outlet) atomicMin(fail, f);
     }
+}
+
+// both receiver launches
+template <class T>
+void launch_receivers(const T* z, const uint8_t* msk, const uint8_t* forced, int nr, int nc, double dx, BC bc,
+                      bool nodata_outlet, bool pits_outlet, int* rcv, uint8_t* dir, int* fail, hipStream_t s)
+{
+    const dim3 b2(64, 4), g2(div_up(nc, 64), div_up(nr, 4 * kRecvRows));
+    k_receivers<T><<<g2, b2, 0, s>>>(z, msk, forced, nr, nc, dx, dx * kSqrt2, nodata_outlet, pits_outlet, rcv, dir,
+                                     fail);
+    LSDTT_LAUNCH_CHECK();
+    const int nring = nc + (nr > 1 ? nc : 0) + (nr > 2 ? (nc > 1 ? 2 : 1) * (nr - 2) : 0);
+    k_receivers_ring<T><<<div_up(nring, 256), 256, 0, s>>>(z, msk, forced, nr, nc, dx, dx * kSqrt2, bc_pack(bc),
+                                                          nodata_outlet, pits_outlet, rcv, dir, fail);
+    LSDTT_LAUNCH_CHECK();
 }
 
 // donor count = number of distinct valid neighbours whose receiver is f
@@ -546,13 +628,11 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
     // misc[0] = first unresolved flat, misc[1] = max depth, misc[2] = n_valid
     int init[3] = {0x7fffffff, -1, 0};
     LSDTT_HIP(hipMemcpyAsync(misc.p, init, sizeof(init), hipMemcpyHostToDevice, s));
-    dim3 b2(64, 4), g2(div_up(nc, 64), div_up(nr, 4));
     {
         TScope ts_(timer, "receivers", s, (sizeof(T) + 6.0) * n);
-        k_receivers<T><<<g2, b2, 0, s>>>(z, msk, forced, nr, nc, opt.dx, opt.dx * kSqrt2, bc_pack(opt.bc),
-                                      opt.interior_nodata_is_outlet, opt.allow_pits_as_outlets, rcv, dir, misc.p);
+        launch_receivers<T>(z, msk, forced, nr, nc, opt.dx, opt.bc, opt.interior_nodata_is_outlet,
+                            opt.allow_pits_as_outlets, rcv, dir, misc.p, s);
     }
-    LSDTT_LAUNCH_CHECK();
     {
         TScope ts_(timer, "donors", s, 9.0 * n);
         k_donors<<<g1, kBlock, 0, s>>>(rcv, msk, nr, nc, opt.bc, donors, flags);
@@ -616,11 +696,9 @@ template <class T>
 void FlowEngine::receivers_only(const T* z, const uint8_t* msk, hipStream_t s)
 {
     mask = msk;
-    dim3 b2(64, 4), g2(div_up(opt.nc, 64), div_up(opt.nr, 4));
     TScope ts_(timer, "receivers", s, (sizeof(T) + 6.0) * n);
-    k_receivers<T><<<g2, b2, 0, s>>>(z, msk, nullptr, opt.nr, opt.nc, opt.dx, opt.dx * kSqrt2, bc_pack(opt.bc),
-                                      opt.interior_nodata_is_outlet, opt.allow_pits_as_outlets, rcv, dir, misc.p);
-    LSDTT_LAUNCH_CHECK();
+    launch_receivers<T>(z, msk, nullptr, opt.nr, opt.nc, opt.dx, opt.bc, opt.interior_nodata_is_outlet,
+                        opt.allow_pits_as_outlets, rcv, dir, misc.p, s);
 }
 
 template void FlowEngine::receivers_only<double>(const double*, const uint8_t*, hipStream_t);

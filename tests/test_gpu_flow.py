@@ -89,6 +89,52 @@ def test_gpu_flow_random_f64_lem_surface(gpu, seed):
     compare(gpu, z, mask, 30.0, "nnon")
 
 
+def near_tie_surface(nr, nc, seed, dtype):
+    """few distinct heights (exact ties between drops) with a third of the
+    cells moved by 1-3 ulps, so drops differ by less than the rounding of
+    their quotients: the first-of-equal-slopes rule decides"""
+    rng = np.random.default_rng(seed)
+    z = (1000.0 + 0.25 * rng.integers(0, 6, (nr, nc))).astype(dtype)
+    moved = rng.random((nr, nc)) < 0.33
+    steps = rng.integers(1, 4, (nr, nc))
+    up = rng.random((nr, nc)) < 0.5
+    for k in range(1, 4):
+        sel = moved & (steps >= k)
+        z[sel] = np.nextafter(z[sel], np.where(up[sel], np.inf, -np.inf).astype(dtype))
+    return z
+
+
+def collapse_surface(nr, nc, seed):
+    """a checkerboard of cells near 1000 and near 500, each a few ulps off:
+    the drops from a high cell to its low neighbours differ by an ulp of
+    themselves, so different drops can give the same rounded slope"""
+    rng = np.random.default_rng(seed)
+    r, c = np.mgrid[0:nr, 0:nc]
+    high = (r + c) % 2 == 0
+    z = np.where(high, 1000.0 + rng.integers(0, 3, (nr, nc)) * np.spacing(1000.0),
+                 500.0 + rng.integers(0, 4, (nr, nc)) * np.spacing(500.0))
+    return z.astype(np.float64)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("dx", [30.0, 7.0])
+def test_gpu_flow_near_ties(gpu, dtype, dx):
+    print(f"[gpu flow ties] {np.dtype(dtype).name} dx={dx}", flush=True)
+    z = near_tie_surface(150, 170, int(dx), dtype)
+    mask = np.ones(z.shape, np.uint8)
+    compare(gpu, z, mask, dx, "oooo")
+
+
+@pytest.mark.parametrize("dx", [30.0, 7.0, 3.0])
+def test_gpu_flow_equal_rounded_slopes(gpu, dx):
+    """different drops whose quotients by the step length round to the same
+    double: the first direction of the largest rounded slope wins"""
+    print(f"[gpu flow collapse] dx={dx}", flush=True)
+    z = collapse_surface(120, 130, int(dx))
+    mask = np.ones(z.shape, np.uint8)
+    compare(gpu, z, mask, dx, "oooo")
+
+
 def test_gpu_flow_long_chain(gpu):
     """a single 1-D channel of 20000 cells: deep levels run in the
     single-workgroup tail"""
