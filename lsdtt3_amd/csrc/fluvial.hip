@@ -194,8 +194,11 @@ __global__ void __launch_bounds__(WL) k_tile_order(const int* __restrict__ rcv, 
         }
         if (ok && mask[f]) xd[lr * TW + lc] = static_cast<uint8_t>(bits);
     }
+    // pointer jumping; a round after which no pointer is left stops it
+    // early (the tile is one wavefront, so the ballot is the tile's)
     for (int round = 0; round < TileShape<TH>::kJumpRounds; ++round) {
         int nj[PER], nd[PER];
+        bool live = false;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int i = lane + WL * k;
@@ -206,6 +209,7 @@ __global__ void __launch_bounds__(WL) k_tile_order(const int* __restrict__ rcv, 
                 nd[k] += dep[j];
                 nj[k] = jmp[j];
             }
+            live |= nj[k] >= 0;
         }
         __syncthreads();
 #pragma unroll
@@ -215,6 +219,7 @@ __global__ void __launch_bounds__(WL) k_tile_order(const int* __restrict__ rcv, 
             dep[i] = nd[k];
         }
         __syncthreads();
+        if (__ballot(live) == 0) break;
     }
     int mx = 0;
 #pragma unroll

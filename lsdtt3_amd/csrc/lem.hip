@@ -139,11 +139,19 @@ __global__ void k_gather(const double* __restrict__ z, const int* __restrict__ i
 // ---------------------------------------------------------------------------
 // uplift, restore, statistics
 // ---------------------------------------------------------------------------
-__global__ void k_uplift(const uint8_t* __restrict__ mask, double* __restrict__ z, double udz, long long n)
+// cell flags: bit 0 the mask, bit 1 uplift (valid cells that are not
+// boundary flats, whose elevation the step sets instead)
+__global__ void k_cflag_init(const uint8_t* __restrict__ mask, uint8_t* __restrict__ cflag, long long n)
 {
     for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
          f += (long long)gridDim.x * blockDim.x)
-        if (mask[f]) z[f] += udz;
+        cflag[f] = mask[f] ? 3 : 0;
+}
+
+__global__ void k_cflag_flats(const int* __restrict__ idx, long long m, uint8_t* __restrict__ cflag)
+{
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < m) cflag[idx[i]] &= 1;
 }
 
 __global__ void k_restore(double* __restrict__ z, const int* __restrict__ idx, const double* __restrict__ bel,
@@ -153,19 +161,28 @@ __global__ void k_restore(double* __restrict__ z, const int* __restrict__ idx, c
     if (i < m) z[idx[i]] = bel[i] - bdz;
 }
 
-// lsdtt_lem_cpu.cpp:655-722 sums and maxima; |dz| as float for p99.9
-// (invalid cells get +inf, which sorts after every valid value)
-__global__ void __launch_bounds__(256) k_stats(const uint8_t* __restrict__ mask, const double* __restrict__ zb,
-                                               const double* __restrict__ za, long long n, double udz,
+// uplift (lsdtt_lem_cpu.cpp:617-625: z += U dt on the cells with bit 1
+// of cflag, udz = 0 without uplift) fused with the step statistics
+// (:655-722 sums and maxima; |dz| as float for p99.9, invalid cells +inf,
+// which sorts after every valid value).  The reference uplifts every valid
+// cell and then resets the boundary flats (:645-652); here the flats are
+// reset first and carry no uplift bit, which leaves the same values.
+__global__ void __launch_bounds__(256) k_stats(const uint8_t* __restrict__ cflag, const double* __restrict__ zb,
+                                               double* __restrict__ za, long long n, double udz,
                                                float* __restrict__ absdz, double* partials)
 {
     double acc[8] = {0.0, 0.0, -1e30, -1e30, 0.0, 0.0, 0.0, 0.0};
     for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
          f += (long long)gridDim.x * blockDim.x) {
-        // loads first, then the mask test (no wait between them)
-        const uint8_t m = mask[f];
-        const double b = zb[f], a = za[f];
-        if (!m) {
+        // loads first, then the flag tests (no wait between them)
+        const uint8_t m = cflag[f];
+        const double b = zb[f];
+        double a = za[f];
+        if ((m & 2) && udz != 0.0) {
+            a += udz;
+            za[f] = a;
+        }
+        if (!(m & 1)) {
             absdz[f] = __int_as_float(0x7f800000);
             continue;
         }
@@ -310,6 +327,10 @@ LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, con
     z_before.reserve(n);
     LSDTT_HIP(hipMemcpy(mask.p, mask_host, n, hipMemcpyHostToDevice));
     mask_h.assign(mask_host, mask_host + n);
+    cflag.reserve(n);
+    k_cflag_init<<<grid_for(n, 4096), kB>>>(mask.p, cflag.p, n);
+    LSDTT_LAUNCH_CHECK();
+    LSDTT_HIP(hipDeviceSynchronize());  // the step's streams may not wait on the null stream
     build_dofs();
     hs_grid = grid_for(n, 2048);
     scal.reserve(kScalCount + 8);
@@ -471,6 +492,13 @@ void LemEngine::set_boundary_flats(const long long* flats, long long m)
         v[static_cast<size_t>(i)] = static_cast<int>(flats[i]);
     }
     if (m) LSDTT_HIP(hipMemcpy(bflats.p, v.data(), sizeof(int) * m, hipMemcpyHostToDevice));
+    k_cflag_init<<<grid_for(n, 4096), kB>>>(mask.p, cflag.p, n);
+    LSDTT_LAUNCH_CHECK();
+    if (m) {
+        k_cflag_flats<<<div_up(m, kB), kB>>>(bflats.p, m, cflag.p);
+        LSDTT_LAUNCH_CHECK();
+    }
+    LSDTT_HIP(hipDeviceSynchronize());
 }
 
 void LemEngine::dinf_stream_power(const SpParams& a, hipStream_t s)
@@ -790,7 +818,7 @@ void LemEngine::stats(LemStats* st, hipStream_t s)
     hist.reserve(kRadixBins + 8);
     const double udz = p.enable_uplift ? p.U * p.dt : 0.0;
     timer.begin("step_stats", s);
-    k_stats<<<hs_grid, kB, 0, s>>>(mask.p, z_before.p, z.p, n, udz, absdz.p, partials.p);
+    k_stats<<<hs_grid, kB, 0, s>>>(cflag.p, z_before.p, z.p, n, udz, absdz.p, partials.p);
     k_stats_fin<<<1, kB, 0, s>>>(partials.p, (unsigned)hs_grid, scal.p + kStatSlot);
     timer.end(s, 21.0 * n);
     LSDTT_LAUNCH_CHECK();
@@ -884,10 +912,8 @@ int LemEngine::step(LemStats* st, hipStream_t s)
         cap_kappa = p.D * p.dt / (dx * dx);
         cap_valid = true;
     }
-    if (p.enable_uplift) {
-        k_uplift<<<g, kB, 0, s>>>(mask.p, z.p, p.U * p.dt, n);
-        LSDTT_LAUNCH_CHECK();
-    }
+    // boundary flats take their base level first; the uplift of every
+    // other valid cell runs inside the statistics pass
     if (n_bflats) {
         k_restore<<<div_up(n_bflats, kB), kB, 0, s>>>(z.p, bflats.p, bel.p, n_bflats, p.baselevel_fall * p.dt);
         LSDTT_LAUNCH_CHECK();

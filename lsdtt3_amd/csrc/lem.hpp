@@ -110,6 +110,7 @@ public:
     long long last_eroded = 0;
 
     dbuf<uint8_t> mask, dof, fast;
+    dbuf<uint8_t> cflag;  // per cell: bit 0 valid, bit 1 uplifted (valid and not a boundary flat)
     // channel set derived each step from the D-infinity drainage area of the
     // post-incision surface (lsdtt_lem_cpu.cpp:86-154): the DOF arrays
     // without channels, the receivers, the area and the set itself
