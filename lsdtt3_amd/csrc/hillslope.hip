@@ -717,6 +717,9 @@ __global__ void __launch_bounds__(256) k_trial(HsDom d, const double* __restrict
                 if (!dl[k]) {
                     b[f[k]] = 0.0f;
                     dinv[f[k]] = 0.0f;
+                    // a trial point is a whole surface: cells off the
+                    // DOFs keep the iterate's value
+                    if (lam != 0.0) z_out[f[k]] = z[f[k]];
                 }
                 continue;
             }

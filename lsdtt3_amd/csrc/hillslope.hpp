@@ -75,8 +75,9 @@ void hs_bicgstab_iteration(const BiArgs& a, int grid, hipStream_t s, class Kerne
 // One evaluation at the trial point zt = z + lam dz (dz unused when lam is
 // 0): scal[HS_TRIAL_F] = max |F(zt)|, scal[HS_TRIAL_RATIO] = max centred-
 // gradient ratio over the DOFs; and the next linear system at zt, should
-// the point be accepted: b = -F(zt), dinv = 1 / J_PP(zt) (FP32), the DOF
-// values of zt into z_out (when lam != 0) and the Krylov start (rho =
+// the point be accepted: b = -F(zt), dinv = 1 / J_PP(zt) (FP32), all of
+// zt into z_out when lam != 0 (the DOFs moved, every other cell z's value)
+// and the Krylov start (rho =
 // ||b||^2, flags cleared).  With `aux` (a second stream of the same device)
 // and two events, the general cells' companion kernel runs on aux beside
 // the main kernel instead of before it.

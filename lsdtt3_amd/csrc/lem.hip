@@ -42,17 +42,6 @@ constexpr int kScalCount = HS_NSCAL + 8;
 
 inline int grid_for(long long n, int cap) { return std::max(1, std::min(cap, div_up(n, kB))); }
 
-__global__ void k_copy3(const double* __restrict__ a, double* __restrict__ b, double* __restrict__ c,
-                        double* __restrict__ d, long long n)
-{
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const double v = a[i];
-        b[i] = v;
-        c[i] = v;
-        d[i] = v;
-    }
-}
-
 __global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, long long n)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -594,7 +583,7 @@ void LemEngine::stream_power(hipStream_t s)
 HsDom LemEngine::hs_domain(double inv_dt) const
 {
     HsDom d{nr, nc, p.bc, mask.p, dof.p, fast.p, gen.p, n_gen, dx, p.D, 0.0, p.Sc, p.slope_ratio_cap, inv_dt,
-            p.nonlinear, zn_hs.p};
+            p.nonlinear, zn};
     d.c9g = c9g.p;
     return with_reciprocals(d);
 }
@@ -699,7 +688,8 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
         st->warm_start_mode = ws_mode;
         st->warm_start_residual = fw;
         if (fw < fnorm && (!p.nonlinear || rw <= feasible)) {
-            std::swap(zc, zt);
+            zc = zt;
+            zt = trial_target();
             fnorm = fw;
             ratio_now = rw;
             st->warm_start_accepted = 1;
@@ -743,7 +733,8 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
             lambda *= 0.5;
         }
         if (!accepted) break;
-        std::swap(zc, zt);  // the accepted trial point is the new iterate
+        zc = zt;  // the accepted trial point is the new iterate
+        zt = trial_target();
         fnorm = fnorm_try;
         ratio_now = ratio_try;
         if (fnorm <= tol) {
@@ -758,6 +749,13 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
     return converged ? it : -1;
 }
 
+double* LemEngine::trial_target() const
+{
+    for (double* q : {z_trial.p, zn_hs.p, z_hs0.p})
+        if (q != zc && q != zn) return q;
+    throw std::logic_error("no free hillslope trial buffer");
+}
+
 // compute_hillslope_newton_transient (newton .cpp:690-788)
 int LemEngine::hillslope(LemStats* st, hipStream_t s)
 {
@@ -770,22 +768,19 @@ int LemEngine::hillslope(LemStats* st, hipStream_t s)
     double dt = total;
     const double dt_min = total * 1e-9, dt_max = total;
     double remaining = total;
-    const int g = grid_for(n, 4096);
-    // zn_hs: the state at the start of the sub-step (backward Euler's z^n,
-    // and what a rejected sub-step restores); z_trial: the line search's
-    // trial points, equal to z off the DOFs; z_hs0: the post-incision state
-    // the Picard fallback restarts from
-    k_copy3<<<g, kB, 0, s>>>(z.p, zn_hs.p, z_trial.p, z_hs0.p, n);
-    LSDTT_LAUNCH_CHECK();
-    zc = z.p;
-    zt = z_trial.p;
+    // a trial writes every cell of its target (the DOFs from the line
+    // search, the others from the iterate), so the trial buffers need no
+    // initial copy; a rejected sub-step goes back to its start and an
+    // accepted one starts the next by moving pointers
+    zn = zc = z.p;
+    zt = trial_target();
     int rc = 0;
     while (remaining > 0.0) {
         const double step = std::min(dt, remaining);
         const int its = newton_solve(1.0 / step, st, s);
         if (its < 0) {
-            k_copy<<<g, kB, 0, s>>>(zn_hs.p, zc, n);
-            LSDTT_LAUNCH_CHECK();
+            zc = zn;
+            zt = trial_target();
             ++st->rejected;
             if (st->rejected > 40 || step <= dt_min) {
                 st->hillslope_converged = 0;
@@ -797,18 +792,27 @@ int LemEngine::hillslope(LemStats* st, hipStream_t s)
         }
         remaining -= step;
         if (remaining > 0.0) {
-            k_copy<<<g, kB, 0, s>>>(zc, zn_hs.p, n);
-            LSDTT_LAUNCH_CHECK();
+            zn = zc;
+            zt = trial_target();
         }
         ++st->substeps;
         if (its <= 4) dt = std::min(dt_max, step * 1.7);
         else if (its >= 15) dt = std::max(dt_min, step * 0.5);
     }
-    if (zc != z.p) {  // the result lives in the trial buffer
-        k_copy<<<g, kB, 0, s>>>(zc, z.p, n);
+    if (rc == 0 && zc != z.p) {
+        // the result becomes z by exchanging storage; the post-incision
+        // surface moves to z_hs0
+        dbuf<double>* owner = zc == z_trial.p ? &z_trial : (zc == zn_hs.p ? &zn_hs : &z_hs0);
+        std::swap(z, *owner);
+        if (owner != &z_hs0) std::swap(*owner, z_hs0);
+    } else if (capture) {
+        // z is the post-incision surface (no accepted trial, or the solve
+        // failed and Picard restarts from it); the captured pair needs it
+        const int g = grid_for(n, 4096);
+        k_copy<<<g, kB, 0, s>>>(z.p, z_hs0.p, n);
         LSDTT_LAUNCH_CHECK();
     }
-    zc = zt = nullptr;
+    zn = zc = zt = nullptr;
     return rc;
 }
 
@@ -881,13 +885,7 @@ int LemEngine::step(LemStats* st, hipStream_t s)
         // surface, the Newton result discarded) or when it is forced
         // (lsdtt_lem_cpu.cpp:531-582)
         bool done = false;
-        if (!p.force_picard) {
-            done = hillslope(st, s) == 0;
-            if (!done) {
-                k_copy<<<g, kB, 0, s>>>(z_hs0.p, z.p, n);
-                LSDTT_LAUNCH_CHECK();
-            }
-        }
+        if (!p.force_picard) done = hillslope(st, s) == 0;  // z is left post-incision when it fails
         if (!done) {
             if (p.force_picard && capture) {
                 z_hs0.reserve(static_cast<size_t>(n));

@@ -129,12 +129,19 @@ public:
     long long n_bflats = 0;
     bool has_K = false;
 
-    // hillslope work
-    dbuf<double> zn_hs, z_trial, z_hs0;  // z_hs0: post-incision state (Picard fallback)
+    // hillslope work: three trial buffers; after a step z_hs0 holds the
+    // post-incision surface (what the Picard fallback and the captured
+    // training pairs start from)
+    dbuf<double> zn_hs, z_trial, z_hs0;
     PicardSolver picard;
-    // hillslope iterate and trial point: z / z_trial, swapped on acceptance
+    // during the hillslope: z keeps the post-incision surface; zn is the
+    // sub-step's start (backward Euler's z^n), zc the iterate, zt the next
+    // trial point, each z or one of the three trial buffers (zt never z,
+    // zn or zc), so no surface is copied while the solve runs
+    double* zn = nullptr;
     double* zc = nullptr;
     double* zt = nullptr;
+    double* trial_target() const;
     dbuf<float> dinv, b, x, r, pv, phat, v, sv, shat, t;
     dbuf<float> c9g;  // general DOFs' Jacobian rows (9 x n_gen)
     dbuf<double> partials, scal;  // reduction partials; scalars
