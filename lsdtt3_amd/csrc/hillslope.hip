@@ -770,6 +770,30 @@ __global__ void __launch_bounds__(256) k_trial_fin(double* partials, unsigned co
 
 int hs_general_blocks(int n_gen) { return gen_grid(n_gen); }
 
+namespace {
+// blocks of 256 threads of `k` that one CU holds at once, times the CUs
+template <class K>
+int resident_blocks(K k)
+{
+    int dev = 0, cus = 0, per = 0;
+    LSDTT_HIP(hipGetDevice(&dev));
+    LSDTT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    LSDTT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, kB, 0));
+    return std::max(1, per) * std::max(1, cus);
+}
+}  // namespace
+
+int hs_bicgstab_grid(long long n, int cap)
+{
+    // one grid for the five kernels (each folds the previous one's block
+    // partials): the blocks that are all resident at once, so no block
+    // waits for a second round behind the others
+    int g = resident_blocks(k_bi_v);
+    for (int v : {resident_blocks(k_bi_t), resident_blocks(k_bi_p), resident_blocks(k_bi_s), resident_blocks(k_bi_x)})
+        g = std::min(g, v);
+    return std::max(1, std::min({g, cap, (int)std::min<long long>(div_up(n, kB), 1ll << 30)}));
+}
+
 void hs_fast_flags(const HsDom& d, uint8_t* fast, hipStream_t s)
 {
     const long long n = (long long)d.nr * d.nc;

@@ -71,6 +71,9 @@ void hs_fast_flags(const HsDom& d, uint8_t* fast, hipStream_t s);
 // companion-kernel blocks for n_gen general cells (their partials follow the
 // hot kernel's `grid` partials, so `partials` holds grid + this many slots)
 int hs_general_blocks(int n_gen);
+// grid (at most cap blocks) of the BiCGSTAB kernels for n cells: as many
+// blocks as the device holds at once
+int hs_bicgstab_grid(long long n, int cap);
 void hs_bicgstab_iteration(const BiArgs& a, int grid, hipStream_t s, class KernelTimer* timer);
 // One evaluation at the trial point zt = z + lam dz (dz unused when lam is
 // 0): scal[HS_TRIAL_F] = max |F(zt)|, scal[HS_TRIAL_RATIO] = max centred-

@@ -321,7 +321,8 @@ LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, con
     LSDTT_LAUNCH_CHECK();
     LSDTT_HIP(hipDeviceSynchronize());  // the step's streams may not wait on the null stream
     build_dofs();
-    hs_grid = grid_for(n, 2048);
+    hs_grid = hs_bicgstab_grid(n, 2048);
+    hs_tgrid = grid_for(n, 2048);  // measured faster than the resident-only grid
     scal.reserve(kScalCount + 8);
     hpin.reserve(32);
     tickets.reserve(4);
@@ -648,7 +649,7 @@ void LemEngine::trial(double lambda, double* fnorm, double* ratio, hipStream_t s
         LSDTT_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     }
     hs_trial(hs_domain(cur_inv_dt), zc, x.p, lambda, n, zt, b.p, dinv.p, partials.p, tickets.p, scal.p, flags.p,
-             hs_grid, s, aux_stream, ev_fork, ev_join);
+             hs_tgrid, s, aux_stream, ev_fork, ev_join);
     // z 8, zn 8, fast 1, dof 1 in; b 4, dinv 4 out; a trial step adds dz 4 in, z_out 8 out
     timer.end(s, (lambda != 0.0 ? 38.0 : 26.0) * n);
     double* h = hpin.p;

@@ -151,7 +151,8 @@ public:
     dbuf<float> absdz;
     dbuf<unsigned> hist;
 
-    int hs_grid = 1024;
+    int hs_grid = 1024;   // BiCGSTAB kernels and the step statistics
+    int hs_tgrid = 1024;  // hillslope trial kernel
     // side stream for the trial's general-cell kernel (default priority)
     hipStream_t aux_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
