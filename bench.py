@@ -123,12 +123,12 @@ def _find(d, suffix):
 def summarize_trace(d, out, title=None, cmd=None, note=None):
     rows = _csv(_find(d, "kernel_stats.csv"))
     title = title or "LEM step at 4096^2 (bench.py workload)"
-    cmd = cmd or "python3 bench.py --steps 3 --warmup 1 --no-cpu-baseline --no-learned"
+    cmd = cmd or "python3 bench.py --steps 3 --warmup 1 --no-cpu-baseline --no-learned --autotune-steps 0"
     note = note or "totals cover the 4 LEM steps of the run (1 warm-up + 3 timed) plus the context set-up"
     lines = [f"# rocprofv3 kernel statistics, {title}", "#",
              "# made by: cd /tmp && rocprofv3 --kernel-trace --stats --output-format csv -d <dir> -o run -- " + cmd,
              "# summarized by: python bench.py --summarize-trace <dir> <this file>"
-             + ("" if cmd.endswith("--no-learned") and title.startswith("LEM") else " <title> <command> <note>"),
+             + ("" if "--no-learned" in cmd and title.startswith("LEM") else " <title> <command> <note>"),
              f"# {note}", "#",
              f"{'kernel':34s} {'calls':>6s} {'total_ms':>10s} {'avg_us':>10s} {'min_us':>9s} {'max_us':>9s} {'pct':>6s}"]
     agg = {}
@@ -162,9 +162,9 @@ def summarize_pmc(fetch_dir, write_dir, out, n_cells=4096 * 4096):
     fw = copy_kib / (cw[1] / cw[0])
     lines = ["# HBM traffic per dispatch from rocprofv3 PMC counters, LEM step at 4096^2 (bench.py workload)", "#",
              "# made by: cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch_dir> -o run -- "
-             "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-learned",
+             "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-learned --autotune-steps 0",
              "#          cd /tmp && rocprofv3 --pmc WRITE_SIZE --output-format csv -d <write_dir> -o run -- "
-             "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-learned",
+             "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-learned --autotune-steps 0",
              "# summarized by: python bench.py --summarize-pmc <fetch_dir> <write_dir> <this file>",
              f"# calibration: k_copy reads and writes {copy_kib:.0f} KiB per dispatch; FETCH_SIZE reads "
              f"{cf[1] / cf[0]:.1f}, WRITE_SIZE {cw[1] / cw[0]:.1f} -> scale factors fetch {ff:.4f}, write {fw:.4f}",
