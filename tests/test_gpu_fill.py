@@ -69,6 +69,33 @@ def test_gpu_fill_deep_maze(gpu):
     assert same(gpu.fill_pits(z, mask), orc.fill(z, mask))
 
 
+@pytest.mark.parametrize("bc", ["oooo", "popo"])
+def test_gpu_fill_plateaus_and_nested_pits(gpu, bc):
+    """heights on a coarse lattice (wide exactly flat areas, ties between
+    neighbours everywhere) with nested bowls and masked holes"""
+    print(f"[gpu fill] plateaus {bc}", flush=True)
+    rng = np.random.default_rng(17)
+    n = 300
+    r, c = np.mgrid[0:n, 0:n]
+    z = np.floor(8.0 * np.sin(r / 23.0) * np.cos(c / 31.0) + rng.integers(0, 2, (n, n))).astype(np.float32)
+    for k in range(12):  # nested bowls: a ring wall around a deeper pit
+        cr, cc = rng.integers(20, n - 20, 2)
+        d = np.hypot(r - cr, c - cc)
+        z[(d > 8) & (d < 10)] = 30.0 + k
+        z[d < 4] = -20.0 - k
+    mask = np.ones(z.shape, np.uint8)
+    mask[rng.random(z.shape) < 0.003] = 0
+    assert same(gpu.fill_pits(z, mask, -9999.0, bc), orc.fill(z, mask, -9999.0, bc))
+
+
+def test_gpu_fill_1024(gpu):
+    print("[gpu fill] 1024^2", flush=True)
+    from lsdtt3_amd.synthetic import routing_surface
+    z = routing_surface(1024, 1024, seed=7)
+    mask = np.ones(z.shape, np.uint8)
+    assert same(gpu.fill_pits(z, mask, -9999.0, "oooo"), orc.fill(z, mask, -9999.0, "oooo"))
+
+
 def test_gpu_fill_rejects_bad_bc(gpu):
     import lsdtt3_amd._native as N
     z = np.zeros((4, 4), np.float32)
