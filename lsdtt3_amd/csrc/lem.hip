@@ -316,6 +316,8 @@ LemEngine::LemEngine(int nr_, int nc_, double dx_, const uint8_t* mask_host, con
     z_before.reserve(n);
     LSDTT_HIP(hipMemcpy(mask.p, mask_host, n, hipMemcpyHostToDevice));
     mask_h.assign(mask_host, mask_host + n);
+    n_mask = 0;
+    for (long long i = 0; i < n; ++i) n_mask += mask_host[i] != 0;
     cflag.reserve(n);
     k_cflag_init<<<grid_for(n, 4096), kB>>>(mask.p, cflag.p, n);
     LSDTT_LAUNCH_CHECK();
@@ -817,6 +819,25 @@ int LemEngine::hillslope(LemStats* st, hipStream_t s)
     return rc;
 }
 
+// the p99.9 selection of the k-th smallest |dz| (k from the valid count),
+// enqueued with its read-back into sel
+void LemEngine::enqueue_p999(long long nv, unsigned* sel, hipStream_t s)
+{
+    sel[0] = 0u;
+    sel[1] = 0u;
+    sel[2] = static_cast<unsigned>(std::floor(0.999 * static_cast<double>(nv - 1)));
+    unsigned* dsel = hist.p + kRadixBins;
+    LSDTT_HIP(hipMemsetAsync(hist.p, 0, sizeof(unsigned) * kRadixBins, s));
+    LSDTT_HIP(hipMemcpyAsync(dsel, sel, sizeof(unsigned) * 3, hipMemcpyHostToDevice, s));
+    const int shifts[3] = {21, 10, 0}, bits[3] = {11, 11, 10};
+    for (int pass = 0; pass < 3; ++pass) {
+        k_radix_hist<<<grid_for(n, 2048), kB, 0, s>>>(absdz.p, n, dsel, shifts[pass], bits[pass], hist.p);
+        k_radix_pick<<<1, kPickThreads, 0, s>>>(hist.p, dsel, shifts[pass], bits[pass]);
+        LSDTT_LAUNCH_CHECK();
+    }
+    LSDTT_HIP(hipMemcpyAsync(sel, dsel, sizeof(unsigned) * 3, hipMemcpyDeviceToHost, s));
+}
+
 void LemEngine::stats(LemStats* st, hipStream_t s)
 {
     absdz.reserve(n);
@@ -829,8 +850,16 @@ void LemEngine::stats(LemStats* st, hipStream_t s)
     LSDTT_LAUNCH_CHECK();
     double* h = hpin.p + 4;
     LSDTT_HIP(hipMemcpyAsync(h, scal.p + kStatSlot, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+    // the valid cells are the mask's, fixed for the context: the selection
+    // is enqueued behind the sums and both come back with one wait
+    unsigned* sel = reinterpret_cast<unsigned*>(hpin.p + 24);
+    if (n_mask > 0) enqueue_p999(n_mask, sel, s);
     LSDTT_HIP(hipStreamSynchronize(s));
     const long long nv = static_cast<long long>(h[7]);
+    if (nv != n_mask && nv > 0) {  // not expected: the statistics count the mask's cells
+        enqueue_p999(nv, sel, s);
+        LSDTT_HIP(hipStreamSynchronize(s));
+    }
     const double inv_dt = 1.0 / p.dt, inv_n = nv > 0 ? 1.0 / static_cast<double>(nv) : 0.0;
     st->n_valid = nv;
     st->delta_max_elevation = std::fabs(h[3] - h[2]);
@@ -840,22 +869,6 @@ void LemEngine::stats(LemStats* st, hipStream_t s)
     st->max_erosion_rate = h[6] * inv_dt;
     st->p999_local_elevation_change = 0.0;
     if (nv > 0) {
-        const unsigned k = static_cast<unsigned>(std::floor(0.999 * static_cast<double>(nv - 1)));
-        unsigned* sel = reinterpret_cast<unsigned*>(hpin.p + 24);
-        sel[0] = 0u;
-        sel[1] = 0u;
-        sel[2] = k;
-        unsigned* dsel = hist.p + kRadixBins;
-        LSDTT_HIP(hipMemsetAsync(hist.p, 0, sizeof(unsigned) * kRadixBins, s));
-        LSDTT_HIP(hipMemcpyAsync(dsel, sel, sizeof(unsigned) * 3, hipMemcpyHostToDevice, s));
-        const int shifts[3] = {21, 10, 0}, bits[3] = {11, 11, 10};
-        for (int pass = 0; pass < 3; ++pass) {
-            k_radix_hist<<<grid_for(n, 2048), kB, 0, s>>>(absdz.p, n, dsel, shifts[pass], bits[pass], hist.p);
-            k_radix_pick<<<1, kPickThreads, 0, s>>>(hist.p, dsel, shifts[pass], bits[pass]);
-            LSDTT_LAUNCH_CHECK();
-        }
-        LSDTT_HIP(hipMemcpyAsync(sel, dsel, sizeof(unsigned) * 3, hipMemcpyDeviceToHost, s));
-        LSDTT_HIP(hipStreamSynchronize(s));
         float pv;
         std::memcpy(&pv, &sel[0], sizeof(float));
         st->p999_local_elevation_change = pv;

@@ -124,6 +124,7 @@ public:
     int last_dinf_passes = 0;
     dbuf<double> z, z_before, K_field, bel, area_sm;
     std::vector<uint8_t> mask_h, channel_h;  // host copies (DOF set rebuilds)
+    long long n_mask = 0;                     // valid cells
     dbuf<int> bflats, gen;
     int n_gen = 0;
     long long n_bflats = 0;
@@ -193,6 +194,7 @@ public:
 
 private:
     void stats(LemStats* st, hipStream_t s);
+    void enqueue_p999(long long nv, unsigned* sel, hipStream_t s);
 };
 
 }  // namespace lsdtt_gpu
