@@ -769,7 +769,12 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, 
     // ring cells: per direction the neighbour's local index (-1: none, the
     // cell itself, a repeat of an earlier direction, or another tile) and
     // the contributions of donors in other tiles, loaded before the sweep
-    __shared__ double xv[128][8];
+    // into a pool: slot sl's donors, in direction order, from xoff[sl].
+    // Each cell of the one-cell frame around the tile (2 TW + 2 TH + 4 at
+    // most) is a donor of at most two ring cells, which bounds the pool.
+    constexpr int kPool = 2 * (2 * TW + 2 * TH + 4);
+    __shared__ double xv[kPool];
+    __shared__ short xoff[128];
     __shared__ short nbl[128][8];
     __shared__ uint8_t xbits[128];
     const int r0 = ty * TH, c0 = tx * TW;
@@ -795,13 +800,15 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, 
         ord[i] = A.ord[(size_t)tile * TN + i];
     }
     const int maxdep = load_levels<TH>(A.lstart, A.tile_maxdep, tile, ls);
+    int gsx[2][8];
+    unsigned bitsx[2], slot1[2];  // donor directions; which receiver slot of the donor
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
         const int sl = lane + WL * half;
         int lrw, lcl;
         const bool ok = slot_cell(sl, h, wd, &lrw, &lcl);
         const int r = r0 + lrw, c = c0 + lcl, v = ok ? r * nc + c : 0;
-        int gs[8];
+        int* gs = gsx[half];
         bool ext[8];
 #pragma unroll
         for (int d = 0; d < 8; ++d) {
@@ -819,27 +826,39 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, 
             ext[d] = u >= 0 && lu < 0;
             nbl[sl][d] = static_cast<short>(lu);
         }
-        int h0[8], h1[8];
+        unsigned bits = 0, s1 = 0;
 #pragma unroll
         for (int d = 0; d < 8; ++d) {
             const int u = ext[d] ? gs[d] : 0;
-            h0[d] = ext[d] ? A.rec[2 * (size_t)u] : -1;
-            h1[d] = ext[d] ? A.rec[2 * (size_t)u + 1] : -1;
-        }
-        unsigned bits = 0;
-#pragma unroll
-        for (int d = 0; d < 8; ++d) {
-            const bool don = ext[d] && (h0[d] == v || h1[d] == v);
-            double x = 0.0;
-            if (don) {
-                const int u = gs[d];
-                const double au = __hip_atomic_load(&A.acc[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                x = au * static_cast<double>(A.w[2 * (size_t)u + (h0[d] == v ? 0 : 1)]);
+            const int h0 = ext[d] ? A.rec[2 * (size_t)u] : -1;
+            const int h1 = ext[d] ? A.rec[2 * (size_t)u + 1] : -1;
+            if (ext[d] && (h0 == v || h1 == v)) {
                 bits |= 1u << d;
+                if (h0 != v) s1 |= 1u << d;
             }
-            xv[sl][d] = x;
         }
+        bitsx[half] = bits;
+        slot1[half] = s1;
         xbits[sl] = static_cast<uint8_t>(bits);
+    }
+    {
+        const int c0n = __popc(bitsx[0]), c1n = __popc(bitsx[1]);
+        int total;
+        const int base = wave_exclusive_scan(c0n + c1n, &total);
+        const int off[2] = {base, base + c0n};
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int sl = lane + WL * half;
+            xoff[sl] = static_cast<short>(off[half]);
+            int e = off[half];
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                if (!((bitsx[half] >> d) & 1u)) continue;
+                const int u = gsx[half][d];
+                const double au = __hip_atomic_load(&A.acc[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                xv[e++] = au * static_cast<double>(A.w[2 * (size_t)u + ((slot1[half] >> d) & 1u)]);
+            }
+        }
     }
     __syncthreads();
     for (int L = 0; L <= maxdep; ++L) {
@@ -859,10 +878,11 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, 
                 }
             } else {
                 const unsigned xb = xbits[sl];
+                const int xo = xoff[sl];
 #pragma unroll
                 for (int d = 0; d < 8; ++d) {
                     if ((xb >> d) & 1u) {
-                        a += xv[sl][d];
+                        a += xv[xo + __popc(xb & ((1u << d) - 1u))];
                     } else {
                         const int lu = nbl[sl][d];
                         if (lu >= 0) {
