@@ -1110,7 +1110,6 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
     if (!P.act[tile]) return;
     __shared__ double val[TN];  // z_old until the cell's level, then its new z
     __shared__ double kam[TN];
-    __shared__ int gr[2][TN];   // global receivers
     __shared__ short lr[2][TN]; // local receivers, -1 outside the tile or none
     __shared__ uint8_t kind[TN];
     __shared__ short ord[TN];
@@ -1151,8 +1150,6 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
         val[i] = zo;
         kam[i] = km;
         kind[i] = kd;
-        gr[0][i] = g0;
-        gr[1][i] = g1;
         lr[0][i] = static_cast<short>((valid[k] && g0 >= 0) ? local_index<TH>(g0, nc, r0, c0) : -1);
         lr[1][i] = static_cast<short>((valid[k] && g1 >= 0) ? local_index<TH>(g1, nc, r0, c0) : -1);
         ord[i] = A.ord[(size_t)tile * TN + i];
@@ -1187,8 +1184,10 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
             const int sl = ring_slot(i / TW, i % TW, h, wd);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                const int l = lr[j][i], g = gr[j][i];
-                ze[j] = l >= 0 ? val[l] : ((g >= 0 && sl >= 0) ? zx[sl][j] : zo);
+                // a receiver outside the tile is a ring cell's, preloaded;
+                // a missing receiver's value is never used by its class
+                const int l = lr[j][i];
+                ze[j] = l >= 0 ? val[l] : (sl >= 0 ? zx[sl][j] : zo);
             }
             const double km = kam[i];
             double zn = zo;
