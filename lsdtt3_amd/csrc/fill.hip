@@ -149,11 +149,17 @@ __global__ void k_fill_seed(const int* __restrict__ ptr, const int* __restrict__
     if (zk[root] == kFixed && wk[root] != kFixed) wk[k] = fkey(z[k]);
 }
 
+// one relaxation pass: pass q reads act, marks act_next and clears
+// act_clear (the next pass's act_next); it does nothing when the previous
+// pass of its batch (prev) changed no tile
 __global__ void __launch_bounds__(256) k_fill_tile(int* __restrict__ wk, const int* __restrict__ zk, int nr,
                                                    int nc, int per_ns, int per_ew, const int* __restrict__ act,
-                                                   int* __restrict__ act_next, int* __restrict__ changed)
+                                                   int* __restrict__ act_next, int* __restrict__ act_clear,
+                                                   const int* __restrict__ prev, int* __restrict__ changed)
 {
     const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x, nty = gridDim.y;
+    if (prev && *prev == 0) return;
+    if (threadIdx.x == 0) act_clear[ty * ntx + tx] = 0;
     if (!act[ty * ntx + tx]) return;
     __shared__ int W[H * H];
     __shared__ int Z[T * T];
@@ -245,16 +251,18 @@ __global__ void k_fill_ones(int* __restrict__ a, int n)
 
 }  // namespace
 
+constexpr int kFillBatchMax = 64;  // relaxation passes per host read-back, at most
+
 size_t fill_scratch_ints(int nr, int nc)
 {
     const long long n = (long long)nr * nc;
     const long long nt = (long long)div_up(nc, T) * div_up(nr, T);
-    return static_cast<size_t>(3 * n + 2 * nt + 64);
+    return static_cast<size_t>(3 * n + 3 * nt + 64 + kFillBatchMax);
 }
 
 // Enqueues the fill on `s` and returns the number of relaxation passes.
 // `scratch` holds fill_scratch_ints(nr, nc) ints.  Blocks on one flag
-// read-back per pass.
+// read-back per batch of passes.
 int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double nodata, BC bc, float* out,
                 int* scratch, hipStream_t s)
 {
@@ -265,9 +273,9 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
     int* zk = scratch;
     int* wk = zk + n;
     int* ptr = wk + n;
-    int* act0 = ptr + n;
-    int* act1 = act0 + nt;
-    int* flags = act1 + nt;   // [0] interior invalid, [1] pass changed, [2..] jump flags
+    int* act[3] = {ptr + n, ptr + n + nt, ptr + n + 2 * nt};
+    int* flags = ptr + n + 3 * nt;   // [0] interior invalid, [2..] jump flags
+    int* pf = flags + 64;            // per-pass changed flags of one batch
     dim3 b2(64, 4), g2(div_up(nc, 64), div_up(nr, 4));
     const int g1 = div_up(n, 256);
     LSDTT_HIP(hipMemsetAsync(flags, 0, sizeof(int) * 64, s));
@@ -286,24 +294,39 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
     }
     k_fill_seed<<<g1, 256, 0, s>>>(ptr, zk, n, wk, z);
     LSDTT_LAUNCH_CHECK();
-    k_fill_ones<<<div_up(nt, 256), 256, 0, s>>>(act0, nt);
+    k_fill_ones<<<div_up(nt, 256), 256, 0, s>>>(act[0], nt);
     LSDTT_LAUNCH_CHECK();
+    LSDTT_HIP(hipMemsetAsync(act[1], 0, sizeof(int) * nt, s));
     dim3 gt(ntx, nty);
-    int passes = 0;
-    int* cur = act0;
-    int* nxt = act1;
+    // passes are launched in batches (8, then doubling) with one read-back
+    // of their flags; a pass after the settling one returns at once
+    int passes = 0, launched = 0;
+    int batch = 8;
     for (;;) {
-        int h = 0;
-        LSDTT_HIP(hipMemsetAsync(flags + 1, 0, sizeof(int), s));
-        LSDTT_HIP(hipMemsetAsync(nxt, 0, sizeof(int) * nt, s));
-        k_fill_tile<<<gt, 256, 0, s>>>(wk, zk, nr, nc, per_ns, per_ew, cur, nxt, flags + 1);
-        LSDTT_LAUNCH_CHECK();
-        LSDTT_HIP(hipMemcpyAsync(&h, flags + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        const int nb = std::min(batch, kFillBatchMax);
+        LSDTT_HIP(hipMemsetAsync(pf, 0, sizeof(int) * nb, s));
+        for (int q = 0; q < nb; ++q) {
+            const int g = launched + q;
+            k_fill_tile<<<gt, 256, 0, s>>>(wk, zk, nr, nc, per_ns, per_ew, act[g % 3], act[(g + 1) % 3],
+                                           act[(g + 2) % 3], q > 0 ? pf + q - 1 : nullptr, pf + q);
+            LSDTT_LAUNCH_CHECK();
+        }
+        int h[kFillBatchMax];
+        LSDTT_HIP(hipMemcpyAsync(h, pf, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
         LSDTT_HIP(hipStreamSynchronize(s));
-        ++passes;
-        if (!h) break;
-        if (passes > n + 2) throw std::runtime_error("fill relaxation did not converge");
-        std::swap(cur, nxt);
+        int settled = -1;
+        for (int q = 0; q < nb; ++q)
+            if (!h[q]) {
+                settled = q;
+                break;
+            }
+        if (settled >= 0) {
+            passes = launched + settled + 1;
+            break;
+        }
+        launched += nb;
+        if (launched > n + 2) throw std::runtime_error("fill relaxation did not converge");
+        batch *= 2;
     }
     k_fill_out<<<g1, 256, 0, s>>>(wk, zk, z, mask, n, nodata, nan_nodata, out);
     LSDTT_LAUNCH_CHECK();
