@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(256) k_receivers(const T* __restrict__ z, cons
                                                    const uint8_t* __restrict__ forced, int nr, int nc,
                                                    double card, double diag, int nodata_outlet, int pits_outlet,
                                                    int* __restrict__ rcv, uint8_t* __restrict__ dir,
-                                                   int* __restrict__ fail)
+                                                   int* __restrict__ fail, T* __restrict__ zcopy)
 {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int r0 = (blockIdx.y * blockDim.y + threadIdx.y) * kRecvRows;
@@ -147,6 +147,7 @@ __global__ void __launch_bounds__(256) k_receivers(const T* __restrict__ z, cons
         const int row = r0 - 1 + k;
         if (row >= nr - 1) break;
         if (row == 0) continue;
+        if (zcopy) zcopy[(size_t)row * nc + col] = zw[k][1];
         const uint8_t mk[8] = {mw[k - 1][1], mw[k - 1][2], mw[k][2], mw[k + 1][2],
                                mw[k + 1][1], mw[k + 1][0], mw[k][0], mw[k - 1][0]};
         const T zn[8] = {zw[k - 1][1], zw[k - 1][2], zw[k][2], zw[k + 1][2],
@@ -163,7 +164,8 @@ __global__ void __launch_bounds__(256) k_receivers_ring(const T* __restrict__ z,
                                                         const uint8_t* __restrict__ forced, int nr, int nc,
                                                         double card, double diag, uint32_t bcw, int nodata_outlet,
                                                         int pits_outlet, int* __restrict__ rcv,
-                                                        uint8_t* __restrict__ dir, int* __restrict__ fail)
+                                                        uint8_t* __restrict__ dir, int* __restrict__ fail,
+                                                        T* __restrict__ zcopy)
 {
     const BC bc = bc_unpack(bcw);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -184,6 +186,7 @@ __global__ void __launch_bounds__(256) k_receivers_ring(const T* __restrict__ z,
         return;
     }
     const int f = row * nc + col;
+    if (zcopy) zcopy[f] = z[f];
     if (!mask[f]) {
         write_receiver(f, kNodataReceiver, kNoDirection, rcv, dir);
         return;
@@ -228,18 +231,19 @@ __global__ void __launch_bounds__(256) k_receivers_ring(const T* __restrict__ z,
     }
 }
 
-// both receiver launches
+// both receiver launches; zcopy (or null) receives a copy of z
 template <class T>
 void launch_receivers(const T* z, const uint8_t* msk, const uint8_t* forced, int nr, int nc, double dx, BC bc,
-                      bool nodata_outlet, bool pits_outlet, int* rcv, uint8_t* dir, int* fail, hipStream_t s)
+                      bool nodata_outlet, bool pits_outlet, int* rcv, uint8_t* dir, int* fail, hipStream_t s,
+                      T* zcopy = nullptr)
 {
     const dim3 b2(64, 4), g2(div_up(nc, 64), div_up(nr, 4 * kRecvRows));
     k_receivers<T><<<g2, b2, 0, s>>>(z, msk, forced, nr, nc, dx, dx * kSqrt2, nodata_outlet, pits_outlet, rcv, dir,
-                                     fail);
+                                     fail, zcopy);
     LSDTT_LAUNCH_CHECK();
     const int nring = nc + (nr > 1 ? nc : 0) + (nr > 2 ? (nc > 1 ? 2 : 1) * (nr - 2) : 0);
     k_receivers_ring<T><<<div_up(nring, 256), 256, 0, s>>>(z, msk, forced, nr, nc, dx, dx * kSqrt2, bc_pack(bc),
-                                                          nodata_outlet, pits_outlet, rcv, dir, fail);
+                                                          nodata_outlet, pits_outlet, rcv, dir, fail, zcopy);
     LSDTT_LAUNCH_CHECK();
 }
 
@@ -693,16 +697,16 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
 }
 
 template <class T>
-void FlowEngine::receivers_only(const T* z, const uint8_t* msk, hipStream_t s)
+void FlowEngine::receivers_only(const T* z, const uint8_t* msk, hipStream_t s, T* zcopy)
 {
     mask = msk;
     TScope ts_(timer, "receivers", s, (sizeof(T) + 6.0) * n);
     launch_receivers<T>(z, msk, nullptr, opt.nr, opt.nc, opt.dx, opt.bc, opt.interior_nodata_is_outlet,
-                        opt.allow_pits_as_outlets, rcv, dir, misc.p, s);
+                        opt.allow_pits_as_outlets, rcv, dir, misc.p, s, zcopy);
 }
 
-template void FlowEngine::receivers_only<double>(const double*, const uint8_t*, hipStream_t);
-template void FlowEngine::receivers_only<float>(const float*, const uint8_t*, hipStream_t);
+template void FlowEngine::receivers_only<double>(const double*, const uint8_t*, hipStream_t, double*);
+template void FlowEngine::receivers_only<float>(const float*, const uint8_t*, hipStream_t, float*);
 
 template long long FlowEngine::route<float>(const float*, const uint8_t*, const uint8_t*, hipStream_t);
 template long long FlowEngine::route<double>(const double*, const uint8_t*, const uint8_t*, hipStream_t);

@@ -43,7 +43,7 @@ public:
     // receivers and directions only (no failure check: for
     // allow_pits_as_outlets routing, where every cell resolves)
     template <class T>
-    void receivers_only(const T* z, const uint8_t* mask, hipStream_t s);
+    void receivers_only(const T* z, const uint8_t* mask, hipStream_t s, T* zcopy = nullptr);  // zcopy: copy of z
 
     // Upstream sweep (requires route()).  keys: Kahn keys too; w/acc: optional
     // weighted double accumulation (w null -> no accumulation).

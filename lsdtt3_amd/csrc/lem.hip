@@ -548,7 +548,9 @@ void LemEngine::stream_power(hipStream_t s)
     }
     // receivers of the pre-step surface, then counts and stream power by
     // tile passes (fluvial.hip)
-    flow.receivers_only<double>(z_before.p, mask.p, s);
+    // the receivers read the pre-step surface from z and leave its copy in
+    // z_before (the step skips its own copy when this path runs)
+    flow.receivers_only<double>(z.p, mask.p, s, z_before.p);
     timer.begin("tile_depths", s);
     tiles.depths(flow.rcv.p, mask.p, s);
     timer.end(s, 9.0 * n);  // rcv 4, mask 1 in; order 2, level starts 2 out
@@ -879,8 +881,10 @@ int LemEngine::step(LemStats* st, hipStream_t s)
 {
     *st = LemStats{};
     const int g = grid_for(n, 4096);
-    k_copy<<<g, kB, 0, s>>>(z.p, z_before.p, n);
-    LSDTT_LAUNCH_CHECK();
+    if (!(p.enable_fluvial && !p.fluvial_dinf)) {  // the D8 receivers copy z themselves
+        k_copy<<<g, kB, 0, s>>>(z.p, z_before.p, n);
+        LSDTT_LAUNCH_CHECK();
+    }
     if (n_bflats) {
         k_gather<<<div_up(n_bflats, kB), kB, 0, s>>>(z.p, bflats.p, n_bflats, bel.p);
         LSDTT_LAUNCH_CHECK();
