@@ -156,18 +156,32 @@ def summarize_pmc(fetch_dir, write_dir, out, n_cells=4096 * 4096):
             a[0] += 1
             a[1] += float(r["Counter_Value"])
     copy_kib = 8.0 * n_cells / 1024.0
-    cf = per["k_copy"]["fetch"]
-    cw = per["k_copy"]["write"]
-    ff = copy_kib / (cf[1] / cf[0])
-    fw = copy_kib / (cw[1] / cw[0])
+    if "k_copy" in per and per["k_copy"]["fetch"][0] and per["k_copy"]["write"][0]:
+        cf = per["k_copy"]["fetch"]
+        cw = per["k_copy"]["write"]
+        ff = copy_kib / (cf[1] / cf[0])
+        fw = copy_kib / (cw[1] / cw[0])
+        calib = (f"# calibration: k_copy reads and writes {copy_kib:.0f} KiB per dispatch; FETCH_SIZE reads "
+                 f"{cf[1] / cf[0]:.1f}, WRITE_SIZE {cw[1] / cw[0]:.1f} -> scale factors fetch {ff:.4f}, "
+                 f"write {fw:.4f}")
+    else:
+        # the D8 step copies the surface inside the receivers kernel, so no
+        # plain copy runs: keep the factors of the last calibration
+        import re
+        old = open(TRAFFIC_FILE).read()
+        m = re.search(r"scale factors fetch ([0-9.]+), write ([0-9.]+)", old)
+        if not m:
+            raise SystemExit("no k_copy dispatch to calibrate against and no earlier calibration")
+        ff, fw = float(m.group(1)), float(m.group(2))
+        calib = (f"# calibration: no plain copy in this run; scale factors fetch {ff:.4f}, write {fw:.4f} "
+                 f"carried over from the previous calibration against k_copy")
     lines = ["# HBM traffic per dispatch from rocprofv3 PMC counters, LEM step at 4096^2 (bench.py workload)", "#",
              "# made by: cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch_dir> -o run -- "
              "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-learned --autotune-steps 0",
              "#          cd /tmp && rocprofv3 --pmc WRITE_SIZE --output-format csv -d <write_dir> -o run -- "
              "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-learned --autotune-steps 0",
              "# summarized by: python bench.py --summarize-pmc <fetch_dir> <write_dir> <this file>",
-             f"# calibration: k_copy reads and writes {copy_kib:.0f} KiB per dispatch; FETCH_SIZE reads "
-             f"{cf[1] / cf[0]:.1f}, WRITE_SIZE {cw[1] / cw[0]:.1f} -> scale factors fetch {ff:.4f}, write {fw:.4f}",
+             calib,
              "#",
              f"{'kernel':34s} {'disp':>5s} {'fetch_KiB':>12s} {'write_KiB':>12s} {'HBM_MB/disp':>12s}"]
     for k, v in sorted(per.items(), key=lambda kv: -(kv[1]["fetch"][1] * ff + kv[1]["write"][1] * fw)):
