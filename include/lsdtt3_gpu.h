@@ -181,6 +181,10 @@ int lsdtt_gpu_flow_chi_threshold(lsdtt_gpu_flow* f, double m_over_n, double A_0,
  * :1203-1394 / priority flood :893-920).  valid = mask != 0 && z != nodata
  * (NaN-aware); invalid cells are written as nodata.  Periodic pairs must
  * match and may not both be periodic (LSDTT_GPU_ERR_ARG).
+ * lsdtt_gpu_fill_device works on device arrays on the caller's stream (a
+ * hipStream_t, NULL = the null stream) and returns when the fill has
+ * finished there.  Its working keys live in one library workspace: calls
+ * from several host threads or streams are safe and run one at a time.
  * ------------------------------------------------------------------- */
 int lsdtt_gpu_fill(const float* z, const uint8_t* mask, int32_t n_rows, int32_t n_cols,
                    double nodata, const char* boundary_conditions, float* filled);
@@ -254,6 +258,10 @@ typedef struct {
      * (compute_dinf_stream_power_incision, lsdtt_stream_power_cpu.cpp:
      * 329-602) when D8 incision is off; default 0 */
     int32_t enable_dinf_fluvial_erosion;
+    /* compute_erosion_rate_field (lsdtt_lem_options, lsdtt_lem_cpu.hpp:170;
+     * filled at :666-696): keep each step's per-pixel erosion rate for
+     * lsdtt_gpu_lem_get_erosion_rate; default 0 */
+    int32_t compute_erosion_rate_field;
 } lsdtt_gpu_lem_options;
 
 typedef struct {
@@ -292,6 +300,11 @@ typedef struct {
     int32_t warm_start_mode;
     int32_t warm_start_accepted;
     double warm_start_residual;
+    /* Newton iterations whose line search ran in restoration mode: the
+     * iterate's largest centred slope ratio above sqrt(slope_ratio_cap),
+     * so a trial is accepted when it lowers that ratio
+     * (lsdtt_hillslope_newton_cpu.cpp:553-633) */
+    int32_t newton_restoration_iterations;
 } lsdtt_gpu_lem_stats;
 
 /* Defaults of lsdtt_lem_options, plus the Newton defaults above. */
@@ -311,7 +324,10 @@ int lsdtt_gpu_lem_set_erodibility_field(lsdtt_gpu_lem* lem, const double* K);
  * cells != 0 are fixed-elevation channel pixels for the hillslope step
  * (not unknowns), unless the regime is coupled.  NULL clears the mask. */
 int lsdtt_gpu_lem_set_channel_mask(lsdtt_gpu_lem* lem, const uint8_t* channel);
-/* Device address of the resident FP64 elevation (for device-side I/O). */
+/* Device address of the resident FP64 elevation (for device-side I/O).
+ * The address is fixed for the life of the context: every step writes its
+ * result there, so a surface written through it before a step is the
+ * surface that step reads.  Fetching it counts as setting the surface. */
 double* lsdtt_gpu_lem_elevation_device(lsdtt_gpu_lem* lem);
 
 /* One step; boundary_flats as from build_boundary_flats (uploaded when it
@@ -320,6 +336,13 @@ double* lsdtt_gpu_lem_elevation_device(lsdtt_gpu_lem* lem);
  * (stats.hillslope_method = 1). */
 int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_t n_boundary,
                        lsdtt_gpu_lem_stats* stats);
+
+/* lsdtt_lem_step_stats::erosion_rate (lsdtt_lem_cpu.hpp:205-210, filled at
+ * lsdtt_lem_cpu.cpp:666-696) of the last step, float[n_rows*n_cols]: per
+ * valid pixel (z_before - (z_after - U dt)) / dt in m/yr (positive =
+ * erosion, the uplift backed out, deposition negative), 0 off the mask.
+ * Needs compute_erosion_rate_field and at least one step. */
+int lsdtt_gpu_lem_get_erosion_rate(lsdtt_gpu_lem* lem, float* rate);
 
 /* The hillslope's Dirichlet channel set of the last step (uint8[n]): the
  * explicit mask, or the cells whose D-infinity drainage area passed

@@ -42,6 +42,7 @@ class LemOptions(C.Structure):
         ("picard_dt_init", C.c_double), ("picard_dt_min", C.c_double), ("picard_dt_max", C.c_double),
         ("picard_grow_factor", C.c_double), ("picard_shrink_factor", C.c_double),
         ("picard_dz_target", C.c_double), ("enable_dinf_fluvial_erosion", C.c_int32),
+        ("compute_erosion_rate_field", C.c_int32),
     ]
 
 
@@ -62,7 +63,7 @@ class LemStats(C.Structure):
         ("picard_rejected", C.c_int32), ("picard_cg_iterations", C.c_int32),
         ("picard_limiter_fired", C.c_int32), ("picard_max_slope_ratio", C.c_double),
         ("warm_start_mode", C.c_int32), ("warm_start_accepted", C.c_int32),
-        ("warm_start_residual", C.c_double),
+        ("warm_start_residual", C.c_double), ("newton_restoration_iterations", C.c_int32),
     ]
 
     def as_dict(self):
@@ -88,6 +89,7 @@ def declare(L, sig):
     sig(L, "lsdtt_gpu_lem_set_tile_height", C.c_int, [vp, i32])
     sig(L, "lsdtt_gpu_lem_tile_height", i32, [vp])
     sig(L, "lsdtt_gpu_lem_get_channel_mask", C.c_int, [vp, vp])
+    sig(L, "lsdtt_gpu_lem_get_erosion_rate", C.c_int, [vp, vp])
     pd = C.POINTER(C.c_double)
     pi64 = C.POINTER(C.c_int64)
     sig(L, "lsdtt_gpu_timestep_options_init", None, [C.POINTER(TimestepOptions)])

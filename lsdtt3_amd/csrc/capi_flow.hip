@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -479,10 +480,17 @@ int lsdtt_gpu_fill_device(const float* d_z, const uint8_t* d_mask, int32_t n_row
         const long long n = (long long)n_rows * n_cols;
         if (n >= (1ll << 31) - 1) return fail_arg("grids of 2^31 cells or more are not supported");
         // working keys; kept for the life of the process (never released at
-        // exit, when the HIP runtime may already be torn down)
+        // exit, when the HIP runtime may already be torn down).  One
+        // workspace serves every caller: calls hold the lock until their
+        // fill has finished on their stream, so calls from other threads or
+        // on other streams never share it while it is in use.
+        static std::mutex* work_lock = new std::mutex();
         static dbuf<int>* work = new dbuf<int>();
+        std::lock_guard<std::mutex> hold(*work_lock);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
         work->reserve(fill_scratch_ints(n_rows, n_cols));
-        fill_device(d_z, d_mask, n_rows, n_cols, nodata, bc, d_filled, work->p, static_cast<hipStream_t>(stream));
+        fill_device(d_z, d_mask, n_rows, n_cols, nodata, bc, d_filled, work->p, s);
+        LSDTT_HIP(hipStreamSynchronize(s));
         return LSDTT_GPU_OK;
     });
 }

@@ -56,6 +56,7 @@ LemParams to_params(const lsdtt_gpu_lem_options& o, double o_cell_size)
     p.width_factor = o.multiply_by_w_over_dx ? o.channel_width_w / o_cell_size : 1.0;
     p.regime = o.hillslope_coupling_regime;
     p.force_picard = o.hillslope_force_picard;
+    p.erosion_rate_field = o.compute_erosion_rate_field;
     p.picard.picard_tol = o.picard_tol;
     p.picard.picard_max_iter = o.picard_max_iter;
     p.picard.omega = o.relaxation_omega;
@@ -119,6 +120,7 @@ void to_c_stats(const LemStats& st, lsdtt_gpu_lem_stats* out)
     out->warm_start_residual = st.warm_start_residual;
     out->stream_power_tile_runs = st.stream_power_tile_runs;
     out->stream_power_passes = st.stream_power_passes;
+    out->newton_restoration_iterations = st.restoration_iterations;
 }
 
 }  // namespace lsdtt_gpu
@@ -274,6 +276,19 @@ int lsdtt_gpu_lem_step(lsdtt_gpu_lem* lem, const int64_t* boundary_flats, int64_
             set_error("hillslope Newton did not converge at the minimum sub-step");
             return LSDTT_GPU_ERR_NOT_CONVERGED;
         }
+        return LSDTT_GPU_OK;
+    });
+}
+
+int lsdtt_gpu_lem_get_erosion_rate(lsdtt_gpu_lem* lem, float* rate)
+{
+    return guarded([&] {
+        if (!lem || !rate) return fail_arg("null argument");
+        LemEngine& e = *lem->eng;
+        if (!e.p.erosion_rate_field) return fail_arg("compute_erosion_rate_field is off");
+        if (!e.ero_valid) return fail_arg("no step has run since the erosion rate field was enabled");
+        LSDTT_HIP(hipDeviceSynchronize());
+        LSDTT_HIP(hipMemcpy(rate, e.ero_rate.p, sizeof(float) * e.n, hipMemcpyDeviceToHost));
         return LSDTT_GPU_OK;
     });
 }

@@ -156,23 +156,33 @@ __global__ void k_restore(double* __restrict__ z, const int* __restrict__ idx, c
 // which sorts after every valid value).  The reference uplifts every valid
 // cell and then resets the boundary flats (:645-652); here the flats are
 // reset first and carry no uplift bit, which leaves the same values.
+// The step's surface is read from zsrc (the hillslope result, wherever the
+// solve left it) and every cell is written to za, the context's state
+// array, whose address therefore never changes between steps.  ero_rate
+// (optional): the per-pixel erosion rate (erosion * inv_dt as float, 0 off
+// the mask, lsdtt_lem_cpu.cpp:666-696).
 __global__ void __launch_bounds__(256) k_stats(const uint8_t* __restrict__ cflag, const double* __restrict__ zb,
-                                               double* __restrict__ za, long long n, double udz,
-                                               float* __restrict__ absdz, double* partials)
+                                               const double* zsrc, double* za, long long n, double udz,
+                                               double inv_dt, float* __restrict__ absdz,
+                                               float* __restrict__ ero_rate, double* partials)
 {
+    const bool moved = zsrc != za;
     double acc[8] = {0.0, 0.0, -1e30, -1e30, 0.0, 0.0, 0.0, 0.0};
     for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
          f += (long long)gridDim.x * blockDim.x) {
         // loads first, then the flag tests (no wait between them)
         const uint8_t m = cflag[f];
         const double b = zb[f];
-        double a = za[f];
+        double a = zsrc[f];
         if ((m & 2) && udz != 0.0) {
             a += udz;
+            za[f] = a;
+        } else if (moved) {
             za[f] = a;
         }
         if (!(m & 1)) {
             absdz[f] = __int_as_float(0x7f800000);
+            if (ero_rate) ero_rate[f] = 0.0f;
             continue;
         }
         acc[0] += b;
@@ -183,6 +193,7 @@ __global__ void __launch_bounds__(256) k_stats(const uint8_t* __restrict__ cflag
         acc[4] = fmax(acc[4], l);
         absdz[f] = static_cast<float>(l);
         const double ero = b - (a - udz);
+        if (ero_rate) ero_rate[f] = static_cast<float>(ero * inv_dt);
         if (ero > 0.0) {
             acc[5] += ero;
             acc[6] = fmax(acc[6], ero);
@@ -713,6 +724,7 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
         st->krylov_iterations += bicgstab(s, it);
         // line search (newton .cpp:553-633)
         const bool restoring = p.nonlinear && ratio_now > feasible;
+        if (restoring) ++st->restoration_iterations;
         double lambda = 1.0, fnorm_try = fnorm, ratio_try = 0.0;
         bool accepted = false;
         for (int ls = 0; ls < p.newton_max_line_search; ++ls) {
@@ -804,19 +816,10 @@ int LemEngine::hillslope(LemStats* st, hipStream_t s)
         if (its <= 4) dt = std::min(dt_max, step * 1.7);
         else if (its >= 15) dt = std::max(dt_min, step * 0.5);
     }
-    if (rc == 0 && zc != z.p) {
-        // the result becomes z by exchanging storage; the post-incision
-        // surface moves to z_hs0
-        dbuf<double>* owner = zc == z_trial.p ? &z_trial : (zc == zn_hs.p ? &zn_hs : &z_hs0);
-        std::swap(z, *owner);
-        if (owner != &z_hs0) std::swap(*owner, z_hs0);
-    } else if (capture) {
-        // z is the post-incision surface (no accepted trial, or the solve
-        // failed and Picard restarts from it); the captured pair needs it
-        const int g = grid_for(n, 4096);
-        k_copy<<<g, kB, 0, s>>>(z.p, z_hs0.p, n);
-        LSDTT_LAUNCH_CHECK();
-    }
+    // z itself is never a trial target, so it still holds the post-incision
+    // surface; the result stays where the solve left it (the statistics
+    // pass writes it into z)
+    hs_result = rc == 0 ? zc : z.p;
     zn = zc = zt = nullptr;
     return rc;
 }
@@ -840,13 +843,15 @@ void LemEngine::enqueue_p999(long long nv, unsigned* sel, hipStream_t s)
     LSDTT_HIP(hipMemcpyAsync(sel, dsel, sizeof(unsigned) * 3, hipMemcpyDeviceToHost, s));
 }
 
-void LemEngine::stats(LemStats* st, hipStream_t s)
+void LemEngine::stats(LemStats* st, const double* zres, hipStream_t s)
 {
     absdz.reserve(n);
     hist.reserve(kRadixBins + 8);
+    if (p.erosion_rate_field) ero_rate.reserve(n);
     const double udz = p.enable_uplift ? p.U * p.dt : 0.0;
     timer.begin("step_stats", s);
-    k_stats<<<hs_grid, kB, 0, s>>>(cflag.p, z_before.p, z.p, n, udz, absdz.p, partials.p);
+    k_stats<<<hs_grid, kB, 0, s>>>(cflag.p, z_before.p, zres, z.p, n, udz, 1.0 / p.dt, absdz.p,
+                                   p.erosion_rate_field ? ero_rate.p : nullptr, partials.p);
     k_stats_fin<<<1, kB, 0, s>>>(partials.p, (unsigned)hs_grid, scal.p + kStatSlot);
     timer.end(s, 21.0 * n);
     LSDTT_LAUNCH_CHECK();
@@ -898,18 +903,31 @@ int LemEngine::step(LemStats* st, hipStream_t s)
     }
     int rc = 0;
     if (threshold_channels()) derive_channels(s);
+    // the step's surface after the hillslope: z, or the trial buffer that
+    // holds the accepted Newton iterate
+    const double* zres = z.p;
     if (p.enable_hillslope) {
         // Newton first; damped Picard when it fails (from the post-incision
         // surface, the Newton result discarded) or when it is forced
         // (lsdtt_lem_cpu.cpp:531-582)
         bool done = false;
-        if (!p.force_picard) done = hillslope(st, s) == 0;  // z is left post-incision when it fails
-        if (!done) {
-            if (p.force_picard && capture) {
-                z_hs0.reserve(static_cast<size_t>(n));
-                k_copy<<<g, kB, 0, s>>>(z.p, z_hs0.p, n);
-                LSDTT_LAUNCH_CHECK();
+        if (!p.force_picard) {
+            done = hillslope(st, s) == 0;  // z is left post-incision when it fails
+            if (done) zres = hs_result;
+        }
+        if (capture) {
+            // the training pair keeps the post-incision surface (z) in z_hs0;
+            // when the result sits in z_hs0 the two internal buffers trade
+            // storage first
+            z_hs0.reserve(static_cast<size_t>(n));
+            if (zres == z_hs0.p) {
+                z_trial.reserve(static_cast<size_t>(n));
+                std::swap(z_hs0, z_trial);
             }
+            k_copy<<<g, kB, 0, s>>>(z.p, z_hs0.p, n);
+            LSDTT_LAUNCH_CHECK();
+        }
+        if (!done) {
             PicardOptions po = p.picard;
             po.D = p.D;
             po.U = 0.0;
@@ -919,22 +937,26 @@ int LemEngine::step(LemStats* st, hipStream_t s)
             picard.run(nr, nc, p.bc, dx, mask.p, dof.p, po, p.dt, z.p, &st->picard, s);
             timer.end(s, 0.0);
             st->hillslope_method = 1;
+            zres = z.p;
         }
     }
     if (p.enable_hillslope && capture) {
         cap_dz.reserve(static_cast<size_t>(n));
-        k_diff_f<<<g, kB, 0, s>>>(z.p, z_hs0.p, cap_dz.p, n);
+        k_diff_f<<<g, kB, 0, s>>>(zres, z_hs0.p, cap_dz.p, n);
         LSDTT_LAUNCH_CHECK();
         cap_kappa = p.D * p.dt / (dx * dx);
         cap_valid = true;
     }
     // boundary flats take their base level first; the uplift of every
-    // other valid cell runs inside the statistics pass
+    // other valid cell runs inside the statistics pass, which also writes
+    // the result into z
     if (n_bflats) {
-        k_restore<<<div_up(n_bflats, kB), kB, 0, s>>>(z.p, bflats.p, bel.p, n_bflats, p.baselevel_fall * p.dt);
+        k_restore<<<div_up(n_bflats, kB), kB, 0, s>>>(const_cast<double*>(zres), bflats.p, bel.p, n_bflats,
+                                                     p.baselevel_fall * p.dt);
         LSDTT_LAUNCH_CHECK();
     }
-    stats(st, s);
+    stats(st, zres, s);
+    ero_valid = p.erosion_rate_field != 0;
     st->cells_eroded = p.enable_fluvial ? last_eroded : 0;
     return rc;
 }

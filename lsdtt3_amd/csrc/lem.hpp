@@ -48,6 +48,7 @@ struct LemParams {
     double width_factor = 1.0;  // w / dx or 1
     int regime = 0;             // 0 automatic, 1 coupled, 2 channels_as_boundaries
     int force_picard = 0;       // hillslope_force_picard
+    int erosion_rate_field = 0; // compute_erosion_rate_field
     PicardOptions picard;       // fallback settings (D, Sc, nonlinear from above; U = 0)
 };
 
@@ -67,6 +68,9 @@ struct LemStats {
     // predictor, 2 learned); accepted when it lowers ||F||inf below F(z^n)
     int warm_start_mode = 0, warm_start_accepted = 0;
     double warm_start_residual = 0;  // ||F||inf at the predicted start
+    // Newton iterations whose line search ran in restoration mode (the
+    // iterate's slope ratio above sqrt(cap), newton .cpp:553-633)
+    int restoration_iterations = 0;
 };
 
 class LemEngine {
@@ -82,6 +86,8 @@ public:
     void get_elevation(double* z_host);
     void set_erodibility(const double* K_host);  // null clears the field
     void set_boundary_flats(const long long* flats, long long n);
+    // the state array: its address stays the same for the life of the
+    // engine (every step writes its result into it)
     double* elevation_device() { return z.p; }
 
     // one timestep on stream `s`; returns 0, or LSDTT_GPU_ERR_NOT_CONVERGED
@@ -151,6 +157,11 @@ public:
     dbuf<int> flags;
     dbuf<float> absdz;
     dbuf<unsigned> hist;
+    // per-pixel erosion rate of the last step (p.erosion_rate_field)
+    dbuf<float> ero_rate;
+    bool ero_valid = false;
+    // where the last hillslope solve left its result (z or a trial buffer)
+    const double* hs_result = nullptr;
 
     int hs_grid = 1024;   // BiCGSTAB kernels and the step statistics
     int hs_tgrid = 1024;  // hillslope trial kernel
@@ -193,7 +204,7 @@ public:
     bool threshold_channels() const;
 
 private:
-    void stats(LemStats* st, hipStream_t s);
+    void stats(LemStats* st, const double* zres, hipStream_t s);
     void enqueue_p999(long long nv, unsigned* sel, hipStream_t s);
 };
 

@@ -164,6 +164,14 @@ class LandscapeModel:
         N.check(N.lib().lsdtt_gpu_lem_get_channel_mask(self._h, N.ptr(out)))
         return out
 
+    @property
+    def erosion_rate(self):
+        """lsdtt_lem_step_stats::erosion_rate of the last step (float32,
+        m/yr, uplift backed out; needs compute_erosion_rate_field=1)"""
+        out = np.empty((self.n_rows, self.n_cols), np.float32)
+        N.check(N.lib().lsdtt_gpu_lem_get_erosion_rate(self._h, N.ptr(out)))
+        return out
+
     # ---- timestep control (src/cpu/lsdtt_lem_timestep_cpu.hpp) --------------
     def median_interior_gradient(self) -> float:
         """median_interior_gradient of the current surface (device)"""

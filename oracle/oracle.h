@@ -323,6 +323,10 @@ int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
                  const orc_lem_params* p, const int64_t* boundary_flats,
                  int64_t n_bf, const double* K_field, double* z,
                  orc_lem_stats* st);
+int orc_lem_step_ex(int nr, int nc, double dx, const uint8_t* mask,
+                    const orc_lem_params* p, const int64_t* boundary_flats,
+                    int64_t n_bf, const double* K_field, double* z,
+                    orc_lem_stats* st, float* erosion_rate);
 
 #ifdef __cplusplus
 }

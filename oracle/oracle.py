@@ -71,6 +71,9 @@ def lib():
         L.orc_lem_step.argtypes = [C.c_int, C.c_int, C.c_double, u8p, C.POINTER(LemParams), i64p,
                                    C.c_int64, C.c_void_p, f64p, C.POINTER(LemStats)]
         L.orc_lem_step.restype = C.c_int
+        L.orc_lem_step_ex.argtypes = [C.c_int, C.c_int, C.c_double, u8p, C.POINTER(LemParams), i64p,
+                                      C.c_int64, C.c_void_p, f64p, C.POINTER(LemStats), C.c_void_p]
+        L.orc_lem_step_ex.restype = C.c_int
         L.orc_picard_transient.argtypes = [C.c_int, C.c_int, C.c_double, u8p, C.c_char_p, C.c_void_p,
                                            C.POINTER(PicardParams), C.c_double, f64p, C.POINTER(PicardResult)]
         L.orc_picard_transient.restype = C.c_int
@@ -500,8 +503,10 @@ def smooth_area(area, mask, kernel_size, bc="oooo"):
     return out
 
 
-def lem_step(z, mask, dx, params, boundary=None, K_field=None):
-    """orc_lem_step on a copy of z; returns (z_new, stats dict, status)"""
+def lem_step(z, mask, dx, params, boundary=None, K_field=None, erosion_rate=None):
+    """orc_lem_step on a copy of z; returns (z_new, stats dict, status).
+    erosion_rate: optional float32 array of z's shape that receives the
+    per-pixel erosion rate field."""
     zz = np.ascontiguousarray(z, dtype=np.float64).copy()
     nr, nc = zz.shape
     m = np.ascontiguousarray(mask, dtype=np.uint8)
@@ -512,8 +517,13 @@ def lem_step(z, mask, dx, params, boundary=None, K_field=None):
     if K_field is not None:
         kf = np.ascontiguousarray(K_field, np.float64).ravel()
         kptr = kf.ctypes.data_as(C.c_void_p)
-    rc = lib().orc_lem_step(nr, nc, float(dx), m.ravel(), C.byref(params), bf, len(bf), kptr,
-                            zz.ravel(), C.byref(st))
+    er = None
+    if erosion_rate is not None:
+        if erosion_rate.dtype != np.float32 or erosion_rate.shape != zz.shape or not erosion_rate.flags.c_contiguous:
+            raise ValueError("erosion_rate must be a C-contiguous float32 array of the DEM's shape")
+        er = erosion_rate.ctypes.data_as(C.c_void_p)
+    rc = lib().orc_lem_step_ex(nr, nc, float(dx), m.ravel(), C.byref(params), bf, len(bf), kptr,
+                               zz.ravel(), C.byref(st), er)
     return zz, st.as_dict(), rc
 
 

@@ -625,6 +625,16 @@ int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
                  int64_t n_bf, const double* K_field, double* z,
                  orc_lem_stats* st)
 {
+    return orc_lem_step_ex(nr, nc, dx, mask, p, boundary_flats, n_bf, K_field, z, st, NULL);
+}
+
+/* run_lem_step; erosion_rate (optional, float[n]): the per-pixel field of
+ * compute_erosion_rate_field (lsdtt_lem_cpu.cpp:666-696) */
+int orc_lem_step_ex(int nr, int nc, double dx, const uint8_t* mask,
+                    const orc_lem_params* p, const int64_t* boundary_flats,
+                    int64_t n_bf, const double* K_field, double* z,
+                    orc_lem_stats* st, float* erosion_rate)
+{
     const int64_t n = (int64_t)nr * nc;
     memset(st, 0, sizeof(*st));
     double* z_before = (double*)malloc(sizeof(double) * (size_t)n);
@@ -748,6 +758,9 @@ int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
     int64_t nv = 0;
     float* lc = (float*)malloc(sizeof(float) * (size_t)(n ? n : 1));
     const double udz = p->enable_uplift ? p->U * p->dt : 0.0;
+    const double inv_dt_rate = 1.0 / p->dt;
+    if (erosion_rate)
+        for (int64_t i = 0; i < n; ++i) erosion_rate[i] = 0.0f;
     for (int64_t i = 0; i < n; ++i) {
         if (!mask[i]) continue;
         const double zb = z_before[i], za = z[i];
@@ -758,6 +771,8 @@ int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
         if (l > ml) ml = l;
         lc[nv++] = (float)l;
         const double ero = zb - (za - udz);
+        /* the field keeps the sign: deposition is a negative rate */
+        if (erosion_rate) erosion_rate[i] = (float)(ero * inv_dt_rate);
         if (ero > 0.0) { se += ero; if (ero > me) me = ero; }
     }
     const double inv_dt = 1.0 / p->dt, inv_n = nv > 0 ? 1.0 / (double)nv : 0.0;

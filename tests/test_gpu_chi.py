@@ -78,6 +78,59 @@ def test_gpu_chi_routing_pipeline_1024(gpu):
     assert np.array_equal(fi.n_contributing_pixels(), flow["s_size"].astype(np.int64))
 
 
+def test_gpu_routing_pipeline_4096_bit_exact(gpu):
+    """the routing workload at its full size: 4096^2 float32 DEM (1 % closed
+    bowls up to 20 m deep, 5 % exactly flat plateaus), dx = 10 m; fill ->
+    D8 (pits as outlets) -> donors, outlets, Kahn order, s-vector, int64
+    counts, float area -> chi raster and the thresholded channel lists,
+    every array bit-exact against the oracle"""
+    from lsdtt3_amd.synthetic import routing_surface
+    print("[gpu routing pipeline 4096] generating", flush=True)
+    z = routing_surface(4096, 4096, seed=7)
+    mask = np.ones(z.shape, np.uint8)
+    zf = gpu.fill_pits(z, mask)
+    zo = orc.fill(z, mask)
+    assert np.array_equal(zf.view(np.uint32), zo.view(np.uint32))
+    assert (zo > z).sum() > 0.005 * z.size   # the bowls were filled
+    del zo
+    print("[gpu routing pipeline 4096] fill ok", flush=True)
+    fi = gpu.FlowInfo(zf, mask, 10.0, "oooo", allow_pits_as_outlets=True)
+    flow = orc.flow_build(zf, mask, 10.0, "oooo", allow_pits_as_outlets=True)
+    assert np.array_equal(fi.receivers, flow["receiver"])
+    assert np.array_equal(fi.receiver_directions, flow["dir"])
+    assert np.array_equal(fi.donor_counts, flow["donors"])
+    assert np.array_equal(fi.outlets(), flow["outlets"])
+    assert np.array_equal(fi.topological_order(), flow["topo"])
+    assert np.array_equal(fi.s_vector, flow["s_vector"])
+    assert np.array_equal(fi.s_vector_start, flow["s_start"])
+    assert np.array_equal(fi.s_vector_size, flow["s_size"])
+    counts = orc.upstream_counts(mask, flow["receiver"], flow["topo"])
+    assert np.array_equal(fi.n_contributing_pixels(), counts)
+    area, amask = fi.get_drainage_area_raster()
+    assert np.array_equal(area.ravel(), (counts * 100).astype(np.float32))
+    print("[gpu routing pipeline 4096] flow object ok", flush=True)
+    cg, mg = fi.get_chi_raster(0.45, 1.0)
+    co, mo = orc.chi_raster(flow, mask, 10.0, m_over_n=0.45, A0=1.0)
+    assert np.array_equal(mg, mo)
+    assert np.array_equal(cg.view(np.uint32), co.view(np.uint32))
+    # The epsilon-free fill leaves its lakes exactly flat and every flat
+    # cell is its own outlet (pits as outlets), so on this surface no basin
+    # reaches 1000 pixels: the 1000-pixel lists are empty on both sides, and
+    # a 30-pixel threshold gives non-empty lists to compare.
+    for mp in (1000, 30):
+        basins = fi.get_chi_from_all_outlets_threshold(mp, 0.45, 1.0)
+        t = orc.chi_threshold(flow, mask, 10.0, mp, m_over_n=0.45, A0=1.0)
+        assert len(basins) == len(t["basin_start"])
+        if mp == 30:
+            assert len(basins) > 100
+        if not basins:
+            continue
+        assert np.array_equal(np.concatenate([b["flats"] for b in basins]), t["flats"])
+        assert np.array_equal(np.concatenate([b["chi"] for b in basins]).view(np.uint32),
+                              t["chi"].view(np.uint32))
+        assert np.array_equal(np.concatenate([b["receiver_flats"] for b in basins]), t["receivers"])
+
+
 def test_gpu_chi_errors(gpu):
     from lsdtt3_amd._native import GpuError
     z = np.random.default_rng(1).random((20, 30)).astype(np.float32)
