@@ -191,6 +191,22 @@ int lsdtt_gpu_fill(const float* z, const uint8_t* mask, int32_t n_rows, int32_t 
 int lsdtt_gpu_fill_device(const float* d_z, const uint8_t* d_mask, int32_t n_rows,
                           int32_t n_cols, double nodata, const char* boundary_conditions,
                           float* d_filled, void* stream);
+/* No counterpart in the reference: lsdtt_gpu_fill_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results
+ * (bit for bit); the working keys live in a fixed device array of the
+ * library (48.3 MiB per GPU, held until the process ends) instead of an
+ * allocated workspace, so grids are limited to
+ * lsdtt_gpu_fill_noalloc_max_cells() = 4194304 cells (2048 x 2048, any
+ * shape); larger grids are rejected with LSDTT_GPU_ERR_ARG.  Calls share
+ * the one array per GPU: within a process the library runs them one at a
+ * time (each returns when its fill has finished on its stream); calls that
+ * overlapped on one GPU would race on it. */
+int lsdtt_gpu_fill_device_noalloc(const float* d_z, const uint8_t* d_mask, int32_t n_rows,
+                                  int32_t n_cols, double nodata,
+                                  const char* boundary_conditions, float* d_filled,
+                                  void* stream);
+int64_t lsdtt_gpu_fill_noalloc_max_cells(void);
 
 /* ---------------------------------------------------------------------
  * LEM timestep.  Replaces lsdtt::run_lem_step (src/cpu/lsdtt_lem_cpu.hpp:

@@ -90,6 +90,8 @@ def lib():
     _sig(L, "lsdtt_gpu_flow_chi_threshold", C.c_int, [vp, dbl, dbl, i64, vp, vp, vp, vp, vp, vp])
     _sig(L, "lsdtt_gpu_fill", C.c_int, [vp, vp, i32, i32, dbl, C.c_char_p, vp])
     _sig(L, "lsdtt_gpu_fill_device", C.c_int, [vp, vp, i32, i32, dbl, C.c_char_p, vp, vp])
+    _sig(L, "lsdtt_gpu_fill_device_noalloc", C.c_int, [vp, vp, i32, i32, dbl, C.c_char_p, vp, vp])
+    _sig(L, "lsdtt_gpu_fill_noalloc_max_cells", i64, [])
     _extend(L)
     _lib = L
     return L

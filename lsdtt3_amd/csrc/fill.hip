@@ -285,8 +285,10 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
     LSDTT_LAUNCH_CHECK();
     k_fill_descent<<<g2, b2, 0, s>>>(z, zk, wk, nr, nc, per_ns, per_ew, ptr);
     LSDTT_LAUNCH_CHECK();
-    const int one = 1;
-    LSDTT_HIP(hipMemcpyAsync(flags + 2, &one, sizeof(int), hipMemcpyHostToDevice, s));
+    // the first jump round runs unconditionally (its "previous round
+    // changed" flag is set on the device, no host copy)
+    k_fill_ones<<<1, 64, 0, s>>>(flags + 2, 1);
+    LSDTT_LAUNCH_CHECK();
     constexpr int kRounds = 34;
     for (int r = 0; r < kRounds; ++r) {
         k_fill_jump<<<g1, 256, 0, s>>>(ptr, n, flags + 2 + r, flags + 3 + r);
