@@ -487,11 +487,17 @@ int lsdtt_gpu_dinf_steady_state(const float* z, const uint8_t* mask, int32_t n_r
 
 /* ---------------------------------------------------------------------
  * Batched ensemble: n independent LEM instances (uneven grids, per-instance
- * options) stepped concurrently on one GPU.  No counterpart in the
- * reference, which runs one model per process (cli/lsdtt-lem.cpp:3553-3700);
- * each instance's step is run_lem_step as above.  masks[i] and options[i]
- * describe instance i; boundary flats come from its boundary codes;
- * n_workers host threads drive the instances (<= 0: 16).
+ * options) stepped together on one GPU.  No counterpart in the reference,
+ * which runs one model per process (cli/lsdtt-lem.cpp:3553-3700); each
+ * instance's step is run_lem_step as above (D8 incision, roering_implicit
+ * Newton hillslope with its Picard fallback, uplift, boundary flats from
+ * the instance's boundary codes, statistics).  Every kernel of a step runs
+ * once for all instances, each instance's solver decisions are taken on
+ * the device, and the step's launches are replayed from a captured
+ * hipGraph.  An instance's results do not depend on the other instances.
+ * Not supported here (LSDTT_GPU_ERR_ARG at creation): D-inf incision, area
+ * smoothing, incision thresholds or channel sets, forced Picard.
+ * n_workers is ignored (kept for source compatibility).
  * ------------------------------------------------------------------- */
 typedef struct lsdtt_gpu_ensemble lsdtt_gpu_ensemble;
 
@@ -501,23 +507,36 @@ lsdtt_gpu_ensemble* lsdtt_gpu_ensemble_create(int32_t n_instances, const int32_t
 void lsdtt_gpu_ensemble_destroy(lsdtt_gpu_ensemble* e);
 int lsdtt_gpu_ensemble_set_elevation(lsdtt_gpu_ensemble* e, int32_t instance, const double* z);
 int lsdtt_gpu_ensemble_get_elevation(lsdtt_gpu_ensemble* e, int32_t instance, double* z);
+/* The last step's per-pixel erosion rate of an instance created with
+ * compute_erosion_rate_field (as lsdtt_gpu_lem_get_erosion_rate). */
+int lsdtt_gpu_ensemble_get_erosion_rate(lsdtt_gpu_ensemble* e, int32_t instance, float* rate);
 /* n_steps steps of every instance; last_stats (n_instances entries, may be
  * NULL) receives each instance's last step; *seconds the wall time from
- * the first launch to the last instance's completion. */
+ * the first launch to the last step's completion. */
 int lsdtt_gpu_ensemble_run(lsdtt_gpu_ensemble* e, int32_t n_steps, lsdtt_gpu_lem_stats* last_stats,
                            double* seconds);
-/* Tile height of instance i's routing sweeps (as lsdtt_gpu_lem_set_tile_height). */
-int lsdtt_gpu_ensemble_set_tile_height(lsdtt_gpu_ensemble* e, int32_t instance, int32_t tile_height);
-/* Chooses every instance's tile height by measurement: per size class
- * (octave of the cell count) the height that steps the class's largest
- * instance fastest alone; then `steps` concurrent steps of the whole
- * ensemble with the current heights, each uniform height and the per-class
- * heights, from the same surfaces (restored afterwards); an assignment
- * replaces the current one only when it is more than 3% faster.
- * tile_heights (n_instances entries, may be NULL) receives it, *seconds
- * (may be NULL) the time the tuning took. */
-int lsdtt_gpu_ensemble_autotune_tiles(lsdtt_gpu_ensemble* e, int32_t steps, int32_t* tile_heights,
-                                      double* seconds);
+/* enable != 0 (the default): each step replays the captured launch
+ * pattern; 0: the same launches are enqueued one by one.  Results are the
+ * same either way. */
+int lsdtt_gpu_ensemble_set_graph(lsdtt_gpu_ensemble* e, int32_t enable);
+/* Launch-pattern counters: graph replays and captures, solver rounds and
+ * tile passes launched outside the pattern (an instance that needed more
+ * than its caps), steps restarted because a sweep did not settle within
+ * the pattern, and caps[3] = the current pattern's count passes, stream
+ * power passes and solver rounds.  Any pointer may be NULL. */
+int lsdtt_gpu_ensemble_pattern(lsdtt_gpu_ensemble* e, int64_t* graph_launches, int64_t* graph_captures,
+                               int64_t* extra_rounds, int64_t* extra_passes, int64_t* restarts,
+                               int32_t* caps);
+/* Tile height of the routing sweeps of every instance (8, 16 or 32; speed
+ * only, results are the same). */
+int lsdtt_gpu_ensemble_set_tile_height(lsdtt_gpu_ensemble* e, int32_t tile_height);
+int32_t lsdtt_gpu_ensemble_tile_height(lsdtt_gpu_ensemble* e);
+/* Times `steps` steps of the whole ensemble with each tile height from the
+ * same surfaces (restored afterwards) and keeps the fastest (the current
+ * height unless another is more than 3% faster); *tile_height receives it,
+ * ms_per_step[3] (may be NULL) each height's time per step. */
+int lsdtt_gpu_ensemble_autotune_tiles(lsdtt_gpu_ensemble* e, int32_t steps, int32_t* tile_height,
+                                      double* ms_per_step);
 
 /* Per-kernel timing with HIP events recorded on the step's stream (no
  * counterpart in the reference; for roofline measurement).  enable != 0

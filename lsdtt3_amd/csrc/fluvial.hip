@@ -109,6 +109,28 @@ __device__ __forceinline__ LaneCell lane_cell(int k, int r0, int c0, int nr, int
     return q;
 }
 
+// Where a tile sits: its slot in the per-tile arrays, the slot of its
+// grid's first tile, and its column and row among that grid's tiles.  A
+// single-grid launch takes them from the block index; a batched launch
+// (many grids at once, their per-cell arrays concatenated) from its tile
+// list.
+struct TileAt {
+    int tile, tbase, tx, ty, ntx;
+};
+
+__device__ __forceinline__ TileAt grid_tile()
+{
+    return TileAt{static_cast<int>(blockIdx.y * gridDim.x + blockIdx.x), 0, static_cast<int>(blockIdx.x),
+                  static_cast<int>(blockIdx.y), static_cast<int>(gridDim.x)};
+}
+
+__device__ __forceinline__ TileAt batch_tile(const BatchTiles& B, BatchGrid& g)
+{
+    const BatchTile t = B.tiles[blockIdx.x];
+    g = B.grids[t.inst];
+    return TileAt{static_cast<int>(blockIdx.x), g.tbase, t.tx, t.ty, g.ntx};
+}
+
 // ---------------------------------------------------------------------------
 // once per set of receivers: each tile's cells sorted by in-tile hop count
 // to the tile's exit (pointer jumping in LDS, then a counting sort), stored
@@ -119,18 +141,18 @@ __device__ __forceinline__ LaneCell lane_cell(int k, int r0, int c0, int nr, int
 // directions, which then counts once).
 // ---------------------------------------------------------------------------
 template <int TH>
-__global__ void __launch_bounds__(WL) k_tile_order(const int* __restrict__ rcv, const uint8_t* __restrict__ mask,
-                                                   int nr, int nc, BC bc, short* __restrict__ ordg,
-                                                   short* __restrict__ lstartg, int* __restrict__ tile_maxdep,
-                                                   uint8_t* __restrict__ xdon)
+__device__ __forceinline__ void tile_order_body(const TileAt& T, const int* __restrict__ rcv,
+                                                const uint8_t* __restrict__ mask, int nr, int nc, BC bc,
+                                                short* __restrict__ ordg, short* __restrict__ lstartg,
+                                                int* __restrict__ tile_maxdep, uint8_t* __restrict__ xdon)
 {
     constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
     __shared__ int jmp[TN];
     __shared__ int dep[TN];
     __shared__ int hist[TN + 1];
     __shared__ uint8_t xd[TN];
-    const int tx = blockIdx.x, ty = blockIdx.y, lane = threadIdx.x;
-    const int tile = ty * gridDim.x + tx;
+    const int tx = T.tx, ty = T.ty, lane = threadIdx.x;
+    const int tile = T.tile;
     const int r0 = ty * TH, c0 = tx * TW;
     const int h = min(TH, nr - r0), w = min(TW, nc - c0);
     bool valid[PER];
@@ -267,17 +289,29 @@ __global__ void __launch_bounds__(WL) k_tile_order(const int* __restrict__ rcv, 
         }
 }
 
+template <int TH>
+__global__ void __launch_bounds__(WL) k_tile_order(const int* __restrict__ rcv, const uint8_t* __restrict__ mask,
+                                                   int nr, int nc, BC bc, short* __restrict__ ordg,
+                                                   short* __restrict__ lstartg, int* __restrict__ tile_maxdep,
+                                                   uint8_t* __restrict__ xdon)
+{
+    tile_order_body<TH>(grid_tile(), rcv, mask, nr, nc, bc, ordg, lstartg, tile_maxdep, xdon);
+}
+
+template <int TH>
+__global__ void __launch_bounds__(WL) kb_tile_order(BatchTiles B, const int* __restrict__ rcv,
+                                                    const uint8_t* __restrict__ mask, short* __restrict__ ordg,
+                                                    short* __restrict__ lstartg, int* __restrict__ tile_maxdep,
+                                                    uint8_t* __restrict__ xdon)
+{
+    BatchGrid g;
+    const TileAt T = batch_tile(B, g);
+    tile_order_body<TH>(T, rcv + g.off, mask + g.off, g.nr, g.nc, g.bc, ordg, lstartg, tile_maxdep, xdon + g.off);
+}
+
 // Pass bookkeeping shared by both sweeps.  Pass p reads act[p % 3], marks
 // act[(p + 1) % 3] and clears act[(p + 2) % 3] for pass p + 1; it does
 // nothing when pass p - 1 marked no tile (the sweep has settled).
-struct PassArgs {
-    const int* act;
-    int* act_next;
-    int* act_clear;
-    const int* prev_marked;  // null for the first pass
-    int* marked;
-    int* runs;  // per tile: times run in this settle (one writer per pass)
-};
 
 // level starts of levels 0..maxdep + 1 into LDS
 template <int TH>
@@ -295,16 +329,16 @@ __device__ __forceinline__ int load_levels(const short* __restrict__ lstartg, co
 // other tiles (HBM, their current values)
 // ---------------------------------------------------------------------------
 template <int TH>
-__global__ void __launch_bounds__(WL) k_tile_counts(const int* __restrict__ rcv, const uint8_t* __restrict__ mask,
-                                                    const short* __restrict__ ordg, const short* __restrict__ lstartg,
-                                                    const int* __restrict__ tile_maxdep,
-                                                    const uint8_t* __restrict__ xdon, int nr, int nc, BC bc,
-                                                    int* __restrict__ cnt, PassArgs P)
+__device__ __forceinline__ void tile_counts_body(const TileAt& T, const int* __restrict__ rcv,
+                                                 const uint8_t* __restrict__ mask, const short* __restrict__ ordg,
+                                                 const short* __restrict__ lstartg,
+                                                 const int* __restrict__ tile_maxdep,
+                                                 const uint8_t* __restrict__ xdon, int nr, int nc, BC bc,
+                                                 int* __restrict__ cnt, const PassArgs& P)
 {
     constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
-    if (P.prev_marked && *P.prev_marked == 0) return;
-    const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x;
-    const int tile = ty * ntx + tx;
+    const int tx = T.tx, ty = T.ty, ntx = T.ntx;
+    const int tile = T.tile;
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
     if (!P.act[tile]) return;
@@ -391,7 +425,7 @@ __global__ void __launch_bounds__(WL) k_tile_counts(const int* __restrict__ rcv,
             const LaneCell q = lane_cell(k, r0, c0, nr, nc);
             __hip_atomic_store(&cnt[q.f], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (lrcv[i] < 0 && g[k] >= 0) {
-                P.act_next[tile_of<TH>(g[k], nc, ntx)] = 1;
+                P.act_next[T.tbase + tile_of<TH>(g[k], nc, ntx)] = 1;
                 marked = true;
             }
         }
@@ -403,26 +437,37 @@ __global__ void __launch_bounds__(WL) k_tile_counts(const int* __restrict__ rcv,
     }
 }
 
+template <int TH>
+__global__ void __launch_bounds__(WL) k_tile_counts(const int* __restrict__ rcv, const uint8_t* __restrict__ mask,
+                                                    const short* __restrict__ ordg, const short* __restrict__ lstartg,
+                                                    const int* __restrict__ tile_maxdep,
+                                                    const uint8_t* __restrict__ xdon, int nr, int nc, BC bc,
+                                                    int* __restrict__ cnt, PassArgs P)
+{
+    if (P.prev_marked && *P.prev_marked == 0) return;
+    tile_counts_body<TH>(grid_tile(), rcv, mask, ordg, lstartg, tile_maxdep, xdon, nr, nc, bc, cnt, P);
+}
+
+template <int TH>
+__global__ void __launch_bounds__(WL) kb_tile_counts(BatchTiles B, const int* __restrict__ rcv,
+                                                     const uint8_t* __restrict__ mask,
+                                                     const short* __restrict__ ordg,
+                                                     const short* __restrict__ lstartg,
+                                                     const int* __restrict__ tile_maxdep,
+                                                     const uint8_t* __restrict__ xdon, int* __restrict__ cnt,
+                                                     PassArgs P)
+{
+    if (P.prev_marked && *P.prev_marked == 0) return;
+    BatchGrid g;
+    const TileAt T = batch_tile(B, g);
+    tile_counts_body<TH>(T, rcv + g.off, mask + g.off, ordg, lstartg, tile_maxdep, xdon + g.off, g.nr, g.nc, g.bc,
+                         cnt + g.off, P);
+}
+
 // ---------------------------------------------------------------------------
 // stream power: outlet side first; a cell's new elevation needs its
 // receiver's new elevation (LDS in the tile, HBM across the edge)
 // ---------------------------------------------------------------------------
-struct SpTileArgs {
-    const int* rcv;
-    const uint8_t* dir;
-    const uint8_t* mask;
-    const int* cnt;
-    const double* area;  // null: count * pixel_area
-    const short* ord;
-    const short* lstart;
-    const int* tile_maxdep;
-    const uint8_t* xdon;
-    const double* z_old;
-    const double* K;
-    double* z;
-    int* tile_eroded;
-    SpParams p;
-};
 
 // Newton / closed form of lsdtt_stream_power_cpu.cpp:257-316 for a cell
 // that is not skipped
@@ -477,13 +522,13 @@ __device__ __forceinline__ short pack_lrd(short l, uint8_t d)
 constexpr int kSpBatch = 4;
 
 template <int TH, bool MH, int NM>
-__global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC bc, PassArgs P)
+__device__ __forceinline__ void tile_sp_body(const TileAt& T, const SpTileArgs& A, int nr, int nc, BC bc,
+                                             const PassArgs& P)
 {
     constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
     static_assert(PER % kSpBatch == 0, "cells per lane must be whole batches");
-    if (P.prev_marked && *P.prev_marked == 0) return;
-    const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x;
-    const int tile = ty * ntx + tx;
+    const int tx = T.tx, ty = T.ty, ntx = T.ntx;
+    const int tile = T.tile;
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
     if (!P.act[tile]) return;
@@ -582,7 +627,7 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
         if (__double_as_longlong(o) != __double_as_longlong(v)) {
             __hip_atomic_store(&A.z[q.f], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             for (unsigned b = x[k]; b; b &= b - 1) {
-                P.act_next[tile_of<TH>(neighbour(q.r, q.c, __builtin_ctz(b), nr, nc, bc), nc, ntx)] = 1;
+                P.act_next[T.tbase + tile_of<TH>(neighbour(q.r, q.c, __builtin_ctz(b), nr, nc, bc), nc, ntx)] = 1;
                 marked = true;
             }
         }
@@ -594,6 +639,37 @@ __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC
         if (wmarked) *P.marked = 1;
         P.runs[tile] += 1;  // a single-address atomic from every tile would serialise
     }
+}
+
+template <int TH, bool MH, int NM>
+__global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC bc, PassArgs P)
+{
+    if (P.prev_marked && *P.prev_marked == 0) return;
+    tile_sp_body<TH, MH, NM>(grid_tile(), A, nr, nc, bc, P);
+}
+
+// batched: A holds the concatenated arrays; each tile's grid supplies its
+// offset and stream-power parameters; tiles of grids whose exponent mode
+// is not this kernel's are left to that mode's launch
+template <int TH, bool MH, int NM>
+__global__ void __launch_bounds__(WL) kb_tile_sp(BatchTiles B, SpTileArgs A, PassArgs P)
+{
+    if (P.prev_marked && *P.prev_marked == 0) return;
+    BatchGrid g;
+    const TileAt T = batch_tile(B, g);
+    if (g.sp_mode != sp_mode_of(MH, NM)) return;
+    SpTileArgs a = A;
+    a.rcv += g.off;
+    a.dir += g.off;
+    a.mask += g.off;
+    a.cnt += g.off;
+    a.xdon += g.off;
+    a.z_old += g.off;
+    a.z += g.off;
+    a.area = nullptr;
+    a.K = nullptr;
+    a.p = g.sp;
+    tile_sp_body<TH, MH, NM>(T, a, g.nr, g.nc, g.bc, P);
 }
 
 // ---------------------------------------------------------------------------
@@ -1546,6 +1622,43 @@ void with_tile_height(int th, F&& f)
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// batched sweeps: one launch covers the tiles of every grid of a batch
+// ---------------------------------------------------------------------------
+void batch_tile_order(int th, const BatchTiles& B, int ntiles, const int* rcv, const uint8_t* mask, short* ord,
+                      short* lstart, int* maxdep, uint8_t* xdon, hipStream_t s)
+{
+    with_tile_height(th, [&](auto H) {
+        kb_tile_order<decltype(H)::value><<<ntiles, WL, 0, s>>>(B, rcv, mask, ord, lstart, maxdep, xdon);
+    });
+    LSDTT_LAUNCH_CHECK();
+}
+
+void batch_tile_counts(int th, const BatchTiles& B, int ntiles, const int* rcv, const uint8_t* mask,
+                       const short* ord, const short* lstart, const int* maxdep, const uint8_t* xdon, int* cnt,
+                       const PassArgs& P, hipStream_t s)
+{
+    with_tile_height(th, [&](auto H) {
+        kb_tile_counts<decltype(H)::value><<<ntiles, WL, 0, s>>>(B, rcv, mask, ord, lstart, maxdep, xdon, cnt, P);
+    });
+    LSDTT_LAUNCH_CHECK();
+}
+
+void batch_tile_sp(int th, unsigned modes, const BatchTiles& B, int ntiles, const SpTileArgs& A, const PassArgs& P,
+                   hipStream_t s)
+{
+    with_tile_height(th, [&](auto H) {
+        constexpr int h = decltype(H)::value;
+        if (modes & (1u << sp_mode_of(false, 0))) kb_tile_sp<h, false, 0><<<ntiles, WL, 0, s>>>(B, A, P);
+        if (modes & (1u << sp_mode_of(false, 1))) kb_tile_sp<h, false, 1><<<ntiles, WL, 0, s>>>(B, A, P);
+        if (modes & (1u << sp_mode_of(false, 2))) kb_tile_sp<h, false, 2><<<ntiles, WL, 0, s>>>(B, A, P);
+        if (modes & (1u << sp_mode_of(true, 0))) kb_tile_sp<h, true, 0><<<ntiles, WL, 0, s>>>(B, A, P);
+        if (modes & (1u << sp_mode_of(true, 1))) kb_tile_sp<h, true, 1><<<ntiles, WL, 0, s>>>(B, A, P);
+        if (modes & (1u << sp_mode_of(true, 2))) kb_tile_sp<h, true, 2><<<ntiles, WL, 0, s>>>(B, A, P);
+    });
+    LSDTT_LAUNCH_CHECK();
+}
 
 bool FluvialTiles::valid_tile_height(int th)
 {

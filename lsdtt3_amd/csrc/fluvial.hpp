@@ -31,6 +31,69 @@ struct SpParams {
     double width;         // w / dx or 1 (multiplies K)
 };
 
+// Pass bookkeeping of the tile sweeps.  Pass p reads act[p % 3], marks
+// act[(p + 1) % 3] and clears act[(p + 2) % 3] for pass p + 1; it does
+// nothing when pass p - 1 marked no tile (the sweep has settled).
+struct PassArgs {
+    const int* act;
+    int* act_next;
+    int* act_clear;
+    const int* prev_marked;  // null for the first pass
+    int* marked;
+    int* runs;  // per tile: times run in this settle (one writer per pass)
+};
+
+// inputs and outputs of a stream-power sweep
+struct SpTileArgs {
+    const int* rcv;
+    const uint8_t* dir;
+    const uint8_t* mask;
+    const int* cnt;
+    const double* area;  // null: count * pixel_area
+    const short* ord;
+    const short* lstart;
+    const int* tile_maxdep;
+    const uint8_t* xdon;
+    const double* z_old;
+    const double* K;
+    double* z;
+    int* tile_eroded;
+    SpParams p;
+};
+
+// exponent mode of a stream-power kernel: m == 0.5 (sqrt), and n == 1
+// (closed form), n - 1 == 0.5 (sqrt) or general (pow)
+__host__ __device__ constexpr int sp_mode_of(bool m_half, int n_mode) { return (m_half ? 3 : 0) + n_mode; }
+
+// A batch of grids swept together: each grid's per-cell arrays are one
+// slice of concatenated arrays (receivers hold indices within the grid),
+// each of its 32-column tiles one slot of the per-tile arrays.
+struct BatchGrid {
+    int nr, nc;
+    BC bc;
+    int ntx;        // tile columns
+    int tbase;      // slot of the grid's first routing tile
+    long long off;  // first cell in the concatenated arrays
+    SpParams sp;    // stream power (K as a scalar, no area field)
+    int sp_mode;    // sp_mode_of for sp
+};
+struct BatchTile {
+    int inst, tx, ty;
+};
+struct BatchTiles {
+    const BatchGrid* grids;
+    const BatchTile* tiles;  // slot -> grid and position
+};
+
+void batch_tile_order(int th, const BatchTiles& B, int ntiles, const int* rcv, const uint8_t* mask, short* ord,
+                      short* lstart, int* maxdep, uint8_t* xdon, hipStream_t s);
+void batch_tile_counts(int th, const BatchTiles& B, int ntiles, const int* rcv, const uint8_t* mask,
+                       const short* ord, const short* lstart, const int* maxdep, const uint8_t* xdon, int* cnt,
+                       const PassArgs& P, hipStream_t s);
+// one pass of every exponent mode in `modes` (bit sp_mode_of)
+void batch_tile_sp(int th, unsigned modes, const BatchTiles& B, int ntiles, const SpTileArgs& A, const PassArgs& P,
+                   hipStream_t s);
+
 class FluvialTiles {
 public:
     static constexpr int TW = 32;

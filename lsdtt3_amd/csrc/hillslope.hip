@@ -766,6 +766,538 @@ __global__ void __launch_bounds__(256) k_trial_fin(double* partials, unsigned co
     }
 }
 
+// ---------------------------------------------------------------------------
+// batched solves (BatchHs): one block per 64 x 16 stencil tile of any grid,
+// or per chunk of 256 general DOFs; partial slot = block (tiles first, then
+// the chunks); per-grid finishers fold a grid's slots in that order
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ HsDom batch_dom(const BatchHs& B, const BatchHsGrid& g, double inv_dt, int zn)
+{
+    HsDom d;
+    d.nr = g.nr;
+    d.nc = g.nc;
+    d.bc = g.bc;
+    d.mask = B.mask + g.off;
+    d.dof = B.dof + g.off;
+    d.fast = B.fast + g.off;
+    d.gen = B.gen + g.gen_off;
+    d.n_gen = g.n_gen;
+    d.h = g.h;
+    d.D = g.D;
+    d.U = 0.0;
+    d.Sc = g.Sc;
+    d.cap = g.cap;
+    d.inv_dt = inv_dt;
+    d.nonlinear = g.nonlinear;
+    d.zn = B.zbuf[zn] + g.off;
+    d.inv_h = 1.0 / d.h;
+    d.inv_2h = 0.5 / d.h;
+    d.inv_Sc = 1.0 / d.Sc;
+    d.inv_Sc2 = 1.0 / (d.Sc * d.Sc);
+    d.c9g = B.c9g + 9ll * g.gen_off;
+    return d;
+}
+
+__device__ __forceinline__ bool trial_phase(int ph) { return ph == HS_START || ph == HS_SEARCH; }
+
+// this block's stencil tile: grid and corner
+struct BTileAt {
+    int gi, r0, c0;
+};
+__device__ __forceinline__ BTileAt btile(const BatchHs& B, int slot)
+{
+    const int gi = B.st_grid[slot];
+    const BatchHsGrid& g = B.grids[gi];
+    const int t = slot - g.stbase;
+    return BTileAt{gi, (t / g.tx_n) * kTH, (t % g.tx_n) * kTW};
+}
+
+// trial of the START / SEARCH grids on their fast cells (k_trial's tile body)
+__global__ void __launch_bounds__(256) kb_trial(BatchHs B)
+{
+    const BTileAt T = btile(B, blockIdx.x);
+    const HsCtl& c = B.ctl[T.gi];
+    const int ph = c.phase;
+    if (!trial_phase(ph)) return;
+    const BatchHsGrid& g = B.grids[T.gi];
+    const double lam = ph == HS_START ? 0.0 : c.lambda;
+    const bool has_dz = lam != 0.0 && c.kits != 0;  // no Krylov iteration ran: the step is zero
+    const HsDom d = batch_dom(B, g, c.inv_dt, c.zn);
+    const double* z = B.zbuf[c.zc] + g.off;
+    double* z_out = B.zbuf[c.zt] + g.off;
+    const float* dz = B.x + g.off;
+    float* b = B.b + g.off;
+    float* dinv = B.dinv + g.off;
+    __shared__ double zs[kSH * kSW];
+    const int nr = d.nr, nc = d.nc, r0 = T.r0, c0 = T.c0;
+    const int cl = threadIdx.x & 63;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (has_dz) stage_tile<true, false>(z, dz, lam, nullptr, nr, nc, r0, c0, zs, nullptr);
+    else stage_tile<false, false>(z, nullptr, 0.0, nullptr, nr, nc, r0, c0, zs, nullptr);
+    long long f[kTH / 4];
+    bool in[kTH / 4];
+    uint8_t fl[kTH / 4], dl[kTH / 4];
+    double znl[kTH / 4];
+#pragma unroll
+    for (int k = 0; k < kTH / 4; ++k) {
+        const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
+        in[k] = rr < nr && cc < nc;
+        f[k] = (long long)min(rr, nr - 1) * nc + min(cc, nc - 1);
+        fl[k] = d.fast[f[k]];
+        dl[k] = d.dof[f[k]];
+        znl[k] = d.zn[f[k]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kTH / 4; ++k) {
+        if (!in[k]) continue;
+        if (!fl[k]) {
+            if (!dl[k]) {
+                b[f[k]] = 0.0f;
+                dinv[f[k]] = 0.0f;
+                if (lam != 0.0) z_out[f[k]] = z[f[k]];
+            }
+            continue;
+        }
+        double zz[9];
+        tile_z9(zs, (threadIdx.x >> 6) + 4 * k, cl, zz);
+        double diag;
+        const double F = eval_fast(d, zz, znl[k], &diag);
+        acc[0] = fmax(acc[0], fabs(F));
+        if (d.nonlinear) {
+            const double sx = (zz[5] - zz[3]) * d.inv_2h, sy = (zz[7] - zz[1]) * d.inv_2h;
+            acc[1] = fmax(acc[1], sx * sx + sy * sy);
+        }
+        const float bf = static_cast<float>(-F);
+        b[f[k]] = bf;
+        dinv[f[k]] = diag != 0.0 ? __builtin_amdgcn_rcpf(static_cast<float>(diag)) : 0.0f;
+        acc[2] += static_cast<double>(bf) * bf;
+        if (lam != 0.0) z_out[f[k]] = zz[4];
+    }
+    const bool mx[3] = {true, true, false};
+    block_partial<3>(acc, mx, B.partials, blockIdx.x);
+}
+
+// trial of the general DOFs (k_trial_gen's body), one chunk of 256 per block
+__global__ void __launch_bounds__(256) kb_trial_gen(BatchHs B)
+{
+    const int slot = blockIdx.x;
+    const int gi = B.gc_grid[slot];
+    const HsCtl& c = B.ctl[gi];
+    const int ph = c.phase;
+    if (!trial_phase(ph)) return;
+    const BatchHsGrid& g = B.grids[gi];
+    const double lam = ph == HS_START ? 0.0 : c.lambda;
+    const bool has_dz = lam != 0.0 && c.kits != 0;
+    const HsDom d = batch_dom(B, g, c.inv_dt, c.zn);
+    const double* z = B.zbuf[c.zc] + g.off;
+    double* z_out = B.zbuf[c.zt] + g.off;
+    const ZView Z{z, has_dz ? B.x + g.off : nullptr, lam};
+    float* b = B.b + g.off;
+    float* dinv = B.dinv + g.off;
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int i = (slot - g.gcbase) * 256 + threadIdx.x;
+    if (i < d.n_gen) {
+        const int f = d.gen[i];
+        const int r = f / d.nc, cc = f - r * d.nc;
+        double c9[9];
+        const double F = cell_general(d, Z, r, cc, c9);
+        acc[0] = fabs(F);
+        if (d.nonlinear) acc[1] = slope2_general(d, Z, r, cc);
+        const float bf = static_cast<float>(-F);
+        b[f] = bf;
+        dinv[f] = c9[4] != 0.0 ? static_cast<float>(1.0 / c9[4]) : 0.0f;
+        acc[2] = static_cast<double>(bf) * bf;
+        if (lam != 0.0) z_out[f] = Z(f);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) d.c9g[(size_t)k * d.n_gen + i] = static_cast<float>(c9[k]);
+    }
+    const bool mx[3] = {true, true, false};
+    block_partial<3>(acc, mx, B.partials, B.NT + slot);
+}
+
+// folds grid g's partial slots (its tiles, then -- for kernels with chunk
+// blocks -- its general chunks) in slot order; result in thread 0
+template <int NV>
+__device__ __forceinline__ void fold_grid(const BatchHs& B, const BatchHsGrid& g, bool with_chunks,
+                                          const bool (&is_max)[NV], double (&w)[NV])
+{
+#pragma unroll
+    for (int i = 0; i < NV; ++i) w[i] = is_max[i] ? -1.7976931348623157e308 : 0.0;
+    const int n = g.nst + (with_chunks ? g.ngc : 0);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int slot = k < g.nst ? g.stbase + k : B.NT + g.gcbase + (k - g.nst);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const double x = B.partials[(size_t)slot * NV + i];
+            w[i] = is_max[i] ? (w[i] > x ? w[i] : x) : w[i] + x;
+        }
+    }
+    block_reduce<NV>(w, is_max);
+}
+
+// max |F|, max slope ratio and the Krylov start of each trial's system
+__global__ void __launch_bounds__(256) kb_trial_fin(BatchHs B)
+{
+    HsCtl& c = B.ctl[blockIdx.x];
+    if (!trial_phase(c.phase)) return;
+    const BatchHsGrid& g = B.grids[blockIdx.x];
+    const bool mx[3] = {true, true, false};
+    double out[3];
+    fold_grid<3>(B, g, true, mx, out);
+    if (threadIdx.x == 0) {
+        c.trial_f = out[0];
+        c.trial_ratio = sqrt(out[1]) * (1.0 / g.Sc);
+        c.rho = out[2];
+        c.bnorm2 = out[2];
+        c.rho_old = 1.0;
+        c.alpha = 1.0;
+        c.omega = 1.0;
+        c.kdone = out[2] == 0.0 ? 1 : 0;
+        c.kits = 0;
+        c.kbreak = 0;
+    }
+}
+
+__device__ __forceinline__ bool krylov_active(const HsCtl& c) { return c.phase == HS_KRYLOV && !c.kdone; }
+
+// one row-major pass over this block's tile: fn(f) for each cell
+template <class F>
+__device__ __forceinline__ void tile_cells(const BatchHsGrid& g, int r0, int c0, F&& fn)
+{
+    const int cc = c0 + (threadIdx.x & 63);
+#pragma unroll
+    for (int k = 0; k < kTH / 4; ++k) {
+        const int rr = r0 + (threadIdx.x >> 6) + 4 * k;
+        if (rr < g.nr && cc < g.nc) fn((long long)rr * g.nc + cc);
+    }
+}
+
+// p = r + beta (p - omega v); phat = M^-1 p  (first iteration: p = b)
+__global__ void __launch_bounds__(256) kb_bi_p(BatchHs B)
+{
+    const BTileAt T = btile(B, blockIdx.x);
+    const HsCtl& c = B.ctl[T.gi];
+    if (!krylov_active(c)) return;
+    const BatchHsGrid& g = B.grids[T.gi];
+    const double beta = (c.rho / c.rho_old) * (c.alpha / c.omega);
+    const float bt = static_cast<float>(beta), om = static_cast<float>(c.omega);
+    const bool first = c.kits == 0;
+    const float* b = B.b + g.off;
+    const float* r = B.r + g.off;
+    const float* v = B.v + g.off;
+    const float* dinv = B.dinv + g.off;
+    float* p = B.p + g.off;
+    float* phat = B.phat + g.off;
+    tile_cells(g, T.r0, T.c0, [&](long long f) {
+        const float pv = first ? b[f] : r[f] + bt * (p[f] - om * v[f]);
+        p[f] = pv;
+        phat[f] = dinv[f] * pv;
+    });
+}
+
+// y = J x on the tile's fast rows (0 off the DOFs) and dot(w, y); the
+// general rows come from the chunk blocks
+template <int NV, class Acc>
+__device__ __forceinline__ void batch_matvec(const BatchHs& B, const float* xin, float* yout, Acc&& accum,
+                                             double (&acc)[NV])
+{
+    __shared__ double zs[kSH * kSW];
+    __shared__ float xs[kSH * kSW];
+    if (blockIdx.x < (unsigned)B.NT) {
+        const BTileAt T = btile(B, blockIdx.x);
+        const HsCtl& c = B.ctl[T.gi];
+        if (!krylov_active(c)) return;
+        const BatchHsGrid& g = B.grids[T.gi];
+        const HsDom d = batch_dom(B, g, c.inv_dt, c.zn);
+        const double* z = B.zbuf[c.zc] + g.off;
+        const float* x = xin + g.off;
+        float* y = yout + g.off;
+        const int nr = g.nr, nc = g.nc, r0 = T.r0, c0 = T.c0, cl = threadIdx.x & 63;
+        stage_tile<false, true>(z, nullptr, 0.0, x, nr, nc, r0, c0, zs, xs);
+        long long f[kTH / 4];
+        bool in[kTH / 4];
+        uint8_t fl[kTH / 4], dl[kTH / 4];
+#pragma unroll
+        for (int k = 0; k < kTH / 4; ++k) {
+            const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
+            in[k] = rr < nr && cc < nc;
+            f[k] = (long long)min(rr, nr - 1) * nc + min(cc, nc - 1);
+            fl[k] = d.fast[f[k]];
+            dl[k] = d.dof[f[k]];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTH / 4; ++k) {
+            if (!in[k]) continue;
+            if (fl[k]) {
+                const float yv = jx_tile(d, zs, xs, (threadIdx.x >> 6) + 4 * k, cl);
+                y[f[k]] = yv;
+                accum(g.off + f[k], yv);
+            } else if (!dl[k]) {
+                y[f[k]] = 0.0f;
+            }
+        }
+    } else {
+        const int slot = blockIdx.x - B.NT;
+        const int gi = B.gc_grid[slot];
+        const HsCtl& c = B.ctl[gi];
+        if (!krylov_active(c)) return;
+        const BatchHsGrid& g = B.grids[gi];
+        const HsDom d = batch_dom(B, g, c.inv_dt, c.zn);
+        const int i = (slot - g.gcbase) * 256 + threadIdx.x;
+        if (i < d.n_gen) {
+            const int f = d.gen[i];
+            const float yv = jx_gen(d, xin + g.off, i, f);
+            yout[g.off + f] = yv;
+            accum(g.off + f, yv);
+        }
+    }
+    const bool mx[NV] = {};
+    block_partial<NV>(acc, mx, B.partials, blockIdx.x);
+}
+
+// v = J phat, partial rhat . v
+__global__ void __launch_bounds__(256) kb_bi_v(BatchHs B)
+{
+    double acc[1] = {0.0};
+    const float* rhat = B.b;
+    batch_matvec<1>(B, B.phat, B.v, [&](long long f, float y) { acc[0] += static_cast<double>(rhat[f]) * y; }, acc);
+}
+
+// t = J shat, partials t . s and t . t
+__global__ void __launch_bounds__(256) kb_bi_t(BatchHs B)
+{
+    double acc[2] = {0.0, 0.0};
+    const float* sv = B.s;
+    batch_matvec<2>(B, B.shat, B.t, [&](long long f, float y) {
+        acc[0] += static_cast<double>(y) * sv[f];
+        acc[1] += static_cast<double>(y) * y;
+    }, acc);
+}
+
+// alpha = rho / (rhat . v) per grid (breakdown stops that grid's solve)
+__global__ void __launch_bounds__(256) kb_bi_vfin(BatchHs B)
+{
+    HsCtl& c = B.ctl[blockIdx.x];
+    if (!krylov_active(c)) return;
+    const bool mx[1] = {false};
+    double out[1];
+    fold_grid<1>(B, B.grids[blockIdx.x], true, mx, out);
+    if (threadIdx.x == 0) {
+        if (out[0] == 0.0) {
+            c.kdone = 1;
+            c.kbreak = 1;
+        } else {
+            c.alpha = c.rho / out[0];
+        }
+    }
+}
+
+// s = r - alpha v; shat = M^-1 s
+__global__ void __launch_bounds__(256) kb_bi_s(BatchHs B)
+{
+    const BTileAt T = btile(B, blockIdx.x);
+    const HsCtl& c = B.ctl[T.gi];
+    if (!krylov_active(c)) return;
+    const BatchHsGrid& g = B.grids[T.gi];
+    const float al = static_cast<float>(c.alpha);
+    const float* r = (c.kits == 0 ? B.b : B.r) + g.off;
+    const float* v = B.v + g.off;
+    const float* dinv = B.dinv + g.off;
+    float* sv = B.s + g.off;
+    float* shat = B.shat + g.off;
+    tile_cells(g, T.r0, T.c0, [&](long long f) {
+        const float s1 = r[f] - al * v[f];
+        sv[f] = s1;
+        shat[f] = dinv[f] * s1;
+    });
+}
+
+// omega = (t . s) / (t . t) per grid
+__global__ void __launch_bounds__(256) kb_bi_tfin(BatchHs B)
+{
+    HsCtl& c = B.ctl[blockIdx.x];
+    if (!krylov_active(c)) return;
+    const bool mx[2] = {false, false};
+    double out[2];
+    fold_grid<2>(B, B.grids[blockIdx.x], true, mx, out);
+    if (threadIdx.x == 0) c.omega = out[1] > 0.0 ? out[0] / out[1] : 0.0;
+}
+
+// x += alpha phat + omega shat; r = s - omega t; partials rhat . r, r . r
+__global__ void __launch_bounds__(256) kb_bi_x(BatchHs B)
+{
+    const BTileAt T = btile(B, blockIdx.x);
+    const HsCtl& c = B.ctl[T.gi];
+    if (!krylov_active(c)) return;
+    const BatchHsGrid& g = B.grids[T.gi];
+    const float al = static_cast<float>(c.alpha), om = static_cast<float>(c.omega);
+    const bool first = c.kits == 0;
+    const float* phat = B.phat + g.off;
+    const float* shat = B.shat + g.off;
+    const float* sv = B.s + g.off;
+    const float* tv = B.t + g.off;
+    const float* rhat = B.b + g.off;
+    float* x = B.x + g.off;
+    float* r = B.r + g.off;
+    double acc[2] = {0.0, 0.0};
+    tile_cells(g, T.r0, T.c0, [&](long long f) {
+        const float dx = al * phat[f] + om * shat[f];
+        x[f] = first ? dx : x[f] + dx;
+        const float rv = sv[f] - om * tv[f];
+        r[f] = rv;
+        acc[0] += static_cast<double>(rhat[f]) * rv;
+        acc[1] += static_cast<double>(rv) * rv;
+    });
+    const bool mx[2] = {false, false};
+    block_partial<2>(acc, mx, B.partials, blockIdx.x);
+}
+
+// rho, ||r||^2 and the convergence / breakdown flags per grid
+__global__ void __launch_bounds__(256) kb_bi_xfin(BatchHs B)
+{
+    HsCtl& c = B.ctl[blockIdx.x];
+    if (!krylov_active(c)) return;
+    const BatchHsGrid& g = B.grids[blockIdx.x];
+    const bool mx[2] = {false, false};
+    double out[2];
+    fold_grid<2>(B, g, false, mx, out);  // kb_bi_x has tile blocks only
+    if (threadIdx.x == 0) {
+        c.rho_old = c.rho;
+        c.rho = out[0];
+        c.rnorm2 = out[1];
+        const int it = c.kits + 1;
+        c.kits = it;
+        const bool conv = out[1] <= g.eta2 * c.bnorm2;
+        const bool stall = (c.omega == 0.0) || (out[0] == 0.0);
+        if (conv || it >= g.krylov_max || stall) c.kdone = 1;
+        if (stall && !conv) c.kbreak = 1;
+    }
+}
+
+// the trial buffer after zc and zn in the order 1, 2, 3
+__device__ __forceinline__ int free_buffer(int zc, int zn)
+{
+    for (int q = 1; q <= 3; ++q)
+        if (q != zc && q != zn) return q;
+    return 1;
+}
+
+// a sub-step's Newton solve ended (its accepted iterations; converged or
+// not): the sub-step controller of compute_hillslope_newton_transient
+// (lsdtt_hillslope_newton_cpu.cpp:690-788)
+__device__ __forceinline__ void end_solve(HsCtl& c, int its, bool converged)
+{
+    c.newton_total += its;
+    if (!converged) {
+        c.zc = c.zn;
+        c.zt = free_buffer(c.zc, c.zn);
+        ++c.rejected;
+        if (c.rejected > 40 || c.step <= c.dt_min) {
+            c.phase = HS_FAILED;
+            return;
+        }
+        c.dt = fmax(c.dt_min, c.step * 0.5);
+    } else {
+        c.remaining -= c.step;
+        if (c.remaining > 0.0) {
+            c.zn = c.zc;
+            c.zt = free_buffer(c.zc, c.zn);
+        }
+        ++c.substeps;
+        if (its <= 4) c.dt = fmin(c.dt_max, c.step * 1.7);
+        else if (its >= 15) c.dt = fmax(c.dt_min, c.step * 0.5);
+        if (!(c.remaining > 0.0)) {
+            c.phase = HS_DONE;
+            return;
+        }
+    }
+    c.step = fmin(c.dt, c.remaining);
+    c.inv_dt = 1.0 / c.step;
+    c.phase = HS_START;
+}
+
+// the host loop's decisions, per grid (one thread each).  stage 0 follows
+// the Krylov iteration: a finished solve starts its line search.  stage 1
+// follows the trials: a START trial sets the tolerance, a SEARCH trial is
+// accepted or halves lambda (compute_hillslope_newton's line search,
+// lsdtt_hillslope_newton_cpu.cpp:553-633, with restoration mode while the
+// iterate's slope ratio is above sqrt(cap)).
+__global__ void kb_control(BatchHs B, int n_grids, int stage)
+{
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= n_grids) return;
+    HsCtl& c = B.ctl[gi];
+    const BatchHsGrid& g = B.grids[gi];
+    const double feasible = sqrt(g.cap);
+    if (stage == 0) {
+        if (c.phase == HS_START || c.phase == HS_KRYLOV || c.phase == HS_SEARCH) ++c.rounds;
+        if (c.phase == HS_KRYLOV && c.kdone) {
+            c.krylov_total += c.kits;
+            c.restoring = (g.nonlinear && c.ratio > feasible) ? 1 : 0;
+            if (c.restoring) ++c.restoration_its;
+            c.lambda = 1.0;
+            c.ls = 0;
+            c.phase = HS_SEARCH;
+        }
+        return;
+    }
+    if (c.phase == HS_START) {
+        c.fnorm = c.trial_f;
+        c.ratio = c.trial_ratio;
+        c.tol = fmax(g.abs_tol, g.rel_tol * c.fnorm);
+        if (c.first_solve) {
+            c.f_init = c.fnorm;
+            c.first_solve = 0;
+        }
+        c.it = 0;
+        if (c.fnorm <= c.tol) end_solve(c, 0, true);
+        else if (g.newton_max <= 0) end_solve(c, 0, false);
+        else c.phase = HS_KRYLOV;
+        return;
+    }
+    if (c.phase != HS_SEARCH) return;
+    const double ft = c.trial_f, rt = c.trial_ratio;
+    bool accept;
+    if (g.nonlinear && c.restoring) accept = rt < c.ratio;
+    else if (g.nonlinear && rt > feasible) accept = false;
+    else accept = ft < c.fnorm;
+    if (!accept) {
+        c.lambda *= 0.5;
+        if (++c.ls >= g.ls_max) end_solve(c, c.it, false);
+        return;
+    }
+    // the accepted trial point is the new iterate; its system (b, dinv,
+    // Krylov start) came with the trial
+    c.zc = c.zt;
+    c.zt = free_buffer(c.zc, c.zn);
+    c.fnorm = ft;
+    c.ratio = rt;
+    const int its = ++c.it;
+    if (c.fnorm <= c.tol) end_solve(c, its, true);
+    else if (its >= g.newton_max) end_solve(c, its, false);
+    else c.phase = HS_KRYLOV;
+}
+
+// a new step: every grid with a hillslope solve starts its first sub-step
+__global__ void kb_hs_start(BatchHs B, int n_grids)
+{
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= n_grids) return;
+    HsCtl& c = B.ctl[gi];
+    const BatchHsGrid& g = B.grids[gi];
+    c = HsCtl{};
+    c.zc = c.zn = 0;
+    c.zt = 1;
+    c.dt = c.remaining = c.dt_max = g.dt_total;
+    c.dt_min = g.dt_total * 1e-9;
+    c.step = fmin(c.dt, c.remaining);
+    c.inv_dt = 1.0 / c.step;
+    c.first_solve = 1;
+    c.phase = g.enabled ? HS_START : HS_IDLE;
+}
+
 }  // namespace
 
 int hs_general_blocks(int n_gen) { return gen_grid(n_gen); }
@@ -839,6 +1371,31 @@ void hs_trial(const HsDom& d, const double* z, const float* dz, double lam, long
     k_trial<<<grid, kB, 0, s>>>(d, z, dz, lam, n, z_out, b, dinv, partials, ticket, scal, flags);
     if (side) LSDTT_HIP(hipStreamWaitEvent(s, join, 0));
     k_trial_fin<<<1, kB, 0, s>>>(partials, (unsigned)(grid + gen_grid(d.n_gen)), d.inv_Sc, scal, flags);
+    LSDTT_LAUNCH_CHECK();
+}
+
+void hsb_start(const BatchHs& B, int n_grids, hipStream_t s)
+{
+    kb_hs_start<<<div_up(n_grids, 64), 64, 0, s>>>(B, n_grids);
+    LSDTT_LAUNCH_CHECK();
+}
+
+void hsb_round(const BatchHs& B, int n_grids, hipStream_t s)
+{
+    const int NT = B.NT, NB = B.NT + B.NGC;
+    kb_bi_p<<<NT, kB, 0, s>>>(B);
+    kb_bi_v<<<NB, kB, 0, s>>>(B);
+    kb_bi_vfin<<<n_grids, kB, 0, s>>>(B);
+    kb_bi_s<<<NT, kB, 0, s>>>(B);
+    kb_bi_t<<<NB, kB, 0, s>>>(B);
+    kb_bi_tfin<<<n_grids, kB, 0, s>>>(B);
+    kb_bi_x<<<NT, kB, 0, s>>>(B);
+    kb_bi_xfin<<<n_grids, kB, 0, s>>>(B);
+    kb_control<<<div_up(n_grids, 64), 64, 0, s>>>(B, n_grids, 0);
+    kb_trial<<<NT, kB, 0, s>>>(B);
+    if (B.NGC > 0) kb_trial_gen<<<B.NGC, kB, 0, s>>>(B);
+    kb_trial_fin<<<n_grids, kB, 0, s>>>(B);
+    kb_control<<<div_up(n_grids, 64), 64, 0, s>>>(B, n_grids, 1);
     LSDTT_LAUNCH_CHECK();
 }
 

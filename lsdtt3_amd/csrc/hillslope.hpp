@@ -67,6 +67,90 @@ enum {
     HS_RHO = 0, HS_RHO_OLD, HS_ALPHA, HS_OMEGA, HS_BNORM2, HS_RNORM2, HS_TRIAL_F, HS_TRIAL_RATIO, HS_NSCAL
 };
 
+// ---------------------------------------------------------------------------
+// Batched hillslope solves: the Newton-Krylov solves of many grids advance
+// together, one launch per kernel for all of them.  Each grid's state
+// machine lives on the device (HsCtl): a sub-step starts with a trial at
+// lambda = 0 (F, the tolerance, the first linear system), BiCGSTAB
+// iterations run until that grid's solve has converged, then line-search
+// trials until one is accepted; a control kernel takes each grid's
+// decisions (acceptance, restoration mode, convergence, sub-step growth,
+// rejection) exactly as the single-grid host loop does.  A kernel skips the
+// grids whose state does not call for it, so launches can be enqueued in a
+// fixed pattern (and captured in a graph) while each grid follows its own
+// path.  Reductions are per grid: per-tile partials folded by a per-grid
+// finisher in tile order, so a grid's arithmetic does not depend on the
+// other grids of the batch.
+// ---------------------------------------------------------------------------
+enum HsPhase : int {
+    HS_IDLE = 0,    // no hillslope this step (or not started)
+    HS_START = 1,   // needs the sub-step's first trial (lambda = 0)
+    HS_KRYLOV = 2,  // BiCGSTAB iterations of the current Newton iteration
+    HS_SEARCH = 3,  // line-search trial at ctl.lambda
+    HS_DONE = 4,    // converged over the whole dt; result in buffer zc
+    HS_FAILED = 5,  // Newton failed at the minimum sub-step (Picard fallback)
+};
+
+struct HsCtl {
+    int phase;
+    int zc, zn, zt;  // buffer ids: 0 = the post-incision surface, 1..3 trial buffers
+    int it, ls, restoring;
+    int substeps, rejected, newton_total, krylov_total, restoration_its;
+    int kdone, kits, kbreak;  // BiCGSTAB: converged or stopped, iterations, breakdown
+    int first_solve;          // the next START is the step's first solve
+    int rounds;               // rounds in which the grid was still solving
+    int stats_done;           // the step's statistics have been taken
+    int method;               // 0 Newton, 1 damped Picard (after a failure)
+    double inv_dt, step, dt, remaining, dt_min, dt_max;
+    double fnorm, ratio, tol, lambda, f_init;
+    double trial_f, trial_ratio;
+    double rho, rho_old, alpha, omega, bnorm2, rnorm2;
+};
+
+// one grid of a batched hillslope solve (cells of the concatenated arrays
+// from `off`; 64 x 16 stencil tiles in slots stbase .. stbase + nst - 1;
+// its general DOFs (grid edge, next to a masked cell) at gen_off in the
+// concatenated lists, handled in chunks of 256 by slots gcbase ..)
+struct BatchHsGrid {
+    int enabled;  // the grid's step has a hillslope solve
+    int nr, nc;
+    BC bc;
+    long long off;
+    int stbase, nst, tx_n;
+    int gen_off, n_gen, gcbase, ngc;
+    double h, D, Sc, cap;
+    int nonlinear;
+    double eta2;       // Krylov relative tolerance squared
+    int krylov_max;
+    int newton_max, ls_max;
+    double abs_tol, rel_tol, dt_total;
+};
+
+struct BatchHs {
+    const BatchHsGrid* grids;
+    const int* st_grid;  // stencil-tile slot -> grid
+    const int* gc_grid;  // general-DOF chunk slot -> grid
+    int NT, NGC;         // stencil tiles, general-DOF chunks
+    HsCtl* ctl;
+    const uint8_t* mask;
+    const uint8_t* dof;
+    const uint8_t* fast;
+    const int* gen;  // per grid: its general DOFs (cell indices within the grid), row-major
+    float* c9g;      // per grid: 9 x n_gen Jacobian rows at 9 * gen_off, slot-major
+    double* zbuf[4];
+    float *b, *dinv, *x, *r, *p, *phat, *v, *s, *shat, *t;
+    double* partials;  // (NT + NGC) slots of up to 3 values
+};
+
+// Batched launches over every grid of B (n_grids of them).
+// hsb_start: each enabled grid enters START with z (buffer 0) as z^n and the
+// whole dt as its first sub-step; the others IDLE.
+void hsb_start(const BatchHs& B, int n_grids, hipStream_t s);
+// One round: a BiCGSTAB iteration of the KRYLOV grids, the control step
+// that sends finished solves to their line search, the trials of the START
+// and SEARCH grids and the control step that judges them.
+void hsb_round(const BatchHs& B, int n_grids, hipStream_t s);
+
 void hs_fast_flags(const HsDom& d, uint8_t* fast, hipStream_t s);
 // companion-kernel blocks for n_gen general cells (their partials follow the
 // hot kernel's `grid` partials, so `partials` holds grid + this many slots)

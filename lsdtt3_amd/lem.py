@@ -311,14 +311,15 @@ class ConvergenceMonitor:
 
 class Ensemble:
     """Independent LEM instances (uneven grids, per-instance parameters)
-    stepped concurrently on the GPU (lsdtt_gpu_ensemble_*).  ``models`` is a
-    list of dicts with ``dem``, ``cell_size``, optional ``mask`` and the
-    LandscapeModel keyword options."""
+    stepped together on the GPU (lsdtt_gpu_ensemble_*): every kernel of a
+    step runs once for all instances and the step's launches replay from a
+    captured hipGraph.  ``models`` is a list of dicts with ``dem``,
+    ``cell_size``, optional ``mask`` and the LandscapeModel keyword
+    options.  An instance's results do not depend on the other instances."""
 
-    def __init__(self, models, workers=16):
+    def __init__(self, models, workers=0):
         L = N.lib()
         self.shapes = []
-        self._keep = []
         nr, nc, dx, masks = [], [], [], []
         opts = (LemOptions * len(models))()
         dems = []
@@ -359,19 +360,42 @@ class Ensemble:
         N.check(N.lib().lsdtt_gpu_ensemble_run(self._h, int(n_steps), st, C.byref(sec)))
         return [s.as_dict() for s in st], sec.value
 
-    def set_tile_height(self, i, th):
-        N.check(N.lib().lsdtt_gpu_ensemble_set_tile_height(self._h, int(i), int(th)))
+    def set_graph(self, enable: bool) -> None:
+        """graph replay of the step's launch pattern (default on)"""
+        N.check(N.lib().lsdtt_gpu_ensemble_set_graph(self._h, int(bool(enable))))
+
+    def pattern(self) -> dict:
+        """launch-pattern counters: graph replays and captures, rounds and
+        passes launched outside the pattern, restarted steps, current caps"""
+        v = [C.c_int64() for _ in range(5)]
+        caps = (C.c_int32 * 3)()
+        N.check(N.lib().lsdtt_gpu_ensemble_pattern(self._h, *[C.byref(x) for x in v], caps))
+        return dict(graph_launches=v[0].value, graph_captures=v[1].value, extra_rounds=v[2].value,
+                    extra_passes=v[3].value, restarts=v[4].value, count_passes=caps[0], stream_power_passes=caps[1],
+                    rounds=caps[2])
+
+    @property
+    def tile_height(self) -> int:
+        return int(N.lib().lsdtt_gpu_ensemble_tile_height(self._h))
+
+    @tile_height.setter
+    def tile_height(self, th: int) -> None:
+        N.check(N.lib().lsdtt_gpu_ensemble_set_tile_height(self._h, int(th)))
 
     def autotune_tiles(self, steps=2):
-        """chooses every instance's tile height by measurement (per size
-        class alone, then the whole ensemble with each uniform height and
-        with the per-class heights); returns (heights, seconds taken)"""
-        th = (C.c_int32 * self.n)()
-        sec = C.c_double(0.0)
-        N.check(N.lib().lsdtt_gpu_ensemble_autotune_tiles(self._h, int(steps), th, C.byref(sec)))
-        return list(th), sec.value
+        """times `steps` steps of the ensemble per tile height (surfaces
+        restored) and keeps the fastest; returns (height, {height: ms per step})"""
+        th = C.c_int32(0)
+        ms = (C.c_double * 3)()
+        N.check(N.lib().lsdtt_gpu_ensemble_autotune_tiles(self._h, int(steps), C.byref(th), ms))
+        return th.value, {h: ms[i] for i, h in enumerate((8, 16, 32))}
 
     def elevation(self, i):
         z = np.empty(self.shapes[i], np.float64)
         N.check(N.lib().lsdtt_gpu_ensemble_get_elevation(self._h, int(i), N.ptr(z)))
         return z
+
+    def erosion_rate(self, i):
+        out = np.empty(self.shapes[i], np.float32)
+        N.check(N.lib().lsdtt_gpu_ensemble_get_erosion_rate(self._h, int(i), N.ptr(out)))
+        return out
