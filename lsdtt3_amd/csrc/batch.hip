@@ -518,6 +518,10 @@ void BatchEngine::build_layout(const std::vector<BatchSpec>& specs)
     if (NGC) LSDTT_HIP(hipMemcpy(gc_grid_d.p, gc_grid_h.data(), sizeof(int) * NGC, hipMemcpyHostToDevice));
     ctl.reserve(G);
     LSDTT_HIP(hipMemset(ctl.p, 0, sizeof(HsCtl) * G));
+    wl_inst.reserve(static_cast<size_t>(WL_N) * G);
+    wl_pre.reserve(static_cast<size_t>(WL_N) * (G + 1));
+    wl_hdr.reserve(2 * WL_N);
+    LSDTT_HIP(hipMemset(wl_hdr.p, 0, sizeof(int) * 2 * WL_N));
     partials.reserve(static_cast<size_t>(NT + NGC) * 3);
     spartials.reserve(static_cast<size_t>(NT) * 8);
     stats_out.reserve(static_cast<size_t>(G) * 8);
@@ -585,6 +589,10 @@ void BatchEngine::rebuild_routing_tables()
 BatchHs BatchEngine::bhs()
 {
     BatchHs B;
+    B.G = G;
+    B.wl_inst = wl_inst.p;
+    B.wl_pre = wl_pre.p;
+    B.wl_hdr = wl_hdr.p;
     B.grids = hsgrids_d.p;
     B.st_grid = st_grid_d.p;
     B.gc_grid = gc_grid_d.p;
@@ -892,10 +900,13 @@ void BatchEngine::finish_step(std::vector<LemStats>& st, hipStream_t s, int* pic
         if (!c.stats_done) pending = true;
     }
     if (pending) enqueue_stats(s);
-    // rounds for the next pattern: what the slowest grid needed, plus two;
-    // shrinks only when well above that
+    // rounds of the next pattern: what the slowest grid needed plus a
+    // quarter (a changed pattern is captured again, so it grows with a
+    // margin and shrinks only when well above the need; an empty round
+    // costs a few short launches)
     const int want = max_rounds + 2;
-    if (want > plan.rounds || want + 4 < plan.rounds) plan.rounds = std::max(want, 1);
+    if (want > plan.rounds) plan.rounds = want + std::max(2, want / 4);
+    else if (want < plan.rounds / 2) plan.rounds = std::max(want + 2, plan.rounds * 3 / 4);
     LSDTT_HIP(hipMemcpyAsync(ctl_h.data(), ctl.p, sizeof(HsCtl) * G, hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipMemcpyAsync(stats_h.data(), stats_out.p, sizeof(double) * 8 * G, hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipMemcpyAsync(eroded_h.data(), eroded_out.p, sizeof(long long) * G, hipMemcpyDeviceToHost, s));

@@ -112,6 +112,7 @@ private:
     dbuf<BatchHsGrid> hsgrids_d;
     dbuf<int> st_grid_d, gc_grid_d;
     dbuf<HsCtl> ctl;
+    dbuf<int> wl_inst, wl_pre, wl_hdr;  // work lists of the batched hillslope kernels
     int G = 0, RT = 0, NT = 0, NGC = 0, th = 16;
     long long N = 0;  // cells of the concatenated arrays
     unsigned sp_modes = 0;

@@ -812,16 +812,39 @@ __device__ __forceinline__ BTileAt btile(const BatchHs& B, int slot)
     return BTileAt{gi, (t / g.tx_n) * kTH, (t % g.tx_n) * kTW};
 }
 
-// trial of the START / SEARCH grids on their fast cells (k_trial's tile body)
-__global__ void __launch_bounds__(256) kb_trial(BatchHs B)
+// Work lists: the instances a kernel covers this round (those in its
+// phase, as the control kernel found them), each with the running count of
+// its items (stencil tiles or general-DOF chunks).  Kernels run a fixed
+// grid whose blocks walk the items, so a round in which few instances are
+// still solving costs launches over their tiles only.  Item w of list L
+// is the tile (or chunk) slot of its instance; partials go to that slot,
+// so the per-instance reductions are the same whatever the list.
+__device__ __forceinline__ int wl_total(const BatchHs& B, int list) { return B.wl_hdr[2 * list + 1]; }
+
+__device__ __forceinline__ int wl_slot(const BatchHs& B, int list, int w, bool chunks)
 {
-    const BTileAt T = btile(B, blockIdx.x);
+    const int* pre = B.wl_pre + (size_t)list * (B.G + 1);
+    const int* inst = B.wl_inst + (size_t)list * B.G;
+    int lo = 0, hi = B.wl_hdr[2 * list] - 1;
+    while (lo < hi) {  // the last instance whose first item is <= w
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchHsGrid& g = B.grids[inst[lo]];
+    return (chunks ? g.gcbase : g.stbase) + (w - pre[lo]);
+}
+
+// trial of the START / SEARCH grids on their fast cells (k_trial's tile body)
+__device__ __forceinline__ void trial_tile(const BatchHs& B, int slot)
+{
+    const BTileAt T = btile(B, slot);
     const HsCtl& c = B.ctl[T.gi];
     const int ph = c.phase;
     if (!trial_phase(ph)) return;
     const BatchHsGrid& g = B.grids[T.gi];
     const double lam = ph == HS_START ? 0.0 : c.lambda;
-    const bool has_dz = lam != 0.0 && c.kits != 0;  // no Krylov iteration ran: the step is zero
+    const bool has_dz = lam != 0.0 && !c.zero_step;
     const HsDom d = batch_dom(B, g, c.inv_dt, c.zn);
     const double* z = B.zbuf[c.zc] + g.off;
     double* z_out = B.zbuf[c.zt] + g.off;
@@ -875,20 +898,28 @@ __global__ void __launch_bounds__(256) kb_trial(BatchHs B)
         if (lam != 0.0) z_out[f[k]] = zz[4];
     }
     const bool mx[3] = {true, true, false};
-    block_partial<3>(acc, mx, B.partials, blockIdx.x);
+    block_partial<3>(acc, mx, B.partials, slot);
 }
 
-// trial of the general DOFs (k_trial_gen's body), one chunk of 256 per block
-__global__ void __launch_bounds__(256) kb_trial_gen(BatchHs B)
+__global__ void __launch_bounds__(256) kb_trial(BatchHs B)
 {
-    const int slot = blockIdx.x;
+    const int total = wl_total(B, WL_TTILE);
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        __syncthreads();
+        trial_tile(B, wl_slot(B, WL_TTILE, w, false));
+    }
+}
+
+// trial of the general DOFs (k_trial_gen's body), one chunk of 256 per item
+__device__ __forceinline__ void trial_chunk(const BatchHs& B, int slot)
+{
     const int gi = B.gc_grid[slot];
     const HsCtl& c = B.ctl[gi];
     const int ph = c.phase;
     if (!trial_phase(ph)) return;
     const BatchHsGrid& g = B.grids[gi];
     const double lam = ph == HS_START ? 0.0 : c.lambda;
-    const bool has_dz = lam != 0.0 && c.kits != 0;
+    const bool has_dz = lam != 0.0 && !c.zero_step;
     const HsDom d = batch_dom(B, g, c.inv_dt, c.zn);
     const double* z = B.zbuf[c.zc] + g.off;
     double* z_out = B.zbuf[c.zt] + g.off;
@@ -914,6 +945,15 @@ __global__ void __launch_bounds__(256) kb_trial_gen(BatchHs B)
     }
     const bool mx[3] = {true, true, false};
     block_partial<3>(acc, mx, B.partials, B.NT + slot);
+}
+
+__global__ void __launch_bounds__(256) kb_trial_gen(BatchHs B)
+{
+    const int total = wl_total(B, WL_TCHUNK);
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        __syncthreads();
+        trial_chunk(B, wl_slot(B, WL_TCHUNK, w, true));
+    }
 }
 
 // folds grid g's partial slots (its tiles, then -- for kernels with chunk
@@ -974,9 +1014,9 @@ __device__ __forceinline__ void tile_cells(const BatchHsGrid& g, int r0, int c0,
 }
 
 // p = r + beta (p - omega v); phat = M^-1 p  (first iteration: p = b)
-__global__ void __launch_bounds__(256) kb_bi_p(BatchHs B)
+__device__ __forceinline__ void bi_p_tile(const BatchHs& B, int slot)
 {
-    const BTileAt T = btile(B, blockIdx.x);
+    const BTileAt T = btile(B, slot);
     const HsCtl& c = B.ctl[T.gi];
     if (!krylov_active(c)) return;
     const BatchHsGrid& g = B.grids[T.gi];
@@ -996,16 +1036,22 @@ __global__ void __launch_bounds__(256) kb_bi_p(BatchHs B)
     });
 }
 
+__global__ void __launch_bounds__(256) kb_bi_p(BatchHs B)
+{
+    const int total = wl_total(B, WL_KTILE);
+    for (int w = blockIdx.x; w < total; w += gridDim.x) bi_p_tile(B, wl_slot(B, WL_KTILE, w, false));
+}
+
 // y = J x on the tile's fast rows (0 off the DOFs) and dot(w, y); the
 // general rows come from the chunk blocks
 template <int NV, class Acc>
-__device__ __forceinline__ void batch_matvec(const BatchHs& B, const float* xin, float* yout, Acc&& accum,
-                                             double (&acc)[NV])
+__device__ __forceinline__ void batch_matvec(const BatchHs& B, bool chunk, int slot, const float* xin, float* yout,
+                                             Acc&& accum, double (&acc)[NV])
 {
     __shared__ double zs[kSH * kSW];
     __shared__ float xs[kSH * kSW];
-    if (blockIdx.x < (unsigned)B.NT) {
-        const BTileAt T = btile(B, blockIdx.x);
+    if (!chunk) {
+        const BTileAt T = btile(B, slot);
         const HsCtl& c = B.ctl[T.gi];
         if (!krylov_active(c)) return;
         const BatchHsGrid& g = B.grids[T.gi];
@@ -1039,7 +1085,6 @@ __device__ __forceinline__ void batch_matvec(const BatchHs& B, const float* xin,
             }
         }
     } else {
-        const int slot = blockIdx.x - B.NT;
         const int gi = B.gc_grid[slot];
         const HsCtl& c = B.ctl[gi];
         if (!krylov_active(c)) return;
@@ -1054,26 +1099,43 @@ __device__ __forceinline__ void batch_matvec(const BatchHs& B, const float* xin,
         }
     }
     const bool mx[NV] = {};
-    block_partial<NV>(acc, mx, B.partials, blockIdx.x);
+    block_partial<NV>(acc, mx, B.partials, chunk ? B.NT + slot : slot);
+}
+
+// the items of a product: the Krylov tiles, then the Krylov chunks
+template <class F>
+__device__ __forceinline__ void matvec_items(const BatchHs& B, F&& fn)
+{
+    const int nt = wl_total(B, WL_KTILE), total = nt + wl_total(B, WL_KCHUNK);
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        __syncthreads();
+        if (w < nt) fn(false, wl_slot(B, WL_KTILE, w, false));
+        else fn(true, wl_slot(B, WL_KCHUNK, w - nt, true));
+    }
 }
 
 // v = J phat, partial rhat . v
 __global__ void __launch_bounds__(256) kb_bi_v(BatchHs B)
 {
-    double acc[1] = {0.0};
     const float* rhat = B.b;
-    batch_matvec<1>(B, B.phat, B.v, [&](long long f, float y) { acc[0] += static_cast<double>(rhat[f]) * y; }, acc);
+    matvec_items(B, [&](bool chunk, int slot) {
+        double acc[1] = {0.0};
+        batch_matvec<1>(B, chunk, slot, B.phat, B.v,
+                        [&](long long f, float y) { acc[0] += static_cast<double>(rhat[f]) * y; }, acc);
+    });
 }
 
 // t = J shat, partials t . s and t . t
 __global__ void __launch_bounds__(256) kb_bi_t(BatchHs B)
 {
-    double acc[2] = {0.0, 0.0};
     const float* sv = B.s;
-    batch_matvec<2>(B, B.shat, B.t, [&](long long f, float y) {
-        acc[0] += static_cast<double>(y) * sv[f];
-        acc[1] += static_cast<double>(y) * y;
-    }, acc);
+    matvec_items(B, [&](bool chunk, int slot) {
+        double acc[2] = {0.0, 0.0};
+        batch_matvec<2>(B, chunk, slot, B.shat, B.t, [&](long long f, float y) {
+            acc[0] += static_cast<double>(y) * sv[f];
+            acc[1] += static_cast<double>(y) * y;
+        }, acc);
+    });
 }
 
 // alpha = rho / (rhat . v) per grid (breakdown stops that grid's solve)
@@ -1095,9 +1157,9 @@ __global__ void __launch_bounds__(256) kb_bi_vfin(BatchHs B)
 }
 
 // s = r - alpha v; shat = M^-1 s
-__global__ void __launch_bounds__(256) kb_bi_s(BatchHs B)
+__device__ __forceinline__ void bi_s_tile(const BatchHs& B, int slot)
 {
-    const BTileAt T = btile(B, blockIdx.x);
+    const BTileAt T = btile(B, slot);
     const HsCtl& c = B.ctl[T.gi];
     if (!krylov_active(c)) return;
     const BatchHsGrid& g = B.grids[T.gi];
@@ -1114,6 +1176,12 @@ __global__ void __launch_bounds__(256) kb_bi_s(BatchHs B)
     });
 }
 
+__global__ void __launch_bounds__(256) kb_bi_s(BatchHs B)
+{
+    const int total = wl_total(B, WL_KTILE);
+    for (int w = blockIdx.x; w < total; w += gridDim.x) bi_s_tile(B, wl_slot(B, WL_KTILE, w, false));
+}
+
 // omega = (t . s) / (t . t) per grid
 __global__ void __launch_bounds__(256) kb_bi_tfin(BatchHs B)
 {
@@ -1126,9 +1194,9 @@ __global__ void __launch_bounds__(256) kb_bi_tfin(BatchHs B)
 }
 
 // x += alpha phat + omega shat; r = s - omega t; partials rhat . r, r . r
-__global__ void __launch_bounds__(256) kb_bi_x(BatchHs B)
+__device__ __forceinline__ void bi_x_tile(const BatchHs& B, int slot)
 {
-    const BTileAt T = btile(B, blockIdx.x);
+    const BTileAt T = btile(B, slot);
     const HsCtl& c = B.ctl[T.gi];
     if (!krylov_active(c)) return;
     const BatchHsGrid& g = B.grids[T.gi];
@@ -1151,7 +1219,16 @@ __global__ void __launch_bounds__(256) kb_bi_x(BatchHs B)
         acc[1] += static_cast<double>(rv) * rv;
     });
     const bool mx[2] = {false, false};
-    block_partial<2>(acc, mx, B.partials, blockIdx.x);
+    block_partial<2>(acc, mx, B.partials, slot);
+}
+
+__global__ void __launch_bounds__(256) kb_bi_x(BatchHs B)
+{
+    const int total = wl_total(B, WL_KTILE);
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        __syncthreads();
+        bi_x_tile(B, wl_slot(B, WL_KTILE, w, false));
+    }
 }
 
 // rho, ||r||^2 and the convergence / breakdown flags per grid
@@ -1224,10 +1301,8 @@ __device__ __forceinline__ void end_solve(HsCtl& c, int its, bool converged)
 // accepted or halves lambda (compute_hillslope_newton's line search,
 // lsdtt_hillslope_newton_cpu.cpp:553-633, with restoration mode while the
 // iterate's slope ratio is above sqrt(cap)).
-__global__ void kb_control(BatchHs B, int n_grids, int stage)
+__device__ __forceinline__ void control_one(const BatchHs& B, int gi, int stage)
 {
-    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= n_grids) return;
     HsCtl& c = B.ctl[gi];
     const BatchHsGrid& g = B.grids[gi];
     const double feasible = sqrt(g.cap);
@@ -1235,6 +1310,10 @@ __global__ void kb_control(BatchHs B, int n_grids, int stage)
         if (c.phase == HS_START || c.phase == HS_KRYLOV || c.phase == HS_SEARCH) ++c.rounds;
         if (c.phase == HS_KRYLOV && c.kdone) {
             c.krylov_total += c.kits;
+            // no iteration ran (b = 0): the correction is zero.  (Each trial
+            // resets kits for the system at its own point, so the line
+            // search keeps this instead.)
+            c.zero_step = c.kits == 0 ? 1 : 0;
             c.restoring = (g.nonlinear && c.ratio > feasible) ? 1 : 0;
             if (c.restoring) ++c.restoration_its;
             c.lambda = 1.0;
@@ -1280,22 +1359,85 @@ __global__ void kb_control(BatchHs B, int n_grids, int stage)
     else c.phase = HS_KRYLOV;
 }
 
-// a new step: every grid with a hillslope solve starts its first sub-step
-__global__ void kb_hs_start(BatchHs B, int n_grids)
+
+// the instances of one phase as work list `list` (their tiles, or their
+// general-DOF chunks): one block, block-wide scans over the instances
+template <class Pred>
+__device__ void build_list(const BatchHs& B, int list, bool chunks, Pred&& on_list)
 {
-    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= n_grids) return;
-    HsCtl& c = B.ctl[gi];
-    const BatchHsGrid& g = B.grids[gi];
-    c = HsCtl{};
-    c.zc = c.zn = 0;
-    c.zt = 1;
-    c.dt = c.remaining = c.dt_max = g.dt_total;
-    c.dt_min = g.dt_total * 1e-9;
-    c.step = fmin(c.dt, c.remaining);
-    c.inv_dt = 1.0 / c.step;
-    c.first_solve = 1;
-    c.phase = g.enabled ? HS_START : HS_IDLE;
+    __shared__ int s_on[256], s_it[256];
+    int* inst = B.wl_inst + (size_t)list * B.G;
+    int* pre = B.wl_pre + (size_t)list * (B.G + 1);
+    int run_n = 0, run_items = 0;
+    for (int base = 0; base < B.G; base += blockDim.x) {
+        const int gi = base + threadIdx.x;
+        int items = 0;
+        if (gi < B.G && on_list(B.ctl[gi])) items = chunks ? B.grids[gi].ngc : B.grids[gi].nst;
+        const int on = items > 0 ? 1 : 0;
+        s_on[threadIdx.x] = on;
+        s_it[threadIdx.x] = items;
+        __syncthreads();
+        for (int o = 1; o < (int)blockDim.x; o <<= 1) {  // inclusive scans
+            const int a = threadIdx.x >= (unsigned)o ? s_on[threadIdx.x - o] : 0;
+            const int b = threadIdx.x >= (unsigned)o ? s_it[threadIdx.x - o] : 0;
+            __syncthreads();
+            s_on[threadIdx.x] += a;
+            s_it[threadIdx.x] += b;
+            __syncthreads();
+        }
+        if (on) {
+            const int k = run_n + s_on[threadIdx.x] - 1;
+            inst[k] = gi;
+            pre[k] = run_items + s_it[threadIdx.x] - items;
+        }
+        run_n += s_on[blockDim.x - 1];
+        run_items += s_it[blockDim.x - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pre[run_n] = run_items;
+        B.wl_hdr[2 * list] = run_n;
+        B.wl_hdr[2 * list + 1] = run_items;
+    }
+}
+
+__device__ __forceinline__ bool trial_listed(const HsCtl& c) { return trial_phase(c.phase); }
+__device__ __forceinline__ bool krylov_listed(const HsCtl& c) { return krylov_active(c); }
+
+// One block: every instance's decisions, then the work lists of what
+// follows (stage 0: the trials of this round; stage 1: the Krylov
+// iteration of the next round).
+__global__ void __launch_bounds__(256) kb_control(BatchHs B, int stage)
+{
+    for (int gi = threadIdx.x; gi < B.G; gi += blockDim.x) control_one(B, gi, stage);
+    __syncthreads();
+    if (stage == 0) {
+        build_list(B, WL_TTILE, false, trial_listed);
+        build_list(B, WL_TCHUNK, true, trial_listed);
+    } else {
+        build_list(B, WL_KTILE, false, krylov_listed);
+        build_list(B, WL_KCHUNK, true, krylov_listed);
+    }
+}
+
+// a new step: every grid with a hillslope solve starts its first sub-step;
+// no Krylov work is listed yet
+__global__ void __launch_bounds__(256) kb_hs_start(BatchHs B)
+{
+    for (int gi = threadIdx.x; gi < B.G; gi += blockDim.x) {
+        HsCtl& c = B.ctl[gi];
+        const BatchHsGrid& g = B.grids[gi];
+        c = HsCtl{};
+        c.zc = c.zn = 0;
+        c.zt = 1;
+        c.dt = c.remaining = c.dt_max = g.dt_total;
+        c.dt_min = g.dt_total * 1e-9;
+        c.step = fmin(c.dt, c.remaining);
+        c.inv_dt = 1.0 / c.step;
+        c.first_solve = 1;
+        c.phase = g.enabled ? HS_START : HS_IDLE;
+    }
+    if (threadIdx.x < 2 * WL_N) B.wl_hdr[threadIdx.x] = 0;
 }
 
 }  // namespace
@@ -1376,26 +1518,29 @@ void hs_trial(const HsDom& d, const double* z, const float* dz, double lam, long
 
 void hsb_start(const BatchHs& B, int n_grids, hipStream_t s)
 {
-    kb_hs_start<<<div_up(n_grids, 64), 64, 0, s>>>(B, n_grids);
+    (void)n_grids;
+    kb_hs_start<<<1, kB, 0, s>>>(B);
     LSDTT_LAUNCH_CHECK();
 }
 
 void hsb_round(const BatchHs& B, int n_grids, hipStream_t s)
 {
-    const int NT = B.NT, NB = B.NT + B.NGC;
-    kb_bi_p<<<NT, kB, 0, s>>>(B);
-    kb_bi_v<<<NB, kB, 0, s>>>(B);
+    // fixed grids; the blocks walk the listed tiles and chunks
+    const int gt = std::max(1, std::min(B.NT, 2048)), gc = std::max(1, std::min(B.NGC, 256));
+    const int gm = std::max(1, std::min(B.NT + B.NGC, 2048));
+    kb_bi_p<<<gt, kB, 0, s>>>(B);
+    kb_bi_v<<<gm, kB, 0, s>>>(B);
     kb_bi_vfin<<<n_grids, kB, 0, s>>>(B);
-    kb_bi_s<<<NT, kB, 0, s>>>(B);
-    kb_bi_t<<<NB, kB, 0, s>>>(B);
+    kb_bi_s<<<gt, kB, 0, s>>>(B);
+    kb_bi_t<<<gm, kB, 0, s>>>(B);
     kb_bi_tfin<<<n_grids, kB, 0, s>>>(B);
-    kb_bi_x<<<NT, kB, 0, s>>>(B);
+    kb_bi_x<<<gt, kB, 0, s>>>(B);
     kb_bi_xfin<<<n_grids, kB, 0, s>>>(B);
-    kb_control<<<div_up(n_grids, 64), 64, 0, s>>>(B, n_grids, 0);
-    kb_trial<<<NT, kB, 0, s>>>(B);
-    if (B.NGC > 0) kb_trial_gen<<<B.NGC, kB, 0, s>>>(B);
+    kb_control<<<1, kB, 0, s>>>(B, 0);
+    kb_trial<<<gt, kB, 0, s>>>(B);
+    if (B.NGC > 0) kb_trial_gen<<<gc, kB, 0, s>>>(B);
     kb_trial_fin<<<n_grids, kB, 0, s>>>(B);
-    kb_control<<<div_up(n_grids, 64), 64, 0, s>>>(B, n_grids, 1);
+    kb_control<<<1, kB, 0, s>>>(B, 1);
     LSDTT_LAUNCH_CHECK();
 }
 

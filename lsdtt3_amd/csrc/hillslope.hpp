@@ -101,6 +101,7 @@ struct HsCtl {
     int rounds;               // rounds in which the grid was still solving
     int stats_done;           // the step's statistics have been taken
     int method;               // 0 Newton, 1 damped Picard (after a failure)
+    int zero_step;            // the line search's Newton step is zero (no Krylov iteration ran)
     double inv_dt, step, dt, remaining, dt_min, dt_max;
     double fnorm, ratio, tol, lambda, f_init;
     double trial_f, trial_ratio;
@@ -126,7 +127,15 @@ struct BatchHsGrid {
     double abs_tol, rel_tol, dt_total;
 };
 
+// work lists of the batched kernels (hillslope.hip): Krylov tiles and
+// chunks, trial tiles and chunks
+enum { WL_KTILE = 0, WL_KCHUNK = 1, WL_TTILE = 2, WL_TCHUNK = 3, WL_N = 4 };
+
 struct BatchHs {
+    int G;  // grids
+    int* wl_inst;  // WL_N x G: the listed grids
+    int* wl_pre;   // WL_N x (G + 1): their first item
+    int* wl_hdr;   // WL_N x 2: grids listed, items
     const BatchHsGrid* grids;
     const int* st_grid;  // stencil-tile slot -> grid
     const int* gc_grid;  // general-DOF chunk slot -> grid

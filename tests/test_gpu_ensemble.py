@@ -214,3 +214,28 @@ def test_gpu_ensemble_rejects_unsupported_options(gpu):
     ms[0]["fluvial_method"] = "dinf"
     with pytest.raises(GpuError):
         gpu.Ensemble(ms)
+
+
+def test_gpu_ensemble_damped_line_search_matches_single_engine(gpu):
+    """an instance whose Newton iterations need line-search steps below
+    lambda = 1 after a few steps (a Roering surface steepened by incision):
+    the batched solve takes the same sub-steps as the single-grid engine
+    (no rejections) and agrees with it within 4 dt tol every step"""
+    print("[gpu ensemble] damped line search, 10 steps", flush=True)
+    from lsdtt3_amd.synthetic import lem_surface
+    m = dict(dem=lem_surface(434, 345, dx=30.0, seed=2024006, max_slope=0.6), cell_size=30.0,
+             timestep_years=500.0, erodibility=7.988049563613928e-05, m_exponent=0.5687867749125224,
+             n_exponent=1.0, uplift_rate=0.0009256564972093344, hillslope_diffusivity=0.007291700011174901,
+             nonlinear_hillslopes=True, critical_hillslope_slope=0.8, boundary_conditions="nnon")
+    ens = gpu.Ensemble([m])
+    kw = dict(m)
+    z = kw.pop("dem")
+    lem = gpu.LandscapeModel(z, kw.pop("cell_size"), **kw)
+    for step in range(10):
+        sb, _ = ens.run(1)
+        ss = lem.step()
+        assert sb[0]["hillslope_rejected"] == ss["hillslope_rejected"] == 0, (step, sb[0], ss)
+        assert sb[0]["hillslope_substeps"] == ss["hillslope_substeps"] == 1, step
+        tol = max(4 * 500.0 * ss["hillslope_tolerance"], 1e-8)
+        assert np.abs(ens.elevation(0) - lem.elevation).max() <= 2 * tol, step
+        lem.elevation = ens.elevation(0)  # keep both on one trajectory
