@@ -23,7 +23,8 @@ namespace lsdtt_gpu {
 namespace {
 
 constexpr int kB = 256;
-constexpr int kTW = 64, kTH = 16;  // stencil tiles (the hillslope kernels' shape)
+// work items: the batched hillslope kernels' tiles
+constexpr int kTW = kBatchTileCols, kTH = kBatchTileRows;
 constexpr int kBins = 2048;
 
 __global__ void kb_fill_i32(int* __restrict__ a, long long n, int v)
@@ -96,20 +97,81 @@ __device__ __forceinline__ void d8_cell(const double* __restrict__ z, const uint
     dir[f] = static_cast<uint8_t>(bdir);
 }
 
+// one stencil tile per block; a thread walks 4 consecutive rows of one
+// column with the 6 x 3 window of z and mask around them loaded first
+// (4.5 loads per cell); cells on the grid's outer ring resolve their
+// steps by the boundary rules in d8_cell
+constexpr int kRows = 4;
 __global__ void __launch_bounds__(256) kb_receivers(const BatchHsGrid* __restrict__ grids,
                                                     const int* __restrict__ st_grid, const double* __restrict__ z,
                                                     const uint8_t* __restrict__ mask, int* __restrict__ rcv,
                                                     uint8_t* __restrict__ dir, double* __restrict__ zcopy)
 {
-    const StTile T = st_tile(grids, st_grid, blockIdx.x);
+    // four blocks per item, one 16-row band each
+    const StTile T = st_tile(grids, st_grid, blockIdx.x / 4);
     const BatchHsGrid& g = grids[T.gi];
+    const int nr = g.nr, nc = g.nc;
     const int col = T.c0 + (threadIdx.x & 63);
-    if (col >= g.nc) return;
+    if (col >= nc) return;
+    {
+    const int r0 = T.r0 + 16 * (blockIdx.x % 4) + kRows * (threadIdx.x >> 6);
+    if (r0 >= nr) return;
     const long long o = g.off;
-    for (int k = 0; k < kTH / 4; ++k) {
-        const int row = T.r0 + (threadIdx.x >> 6) + 4 * k;
-        if (row >= g.nr) break;
-        d8_cell(z + o, mask + o, g.nr, g.nc, g.bc, g.h, g.h * kSqrt2, row, col, rcv + o, dir + o, zcopy + o);
+    const double* zg = z + o;
+    const uint8_t* mg = mask + o;
+    const double card = g.h, diag = g.h * kSqrt2;
+    const int cm = col > 0 ? col - 1 : col, cp = col < nc - 1 ? col + 1 : col;
+    double zw[kRows + 2][3];
+    uint8_t mw[kRows + 2][3];
+#pragma unroll
+    for (int k = 0; k < kRows + 2; ++k) {
+        const int rr = min(max(r0 - 1 + k, 0), nr - 1);
+        const long long b = (long long)rr * nc;
+        zw[k][0] = zg[b + cm];
+        zw[k][1] = zg[b + col];
+        zw[k][2] = zg[b + cp];
+        mw[k][0] = mg[b + cm];
+        mw[k][1] = mg[b + col];
+        mw[k][2] = mg[b + cp];
+    }
+    const bool edge_col = col == 0 || col == nc - 1;
+#pragma unroll
+    for (int k = 1; k <= kRows; ++k) {
+        const int row = r0 - 1 + k;
+        if (row >= nr) break;
+        if (edge_col || row == 0 || row == nr - 1) {
+            d8_cell(zg, mg, nr, nc, g.bc, card, diag, row, col, rcv + o, dir + o, zcopy + o);
+            continue;
+        }
+        const int f = row * nc + col;
+        const double zc = zw[k][1];
+        zcopy[o + f] = zc;
+        if (!mw[k][1]) {
+            rcv[o + f] = kNodataReceiver;
+            dir[o + f] = kNoDirection;
+            continue;
+        }
+        // neighbours in D8 order N, NE, E, SE, S, SW, W, NW
+        const double zn[8] = {zw[k - 1][1], zw[k - 1][2], zw[k][2], zw[k + 1][2],
+                              zw[k + 1][1], zw[k + 1][0], zw[k][0], zw[k - 1][0]};
+        const uint8_t mk[8] = {mw[k - 1][1], mw[k - 1][2], mw[k][2], mw[k + 1][2],
+                               mw[k + 1][1], mw[k + 1][0], mw[k][0], mw[k - 1][0]};
+        double best = 0.0;
+        int bdir = kNoDirection;
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            if (!mk[d]) continue;
+            const double dz = zc - zn[d];
+            if (dz <= 0.0) continue;
+            const double sl = dz / ((d & 1) ? diag : card);
+            if (sl > best) {
+                best = sl;
+                bdir = d;
+            }
+        }
+        rcv[o + f] = bdir != kNoDirection ? f + d8_row[bdir] * nc + d8_col[bdir] : kOutletReceiver;
+        dir[o + f] = static_cast<uint8_t>(bdir);
+    }
     }
 }
 
@@ -164,7 +226,7 @@ __global__ void __launch_bounds__(256) kb_stats(const BatchHsGrid* __restrict__ 
     const int col = T.c0 + (threadIdx.x & 63);
     for (int k = 0; k < kTH / 4; ++k) {
         const int row = T.r0 + (threadIdx.x >> 6) + 4 * k;
-        if (row >= g.nr || col >= g.nc) continue;
+        if (row >= g.nr || col >= g.nc) break;
         const long long f = (long long)row * g.nc + col;
         const uint8_t m = cflag[o + f];
         const double bb = zbefore[o + f];
@@ -754,7 +816,7 @@ void BatchEngine::enqueue_stats(hipStream_t s)
 // the whole step with fixed caps: only kernel launches (capturable)
 void BatchEngine::enqueue_step(const Plan& pl, hipStream_t s)
 {
-    kb_receivers<<<NT, kB, 0, s>>>(hsgrids_d.p, st_grid_d.p, z.p, mask.p, rcv.p, dir.p, z_before.p);
+    kb_receivers<<<4 * NT, kB, 0, s>>>(hsgrids_d.p, st_grid_d.p, z.p, mask.p, rcv.p, dir.p, z_before.p);
     if (n_bflats) kb_gather<<<blocks_for(n_bflats), kB, 0, s>>>(z.p, bflat.p, n_bflats, bel.p);
     LSDTT_LAUNCH_CHECK();
     batch_tile_order(th, btiles(), RT, rcv.p, mask.p, ord.p, lstart.p, maxdep.p, xdon.p, s);
@@ -826,7 +888,7 @@ int BatchEngine::step(std::vector<LemStats>& st)
         ++counters.refills;
         kb_copy<<<std::min(blocks_for(N), 8192), kB, 0, s>>>(z_before.p, z.p, N);
         LSDTT_LAUNCH_CHECK();
-        kb_receivers<<<NT, kB, 0, s>>>(hsgrids_d.p, st_grid_d.p, z.p, mask.p, rcv.p, dir.p, z_before.p);
+        kb_receivers<<<4 * NT, kB, 0, s>>>(hsgrids_d.p, st_grid_d.p, z.p, mask.p, rcv.p, dir.p, z_before.p);
         batch_tile_order(th, btiles(), RT, rcv.p, mask.p, ord.p, lstart.p, maxdep.p, xdon.p, s);
         int launched = 0;
         for (int batch = std::max(4, plan.count_passes);; batch *= 2) {

@@ -801,6 +801,12 @@ __device__ __forceinline__ HsDom batch_dom(const BatchHs& B, const BatchHsGrid& 
 __device__ __forceinline__ bool trial_phase(int ph) { return ph == HS_START || ph == HS_SEARCH; }
 
 // this block's stencil tile: grid and corner
+// a batched work item is a 64 x 64 tile, swept as kBands bands of the
+// stencil kernels' 64 x 16 LDS tiles: the item's lookups, its reduction
+// and its partial are shared by 4096 cells
+constexpr int kBTH = kBatchTileRows, kBands = kBTH / kTH;
+static_assert(kBatchTileCols == kTW && kBTH % kTH == 0, "batched items are whole columns of LDS tiles");
+
 struct BTileAt {
     int gi, r0, c0;
 };
@@ -809,7 +815,7 @@ __device__ __forceinline__ BTileAt btile(const BatchHs& B, int slot)
     const int gi = B.st_grid[slot];
     const BatchHsGrid& g = B.grids[gi];
     const int t = slot - g.stbase;
-    return BTileAt{gi, (t / g.tx_n) * kTH, (t % g.tx_n) * kTW};
+    return BTileAt{gi, (t / g.tx_n) * kBTH, (t % g.tx_n) * kTW};
 }
 
 // Work lists: the instances a kernel covers this round (those in its
@@ -852,50 +858,55 @@ __device__ __forceinline__ void trial_tile(const BatchHs& B, int slot)
     float* b = B.b + g.off;
     float* dinv = B.dinv + g.off;
     __shared__ double zs[kSH * kSW];
-    const int nr = d.nr, nc = d.nc, r0 = T.r0, c0 = T.c0;
+    const int nr = d.nr, nc = d.nc, c0 = T.c0;
     const int cl = threadIdx.x & 63;
     double acc[3] = {0.0, 0.0, 0.0};
-    if (has_dz) stage_tile<true, false>(z, dz, lam, nullptr, nr, nc, r0, c0, zs, nullptr);
-    else stage_tile<false, false>(z, nullptr, 0.0, nullptr, nr, nc, r0, c0, zs, nullptr);
-    long long f[kTH / 4];
-    bool in[kTH / 4];
-    uint8_t fl[kTH / 4], dl[kTH / 4];
-    double znl[kTH / 4];
+    for (int band = 0; band < kBands; ++band) {
+        const int r0 = T.r0 + band * kTH;
+        if (r0 >= nr) break;
+        __syncthreads();
+        if (has_dz) stage_tile<true, false>(z, dz, lam, nullptr, nr, nc, r0, c0, zs, nullptr);
+        else stage_tile<false, false>(z, nullptr, 0.0, nullptr, nr, nc, r0, c0, zs, nullptr);
+        long long f[kTH / 4];
+        bool in[kTH / 4];
+        uint8_t fl[kTH / 4], dl[kTH / 4];
+        double znl[kTH / 4];
 #pragma unroll
-    for (int k = 0; k < kTH / 4; ++k) {
-        const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
-        in[k] = rr < nr && cc < nc;
-        f[k] = (long long)min(rr, nr - 1) * nc + min(cc, nc - 1);
-        fl[k] = d.fast[f[k]];
-        dl[k] = d.dof[f[k]];
-        znl[k] = d.zn[f[k]];
-    }
-    __syncthreads();
+        for (int k = 0; k < kTH / 4; ++k) {
+            const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
+            in[k] = rr < nr && cc < nc;
+            f[k] = (long long)min(rr, nr - 1) * nc + min(cc, nc - 1);
+            fl[k] = d.fast[f[k]];
+            dl[k] = d.dof[f[k]];
+            znl[k] = d.zn[f[k]];
+        }
+        __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kTH / 4; ++k) {
-        if (!in[k]) continue;
-        if (!fl[k]) {
-            if (!dl[k]) {
-                b[f[k]] = 0.0f;
-                dinv[f[k]] = 0.0f;
-                if (lam != 0.0) z_out[f[k]] = z[f[k]];
+        for (int k = 0; k < kTH / 4; ++k) {
+            if (!in[k]) continue;
+            if (!fl[k]) {
+                if (!dl[k]) {
+                    b[f[k]] = 0.0f;
+                    dinv[f[k]] = 0.0f;
+                    if (lam != 0.0) z_out[f[k]] = z[f[k]];
+                }
+                continue;
             }
-            continue;
+            double zz[9];
+            tile_z9(zs, (threadIdx.x >> 6) + 4 * k, cl, zz);
+            double diag;
+            const double F = eval_fast(d, zz, znl[k], &diag);
+            acc[0] = fmax(acc[0], fabs(F));
+            if (d.nonlinear) {
+                const double sx = (zz[5] - zz[3]) * d.inv_2h, sy = (zz[7] - zz[1]) * d.inv_2h;
+                acc[1] = fmax(acc[1], sx * sx + sy * sy);
+            }
+            const float bf = static_cast<float>(-F);
+            b[f[k]] = bf;
+            dinv[f[k]] = diag != 0.0 ? __builtin_amdgcn_rcpf(static_cast<float>(diag)) : 0.0f;
+            acc[2] += static_cast<double>(bf) * bf;
+            if (lam != 0.0) z_out[f[k]] = zz[4];
         }
-        double zz[9];
-        tile_z9(zs, (threadIdx.x >> 6) + 4 * k, cl, zz);
-        double diag;
-        const double F = eval_fast(d, zz, znl[k], &diag);
-        acc[0] = fmax(acc[0], fabs(F));
-        if (d.nonlinear) {
-            const double sx = (zz[5] - zz[3]) * d.inv_2h, sy = (zz[7] - zz[1]) * d.inv_2h;
-            acc[1] = fmax(acc[1], sx * sx + sy * sy);
-        }
-        const float bf = static_cast<float>(-F);
-        b[f[k]] = bf;
-        dinv[f[k]] = diag != 0.0 ? __builtin_amdgcn_rcpf(static_cast<float>(diag)) : 0.0f;
-        acc[2] += static_cast<double>(bf) * bf;
-        if (lam != 0.0) z_out[f[k]] = zz[4];
     }
     const bool mx[3] = {true, true, false};
     block_partial<3>(acc, mx, B.partials, slot);
@@ -1006,10 +1017,11 @@ template <class F>
 __device__ __forceinline__ void tile_cells(const BatchHsGrid& g, int r0, int c0, F&& fn)
 {
     const int cc = c0 + (threadIdx.x & 63);
-#pragma unroll
-    for (int k = 0; k < kTH / 4; ++k) {
+    if (cc >= g.nc) return;
+#pragma unroll 4
+    for (int k = 0; k < kBTH / 4; ++k) {
         const int rr = r0 + (threadIdx.x >> 6) + 4 * k;
-        if (rr < g.nr && cc < g.nc) fn((long long)rr * g.nc + cc);
+        if (rr < g.nr) fn((long long)rr * g.nc + cc);
     }
 }
 
@@ -1044,9 +1056,23 @@ __global__ void __launch_bounds__(256) kb_bi_p(BatchHs B)
 
 // y = J x on the tile's fast rows (0 off the DOFs) and dot(w, y); the
 // general rows come from the chunk blocks
-template <int NV, class Acc>
+// y = J x on the item (its fast rows from the tile, or its chunk of general
+// rows); partials w . y (NV = 1) or y . w and y . y (NV = 2).  w is loaded
+// with the other per-cell inputs, before the barrier.
+template <int NV>
+__device__ __forceinline__ void dot_acc(double (&acc)[NV], float y, float w)
+{
+    if constexpr (NV == 1) {
+        acc[0] += static_cast<double>(w) * y;
+    } else {
+        acc[0] += static_cast<double>(y) * w;
+        acc[1] += static_cast<double>(y) * y;
+    }
+}
+
+template <int NV>
 __device__ __forceinline__ void batch_matvec(const BatchHs& B, bool chunk, int slot, const float* xin, float* yout,
-                                             Acc&& accum, double (&acc)[NV])
+                                             const float* win, double (&acc)[NV])
 {
     __shared__ double zs[kSH * kSW];
     __shared__ float xs[kSH * kSW];
@@ -1058,30 +1084,38 @@ __device__ __forceinline__ void batch_matvec(const BatchHs& B, bool chunk, int s
         const HsDom d = batch_dom(B, g, c.inv_dt, c.zn);
         const double* z = B.zbuf[c.zc] + g.off;
         const float* x = xin + g.off;
+        const float* wv = win + g.off;
         float* y = yout + g.off;
-        const int nr = g.nr, nc = g.nc, r0 = T.r0, c0 = T.c0, cl = threadIdx.x & 63;
-        stage_tile<false, true>(z, nullptr, 0.0, x, nr, nc, r0, c0, zs, xs);
-        long long f[kTH / 4];
-        bool in[kTH / 4];
-        uint8_t fl[kTH / 4], dl[kTH / 4];
+        const int nr = g.nr, nc = g.nc, c0 = T.c0, cl = threadIdx.x & 63;
+        for (int band = 0; band < kBands; ++band) {
+            const int r0 = T.r0 + band * kTH;
+            if (r0 >= nr) break;
+            __syncthreads();
+            stage_tile<false, true>(z, nullptr, 0.0, x, nr, nc, r0, c0, zs, xs);
+            long long f[kTH / 4];
+            bool in[kTH / 4];
+            uint8_t fl[kTH / 4], dl[kTH / 4];
+            float wl[kTH / 4];
 #pragma unroll
-        for (int k = 0; k < kTH / 4; ++k) {
-            const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
-            in[k] = rr < nr && cc < nc;
-            f[k] = (long long)min(rr, nr - 1) * nc + min(cc, nc - 1);
-            fl[k] = d.fast[f[k]];
-            dl[k] = d.dof[f[k]];
-        }
-        __syncthreads();
+            for (int k = 0; k < kTH / 4; ++k) {
+                const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
+                in[k] = rr < nr && cc < nc;
+                f[k] = (long long)min(rr, nr - 1) * nc + min(cc, nc - 1);
+                fl[k] = d.fast[f[k]];
+                dl[k] = d.dof[f[k]];
+                wl[k] = wv[f[k]];
+            }
+            __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kTH / 4; ++k) {
-            if (!in[k]) continue;
-            if (fl[k]) {
-                const float yv = jx_tile(d, zs, xs, (threadIdx.x >> 6) + 4 * k, cl);
-                y[f[k]] = yv;
-                accum(g.off + f[k], yv);
-            } else if (!dl[k]) {
-                y[f[k]] = 0.0f;
+            for (int k = 0; k < kTH / 4; ++k) {
+                if (!in[k]) continue;
+                if (fl[k]) {
+                    const float yv = jx_tile(d, zs, xs, (threadIdx.x >> 6) + 4 * k, cl);
+                    y[f[k]] = yv;
+                    dot_acc<NV>(acc, yv, wl[k]);
+                } else if (!dl[k]) {
+                    y[f[k]] = 0.0f;
+                }
             }
         }
     } else {
@@ -1093,9 +1127,10 @@ __device__ __forceinline__ void batch_matvec(const BatchHs& B, bool chunk, int s
         const int i = (slot - g.gcbase) * 256 + threadIdx.x;
         if (i < d.n_gen) {
             const int f = d.gen[i];
+            const float w = win[g.off + f];
             const float yv = jx_gen(d, xin + g.off, i, f);
             yout[g.off + f] = yv;
-            accum(g.off + f, yv);
+            dot_acc<NV>(acc, yv, w);
         }
     }
     const bool mx[NV] = {};
@@ -1114,27 +1149,21 @@ __device__ __forceinline__ void matvec_items(const BatchHs& B, F&& fn)
     }
 }
 
-// v = J phat, partial rhat . v
+// v = J phat, partial rhat . v (rhat = b)
 __global__ void __launch_bounds__(256) kb_bi_v(BatchHs B)
 {
-    const float* rhat = B.b;
     matvec_items(B, [&](bool chunk, int slot) {
         double acc[1] = {0.0};
-        batch_matvec<1>(B, chunk, slot, B.phat, B.v,
-                        [&](long long f, float y) { acc[0] += static_cast<double>(rhat[f]) * y; }, acc);
+        batch_matvec<1>(B, chunk, slot, B.phat, B.v, B.b, acc);
     });
 }
 
 // t = J shat, partials t . s and t . t
 __global__ void __launch_bounds__(256) kb_bi_t(BatchHs B)
 {
-    const float* sv = B.s;
     matvec_items(B, [&](bool chunk, int slot) {
         double acc[2] = {0.0, 0.0};
-        batch_matvec<2>(B, chunk, slot, B.shat, B.t, [&](long long f, float y) {
-            acc[0] += static_cast<double>(y) * sv[f];
-            acc[1] += static_cast<double>(y) * y;
-        }, acc);
+        batch_matvec<2>(B, chunk, slot, B.shat, B.t, B.s, acc);
     });
 }
 
@@ -1525,9 +1554,10 @@ void hsb_start(const BatchHs& B, int n_grids, hipStream_t s)
 
 void hsb_round(const BatchHs& B, int n_grids, hipStream_t s)
 {
-    // fixed grids; the blocks walk the listed tiles and chunks
-    const int gt = std::max(1, std::min(B.NT, 2048)), gc = std::max(1, std::min(B.NGC, 256));
-    const int gm = std::max(1, std::min(B.NT + B.NGC, 2048));
+    // one block per tile (chunk) of the batch, enough for every list; the
+    // blocks beyond a list's length exit at once, and the item lookups of
+    // the others overlap across the grid
+    const int gt = std::max(1, B.NT), gc = std::max(1, B.NGC), gm = std::max(1, B.NT + B.NGC);
     kb_bi_p<<<gt, kB, 0, s>>>(B);
     kb_bi_v<<<gm, kB, 0, s>>>(B);
     kb_bi_vfin<<<n_grids, kB, 0, s>>>(B);

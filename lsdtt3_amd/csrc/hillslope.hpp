@@ -108,8 +108,12 @@ struct HsCtl {
     double rho, rho_old, alpha, omega, bnorm2, rnorm2;
 };
 
+// work item of the batched kernels: a tile of kBatchTileRows x
+// kBatchTileCols cells
+constexpr int kBatchTileRows = 64, kBatchTileCols = 64;
+
 // one grid of a batched hillslope solve (cells of the concatenated arrays
-// from `off`; 64 x 16 stencil tiles in slots stbase .. stbase + nst - 1;
+// from `off`; its work-item tiles in slots stbase .. stbase + nst - 1;
 // its general DOFs (grid edge, next to a masked cell) at gen_off in the
 // concatenated lists, handled in chunks of 256 by slots gcbase ..)
 struct BatchHsGrid {
