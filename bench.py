@@ -9,8 +9,8 @@ Newton (FP64 residual, FP32 BiCGSTAB) to a 1e-8 relative residual; stream
 power K log-uniform in [1e-6, 1e-4] (seeded), m = 0.5, n = 1.5; U = 5e-4
 m/yr, dt = 500 yr; boundary "nnon" (south edge fixed).
 
-One step = one run_lem_step on the resident state.  value = valid cells x
-timed steps / wall seconds, inputs resident in HBM before the clock starts,
+One step = one run_lem_step on the resident state, 500 steps.  value =
+valid cells x timed steps / wall seconds, inputs resident in HBM before the clock starts,
 with no instrumentation in the stream.  The per-kernel breakdown and the
 roofline object (the kernel group with the largest share) come from a
 separate run of --profile-steps steps timed with HIP events recorded on
@@ -31,11 +31,20 @@ log-uniform 256^2..2048^2, per-instance K, m/n, D, U) on the batched engine
 captured hipGraph -- with its throughput relative to the plain step.
 The relative residual r reported is ||F||inf over ||F(z^n)||inf of the
 sub-step the Newton stop judged (the stop is r <= 1e-8).
+The timed steps run from the initial surface (warm-up and tile autotuning
+steps are discarded by resetting the elevation), so the default 500 timed
+steps are the workload's whole trajectory, reported per 100-step segment
+as well (Newton and Krylov counts grow as relief builds).  unsmoothed is the plain CPU
+workload's recipe at the same size: the raw surface (steepest centred slope
+about 3, far above Sc, which is why the Roering workload smooths it first)
+with linear diffusion D = 0.01, K = 1e-5, n = 1, U = 1e-3, dt = 1000 yr.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-learned]
        python bench.py --workload ensemble   (64-instance batched ensemble)
        python bench.py --workload cnn        (the U-Net alone: MFMA utilisation)
        python bench.py --workload routing    (fill + D8 flow object + chi on a 4096^2 FP32 DEM)
+       python bench.py --workload plain512   (the plain CPU workload: 512^2 raw surface, linear,
+                                              200 steps, GPU and OpenMP oracle)
 
 Profile summaries (text, for profiles/) from rocprofv3 CSV output:
   python bench.py --summarize-trace DIR OUT [TITLE CMD NOTE]   (--kernel-trace --stats run)
@@ -81,9 +90,90 @@ def lem_kwargs(K):
                 boundary_conditions="nnon", newton_rel_tol=1e-8, **STEP_VARIANT)
 
 
-def cpu_baseline(K, budget_s=15.0, size=512):
-    """oracle FP64 LEM step (single thread) on a crop of the same recipe"""
+def host_threads():
+    """threads the CPU baseline uses: the cores in this process's affinity
+    mask, capped by OMP_NUM_THREADS when that is set"""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+# the plain CPU workload's physics: linear diffusion and n = 1 incision on
+# the raw (unsmoothed) surface
+PLAIN_CPU_KW = dict(timestep_years=1000.0, erodibility=1e-5, m_exponent=0.5, n_exponent=1.0, uplift_rate=1e-3,
+                    hillslope_diffusivity=0.01, boundary_conditions="nnon", newton_rel_tol=1e-8)
+
+
+def unsmoothed_measure(args, sync, steps=50):
+    """the plain CPU workload's recipe scaled to size^2: raw surface (no
+    smoothing: steepest centred slope ~3), linear D = 0.01, K = 1e-5, n = 1,
+    U = 1e-3, dt = 1000 yr"""
+    import lsdtt3_amd
+    from lsdtt3_amd.synthetic import lem_surface, max_centred_slope
+    z = lem_surface(args.size, args.size, dx=30.0, seed=2024)
+    lem = lsdtt3_amd.LandscapeModel(z, 30.0, **PLAIN_CPU_KW)
+    for _ in range(3):
+        lem.step()
+    dt, a, st, _ = run_steps(lem, steps, sync)
+    nv = int(st["n_valid"])
+    return {"value": round(nv * steps / dt, 1), "unit": "cell-steps/s", "steps": steps,
+            "ms_per_step": round(1e3 * dt / steps, 3),
+            "workload": f"raw {args.size}x{args.size} surface (max centred slope "
+                        f"{max_centred_slope(z, 30.0):.2f}), linear D=0.01, K=1e-5 m=0.5 n=1, U=1e-3, "
+                        f"dt=1000, bc=nnon, r<=1e-8",
+            "newton_iters_per_step": round(a["newton"] / steps, 2),
+            "krylov_iters_per_step": round(a["krylov"] / steps, 2),
+            "worst_relative_residual": a["worst_r"]}
+
+
+def bench_plain_cpu_workload(args):
+    """The plain CPU workload as written (512^2 raw surface, linear D = 0.01,
+    K = 1e-5, n = 1, U = 1e-3, dt = 1000 yr, 200 steps) on the GPU, with the
+    oracle's OpenMP FP64 run of the same 200 steps beside it and the largest
+    elevation difference between the two end states."""
+    import lsdtt3_amd
+    from lsdtt3_amd import _native as N
+    from lsdtt3_amd.synthetic import lem_surface
     from oracle import oracle as orc
+    n, steps = 512, 200
+    z0 = lem_surface(n, n, dx=30.0, seed=2024)
+    lem = lsdtt3_amd.LandscapeModel(z0, 30.0, **PLAIN_CPU_KW)
+    lem.step()                      # warm the library; the timed run restarts from z0
+    lem.elevation = z0
+    dt, a, st, _ = run_steps(lem, steps, N.synchronize)
+    zg = lem.elevation
+    out = {"metric": "LEM cell-steps/s, plain CPU workload (512^2, 200 steps)",
+           "value": round(n * n * steps / dt, 1), "unit": "cell-steps/s", "n_gpus": 1, "steps": steps,
+           "warmup": 0, "ms_per_step": round(1e3 * dt / steps, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "raw 512x512 surface dx=30 (0.01 tilt, beta 2, 50 m std, 200 pits), linear "
+                                  "D=0.01, K=1e-5 m=0.5 n=1, U=1e-3, dt=1000, bc=nnon, r<=1e-8",
+                      "newton_iters_per_step": round(a["newton"] / steps, 2),
+                      "krylov_iters_per_step": round(a["krylov"] / steps, 2)}}
+    if not args.no_cpu_baseline:
+        threads = orc.set_threads(host_threads())
+        mask = np.ones(z0.shape, np.uint8)
+        p = orc.lem_params(dt=1000.0, K=1e-5, m=0.5, n=1.0, U=1e-3, D=0.01, bc="nnon", rel_tol=1e-8)
+        bf = orc.boundary_flats(mask, "nnon")
+        zo = z0.copy()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            zo, so, rc = orc.lem_step(zo, mask, 30.0, p, bf)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n * n * steps / el, 1), "unit": "cell-steps/s", "cores": threads,
+                               "kind": "port", "sample": f"the whole workload (200 steps) by the oracle, "
+                                                         f"{threads} OpenMP threads, {el:.1f} s"}
+        out["config"]["max_abs_dz_vs_oracle_after_200_steps"] = float(np.abs(zg - zo).max())
+    print(json.dumps(out))
+
+
+def cpu_baseline(K, budget_s=15.0, size=512):
+    """oracle FP64 LEM step, OpenMP over the host cores, on a crop of the same
+    recipe (results do not depend on the thread count)"""
+    from oracle import oracle as orc
+    threads = orc.set_threads(host_threads())
     z, _, _ = workload(size)
     mask = np.ones(z.shape, np.uint8)
     p = orc.lem_params(500.0, K, 0.5, 1.5, 5e-4, 0.005, Sc=0.8, nonlinear=True, bc="nnon", rel_tol=1e-8)
@@ -95,9 +185,17 @@ def cpu_baseline(K, budget_s=15.0, size=512):
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 200:
             break
-    return {"value": size * size * steps / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP64 LEM step (orc_lem_step, GMRES Newton) on a {size}x{size} crop of the "
-                      f"same recipe, {steps} steps in {el:.1f} s, 1 thread"}
+    cpu = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": size * size * steps / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 LEM step (orc_lem_step: Newton with OpenMP GMRES, serial fill/routing/"
+                      f"stream power as in the reference) on a {size}x{size} crop of the same recipe, "
+                      f"{steps} steps in {el:.1f} s, {threads} OpenMP threads"
+                      + (f" on {cpu}" if cpu else "")}
 
 
 # timer group -> the kernel whose dispatches it times
@@ -241,8 +339,8 @@ def ensemble_member(args):
 
 
 def ensemble_measure(args):
-    """config 4: the 64-instance ensemble on the batched engine, tiles
-    autotuned, graph replay; returns the measurement dict"""
+    """the 64-instance ensemble on the batched engine, tiles autotuned, graph
+    replay, --ensemble-steps timed steps; returns the measurement dict"""
     import lsdtt3_amd
     from concurrent.futures import ProcessPoolExecutor
     t0 = time.perf_counter()
@@ -259,14 +357,14 @@ def ensemble_measure(args):
         tuning = {"tile_height": th, "ms_per_step": {str(h): round(v, 3) for h, v in tms.items()},
                   "steps_per_trial": args.autotune_steps}
         ens.run(1)  # the pattern of the chosen height
-    stats, sec = ens.run(args.steps)
-    value = total * args.steps / sec
+    stats, sec = ens.run(args.ensemble_steps)
+    value = total * args.ensemble_steps / sec
     # relative residual as the Newton stop measures it: ||F|| over ||F(z^n)|| of the
     # sub-step start (tolerance = rel_tol * ||F(z^n)||)
     worst = max(s["hillslope_residual"] / max(s["hillslope_tolerance"] / 1e-8, 1e-300) for s in stats)
     pat = ens.pattern()
-    return {"value": round(value, 1), "unit": "cell-steps/s", "ms_per_step": round(1e3 * sec / args.steps, 3),
-            "steps": args.steps,
+    return {"value": round(value, 1), "unit": "cell-steps/s", "ms_per_step": round(1e3 * sec / args.ensemble_steps, 3),
+            "steps": args.ensemble_steps,
             "workload": f"batched ensemble of {len(ms)} LEM instances, cells log-uniform 256^2..2048^2 "
                         f"(total {total}), per-instance K, m/n, D, U; Roering Newton r<=1e-8, stream power n=1; "
                         f"one launch per kernel for all instances, step replayed from a captured hipGraph",
@@ -280,7 +378,7 @@ def ensemble_measure(args):
 
 def bench_ensemble(args):
     m = ensemble_measure(args)
-    out = {"metric": METRIC, "value": m["value"], "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+    out = {"metric": METRIC, "value": m["value"], "unit": "cell-steps/s", "n_gpus": 1, "steps": args.ensemble_steps,
            "warmup": args.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {k: v for k, v in m.items() if k not in ("value", "unit", "ms_per_step", "steps")}}
@@ -382,6 +480,7 @@ def learned_comparison(args, z, K, sync, plain_value):
         Lm.set_warm_start(lem, mode, net if mode == Lm.WARM_LEARNED else None)
         for _ in range(args.warmup):
             lem.step()
+        lem.elevation = z            # the same trajectory as the plain run
         dt, a, st, _ = run_steps(lem, args.steps, sync)
         _, _, _, prof = run_steps(lem, min(args.steps, args.profile_steps), sync, profile=True)
         newton, krylov, acc = a["newton"], a["krylov"], a["accepted"]
@@ -506,7 +605,42 @@ def bench_routing(args):
            "roofline": {"bound": "hbm", "kernel": "whole pipeline", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": None, "algorithmic_bytes_per_cell": bpc}}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"], ref = routing_cpu_baseline(z, per["fill"])
+        if ref is not None:
+            out["reference_fill"] = ref
     print(json.dumps(out))
+
+
+def routing_cpu_baseline(z, gpu_fill_ms):
+    """The same pipeline on the host: the oracle's fill, flow object and chi
+    (serial, as the reference's fill and accumulation are), once over the
+    full surface; and, where it was built, the reference's own fill
+    (lsdtt::fill_pits, compiled from its sources) on the same surface."""
+    from oracle import oracle as orc
+    n = z.shape[0] * z.shape[1]
+    mask = np.ones(z.shape, np.uint8)
+    t0 = time.perf_counter()
+    zf = orc.fill(z, mask)
+    t1 = time.perf_counter()
+    flow = orc.flow_build(zf, mask, 10.0, "oooo", allow_pits_as_outlets=True)
+    orc.chi_raster(flow, mask, 10.0, 0.45, 1.0)
+    t2 = time.perf_counter()
+    base = {"value": round(n / (t2 - t0), 1), "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"oracle fill + D8 flow object + chi on the same {z.shape[0]}x{z.shape[1]} surface, "
+                      f"once, 1 thread (fill {1e3 * (t1 - t0):.0f} ms, flow+chi {1e3 * (t2 - t1):.0f} ms)"}
+    ref = None
+    if orc.ref_fill_available():
+        t0 = time.perf_counter()
+        zr = orc.ref_fill(z, mask, 10.0, "oooo", n_threads=1)
+        el = time.perf_counter() - t0
+        ref = {"value": round(n / el, 1), "unit": "cells/s", "cores": 1, "kind": "reference",
+               "ms": round(1e3 * el, 1), "gpu_fill_ms": round(gpu_fill_ms, 3),
+               "gpu_speedup": round(1e3 * el / gpu_fill_ms, 1),
+               "bit_exact_vs_oracle": bool(np.array_equal(zr.view(np.uint32), zf.view(np.uint32))),
+               "sample": "lsdtt::fill_pits(dem, 0, \"oooo\", 1) built from the reference's sources, "
+                         "once over the full surface"}
+    return base, ref
 
 
 def main():
@@ -518,12 +652,12 @@ def main():
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["lem4096", "ensemble", "cnn", "routing"], default="lem4096")
+    ap.add_argument("--workload", choices=["lem4096", "ensemble", "cnn", "routing", "plain512"], default="lem4096")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--fluvial", choices=["d8", "dinf"], default="d8",
                     help="lem4096: D8 (the metric's workload) or D-infinity incision")
@@ -534,6 +668,9 @@ def main():
                     help="timed steps per tile height when autotuning the routing tiles (0: default height)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--ensemble-steps", type=int, default=300, help="batched ensemble: timed steps")
+    ap.add_argument("--no-unsmoothed", action="store_true",
+                    help="skip the raw-surface (linear, n = 1) measurement at the same size")
     ap.add_argument("--profile-steps", type=int, default=10,
                     help="steps of the separate HIP-event-profiled run (per-kernel breakdown, roofline)")
     ap.add_argument("--pairs", type=int, default=20000, help="learned: training tile pairs")
@@ -557,6 +694,9 @@ def main():
     if args.workload == "routing":
         bench_routing(args)
         return
+    if args.workload == "plain512":
+        bench_plain_cpu_workload(args)
+        return
     from lsdtt3_amd import _native as N
     sync = N.synchronize
 
@@ -574,8 +714,17 @@ def main():
         args.tile_height = th
         tuned = {"tile_height": th, "ms_per_step": {str(h): round(v, 3) for h, v in tms.items()},
                  "steps_per_trial": args.autotune_steps}
-    dt, a, st, _ = run_steps(lem, args.steps, sync)
-    newton, krylov, worst_r = a["newton"], a["krylov"], a["worst_r"]
+    # the timed run is the workload's trajectory from the initial surface:
+    # warm-up and autotuning steps only warmed the library
+    lem.elevation = z
+    seg = 100 if args.steps % 100 == 0 else args.steps
+    dt, newton, krylov, worst_r, segs = 0.0, 0, 0, 0.0, []
+    for _ in range(args.steps // seg):
+        sdt, a, st, _ = run_steps(lem, seg, sync)
+        dt += sdt
+        newton, krylov, worst_r = newton + a["newton"], krylov + a["krylov"], max(worst_r, a["worst_r"])
+        segs.append({"ms_per_step": round(1e3 * sdt / seg, 3), "newton_iters_per_step": round(a["newton"] / seg, 2),
+                     "krylov_iters_per_step": round(a["krylov"] / seg, 2)})
     psteps = min(args.steps, args.profile_steps)
     _, _, _, prof = run_steps(lem, psteps, sync, profile=True)
 
@@ -603,6 +752,8 @@ def main():
                    "newton_iters_per_step": round(newton / args.steps, 2),
                    "krylov_iters_per_step": round(krylov / args.steps, 2),
                    "worst_relative_residual": worst_r,
+                   "timed_steps_from_initial_surface": args.steps,
+                   "segments": segs if len(segs) > 1 else None,
                    "count_passes_last_step": int(st["count_passes"]),
                    "stream_power_passes_last_step": int(st["stream_power_passes"]),
                    "count_tile_runs_last_step": int(st["count_tile_runs"]),
@@ -612,6 +763,8 @@ def main():
         "kernels_ms_per_step": {k: round(v[0] / psteps, 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
         "profiled_steps": psteps,
     }
+    if not args.no_unsmoothed:
+        out["unsmoothed"] = unsmoothed_measure(args, sync)
     if not args.no_batched:
         del lem
         b = ensemble_measure(args)

@@ -277,7 +277,17 @@ __global__ void __launch_bounds__(256) kb_stats_fin(const BatchHsGrid* __restric
     fold_partials<8>(spartials + (size_t)g.stbase * 8, g.nst, mx, w);
     __shared__ long long es[kB];
     long long e = 0;
-    for (int t = threadIdx.x; t < sg[gi].nrt; t += blockDim.x) e += eroded[sg[gi].rtbase + t];
+    const int* er = eroded + sg[gi].rtbase;
+    const int nrt = sg[gi].nrt;
+    int t = threadIdx.x;
+    for (; t + 7 * kB < nrt; t += 8 * kB) {  // eight loads in flight per thread
+        int v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = er[t + j * kB];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) e += v[j];
+    }
+    for (; t < nrt; t += kB) e += er[t];
     es[threadIdx.x] = e;
     __syncthreads();
     for (int s = kB / 2; s > 0; s >>= 1) {

@@ -323,6 +323,11 @@ int orc_lem_step(int nr, int nc, double dx, const uint8_t* mask,
                  const orc_lem_params* p, const int64_t* boundary_flats,
                  int64_t n_bf, const double* K_field, double* z,
                  orc_lem_stats* st);
+/* OpenMP threads of the oracle's parallel loops (results do not depend on
+ * the count); orc_max_threads reports the current setting */
+void orc_set_threads(int n);
+int orc_max_threads(void);
+
 int orc_lem_step_ex(int nr, int nc, double dx, const uint8_t* mask,
                     const orc_lem_params* p, const int64_t* boundary_flats,
                     int64_t n_bf, const double* K_field, double* z,

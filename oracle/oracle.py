@@ -53,6 +53,10 @@ def lib():
         L.orc_fill_priority_flood.argtypes = [C.c_int, C.c_int, f32p, u8p,
                                               C.c_double, C.c_char_p, f32p]
         L.orc_fill_priority_flood.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = None
+        L.orc_max_threads.argtypes = []
+        L.orc_max_threads.restype = C.c_int
         L.orc_fnv1a.argtypes = [C.c_uint64, C.c_void_p, C.c_int64]
         L.orc_fnv1a.restype = C.c_uint64
         L.orc_stream_power_d8.argtypes = [C.c_int64, u8p, i64p, u8p, i64p, C.c_int64, f64p,
@@ -126,6 +130,12 @@ def lib():
         L.orc_ts_monitor_update.restype = C.c_int
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's parallel loops; returns the setting"""
+    lib().orc_set_threads(int(n))
+    return lib().orc_max_threads()
 
 
 def fnv1a(h: int, data: bytes) -> int:
@@ -260,6 +270,42 @@ def fill(z, mask, nodata=-9999.0, bc="oooo"):
         raise OracleError(f"fill failed (status {st}, bc {bc!r})")
     return out
 
+
+
+# ---------------------------------------------------------------------------
+# The reference's own fill, built from its sources by `make -C oracle ref`
+# (oracle/ref_fill.cpp).  Absent unless the reference tree was present at build.
+# ---------------------------------------------------------------------------
+REF_FILL_PATH = os.path.join(_HERE, "_ref", "libref_fill.so")
+_ref_lib = None
+
+
+def ref_fill_available() -> bool:
+    return os.path.exists(REF_FILL_PATH)
+
+
+def ref_fill(z, mask, dx=1.0, bc="oooo", min_slope=0.0, n_threads=1, nodata=None, mode=0):
+    """lsdtt::fill_pits(raster, min_slope, bc, n_threads) (mode 0, the call of
+    src/cpu/lsdtt_lem_cpu.cpp:826) or fill_pits_priority_flood (mode 1) of the
+    reference itself.  Cells equal to `nodata` join the invalid mask."""
+    global _ref_lib
+    if _ref_lib is None:
+        L = C.CDLL(REF_FILL_PATH)
+        L.ref_fill.argtypes = [C.c_int, C.c_int, C.c_double, f32p, u8p, C.c_float,
+                               C.c_char_p, C.c_int, C.c_int, f32p]
+        L.ref_fill.restype = C.c_int
+        L.ref_fill_error.restype = C.c_char_p
+        _ref_lib = L
+    nr, nc = z.shape
+    zf = np.ascontiguousarray(z, dtype=np.float32)
+    m = np.ascontiguousarray(mask, dtype=np.uint8).copy()
+    if nodata is not None:
+        m[zf == np.float32(nodata)] = 0
+    out = np.empty_like(zf)
+    if _ref_lib.ref_fill(nr, nc, float(dx), zf.ravel(), m.ravel(), float(min_slope),
+                         bc.encode(), int(n_threads), int(mode), out.ravel()) != 0:
+        raise OracleError("reference fill: " + _ref_lib.ref_fill_error().decode())
+    return out
 
 # ---------------------------------------------------------------------------
 # LEM pieces (oracle/orc_lem.c)

@@ -21,7 +21,11 @@
  *     Eigen SparseLU / BiCGSTAB+ILUT, absent here); the Newton iteration,
  *     line search, restoration mode and sub-stepping are the reference's;
  *   - convergence: ||F||inf <= max(abs_tol, rel_tol * max(||F(z^n)||inf, U))
- *     (abs_tol 1e-12 m/yr is the reference's default criterion).
+ *     (abs_tol 1e-12 m/yr is the reference's default criterion);
+ *   - the hillslope solve's per-cell loops run under OpenMP (the reference's
+ *     Newton is OpenMP-parallel as well, lsdtt_hillslope_newton_cpu.cpp:216);
+ *     every sum is taken over fixed 4096-element blocks added in block
+ *     order, so results do not depend on the thread count.
  *
  * Parity status: stream power pinned by the n = 1 closed form and the n != 1
  * root residual (tests/test_oracle_lem.py); the hillslope solver by the
@@ -38,6 +42,45 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int orc_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* sum of a[k]*b[k] over fixed blocks added in block order (thread-count
+ * independent); part holds (n + ORC_BLK - 1) / ORC_BLK doubles */
+#define ORC_BLK 4096
+static double dot_blocks(const double* a, const double* b, int64_t n, double* part)
+{
+    const int64_t nb = (n + ORC_BLK - 1) / ORC_BLK;
+    #pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < nb; ++q) {
+        const int64_t k1 = (q + 1) * ORC_BLK < n ? (q + 1) * ORC_BLK : n;
+        double acc = 0.0;
+        for (int64_t k = q * ORC_BLK; k < k1; ++k) acc += a[k] * b[k];
+        part[q] = acc;
+    }
+    double acc = 0.0;
+    for (int64_t q = 0; q < nb; ++q) acc += part[q];
+    return acc;
+}
 
 /* ----------------------------------------------------------------------- */
 /* stream power                                                            */
@@ -174,6 +217,7 @@ static double residual(const hs_t* s, const double* z, double* F)
 {
     const double DA = s->D / (s->d.h * s->d.h);
     double norm = 0.0;
+    #pragma omp parallel for schedule(static) reduction(max:norm)
     for (int64_t k = 0; k < s->ndof; ++k) {
         const int r = s->rc[2 * k], c = s->rc[2 * k + 1];
         double a[4], sn[4];
@@ -194,6 +238,7 @@ static double residual(const hs_t* s, const double* z, double* F)
 static double max_ratio(const hs_t* s, const double* z)
 {
     double m = 0.0;
+    #pragma omp parallel for schedule(static) reduction(max:m)
     for (int64_t k = 0; k < s->ndof; ++k) {
         const int r = s->rc[2 * k], c = s->rc[2 * k + 1];
         const double sx = slope_x(&s->d, z, r, c), sy = slope_y(&s->d, z, r, c);
@@ -225,6 +270,7 @@ static void assemble(const hs_t* s, const double* z, row_t* J)
 {
     const double DA = s->D / (s->d.h * s->d.h);
     const double h = s->d.h, Sc2 = s->Sc * s->Sc;
+    #pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < s->ndof; ++k) {
         const int r = s->rc[2 * k], c = s->rc[2 * k + 1];
         row_t* row = &J[k];
@@ -269,6 +315,7 @@ static void assemble(const hs_t* s, const double* z, row_t* J)
 
 static void matvec(const row_t* J, int64_t n, const double* x, double* y)
 {
+    #pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < n; ++k) {
         double a = 0.0;
         for (int i = 0; i < J[k].n; ++i) a += J[k].val[i] * x[J[k].col[i]];
@@ -281,6 +328,8 @@ static int gmres(const row_t* J, int64_t n, const double* b, double* x, double r
 {
     const int m = 60;
     double* dinv = (double*)malloc(sizeof(double) * (size_t)n);
+    double* part = (double*)malloc(sizeof(double) * (size_t)((n + ORC_BLK - 1) / ORC_BLK));
+    #pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < n; ++k) {
         double dg = 0.0;
         for (int i = 0; i < J[k].n; ++i) if (J[k].col[i] == k) dg = J[k].val[i];
@@ -293,18 +342,17 @@ static int gmres(const row_t* J, int64_t n, const double* b, double* x, double r
     double* g = (double*)malloc(sizeof(double) * (m + 1));
     double* w = (double*)malloc(sizeof(double) * (size_t)n);
     double* t = (double*)malloc(sizeof(double) * (size_t)n);
-    double bn = 0.0;
-    for (int64_t k = 0; k < n; ++k) bn += b[k] * b[k];
-    bn = sqrt(bn);
-    for (int64_t k = 0; k < n; ++k) x[k] = 0.0;
+    double bn = sqrt(dot_blocks(b, b, n, part));
+    memset(x, 0, sizeof(double) * (size_t)n);
     int ok = (bn == 0.0);
     int it = 0;
     while (!ok && it < maxit) {
         matvec(J, n, x, w);
-        double beta = 0.0;
-        for (int64_t k = 0; k < n; ++k) { V[k] = b[k] - w[k]; beta += V[k] * V[k]; }
-        beta = sqrt(beta);
+        #pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < n; ++k) V[k] = b[k] - w[k];
+        const double beta = sqrt(dot_blocks(V, V, n, part));
         if (beta <= rtol * bn) { ok = 1; break; }
+        #pragma omp parallel for schedule(static)
         for (int64_t k = 0; k < n; ++k) V[k] /= beta;
         memset(g, 0, sizeof(double) * (m + 1));
         g[0] = beta;
@@ -312,20 +360,22 @@ static int gmres(const row_t* J, int64_t n, const double* b, double* x, double r
         for (j = 0; j < m && it < maxit; ++j, ++it) {
             double* vj = V + (size_t)j * n;
             double* vn = V + (size_t)(j + 1) * n;
+            #pragma omp parallel for schedule(static)
             for (int64_t k = 0; k < n; ++k) t[k] = vj[k] * dinv[k];
             matvec(J, n, t, vn);
             for (int i = 0; i <= j; ++i) {
                 const double* vi = V + (size_t)i * n;
-                double hij = 0.0;
-                for (int64_t k = 0; k < n; ++k) hij += vn[k] * vi[k];
+                const double hij = dot_blocks(vn, vi, n, part);
                 Hm[i * m + j] = hij;
+                #pragma omp parallel for schedule(static)
                 for (int64_t k = 0; k < n; ++k) vn[k] -= hij * vi[k];
             }
-            double hn = 0.0;
-            for (int64_t k = 0; k < n; ++k) hn += vn[k] * vn[k];
-            hn = sqrt(hn);
+            const double hn = sqrt(dot_blocks(vn, vn, n, part));
             Hm[(j + 1) * m + j] = hn;
-            if (hn > 0.0) for (int64_t k = 0; k < n; ++k) vn[k] /= hn;
+            if (hn > 0.0) {
+                #pragma omp parallel for schedule(static)
+                for (int64_t k = 0; k < n; ++k) vn[k] /= hn;
+            }
             for (int i = 0; i < j; ++i) {
                 const double a0 = Hm[i * m + j], a1 = Hm[(i + 1) * m + j];
                 Hm[i * m + j] = cs[i] * a0 + sn[i] * a1;
@@ -348,6 +398,7 @@ static int gmres(const row_t* J, int64_t n, const double* b, double* x, double r
             for (int q = i + 1; q < j; ++q) s -= Hm[i * m + q] * y[q];
             y[i] = Hm[i * m + i] != 0.0 ? s / Hm[i * m + i] : 0.0;
         }
+        #pragma omp parallel for schedule(static)
         for (int64_t k = 0; k < n; ++k) {
             double s = 0.0;
             for (int i = 0; i < j; ++i) s += V[(size_t)i * n + k] * y[i];
@@ -357,10 +408,11 @@ static int gmres(const row_t* J, int64_t n, const double* b, double* x, double r
     }
     if (!ok) {   /* final check */
         matvec(J, n, x, w);
-        double rn = 0.0;
-        for (int64_t k = 0; k < n; ++k) rn += (b[k] - w[k]) * (b[k] - w[k]);
-        ok = sqrt(rn) <= rtol * bn * 10.0;
+        #pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < n; ++k) t[k] = b[k] - w[k];
+        ok = sqrt(dot_blocks(t, t, n, part)) <= rtol * bn * 10.0;
     }
+    free(part);
     free(dinv); free(V); free(Hm); free(cs); free(sn); free(g); free(w); free(t);
     return ok ? 0 : 1;
 }
@@ -418,6 +470,7 @@ int orc_hillslope_newton(int nr, int nc, double h, const uint8_t* mask,
     for (; it < p->max_iterations; ++it) {
         if (fnorm <= tol) { converged = 1; break; }
         assemble(&s, z, J);
+        #pragma omp parallel for schedule(static)
         for (int64_t k = 0; k < nd; ++k) rhs[k] = -F[k];
         if (gmres(J, nd, rhs, step, 1e-13, 20000) != 0) break;
         double lambda = 1.0, fnorm_try = fnorm, max_dz = 0.0;
@@ -427,6 +480,7 @@ int orc_hillslope_newton(int nr, int nc, double h, const uint8_t* mask,
         for (int ls = 0; ls < p->max_line_search; ++ls) {
             memcpy(zt, z, sizeof(double) * (size_t)n);
             max_dz = 0.0;
+            #pragma omp parallel for schedule(static) reduction(max:max_dz)
             for (int64_t k = 0; k < nd; ++k) {
                 const double dz = lambda * step[k];
                 zt[(int64_t)s.rc[2 * k] * nc + s.rc[2 * k + 1]] += dz;

@@ -92,6 +92,11 @@ def test_gpu_routing_pipeline_4096_bit_exact(gpu):
     zo = orc.fill(z, mask)
     assert np.array_equal(zf.view(np.uint32), zo.view(np.uint32))
     assert (zo > z).sum() > 0.005 * z.size   # the bowls were filled
+    if orc.ref_fill_available():     # the reference's own fill, built from its sources
+        zr = orc.ref_fill(z, mask, 10.0, "oooo")
+        assert np.array_equal(zf.view(np.uint32), zr.view(np.uint32))
+        print("[gpu routing pipeline 4096] reference fill agrees", flush=True)
+        del zr
     del zo
     print("[gpu routing pipeline 4096] fill ok", flush=True)
     fi = gpu.FlowInfo(zf, mask, 10.0, "oooo", allow_pits_as_outlets=True)

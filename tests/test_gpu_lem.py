@@ -81,6 +81,36 @@ def test_gpu_linear_lem_matches_oracle(gpu):
     assert np.abs(zg - zo).max() <= max(tol, 1e-8), (np.abs(zg - zo).max(), tol)
 
 
+def test_gpu_plain_cpu_workload_full_trajectory(gpu):
+    """the plain CPU workload as written: 512 x 512 raw (unsmoothed) surface
+    at dx = 30 m (0.01 tilt, beta-2 noise with 50 m std, 200 pits 1-5 m),
+    linear D = 0.01, K = 1e-5, m = 0.5, n = 1, U = 1e-3, dt = 1000 yr, south
+    edge fixed, 200 steps.  Each step solves to ||F|| <= tol, so the states
+    may drift apart by at most about 4 dt tol per step; a receiver choice
+    flipped by a last-bit difference would show as a jump far above that."""
+    print("[gpu lem plain cpu workload 512^2 x 200]", flush=True)
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(512, 512, dx=30.0, seed=2024)
+    mask = np.ones(z.shape, np.uint8)
+    kw = dict(timestep_years=1000.0, erodibility=1e-5, m_exponent=0.5, n_exponent=1.0, uplift_rate=1e-3,
+              hillslope_diffusivity=0.01, boundary_conditions="nnon")
+    lem = gpu.LandscapeModel(z, 30.0, mask=mask, **kw)
+    p = orc.lem_params(dt=1000.0, K=1e-5, m=0.5, n=1.0, U=1e-3, D=0.01, bc="nnon")
+    bf = orc.boundary_flats(mask, "nnon")
+    zo = z.copy()
+    bound = 0.0
+    for i in range(200):
+        sg = lem.step()
+        zo, so, rc = orc.lem_step(zo, mask, 30.0, p, bf)
+        assert rc == 0 and sg["hillslope_converged"] == 1
+        assert sg["cells_eroded"] == so["cells_eroded"], i
+        bound += 4 * 1000.0 * max(sg["hillslope_tolerance"], 1e-12)
+    d = np.abs(lem.elevation - zo).max()
+    print(f"[gpu lem plain cpu workload] max |dz| after 200 steps {d:.3e} (bound {bound:.3e})", flush=True)
+    assert d <= max(bound, 1e-7)
+    assert abs(sg["mean_erosion_rate"] - so["mean_erosion_rate"]) <= 1e-9
+
+
 def test_gpu_roering_lem_matches_oracle(gpu):
     """config-2 physics (Roering, Sc = 0.8, n = 1.5) on a sub-critical start"""
     print("[gpu lem roering]", flush=True)
