@@ -278,6 +278,14 @@ typedef struct {
      * filled at :666-696): keep each step's per-pixel erosion rate for
      * lsdtt_gpu_lem_get_erosion_rate; default 0 */
     int32_t compute_erosion_rate_field;
+    /* Krylov forcing of the Newton solve (no reference counterpart: the
+     * reference factorises exactly).  0: every linear solve to
+     * krylov_rel_tol; 1 (default): krylov_rel_tol for the first Newton
+     * iteration, then Eisenstat-Walker choice 2 (gamma 0.9, exponent 2,
+     * with its safeguard), never tighter than needed to reach the Newton
+     * tolerance from the current residual, within [1e-6, 0.1].  The Newton
+     * stop is unchanged, so either setting meets the same tolerance. */
+    int32_t krylov_forcing;
 } lsdtt_gpu_lem_options;
 
 typedef struct {

@@ -42,7 +42,7 @@ class LemOptions(C.Structure):
         ("picard_dt_init", C.c_double), ("picard_dt_min", C.c_double), ("picard_dt_max", C.c_double),
         ("picard_grow_factor", C.c_double), ("picard_shrink_factor", C.c_double),
         ("picard_dz_target", C.c_double), ("enable_dinf_fluvial_erosion", C.c_int32),
-        ("compute_erosion_rate_field", C.c_int32),
+        ("compute_erosion_rate_field", C.c_int32), ("krylov_forcing", C.c_int32),
     ]
 
 

@@ -477,6 +477,7 @@ void BatchEngine::build_layout(const std::vector<BatchSpec>& specs)
         h.cap = p.slope_ratio_cap;
         h.nonlinear = p.nonlinear;
         h.eta2 = p.krylov_rel_tol * p.krylov_rel_tol;
+        h.forcing = p.krylov_forcing;
         h.krylov_max = p.krylov_max_iterations;
         h.newton_max = p.newton_max_iterations;
         h.ls_max = p.newton_max_line_search;
@@ -925,8 +926,8 @@ int BatchEngine::step(std::vector<LemStats>& st)
         hsb_start(bhs(), G, s);
         read_status(s);
     }
-    plan.count_passes = std::min(kMaxPass / 2, std::max(plan.count_passes, cset + 3));
-    plan.sp_passes = std::min(kMaxPass / 2, std::max(plan.sp_passes, sset + 3));
+    plan.count_passes = std::min(kMaxPatternPasses, std::max(plan.count_passes, cset + 3));
+    plan.sp_passes = std::min(kMaxPatternPasses, std::max(plan.sp_passes, sset + 3));
     int picard_count = 0;
     finish_step(st, s, &picard_count);
     return picard_count;
@@ -939,13 +940,20 @@ void BatchEngine::finish_step(std::vector<LemStats>& st, hipStream_t s, int* pic
     const BatchHs B = bhs();
     int more = 0;
     for (;;) {
-        bool any = false;
-        for (int i = 0; i < G; ++i) any |= solving(ctl_h[static_cast<size_t>(i)].phase);
-        if (!any) break;
+        // launches sized for the grids still solving (their lists only shrink)
+        int tiles = 0, chunks = 0;
+        for (int i = 0; i < G; ++i)
+            if (solving(ctl_h[static_cast<size_t>(i)].phase)) {
+                tiles += hsgrids_h[static_cast<size_t>(i)].nst;
+                chunks += hsgrids_h[static_cast<size_t>(i)].ngc;
+            }
+        if (tiles + chunks == 0) break;
+        // 2, 4, 8, ... rounds between status reads, at most 64 (a read is
+        // one synchronisation; rounds after a grid finishes are wasted)
         const int k = std::max(2, more);
-        for (int j = 0; j < k; ++j) hsb_round(B, G, s);
+        for (int j = 0; j < k; ++j) hsb_round(B, G, s, tiles, chunks);
         counters.eager_rounds += k;
-        more = 2 * k;
+        more = std::min(2 * k, 64);
         read_status(s);
     }
     int max_rounds = 0;
@@ -979,6 +987,7 @@ void BatchEngine::finish_step(std::vector<LemStats>& st, hipStream_t s, int* pic
     const int want = max_rounds + 2;
     if (want > plan.rounds) plan.rounds = want + std::max(2, want / 4);
     else if (want < plan.rounds / 2) plan.rounds = std::max(want + 2, plan.rounds * 3 / 4);
+    plan.rounds = std::min(plan.rounds, kMaxPatternRounds);
     LSDTT_HIP(hipMemcpyAsync(ctl_h.data(), ctl.p, sizeof(HsCtl) * G, hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipMemcpyAsync(stats_h.data(), stats_out.p, sizeof(double) * 8 * G, hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipMemcpyAsync(eroded_h.data(), eroded_out.p, sizeof(long long) * G, hipMemcpyDeviceToHost, s));

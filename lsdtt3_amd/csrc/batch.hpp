@@ -153,6 +153,11 @@ private:
     PicardSolver picard;
 
     static constexpr int kMaxPass = 4096;
+    // caps of the captured launch pattern, so that its graph stays a few
+    // thousand nodes (a solver round is 14 launches): sweeps that need more
+    // passes restart eagerly, solver rounds beyond the cap run eagerly after
+    // the replay
+    static constexpr int kMaxPatternPasses = 64, kMaxPatternRounds = 160;
 
     void build_layout(const std::vector<BatchSpec>& specs);
     void rebuild_routing_tables();

@@ -57,6 +57,7 @@ LemParams to_params(const lsdtt_gpu_lem_options& o, double o_cell_size)
     p.regime = o.hillslope_coupling_regime;
     p.force_picard = o.hillslope_force_picard;
     p.erosion_rate_field = o.compute_erosion_rate_field;
+    p.krylov_forcing = o.krylov_forcing;
     p.picard.picard_tol = o.picard_tol;
     p.picard.picard_max_iter = o.picard_max_iter;
     p.picard.omega = o.relaxation_omega;
@@ -157,6 +158,7 @@ void lsdtt_gpu_lem_options_init(lsdtt_gpu_lem_options* o)
     o->slope_ratio_cap = 0.9999;
     o->krylov_rel_tol = 1e-4;
     o->krylov_max_iterations = 500;
+    o->krylov_forcing = 1;
     o->drainage_area_kernel_size = 1;
     o->multiply_by_w_over_dx = 0;
     o->channel_width_w = 1.0;

@@ -1275,7 +1275,7 @@ __global__ void __launch_bounds__(256) kb_bi_xfin(BatchHs B)
         c.rnorm2 = out[1];
         const int it = c.kits + 1;
         c.kits = it;
-        const bool conv = out[1] <= g.eta2 * c.bnorm2;
+        const bool conv = out[1] <= c.eta * c.eta * c.bnorm2;
         const bool stall = (c.omega == 0.0) || (out[0] == 0.0);
         if (conv || it >= g.krylov_max || stall) c.kdone = 1;
         if (stall && !conv) c.kbreak = 1;
@@ -1360,6 +1360,7 @@ __device__ __forceinline__ void control_one(const BatchHs& B, int gi, int stage)
             c.first_solve = 0;
         }
         c.it = 0;
+        c.eta = sqrt(g.eta2);
         if (c.fnorm <= c.tol) end_solve(c, 0, true);
         else if (g.newton_max <= 0) end_solve(c, 0, false);
         else c.phase = HS_KRYLOV;
@@ -1380,12 +1381,18 @@ __device__ __forceinline__ void control_one(const BatchHs& B, int gi, int stage)
     // Krylov start) came with the trial
     c.zc = c.zt;
     c.zt = free_buffer(c.zc, c.zn);
+    const double fprev = c.fnorm;
     c.fnorm = ft;
     c.ratio = rt;
     const int its = ++c.it;
-    if (c.fnorm <= c.tol) end_solve(c, its, true);
-    else if (its >= g.newton_max) end_solve(c, its, false);
-    else c.phase = HS_KRYLOV;
+    if (c.fnorm <= c.tol) {
+        end_solve(c, its, true);
+    } else if (its >= g.newton_max) {
+        end_solve(c, its, false);
+    } else {
+        if (g.forcing == 1) c.eta = forcing_eta(c.eta, c.fnorm, fprev, c.tol);
+        c.phase = HS_KRYLOV;
+    }
 }
 
 
@@ -1552,12 +1559,14 @@ void hsb_start(const BatchHs& B, int n_grids, hipStream_t s)
     LSDTT_LAUNCH_CHECK();
 }
 
-void hsb_round(const BatchHs& B, int n_grids, hipStream_t s)
+void hsb_round(const BatchHs& B, int n_grids, hipStream_t s, int tiles, int chunks)
 {
     // one block per tile (chunk) of the batch, enough for every list; the
     // blocks beyond a list's length exit at once, and the item lookups of
     // the others overlap across the grid
-    const int gt = std::max(1, B.NT), gc = std::max(1, B.NGC), gm = std::max(1, B.NT + B.NGC);
+    const int nt = tiles >= 0 ? std::min(tiles, B.NT) : B.NT;
+    const int ng = chunks >= 0 ? std::min(chunks, B.NGC) : B.NGC;
+    const int gt = std::max(1, nt), gc = std::max(1, ng), gm = std::max(1, nt + ng);
     kb_bi_p<<<gt, kB, 0, s>>>(B);
     kb_bi_v<<<gm, kB, 0, s>>>(B);
     kb_bi_vfin<<<n_grids, kB, 0, s>>>(B);

@@ -106,7 +106,25 @@ struct HsCtl {
     double fnorm, ratio, tol, lambda, f_init;
     double trial_f, trial_ratio;
     double rho, rho_old, alpha, omega, bnorm2, rnorm2;
+    double eta;  // relative tolerance of the current linear solve
 };
+
+// Relative tolerance of the next Newton iteration's linear solve after one
+// that took the residual from fprev to fnorm: Eisenstat-Walker choice 2
+// (SIAM J. Sci. Comput. 17, 1996), eta = 0.9 (fnorm / fprev)^2, kept at
+// least 0.9 eta_prev^2 when that exceeds 0.1, no tighter than half of what
+// reaching tol from fnorm needs, within [1e-6, 0.1].
+__host__ __device__ inline double forcing_eta(double eta_prev, double fnorm, double fprev, double tol)
+{
+    const double ratio = fnorm / fprev;
+    double e = 0.9 * ratio * ratio;
+    const double guard = 0.9 * eta_prev * eta_prev;
+    if (guard > 0.1) e = e > guard ? e : guard;
+    const double need = 0.5 * tol / fnorm;
+    e = e > need ? e : need;
+    e = e > 1e-6 ? e : 1e-6;
+    return e < 0.1 ? e : 0.1;
+}
 
 // work item of the batched kernels: a tile of kBatchTileRows x
 // kBatchTileCols cells
@@ -125,7 +143,8 @@ struct BatchHsGrid {
     int gen_off, n_gen, gcbase, ngc;
     double h, D, Sc, cap;
     int nonlinear;
-    double eta2;       // Krylov relative tolerance squared
+    double eta2;       // Krylov relative tolerance squared (the first Newton iteration's)
+    int forcing;       // 1: later iterations by forcing_eta, 0: eta2 throughout
     int krylov_max;
     int newton_max, ls_max;
     double abs_tol, rel_tol, dt_total;
@@ -162,7 +181,10 @@ void hsb_start(const BatchHs& B, int n_grids, hipStream_t s);
 // One round: a BiCGSTAB iteration of the KRYLOV grids, the control step
 // that sends finished solves to their line search, the trials of the START
 // and SEARCH grids and the control step that judges them.
-void hsb_round(const BatchHs& B, int n_grids, hipStream_t s);
+// one solver round; `tiles` / `chunks` (when >= 0) bound the work items
+// listed in it (the kernels stride over their lists, so any launch size is
+// correct; a bound only trims empty blocks)
+void hsb_round(const BatchHs& B, int n_grids, hipStream_t s, int tiles = -1, int chunks = -1);
 
 void hs_fast_flags(const HsDom& d, uint8_t* fast, hipStream_t s);
 // companion-kernel blocks for n_gen general cells (their partials follow the

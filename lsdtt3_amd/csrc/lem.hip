@@ -604,7 +604,7 @@ HsDom LemEngine::hs_domain(double inv_dt) const
     return with_reciprocals(d);
 }
 
-int LemEngine::bicgstab(hipStream_t s, int newton_it)
+int LemEngine::bicgstab(hipStream_t s, int newton_it, double eta)
 {
     BiArgs a;
     a.d = hs_domain(cur_inv_dt);
@@ -625,7 +625,7 @@ int LemEngine::bicgstab(hipStream_t s, int newton_it)
     a.partials = partials.p;
     a.ticket = tickets.p;
     a.flags = flags.p;
-    a.eta2 = p.krylov_rel_tol * p.krylov_rel_tol;
+    a.eta2 = eta * eta;
     a.max_it = p.krylov_max_iterations;
     a.first = 0;
     // the start (rho = ||b||^2, flags) was set by the trial that produced b
@@ -715,13 +715,16 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
     }
     int it = 0;
     bool converged = false;
+    double eta = p.krylov_rel_tol, fprev = 0.0;
     for (; it < p.newton_max_iterations; ++it) {
         if (fnorm <= tol) {
             converged = true;
             break;
         }
+        if (p.krylov_forcing == 1 && it > 0) eta = forcing_eta(eta, fnorm, fprev, tol);
+        fprev = fnorm;
         // b and dinv at the iterate come from the trial that accepted it
-        st->krylov_iterations += bicgstab(s, it);
+        st->krylov_iterations += bicgstab(s, it, eta);
         // line search (newton .cpp:553-633)
         const bool restoring = p.nonlinear && ratio_now > feasible;
         if (restoring) ++st->restoration_iterations;

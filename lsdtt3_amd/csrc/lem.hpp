@@ -49,6 +49,7 @@ struct LemParams {
     int regime = 0;             // 0 automatic, 1 coupled, 2 channels_as_boundaries
     int force_picard = 0;       // hillslope_force_picard
     int erosion_rate_field = 0; // compute_erosion_rate_field
+    int krylov_forcing = 1;     // 0 fixed krylov_rel_tol, 1 Eisenstat-Walker
     PicardOptions picard;       // fallback settings (D, Sc, nonlinear from above; U = 0)
 };
 
@@ -189,7 +190,7 @@ private:
     void dinf_stream_power(const SpParams& a, hipStream_t s);
     int hillslope(LemStats* st, hipStream_t s);
     int newton_solve(double inv_dt, LemStats* st, hipStream_t s);
-    int bicgstab(hipStream_t s, int newton_it);
+    int bicgstab(hipStream_t s, int newton_it, double eta);
     void trial(double lambda, double* fnorm, double* ratio, hipStream_t s);
     HsDom hs_domain(double inv_dt) const;
     // hillslope DOFs (valid, not on an 'o' edge, not a channel pixel of an

@@ -175,6 +175,7 @@ def test_gpu_ensemble_solver_rounds_beyond_the_pattern(gpu):
     ms = models()[:3]
     for m in ms:
         m["krylov_rel_tol"] = 1e-7   # many BiCGSTAB iterations per Newton iteration
+        m["krylov_forcing"] = 0      # at every Newton iteration
     eg = gpu.Ensemble(ms)
     ed = gpu.Ensemble(ms)
     ed.set_graph(False)
