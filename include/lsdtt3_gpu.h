@@ -180,11 +180,13 @@ int lsdtt_gpu_flow_chi_threshold(lsdtt_gpu_flow* f, double m_over_n, double A_0,
  * lsdtt_raster_flow_preprocessing_cpu.cpp:707-782 -> reconstruction
  * :1203-1394 / priority flood :893-920).  valid = mask != 0 && z != nodata
  * (NaN-aware); invalid cells are written as nodata.  Periodic pairs must
- * match and may not both be periodic (LSDTT_GPU_ERR_ARG).
+ * match and may not both be periodic (LSDTT_GPU_ERR_ARG); grids of 2^29
+ * cells or more are rejected (LSDTT_GPU_ERR_ARG).
  * lsdtt_gpu_fill_device works on device arrays on the caller's stream (a
  * hipStream_t, NULL = the null stream) and returns when the fill has
- * finished there.  Its working keys live in one library workspace: calls
- * from several host threads or streams are safe and run one at a time.
+ * finished there.  Its working arrays (44 bytes per cell) live in one
+ * library workspace: calls from several host threads or streams are safe
+ * and run one at a time.
  * ------------------------------------------------------------------- */
 int lsdtt_gpu_fill(const float* z, const uint8_t* mask, int32_t n_rows, int32_t n_cols,
                    double nodata, const char* boundary_conditions, float* filled);
@@ -194,8 +196,8 @@ int lsdtt_gpu_fill_device(const float* d_z, const uint8_t* d_mask, int32_t n_row
 /* No counterpart in the reference: lsdtt_gpu_fill_device for callers that
  * cannot allocate device memory (code under HIP graph capture, serving
  * loops, harnesses that forbid hipMalloc).  Same arguments and results
- * (bit for bit); the working keys live in a fixed device array of the
- * library (48.3 MiB per GPU, held until the process ends) instead of an
+ * (bit for bit); the working arrays live in a fixed device array of the
+ * library (176 MiB per GPU, held until the process ends) instead of an
  * allocated workspace, so grids are limited to
  * lsdtt_gpu_fill_noalloc_max_cells() = 4194304 cells (2048 x 2048, any
  * shape); larger grids are rejected with LSDTT_GPU_ERR_ARG.  Calls share

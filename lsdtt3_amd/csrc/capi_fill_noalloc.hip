@@ -27,9 +27,11 @@ using namespace lsdtt_gpu;
 
 namespace {
 
-// room for the fill's keys of every grid of up to 2^22 cells (2048 x 2048,
-// any shape: 3 ints per cell, 3 per 64 x 64 tile, 128 flags): 48.3 MiB
-constexpr size_t kScratchInts = 3ull * (1ull << 22) + 3ull * (1ull << 16) + 128ull;
+// room for the fill's working arrays of every grid of up to 2^22 cells
+// (2048 x 2048, any shape: 11 ints per cell -- cheapest edges 2, keys 2,
+// roots, labels, two component lists, parents, hooks 2 -- and 4096 for
+// flags and compaction offsets): 176 MiB
+constexpr size_t kScratchInts = 11ull * (1ull << 22) + 4096ull;
 
 static __device__ __align__(256) int scratch[kScratchInts];
 
