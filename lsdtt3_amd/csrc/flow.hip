@@ -310,16 +310,22 @@ __global__ void k_pj_init(const int* __restrict__ rcv, const uint8_t* __restrict
 __global__ void k_pj_round(unsigned long long* __restrict__ P, long long n, const int* __restrict__ prev,
                            int* __restrict__ cur)
 {
+    // grid-stride over a few thousand blocks: a round after convergence
+    // returns at once, and costs little as a launch
     if (*prev == 0) return;
-    const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (f >= n) return;
-    const unsigned long long w = ld_relaxed(&P[f]);
-    const unsigned nx = static_cast<unsigned>(w);
-    if (nx == f) return;
-    const unsigned long long wn = ld_relaxed(&P[nx]);
-    const unsigned nn = static_cast<unsigned>(wn);
-    const bool ch = nn != nx;
-    if (ch) st_relaxed(&P[f], pack(static_cast<unsigned>(w >> 32) + static_cast<unsigned>(wn >> 32), nn));
+    bool ch = false;
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
+         f += (long long)gridDim.x * blockDim.x) {
+        const unsigned long long w = ld_relaxed(&P[f]);
+        const unsigned nx = static_cast<unsigned>(w);
+        if (nx == f) continue;
+        const unsigned long long wn = ld_relaxed(&P[nx]);
+        const unsigned nn = static_cast<unsigned>(wn);
+        if (nn != nx) {
+            st_relaxed(&P[f], pack(static_cast<unsigned>(w >> 32) + static_cast<unsigned>(wn >> 32), nn));
+            ch = true;
+        }
+    }
     // one flag store per wavefront
     const unsigned long long m = __ballot(ch);
     if (ch && (threadIdx.x & 63) == __ffsll(m) - 1) *cur = 1;
@@ -726,7 +732,7 @@ void FlowEngine::pointer_jump(const int* weight, int* value_out, int* root_out, 
     for (int r = 0; r < kRounds; ++r) {
         {
             TScope ts_(timer, "pointer_jump", s, 24.0 * n);
-            k_pj_round<<<g1, kBlock, 0, s>>>(pj, n, chg + r, chg + r + 1);
+            k_pj_round<<<std::min(g1, 2048), kBlock, 0, s>>>(pj, n, chg + r, chg + r + 1);
         }
         LSDTT_LAUNCH_CHECK();
     }

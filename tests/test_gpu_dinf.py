@@ -96,6 +96,7 @@ def test_gpu_lem_threshold_channels_match_oracle(gpu, bc, nexp):
     p = orc.lem_params(500.0, 2e-5, 0.5, nexp, 5e-4, 0.02, Sc=0.8, nonlinear=True, bc=bc, threshold=600.0)
     pf = orc.lem_params(500.0, 2e-5, 0.5, nexp, 5e-4, 0.02, Sc=0.8, nonlinear=True, bc=bc, threshold=600.0,
                         hillslope=False, uplift=False)
+    bound = 0.0
     for step in range(2):
         # the oracle's channel set on the post-incision surface of this step
         zi, _, _ = orc.lem_step(zo, mask, 30.0, pf, boundary=np.zeros(0, np.int64))
@@ -106,7 +107,12 @@ def test_gpu_lem_threshold_channels_match_oracle(gpu, bc, nexp):
         ch = lem.channel_mask
         assert 0 < ch.sum() < ch.size // 4
         assert np.array_equal(ch, ch_o), step
-        tol = max(4 * 500.0 * max(so["hillslope_residual"], 1e-12), 1e-8)
+        # both stop at ||F||inf <= 1e-8 ||F(z^n)||inf; each is within dt ||F|| / (1 - rho)
+        # of the root, with the residual each actually reached (the GPU's adaptive
+        # Krylov forcing stops nearer the tolerance than the oracle's exact solves);
+        # a later step starts from states already that far apart
+        bound += 4 * 500.0 * max(so["hillslope_residual"], st["hillslope_residual"], 1e-12)
+        tol = max(bound, 1e-8)
         assert np.abs(lem.elevation - zo).max() <= max(tol, 1e-6), step
         assert st["cells_eroded"] == so["cells_eroded"]
 
@@ -173,11 +179,17 @@ def test_gpu_dinf_lem_full_step_matches_oracle(gpu):
     p = orc.lem_params(500.0, 2e-5, 0.5, 1.0, 5e-4, 0.02, Sc=0.8, nonlinear=True, bc="nnon", threshold=600.0,
                        fluvial_dinf=True)
     zo = z.copy()
+    bound = 0.0
     for step in range(2):
         st = lem.step()
         zo, so, rc = orc.lem_step(zo, mask, 30.0, p)
         assert rc == 0
-        tol = max(4 * 500.0 * max(so["hillslope_residual"], 1e-12), 1e-8)
+        # both stop at ||F||inf <= 1e-8 ||F(z^n)||inf; each is within dt ||F|| / (1 - rho)
+        # of the root, with the residual each actually reached (the GPU's adaptive
+        # Krylov forcing stops nearer the tolerance than the oracle's exact solves);
+        # a later step starts from states already that far apart
+        bound += 4 * 500.0 * max(so["hillslope_residual"], st["hillslope_residual"], 1e-12)
+        tol = max(bound, 1e-8)
         assert np.abs(lem.elevation - zo).max() <= max(tol, 1e-6), step
         assert st["cells_eroded"] == so["cells_eroded"]
 
