@@ -48,7 +48,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-lear
 
 Profile summaries (text, for profiles/) from rocprofv3 CSV output:
   python bench.py --summarize-trace DIR OUT [TITLE CMD NOTE]   (--kernel-trace --stats run)
-  python bench.py --summarize-pmc FETCH_DIR WRITE_DIR OUT
+  python bench.py --summarize-pmc FETCH_DIR WRITE_DIR OUT [TITLE CMD]
                                               (--pmc FETCH_SIZE / WRITE_SIZE runs)
 The PMC summary scales each counter by what it reads for the step's plain
 copy kernel (k_copy: n doubles read and written) and records the HBM bytes
@@ -200,7 +200,9 @@ def cpu_baseline(K, budget_s=15.0, size=512):
 
 # timer group -> the kernel whose dispatches it times
 GROUP_KERNEL = {"stream_power": "k_tile_sp", "upstream_counts": "k_tile_counts", "tile_depths": "k_tile_order",
-                "hillslope_residual": "k_trial", "hillslope_prepare": "k_prepare", "receivers": "k_receivers"}
+                "hillslope_residual": "k_trial", "hillslope_prepare": "k_prepare", "receivers": "k_receivers",
+                "bicgstab_matvec": "k_bi_v", "bicgstab_x": "k_bi_x", "bicgstab_s": "k_bi_s", "bicgstab_p": "k_bi_p",
+                "step_stats": "k_stats"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lem_step_4096_hbm_traffic.txt")
 
 
@@ -250,7 +252,11 @@ def summarize_trace(d, out, title=None, cmd=None, note=None):
         f.write("\n".join(lines) + "\n")
 
 
-def summarize_pmc(fetch_dir, write_dir, out, n_cells=4096 * 4096):
+PMC_CMD = "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-learned --no-batched --no-unsmoothed " \
+          "--autotune-steps 0 --profile-steps 1"
+
+
+def summarize_pmc(fetch_dir, write_dir, out, title=None, cmd=None, n_cells=4096 * 4096):
     per = {}
     for key, d in (("fetch", fetch_dir), ("write", write_dir)):
         for r in _csv(_find(d, "counter_collection.csv")):
@@ -278,12 +284,13 @@ def summarize_pmc(fetch_dir, write_dir, out, n_cells=4096 * 4096):
         ff, fw = float(m.group(1)), float(m.group(2))
         calib = (f"# calibration: no plain copy in this run; scale factors fetch {ff:.4f}, write {fw:.4f} "
                  f"carried over from the previous calibration against k_copy")
-    lines = ["# HBM traffic per dispatch from rocprofv3 PMC counters, LEM step at 4096^2 (bench.py workload)", "#",
-             "# made by: cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch_dir> -o run -- "
-             "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-learned --autotune-steps 0",
-             "#          cd /tmp && rocprofv3 --pmc WRITE_SIZE --output-format csv -d <write_dir> -o run -- "
-             "python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-learned --autotune-steps 0",
-             "# summarized by: python bench.py --summarize-pmc <fetch_dir> <write_dir> <this file>",
+    title = title or "LEM step at 4096^2 (bench.py workload)"
+    cmd = cmd or PMC_CMD
+    lines = [f"# HBM traffic per dispatch from rocprofv3 PMC counters, {title}", "#",
+             "# made by: cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch_dir> -o run -- " + cmd,
+             "#          cd /tmp && rocprofv3 --pmc WRITE_SIZE --output-format csv -d <write_dir> -o run -- " + cmd,
+             "# summarized by: python bench.py --summarize-pmc <fetch_dir> <write_dir> <this file>"
+             + ("" if cmd == PMC_CMD else " <title> <command>"),
              calib,
              "#",
              f"{'kernel':34s} {'disp':>5s} {'fetch_KiB':>12s} {'write_KiB':>12s} {'HBM_MB/disp':>12s}"]
@@ -648,7 +655,7 @@ def main():
         summarize_trace(*sys.argv[2:7])
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-pmc":
-        summarize_pmc(sys.argv[2], sys.argv[3], sys.argv[4])
+        summarize_pmc(*sys.argv[2:7])
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
