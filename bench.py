@@ -481,7 +481,8 @@ def learned_comparison(args, z, K, sync, plain_value):
                         "mfma_roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": BF16_PEAK_TFLOPS,
                                           "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 4),
                                           "kernel": "k_conv_tile (3x3 conv stack, forward)"}}
-    for name, mode in (("explicit_start", Lm.WARM_EXPLICIT), ("learned_start", Lm.WARM_LEARNED)):
+    for name, mode in (("explicit_start", Lm.WARM_EXPLICIT), ("previous_step_start", Lm.WARM_PREVIOUS),
+                       ("learned_start", Lm.WARM_LEARNED)):
         lem = lsdtt3_amd.LandscapeModel(z, 30.0, **lem_kwargs(K))
         lem.tile_height = getattr(args, "tile_height", 16)  # as the plain run
         Lm.set_warm_start(lem, mode, net if mode == Lm.WARM_LEARNED else None)

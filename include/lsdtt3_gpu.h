@@ -640,8 +640,11 @@ int lsdtt_gpu_cnn_evaluate(lsdtt_gpu_cnn* net, lsdtt_gpu_cnn_dataset* ds, int32_
                            uint64_t seed, double* loss, double* zero_loss);
 /* Warm start of each step's first Newton solve: mode 0 off (z^n), 1 the
  * explicit Euler predictor z^n + dt (-F(z^n)), 2 the network `net` (which
- * must outlive its use here).  The predicted start is kept only when it
- * lowers ||F||inf; the converged residual is judged as without it. */
+ * must outlive its use here), 3 the previous step's hillslope change
+ * (z^n + (z_hs - z_in) of the last step, used when that step's Newton
+ * solve converged over the same dt and the surface was not set since).
+ * The predicted start is kept only when it lowers ||F||inf; the converged
+ * residual is judged as without it. */
 int lsdtt_gpu_lem_set_warm_start(lsdtt_gpu_lem* lem, lsdtt_gpu_cnn* net, int32_t mode);
 
 /* Valid cells with an 8-neighbour off a fixed ('o'/'b') edge or masked in

@@ -284,10 +284,11 @@ int lsdtt_gpu_lem_set_warm_start(lsdtt_gpu_lem* lem, lsdtt_gpu_cnn* net, int32_t
 {
     return guarded([&] {
         if (!lem) return fail_arg("null argument");
-        if (mode < 0 || mode > 2) return fail_arg("warm start mode must be 0, 1 or 2");
+        if (mode < 0 || mode > 3) return fail_arg("warm start mode must be 0, 1, 2 or 3");
         if (mode == 2 && !net) return fail_arg("the learned warm start needs a network");
         if (mode == 2 && net->net->in_channels() != 4) return fail_arg("the warm-start network takes 4 input channels");
         lem->eng->ws_mode = mode;
+        lem->eng->ws_dz_valid = false;
         lem->eng->ws = mode == 2 ? net->predictor.get() : nullptr;
         return LSDTT_GPU_OK;
     });

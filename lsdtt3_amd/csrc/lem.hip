@@ -465,6 +465,7 @@ void LemEngine::set_channel_mask(const uint8_t* ch)
 void LemEngine::set_elevation(const double* zh)
 {
     LSDTT_HIP(hipMemcpy(z.p, zh, sizeof(double) * n, hipMemcpyHostToDevice));
+    ws_dz_valid = false;  // the previous step's hillslope change no longer applies
 }
 
 void LemEngine::get_elevation(double* zh)
@@ -687,13 +688,17 @@ int LemEngine::newton_solve(double inv_dt, LemStats* st, hipStream_t s)
     if (st->substeps == 0 && st->rejected == 0) st->hillslope_initial_residual = fnorm;
     st->hillslope_tolerance = tol;
     const double feasible = std::sqrt(p.slope_ratio_cap);
-    if (ws_mode != 0 && st->substeps == 0 && st->rejected == 0 && fnorm > tol) {
+    const bool ws_usable = ws_mode != 3 || (ws_dz_valid && ws_dz_dt == p.dt);
+    if (ws_mode != 0 && ws_usable && st->substeps == 0 && st->rejected == 0 && fnorm > tol) {
         // predicted start z^n + dz; plain Newton from z^n stays the fallback
         timer.begin("warm_start", s);
         if (ws_mode == 1) {
             // explicit Euler: b = -F(z^n) from the trial above
             k_scale_f<<<grid_for(n, 4096), kB, 0, s>>>(b.p, static_cast<float>(1.0 / inv_dt), x.p, n);
             LSDTT_LAUNCH_CHECK();
+        } else if (ws_mode == 3) {
+            // the previous step's hillslope change over the same dt
+            LSDTT_HIP(hipMemcpyAsync(x.p, ws_dz.p, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         } else {
             if (!ws) throw std::logic_error("learned warm start without a predictor");
             ws->predict(zc, mask.p, dof.p, nr, nc, dx, p.D / (inv_dt * dx * dx), p.Sc, x.p, s);
@@ -917,6 +922,17 @@ int LemEngine::step(LemStats* st, hipStream_t s)
         if (!p.force_picard) {
             done = hillslope(st, s) == 0;  // z is left post-incision when it fails
             if (done) zres = hs_result;
+        }
+        if (ws_mode == 3) {
+            // this step's hillslope change, the next step's predicted start
+            // (z still holds the post-incision surface)
+            ws_dz_valid = done;
+            if (done) {
+                ws_dz.reserve(static_cast<size_t>(n));
+                k_diff_f<<<g, kB, 0, s>>>(zres, z.p, ws_dz.p, n);
+                LSDTT_LAUNCH_CHECK();
+                ws_dz_dt = p.dt;
+            }
         }
         if (capture) {
             // the training pair keeps the post-incision surface (z) in z_hs0;

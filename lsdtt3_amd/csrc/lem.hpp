@@ -170,9 +170,14 @@ public:
     hipStream_t aux_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // warm start of the first Newton solve of a step: 0 off, 1 explicit
-    // Euler (dz = dt (-F(z^n))), 2 the learned predictor `ws`
+    // Euler (dz = dt (-F(z^n))), 2 the learned predictor `ws`, 3 the
+    // previous step's hillslope change (ws_dz, kept when that step's solve
+    // converged with the same dt and the surface was not set since)
     int ws_mode = 0;
     WarmStartPredictor* ws = nullptr;
+    dbuf<float> ws_dz;
+    bool ws_dz_valid = false;
+    double ws_dz_dt = 0.0;
     // training capture: after each step, cap_dz = hillslope change of z
     // (post-hillslope minus post-incision, FP32) and cap_kappa = dt D / h^2;
     // z_hs0 holds the post-incision surface

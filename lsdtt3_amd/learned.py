@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _native as N
 
-WARM_NONE, WARM_EXPLICIT, WARM_LEARNED = 0, 1, 2
+WARM_NONE, WARM_EXPLICIT, WARM_LEARNED, WARM_PREVIOUS = 0, 1, 2, 3
 
 
 class UNet:
@@ -154,8 +154,9 @@ def capture(model, enable=True) -> None:
 
 
 def set_warm_start(model, mode, net=None) -> None:
-    """mode WARM_NONE (z^n), WARM_EXPLICIT (z^n - dt F(z^n)) or WARM_LEARNED
-    (the network `net`, kept alive by the model while set)."""
+    """mode WARM_NONE (z^n), WARM_EXPLICIT (z^n - dt F(z^n)), WARM_LEARNED
+    (the network `net`, kept alive by the model while set) or WARM_PREVIOUS
+    (z^n plus the previous step's hillslope change)."""
     N.check(N.lib().lsdtt_gpu_lem_set_warm_start(model._h, net._h if net is not None else None, int(mode)))
     model._warm_net = net if mode == WARM_LEARNED else None
 

@@ -177,6 +177,32 @@ def test_warm_start_explicit_keeps_the_solution(learned):
     assert np.abs(a.elevation - b.elevation).max() <= max(tol, 1e-8)
 
 
+def test_warm_start_previous_step_keeps_the_solution(learned):
+    """the previous step's hillslope change as the start: not used on the
+    first step (nothing to repeat) nor after the surface is set, accepted
+    later with a lower starting residual, and the steps still converge to the
+    same tolerance, to the same surface (each within 4 dt tol of the root,
+    accumulated over the steps)"""
+    import lsdtt3_amd
+    from lsdtt3_amd.synthetic import lem_surface
+    z = lem_surface(128, 160, dx=30.0, seed=22, max_slope=0.6)
+    a, b = _lem(lsdtt3_amd, z), _lem(lsdtt3_amd, z)
+    learned.set_warm_start(b, learned.WARM_PREVIOUS)
+    bound = 0.0
+    for k in range(6):
+        sa, sb = a.step(), b.step()
+        assert sb["hillslope_converged"] == 1 and sb["hillslope_residual"] <= sb["hillslope_tolerance"]
+        if k == 0:
+            assert sb["warm_start_mode"] == 0      # no previous step yet
+        else:
+            assert sb["warm_start_mode"] == 3 and sb["warm_start_accepted"] == 1
+            assert sb["warm_start_residual"] < sb["hillslope_initial_residual"]
+        bound += 4 * 500.0 * max(sa["hillslope_tolerance"], sb["hillslope_tolerance"])
+    assert np.abs(a.elevation - b.elevation).max() <= max(bound, 1e-8)
+    b.elevation = b.elevation      # a new surface: the next step starts plainly again
+    assert b.step()["warm_start_mode"] == 0
+
+
 def test_warm_start_learned_trained_on_lem_pairs(learned):
     """train the network on pairs captured from the LEM's own steps and
     show what it gains: its start lowers ||F||inf by a large factor over
