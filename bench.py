@@ -28,7 +28,9 @@ training figures and its MFMA utilisation at the 4096^2 inference shape
 three.  The line also carries batched: the 64-instance ensemble (cells
 log-uniform 256^2..2048^2, per-instance K, m/n, D, U) on the batched engine
 -- one launch per kernel for all instances, the step replayed from a
-captured hipGraph -- with its throughput relative to the plain step.
+captured hipGraph -- with its throughput relative to the plain step, and
+the same ensemble run again with each instance's Newton solve started from
+its previous step's hillslope change (previous_step_start).
 The relative residual r reported is ||F||inf over ||F(z^n)||inf of the
 sub-step the Newton stop judged (the stop is r <= 1e-8).
 The timed steps run from the initial surface (warm-up and tile autotuning
@@ -347,40 +349,78 @@ def ensemble_member(args):
 
 def ensemble_measure(args):
     """the 64-instance ensemble on the batched engine, tiles autotuned, graph
-    replay, --ensemble-steps timed steps; returns the measurement dict"""
+    replay, --ensemble-steps timed steps from the plain start z^n and again
+    with each instance's previous hillslope change as the start (same
+    surfaces, same tile height); returns the measurement dict"""
     import lsdtt3_amd
+    from lsdtt3_amd.learned import WARM_PREVIOUS
     from concurrent.futures import ProcessPoolExecutor
     t0 = time.perf_counter()
     with ProcessPoolExecutor(max_workers=8) as ex:
         ms = list(ex.map(ensemble_member, [(i, 2024) for i in range(args.instances)]))
     gen_s = time.perf_counter() - t0
     cells = [m["dem"].size for m in ms]
+
+    def timed(ens, tune):
+        ens.run(max(1, args.warmup))
+        tuning = None
+        if tune and args.autotune_steps > 0:
+            th, tms = ens.autotune_tiles(args.autotune_steps)
+            tuning = {"tile_height": th, "ms_per_step": {str(h): round(v, 3) for h, v in tms.items()},
+                      "steps_per_trial": args.autotune_steps}
+        ens.run(1)  # the pattern of the chosen height (tuning restores the surfaces)
+        stats, sec = ens.run(args.ensemble_steps)
+        # relative residual as the Newton stop measures it: ||F|| over ||F(z^n)|| of
+        # the sub-step start (tolerance = rel_tol * ||F(z^n)||)
+        worst = max(s["hillslope_residual"] / max(s["hillslope_tolerance"] / 1e-8, 1e-300) for s in stats)
+        return stats, sec, worst, tuning
+
     ens = lsdtt3_amd.Ensemble(ms)
-    ens.run(max(1, args.warmup))
     total = sum(ens.n_valid)
-    tuning = None
-    if args.autotune_steps > 0:
-        th, tms = ens.autotune_tiles(args.autotune_steps)
-        tuning = {"tile_height": th, "ms_per_step": {str(h): round(v, 3) for h, v in tms.items()},
-                  "steps_per_trial": args.autotune_steps}
-        ens.run(1)  # the pattern of the chosen height
-    stats, sec = ens.run(args.ensemble_steps)
+    stats, sec, worst, tuning = timed(ens, True)
     value = total * args.ensemble_steps / sec
-    # relative residual as the Newton stop measures it: ||F|| over ||F(z^n)|| of the
-    # sub-step start (tolerance = rel_tol * ||F(z^n)||)
-    worst = max(s["hillslope_residual"] / max(s["hillslope_tolerance"] / 1e-8, 1e-300) for s in stats)
     pat = ens.pattern()
-    return {"value": round(value, 1), "unit": "cell-steps/s", "ms_per_step": round(1e3 * sec / args.ensemble_steps, 3),
-            "steps": args.ensemble_steps,
-            "workload": f"batched ensemble of {len(ms)} LEM instances, cells log-uniform 256^2..2048^2 "
-                        f"(total {total}), per-instance K, m/n, D, U; Roering Newton r<=1e-8, stream power n=1; "
-                        f"one launch per kernel for all instances, step replayed from a captured hipGraph",
-            "instances": len(ms), "total_cells": total, "min_cells": min(cells), "max_cells": max(cells),
-            "worst_relative_residual": worst,
-            "newton_iters_per_step_max": max(s["newton_iterations"] for s in stats),
-            "krylov_iters_per_step_max": max(s["krylov_iterations"] for s in stats),
-            "picard_fallbacks": sum(1 for s in stats if s["hillslope_method"] == 1),
-            "launch_pattern": pat, "autotuned_tiles": tuning, "generate_s": round(gen_s, 2)}
+    out = {"value": round(value, 1), "unit": "cell-steps/s", "ms_per_step": round(1e3 * sec / args.ensemble_steps, 3),
+           "steps": args.ensemble_steps,
+           "workload": f"batched ensemble of {len(ms)} LEM instances, cells log-uniform 256^2..2048^2 "
+                       f"(total {total}), per-instance K, m/n, D, U; Roering Newton r<=1e-8, stream power n=1; "
+                       f"one launch per kernel for all instances, step replayed from a captured hipGraph",
+           "instances": len(ms), "total_cells": total, "min_cells": min(cells), "max_cells": max(cells),
+           "worst_relative_residual": worst,
+           "newton_iters_per_step_max": max(s["newton_iterations"] for s in stats),
+           "krylov_iters_per_step_max": max(s["krylov_iterations"] for s in stats),
+           "picard_fallbacks": sum(1 for s in stats if s["hillslope_method"] == 1),
+           "launch_pattern": pat, "autotuned_tiles": tuning, "generate_s": round(gen_s, 2)}
+    if args.no_ensemble_warm_start:
+        return out
+    # the same run from each instance's previous hillslope change
+    plain = [ens.elevation(i) for i in range(len(ms))]
+    th = ens.tile_height
+    del ens
+    ew = lsdtt3_amd.Ensemble(ms)
+    ew.tile_height = th
+    ew.set_warm_start(WARM_PREVIOUS)
+    wstats, wsec, wworst, _ = timed(ew, False)
+    # the two trajectories agree step by step to the Newton tolerance; over
+    # hundreds of steps a tolerance-sized difference can flip the D8 receiver
+    # of a near-tied cell and move its channel, as any other valid solve of
+    # the same tolerance does (another Krylov forcing, another thread count)
+    diffs = [np.abs(ew.elevation(i) - plain[i]) for i in range(len(ms))]
+    diff = max(float(d.max()) for d in diffs)
+    moved = sum(int((d > 1e-3).sum()) for d in diffs)
+    moved_inst = sum(1 for d in diffs if d.max() > 1e-3)
+    del diffs
+    wvalue = total * args.ensemble_steps / wsec
+    out["previous_step_start"] = {
+        "value": round(wvalue, 1), "ms_per_step": round(1e3 * wsec / args.ensemble_steps, 3),
+        "vs_plain_start": round(wvalue / value, 3), "worst_relative_residual": wworst,
+        "accepted_last_step": sum(s["warm_start_accepted"] for s in wstats),
+        "newton_iters_per_step_max": max(s["newton_iterations"] for s in wstats),
+        "krylov_iters_per_step_max": max(s["krylov_iterations"] for s in wstats),
+        "picard_fallbacks": sum(1 for s in wstats if s["hillslope_method"] == 1),
+        "max_abs_diff_vs_plain_m": diff, "cells_over_1mm_vs_plain": moved, "instances_over_1mm_vs_plain": moved_inst,
+        "launch_pattern": ew.pattern()}
+    return out
 
 
 def bench_ensemble(args):
@@ -677,6 +717,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--ensemble-steps", type=int, default=300, help="batched ensemble: timed steps")
+    ap.add_argument("--no-ensemble-warm-start", action="store_true",
+                    help="batched ensemble: skip the run from the previous hillslope change")
     ap.add_argument("--no-unsmoothed", action="store_true",
                     help="skip the raw-surface (linear, n = 1) measurement at the same size")
     ap.add_argument("--profile-steps", type=int, default=10,

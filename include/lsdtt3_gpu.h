@@ -529,6 +529,16 @@ int lsdtt_gpu_ensemble_run(lsdtt_gpu_ensemble* e, int32_t n_steps, lsdtt_gpu_lem
  * pattern; 0: the same launches are enqueued one by one.  Results are the
  * same either way. */
 int lsdtt_gpu_ensemble_set_graph(lsdtt_gpu_ensemble* e, int32_t enable);
+/* Warm start of every instance's first Newton solve of a step (no
+ * counterpart in the reference): mode 3 tries z^n plus the instance's
+ * previous step's hillslope change as the first iterate, kept when its
+ * residual is below the residual at z^n (and, nonlinear, its slope ratio
+ * feasible); mode 0 (the default) starts from z^n.  The change is recorded
+ * only by steps whose solve converged by Newton and is forgotten by
+ * _set_elevation.  Results agree with mode 0 to the Newton tolerance;
+ * the stats' warm_start_* fields report each instance's trial.  Needs
+ * 4 bytes per cell more device memory. */
+int lsdtt_gpu_ensemble_set_warm_start(lsdtt_gpu_ensemble* e, int32_t mode);
 /* Launch-pattern counters: graph replays and captures, solver rounds and
  * tile passes launched outside the pattern (an instance that needed more
  * than its caps), steps restarted because a sweep did not settle within

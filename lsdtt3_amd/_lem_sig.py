@@ -86,6 +86,7 @@ def declare(L, sig):
     sig(L, "lsdtt_gpu_ensemble_run", C.c_int, [vp, C.c_int32, vp, vp])
     sig(L, "lsdtt_gpu_ensemble_get_erosion_rate", C.c_int, [vp, i32, vp])
     sig(L, "lsdtt_gpu_ensemble_set_graph", C.c_int, [vp, i32])
+    sig(L, "lsdtt_gpu_ensemble_set_warm_start", C.c_int, [vp, i32])
     sig(L, "lsdtt_gpu_ensemble_pattern", C.c_int, [vp, vp, vp, vp, vp, vp, vp])
     sig(L, "lsdtt_gpu_ensemble_set_tile_height", C.c_int, [vp, i32])
     sig(L, "lsdtt_gpu_ensemble_tile_height", i32, [vp])

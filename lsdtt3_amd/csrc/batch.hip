@@ -209,7 +209,8 @@ __global__ void __launch_bounds__(256) kb_stats(const BatchHsGrid* __restrict__ 
                                                 const uint8_t* __restrict__ cflag, const double* __restrict__ zbefore,
                                                 const double* zb0, const double* zb1, const double* zb2,
                                                 const double* zb3, double* z, float* __restrict__ absdz,
-                                                float* __restrict__ ero_rate, double* spartials)
+                                                float* __restrict__ ero_rate, float* __restrict__ ws_dz,
+                                                double* spartials)
 {
     const StTile T = st_tile(grids, st_grid, blockIdx.x);
     const HsCtl& c = ctl[T.gi];
@@ -222,6 +223,8 @@ __global__ void __launch_bounds__(256) kb_stats(const BatchHsGrid* __restrict__ 
     const bool moved = c.zc != 0;
     const double udz = q.udz, inv_dt = q.inv_dt;
     float* er = q.erosion_field ? ero_rate + o : nullptr;
+    // warm start of the next step: the hillslope change (result - input)
+    float* wd = g.ws_mode == 3 && c.method == 0 ? ws_dz + o : nullptr;
     double acc[8] = {0.0, 0.0, -1e30, -1e30, 0.0, 0.0, 0.0, 0.0};
     const int col = T.c0 + (threadIdx.x & 63);
     for (int k = 0; k < kTH / 4; ++k) {
@@ -231,6 +234,7 @@ __global__ void __launch_bounds__(256) kb_stats(const BatchHsGrid* __restrict__ 
         const uint8_t m = cflag[o + f];
         const double bb = zbefore[o + f];
         double a = zsrc[f];
+        if (wd) wd[f] = moved ? static_cast<float>(a - za[f]) : 0.0f;
         if ((m & 2) && udz != 0.0) {
             a += udz;
             za[f] = a;
@@ -266,7 +270,8 @@ __global__ void __launch_bounds__(256) kb_stats(const BatchHsGrid* __restrict__ 
 __global__ void __launch_bounds__(256) kb_stats_fin(const BatchHsGrid* __restrict__ grids,
                                                     const BatchStepGrid* __restrict__ sg, HsCtl* ctl,
                                                     const double* spartials, const int* __restrict__ eroded,
-                                                    double* out, long long* eroded_out, unsigned* sel)
+                                                    double* out, long long* eroded_out, unsigned* sel,
+                                                    int* __restrict__ ws_ok)
 {
     const int gi = blockIdx.x;
     HsCtl& c = ctl[gi];
@@ -301,6 +306,7 @@ __global__ void __launch_bounds__(256) kb_stats_fin(const BatchHsGrid* __restric
         sel[gi * 4 + 1] = 0u;
         sel[gi * 4 + 2] = sg[gi].p999_k;
         sel[gi * 4 + 3] = w[7] > 0.0 ? 1u : 0u;
+        ws_ok[gi] = g.ws_mode == 3 && c.phase == HS_DONE && c.method == 0 ? 1 : 0;
         c.stats_done = 1;
     }
 }
@@ -599,6 +605,8 @@ void BatchEngine::build_layout(const std::vector<BatchSpec>& specs)
     spartials.reserve(static_cast<size_t>(NT) * 8);
     stats_out.reserve(static_cast<size_t>(G) * 8);
     eroded_out.reserve(G);
+    ws_ok.reserve(G);
+    LSDTT_HIP(hipMemset(ws_ok.p, 0, sizeof(int) * G));
     hist.reserve(static_cast<size_t>(G) * kBins);
     LSDTT_HIP(hipMemset(hist.p, 0, sizeof(unsigned) * G * kBins));
     sel.reserve(static_cast<size_t>(G) * 4);
@@ -692,6 +700,8 @@ BatchHs BatchEngine::bhs()
     B.shat = shat.p;
     B.t = t.p;
     B.partials = partials.p;
+    B.ws_dz = ws_dz.p;
+    B.ws_ok = ws_ok.p;
     return B;
 }
 
@@ -701,6 +711,22 @@ void BatchEngine::set_elevation(int i, const double* zh)
     const BatchHsGrid& h = hsgrids_h[static_cast<size_t>(i)];
     LSDTT_HIP(hipDeviceSynchronize());
     LSDTT_HIP(hipMemcpy(z.p + h.off, zh, sizeof(double) * cells(i), hipMemcpyHostToDevice));
+    LSDTT_HIP(hipMemset(ws_ok.p + i, 0, sizeof(int)));  // no previous step for this surface
+}
+
+void BatchEngine::set_warm_start(int mode)
+{
+    if (mode != 0 && mode != 3) throw std::invalid_argument("ensemble warm start mode must be 0 or 3");
+    LSDTT_HIP(hipDeviceSynchronize());
+    if (mode == 3 && !ws_dz.p) {
+        ws_dz.reserve(static_cast<size_t>(N));
+        LSDTT_HIP(hipMemset(ws_dz.p, 0, sizeof(float) * N));
+    }
+    for (auto& h : hsgrids_h) h.ws_mode = mode;
+    LSDTT_HIP(hipMemcpy(hsgrids_d.p, hsgrids_h.data(), sizeof(BatchHsGrid) * G, hipMemcpyHostToDevice));
+    LSDTT_HIP(hipMemset(ws_ok.p, 0, sizeof(int) * G));
+    ws_mode = mode;
+    release_graph();  // the pattern holds the change buffer's address
 }
 
 void BatchEngine::get_elevation(int i, double* zh)
@@ -813,9 +839,9 @@ void BatchEngine::enqueue_stats(hipStream_t s)
                                                       z.p, zA.p, zB.p, zC.p);
     }
     kb_stats<<<NT, kB, 0, s>>>(hsgrids_d.p, st_grid_d.p, step_grids.p, ctl.p, cflag.p, z_before.p, z.p, zA.p, zB.p,
-                               zC.p, z.p, absdz.p, ero_rate.p, spartials.p);
+                               zC.p, z.p, absdz.p, ero_rate.p, ws_dz.p, spartials.p);
     kb_stats_fin<<<G, kB, 0, s>>>(hsgrids_d.p, step_grids.p, ctl.p, spartials.p, eroded.p, stats_out.p, eroded_out.p,
-                                  sel.p);
+                                  sel.p, ws_ok.p);
     const int shifts[3] = {21, 10, 0}, bits[3] = {11, 11, 10};
     for (int pass = 0; pass < 3; ++pass) {
         kb_radix_hist<<<NT, kB, 0, s>>>(hsgrids_d.p, st_grid_d.p, absdz.p, sel.p, shifts[pass], bits[pass], hist.p);
@@ -855,7 +881,7 @@ int settled_at(const int* flags, int launched)
         if (!flags[q]) return q;
     return -1;
 }
-bool solving(int ph) { return ph == HS_START || ph == HS_KRYLOV || ph == HS_SEARCH; }
+bool solving(int ph) { return ph == HS_START || ph == HS_KRYLOV || ph == HS_SEARCH || ph == HS_WARM; }
 }  // namespace
 
 int BatchEngine::step(std::vector<LemStats>& st)
@@ -899,6 +925,9 @@ int BatchEngine::step(std::vector<LemStats>& st)
         ++counters.refills;
         kb_copy<<<std::min(blocks_for(N), 8192), kB, 0, s>>>(z_before.p, z.p, N);
         LSDTT_LAUNCH_CHECK();
+        // the abandoned pass recorded its own hillslope changes: the
+        // restarted step starts every solve from z^n
+        LSDTT_HIP(hipMemsetAsync(ws_ok.p, 0, sizeof(int) * G, s));
         kb_receivers<<<4 * NT, kB, 0, s>>>(hsgrids_d.p, st_grid_d.p, z.p, mask.p, rcv.p, dir.p, z_before.p);
         batch_tile_order(th, btiles(), RT, rcv.p, mask.p, ord.p, lstart.p, maxdep.p, xdon.p, s);
         int launched = 0;
@@ -1016,6 +1045,9 @@ void BatchEngine::finish_step(std::vector<LemStats>& st, hipStream_t s, int* pic
             o.p999_local_elevation_change = pvv;
         }
         o.cells_eroded = p.enable_fluvial ? eroded_h[static_cast<size_t>(i)] : 0;
+        o.warm_start_mode = c.ws_tried ? ws_mode : 0;
+        o.warm_start_accepted = c.ws_accepted;
+        o.warm_start_residual = c.ws_f;
         o.count_passes = cp;
         o.stream_power_passes = spp;
         if (p.enable_hillslope) {
@@ -1079,6 +1111,8 @@ int BatchEngine::autotune_tiles(int steps, double* ms_per_step)
         throw;
     }
     set_tile_height(best);
+    // the trials' hillslope changes belong to the discarded surfaces
+    LSDTT_HIP(hipMemsetAsync(ws_ok.p, 0, sizeof(int) * G, s));
     LSDTT_HIP(hipStreamSynchronize(s));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -74,6 +74,12 @@ public:
     // (they took the damped Picard fallback, as run_lem_step does).
     int step(std::vector<LemStats>& st);
 
+    // warm start of each grid's first Newton solve: 0 off, 3 the grid's
+    // previous step's hillslope change (kept when that solve converged by
+    // Newton; set_elevation, a restarted step and autotune_tiles forget it)
+    void set_warm_start(int mode);
+    int warm_start() const { return ws_mode; }
+
     // graph replay of the step pattern (default on); off: the same launches
     // enqueued directly
     void set_graph(bool on);
@@ -117,12 +123,15 @@ private:
     long long N = 0;  // cells of the concatenated arrays
     unsigned sp_modes = 0;
     bool any_hillslope = false;
+    int ws_mode = 0;
 
     // per-cell arrays (concatenated)
     dbuf<double> z, z_before, zA, zB, zC;
     dbuf<uint8_t> mask, dof, fast, cflag, dir, xdon;
     dbuf<int> rcv, cnt, gen;
     dbuf<float> c9g, b, dinv, x, r, pv, phat, v, sv, shat, t, absdz, ero_rate;
+    dbuf<float> ws_dz;  // warm start: the last step's hillslope change (allocated with the mode)
+    dbuf<int> ws_ok;    // per grid: ws_dz is valid
     dbuf<BatchStepGrid> step_grids;
     // per routing tile
     dbuf<short> ord, lstart;

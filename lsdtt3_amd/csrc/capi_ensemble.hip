@@ -138,6 +138,16 @@ int lsdtt_gpu_ensemble_set_graph(lsdtt_gpu_ensemble* e, int32_t enable)
     });
 }
 
+int lsdtt_gpu_ensemble_set_warm_start(lsdtt_gpu_ensemble* e, int32_t mode)
+{
+    return guarded([&] {
+        if (!e) return fail_arg("null ensemble");
+        if (mode != 0 && mode != 3) return fail_arg("ensemble warm start mode must be 0 or 3");
+        e->eng->set_warm_start(mode);
+        return LSDTT_GPU_OK;
+    });
+}
+
 int lsdtt_gpu_ensemble_pattern(lsdtt_gpu_ensemble* e, int64_t* graph_launches, int64_t* graph_captures,
                                int64_t* extra_rounds, int64_t* extra_passes, int64_t* restarts,
                                int32_t* caps)

@@ -798,7 +798,7 @@ __device__ __forceinline__ HsDom batch_dom(const BatchHs& B, const BatchHsGrid& 
     return d;
 }
 
-__device__ __forceinline__ bool trial_phase(int ph) { return ph == HS_START || ph == HS_SEARCH; }
+__device__ __forceinline__ bool trial_phase(int ph) { return ph == HS_START || ph == HS_SEARCH || ph == HS_WARM; }
 
 // this block's stencil tile: grid and corner
 // a batched work item is a 64 x 64 tile, swept as kBands bands of the
@@ -854,7 +854,7 @@ __device__ __forceinline__ void trial_tile(const BatchHs& B, int slot)
     const HsDom d = batch_dom(B, g, c.inv_dt, c.zn);
     const double* z = B.zbuf[c.zc] + g.off;
     double* z_out = B.zbuf[c.zt] + g.off;
-    const float* dz = B.x + g.off;
+    const float* dz = (ph == HS_WARM ? B.ws_dz : B.x) + g.off;
     float* b = B.b + g.off;
     float* dinv = B.dinv + g.off;
     __shared__ double zs[kSH * kSW];
@@ -934,7 +934,7 @@ __device__ __forceinline__ void trial_chunk(const BatchHs& B, int slot)
     const HsDom d = batch_dom(B, g, c.inv_dt, c.zn);
     const double* z = B.zbuf[c.zc] + g.off;
     double* z_out = B.zbuf[c.zt] + g.off;
-    const ZView Z{z, has_dz ? B.x + g.off : nullptr, lam};
+    const ZView Z{z, has_dz ? (ph == HS_WARM ? B.ws_dz : B.x) + g.off : nullptr, lam};
     float* b = B.b + g.off;
     float* dinv = B.dinv + g.off;
     double acc[3] = {0.0, 0.0, 0.0};
@@ -1336,7 +1336,7 @@ __device__ __forceinline__ void control_one(const BatchHs& B, int gi, int stage)
     const BatchHsGrid& g = B.grids[gi];
     const double feasible = sqrt(g.cap);
     if (stage == 0) {
-        if (c.phase == HS_START || c.phase == HS_KRYLOV || c.phase == HS_SEARCH) ++c.rounds;
+        if (c.phase == HS_START || c.phase == HS_KRYLOV || c.phase == HS_SEARCH || c.phase == HS_WARM) ++c.rounds;
         if (c.phase == HS_KRYLOV && c.kdone) {
             c.krylov_total += c.kits;
             // no iteration ran (b = 0): the correction is zero.  (Each trial
@@ -1361,9 +1361,37 @@ __device__ __forceinline__ void control_one(const BatchHs& B, int gi, int stage)
         }
         c.it = 0;
         c.eta = sqrt(g.eta2);
-        if (c.fnorm <= c.tol) end_solve(c, 0, true);
-        else if (g.newton_max <= 0) end_solve(c, 0, false);
-        else c.phase = HS_KRYLOV;
+        if (c.fnorm <= c.tol) {
+            end_solve(c, 0, true);
+        } else if (g.newton_max <= 0) {
+            end_solve(c, 0, false);
+        } else if (g.ws_mode == 3 && !c.ws_tried && c.substeps == 0 && c.rejected == 0 && B.ws_ok[gi]) {
+            // first try z^n + the last step's hillslope change (a trial at
+            // lambda 1 of that change); plain Newton from z^n is the fallback
+            c.ws_tried = 1;
+            c.lambda = 1.0;
+            c.zero_step = 0;
+            c.phase = HS_WARM;
+        } else {
+            c.phase = HS_KRYLOV;
+        }
+        return;
+    }
+    if (c.phase == HS_WARM) {
+        const double ft = c.trial_f, rt = c.trial_ratio;
+        c.ws_f = ft;
+        if (ft < c.fnorm && (!g.nonlinear || rt <= feasible)) {
+            // the predicted start is the first iterate; its system came with the trial
+            c.zc = c.zt;
+            c.zt = free_buffer(c.zc, c.zn);
+            c.fnorm = ft;
+            c.ratio = rt;
+            c.ws_accepted = 1;
+            if (c.fnorm <= c.tol) end_solve(c, 0, true);
+            else c.phase = HS_KRYLOV;
+        } else {
+            c.phase = HS_START;  // the system at z^n again
+        }
         return;
     }
     if (c.phase != HS_SEARCH) return;

@@ -89,6 +89,7 @@ enum HsPhase : int {
     HS_SEARCH = 3,  // line-search trial at ctl.lambda
     HS_DONE = 4,    // converged over the whole dt; result in buffer zc
     HS_FAILED = 5,  // Newton failed at the minimum sub-step (Picard fallback)
+    HS_WARM = 6,    // trial of the predicted start z^n + (last step's hillslope change)
 };
 
 struct HsCtl {
@@ -107,6 +108,8 @@ struct HsCtl {
     double trial_f, trial_ratio;
     double rho, rho_old, alpha, omega, bnorm2, rnorm2;
     double eta;  // relative tolerance of the current linear solve
+    int ws_tried, ws_accepted;  // this step's warm-start trial ran / was kept
+    double ws_f;                // ||F||inf at the predicted start
 };
 
 // Relative tolerance of the next Newton iteration's linear solve after one
@@ -143,6 +146,7 @@ struct BatchHsGrid {
     int gen_off, n_gen, gcbase, ngc;
     double h, D, Sc, cap;
     int nonlinear;
+    int ws_mode;       // 3: start each step from the last step's hillslope change
     double eta2;       // Krylov relative tolerance squared (the first Newton iteration's)
     int forcing;       // 1: later iterations by forcing_eta, 0: eta2 throughout
     int krylov_max;
@@ -172,6 +176,8 @@ struct BatchHs {
     double* zbuf[4];
     float *b, *dinv, *x, *r, *p, *phat, *v, *s, *shat, *t;
     double* partials;  // (NT + NGC) slots of up to 3 values
+    const float* ws_dz;  // per cell: the last step's hillslope change (warm start)
+    const int* ws_ok;    // per grid: ws_dz holds a converged step's change
 };
 
 // Batched launches over every grid of B (n_grids of them).

@@ -364,6 +364,11 @@ class Ensemble:
         """graph replay of the step's launch pattern (default on)"""
         N.check(N.lib().lsdtt_gpu_ensemble_set_graph(self._h, int(bool(enable))))
 
+    def set_warm_start(self, mode: int) -> None:
+        """warm start of each instance's first Newton solve: 0 off, 3 the
+        instance's previous step's hillslope change (WARM_PREVIOUS)"""
+        N.check(N.lib().lsdtt_gpu_ensemble_set_warm_start(self._h, int(mode)))
+
     def pattern(self) -> dict:
         """launch-pattern counters: graph replays and captures, rounds and
         passes launched outside the pattern, restarted steps, current caps"""
@@ -394,6 +399,12 @@ class Ensemble:
         z = np.empty(self.shapes[i], np.float64)
         N.check(N.lib().lsdtt_gpu_ensemble_get_elevation(self._h, int(i), N.ptr(z)))
         return z
+
+    def set_elevation(self, i, z):
+        z = np.ascontiguousarray(z, np.float64)
+        if z.shape != tuple(self.shapes[i]):
+            raise ValueError(f"instance {i} is {self.shapes[i]}, got {z.shape}")
+        N.check(N.lib().lsdtt_gpu_ensemble_set_elevation(self._h, int(i), N.ptr(z)))
 
     def erosion_rate(self, i):
         out = np.empty(self.shapes[i], np.float32)

@@ -15,7 +15,11 @@ What is checked, and to which bar:
     solves to its own residual tolerance; the Krylov iterates differ by
     rounding);
   * the Picard fallback of a failing Newton solve against the oracle
-    (1e-7 m, as test_gpu_lem.py)."""
+    (1e-7 m, as test_gpu_lem.py);
+  * the warm start from each instance's previous hillslope change: tried
+    from the second step on, forgotten when a surface is set, graph replay
+    and direct launches still bit-identical, and the surfaces within the
+    Newton tolerance of the plain start's (4 dt tol per step, accumulated)."""
 import numpy as np
 import pytest
 
@@ -240,3 +244,39 @@ def test_gpu_ensemble_damped_line_search_matches_single_engine(gpu):
         tol = max(4 * 500.0 * ss["hillslope_tolerance"], 1e-8)
         assert np.abs(ens.elevation(0) - lem.elevation).max() <= 2 * tol, step
         lem.elevation = ens.elevation(0)  # keep both on one trajectory
+
+
+def test_gpu_ensemble_warm_start_previous_step(gpu):
+    print("[gpu ensemble] warm start from the previous hillslope change, 7 instances x 6 steps", flush=True)
+    ms = models()
+    a, b, d = gpu.Ensemble(ms), gpu.Ensemble(ms), gpu.Ensemble(ms)
+    b.set_warm_start(3)
+    d.set_warm_start(3)
+    d.set_graph(False)
+    bound = [0.0] * len(ms)
+    accepted = 0
+    for k in range(6):
+        sa, _ = a.run(1)
+        sb, _ = b.run(1)
+        sd, _ = d.run(1)
+        for i in range(len(ms)):
+            assert sb[i] == sd[i], (k, i)
+            assert np.array_equal(b.elevation(i), d.elevation(i)), (k, i)
+            s = sb[i]
+            assert s["hillslope_converged"] == 1 and s["hillslope_residual"] <= s["hillslope_tolerance"], (k, i)
+            assert s["warm_start_mode"] == (0 if k == 0 else 3), (k, i)
+            if s["warm_start_accepted"]:
+                accepted += 1
+                assert s["warm_start_residual"] < s["hillslope_initial_residual"], (k, i)
+            bound[i] += 4 * 500.0 * max(sa[i]["hillslope_tolerance"], s["hillslope_tolerance"])
+    assert accepted >= 3 * len(ms)  # most instances keep the predicted start
+    for i in range(len(ms)):
+        assert np.abs(a.elevation(i) - b.elevation(i)).max() <= max(bound[i], 1e-8), i
+    # a new surface forgets its instance's change only
+    b.set_elevation(0, b.elevation(0))
+    sb, _ = b.run(1)
+    assert sb[0]["warm_start_mode"] == 0
+    assert all(s["warm_start_mode"] == 3 for s in sb[1:])
+    from lsdtt3_amd._native import GpuError
+    with pytest.raises(GpuError):
+        b.set_warm_start(1)
