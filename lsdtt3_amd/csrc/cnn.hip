@@ -95,101 +95,179 @@ __global__ void __launch_bounds__(kB) k_conv(const __bf16* __restrict__ X, const
 }
 
 // ---------------------------------------------------------------------------
-// 3x3 convolution for cin a multiple of 32: a workgroup computes a 16 x 16
-// pixel block of one image for 16 NCG output channels.  Per 32-channel chunk
-// of the input it stages the block plus a one-pixel halo (18 x 18 pixels x
-// 32 channels, pixel stride 40 elements so that 16-byte reads of consecutive
-// pixels fall in distinct LDS banks) and then, for each of the 9 taps, forms
-// the A operands of all 16 x 16 pixels from LDS (shifted reads, no im2col
-// buffer) while the B operands (weights of that tap and chunk) come from
-// global memory, shared by the wavefronts through L1.  Wavefront w owns
-// block rows 4w..4w+3: 4 pixel groups x NCG channel groups of MFMA tiles per
-// tap, so each A fragment feeds NCG MFMAs and each B fragment four.
+// 3x3 convolution for cin and cout multiples of 32: a workgroup computes a
+// block of RB rows x 16 pixels of one image for 32 WC output channels.  Its
+// four wavefronts are WR = 4 / WC along the rows (R output rows each) by WC
+// along the channels (32 each, two 16-channel MFMA groups).  Per 32-channel
+// chunk of the input the block plus a one-pixel halo ((RB + 2) x 18 pixels x
+// 32 channels, pixel stride 40 elements so that 16-byte reads of 16
+// consecutive pixels fall in distinct LDS banks) is staged in LDS; the next
+// chunk is loaded into registers while this one is used, and the chunk's
+// weights (9 taps x 2 groups of B operands) are held in registers.
+//
+// Operand reuse is what the MFMA rate needs: for one tap column dx, the A
+// fragment of staged row s (16 pixels x 32 channels, one 16-byte LDS read
+// per lane) serves output rows s, s - 1 and s - 2 through taps dy = 0, 1, 2,
+// and each of the two channel groups: 3 (R + 2) LDS reads per chunk feed
+// 18 R MFMAs, and each B operand, loaded once per chunk, feeds R rows (8;
+// 6 or 4 for 32-channel blocks, whose taller staged block needs more
+// prefetch registers, 4 where all nine taps' weights stay in registers).  The B operands are held one tap column at a time (the next
+// column's load in flight), which keeps the registers within two
+// wavefronts per SIMD.
 //
 // The input may be the decoder's concatenation [upsample2(U), X]: channels
 // below cs come from U (cs channels at half resolution, pixel (y/2, x/2)),
 // the rest from X (cin - cs channels); cs is a multiple of 32 (0: X only).
 // ---------------------------------------------------------------------------
-constexpr int kTile = 16;                // output block side (pixels)
-constexpr int kHalo = kTile + 2;         // staged side
+constexpr int kTile = 16;                // output block width (pixels): one MFMA's rows
+constexpr int kHalo = kTile + 2;         // staged width
 constexpr int kPixStride = 40;           // LDS elements per staged pixel (32 + 8 pad)
 
-template <bool RELU, bool OUT_BF16, int NCG>
-__global__ void __launch_bounds__(kB) k_conv_tile(const __bf16* __restrict__ U, int cs, const __bf16* __restrict__ X,
-                                                  const __bf16* __restrict__ Wt, const float* __restrict__ bias,
-                                                  void* __restrict__ Y, int N, int H, int W, int cin, int cout,
-                                                  int kpad, int tiles_x, int tiles_y)
+template <int WC, int R>
+struct ConvRowsShape {
+    static constexpr int WR = 4 / WC;                 // wavefronts along the rows
+    static constexpr int RB = WR * R;                 // block rows
+    static constexpr int SR = RB + 2;                 // staged rows
+    static constexpr int UNITS = SR * kHalo * 4;      // 16-byte units staged per chunk
+    static constexpr int PF = (UNITS + kB - 1) / kB;  // units per thread
+};
+
+template <bool RELU, bool OUT_BF16, int WC, int R, bool ONE>
+__global__ void __launch_bounds__(kB, 2) k_conv_tile(const __bf16* __restrict__ U, int cs,
+                                                     const __bf16* __restrict__ X, const __bf16* __restrict__ Wt,
+                                                     const float* __restrict__ bias, void* __restrict__ Y, int N,
+                                                     int H, int W, int cin, int cout, int kpad, int tiles_x,
+                                                     int tiles_y)
 {
-    __shared__ __align__(16) __bf16 tile[kHalo * kHalo * kPixStride];
+    using S = ConvRowsShape<WC, R>;
+    __shared__ __align__(16) __bf16 tile[S::SR * kHalo * kPixStride];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int b = blockIdx.x;
-    const int tx0 = (b % tiles_x) * kTile;
-    b /= tiles_x;
-    const int ty0 = (b % tiles_y) * kTile;
-    const int n = b / tiles_y;
-    const int co0 = blockIdx.y * (16 * NCG);
+    const int wr = wave % S::WR, wc = wave / S::WR;
+    const int ntiles = tiles_x * tiles_y * N;
+    const int nchunk = cin / 32;
+    const int co0 = blockIdx.y * (32 * WC) + 32 * wc;  // this wavefront's first output channel
     const int col = lane & 15, kg = lane >> 4;
     const __bf16* wrow = Wt + (size_t)(co0 + col) * kpad + 8 * kg;
-    floatx4 acc[4][NCG];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int g = 0; g < NCG; ++g) acc[r][g] = floatx4{0.f, 0.f, 0.f, 0.f};
     const int ck = cin - cs, Hh = H >> 1, Wh = W >> 1;
-    for (int ci0 = 0; ci0 < cin; ci0 += 32) {
-        __syncthreads();  // the previous chunk's reads are done
+    const int srow0 = wr * R;  // staged row of this wavefront's first output row (minus the halo)
+    // persistent: the block walks the pixel blocks t = blockIdx.x + k
+    // gridDim.x, and the 32-channel chunks of each; the input of the next
+    // (block, chunk) is loaded into registers while this one computes
+    bf16x8 pf[S::PF];
+    auto fetch = [&](int t, int ci0) {
+        const int tx0 = (t % tiles_x) * kTile;
+        const int rest = t / tiles_x;
+        const int ty0 = (rest % tiles_y) * S::RB;
+        const int n = rest / tiles_y;
         const bool from_u = ci0 < cs;
-        for (int i = threadIdx.x; i < kHalo * kHalo * 4; i += kB) {
-            const int pix = i >> 2, c8 = (i & 3) * 8;
-            const int yy = ty0 - 1 + pix / kHalo, xx = tx0 - 1 + pix % kHalo;
+#pragma unroll
+        for (int j = 0; j < S::PF; ++j) {
+            const int u = threadIdx.x + kB * j;
             bf16x8 v = zero8();
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                const __bf16* src =
-                    from_u ? U + ((((size_t)n * Hh + (yy >> 1)) * Wh + (xx >> 1)) * cs + ci0 + c8)
-                           : X + ((((size_t)n * H + yy) * W + xx) * ck + (ci0 - cs) + c8);
-                v = *reinterpret_cast<const bf16x8*>(src);
+            if (u < S::UNITS) {
+                const int pix = u >> 2, c8 = (u & 3) * 8;
+                const int yy = ty0 - 1 + pix / kHalo, xx = tx0 - 1 + pix % kHalo;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                    const __bf16* src =
+                        from_u ? U + ((((size_t)n * Hh + (yy >> 1)) * Wh + (xx >> 1)) * cs + ci0 + c8)
+                               : X + ((((size_t)n * H + yy) * W + xx) * ck + (ci0 - cs) + c8);
+                    v = *reinterpret_cast<const bf16x8*>(src);
+                }
             }
-            *reinterpret_cast<bf16x8*>(&tile[pix * kPixStride + c8]) = v;
+            pf[j] = v;
         }
-        __syncthreads();
+    };
+    // B operands of one tap column (3 taps x 2 channel groups), double
+    // buffered across the columns of a chunk
+    auto load_column = [&](bf16x8 (&bw)[3][2], int ci0, int dx) {
 #pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            const int dy = tap / 3, dx = tap % 3;  // offsets into the staged block (halo included)
-            bf16x8 bw[NCG];
-#pragma unroll
-            for (int g = 0; g < NCG; ++g)
-                bw[g] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)g * 16 * kpad + tap * cin + ci0);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 4 * wave + r;
-                const bf16x8 a =
-                    *reinterpret_cast<const bf16x8*>(&tile[((row + dy) * kHalo + col + dx) * kPixStride + 8 * kg]);
-#pragma unroll
-                for (int g = 0; g < NCG; ++g)
-                    acc[r][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[g], acc[r][g], 0, 0, 0);
-            }
+        for (int dy = 0; dy < 3; ++dy) {
+            const int tap = 3 * dy + dx;
+            bw[dy][0] = *reinterpret_cast<const bf16x8*>(wrow + tap * cin + ci0);
+            bw[dy][1] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)16 * kpad + tap * cin + ci0);
         }
+    };
+    float bs[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) bs[g] = bias ? bias[co0 + 16 * g + col] : 0.f;
+    int t = blockIdx.x;
+    if (t >= ntiles) return;
+    fetch(t, 0);
+    bf16x8 bA[3][2], bB[3][2];
+    // a single input chunk: the weights of all nine taps stay in registers
+    // for every pixel block the block computes
+    bf16x8 b0[3][2], b1[3][2], b2[3][2];
+    if (ONE) {
+        load_column(b0, 0, 0);
+        load_column(b1, 0, 1);
+        load_column(b2, 0, 2);
     }
-    // lane (col, kg) holds, for block row `row`, the pixels 4 kg + i of that
-    // row and output channels co0 + 16 g + col
-    float bs[NCG];
+    for (; t < ntiles; t += gridDim.x) {
+        floatx4 acc[R][2];
 #pragma unroll
-    for (int g = 0; g < NCG; ++g) bs[g] = bias ? bias[co0 + 16 * g + col] : 0.f;
+        for (int r = 0; r < R; ++r) {
+            acc[r][0] = floatx4{0.f, 0.f, 0.f, 0.f};
+            acc[r][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+        for (int c = 0; c < nchunk; ++c) {
+            __syncthreads();  // the previous chunk's reads are done
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int yy = ty0 + 4 * wave + r;
-        if (yy >= H) continue;
+            for (int j = 0; j < S::PF; ++j) {
+                const int u = threadIdx.x + kB * j;
+                if (u < S::UNITS) *reinterpret_cast<bf16x8*>(&tile[(u >> 2) * kPixStride + (u & 3) * 8]) = pf[j];
+            }
+            if (!ONE) load_column(bA, 32 * c, 0);
+            if (c + 1 < nchunk) fetch(t, 32 * (c + 1));
+            else if (t + (int)gridDim.x < ntiles) fetch(t + gridDim.x, 0);
+            __syncthreads();
+            auto column = [&](const bf16x8 (&bw)[3][2], int dx) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int xx = tx0 + 4 * kg + i;
-            if (xx >= W) continue;
-            const size_t p = ((size_t)n * H + yy) * W + xx;
+                for (int s = 0; s < R + 2; ++s) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(
+                        &tile[((srow0 + s) * kHalo + col + dx) * kPixStride + 8 * kg]);
 #pragma unroll
-            for (int g = 0; g < NCG; ++g) {
-                float v = acc[r][g][i] + bs[g];
-                if (RELU) v = fmaxf(v, 0.f);
-                if (OUT_BF16) static_cast<__bf16*>(Y)[p * cout + co0 + 16 * g + col] = __bf16(v);
-                else static_cast<float*>(Y)[p * cout + co0 + 16 * g + col] = v;
+                    for (int dy = 0; dy < 3; ++dy) {
+                        const int r = s - dy;
+                        if (r < 0 || r >= R) continue;
+                        acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[dy][0], acc[r][0], 0, 0, 0);
+                        acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[dy][1], acc[r][1], 0, 0, 0);
+                    }
+                }
+            };
+            if (ONE) {
+                column(b0, 0);
+                column(b1, 1);
+                column(b2, 2);
+            } else {
+                load_column(bB, 32 * c, 1);
+                column(bA, 0);
+                load_column(bA, 32 * c, 2);
+                column(bB, 1);
+                column(bA, 2);
+            }
+        }
+        // lane (col, kg) holds, for output row r, the pixels 4 kg + i of
+        // that row and output channels co0 + 16 g + col
+        const int tx0 = (t % tiles_x) * kTile;
+        const int rest = t / tiles_x;
+        const int ty0 = (rest % tiles_y) * S::RB;
+        const int n = rest / tiles_y;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int yy = ty0 + srow0 + r;
+            if (yy >= H) continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int xx = tx0 + 4 * kg + i;
+                if (xx >= W) continue;
+                const size_t p = ((size_t)n * H + yy) * W + xx;
+#pragma unroll
+                for (int g = 0; g < 2; ++g) {
+                    float v = acc[r][g][i] + bs[g];
+                    if (RELU) v = fmaxf(v, 0.f);
+                    if (OUT_BF16) static_cast<__bf16*>(Y)[p * cout + co0 + 16 * g + col] = __bf16(v);
+                    else static_cast<float*>(Y)[p * cout + co0 + 16 * g + col] = v;
+                }
             }
         }
     }
@@ -330,23 +408,33 @@ __global__ void k_pack_input(const float* __restrict__ x, long long P, int cin, 
     y[i] = __bf16(c < cin ? x[p * cin + c] : 0.f);
 }
 
-// 2x2 average pooling, BF16 NHWC
+// 2x2 average pooling, BF16 NHWC, C a multiple of 8: a thread owns 8
+// consecutive channels of one output pixel (16-byte loads and store)
 __global__ void k_pool(const __bf16* __restrict__ x, int N, int H, int W, int C, __bf16* __restrict__ y)
 {
-    const int Ho = H / 2, Wo = W / 2;
-    const long long total = (long long)N * Ho * Wo * C;
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    const long long total = (long long)N * Ho * Wo * C8;
     const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (i >= total) return;
-    const int c = static_cast<int>(i % C);
-    long long t = i / C;
+    const int c = static_cast<int>(i % C8) * 8;
+    long long t = i / C8;
     const int xo = static_cast<int>(t % Wo);
     t /= Wo;
     const int yo = static_cast<int>(t % Ho);
     const int n = static_cast<int>(t / Ho);
     const __bf16* b = x + (((size_t)n * H + 2 * yo) * W + 2 * xo) * C + c;
-    const float s = (static_cast<float>(b[0]) + static_cast<float>(b[C])) +
-                    (static_cast<float>(b[(size_t)W * C]) + static_cast<float>(b[(size_t)W * C + C]));
-    y[i] = __bf16(0.25f * s);
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(b);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(b + C);
+    const bf16x8 a2 = *reinterpret_cast<const bf16x8*>(b + (size_t)W * C);
+    const bf16x8 a3 = *reinterpret_cast<const bf16x8*>(b + (size_t)W * C + C);
+    bf16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float s = (static_cast<float>(a0[k]) + static_cast<float>(a1[k])) +
+                        (static_cast<float>(a2[k]) + static_cast<float>(a3[k]));
+        o[k] = __bf16(0.25f * s);
+    }
+    *reinterpret_cast<bf16x8*>(y + (((size_t)n * Ho + yo) * Wo + xo) * C + c) = o;
 }
 
 // g_x[y][x] += g_y[y / 2][x / 2] / 4 (FP32, fine grid)
@@ -401,13 +489,30 @@ __global__ void k_mask_grad(const float* __restrict__ g, const __bf16* __restric
 }
 
 // head: out[p] = sum_c w[c] x[p][c] + b (FP32)
+// C a multiple of 8: the pixel's channels are read as 16-byte vectors, all
+// issued before the sum, which adds them in channel order
 __global__ void k_head(const __bf16* __restrict__ x, long long P, int C, const float* __restrict__ w,
                        const float* __restrict__ b, float* __restrict__ out)
 {
     const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (p >= P) return;
     float s = b[0];
-    for (int c = 0; c < C; ++c) s += w[c] * static_cast<float>(x[p * C + c]);
+    const bf16x8* xv = reinterpret_cast<const bf16x8*>(x + p * C);
+    int c = 0;
+    for (; c + 32 <= C; c += 32) {
+        bf16x8 v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = xv[c / 8 + j];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s += w[c + 8 * j + k] * static_cast<float>(v[j][k]);
+    }
+    for (; c < C; c += 8) {
+        const bf16x8 v = xv[c / 8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += w[c + k] * static_cast<float>(v[k]);
+    }
     out[p] = s;
 }
 
@@ -641,6 +746,17 @@ void UNet::get_grads(float* host) const
 }
 
 namespace {
+// blocks of `k` (256 threads) that the device holds at once
+template <class K>
+int resident_conv_blocks(K k)
+{
+    int dev = 0, cus = 0, per = 0;
+    LSDTT_HIP(hipGetDevice(&dev));
+    LSDTT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    LSDTT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, kB, 0));
+    return std::max(1, per) * std::max(1, cus);
+}
+
 // y = conv3x3([upsample2(u), x]) (+ bias, ReLU), u with cs channels (cs = 0:
 // no u): the LDS block kernel when cin is a multiple of 32 (64-channel
 // output blocks when cout allows), the direct kernel otherwise (the
@@ -649,15 +765,35 @@ template <bool RELU, bool OUT_BF16>
 void launch_conv(const __bf16* u, int cs, const __bf16* x, const __bf16* w, const float* bias, void* y, int N, int H,
                  int W, int cin, int cout, int kpad, hipStream_t s)
 {
-    if (cin % 32 == 0 && cs % 32 == 0) {
-        const int tx = (W + kTile - 1) / kTile, ty = (H + kTile - 1) / kTile;
-        const unsigned nb = (unsigned)((long long)tx * ty * N);
-        if (cout % 64 == 0)
-            k_conv_tile<RELU, OUT_BF16, 4><<<dim3(nb, (unsigned)(cout / 64)), kB, 0, s>>>(
-                u, cs, x, w, bias, y, N, H, W, cin, cout, kpad, tx, ty);
-        else
-            k_conv_tile<RELU, OUT_BF16, 2><<<dim3(nb, (unsigned)(cout / 32)), kB, 0, s>>>(
-                u, cs, x, w, bias, y, N, H, W, cin, cout, kpad, tx, ty);
+    if (cin % 32 == 0 && cs % 32 == 0 && cout % 32 == 0) {
+        // the widest channel block that divides cout: 128, 64 or 32 channels
+        // over 8, 16 or 32 rows
+        const int wc = cout % 128 == 0 ? 4 : (cout % 64 == 0 ? 2 : 1);
+        const bool one = cin == 32;
+        const int rb = wc == 4 ? ConvRowsShape<4, 8>::RB
+                               : (wc == 2 ? ConvRowsShape<2, 8>::RB
+                                          : (one ? ConvRowsShape<1, 4>::RB : ConvRowsShape<1, 6>::RB));
+        const int tx = (W + kTile - 1) / kTile, ty = (H + rb - 1) / rb;
+        const long long nt = (long long)tx * ty * N;
+        if (nt >= (1LL << 31)) throw std::invalid_argument("conv3x3: too many pixel blocks");
+        const int gy = cout / (32 * wc);
+        // one input chunk: persistent blocks (what the device holds at once,
+        // shared among the channel blocks), each loading the next pixel
+        // block while it computes this one; several chunks: a block per
+        // pixel block (the chunk pipeline hides the loads)
+        auto launch = [&](auto k) {
+            const long long gx = one ? std::max(1LL, std::min(nt, (long long)resident_conv_blocks(k) / gy)) : nt;
+            k<<<dim3((unsigned)gx, (unsigned)gy), kB, 0, s>>>(u, cs, x, w, bias, y, N, H, W, cin, cout, kpad, tx, ty);
+        };
+        if (one) {
+            if (wc == 4) launch(k_conv_tile<RELU, OUT_BF16, 4, 8, true>);
+            else if (wc == 2) launch(k_conv_tile<RELU, OUT_BF16, 2, 8, true>);
+            else launch(k_conv_tile<RELU, OUT_BF16, 1, 4, true>);
+        } else {
+            if (wc == 4) launch(k_conv_tile<RELU, OUT_BF16, 4, 8, false>);
+            else if (wc == 2) launch(k_conv_tile<RELU, OUT_BF16, 2, 8, false>);
+            else launch(k_conv_tile<RELU, OUT_BF16, 1, 6, false>);
+        }
     } else {
         if (cs != 0) throw std::logic_error("conv3x3: an upsampled input needs cin and cs multiples of 32");
         const long long P = (long long)N * H * W;
@@ -712,7 +848,7 @@ void UNet::forward(const float* x, int N, int H, int W, float* y, hipStream_t s)
         conv(enc_a[l], wbf_off[l], nullptr, 0, in, N, Hl, Wl, act[S_EA(l)].p, s);
         conv(enc_b[l], wbf_off[L + l], nullptr, 0, act[S_EA(l)].p, N, Hl, Wl, act[S_EB(l)].p, s);
         if (l < L - 1)
-            k_pool<<<nblk(px(l + 1) * (base_ << l)), kB, 0, s>>>(act[S_EB(l)].p, N, Hl, Wl, base_ << l,
+            k_pool<<<nblk(px(l + 1) * (base_ << l) / 8), kB, 0, s>>>(act[S_EB(l)].p, N, Hl, Wl, base_ << l,
                                                                  act[S_P(l)].p);
     }
     for (int l = L - 2; l >= 0; --l) {

@@ -324,6 +324,37 @@ __device__ __forceinline__ int load_levels(const short* __restrict__ lstartg, co
     return maxdep;
 }
 
+// a tile's level starts held in registers: lane l keeps entries l, l + 64,
+// ... of the tile's lstart row, and a level's start is read back with a
+// readlane (the level index is the same in every lane).  Walking the levels
+// then waits on no memory access: reading them from global memory one level
+// ahead left a cache round trip per level exposed whenever a level had
+// little work.
+template <int TH>
+struct LevelStarts {
+    static constexpr int TN = TileShape<TH>::TN;
+    static constexpr int NJ = (TN + 1 + WL - 1) / WL;
+    int v[NJ];
+    __device__ __forceinline__ void load(const short* __restrict__ row)
+    {
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int idx = static_cast<int>(threadIdx.x) + WL * j;
+            v[j] = idx <= TN ? row[idx] : 0;
+        }
+    }
+    // L: 0..TN, uniform across the wavefront
+    __device__ __forceinline__ int at(int L) const
+    {
+        const int j = L >> 6, l = L & 63;
+        int r = 0;
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj)
+            if (jj == j) r = __builtin_amdgcn_readlane(v[jj], l);
+        return r;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // counts: deep levels first; count = 1 + in-tile donors (LDS) + donors in
 // other tiles (HBM, their current values)
@@ -341,7 +372,7 @@ __device__ __forceinline__ void tile_counts_body(const TileAt& T, const int* __r
     const int tile = T.tile;
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
-    if (!P.act[tile]) return;
+    if (P.prev_marked && !P.act[tile]) return;  // the first pass runs every tile
     __shared__ int acc[TN];
     __shared__ short lrcv[TN];
     __shared__ short ord[TN];
@@ -349,23 +380,29 @@ __device__ __forceinline__ void tile_counts_body(const TileAt& T, const int* __r
     const int r0 = ty * TH, c0 = tx * TW;
     int old[PER], g[PER];
     bool valid[PER];
+    short ordv[PER];
     {
+        // every load is issued before any is used (a use of a later load
+        // waits for all earlier ones)
         uint8_t m[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+            ordv[k] = ordg[(size_t)tile * TN + lane + WL * k];
             m[k] = mask[q.f];
             g[k] = rcv[q.f];
             old[k] = cnt[q.f];  // only this tile writes its cells
-            valid[k] = q.in && m[k];
+            valid[k] = q.in;
         }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) valid[k] = valid[k] && m[k];
     }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int i = lane + WL * k;
         acc[i] = valid[k] ? 1 : 0;
         lrcv[i] = static_cast<short>((valid[k] && g[k] >= 0) ? local_index<TH>(g[k], nc, r0, c0) : -1);
-        ord[i] = ordg[(size_t)tile * TN + i];
+        ord[i] = ordv[k];
     }
     const int maxdep = load_levels<TH>(lstartg, tile_maxdep, tile, ls);
     __syncthreads();
@@ -433,7 +470,9 @@ __device__ __forceinline__ void tile_counts_body(const TileAt& T, const int* __r
     const bool wmarked = __ballot(marked) != 0;
     if (lane == 0) {
         if (wmarked) *P.marked = 1;
-        P.runs[tile] += 1;  // a single-address atomic from every tile would serialise
+        // per-tile counter (a single-address atomic from every tile would
+        // serialise); an atomic whose result is unused is not waited on
+        __hip_atomic_fetch_add(&P.runs[tile], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -517,102 +556,130 @@ __device__ __forceinline__ short pack_lrd(short l, uint8_t d)
     return static_cast<short>(lr | ((d & 7) << 12));
 }
 
-// cells whose inputs are loaded together per lane (register pressure vs
-// loads in flight)
-constexpr int kSpBatch = 4;
 
-template <int TH, bool MH, int NM>
+// RR: a re-run (any pass after the first).  The tile's cells then hold the
+// results of its previous run, computed from the receivers' values of that
+// run, so a cell is solved again only when its receiver changed since: its
+// in-tile receiver changed in this run, or its receiver lies in another tile
+// (those cells are always solved again, and count as changed only if the
+// result differs).  Every other cell keeps its value without a Newton solve,
+// which is what makes the passes after the first cheap: a change at a tile's
+// mouth decays within a few cells upstream.
+template <int TH, bool MH, int NM, bool RR>
 __device__ __forceinline__ void tile_sp_body(const TileAt& T, const SpTileArgs& A, int nr, int nc, BC bc,
                                              const PassArgs& P)
 {
     constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
-    static_assert(PER % kSpBatch == 0, "cells per lane must be whole batches");
     const int tx = T.tx, ty = T.ty, ntx = T.ntx;
     const int tile = T.tile;
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
-    if (!P.act[tile]) return;
-    // 20 bytes of LDS per cell (10 KB for a 32 x 16 tile): the level starts
-    // are read from global memory one level ahead, the D8 code shares a word
-    // with the local receiver
-    __shared__ double val[TN];   // z_old until the cell's level, then its new z
+    // the first pass runs every tile
+    if (RR && !P.act[tile]) return;
+    // 20 bytes of LDS per cell (10 KB for a 32 x 16 tile), 29 in a re-run:
+    // the D8 code shares a word with the local receiver
+    // first run: z_old until the cell's level, then its new z; re-run: the
+    // previous run's z until the cell's level, then its new z
+    __shared__ double val[TN];
     __shared__ double kam[TN];   // K A^m; < 0: the reference skips the cell
     __shared__ short lrd[TN];    // local receiver (low 12 bits: index, kNoRcvP, kOutRcvP) | D8 code << 12
     __shared__ short ord[TN];
+    // re-run only: z_old, and whether the cell's value changed in this run
+    __shared__ double zos[RR ? TN : 1];
+    __shared__ uint8_t chg[RR ? TN : 1];
     const int r0 = ty * TH, c0 = tx * TW;
     const SpParams& p = A.p;
+    // every load of the tile is issued before any is used: a use of a later
+    // load waits for all earlier ones, so loads interleaved with their uses
+    // cost one memory round trip each
+    LevelStarts<TH> lvl;
+    lvl.load(A.lstart + (size_t)tile * (TN + 1));
     bool valid[PER];
-    uint8_t x[PER];
+    uint8_t x[PER], m[PER], dr[PER];
+    short ordv[PER];
+    int g[PER], ct[PER];
+    double zo[PER], Kf[PER], ar[PER];
     double zcur[PER];  // the cells' current z (only this tile writes them)
 #pragma unroll
-    for (int k0 = 0; k0 < PER; k0 += kSpBatch) {
-        uint8_t m[kSpBatch], dr[kSpBatch];
-        int g[kSpBatch], ct[kSpBatch];
-        double zo[kSpBatch], Kf[kSpBatch], ar[kSpBatch];
-#pragma unroll
-        for (int j = 0; j < kSpBatch; ++j) {
-            const LaneCell q = lane_cell(k0 + j, r0, c0, nr, nc);
-            m[j] = A.mask[q.f];
-            g[j] = A.rcv[q.f];
-            dr[j] = A.dir[q.f];
-            ct[j] = A.cnt[q.f];
-            zo[j] = A.z_old[q.f];
-            Kf[j] = A.K ? A.K[q.f] : p.K0;
-            ar[j] = A.area ? A.area[q.f] : 0.0;
-            x[k0 + j] = A.xdon[q.f];
-            zcur[k0 + j] = A.z[q.f];
-            valid[k0 + j] = q.in && m[j];
-        }
-#pragma unroll
-        for (int j = 0; j < kSpBatch; ++j) {
-            const int k = k0 + j, i = lane + WL * k;
-            double km = -1.0;
-            short l = kNoRcv;
-            if (valid[k] && g[j] >= 0) {
-                double area = A.area ? ar[j] : static_cast<double>(ct[j]) * p.pixel_area;
-                if (p.divide) area /= p.dx;
-                if (!(p.threshold > 0.0 && area < p.threshold)) {
-                    // K_i * width_factor * pow(area, m) (lsdtt_stream_power_cpu.cpp:258-260)
-                    km = Kf[j] * p.width * (MH ? sqrt(area) : pow(area, p.m));
-                    const int li = local_index<TH>(g[j], nc, r0, c0);
-                    l = li >= 0 ? static_cast<short>(li) : kOutRcv;
-                }
-            }
-            val[i] = zo[j];
-            kam[i] = km;
-            lrd[i] = pack_lrd(l, dr[j]);
-            ord[i] = A.ord[(size_t)tile * TN + i];
-        }
+    for (int k = 0; k < PER; ++k) {
+        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+        ordv[k] = A.ord[(size_t)tile * TN + lane + WL * k];
+        m[k] = A.mask[q.f];
+        g[k] = A.rcv[q.f];
+        dr[k] = A.dir[q.f];
+        ct[k] = A.cnt[q.f];
+        zo[k] = A.z_old[q.f];
+        Kf[k] = A.K ? A.K[q.f] : p.K0;
+        ar[k] = A.area ? A.area[q.f] : 0.0;
+        x[k] = A.xdon[q.f];
+        zcur[k] = A.z[q.f];
+        valid[k] = q.in;
     }
-    const short* __restrict__ lsg = A.lstart + (size_t)tile * (TN + 1);
     const int maxdep = A.tile_maxdep[tile];
-    int lb = lsg[0], le = lsg[1];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = lane + WL * k;
+        valid[k] = valid[k] && m[k];
+        double km = -1.0;
+        short l = kNoRcv;
+        if (valid[k] && g[k] >= 0) {
+            double area = A.area ? ar[k] : static_cast<double>(ct[k]) * p.pixel_area;
+            if (p.divide) area /= p.dx;
+            if (!(p.threshold > 0.0 && area < p.threshold)) {
+                // K_i * width_factor * pow(area, m) (lsdtt_stream_power_cpu.cpp:258-260)
+                km = Kf[k] * p.width * (MH ? sqrt(area) : pow(area, p.m));
+                const int li = local_index<TH>(g[k], nc, r0, c0);
+                l = li >= 0 ? static_cast<short>(li) : kOutRcv;
+            }
+        }
+        if (RR) {
+            val[i] = zcur[k];
+            zos[i] = zo[k];
+            chg[i] = 0;
+        } else {
+            val[i] = zo[k];
+        }
+        kam[i] = km;
+        lrd[i] = pack_lrd(l, dr[k]);
+        ord[i] = ordv[k];
+    }
+    int lb = lvl.at(0);
     __syncthreads();
     int eroded = 0;
     for (int L = 0; L <= maxdep; ++L) {
-        // next level's end, fetched while this level computes
-        const int lnext = L + 2 <= TN ? lsg[min(L + 2, TN)] : le;
+        const int le = lvl.at(L + 1);
         for (int q = lb + lane; q < le; q += WL) {
             const int i = ord[q];
             const int w = lrd[i];
             const int lr = w & 0xFFF;
             if (lr == kNoRcvP) continue;
             const int dcode = (w >> 12) & 7;
-            const double z0 = val[i];
+            const double z0 = RR ? zos[i] : val[i];
             double zr;
             if (lr != kOutRcvP) {
                 zr = val[lr];
+                if (RR && !chg[lr]) {
+                    // receiver unchanged: so is the cell (and its erosion)
+                    eroded += zr < z0;
+                    continue;
+                }
             } else {
                 const int g = neighbour(r0 + i / TW, c0 + i % TW, dcode, nr, nc, bc);
                 zr = __hip_atomic_load(&A.z[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            if (zr < z0) {
+            if (RR) {
+                const double v = zr < z0 ? sp_solve<NM>(p, z0, zr, kam[i], (dcode & 1) ? p.dx * kSqrt2 : p.dx) : z0;
+                eroded += zr < z0;
+                if (__double_as_longlong(v) != __double_as_longlong(val[i])) {
+                    val[i] = v;
+                    chg[i] = 1;
+                }
+            } else if (zr < z0) {
                 val[i] = sp_solve<NM>(p, z0, zr, kam[i], (dcode & 1) ? p.dx * kSqrt2 : p.dx);
                 ++eroded;
             }
         }
         lb = le;
-        le = lnext;
         __syncthreads();
     }
     // a changed elevation with donors in other tiles re-runs those tiles
@@ -637,21 +704,23 @@ __device__ __forceinline__ void tile_sp_body(const TileAt& T, const SpTileArgs& 
     if (lane == 0) {
         A.tile_eroded[tile] = eroded;
         if (wmarked) *P.marked = 1;
-        P.runs[tile] += 1;  // a single-address atomic from every tile would serialise
+        // per-tile counter (a single-address atomic from every tile would
+        // serialise); an atomic whose result is unused is not waited on
+        __hip_atomic_fetch_add(&P.runs[tile], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
-template <int TH, bool MH, int NM>
+template <int TH, bool MH, int NM, bool RR>
 __global__ void __launch_bounds__(WL) k_tile_sp(SpTileArgs A, int nr, int nc, BC bc, PassArgs P)
 {
     if (P.prev_marked && *P.prev_marked == 0) return;
-    tile_sp_body<TH, MH, NM>(grid_tile(), A, nr, nc, bc, P);
+    tile_sp_body<TH, MH, NM, RR>(grid_tile(), A, nr, nc, bc, P);
 }
 
 // batched: A holds the concatenated arrays; each tile's grid supplies its
 // offset and stream-power parameters; tiles of grids whose exponent mode
 // is not this kernel's are left to that mode's launch
-template <int TH, bool MH, int NM>
+template <int TH, bool MH, int NM, bool RR>
 __global__ void __launch_bounds__(WL) kb_tile_sp(BatchTiles B, SpTileArgs A, PassArgs P)
 {
     if (P.prev_marked && *P.prev_marked == 0) return;
@@ -669,7 +738,7 @@ __global__ void __launch_bounds__(WL) kb_tile_sp(BatchTiles B, SpTileArgs A, Pas
     a.area = nullptr;
     a.K = nullptr;
     a.p = g.sp;
-    tile_sp_body<TH, MH, NM>(T, a, g.nr, g.nc, g.bc, P);
+    tile_sp_body<TH, MH, NM, RR>(T, a, g.nr, g.nc, g.bc, P);
 }
 
 // ---------------------------------------------------------------------------
@@ -836,7 +905,7 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, 
     const int tile = ty * ntx + tx;
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
-    if (!P.act[tile]) return;
+    if (P.prev_marked && !P.act[tile]) return;  // the first pass runs every tile
     __shared__ double val[TN];
     __shared__ float wl[2][TN];
     __shared__ short lr[2][TN];  // local receivers (-1: none or in another tile)
@@ -858,22 +927,35 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, 
     int g[PER][2];
     double old[PER];
     bool valid[PER];
+    {
+        // every load is issued before any is used (a use of a later load
+        // waits for all earlier ones)
+        float wv[PER][2];
+        short ordv[PER];
+        uint8_t mk[PER];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
-        const uint8_t m = A.mask[q.f];
-        g[k][0] = A.rec[2 * (size_t)q.f];
-        g[k][1] = A.rec[2 * (size_t)q.f + 1];
-        const float w0 = A.w[2 * (size_t)q.f], w1 = A.w[2 * (size_t)q.f + 1];
-        old[k] = A.acc[q.f];  // only this tile writes its cells
-        valid[k] = q.in && m;
-        const int i = lane + WL * k;
-        lr[0][i] = static_cast<short>((valid[k] && g[k][0] >= 0) ? local_index<TH>(g[k][0], nc, r0, c0) : -1);
-        lr[1][i] = static_cast<short>((valid[k] && g[k][1] >= 0) ? local_index<TH>(g[k][1], nc, r0, c0) : -1);
-        wl[0][i] = w0;
-        wl[1][i] = w1;
-        val[i] = 0.0;
-        ord[i] = A.ord[(size_t)tile * TN + i];
+        for (int k = 0; k < PER; ++k) {
+            const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+            ordv[k] = A.ord[(size_t)tile * TN + lane + WL * k];
+            mk[k] = A.mask[q.f];
+            g[k][0] = A.rec[2 * (size_t)q.f];
+            g[k][1] = A.rec[2 * (size_t)q.f + 1];
+            wv[k][0] = A.w[2 * (size_t)q.f];
+            wv[k][1] = A.w[2 * (size_t)q.f + 1];
+            old[k] = A.acc[q.f];  // only this tile writes its cells
+            valid[k] = q.in;
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            valid[k] = valid[k] && mk[k];
+            const int i = lane + WL * k;
+            lr[0][i] = static_cast<short>((valid[k] && g[k][0] >= 0) ? local_index<TH>(g[k][0], nc, r0, c0) : -1);
+            lr[1][i] = static_cast<short>((valid[k] && g[k][1] >= 0) ? local_index<TH>(g[k][1], nc, r0, c0) : -1);
+            wl[0][i] = wv[k][0];
+            wl[1][i] = wv[k][1];
+            val[i] = 0.0;
+            ord[i] = ordv[k];
+        }
     }
     const int maxdep = load_levels<TH>(A.lstart, A.tile_maxdep, tile, ls);
     int gsx[2][8];
@@ -993,7 +1075,7 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, 
     const bool wmarked = __ballot(marked) != 0;
     if (lane == 0) {
         if (wmarked) *P.marked = 1;
-        P.runs[tile] += 1;
+        __hip_atomic_fetch_add(&P.runs[tile], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1183,7 +1265,7 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
     const int tile = ty * ntx + tx;
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
-    if (!P.act[tile]) return;
+    if (P.prev_marked && !P.act[tile]) return;  // the first pass runs every tile
     __shared__ double val[TN];  // z_old until the cell's level, then its new z
     __shared__ double kam[TN];
     __shared__ short lr[2][TN]; // local receivers, -1 outside the tile or none
@@ -1196,18 +1278,35 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
     bool valid[PER];
     uint8_t x[PER];
     double zcur[PER];
+    // every load is issued before any is used (a use of a later load waits
+    // for all earlier ones)
+    uint8_t mk[PER];
+    int gv[PER][2];
+    float wv[PER][2];
+    double zov[PER], Kv[PER], arv[PER];
+    short ordv[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const LaneCell q = lane_cell(k, r0, c0, nr, nc);
-        const uint8_t m = A.mask[q.f];
-        const int g0 = A.rec[2 * (size_t)q.f], g1 = A.rec[2 * (size_t)q.f + 1];
-        const float w0 = A.w[2 * (size_t)q.f], w1 = A.w[2 * (size_t)q.f + 1];
-        const double zo = A.z_old[q.f];
-        const double Kf = A.K ? A.K[q.f] : p.K0;
-        const double ar = A.area[q.f];
+        ordv[k] = A.ord[(size_t)tile * TN + lane + WL * k];
+        mk[k] = A.mask[q.f];
+        gv[k][0] = A.rec[2 * (size_t)q.f];
+        gv[k][1] = A.rec[2 * (size_t)q.f + 1];
+        wv[k][0] = A.w[2 * (size_t)q.f];
+        wv[k][1] = A.w[2 * (size_t)q.f + 1];
+        zov[k] = A.z_old[q.f];
+        Kv[k] = A.K ? A.K[q.f] : p.K0;
+        arv[k] = A.area[q.f];
         x[k] = A.xdon[q.f];
         zcur[k] = A.z[q.f];
-        valid[k] = q.in && m;
+        valid[k] = q.in;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int g0 = gv[k][0], g1 = gv[k][1];
+        const float w0 = wv[k][0], w1 = wv[k][1];
+        const double zo = zov[k], Kf = Kv[k], ar = arv[k];
+        valid[k] = valid[k] && mk[k];
         const int i = lane + WL * k;
         const int nrec = (g0 >= 0) + (g1 >= 0);
         uint8_t kd = kDinfSkip;
@@ -1228,7 +1327,7 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
         kind[i] = kd;
         lr[0][i] = static_cast<short>((valid[k] && g0 >= 0) ? local_index<TH>(g0, nc, r0, c0) : -1);
         lr[1][i] = static_cast<short>((valid[k] && g1 >= 0) ? local_index<TH>(g1, nc, r0, c0) : -1);
-        ord[i] = A.ord[(size_t)tile * TN + i];
+        ord[i] = ordv[k];
     }
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
@@ -1242,14 +1341,15 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
         zx[sl][0] = x0 ? __hip_atomic_load(&A.z[g0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
         zx[sl][1] = x1 ? __hip_atomic_load(&A.z[g1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
     }
-    const short* __restrict__ lsg = A.lstart + (size_t)tile * (TN + 1);
+    LevelStarts<TH> lvl;
+    lvl.load(A.lstart + (size_t)tile * (TN + 1));
     const int maxdep = A.tile_maxdep[tile];
-    int lb = lsg[0], le = lsg[1];
+    int lb = lvl.at(0);
     __syncthreads();
     const double dxc = p.dx, dxd = p.dx * kSqrt2, tol = p.slope_clamp * p.dx;
     int eroded = 0;
     for (int L = 0; L <= maxdep; ++L) {
-        const int lnext = L + 2 <= TN ? lsg[min(L + 2, TN)] : le;
+        const int le = lvl.at(L + 1);
         for (int q = lb + lane; q < le; q += WL) {
             const int i = ord[q];
             const uint8_t kd = kind[i];
@@ -1329,7 +1429,6 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
             val[i] = zn;
         }
         lb = le;
-        le = lnext;
         __syncthreads();
     }
     bool marked = false;
@@ -1352,7 +1451,7 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
     if (lane == 0) {
         A.tile_eroded[tile] = eroded;
         if (wmarked) *P.marked = 1;
-        P.runs[tile] += 1;
+        __hip_atomic_fetch_add(&P.runs[tile], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1393,7 +1492,7 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_chi(DinfChiArgs A, int nr, int
     const int tile = ty * ntx + tx;
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
-    if (!P.act[tile]) return;
+    if (P.prev_marked && !P.act[tile]) return;  // the first pass runs every tile
     __shared__ float val[TN];
     __shared__ float amn[TN];
     __shared__ short lr[2][TN];  // local receivers, -1 outside the tile or none
@@ -1405,15 +1504,32 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_chi(DinfChiArgs A, int nr, int
     bool valid[PER];
     uint8_t x[PER];
     float ccur[PER];
+    // every load is issued before any is used (a use of a later load waits
+    // for all earlier ones)
+    uint8_t mk[PER];
+    int gv[PER][2];
+    float wv[PER][2], amv[PER];
+    short ordv[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const LaneCell q = lane_cell(k, r0, c0, nr, nc);
-        const uint8_t m = A.mask[q.f];
-        const int g0 = A.rec[2 * (size_t)q.f], g1 = A.rec[2 * (size_t)q.f + 1];
-        const float w0 = A.w[2 * (size_t)q.f], w1 = A.w[2 * (size_t)q.f + 1];
+        ordv[k] = A.ord[(size_t)tile * TN + lane + WL * k];
+        mk[k] = A.mask[q.f];
+        gv[k][0] = A.rec[2 * (size_t)q.f];
+        gv[k][1] = A.rec[2 * (size_t)q.f + 1];
+        wv[k][0] = A.w[2 * (size_t)q.f];
+        wv[k][1] = A.w[2 * (size_t)q.f + 1];
+        amv[k] = A.amn[q.f];
         x[k] = A.xdon[q.f];
         ccur[k] = A.chi[q.f];
-        valid[k] = q.in && m;
+        valid[k] = q.in;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const LaneCell q = lane_cell(k, r0, c0, nr, nc);
+        const int g0 = gv[k][0], g1 = gv[k][1];
+        const float w0 = wv[k][0], w1 = wv[k][1];
+        valid[k] = valid[k] && mk[k];
         const int nrec = (g0 >= 0) + (g1 >= 0);
         uint8_t kd = kChiSkip;
         if (valid[k] && nrec > 0) {
@@ -1428,11 +1544,11 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_chi(DinfChiArgs A, int nr, int
         }
         const int i = lane + WL * k;
         val[i] = ccur[k];
-        amn[i] = A.amn[q.f];
+        amn[i] = amv[k];
         kind[i] = kd;
         lr[0][i] = static_cast<short>((valid[k] && g0 >= 0) ? local_index<TH>(g0, nc, r0, c0) : -1);
         lr[1][i] = static_cast<short>((valid[k] && g1 >= 0) ? local_index<TH>(g1, nc, r0, c0) : -1);
-        ord[i] = A.ord[(size_t)tile * TN + i];
+        ord[i] = ordv[k];
     }
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
@@ -1446,13 +1562,14 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_chi(DinfChiArgs A, int nr, int
         cx[sl][0] = x0 ? __hip_atomic_load(&A.chi[g0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0f;
         cx[sl][1] = x1 ? __hip_atomic_load(&A.chi[g1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0f;
     }
-    const short* __restrict__ lsg = A.lstart + (size_t)tile * (TN + 1);
+    LevelStarts<TH> lvl;
+    lvl.load(A.lstart + (size_t)tile * (TN + 1));
     const int maxdep = A.tile_maxdep[tile];
-    int lb = lsg[0], le = lsg[1];
+    int lb = lvl.at(0);
     __syncthreads();
     const float dx = A.dx, dxd = A.dxd;
     for (int L = 0; L <= maxdep; ++L) {
-        const int lnext = L + 2 <= TN ? lsg[min(L + 2, TN)] : le;
+        const int le = lvl.at(L + 1);
         for (int q = lb + lane; q < le; q += WL) {
             const int i = ord[q];
             const uint8_t kd = kind[i];
@@ -1477,7 +1594,6 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_chi(DinfChiArgs A, int nr, int
             val[i] = cn;
         }
         lb = le;
-        le = lnext;
         __syncthreads();
     }
     bool marked = false;
@@ -1498,7 +1614,7 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_chi(DinfChiArgs A, int nr, int
     const bool wmarked = __ballot(marked) != 0;
     if (lane == 0) {
         if (wmarked) *P.marked = 1;
-        P.runs[tile] += 1;
+        __hip_atomic_fetch_add(&P.runs[tile], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1650,12 +1766,19 @@ void batch_tile_sp(int th, unsigned modes, const BatchTiles& B, int ntiles, cons
 {
     with_tile_height(th, [&](auto H) {
         constexpr int h = decltype(H)::value;
-        if (modes & (1u << sp_mode_of(false, 0))) kb_tile_sp<h, false, 0><<<ntiles, WL, 0, s>>>(B, A, P);
-        if (modes & (1u << sp_mode_of(false, 1))) kb_tile_sp<h, false, 1><<<ntiles, WL, 0, s>>>(B, A, P);
-        if (modes & (1u << sp_mode_of(false, 2))) kb_tile_sp<h, false, 2><<<ntiles, WL, 0, s>>>(B, A, P);
-        if (modes & (1u << sp_mode_of(true, 0))) kb_tile_sp<h, true, 0><<<ntiles, WL, 0, s>>>(B, A, P);
-        if (modes & (1u << sp_mode_of(true, 1))) kb_tile_sp<h, true, 1><<<ntiles, WL, 0, s>>>(B, A, P);
-        if (modes & (1u << sp_mode_of(true, 2))) kb_tile_sp<h, true, 2><<<ntiles, WL, 0, s>>>(B, A, P);
+        // the first pass of a settle runs every tile from z_old; later passes
+        // are re-runs
+        auto launch = [&](auto RR) {
+            constexpr bool rr = decltype(RR)::value;
+            if (modes & (1u << sp_mode_of(false, 0))) kb_tile_sp<h, false, 0, rr><<<ntiles, WL, 0, s>>>(B, A, P);
+            if (modes & (1u << sp_mode_of(false, 1))) kb_tile_sp<h, false, 1, rr><<<ntiles, WL, 0, s>>>(B, A, P);
+            if (modes & (1u << sp_mode_of(false, 2))) kb_tile_sp<h, false, 2, rr><<<ntiles, WL, 0, s>>>(B, A, P);
+            if (modes & (1u << sp_mode_of(true, 0))) kb_tile_sp<h, true, 0, rr><<<ntiles, WL, 0, s>>>(B, A, P);
+            if (modes & (1u << sp_mode_of(true, 1))) kb_tile_sp<h, true, 1, rr><<<ntiles, WL, 0, s>>>(B, A, P);
+            if (modes & (1u << sp_mode_of(true, 2))) kb_tile_sp<h, true, 2, rr><<<ntiles, WL, 0, s>>>(B, A, P);
+        };
+        if (P.prev_marked) launch(std::true_type{});
+        else launch(std::false_type{});
     });
     LSDTT_LAUNCH_CHECK();
 }
@@ -1805,8 +1928,8 @@ int FluvialTiles::counts(const int* rcv, const uint8_t* mask, int* cnt, bool war
         "upstream_counts", 9.0 * n,
         [&](const PassArgs& P) {
             with_tile_height(th, [&](auto H) {
-                k_tile_counts<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(rcv, mask, ord.p, lstart.p, tile_maxdep.p, xdon.p,
-                                                                 nr, nc, bc_, cnt, P);
+                k_tile_counts<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(rcv, mask, ord.p, lstart.p, tile_maxdep.p,
+                                                                               xdon.p, nr, nc, bc_, cnt, P);
             });
             LSDTT_LAUNCH_CHECK();
         },
@@ -1830,17 +1953,25 @@ int FluvialTiles::stream_power(const int* rcv, const uint8_t* dir, const uint8_t
         [&](const PassArgs& P) {
             const dim3 g(ntx, nty);
             const int nm = sp.n_is_one ? 0 : (sp.n1_half ? 1 : 2);
-            with_tile_height(th, [&](auto H) {
+            // the first pass runs every tile from z_old; later passes are
+            // re-runs that solve only the cells whose receiver changed
+            auto launch_sp = [&](auto k) { k<<<g, WL, 0, s>>>(A, nr, nc, bc_, P); };
+            auto launch = [&](auto H, auto RR) {
                 constexpr int h = decltype(H)::value;
+                constexpr bool rr = decltype(RR)::value;
                 if (sp.m_half) {
-                    if (nm == 0) k_tile_sp<h, true, 0><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
-                    else if (nm == 1) k_tile_sp<h, true, 1><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
-                    else k_tile_sp<h, true, 2><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+                    if (nm == 0) launch_sp(k_tile_sp<h, true, 0, rr>);
+                    else if (nm == 1) launch_sp(k_tile_sp<h, true, 1, rr>);
+                    else launch_sp(k_tile_sp<h, true, 2, rr>);
                 } else {
-                    if (nm == 0) k_tile_sp<h, false, 0><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
-                    else if (nm == 1) k_tile_sp<h, false, 1><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
-                    else k_tile_sp<h, false, 2><<<g, WL, 0, s>>>(A, nr, nc, bc_, P);
+                    if (nm == 0) launch_sp(k_tile_sp<h, false, 0, rr>);
+                    else if (nm == 1) launch_sp(k_tile_sp<h, false, 1, rr>);
+                    else launch_sp(k_tile_sp<h, false, 2, rr>);
                 }
+            };
+            with_tile_height(th, [&](auto H) {
+                if (P.prev_marked) launch(H, std::true_type{});
+                else launch(H, std::false_type{});
             });
             LSDTT_LAUNCH_CHECK();
         },
