@@ -66,6 +66,11 @@ import time
 
 import numpy as np
 
+
+def progress(msg):
+    """One line per finished stage on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -775,6 +780,7 @@ def main():
         newton, krylov, worst_r = newton + a["newton"], krylov + a["krylov"], max(worst_r, a["worst_r"])
         segs.append({"ms_per_step": round(1e3 * sdt / seg, 3), "newton_iters_per_step": round(a["newton"] / seg, 2),
                      "krylov_iters_per_step": round(a["krylov"] / seg, 2)})
+        progress(f"steps {len(segs) * seg}/{args.steps}: {segs[-1]}")
     psteps = min(args.steps, args.profile_steps)
     _, _, _, prof = run_steps(lem, psteps, sync, profile=True)
 
@@ -813,17 +819,22 @@ def main():
         "kernels_ms_per_step": {k: round(v[0] / psteps, 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
         "profiled_steps": psteps,
     }
+    progress(f"plain trajectory done: {out['value']:.4g} cell-steps/s")
     if not args.no_unsmoothed:
         out["unsmoothed"] = unsmoothed_measure(args, sync)
+        progress("raw-surface workload done")
     if not args.no_batched:
         del lem
         b = ensemble_measure(args)
         b["vs_plain"] = round(b["value"] / value, 4)
         out["batched"] = b
+        progress("batched ensemble done")
     if not args.no_learned:
         out["learned_warm_start"] = learned_comparison(args, z, K, sync, value)
+        progress("learned warm start done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(K, args.cpu_budget)
+        progress("cpu baseline done")
     print(json.dumps(out))
 
 

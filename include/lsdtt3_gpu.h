@@ -663,6 +663,40 @@ int lsdtt_gpu_lem_set_warm_start(lsdtt_gpu_lem* lem, lsdtt_gpu_cnn* net, int32_t
 int lsdtt_gpu_build_boundary_flats(int32_t n_rows, int32_t n_cols, const uint8_t* mask,
                                    const char* boundary_conditions, int64_t* out, int64_t* n_out);
 
+/* ------------------------------------------------------------------------
+ * Raster files on either side of the LEM path (host code, no GPU needed):
+ * single-band GeoTIFF (.tif/.tiff: strips or tiles, uncompressed, LZW or
+ * DEFLATE, predictors 2 and 3, integer or float samples; written as float32
+ * strips, DEFLATE + floating-point predictor when compress != 0) and ESRI
+ * ASCII grids (.asc).  They replace the GDAL-backed lsdtt_data_loader's
+ * read_full / write_full for these formats (src/core/lsdtt_data_loader.hpp,
+ * src/core/lsdtt_data_loader.cpp:700-765): values as float32, mask 0 where a
+ * value equals the no-data value or is not finite (those values become the
+ * no-data value), north-up and square pixels required (geographic CRS
+ * excepted), no-data -9999 when the file names none; the meta mirrors
+ * lsdtt_raster_meta (src/core/lsdtt_raster_meta.hpp:44-84) with the CRS as
+ * an EPSG code instead of WKT.
+ * ------------------------------------------------------------------------ */
+typedef struct lsdtt_gpu_raster_meta {
+    int32_t n_rows, n_cols;
+    double x_min;          /* west edge */
+    double y_min;          /* south edge */
+    double cell_size;      /* pixel width */
+    double cell_size_y;    /* pixel height when it differs (geographic CRS), else 0 */
+    double no_data_value;
+    int32_t epsg;          /* CRS code, 0 when unknown */
+    int32_t geographic;    /* 1: geographic CRS */
+} lsdtt_gpu_raster_meta;
+
+/* Header only: fills *meta. */
+int lsdtt_gpu_raster_info(const char* path, lsdtt_gpu_raster_meta* meta);
+/* data float[n_rows*n_cols] (row-major, north row first), mask uint8[n] or
+ * NULL; meta must hold the shape lsdtt_gpu_raster_info returned. */
+int lsdtt_gpu_raster_read(const char* path, float* data, uint8_t* mask, lsdtt_gpu_raster_meta* meta);
+/* Cells with mask 0 (mask may be NULL) are written as the no-data value. */
+int lsdtt_gpu_raster_write(const char* path, const float* data, const uint8_t* mask,
+                           const lsdtt_gpu_raster_meta* meta, int32_t compress);
+
 #ifdef __cplusplus
 }
 #endif

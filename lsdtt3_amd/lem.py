@@ -151,9 +151,19 @@ class LandscapeModel:
         self.last_stats = st.as_dict()
         return self.last_stats
 
-    def run(self, n_steps: int) -> dict:
+    def run(self, n_steps: int, stats_csv=None) -> dict:
+        """n_steps steps; with ``stats_csv`` (a path) the per-step statistics
+        table of lsdtt-lem (``formats.ElevationStatsCSV``) is written there."""
+        table = None
+        if stats_csv is not None:
+            from .formats import ElevationStatsCSV
+            table = ElevationStatsCSV.for_model(self)
         for _ in range(int(n_steps)):
-            self.step()
+            st = self.step()
+            if table is not None:
+                table.add(self, st)
+        if table is not None:
+            table.write(stats_csv)
         return self.last_stats
 
     @property
