@@ -208,7 +208,7 @@ def cpu_baseline(K, budget_s=15.0, size=512):
 # timer group -> the kernel whose dispatches it times
 GROUP_KERNEL = {"stream_power": "k_tile_sp", "upstream_counts": "k_tile_counts", "tile_depths": "k_tile_order",
                 "hillslope_residual": "k_trial", "hillslope_prepare": "k_prepare", "receivers": "k_receivers",
-                "bicgstab_matvec": "k_bi_v", "bicgstab_x": "k_bi_x", "bicgstab_s": "k_bi_s", "bicgstab_p": "k_bi_p",
+                "bicgstab_pv": "k_bi_pv", "bicgstab_st": "k_bi_st", "bicgstab_x": "k_bi_x",
                 "step_stats": "k_stats"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lem_step_4096_hbm_traffic.txt")
 

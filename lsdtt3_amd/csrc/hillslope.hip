@@ -387,44 +387,120 @@ __device__ __forceinline__ void st4(float* __restrict__ p, long long i, float4 v
     reinterpret_cast<float4*>(p)[i] = v;
 }
 
-// p = r + beta (p - omega v); phat = M^-1 p  (first iteration: p = b)
-__global__ void __launch_bounds__(256) k_bi_p(BiArgs a)
-{
-    if (a.flags[0]) return;
-    const double beta = (a.scal[HS_RHO] / a.scal[HS_RHO_OLD]) * (a.scal[HS_ALPHA] / a.scal[HS_OMEGA]);
-    const float bt = static_cast<float>(beta), om = static_cast<float>(a.scal[HS_OMEGA]);
-    const float* __restrict__ b = a.b;
-    const float* __restrict__ r = a.r;
-    const float* __restrict__ v = a.v;
-    const float* __restrict__ dinv = a.dinv;
-    float* __restrict__ p = a.p;
-    float* __restrict__ phat = a.phat;
-    const long long n4 = a.n >> 2, stride = (long long)gridDim.x * blockDim.x;
-    const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    for (long long i = t0; i < n4; i += stride) {
-        const float4 di = ld4(dinv, i);
-        float4 pv;
-        if (a.first) {
-            pv = ld4(b, i);
+// One BiCGSTAB iteration (right Jacobi preconditioning M^-1 = dinv) in three
+// kernels.  The vectors phat = M^-1 p, s = r - alpha v and shat = M^-1 s are
+// never stored: each kernel forms them where it needs them, around its
+// staged tiles too, with the same float expressions as stored vectors would
+// hold, so the iterates are the same as with separate update kernels.
+//   k_bi_pv: p = r + beta (p_old - omega v_old) (first: p = b); v = J phat;
+//            partial sums of rhat . v
+//   k_bi_st: alpha = rho / (rhat . v); t = J shat; partial sums of t . s, t . t
+//   k_bi_x:  omega = (t . s) / (t . t); x += alpha phat + omega shat;
+//            r = s - omega t; partial sums of rhat . r, r . r
+
+// the staged input of a Jacobian product: one value per staged cell, from
+// the loads V::load issued before any is used
+struct PhatView {
+    const float *b, *r, *p, *v, *dinv;
+    float bt, om;
+    bool first;
+    struct L {
+        float di, a0, a1, a2;
+    };
+    __device__ __forceinline__ L load(long long g) const
+    {
+        L l;
+        l.di = dinv[g];
+        if (first) {
+            l.a0 = b[g];
+            l.a1 = l.a2 = 0.0f;
         } else {
-            const float4 rv = ld4(r, i), po = ld4(p, i), vv = ld4(v, i);
-            pv.x = rv.x + bt * (po.x - om * vv.x);
-            pv.y = rv.y + bt * (po.y - om * vv.y);
-            pv.z = rv.z + bt * (po.z - om * vv.z);
-            pv.w = rv.w + bt * (po.w - om * vv.w);
+            l.a0 = r[g];
+            l.a1 = p[g];
+            l.a2 = v[g];
         }
-        st4(p, i, pv);
-        st4(phat, i, make_float4(di.x * pv.x, di.y * pv.y, di.z * pv.z, di.w * pv.w));
+        return l;
     }
-    for (long long f = 4 * n4 + t0; f < a.n; f += stride) {
-        const float pv = a.first ? b[f] : r[f] + bt * (p[f] - om * v[f]);
-        p[f] = pv;
-        phat[f] = dinv[f] * pv;
+    __device__ __forceinline__ float pval(const L& l) const { return first ? l.a0 : l.a0 + bt * (l.a1 - om * l.a2); }
+    __device__ __forceinline__ float value(const L& l) const { return l.di * pval(l); }
+};
+
+struct ShatView {
+    const float *r, *v, *dinv;
+    float al;
+    struct L {
+        float di, a0, a1;
+    };
+    __device__ __forceinline__ L load(long long g) const
+    {
+        L l;
+        l.di = dinv[g];
+        l.a0 = r[g];
+        l.a1 = v[g];
+        return l;
+    }
+    __device__ __forceinline__ float sval(const L& l) const { return l.a0 - al * l.a1; }
+    __device__ __forceinline__ float value(const L& l) const { return l.di * sval(l); }
+};
+
+// stage z and the view's values over the tile plus halo (stage_tile's
+// clamped, branch-free loads)
+template <class V>
+__device__ __forceinline__ void stage_tile_view(const double* __restrict__ z, const V& view, int nr, int nc, int r0,
+                                                int c0, double* __restrict__ zs, float* __restrict__ xs)
+{
+    constexpr int kStage = (kSH * kSW + kB - 1) / kB;
+    double zv[kStage];
+    typename V::L lv[kStage];
+#pragma unroll
+    for (int j = 0; j < kStage; ++j) {
+        const int i = min(static_cast<int>(threadIdx.x) + j * kB, kSH * kSW - 1);
+        const int rr = min(max(r0 - 1 + i / kSW, 0), nr - 1), cc = min(max(c0 - 1 + i % kSW, 0), nc - 1);
+        const long long g = (long long)rr * nc + cc;
+        zv[j] = z[g];
+        lv[j] = view.load(g);
+    }
+#pragma unroll
+    for (int j = 0; j < kStage; ++j) {
+        const int i = threadIdx.x + j * kB;
+        if (i >= kSH * kSW) continue;
+        zs[i] = zv[j];
+        xs[i] = view.value(lv[j]);
     }
 }
 
-// v = J phat; alpha = rho / (rhat . v)
-__global__ void __launch_bounds__(256) k_bi_v(BiArgs a)
+// y = J x on general row i, x given by a view (zero on non-DOF cells)
+template <class V>
+__device__ __forceinline__ float jx_gen_view(const HsDom& d, const V& view, int i, int f)
+{
+    const int r = f / d.nc, c = f - r * d.nc;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc += d.c9g[(size_t)k * d.n_gen + i] * view.value(view.load(slot_cell(d, r, c, k)));
+    return acc;
+}
+
+__device__ __forceinline__ PhatView phat_view(const BiArgs& a)
+{
+    PhatView pv;
+    pv.b = a.b;
+    pv.r = a.r;
+    pv.p = a.p_old;
+    pv.v = a.v_old;
+    pv.dinv = a.dinv;
+    pv.first = a.first != 0;
+    if (pv.first) {
+        pv.bt = pv.om = 0.0f;
+    } else {
+        const double beta = (a.scal[HS_RHO] / a.scal[HS_RHO_OLD]) * (a.scal[HS_ALPHA] / a.scal[HS_OMEGA]);
+        pv.bt = static_cast<float>(beta);
+        pv.om = static_cast<float>(a.scal[HS_OMEGA]);
+    }
+    return pv;
+}
+
+// p = r + beta (p_old - omega v_old); v = J phat; alpha's denominator
+__global__ void __launch_bounds__(256) k_bi_pv(BiArgs a)
 {
     if (a.flags[0]) return;
     __shared__ double zs[kSH * kSW];
@@ -435,17 +511,20 @@ __global__ void __launch_bounds__(256) k_bi_v(BiArgs a)
     const uint8_t* __restrict__ dof = a.d.dof;
     const float* __restrict__ rhat = a.rhat;
     float* __restrict__ vout = a.v;
+    float* __restrict__ pout = a.p;
+    const PhatView view = phat_view(a);
     double acc[1] = {0.0};
     for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
         const int r0 = (tile / tx_n) * kTH, c0 = (tile % tx_n) * kTW;
         __syncthreads();
-        stage_tile<false, true>(a.z, nullptr, 0.0, a.phat, nr, nc, r0, c0, zs, xs);
+        stage_tile_view(a.z, view, nr, nc, r0, c0, zs, xs);
         // per-cell inputs of this thread's four rows, loaded before the
         // barrier (they do not depend on the tile)
         long long f[kTH / 4];
         bool in[kTH / 4];
         uint8_t fl[kTH / 4], dl[kTH / 4];
         float rh[kTH / 4];
+        PhatView::L pl[kTH / 4];
 #pragma unroll
         for (int k = 0; k < kTH / 4; ++k) {
             const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
@@ -454,11 +533,13 @@ __global__ void __launch_bounds__(256) k_bi_v(BiArgs a)
             fl[k] = fast[f[k]];
             dl[k] = dof[f[k]];
             rh[k] = rhat[f[k]];
+            pl[k] = view.load(f[k]);
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < kTH / 4; ++k) {
             if (!in[k]) continue;
+            pout[f[k]] = view.pval(pl[k]);
             if (fl[k]) {
                 const float vv = jx_tile(a.d, zs, xs, (threadIdx.x >> 6) + 4 * k, cl);
                 vout[f[k]] = vv;
@@ -471,7 +552,7 @@ __global__ void __launch_bounds__(256) k_bi_v(BiArgs a)
     // the general rows (grid edge, next to a masked cell): stored Jacobian rows
     for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < a.d.n_gen; gi += gridDim.x * blockDim.x) {
         const int f = a.d.gen[gi];
-        const float y = jx_gen(a.d, a.phat, gi, f);
+        const float y = jx_gen_view(a.d, view, gi, f);
         vout[f] = y;
         acc[0] += static_cast<double>(rhat[f]) * y;
     }
@@ -479,12 +560,12 @@ __global__ void __launch_bounds__(256) k_bi_v(BiArgs a)
     block_partial<1>(acc, mx, a.partials + kPartV, blockIdx.x);
 }
 
-// s = r - alpha v; shat = M^-1 s
-__global__ void __launch_bounds__(256) k_bi_s(BiArgs a)
+// alpha = rho / (rhat . v); t = J shat (s = r - alpha v); omega's sums
+__global__ void __launch_bounds__(256) k_bi_st(BiArgs a)
 {
     if (a.flags[0]) return;
-    // alpha = rho / (rhat . v): every block folds k_bi_v's partials in the
-    // same order (so all agree), block 0 keeps it for k_bi_x
+    // alpha from k_bi_pv's partials: every block folds them in the same
+    // order (so all agree), block 0 keeps it for k_bi_x
     double rv1[1];
     const bool mx1[1] = {false};
     fold_partials<1>(a.partials + kPartV, gridDim.x, mx1, rv1);
@@ -504,48 +585,27 @@ __global__ void __launch_bounds__(256) k_bi_s(BiArgs a)
     }
     __syncthreads();
     if (brk) return;
-    const float al = static_cast<float>(alpha_s);
-    const float* __restrict__ r = a.first ? a.b : a.r;
-    const float* __restrict__ v = a.v;
-    const float* __restrict__ dinv = a.dinv;
-    float* __restrict__ sv = a.s;
-    float* __restrict__ shat = a.shat;
-    const long long n4 = a.n >> 2, stride = (long long)gridDim.x * blockDim.x;
-    const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    for (long long i = t0; i < n4; i += stride) {
-        const float4 rv = ld4(r, i), vv = ld4(v, i), di = ld4(dinv, i);
-        const float4 s4 = make_float4(rv.x - al * vv.x, rv.y - al * vv.y, rv.z - al * vv.z, rv.w - al * vv.w);
-        st4(sv, i, s4);
-        st4(shat, i, make_float4(di.x * s4.x, di.y * s4.y, di.z * s4.z, di.w * s4.w));
-    }
-    for (long long f = 4 * n4 + t0; f < a.n; f += stride) {
-        const float s1 = r[f] - al * v[f];
-        sv[f] = s1;
-        shat[f] = dinv[f] * s1;
-    }
-}
-
-// t = J shat; omega = (t . s) / (t . t)
-__global__ void __launch_bounds__(256) k_bi_t(BiArgs a)
-{
-    if (a.flags[0]) return;
     __shared__ double zs[kSH * kSW];
     __shared__ float xs[kSH * kSW];
     const int nr = a.d.nr, nc = a.d.nc, tx_n = (nc + kTW - 1) / kTW, nt = tx_n * ((nr + kTH - 1) / kTH);
     const int cl = threadIdx.x & 63;
     const uint8_t* __restrict__ fast = a.d.fast;
     const uint8_t* __restrict__ dof = a.d.dof;
-    const float* __restrict__ sv = a.s;
     float* __restrict__ tout = a.t;
+    ShatView view;
+    view.r = a.first ? a.b : a.r;
+    view.v = a.v;
+    view.dinv = a.dinv;
+    view.al = static_cast<float>(alpha_s);
     double acc[2] = {0.0, 0.0};
     for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
         const int r0 = (tile / tx_n) * kTH, c0 = (tile % tx_n) * kTW;
         __syncthreads();
-        stage_tile<false, true>(a.z, nullptr, 0.0, a.shat, nr, nc, r0, c0, zs, xs);
+        stage_tile_view(a.z, view, nr, nc, r0, c0, zs, xs);
         long long f[kTH / 4];
         bool in[kTH / 4];
         uint8_t fl[kTH / 4], dl[kTH / 4];
-        float sl[kTH / 4];
+        ShatView::L sl[kTH / 4];
 #pragma unroll
         for (int k = 0; k < kTH / 4; ++k) {
             const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
@@ -553,7 +613,7 @@ __global__ void __launch_bounds__(256) k_bi_t(BiArgs a)
             f[k] = (long long)min(rr, nr - 1) * nc + min(cc, nc - 1);
             fl[k] = fast[f[k]];
             dl[k] = dof[f[k]];
-            sl[k] = sv[f[k]];
+            sl[k] = view.load(f[k]);
         }
         __syncthreads();
 #pragma unroll
@@ -562,7 +622,7 @@ __global__ void __launch_bounds__(256) k_bi_t(BiArgs a)
             if (fl[k]) {
                 const float tv = jx_tile(a.d, zs, xs, (threadIdx.x >> 6) + 4 * k, cl);
                 tout[f[k]] = tv;
-                acc[0] += static_cast<double>(tv) * sl[k];
+                acc[0] += static_cast<double>(tv) * view.sval(sl[k]);
                 acc[1] += static_cast<double>(tv) * tv;
             } else if (!dl[k]) {
                 tout[f[k]] = 0.0f;
@@ -571,9 +631,9 @@ __global__ void __launch_bounds__(256) k_bi_t(BiArgs a)
     }
     for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < a.d.n_gen; gi += gridDim.x * blockDim.x) {
         const int f = a.d.gen[gi];
-        const float tv = jx_gen(a.d, a.shat, gi, f);
+        const float tv = jx_gen_view(a.d, view, gi, f);
         tout[f] = tv;
-        acc[0] += static_cast<double>(tv) * sv[f];
+        acc[0] += static_cast<double>(tv) * view.sval(view.load(f));
         acc[1] += static_cast<double>(tv) * tv;
     }
     const bool mx[2] = {false, false};
@@ -584,7 +644,7 @@ __global__ void __launch_bounds__(256) k_bi_t(BiArgs a)
 __global__ void __launch_bounds__(256) k_bi_x(BiArgs a)
 {
     if (a.flags[0]) return;
-    // omega = (t . s) / (t . t) from k_bi_t's partials, folded by every block
+    // omega = (t . s) / (t . t) from k_bi_st's partials, folded by every block
     double tt[2];
     const bool mx2[2] = {false, false};
     fold_partials<2>(a.partials + kPartT * gridDim.x, gridDim.x, mx2, tt);
@@ -595,9 +655,10 @@ __global__ void __launch_bounds__(256) k_bi_x(BiArgs a)
     }
     __syncthreads();
     const float al = static_cast<float>(a.scal[HS_ALPHA]), om = static_cast<float>(omega_s);
-    const float* __restrict__ phat = a.phat;
-    const float* __restrict__ shat = a.shat;
-    const float* __restrict__ sv = a.s;
+    const float* __restrict__ pv = a.p;
+    const float* __restrict__ dinv = a.dinv;
+    const float* __restrict__ rin = a.first ? a.b : a.r;
+    const float* __restrict__ vv = a.v;
     const float* __restrict__ tv = a.t;
     const float* __restrict__ rhat = a.rhat;
     float* __restrict__ x = a.x;
@@ -606,8 +667,12 @@ __global__ void __launch_bounds__(256) k_bi_x(BiArgs a)
     const long long n4 = a.n >> 2, stride = (long long)gridDim.x * blockDim.x;
     const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     for (long long i = t0; i < n4; i += stride) {
-        const float4 ph = ld4(phat, i), sh = ld4(shat, i), s4 = ld4(sv, i), t4 = ld4(tv, i), rh = ld4(rhat, i);
+        const float4 p4 = ld4(pv, i), di = ld4(dinv, i), r4 = ld4(rin, i), v4 = ld4(vv, i), t4 = ld4(tv, i),
+                     rh = ld4(rhat, i);
         const float4 xo = a.first ? make_float4(0.f, 0.f, 0.f, 0.f) : ld4(x, i);
+        const float4 s4 = make_float4(r4.x - al * v4.x, r4.y - al * v4.y, r4.z - al * v4.z, r4.w - al * v4.w);
+        const float4 ph = make_float4(di.x * p4.x, di.y * p4.y, di.z * p4.z, di.w * p4.w);
+        const float4 sh = make_float4(di.x * s4.x, di.y * s4.y, di.z * s4.z, di.w * s4.w);
         const float4 dx = make_float4(al * ph.x + om * sh.x, al * ph.y + om * sh.y, al * ph.z + om * sh.z,
                                       al * ph.w + om * sh.w);
         st4(x, i, a.first ? dx : make_float4(xo.x + dx.x, xo.y + dx.y, xo.z + dx.z, xo.w + dx.w));
@@ -623,9 +688,10 @@ __global__ void __launch_bounds__(256) k_bi_x(BiArgs a)
         acc[1] += static_cast<double>(rv.w) * rv.w;
     }
     for (long long f = 4 * n4 + t0; f < a.n; f += stride) {
-        const float dx = al * phat[f] + om * shat[f];
+        const float s1 = rin[f] - al * vv[f];
+        const float dx = al * (dinv[f] * pv[f]) + om * (dinv[f] * s1);
         x[f] = a.first ? dx : x[f] + dx;
-        const float rv = sv[f] - om * tv[f];
+        const float rv = s1 - om * tv[f];
         r[f] = rv;
         acc[0] += static_cast<double>(rhat[f]) * rv;
         acc[1] += static_cast<double>(rv) * rv;
@@ -1523,12 +1589,11 @@ int resident_blocks(K k)
 
 int hs_bicgstab_grid(long long n, int cap)
 {
-    // one grid for the five kernels (each folds the previous one's block
+    // one grid for the three kernels (each folds the previous one's block
     // partials): the blocks that are all resident at once, so no block
     // waits for a second round behind the others
-    int g = resident_blocks(k_bi_v);
-    for (int v : {resident_blocks(k_bi_t), resident_blocks(k_bi_p), resident_blocks(k_bi_s), resident_blocks(k_bi_x)})
-        g = std::min(g, v);
+    int g = resident_blocks(k_bi_pv);
+    for (int v : {resident_blocks(k_bi_st), resident_blocks(k_bi_x)}) g = std::min(g, v);
     return std::max(1, std::min({g, cap, (int)std::min<long long>(div_up(n, kB), 1ll << 30)}));
 }
 
@@ -1542,22 +1607,19 @@ void hs_fast_flags(const HsDom& d, uint8_t* fast, hipStream_t s)
 void hs_bicgstab_iteration(const BiArgs& a, int grid, hipStream_t s, KernelTimer* t)
 {
     const double n = static_cast<double>(a.n);
-    if (t) t->begin("bicgstab_p", s);
-    k_bi_p<<<grid, kB, 0, s>>>(a);
-    if (t) t->end(s, 24.0 * n);  // r, p, v, dinv in; p, phat out
-    if (t) t->begin("bicgstab_matvec", s);
-    k_bi_v<<<grid, kB, 0, s>>>(a);
-    if (t) t->end(s, 22.0 * n);  // z 8, phat 4, fast 1, rhat 4 in; v 4 out (+ dof 1)
-    if (t) t->begin("bicgstab_s", s);
-    k_bi_s<<<grid, kB, 0, s>>>(a);
-    if (t) t->end(s, 20.0 * n);
-    if (t) t->begin("bicgstab_matvec", s);
-    k_bi_t<<<grid, kB, 0, s>>>(a);
-    if (t) t->end(s, 22.0 * n);
+    if (t) t->begin("bicgstab_pv", s);
+    k_bi_pv<<<grid, kB, 0, s>>>(a);
+    // z 8, r 4, p_old 4, v_old 4, dinv 4, fast 1, dof 1, rhat 4 in; p 4, v 4 out
+    if (t) t->end(s, 38.0 * n);
+    if (t) t->begin("bicgstab_st", s);
+    k_bi_st<<<grid, kB, 0, s>>>(a);
+    // z 8, r 4, v 4, dinv 4, fast 1, dof 1 in; t 4 out
+    if (t) t->end(s, 26.0 * n);
     if (t) t->begin("bicgstab_x", s);
     k_bi_x<<<grid, kB, 0, s>>>(a);
     k_bi_x_fin<<<1, kB, 0, s>>>(a, (unsigned)grid);
-    if (t) t->end(s, 32.0 * n);
+    // p 4, dinv 4, r 4, v 4, t 4, rhat 4, x 4 in; x 4, r 4 out
+    if (t) t->end(s, 36.0 * n);
     LSDTT_LAUNCH_CHECK();
 }
 

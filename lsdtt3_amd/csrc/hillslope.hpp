@@ -52,7 +52,10 @@ struct BiArgs {
     const double* z;     // Newton iterate (the Jacobian is taken here)
     const float* dinv;
     const float* b;
-    float *x, *r, *rhat, *p, *phat, *v, *s, *shat, *t;
+    // p and v alternate between two buffers from one iteration to the next:
+    // an iteration reads the previous p and v (p_old, v_old) around every
+    // tile it stages while other tiles already write the new ones
+    float *x, *r, *rhat, *p_old, *p, *v_old, *v, *t;
     double* scal;
     double* partials;
     unsigned* ticket;

@@ -616,11 +616,6 @@ int LemEngine::bicgstab(hipStream_t s, int newton_it, double eta)
     a.x = x.p;
     a.r = r.p;
     a.rhat = b.p;  // rhat = r_0 = b, unchanged during the solve
-    a.p = pv.p;
-    a.phat = phat.p;
-    a.v = v.p;
-    a.s = sv.p;
-    a.shat = shat.p;
     a.t = t.p;
     a.scal = scal.p;
     a.partials = partials.p;
@@ -643,6 +638,12 @@ int LemEngine::bicgstab(hipStream_t s, int newton_it, double eta)
     for (int done = 0, round = 0; !done; ++round, chunk = round == 1 ? 1 : std::min(2 * chunk, 16)) {
         for (int k = 0; k < chunk; ++k) {
             BiArgs ai = a;
+            // p and v alternate between two buffers per iteration
+            const bool odd = it_launched & 1;
+            ai.p_old = odd ? pv.p : pv2.p;
+            ai.p = odd ? pv2.p : pv.p;
+            ai.v_old = odd ? v.p : v2.p;
+            ai.v = odd ? v2.p : v.p;
             ai.first = (it_launched++ == 0) ? 1 : 0;
             hs_bicgstab_iteration(ai, hs_grid, s, timer.enabled ? &timer : nullptr);
         }
@@ -785,7 +786,7 @@ double* LemEngine::trial_target() const
 int LemEngine::hillslope(LemStats* st, hipStream_t s)
 {
     const size_t nn = static_cast<size_t>(n);
-    for (auto* bf : {&dinv, &b, &x, &r, &pv, &phat, &v, &sv, &shat, &t}) bf->reserve(nn);
+    for (auto* bf : {&dinv, &b, &x, &r, &pv, &pv2, &v, &v2, &t}) bf->reserve(nn);
     zn_hs.reserve(nn);
     z_trial.reserve(nn);
     z_hs0.reserve(nn);

@@ -150,7 +150,7 @@ public:
     double* zc = nullptr;
     double* zt = nullptr;
     double* trial_target() const;
-    dbuf<float> dinv, b, x, r, pv, phat, v, sv, shat, t;
+    dbuf<float> dinv, b, x, r, pv, pv2, v, v2, t;  // BiCGSTAB (p and v in two buffers each)
     dbuf<float> c9g;  // general DOFs' Jacobian rows (9 x n_gen)
     dbuf<double> partials, scal;  // reduction partials; scalars
     hbuf<double> hpin;            // pinned host slots for read-backs
