@@ -54,6 +54,20 @@ __device__ __forceinline__ int local_index(int g, int nc, int r0, int c0)
     return (gr >= r0 && gr < r0 + TH && gc >= c0 && gc < c0 + TW) ? (gr - r0) * TW + (gc - c0) : -1;
 }
 
+// local index of the receiver g of this lane's cell k (-1 outside the
+// tile), from its D8 code d: the geometric neighbour when g is that cell
+// (no division), else (a step across a periodic edge) from g itself
+template <int TH>
+__device__ __forceinline__ int local_receiver(int g, int d, int lane, int k, int nc, int r0, int c0)
+{
+    const int lr = 2 * k + (lane >> 5), lc = lane & 31;
+    const int rr = lr + static_cast<int>((kRowBits >> (2 * (d & 7))) & 3u) - 1;
+    const int cc = lc + static_cast<int>((kColBits >> (2 * (d & 7))) & 3u) - 1;
+    if ((r0 + rr) * nc + (c0 + cc) == g)
+        return (rr >= 0 && rr < TH && cc >= 0 && cc < TW) ? rr * TW + cc : -1;
+    return local_index<TH>(g, nc, r0, c0);
+}
+
 template <int TH>
 __device__ __forceinline__ int tile_of(int g, int nc, int ntx)
 {
@@ -512,8 +526,14 @@ __global__ void __launch_bounds__(WL) kb_tile_counts(BatchTiles B, const int* __
 // that is not skipped
 // NM: 0 n == 1, 1 n - 1 == 0.5 (sqrt), 2 general (pow): a kernel per mode so
 // that the common ones carry no pow registers
+// Newton (n != 1): the per-node constants are formed once, the slope is a
+// product by 1/dx_eff and the correction a product by the reciprocal of
+// f'(z) (one refinement of the hardware estimate, ~1e-16 relative): the
+// iterates stop at the same |eps| <= tol as the reference's divisions
+// would, within rounding of the root.
 template <int NM>
-__device__ __forceinline__ double sp_solve(const SpParams& a, double zo, double zr, double kam, double dx_eff)
+__device__ __forceinline__ double sp_solve(const SpParams& a, double zo, double zr, double kam, double dx_eff,
+                                           double inv_dx_eff)
 {
     const double tol = a.slope_clamp * dx_eff;
     double zn = zo;
@@ -522,19 +542,23 @@ __device__ __forceinline__ double sp_solve(const SpParams& a, double zo, double 
         zn = (zo + zr * spf) / (1.0 + spf);
         if ((zn - zo) > tol || isnan(zn)) zn = zo;
     } else {
+        const double kdt = kam * a.dt;
+        const double kdtn = kdt * (a.nexp * inv_dx_eff);
         double eps = tol + 1.0;
         int it = 0;
         do {
-            const double slope = (zn - zr) / dx_eff;
+            const double slope = (zn - zr) * inv_dx_eff;
             if (slope <= 0.0) {
                 eps = 0.0;
             } else {
                 // S^n = S^(n-1) * S: one pow per iteration; sqrt is the
                 // correctly rounded pow(S, 0.5)
                 const double pn1 = NM == 1 ? sqrt(slope) : pow(slope, a.nexp - 1.0);
-                const double fz = zn - zo + kam * a.dt * (pn1 * slope);
-                const double fd = 1.0 + kam * a.dt * (a.nexp / dx_eff) * pn1;
-                eps = fz / fd;
+                const double fz = zn - zo + kdt * (pn1 * slope);
+                const double fd = 1.0 + kdtn * pn1;
+                double rf = __builtin_amdgcn_rcp(fd);
+                rf = fma(fma(-fd, rf, 1.0), rf, rf);
+                eps = fz * rf;
             }
             zn -= eps;
             ++it;
@@ -628,7 +652,7 @@ __device__ __forceinline__ void tile_sp_body(const TileAt& T, const SpTileArgs& 
             if (!(p.threshold > 0.0 && area < p.threshold)) {
                 // K_i * width_factor * pow(area, m) (lsdtt_stream_power_cpu.cpp:258-260)
                 km = Kf[k] * p.width * (MH ? sqrt(area) : pow(area, p.m));
-                const int li = local_index<TH>(g[k], nc, r0, c0);
+                const int li = local_receiver<TH>(g[k], dr[k], lane, k, nc, r0, c0);
                 l = li >= 0 ? static_cast<short>(li) : kOutRcv;
             }
         }
@@ -644,6 +668,7 @@ __device__ __forceinline__ void tile_sp_body(const TileAt& T, const SpTileArgs& 
         ord[i] = ordv[k];
     }
     int lb = lvl.at(0);
+    const double dxd = p.dx * kSqrt2, inv_dx = 1.0 / p.dx, inv_dxd = 1.0 / dxd;
     __syncthreads();
     int eroded = 0;
     for (int L = 0; L <= maxdep; ++L) {
@@ -668,14 +693,16 @@ __device__ __forceinline__ void tile_sp_body(const TileAt& T, const SpTileArgs& 
                 zr = __hip_atomic_load(&A.z[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             if (RR) {
-                const double v = zr < z0 ? sp_solve<NM>(p, z0, zr, kam[i], (dcode & 1) ? p.dx * kSqrt2 : p.dx) : z0;
+                const double v =
+                    zr < z0 ? sp_solve<NM>(p, z0, zr, kam[i], (dcode & 1) ? dxd : p.dx, (dcode & 1) ? inv_dxd : inv_dx)
+                            : z0;
                 eroded += zr < z0;
                 if (__double_as_longlong(v) != __double_as_longlong(val[i])) {
                     val[i] = v;
                     chg[i] = 1;
                 }
             } else if (zr < z0) {
-                val[i] = sp_solve<NM>(p, z0, zr, kam[i], (dcode & 1) ? p.dx * kSqrt2 : p.dx);
+                val[i] = sp_solve<NM>(p, z0, zr, kam[i], (dcode & 1) ? dxd : p.dx, (dcode & 1) ? inv_dxd : inv_dx);
                 ++eroded;
             }
         }
