@@ -555,7 +555,7 @@ void BatchEngine::build_layout(const std::vector<BatchSpec>& specs)
     cnt.reserve(nn);
     absdz.reserve(nn);
     if (any_hillslope)
-        for (auto* a : {&b, &dinv, &x, &r, &pv, &phat, &v, &sv, &shat, &t}) a->reserve(nn);
+        for (auto* a : {&b, &dinv, &x, &r, &pv, &pv2, &v, &v2, &t}) a->reserve(nn);
     gen.reserve(std::max<size_t>(1, gen_h.size()));
     c9g.reserve(std::max<size_t>(9, 9 * gen_h.size()));
     if (!gen_h.empty()) LSDTT_HIP(hipMemcpy(gen.p, gen_h.data(), sizeof(int) * gen_h.size(), hipMemcpyHostToDevice));
@@ -694,10 +694,9 @@ BatchHs BatchEngine::bhs()
     B.x = x.p;
     B.r = r.p;
     B.p = pv.p;
-    B.phat = phat.p;
+    B.p2 = pv2.p;
     B.v = v.p;
-    B.s = sv.p;
-    B.shat = shat.p;
+    B.v2 = v2.p;
     B.t = t.p;
     B.partials = partials.p;
     B.ws_dz = ws_dz.p;

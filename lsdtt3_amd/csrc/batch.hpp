@@ -129,7 +129,7 @@ private:
     dbuf<double> z, z_before, zA, zB, zC;
     dbuf<uint8_t> mask, dof, fast, cflag, dir, xdon;
     dbuf<int> rcv, cnt, gen;
-    dbuf<float> c9g, b, dinv, x, r, pv, phat, v, sv, shat, t, absdz, ero_rate;
+    dbuf<float> c9g, b, dinv, x, r, pv, pv2, v, v2, t, absdz, ero_rate;
     dbuf<float> ws_dz;  // warm start: the last step's hillslope change (allocated with the mode)
     dbuf<int> ws_ok;    // per grid: ws_dz is valid
     dbuf<BatchStepGrid> step_grids;

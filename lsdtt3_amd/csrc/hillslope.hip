@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "hillslope.hpp"
 #include "ktimer.hpp"
@@ -1091,33 +1092,47 @@ __device__ __forceinline__ void tile_cells(const BatchHsGrid& g, int r0, int c0,
     }
 }
 
-// p = r + beta (p - omega v); phat = M^-1 p  (first iteration: p = b)
-__device__ __forceinline__ void bi_p_tile(const BatchHs& B, int slot)
+// a grid's Krylov vectors for its current iteration: p and v are read from
+// the pair of the previous iteration and written to the other
+struct BatchKrylov {
+    const float *p_old, *v_old;
+    float *p, *v;
+};
+__device__ __forceinline__ BatchKrylov batch_krylov(const BatchHs& B, const HsCtl& c, long long off)
 {
-    const BTileAt T = btile(B, slot);
-    const HsCtl& c = B.ctl[T.gi];
-    if (!krylov_active(c)) return;
-    const BatchHsGrid& g = B.grids[T.gi];
-    const double beta = (c.rho / c.rho_old) * (c.alpha / c.omega);
-    const float bt = static_cast<float>(beta), om = static_cast<float>(c.omega);
-    const bool first = c.kits == 0;
-    const float* b = B.b + g.off;
-    const float* r = B.r + g.off;
-    const float* v = B.v + g.off;
-    const float* dinv = B.dinv + g.off;
-    float* p = B.p + g.off;
-    float* phat = B.phat + g.off;
-    tile_cells(g, T.r0, T.c0, [&](long long f) {
-        const float pv = first ? b[f] : r[f] + bt * (p[f] - om * v[f]);
-        p[f] = pv;
-        phat[f] = dinv[f] * pv;
-    });
+    const bool odd = c.kits & 1;
+    return BatchKrylov{(odd ? B.p2 : B.p) + off, (odd ? B.v2 : B.v) + off, (odd ? B.p : B.p2) + off,
+                       (odd ? B.v : B.v2) + off};
 }
 
-__global__ void __launch_bounds__(256) kb_bi_p(BatchHs B)
+// the views of the single-grid kernels on one grid of the batch
+__device__ __forceinline__ PhatView batch_phat_view(const BatchHs& B, const HsCtl& c, const BatchHsGrid& g)
 {
-    const int total = wl_total(B, WL_KTILE);
-    for (int w = blockIdx.x; w < total; w += gridDim.x) bi_p_tile(B, wl_slot(B, WL_KTILE, w, false));
+    const BatchKrylov k = batch_krylov(B, c, g.off);
+    PhatView pv;
+    pv.b = B.b + g.off;
+    pv.r = B.r + g.off;
+    pv.p = k.p_old;
+    pv.v = k.v_old;
+    pv.dinv = B.dinv + g.off;
+    pv.first = c.kits == 0;
+    if (pv.first) {
+        pv.bt = pv.om = 0.0f;
+    } else {
+        pv.bt = static_cast<float>((c.rho / c.rho_old) * (c.alpha / c.omega));
+        pv.om = static_cast<float>(c.omega);
+    }
+    return pv;
+}
+
+__device__ __forceinline__ ShatView batch_shat_view(const BatchHs& B, const HsCtl& c, const BatchHsGrid& g)
+{
+    ShatView sv;
+    sv.r = (c.kits == 0 ? B.b : B.r) + g.off;
+    sv.v = batch_krylov(B, c, g.off).v;
+    sv.dinv = B.dinv + g.off;
+    sv.al = static_cast<float>(c.alpha);
+    return sv;
 }
 
 // y = J x on the tile's fast rows (0 off the DOFs) and dot(w, y); the
@@ -1136,12 +1151,19 @@ __device__ __forceinline__ void dot_acc(double (&acc)[NV], float y, float w)
     }
 }
 
-template <int NV>
-__device__ __forceinline__ void batch_matvec(const BatchHs& B, bool chunk, int slot, const float* xin, float* yout,
-                                             const float* win, double (&acc)[NV])
+// y = J x with x given by the grid's view (PhatView: also writes p over
+// the tile; ShatView: the dot products take s from the view); the general
+// rows come from the chunk items
+template <int NV, bool PV>
+__device__ __forceinline__ void batch_matvec(const BatchHs& B, bool chunk, int slot, double (&acc)[NV])
 {
     __shared__ double zs[kSH * kSW];
     __shared__ float xs[kSH * kSW];
+    using View = typename std::conditional<PV, PhatView, ShatView>::type;
+    auto make_view = [&](const HsCtl& c, const BatchHsGrid& g) {
+        if constexpr (PV) return batch_phat_view(B, c, g);
+        else return batch_shat_view(B, c, g);
+    };
     if (!chunk) {
         const BTileAt T = btile(B, slot);
         const HsCtl& c = B.ctl[T.gi];
@@ -1149,19 +1171,21 @@ __device__ __forceinline__ void batch_matvec(const BatchHs& B, bool chunk, int s
         const BatchHsGrid& g = B.grids[T.gi];
         const HsDom d = batch_dom(B, g, c.inv_dt, c.zn);
         const double* z = B.zbuf[c.zc] + g.off;
-        const float* x = xin + g.off;
-        const float* wv = win + g.off;
-        float* y = yout + g.off;
+        const View view = make_view(c, g);
+        const BatchKrylov kv = batch_krylov(B, c, g.off);
+        const float* rhat = B.b + g.off;
+        float* y = PV ? kv.v : B.t + g.off;
         const int nr = g.nr, nc = g.nc, c0 = T.c0, cl = threadIdx.x & 63;
         for (int band = 0; band < kBands; ++band) {
             const int r0 = T.r0 + band * kTH;
             if (r0 >= nr) break;
             __syncthreads();
-            stage_tile<false, true>(z, nullptr, 0.0, x, nr, nc, r0, c0, zs, xs);
+            stage_tile_view(z, view, nr, nc, r0, c0, zs, xs);
             long long f[kTH / 4];
             bool in[kTH / 4];
             uint8_t fl[kTH / 4], dl[kTH / 4];
             float wl[kTH / 4];
+            typename View::L vl[kTH / 4];
 #pragma unroll
             for (int k = 0; k < kTH / 4; ++k) {
                 const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
@@ -1169,16 +1193,24 @@ __device__ __forceinline__ void batch_matvec(const BatchHs& B, bool chunk, int s
                 f[k] = (long long)min(rr, nr - 1) * nc + min(cc, nc - 1);
                 fl[k] = d.fast[f[k]];
                 dl[k] = d.dof[f[k]];
-                wl[k] = wv[f[k]];
+                if constexpr (PV) wl[k] = rhat[f[k]];
+                vl[k] = view.load(f[k]);
             }
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < kTH / 4; ++k) {
                 if (!in[k]) continue;
+                float w;
+                if constexpr (PV) {
+                    kv.p[f[k]] = view.pval(vl[k]);
+                    w = wl[k];
+                } else {
+                    w = view.sval(vl[k]);
+                }
                 if (fl[k]) {
                     const float yv = jx_tile(d, zs, xs, (threadIdx.x >> 6) + 4 * k, cl);
                     y[f[k]] = yv;
-                    dot_acc<NV>(acc, yv, wl[k]);
+                    dot_acc<NV>(acc, yv, w);
                 } else if (!dl[k]) {
                     y[f[k]] = 0.0f;
                 }
@@ -1192,10 +1224,14 @@ __device__ __forceinline__ void batch_matvec(const BatchHs& B, bool chunk, int s
         const HsDom d = batch_dom(B, g, c.inv_dt, c.zn);
         const int i = (slot - g.gcbase) * 256 + threadIdx.x;
         if (i < d.n_gen) {
+            const View view = make_view(c, g);
             const int f = d.gen[i];
-            const float w = win[g.off + f];
-            const float yv = jx_gen(d, xin + g.off, i, f);
-            yout[g.off + f] = yv;
+            float w;
+            if constexpr (PV) w = B.b[g.off + f];
+            else w = view.sval(view.load(f));
+            const float yv = jx_gen_view(d, view, i, f);
+            if constexpr (PV) batch_krylov(B, c, g.off).v[f] = yv;
+            else B.t[g.off + f] = yv;
             dot_acc<NV>(acc, yv, w);
         }
     }
@@ -1215,21 +1251,21 @@ __device__ __forceinline__ void matvec_items(const BatchHs& B, F&& fn)
     }
 }
 
-// v = J phat, partial rhat . v (rhat = b)
+// p = r + beta (p_old - omega v_old); v = J phat, partial rhat . v (rhat = b)
 __global__ void __launch_bounds__(256) kb_bi_v(BatchHs B)
 {
     matvec_items(B, [&](bool chunk, int slot) {
         double acc[1] = {0.0};
-        batch_matvec<1>(B, chunk, slot, B.phat, B.v, B.b, acc);
+        batch_matvec<1, true>(B, chunk, slot, acc);
     });
 }
 
-// t = J shat, partials t . s and t . t
+// t = J shat (s = r - alpha v), partials t . s and t . t
 __global__ void __launch_bounds__(256) kb_bi_t(BatchHs B)
 {
     matvec_items(B, [&](bool chunk, int slot) {
         double acc[2] = {0.0, 0.0};
-        batch_matvec<2>(B, chunk, slot, B.shat, B.t, B.s, acc);
+        batch_matvec<2, false>(B, chunk, slot, acc);
     });
 }
 
@@ -1249,32 +1285,6 @@ __global__ void __launch_bounds__(256) kb_bi_vfin(BatchHs B)
             c.alpha = c.rho / out[0];
         }
     }
-}
-
-// s = r - alpha v; shat = M^-1 s
-__device__ __forceinline__ void bi_s_tile(const BatchHs& B, int slot)
-{
-    const BTileAt T = btile(B, slot);
-    const HsCtl& c = B.ctl[T.gi];
-    if (!krylov_active(c)) return;
-    const BatchHsGrid& g = B.grids[T.gi];
-    const float al = static_cast<float>(c.alpha);
-    const float* r = (c.kits == 0 ? B.b : B.r) + g.off;
-    const float* v = B.v + g.off;
-    const float* dinv = B.dinv + g.off;
-    float* sv = B.s + g.off;
-    float* shat = B.shat + g.off;
-    tile_cells(g, T.r0, T.c0, [&](long long f) {
-        const float s1 = r[f] - al * v[f];
-        sv[f] = s1;
-        shat[f] = dinv[f] * s1;
-    });
-}
-
-__global__ void __launch_bounds__(256) kb_bi_s(BatchHs B)
-{
-    const int total = wl_total(B, WL_KTILE);
-    for (int w = blockIdx.x; w < total; w += gridDim.x) bi_s_tile(B, wl_slot(B, WL_KTILE, w, false));
 }
 
 // omega = (t . s) / (t . t) per grid
@@ -1297,18 +1307,23 @@ __device__ __forceinline__ void bi_x_tile(const BatchHs& B, int slot)
     const BatchHsGrid& g = B.grids[T.gi];
     const float al = static_cast<float>(c.alpha), om = static_cast<float>(c.omega);
     const bool first = c.kits == 0;
-    const float* phat = B.phat + g.off;
-    const float* shat = B.shat + g.off;
-    const float* sv = B.s + g.off;
+    const BatchKrylov kv = batch_krylov(B, c, g.off);
+    const float* pv = kv.p;
+    const float* vv = kv.v;
+    const float* dinv = B.dinv + g.off;
+    const float* rin = (first ? B.b : B.r) + g.off;
     const float* tv = B.t + g.off;
     const float* rhat = B.b + g.off;
     float* x = B.x + g.off;
     float* r = B.r + g.off;
     double acc[2] = {0.0, 0.0};
     tile_cells(g, T.r0, T.c0, [&](long long f) {
-        const float dx = al * phat[f] + om * shat[f];
+        // phat = M^-1 p, s = r - alpha v, shat = M^-1 s: the expressions of
+        // the products' views
+        const float s1 = rin[f] - al * vv[f];
+        const float dx = al * (dinv[f] * pv[f]) + om * (dinv[f] * s1);
         x[f] = first ? dx : x[f] + dx;
-        const float rv = sv[f] - om * tv[f];
+        const float rv = s1 - om * tv[f];
         r[f] = rv;
         acc[0] += static_cast<double>(rhat[f]) * rv;
         acc[1] += static_cast<double>(rv) * rv;
@@ -1657,10 +1672,8 @@ void hsb_round(const BatchHs& B, int n_grids, hipStream_t s, int tiles, int chun
     const int nt = tiles >= 0 ? std::min(tiles, B.NT) : B.NT;
     const int ng = chunks >= 0 ? std::min(chunks, B.NGC) : B.NGC;
     const int gt = std::max(1, nt), gc = std::max(1, ng), gm = std::max(1, nt + ng);
-    kb_bi_p<<<gt, kB, 0, s>>>(B);
     kb_bi_v<<<gm, kB, 0, s>>>(B);
     kb_bi_vfin<<<n_grids, kB, 0, s>>>(B);
-    kb_bi_s<<<gt, kB, 0, s>>>(B);
     kb_bi_t<<<gm, kB, 0, s>>>(B);
     kb_bi_tfin<<<n_grids, kB, 0, s>>>(B);
     kb_bi_x<<<gt, kB, 0, s>>>(B);

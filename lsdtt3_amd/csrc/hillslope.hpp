@@ -177,7 +177,9 @@ struct BatchHs {
     const int* gen;  // per grid: its general DOFs (cell indices within the grid), row-major
     float* c9g;      // per grid: 9 x n_gen Jacobian rows at 9 * gen_off, slot-major
     double* zbuf[4];
-    float *b, *dinv, *x, *r, *p, *phat, *v, *s, *shat, *t;
+    // p and v each in two buffers: a grid's iteration k reads the pair
+    // written at k - 1 (p2, v2 when k is odd) and writes the other
+    float *b, *dinv, *x, *r, *p, *p2, *v, *v2, *t;
     double* partials;  // (NT + NGC) slots of up to 3 values
     const float* ws_dz;  // per cell: the last step's hillslope change (warm start)
     const int* ws_ok;    // per grid: ws_dz holds a converged step's change
