@@ -424,6 +424,7 @@ struct PhatView {
     }
     __device__ __forceinline__ float pval(const L& l) const { return first ? l.a0 : l.a0 + bt * (l.a1 - om * l.a2); }
     __device__ __forceinline__ float value(const L& l) const { return l.di * pval(l); }
+    __device__ __forceinline__ float raw(const L& l) const { return pval(l); }
 };
 
 struct ShatView {
@@ -442,13 +443,17 @@ struct ShatView {
     }
     __device__ __forceinline__ float sval(const L& l) const { return l.a0 - al * l.a1; }
     __device__ __forceinline__ float value(const L& l) const { return l.di * sval(l); }
+    __device__ __forceinline__ float raw(const L& l) const { return sval(l); }
 };
 
 // stage z and the view's values over the tile plus halo (stage_tile's
-// clamped, branch-free loads)
+// clamped, branch-free loads); rs receives the view's vector before the
+// preconditioner (p or s), which the tile's own cells read back from LDS
+// rather than loading their inputs a second time
 template <class V>
 __device__ __forceinline__ void stage_tile_view(const double* __restrict__ z, const V& view, int nr, int nc, int r0,
-                                                int c0, double* __restrict__ zs, float* __restrict__ xs)
+                                                int c0, double* __restrict__ zs, float* __restrict__ xs,
+                                                float* __restrict__ rs)
 {
     constexpr int kStage = (kSH * kSW + kB - 1) / kB;
     double zv[kStage];
@@ -467,6 +472,7 @@ __device__ __forceinline__ void stage_tile_view(const double* __restrict__ z, co
         if (i >= kSH * kSW) continue;
         zs[i] = zv[j];
         xs[i] = view.value(lv[j]);
+        rs[i] = view.raw(lv[j]);
     }
 }
 
@@ -505,7 +511,7 @@ __global__ void __launch_bounds__(256) k_bi_pv(BiArgs a)
 {
     if (a.flags[0]) return;
     __shared__ double zs[kSH * kSW];
-    __shared__ float xs[kSH * kSW];
+    __shared__ float xs[kSH * kSW], ps[kSH * kSW];
     const int nr = a.d.nr, nc = a.d.nc, tx_n = (nc + kTW - 1) / kTW, nt = tx_n * ((nr + kTH - 1) / kTH);
     const int cl = threadIdx.x & 63;
     const uint8_t* __restrict__ fast = a.d.fast;
@@ -518,14 +524,13 @@ __global__ void __launch_bounds__(256) k_bi_pv(BiArgs a)
     for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
         const int r0 = (tile / tx_n) * kTH, c0 = (tile % tx_n) * kTW;
         __syncthreads();
-        stage_tile_view(a.z, view, nr, nc, r0, c0, zs, xs);
+        stage_tile_view(a.z, view, nr, nc, r0, c0, zs, xs, ps);
         // per-cell inputs of this thread's four rows, loaded before the
         // barrier (they do not depend on the tile)
         long long f[kTH / 4];
         bool in[kTH / 4];
         uint8_t fl[kTH / 4], dl[kTH / 4];
         float rh[kTH / 4];
-        PhatView::L pl[kTH / 4];
 #pragma unroll
         for (int k = 0; k < kTH / 4; ++k) {
             const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
@@ -534,13 +539,12 @@ __global__ void __launch_bounds__(256) k_bi_pv(BiArgs a)
             fl[k] = fast[f[k]];
             dl[k] = dof[f[k]];
             rh[k] = rhat[f[k]];
-            pl[k] = view.load(f[k]);
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < kTH / 4; ++k) {
             if (!in[k]) continue;
-            pout[f[k]] = view.pval(pl[k]);
+            pout[f[k]] = ps[((threadIdx.x >> 6) + 4 * k + 1) * kSW + cl + 1];
             if (fl[k]) {
                 const float vv = jx_tile(a.d, zs, xs, (threadIdx.x >> 6) + 4 * k, cl);
                 vout[f[k]] = vv;
@@ -587,7 +591,7 @@ __global__ void __launch_bounds__(256) k_bi_st(BiArgs a)
     __syncthreads();
     if (brk) return;
     __shared__ double zs[kSH * kSW];
-    __shared__ float xs[kSH * kSW];
+    __shared__ float xs[kSH * kSW], ss[kSH * kSW];
     const int nr = a.d.nr, nc = a.d.nc, tx_n = (nc + kTW - 1) / kTW, nt = tx_n * ((nr + kTH - 1) / kTH);
     const int cl = threadIdx.x & 63;
     const uint8_t* __restrict__ fast = a.d.fast;
@@ -602,11 +606,10 @@ __global__ void __launch_bounds__(256) k_bi_st(BiArgs a)
     for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
         const int r0 = (tile / tx_n) * kTH, c0 = (tile % tx_n) * kTW;
         __syncthreads();
-        stage_tile_view(a.z, view, nr, nc, r0, c0, zs, xs);
+        stage_tile_view(a.z, view, nr, nc, r0, c0, zs, xs, ss);
         long long f[kTH / 4];
         bool in[kTH / 4];
         uint8_t fl[kTH / 4], dl[kTH / 4];
-        ShatView::L sl[kTH / 4];
 #pragma unroll
         for (int k = 0; k < kTH / 4; ++k) {
             const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
@@ -614,7 +617,6 @@ __global__ void __launch_bounds__(256) k_bi_st(BiArgs a)
             f[k] = (long long)min(rr, nr - 1) * nc + min(cc, nc - 1);
             fl[k] = fast[f[k]];
             dl[k] = dof[f[k]];
-            sl[k] = view.load(f[k]);
         }
         __syncthreads();
 #pragma unroll
@@ -623,7 +625,7 @@ __global__ void __launch_bounds__(256) k_bi_st(BiArgs a)
             if (fl[k]) {
                 const float tv = jx_tile(a.d, zs, xs, (threadIdx.x >> 6) + 4 * k, cl);
                 tout[f[k]] = tv;
-                acc[0] += static_cast<double>(tv) * view.sval(sl[k]);
+                acc[0] += static_cast<double>(tv) * ss[((threadIdx.x >> 6) + 4 * k + 1) * kSW + cl + 1];
                 acc[1] += static_cast<double>(tv) * tv;
             } else if (!dl[k]) {
                 tout[f[k]] = 0.0f;
@@ -1158,7 +1160,7 @@ template <int NV, bool PV>
 __device__ __forceinline__ void batch_matvec(const BatchHs& B, bool chunk, int slot, double (&acc)[NV])
 {
     __shared__ double zs[kSH * kSW];
-    __shared__ float xs[kSH * kSW];
+    __shared__ float xs[kSH * kSW], rs[kSH * kSW];
     using View = typename std::conditional<PV, PhatView, ShatView>::type;
     auto make_view = [&](const HsCtl& c, const BatchHsGrid& g) {
         if constexpr (PV) return batch_phat_view(B, c, g);
@@ -1180,12 +1182,11 @@ __device__ __forceinline__ void batch_matvec(const BatchHs& B, bool chunk, int s
             const int r0 = T.r0 + band * kTH;
             if (r0 >= nr) break;
             __syncthreads();
-            stage_tile_view(z, view, nr, nc, r0, c0, zs, xs);
+            stage_tile_view(z, view, nr, nc, r0, c0, zs, xs, rs);
             long long f[kTH / 4];
             bool in[kTH / 4];
             uint8_t fl[kTH / 4], dl[kTH / 4];
             float wl[kTH / 4];
-            typename View::L vl[kTH / 4];
 #pragma unroll
             for (int k = 0; k < kTH / 4; ++k) {
                 const int rr = r0 + (threadIdx.x >> 6) + 4 * k, cc = c0 + cl;
@@ -1194,18 +1195,18 @@ __device__ __forceinline__ void batch_matvec(const BatchHs& B, bool chunk, int s
                 fl[k] = d.fast[f[k]];
                 dl[k] = d.dof[f[k]];
                 if constexpr (PV) wl[k] = rhat[f[k]];
-                vl[k] = view.load(f[k]);
             }
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < kTH / 4; ++k) {
                 if (!in[k]) continue;
+                const float own = rs[((threadIdx.x >> 6) + 4 * k + 1) * kSW + cl + 1];  // p or s of the cell
                 float w;
                 if constexpr (PV) {
-                    kv.p[f[k]] = view.pval(vl[k]);
+                    kv.p[f[k]] = own;
                     w = wl[k];
                 } else {
-                    w = view.sval(vl[k]);
+                    w = own;
                 }
                 if (fl[k]) {
                     const float yv = jx_tile(d, zs, xs, (threadIdx.x >> 6) + 4 * k, cl);
