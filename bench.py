@@ -732,7 +732,7 @@ def main():
     ap.add_argument("--train-size", type=int, default=2048, help="learned: training LEM grid side")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
+        print(f"bench.py runs on one GPU only: the library has no multi-GPU path (asked for {args.gpus}); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 

@@ -63,8 +63,24 @@ __device__ __forceinline__ void st_relaxed(unsigned long long* p, unsigned long 
 // ---------------------------------------------------------------------------
 constexpr int kRecvRows = 4;
 
-__device__ __forceinline__ void write_receiver(int f, int r, int d, int* __restrict__ rcv, uint8_t* __restrict__ dir)
+__device__ __forceinline__ void mark_tile(const TileMark& m, int g)
 {
+    const int r = g / m.nc, c = g - r * m.nc;
+    m.dirty[(r / m.th) * m.ntx + c / 32] = 1;
+}
+
+// old: the receiver stored before (loaded with the kernel's other inputs
+// when mark.dirty is set)
+__device__ __forceinline__ void write_receiver(int f, int r, int d, int* __restrict__ rcv, uint8_t* __restrict__ dir,
+                                               const TileMark& mark, int old)
+{
+    if (mark.dirty) {
+        if (old != r) {
+            mark_tile(mark, f);
+            if (old >= 0) mark_tile(mark, old);
+            if (r >= 0) mark_tile(mark, r);
+        }
+    }
     rcv[f] = r;
     dir[f] = static_cast<uint8_t>(d);
 }
@@ -76,15 +92,15 @@ __device__ __forceinline__ void decide_interior(const uint8_t* __restrict__ forc
                                                 double card, double diag, int nodata_outlet, int pits_outlet,
                                                 uint8_t mc, T zcell, const uint8_t (&mk)[8], const T (&zn)[8],
                                                 int* __restrict__ rcv, uint8_t* __restrict__ dir,
-                                                int* __restrict__ fail)
+                                                int* __restrict__ fail, const TileMark& mark, int old)
 {
     const int f = row * nc + col;
     if (!mc) {
-        write_receiver(f, kNodataReceiver, kNoDirection, rcv, dir);
+        write_receiver(f, kNodataReceiver, kNoDirection, rcv, dir, mark, old);
         return;
     }
     if (forced && forced[f]) {
-        write_receiver(f, kOutletReceiver, kNoDirection, rcv, dir);
+        write_receiver(f, kOutletReceiver, kNoDirection, rcv, dir, mark, old);
         return;
     }
     constexpr int kdr[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
@@ -108,9 +124,9 @@ __device__ __forceinline__ void decide_interior(const uint8_t* __restrict__ forc
         }
     }
     if (bdir != kNoDirection) {
-        write_receiver(f, f + kdr[bdir] * nc + kdc[bdir], bdir, rcv, dir);
+        write_receiver(f, f + kdr[bdir] * nc + kdc[bdir], bdir, rcv, dir, mark, old);
     } else {
-        write_receiver(f, kOutletReceiver, kNoDirection, rcv, dir);
+        write_receiver(f, kOutletReceiver, kNoDirection, rcv, dir, mark, old);
         if (!allowed && !pits_outlet) atomicMin(fail, f);
     }
 }
@@ -120,7 +136,7 @@ __global__ void __launch_bounds__(256) k_receivers(const T* __restrict__ z, cons
                                                    const uint8_t* __restrict__ forced, int nr, int nc,
                                                    double card, double diag, int nodata_outlet, int pits_outlet,
                                                    int* __restrict__ rcv, uint8_t* __restrict__ dir,
-                                                   int* __restrict__ fail, T* __restrict__ zcopy)
+                                                   int* __restrict__ fail, T* __restrict__ zcopy, TileMark mark)
 {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int r0 = (blockIdx.y * blockDim.y + threadIdx.y) * kRecvRows;
@@ -130,6 +146,7 @@ __global__ void __launch_bounds__(256) k_receivers(const T* __restrict__ z, cons
     const int cm = col > 0 ? col - 1 : col, cp = col < nc - 1 ? col + 1 : col;
     T zw[kRecvRows + 2][3];
     uint8_t mw[kRecvRows + 2][3];
+    int oldr[kRecvRows + 2];
 #pragma unroll
     for (int k = 0; k < kRecvRows + 2; ++k) {
         const int rr = min(max(r0 - 1 + k, 0), nr - 1);
@@ -140,6 +157,7 @@ __global__ void __launch_bounds__(256) k_receivers(const T* __restrict__ z, cons
         mw[k][0] = mask[b + cm];
         mw[k][1] = mask[b + col];
         mw[k][2] = mask[b + cp];
+        oldr[k] = (mark.dirty && k >= 1 && k <= kRecvRows) ? rcv[b + col] : 0;
     }
     if (col == 0 || col == nc - 1) return;
 #pragma unroll
@@ -153,7 +171,7 @@ __global__ void __launch_bounds__(256) k_receivers(const T* __restrict__ z, cons
         const T zn[8] = {zw[k - 1][1], zw[k - 1][2], zw[k][2], zw[k + 1][2],
                          zw[k + 1][1], zw[k + 1][0], zw[k][0], zw[k - 1][0]};
         decide_interior<T>(forced, nc, row, col, card, diag, nodata_outlet, pits_outlet, mw[k][1], zw[k][1], mk, zn,
-                           rcv, dir, fail);
+                           rcv, dir, fail, mark, oldr[k]);
     }
 }
 
@@ -165,7 +183,7 @@ __global__ void __launch_bounds__(256) k_receivers_ring(const T* __restrict__ z,
                                                         double card, double diag, uint32_t bcw, int nodata_outlet,
                                                         int pits_outlet, int* __restrict__ rcv,
                                                         uint8_t* __restrict__ dir, int* __restrict__ fail,
-                                                        T* __restrict__ zcopy)
+                                                        T* __restrict__ zcopy, TileMark mark)
 {
     const BC bc = bc_unpack(bcw);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -186,15 +204,16 @@ __global__ void __launch_bounds__(256) k_receivers_ring(const T* __restrict__ z,
         return;
     }
     const int f = row * nc + col;
+    const int old = mark.dirty ? rcv[f] : 0;
     if (zcopy) zcopy[f] = z[f];
     if (!mask[f]) {
-        write_receiver(f, kNodataReceiver, kNoDirection, rcv, dir);
+        write_receiver(f, kNodataReceiver, kNoDirection, rcv, dir, mark, old);
         return;
     }
     const bool bedge = (row == 0 && bc.n == 'b') || (col == nc - 1 && bc.e == 'b') ||
                        (row == nr - 1 && bc.s == 'b') || (col == 0 && bc.w == 'b');
     if (bedge || (forced && forced[f])) {
-        write_receiver(f, kOutletReceiver, kNoDirection, rcv, dir);
+        write_receiver(f, kOutletReceiver, kNoDirection, rcv, dir, mark, old);
         return;
     }
     const double zc = static_cast<double>(z[f]);
@@ -224,9 +243,9 @@ __global__ void __launch_bounds__(256) k_receivers_ring(const T* __restrict__ z,
         }
     }
     if (bdir != kNoDirection) {
-        write_receiver(f, brcv, bdir, rcv, dir);
+        write_receiver(f, brcv, bdir, rcv, dir, mark, old);
     } else {
-        write_receiver(f, kOutletReceiver, kNoDirection, rcv, dir);
+        write_receiver(f, kOutletReceiver, kNoDirection, rcv, dir, mark, old);
         if (!allowed && !pits_outlet) atomicMin(fail, f);
     }
 }
@@ -235,15 +254,15 @@ __global__ void __launch_bounds__(256) k_receivers_ring(const T* __restrict__ z,
 template <class T>
 void launch_receivers(const T* z, const uint8_t* msk, const uint8_t* forced, int nr, int nc, double dx, BC bc,
                       bool nodata_outlet, bool pits_outlet, int* rcv, uint8_t* dir, int* fail, hipStream_t s,
-                      T* zcopy = nullptr)
+                      T* zcopy = nullptr, const TileMark& mark = TileMark{})
 {
     const dim3 b2(64, 4), g2(div_up(nc, 64), div_up(nr, 4 * kRecvRows));
     k_receivers<T><<<g2, b2, 0, s>>>(z, msk, forced, nr, nc, dx, dx * kSqrt2, nodata_outlet, pits_outlet, rcv, dir,
-                                     fail, zcopy);
+                                     fail, zcopy, mark);
     LSDTT_LAUNCH_CHECK();
     const int nring = nc + (nr > 1 ? nc : 0) + (nr > 2 ? (nc > 1 ? 2 : 1) * (nr - 2) : 0);
     k_receivers_ring<T><<<div_up(nring, 256), 256, 0, s>>>(z, msk, forced, nr, nc, dx, dx * kSqrt2, bc_pack(bc),
-                                                          nodata_outlet, pits_outlet, rcv, dir, fail, zcopy);
+                                                          nodata_outlet, pits_outlet, rcv, dir, fail, zcopy, mark);
     LSDTT_LAUNCH_CHECK();
 }
 
@@ -703,16 +722,17 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
 }
 
 template <class T>
-void FlowEngine::receivers_only(const T* z, const uint8_t* msk, hipStream_t s, T* zcopy)
+void FlowEngine::receivers_only(const T* z, const uint8_t* msk, hipStream_t s, T* zcopy, const TileMark& mark)
 {
     mask = msk;
     TScope ts_(timer, "receivers", s, (sizeof(T) + 6.0) * n);
     launch_receivers<T>(z, msk, nullptr, opt.nr, opt.nc, opt.dx, opt.bc, opt.interior_nodata_is_outlet,
-                        opt.allow_pits_as_outlets, rcv, dir, misc.p, s, zcopy);
+                        opt.allow_pits_as_outlets, rcv, dir, misc.p, s, zcopy, mark);
 }
 
-template void FlowEngine::receivers_only<double>(const double*, const uint8_t*, hipStream_t, double*);
-template void FlowEngine::receivers_only<float>(const float*, const uint8_t*, hipStream_t, float*);
+template void FlowEngine::receivers_only<double>(const double*, const uint8_t*, hipStream_t, double*,
+                                                 const TileMark&);
+template void FlowEngine::receivers_only<float>(const float*, const uint8_t*, hipStream_t, float*, const TileMark&);
 
 template long long FlowEngine::route<float>(const float*, const uint8_t*, const uint8_t*, hipStream_t);
 template long long FlowEngine::route<double>(const double*, const uint8_t*, const uint8_t*, hipStream_t);

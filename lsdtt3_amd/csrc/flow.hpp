@@ -30,6 +30,15 @@ struct FlowOptions {
     bool allow_pits_as_outlets = false;
 };
 
+// Routing tiles (32 columns by th rows, ntx per tile row) whose receiver
+// structure changed: when dirty is set, a receiver kernel that writes a
+// receiver different from the one already stored flags the tile of the cell
+// and the tiles of its old and new receivers (their cross-tile donors).
+struct TileMark {
+    int* dirty = nullptr;
+    int nc = 0, th = 0, ntx = 0;
+};
+
 class FlowEngine {
 public:
     explicit FlowEngine(const FlowOptions& o);
@@ -41,9 +50,11 @@ public:
     long long route(const T* z, const uint8_t* mask, const uint8_t* forced, hipStream_t s);
 
     // receivers and directions only (no failure check: for
-    // allow_pits_as_outlets routing, where every cell resolves)
+    // allow_pits_as_outlets routing, where every cell resolves); zcopy: a
+    // copy of z; mark: routing tiles to flag where a receiver changed
     template <class T>
-    void receivers_only(const T* z, const uint8_t* mask, hipStream_t s, T* zcopy = nullptr);  // zcopy: copy of z
+    void receivers_only(const T* z, const uint8_t* mask, hipStream_t s, T* zcopy = nullptr,
+                        const TileMark& mark = TileMark{});
 
     // Upstream sweep (requires route()).  keys: Kahn keys too; w/acc: optional
     // weighted double accumulation (w null -> no accumulation).

@@ -303,13 +303,16 @@ __device__ __forceinline__ void tile_order_body(const TileAt& T, const int* __re
         }
 }
 
+// dirty: only the flagged tiles (null: all)
 template <int TH>
 __global__ void __launch_bounds__(WL) k_tile_order(const int* __restrict__ rcv, const uint8_t* __restrict__ mask,
                                                    int nr, int nc, BC bc, short* __restrict__ ordg,
                                                    short* __restrict__ lstartg, int* __restrict__ tile_maxdep,
-                                                   uint8_t* __restrict__ xdon)
+                                                   uint8_t* __restrict__ xdon, const int* __restrict__ dirty)
 {
-    tile_order_body<TH>(grid_tile(), rcv, mask, nr, nc, bc, ordg, lstartg, tile_maxdep, xdon);
+    const TileAt T = grid_tile();
+    if (dirty && !dirty[T.tile]) return;
+    tile_order_body<TH>(T, rcv, mask, nr, nc, bc, ordg, lstartg, tile_maxdep, xdon);
 }
 
 template <int TH>
@@ -386,7 +389,7 @@ __device__ __forceinline__ void tile_counts_body(const TileAt& T, const int* __r
     const int tile = T.tile;
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
-    if (P.prev_marked && !P.act[tile]) return;  // the first pass runs every tile
+    if ((P.prev_marked || P.first_filtered) && !P.act[tile]) return;  // the first pass may run every tile
     __shared__ int acc[TN];
     __shared__ short lrcv[TN];
     __shared__ short ord[TN];
@@ -1730,6 +1733,12 @@ __global__ void k_fill_i32(int* __restrict__ a, long long n, int v)
     if (i < n) a[i] = v;
 }
 
+__global__ void k_copy_i32(const int* __restrict__ a, long long n, int* __restrict__ b)
+{
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
 // one-block sum (deterministic): kSumThreads threads, loads unrolled so that
 // each thread keeps several in flight
 constexpr int kSumThreads = 1024;
@@ -1837,6 +1846,8 @@ void FluvialTiles::resize(int nr_, int nc_, BC bc)
     nty = div_up(nr, th);
     ntiles = ntx * nty;
     max_passes = ntiles + 2;
+    d8_valid = false;
+    dirty.reserve(ntiles);
     for (auto& a : act) a.reserve(ntiles);
     tile_eroded.reserve(ntiles);
     tile_maxdep.reserve(ntiles);
@@ -1876,13 +1887,21 @@ void FluvialTiles::smooth_area(const double* area, const uint8_t* mask, int kern
     smooth<double>(area, mask, 1.0, kernel_size, out, s);
 }
 
-void FluvialTiles::depths(const int* rcv, const uint8_t* mask, hipStream_t s)
+void FluvialTiles::clear_dirty(hipStream_t s)
 {
+    k_fill_i32<<<div_up(ntiles, 256), 256, 0, s>>>(dirty.p, ntiles, 0);
+    LSDTT_LAUNCH_CHECK();
+}
+
+void FluvialTiles::depths(const int* rcv, const uint8_t* mask, hipStream_t s, bool only_dirty)
+{
+    const int* d = only_dirty && d8_valid ? dirty.p : nullptr;
     with_tile_height(th, [&](auto H) {
-        k_tile_order<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(rcv, mask, nr, nc, bc_, ord.p, lstart.p, tile_maxdep.p,
-                                                         xdon.p);
+        k_tile_order<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(rcv, mask, nr, nc, bc_, ord.p, lstart.p,
+                                                                       tile_maxdep.p, xdon.p, d);
     });
     LSDTT_LAUNCH_CHECK();
+    d8_valid = true;
 }
 
 // Runs passes in batches with one host read-back per batch: a pass after
@@ -1890,14 +1909,16 @@ void FluvialTiles::depths(const int* rcv, const uint8_t* mask, hipStream_t s)
 // empty launches.  `after_batch` may enqueue work whose result is read back
 // with the flags (the eroded-cell sum).
 template <class Pass, class After>
-int FluvialTiles::settle(const char* name, double first_pass_bytes, Pass&& pass, After&& after_batch, hipStream_t s)
+int FluvialTiles::settle(const char* name, double first_pass_bytes, Pass&& pass, After&& after_batch, hipStream_t s,
+                         const int* start)
 {
     const size_t nflags = 4 + (size_t)max_passes;
     k_fill_i32<<<div_up(nflags, 256), 256, 0, s>>>(flags.p, nflags, 0);
     LSDTT_LAUNCH_CHECK();
     k_fill_i32<<<div_up(ntiles, 256), 256, 0, s>>>(tile_run.p, ntiles, 0);
     LSDTT_LAUNCH_CHECK();
-    k_fill_i32<<<div_up(ntiles, 256), 256, 0, s>>>(act[0].p, ntiles, 1);
+    if (start) k_copy_i32<<<div_up(ntiles, 256), 256, 0, s>>>(start, ntiles, act[0].p);
+    else k_fill_i32<<<div_up(ntiles, 256), 256, 0, s>>>(act[0].p, ntiles, 1);
     LSDTT_LAUNCH_CHECK();
     k_fill_i32<<<div_up(ntiles, 256), 256, 0, s>>>(act[1].p, ntiles, 0);
     LSDTT_LAUNCH_CHECK();
@@ -1907,7 +1928,7 @@ int FluvialTiles::settle(const char* name, double first_pass_bytes, Pass&& pass,
         const int end = std::min(max_passes, launched + batch);
         for (int q = launched; q < end; ++q) {
             PassArgs P{act[q % 3].p, act[(q + 1) % 3].p, act[(q + 2) % 3].p, q > 0 ? flags.p + 4 + q - 1 : nullptr,
-                       flags.p + 4 + q, tile_run.p};
+                       flags.p + 4 + q, tile_run.p, start != nullptr};
             if (timer) timer->begin(name, s);
             pass(P);
             if (timer) timer->end(s, q == 0 ? first_pass_bytes : 0.0);
@@ -1937,7 +1958,7 @@ int FluvialTiles::settle(const char* name, double first_pass_bytes, Pass&& pass,
     }
 }
 
-int FluvialTiles::counts(const int* rcv, const uint8_t* mask, int* cnt, bool warm, hipStream_t s)
+int FluvialTiles::counts(const int* rcv, const uint8_t* mask, int* cnt, bool warm, hipStream_t s, bool only_dirty)
 {
     // The counts are the unique solution of cnt(v) = 1 + sum of the donors'
     // counts, and a pass re-runs every tile whose inputs differ from what it
@@ -1960,7 +1981,7 @@ int FluvialTiles::counts(const int* rcv, const uint8_t* mask, int* cnt, bool war
             });
             LSDTT_LAUNCH_CHECK();
         },
-        [] {}, s);
+        [] {}, s, warm && only_dirty ? dirty.p : nullptr);
     last_count_passes = passes;
     return passes;
 }
@@ -2014,6 +2035,7 @@ int FluvialTiles::stream_power(const int* rcv, const uint8_t* dir, const uint8_t
 
 void FluvialTiles::dinf_up_order(const int* rec, const uint8_t* mask, hipStream_t s)
 {
+    d8_valid = false;  // the D8 order shares these buffers
     with_tile_height(th, [&](auto H) {
         k_dinf_tile_up_order<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(rec, mask, nr, nc, bc_, ord.p,
                                                                                lstart.p, tile_maxdep.p, xdon.p);
@@ -2077,6 +2099,7 @@ int FluvialTiles::dinf_chi(const int* rec, const float* w, const uint8_t* mask, 
 
 void FluvialTiles::dinf_order(const int* rec, const uint8_t* mask, hipStream_t s)
 {
+    d8_valid = false;  // the D8 order shares these buffers
     with_tile_height(th, [&](auto H) {
         k_dinf_tile_order<decltype(H)::value><<<dim3(ntx, nty), WL, 0, s>>>(rec, mask, nr, nc, ord.p, lstart.p,
                                                                             tile_maxdep.p);

@@ -41,6 +41,7 @@ struct PassArgs {
     const int* prev_marked;  // null for the first pass
     int* marked;
     int* runs;  // per tile: times run in this settle (one writer per pass)
+    int first_filtered = 0;  // the first pass too runs only the tiles in act (else every tile)
 };
 
 // inputs and outputs of a stream-power sweep
@@ -106,12 +107,23 @@ public:
     // preceded by depths().
     void set_tile_height(int th);
     int tile_height() const { return th; }
-    // each tile's cells in level order, once per set of receivers
-    void depths(const int* rcv, const uint8_t* mask, hipStream_t s);
+    // each tile's cells in level order, once per set of receivers; with
+    // only_dirty, only the tiles flagged in `dirty` (the receivers of the
+    // others are those of the last depths(), which needs d8_order_valid())
+    void depths(const int* rcv, const uint8_t* mask, hipStream_t s, bool only_dirty = false);
     // counts (int32) of valid cells from receivers; warm: cnt holds counts
-    // of earlier receivers on the same mask (the starting values); returns
+    // of earlier receivers on the same mask (the starting values); with
+    // only_dirty the first pass runs only the flagged tiles (cnt must hold
+    // the counts of the last receivers, which changed only there); returns
     // passes used
-    int counts(const int* rcv, const uint8_t* mask, int* cnt, bool warm, hipStream_t s);
+    int counts(const int* rcv, const uint8_t* mask, int* cnt, bool warm, hipStream_t s, bool only_dirty = false);
+    // the D8 level order of every tile is that of the receivers last given
+    // to depths() (the D-infinity orders share its buffers)
+    bool d8_order_valid() const { return d8_valid; }
+    // per tile: a receiver in it, or one leading into it, changed since the
+    // last depths() (set by the receiver kernels, cleared by clear_dirty)
+    dbuf<int> dirty;
+    void clear_dirty(hipStream_t s);
     // stream power in place on z (z == z_old on entry); returns passes used
     // (area: per-cell drainage area in m^2, or null for count * dx^2)
     int stream_power(const int* rcv, const uint8_t* dir, const uint8_t* mask, const int* cnt, const double* area,
@@ -159,11 +171,14 @@ public:
 
 private:
     int th = 16;
+    bool d8_valid = false;
     // passes of the last call of each sweep: the next call's first batch
     int last_passes = 0, last_count_passes = 0, last_sp_passes = 0, last_dinf_passes = 0, last_chi_passes = 0;
     // runs `pass(PassArgs)` until a pass marks no tile
+    // start: the tiles active in the first pass (null: all)
     template <class Pass, class After>
-    int settle(const char* name, double first_pass_bytes, Pass&& pass, After&& after_batch, hipStream_t s);
+    int settle(const char* name, double first_pass_bytes, Pass&& pass, After&& after_batch, hipStream_t s,
+               const int* start = nullptr);
     template <class T>
     void smooth(const T* v, const uint8_t* mask, double pa, int kernel_size, double* out, hipStream_t s);
 };

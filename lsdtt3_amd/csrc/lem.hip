@@ -562,12 +562,23 @@ void LemEngine::stream_power(hipStream_t s)
     // tile passes (fluvial.hip)
     // the receivers read the pre-step surface from z and leave its copy in
     // z_before (the step skips its own copy when this path runs)
-    flow.receivers_only<double>(z.p, mask.p, s, z_before.p);
+    // Once the tiles hold the level order of the last receivers and the
+    // counts are those receivers' counts, only the tiles where a receiver
+    // changed (or a changed receiver leads in or out) need a new order, and
+    // the counts' passes start from those tiles alone: the others' counts
+    // hold until a changed inflow re-runs them, which the passes do.
+    const bool incremental = counts_warm && tiles.d8_order_valid();
+    TileMark mark;
+    if (incremental) {
+        tiles.clear_dirty(s);
+        mark = TileMark{tiles.dirty.p, nc, tiles.tile_height(), tiles.ntx};
+    }
+    flow.receivers_only<double>(z.p, mask.p, s, z_before.p, mark);
     timer.begin("tile_depths", s);
-    tiles.depths(flow.rcv.p, mask.p, s);
+    tiles.depths(flow.rcv.p, mask.p, s, incremental);
     timer.end(s, 9.0 * n);  // rcv 4, mask 1 in; order 2, level starts 2 out
     flow.cnt.reserve(n);
-    last_count_passes = tiles.counts(flow.rcv.p, mask.p, flow.cnt.p, counts_warm, s);
+    last_count_passes = tiles.counts(flow.rcv.p, mask.p, flow.cnt.p, counts_warm, s, incremental);
     counts_warm = true;
     last_count_tiles = tiles.tile_runs;
     SpParams a;
