@@ -74,7 +74,7 @@ int finish_build(lsdtt_gpu_flow* f, long long rc)
     hipStream_t s = nullptr;
     const bool topo = f->o.build_topological_order != 0;
     const bool sv = f->o.build_s_vector != 0;
-    f->eng->upstream(true, nullptr, nullptr, s);
+    f->eng->upstream(true, nullptr, nullptr, s, sv);
     if (topo) f->eng->topological_order(s);
     if (sv) f->eng->s_vector(s);
     LSDTT_HIP(hipStreamSynchronize(s));
@@ -230,7 +230,7 @@ int lsdtt_gpu_flow_build_device(lsdtt_gpu_flow* f, const void* d_z, int32_t z_is
         if (rc < 0) return finish_build(f, rc);
         const bool topo = f->o.build_topological_order != 0;
         const bool sv = f->o.build_s_vector != 0;
-        f->eng->upstream(true, nullptr, nullptr, s);
+        f->eng->upstream(true, nullptr, nullptr, s, sv);
         if (topo) f->eng->topological_order(s);
         if (sv) f->eng->s_vector(s);
         f->built = true;

@@ -5,6 +5,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include "chi.hpp"
+#include "hostpool.hpp"
 
 namespace lsdtt_gpu {
 
@@ -126,7 +127,23 @@ __global__ void k_select_scatter(const int* __restrict__ svec, const int* __rest
     }
 }
 
+// the counts present in increasing order: word w's set bits land at
+// wrank[w] on; the last word also writes the total
+__global__ void k_present_list(const unsigned* __restrict__ bits, const int* __restrict__ wrank, long long nw,
+                               int* __restrict__ out, int* __restrict__ total)
+{
+    const long long w = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    unsigned b = bits[w];
+    int o = wrank[w];
+    if (w == nw - 1) *total = o + __popc(b);
+    for (; b; b &= b - 1) out[o++] = static_cast<int>(w * 32 + __builtin_ctz(b));
+}
+
 }  // namespace
+
+// room for the list of counts present: at most one per valid cell
+long long ChiEngine::n_present_cap(long long n) { return n; }
 
 void ChiEngine::raster(FlowEngine& e, double m_over_n, double A0, float nodata, float* chi, hipStream_t s)
 {
@@ -145,27 +162,37 @@ void ChiEngine::raster(FlowEngine& e, double m_over_n, double A0, float nodata, 
     // rank table: set bits before each word
     wpc.reserve(nw + 1);
     wrank.reserve(nw + 1);
+    present.reserve(n_present_cap(n) + 1);
     k_word_popc<<<div_up(nw, kB), kB, 0, s>>>(bits.p, nw, wpc.p);
     LSDTT_LAUNCH_CHECK();
     size_t tb = 0;
     LSDTT_HIP(rocprim::exclusive_scan(nullptr, tb, wpc.p, wrank.p, 0, (size_t)nw, rocprim::plus<int>(), s));
     LSDTT_HIP(rocprim::exclusive_scan(tmp.need(tb), tb, wpc.p, wrank.p, 0, (size_t)nw, rocprim::plus<int>(), s));
-    bits_h.resize(static_cast<size_t>(nw));
-    LSDTT_HIP(hipMemcpyAsync(bits_h.data(), bits.p, sizeof(unsigned) * nw, hipMemcpyDeviceToHost, s));
+    // the counts present, in increasing order, then their number
+    k_present_list<<<div_up(nw, kB), kB, 0, s>>>(bits.p, wrank.p, nw, present.p, present.p + n_present_cap(n));
+    LSDTT_LAUNCH_CHECK();
+    hread.reserve(1);
+    LSDTT_HIP(hipMemcpyAsync(hread.p, present.p + n_present_cap(n), sizeof(int), hipMemcpyDeviceToHost, s));
+    LSDTT_HIP(hipStreamSynchronize(s));
+    const long long K = hread.p[0];
+    kh.reserve(std::max<long long>(K, 1));
+    LSDTT_HIP(hipMemcpyAsync(kh.p, present.p, sizeof(int) * K, hipMemcpyDeviceToHost, s));
     LSDTT_HIP(hipStreamSynchronize(s));
     // f(k) = pow(A0 / (k * dx^2), theta), the host libm's values
     // (lsdtt_chi_extraction_cpu.cpp:154-163 with A from :748-751), for the
     // counts present only, in increasing k
     const double pa = e.opt.dx * e.opt.dx;
-    table_h.clear();
-    for (long long w = 0; w < nw; ++w)
-        for (unsigned b = bits_h[w]; b; b &= b - 1) {
-            const long long k = w * 32 + __builtin_ctz(b);
-            const double denom = static_cast<double>(k) * pa;
-            table_h.push_back(std::pow(A0 / denom, m_over_n));
+    table_h.reserve(std::max<long long>(K, 1));
+    const int* ks = kh.p;
+    double* tv = table_h.p;
+    host_parallel_for(K, 4096, [&](long long b, long long en) {
+        for (long long i = b; i < en; ++i) {
+            const double denom = static_cast<double>(ks[i]) * pa;
+            tv[i] = std::pow(A0 / denom, m_over_n);
         }
-    table.reserve(std::max<size_t>(table_h.size(), 1));
-    LSDTT_HIP(hipMemcpyAsync(table.p, table_h.data(), sizeof(double) * table_h.size(), hipMemcpyHostToDevice, s));
+    });
+    table.reserve(std::max<long long>(K, 1));
+    LSDTT_HIP(hipMemcpyAsync(table.p, table_h.p, sizeof(double) * K, hipMemcpyHostToDevice, s));
     ChiArgs a{e.rcv.p, e.dir.p, e.cnt.p, table.p, bits.p, wrank.p, e.opt.dx, chi};
     const int nl = e.max_depth + 1;
     const int split = std::max(e.tail_from, 1);
@@ -180,7 +207,8 @@ void ChiEngine::raster(FlowEngine& e, double m_over_n, double A0, float nodata, 
         k_chi_tail<<<1, 1024, 0, s>>>(e.order.p, e.level_start.p, split, nl - 1, a);
         LSDTT_LAUNCH_CHECK();
     }
-    // the table must outlive the kernels that read it
+    // the pinned host table is the source of an asynchronous copy: it must
+    // not change before the copy is made
     LSDTT_HIP(hipStreamSynchronize(s));
 }
 

@@ -30,13 +30,16 @@ public:
     long long threshold(FlowEngine& e, const float* chi, long long min_pixels, hipStream_t s);
 
     dbuf<unsigned> bits;
-    dbuf<int> wpc, wrank;
+    dbuf<int> wpc, wrank, present;
     dbuf<double> table;
     dbuf<int> flag, pos, sel_flat, sel_rcv;
     dbuf<float> sel_chi;
     dscratch tmp;
-    std::vector<unsigned> bits_h;
-    std::vector<double> table_h;
+    hbuf<int> hread, kh;   // pinned: the number of counts present, the counts
+    hbuf<double> table_h;  // pinned: f of each count present
+
+private:
+    static long long n_present_cap(long long n);
 };
 
 }  // namespace lsdtt_gpu

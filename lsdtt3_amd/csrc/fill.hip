@@ -36,12 +36,21 @@
 //     components holding the cell so far.  Hence
 //         L(basin b) = the largest hook weight on b's chain of hooks up to
 //                      the marker,
-//     computed by pointer jumping with max once the rounds end (each round at
-//     least halves the components that have not reached the marker).
+//     computed once the rounds end by walking every basin's chain in place
+//     (each round at least halves the components that have not reached the
+//     marker).
+//     Work per round follows the component boundaries, not the grid: the
+//     first round scans every cell and lists those with a neighbour in
+//     another component; later rounds scan only that list and keep the
+//     cells still on a boundary.  Cells keep their basin label; the basin's
+//     component is found through the hook parents (flattened every round,
+//     compressed when read), so no pass relabels the grid.
 //  3. Output: max(z, L(root)) for pit-basin cells whose chain reaches the
 //     marker, z otherwise.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <stdexcept>
@@ -55,27 +64,25 @@ namespace {
 
 constexpr int kFixed = INT_MAX;  // z key of exits and invalid cells
 constexpr unsigned long long kNoEdge = ~0ull;
-constexpr int kJumpRounds = 30;    // descent pointer jumping (2^30 > any grid the fill takes)
-constexpr int kMaxRounds = 40;     // Boruvka rounds (each at least halves the components)
-constexpr int kParJumps = 6;       // per round: launches of the hook-tree flattening
-constexpr int kLevelJumps = 32;    // hook-chain maxima (2^32 > any chain)
-constexpr int kListGrid = 512;     // blocks of the kernels over a component list
-constexpr int kChunkBlocks = 1024; // blocks of a list compaction (one contiguous chunk each)
+constexpr int kJumpRounds = 30;      // descent pointer jumping (2^30 > any grid the fill takes)
+constexpr int kMaxRounds = 40;       // Boruvka rounds (each at least halves the components)
+constexpr int kWalkSteps = 1 << 22;  // bound on one pointer walk (hook trees, chains): never reached
+constexpr int kScanGrid = 2048;      // most blocks of a kernel over a list
+constexpr int kChunkBlocks = 1024;   // blocks of a list compaction (one contiguous chunk each)
 constexpr int kB = 256;
-constexpr int kStrideGrid = 2048;  // blocks of the grid-stride full-grid kernels
+constexpr int kStrideGrid = 2048;    // blocks of the grid-stride full-grid kernels
 
-// flag slots (ints): [0] interior invalid, [2, 2+kJumpRounds] descent jumps,
-// [64, 64+kMaxRounds] round go flags, [128, 128+kMaxRounds*(kParJumps+1))
-// flattening flags, [512, 512+kLevelJumps] chain flags, [600..601] list
-// lengths, [602] the last chain launch that ran, [640, 640+2 kChunkBlocks)
-// per-chunk counts and offsets of a compaction
-constexpr int kFlagJump = 2, kFlagRound = 64, kFlagPar = 128, kFlagLevel = 512, kFlagCount = 600;
-constexpr int kFlagLast = 602, kFlagChunks = 640;
+// flag slots (ints): [0] interior invalid, [1] a pointer walk hit its bound,
+// [2, 2+kJumpRounds] descent jumps, [64, 64+kMaxRounds] round go flags,
+// [600..602] lengths of the two component lists and of the basin list,
+// [603..604] lengths of the two boundary-cell lists, [640, 640+2
+// kChunkBlocks) per-chunk counts and offsets of a compaction
+constexpr int kFlagStuck = 1, kFlagJump = 2, kFlagRound = 64, kFlagCount = 600, kFlagBoundary = 603;
+constexpr int kFlagChunks = 640;
 constexpr int kFlagInts = kFlagChunks + 2 * kChunkBlocks;
 static_assert(kFlagJump + kJumpRounds + 1 <= kFlagRound, "flag slots");
-static_assert(kFlagRound + kMaxRounds + 1 <= kFlagPar, "flag slots");
-static_assert(kFlagPar + kMaxRounds * (kParJumps + 1) <= kFlagLevel, "flag slots");
-static_assert(kFlagLevel + kLevelJumps + 1 <= kFlagCount, "flag slots");
+static_assert(kFlagRound + kMaxRounds + 1 <= kFlagCount, "flag slots");
+static_assert(kFlagBoundary + 2 <= kFlagChunks, "flag slots");
 
 __device__ __forceinline__ int fkey(float f)
 {
@@ -196,56 +203,145 @@ __global__ void k_fill_jump(int* __restrict__ ptr, long long n, const int* __res
     wave_flag(ch, cur);
 }
 
-// Components: lab[k] = X (= n, the marker) when the cell's root is an exit,
-// its root otherwise, -1 when invalid.  Every slot c <= n starts with no hook
-// (hp -1, hw INT_MIN), par[c] = c and no cheapest edge.
-__global__ void k_bv_label(const int* __restrict__ zk, const int* __restrict__ wk, const int* __restrict__ ptr,
-                           long long n, int* __restrict__ lab, int* __restrict__ par, int* __restrict__ hp,
-                           int* __restrict__ hw, unsigned long long* __restrict__ best)
+__device__ __forceinline__ int ld_rel(const int* p)
 {
-    const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (k > n) return;
-    par[k] = (int)k;
-    hp[k] = -1;
-    hw[k] = INT_MIN;
-    best[k] = kNoEdge;
-    if (k == n) return;  // the marker's slots
-    if (wk[k] == kFixed) {
-        lab[k] = -1;
-        return;
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(int* p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_zero_ints(int* __restrict__ p, int count)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// Labels, in place over the descent roots: lab[k] = X (= n, the marker) when
+// the cell's root is an exit, the root (its basin) otherwise, -1 when
+// invalid.  Also counts the pit basins (lab[k] == k) of each contiguous
+// chunk, the first launch of their list's compaction.
+__global__ void __launch_bounds__(kB) k_bv_label(const int* __restrict__ zk, const int* __restrict__ wk,
+                                                 int* __restrict__ lab, long long n, int* __restrict__ chunk_count)
+{
+    const long long len = (n + gridDim.x - 1) / gridDim.x;
+    const long long b0 = blockIdx.x * len, b1 = std::min(n, b0 + len);
+    int c = 0;
+    for (long long k0 = b0 + threadIdx.x; k0 < b1; k0 += 4ll * blockDim.x) {
+        int w[4], r[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long long k = k0 + j * (long long)blockDim.x;
+            w[j] = k < b1 ? wk[k] : kFixed;
+            r[j] = k < b1 ? lab[k] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long long k = k0 + j * (long long)blockDim.x;
+            if (k >= b1) continue;
+            int v = -1;
+            if (w[j] != kFixed) v = zk[r[j]] == kFixed ? static_cast<int>(n) : r[j];
+            lab[k] = v;
+            c += v == k;
+        }
     }
-    const int r = ptr[k];
-    lab[k] = zk[r] == kFixed ? (int)n : r;
+    __shared__ int red[kB / 64];
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int wv = 0; wv < kB / 64; ++wv) t += red[wv];
+        chunk_count[blockIdx.x] = t;
+    }
+}
+
+// Every basin slot and the marker's start as their own component with no
+// hook (hp -1, hw INT_MIN) and no cheapest edge; no other slot is read
+__global__ void k_basin_init(const int* __restrict__ roots, const int* __restrict__ count, int X,
+                             int* __restrict__ par, int* __restrict__ hp, int* __restrict__ hw,
+                             unsigned long long* __restrict__ best)
+{
+    const int cnt = *count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= cnt; i += gridDim.x * blockDim.x) {
+        const int b = i < cnt ? roots[i] : X;
+        par[b] = b;
+        hp[b] = -1;
+        hw[b] = INT_MIN;
+        best[b] = kNoEdge;
+    }
+}
+
+// Every basin's parent made its component (in place, after the round's
+// flattening): components still in the rounds and the marker are their own
+// parents, every other slot's parent lies on the way to its component, and
+// a parent pointer only ever moves towards it, so walks that overlap stay
+// correct.  The scans and hook targets then read a basin's component as
+// par[basin].
+__global__ void k_bv_compress(int* __restrict__ par, const int* __restrict__ roots, const int* __restrict__ count,
+                              const int* __restrict__ go, int* __restrict__ stuck)
+{
+    if (*go == 0) return;
+    const int cnt = *count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const int b = roots[i];
+        const int a = ld_rel(&par[b]);
+        int p = ld_rel(&par[a]);
+        if (p == a) continue;
+        for (int s = 0;; ++s) {
+            const int q = ld_rel(&par[p]);
+            if (q == p) break;
+            if (s == kWalkSteps) {
+                *stuck = 1;
+                break;
+            }
+            p = q;
+        }
+        st_rel(&par[b], p);
+    }
 }
 
 // List compaction in three launches (a count per contiguous chunk, one scan,
 // the writes in order), so no counter is shared by the whole grid.
-// kRoots: over the cells, those that are pit roots (lab[c] == c); kSurvivors:
-// over a component list, the entries that did not hook (par[c] == c), whose
-// cheapest edge is cleared for the next round.
-enum { kRoots = 0, kSurvivors = 1 };
+// kRoots: over the cells, those that are pit basins' roots (lab[c] == c);
+// kSurvivors: over a component list, the entries that did not hook
+// (par[c] == c), whose cheapest edge is cleared for the next round;
+// kKeepCells / kKeepList: the cells / list entries whose keep byte is set.
+enum { kRoots = 0, kSurvivors = 1, kKeepCells = 2, kKeepList = 3 };
 
 template <int MODE>
 __device__ __forceinline__ int list_item(int i, const int* __restrict__ lab, const int* __restrict__ par,
-                                         const int* __restrict__ list)
+                                         const int* __restrict__ list, const uint8_t* __restrict__ keep)
 {
     if (MODE == kRoots) return lab[i] == i ? i : -1;
+    if (MODE == kKeepCells) return keep[i] ? i : -1;
+    if (MODE == kKeepList) return keep[i] ? list[i] : -1;
     const int c = list[i];
     return par[c] == c ? c : -1;
 }
 
 template <int MODE>
 __global__ void __launch_bounds__(kB) k_list_count(const int* __restrict__ lab, const int* __restrict__ par,
-                                                   const int* __restrict__ list, long long m_fixed,
-                                                   const int* __restrict__ m_dev, int* __restrict__ chunk_count,
-                                                   const int* __restrict__ go)
+                                                   const int* __restrict__ list, const uint8_t* __restrict__ keep,
+                                                   long long m_fixed, const int* __restrict__ m_dev,
+                                                   int* __restrict__ chunk_count, const int* __restrict__ go)
 {
     if (go && *go == 0) return;
     const long long m = m_dev ? *m_dev : m_fixed;
     const long long len = (m + gridDim.x - 1) / gridDim.x;
     const long long b0 = blockIdx.x * len, b1 = std::min(m, b0 + len);
     int c = 0;
-    for (long long i = b0 + threadIdx.x; i < b1; i += blockDim.x) c += list_item<MODE>((int)i, lab, par, list) >= 0;
+    // four items per thread per step, their loads issued together
+    for (long long i = b0 + threadIdx.x; i < b1; i += 4 * blockDim.x) {
+        int v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long long ij = i + j * (long long)blockDim.x;
+            v[j] = ij < b1 ? list_item<MODE>((int)ij, lab, par, list, keep) : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c += v[j] >= 0;
+    }
     __shared__ int red[kB / 64];
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
@@ -280,10 +376,10 @@ __global__ void __launch_bounds__(kChunkBlocks) k_list_scan(const int* __restric
 
 template <int MODE>
 __global__ void __launch_bounds__(kB) k_list_write(const int* __restrict__ lab, const int* __restrict__ par,
-                                                   const int* __restrict__ list, long long m_fixed,
-                                                   const int* __restrict__ m_dev, const int* __restrict__ chunk_off,
-                                                   int* __restrict__ out, unsigned long long* __restrict__ best,
-                                                   const int* __restrict__ go)
+                                                   const int* __restrict__ list, const uint8_t* __restrict__ keep,
+                                                   long long m_fixed, const int* __restrict__ m_dev,
+                                                   const int* __restrict__ chunk_off, int* __restrict__ out,
+                                                   unsigned long long* __restrict__ best, const int* __restrict__ go)
 {
     if (go && *go == 0) return;
     const long long m = m_dev ? *m_dev : m_fixed;
@@ -292,38 +388,111 @@ __global__ void __launch_bounds__(kB) k_list_write(const int* __restrict__ lab, 
     __shared__ int wsum[kB / 64];
     int base = chunk_off[blockIdx.x];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (long long i0 = b0; i0 < b1; i0 += blockDim.x) {
-        const long long i = i0 + threadIdx.x;
-        const int v = i < b1 ? list_item<MODE>((int)i, lab, par, list) : -1;
-        const unsigned long long bal = __ballot(v >= 0);
-        if (lane == 0) wsum[wave] = __popcll(bal);
-        __syncthreads();
-        int before = 0, all = 0;
-        for (int w = 0; w < kB / 64; ++w) {
-            before += w < wave ? wsum[w] : 0;
-            all += wsum[w];
+    // four rows of blockDim.x items per step, their loads issued together,
+    // then written row by row in order
+    for (long long i0 = b0; i0 < b1; i0 += 4 * blockDim.x) {
+        int v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long long i = i0 + j * (long long)blockDim.x + threadIdx.x;
+            v[j] = i < b1 ? list_item<MODE>((int)i, lab, par, list, keep) : -1;
         }
-        if (v >= 0) {
-            out[base + before + __popcll(bal & ((1ull << lane) - 1ull))] = v;
-            if (MODE == kSurvivors) best[v] = kNoEdge;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const unsigned long long bal = __ballot(v[j] >= 0);
+            if (lane == 0) wsum[wave] = __popcll(bal);
+            __syncthreads();
+            int before = 0, all = 0;
+            for (int w = 0; w < kB / 64; ++w) {
+                before += w < wave ? wsum[w] : 0;
+                all += wsum[w];
+            }
+            if (v[j] >= 0) {
+                out[base + before + __popcll(bal & ((1ull << lane) - 1ull))] = v[j];
+                if (MODE == kSurvivors) best[v[j]] = kNoEdge;
+            }
+            base += all;
+            __syncthreads();
         }
-        base += all;
-        __syncthreads();
     }
 }
 
-// Each cell of a non-marker component offers its cheapest edge to another
-// component: key = (w key with the sign bit flipped) << 32 | edge id, the
-// edge id min(p, q) * 8 + direction from the lower to the higher index.  A
-// block covers 64 x 4 cells; their labels and keys, with a one-cell halo
-// (periodic axes wrap; -1 outside the grid), are staged in LDS first.
+// the key of the edge from cell k (key kp) to its neighbour q in direction d
+// (key kq): (w key with the sign bit flipped) << 32 | edge id, the edge id
+// min(k, q) * 8 + direction from the lower to the higher index
+__device__ __forceinline__ unsigned long long edge_key(int k, int q, int d, int kp, int kq)
+{
+    const unsigned uw = static_cast<unsigned>(max(kp, kq)) ^ 0x80000000u;
+    const unsigned eid = k < q ? static_cast<unsigned>(k) * 8u + d : static_cast<unsigned>(q) * 8u + ((d + 4) & 7);
+    return (static_cast<unsigned long long>(uw) << 32) | eid;
+}
+
+// Offers of one wavefront's cells, in list (or row) order: runs of one
+// component fold their minima towards the run's first lane, which alone
+// issues the atomic
+__device__ __forceinline__ void offer(int A, unsigned long long m, unsigned long long* __restrict__ best)
+{
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long om = __shfl_down(m, o);
+        const int oa = __shfl_down(A, o);
+        if (lane + o < 64 && oa == A && om < m) m = om;
+    }
+    const int left = __shfl_up(A, 1);
+    const bool head = lane == 0 || left != A;
+    if (head && A >= 0 && m != kNoEdge && m < best[A]) atomicMin(&best[A], m);
+}
+
+// Offers of a block's cells (every thread calls it): after the wavefront
+// fold, the run heads meet in an LDS table keyed by component, so the block
+// issues one global atomic per component it saw (large components late in
+// the rounds would otherwise take one per run from every wavefront)
+__device__ __forceinline__ void offer_block(int A, unsigned long long m, unsigned long long* __restrict__ best)
+{
+    __shared__ int tkey[kB];
+    __shared__ unsigned long long tval[kB];
+    const int lane = threadIdx.x & 63;
+    tkey[threadIdx.x] = -1;
+    tval[threadIdx.x] = kNoEdge;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long om = __shfl_down(m, o);
+        const int oa = __shfl_down(A, o);
+        if (lane + o < 64 && oa == A && om < m) m = om;
+    }
+    const int left = __shfl_up(A, 1);
+    const bool head = (lane == 0 || left != A) && A >= 0 && m != kNoEdge;
+    __syncthreads();
+    if (head) {
+        // at most one head per thread, so the table never fills
+        int slot = static_cast<int>((static_cast<unsigned>(A) * 2654435761u) >> 24) & (kB - 1);
+        for (int probe = 0; probe < kB; ++probe) {
+            const int prev = atomicCAS(&tkey[slot], -1, A);
+            if (prev == -1 || prev == A) {
+                atomicMin(&tval[slot], m);
+                break;
+            }
+            slot = (slot + 1) & (kB - 1);
+        }
+    }
+    __syncthreads();
+    const int key = tkey[threadIdx.x];
+    const unsigned long long v = tval[threadIdx.x];
+    if (key >= 0 && v < best[key]) atomicMin(&best[key], v);
+    __syncthreads();
+}
+
+// First round: each cell of a non-marker component offers its cheapest edge
+// to another component, and the cells with such an edge are flagged for the
+// boundary list.  Components are the basins themselves (no hook yet), so the labels
+// are read directly.  A block covers 64 x 4 cells; their labels and keys,
+// with a one-cell halo (periodic axes wrap; -1 outside the grid), are
+// staged in LDS first.
 constexpr int kScanW = 64, kScanH = 4;
 __global__ void __launch_bounds__(kScanW * kScanH) k_bv_scan(const int* __restrict__ wk, const int* __restrict__ lab,
                                                              int nr, int nc, int per_ns, int per_ew, int X,
                                                              unsigned long long* __restrict__ best,
-                                                             const int* __restrict__ go)
+                                                             uint8_t* __restrict__ keep_out)
 {
-    if (*go == 0) return;
     constexpr int TW = kScanW + 2, TH = kScanH + 2;
     __shared__ int sl[TH * TW], sk[TH * TW];
     const int c0 = blockIdx.x * kScanW, r0 = blockIdx.y * kScanH;
@@ -345,38 +514,101 @@ __global__ void __launch_bounds__(kScanW * kScanH) k_bv_scan(const int* __restri
     int A = in ? sl[li] : -1;
     if (A == X) A = -1;
     unsigned long long m = kNoEdge;
+    bool keep = false;
     if (A >= 0) {
         const int kp = sk[li];
         for (int d = 0; d < 8; ++d) {
             const int lq = li + d8_row[d] * TW + d8_col[d];
             const int B = sl[lq];
             if (B < 0 || B == A) continue;
+            keep = true;
             const int q = nbr_flat(r, c, d, nr, nc, per_ns, per_ew);
-            const int w = max(kp, sk[lq]);
-            const unsigned uw = static_cast<unsigned>(w) ^ 0x80000000u;
-            const unsigned eid =
-                k < q ? static_cast<unsigned>(k) * 8u + d : static_cast<unsigned>(q) * 8u + ((d + 4) & 7);
-            const unsigned long long key = (static_cast<unsigned long long>(uw) << 32) | eid;
+            const unsigned long long key = edge_key(k, q, d, kp, sk[lq]);
             m = key < m ? key : m;
         }
     }
-    // the wavefront is one row segment: runs of one component fold their
-    // minima towards the run's first lane, which alone issues the atomic
-    const int lane = threadIdx.x & 63;
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned long long om = __shfl_down(m, o);
-        const int oa = __shfl_down(A, o);
-        if (lane + o < 64 && oa == A && om < m) m = om;
-    }
-    const int left = __shfl_up(A, 1);
-    const bool head = lane == 0 || left != A;
-    if (head && A >= 0 && m != kNoEdge && m < best[A]) atomicMin(&best[A], m);
+    if (in) keep_out[k] = keep;
+    offer(A, m, best);
 }
 
-// Each listed component with an outgoing edge hooks onto the component across
-// its cheapest edge, except the lower id of a pair that chose the same edge.
-__global__ void k_bv_hook(const int* __restrict__ lab, const unsigned long long* __restrict__ best, int nr, int nc,
-                          int per_ns, int per_ew, int X, const int* __restrict__ list,
+// Later rounds: the cells of the boundary list offer their cheapest edges
+// to other components; those that still have one are flagged to stay on
+// the list.  A cell's component is its basin's parent; the loads of all
+// nine cells are issued together, one step at a time.
+__global__ void __launch_bounds__(kB) k_bv_scan_list(const int* __restrict__ wk, const int* __restrict__ lab,
+                                                     const int* __restrict__ par, int nr, int nc, int per_ns, int per_ew,
+                                                     int X, const int* __restrict__ list,
+                                                     const int* __restrict__ count, uint8_t* __restrict__ keep_out,
+                                                     unsigned long long* __restrict__ best,
+                                                     const int* __restrict__ go)
+{
+    if (*go == 0) return;
+    const int cnt = *count;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i0 = blockIdx.x * blockDim.x; i0 < cnt; i0 += stride) {
+        const int i = i0 + threadIdx.x;
+        const bool in = i < cnt;
+        const int k = in ? list[i] : 0;
+        const int r = k / nc, c = k - r * nc;
+        int q[8], lq[8], kq[8];
+#pragma unroll
+        for (int d = 0; d < 8; ++d) q[d] = in ? nbr_flat(r, c, d, nr, nc, per_ns, per_ew) : -1;
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            lq[d] = q[d] >= 0 ? lab[q[d]] : -1;
+            kq[d] = q[d] >= 0 ? wk[q[d]] : 0;
+        }
+        const int lk = in ? lab[k] : -1;
+        const int kp = in ? wk[k] : 0;
+        int A = -1;
+        unsigned long long m = kNoEdge;
+        bool keep = false;
+        if (lk >= 0) {
+            int cq[8];
+            A = par[lk];
+#pragma unroll
+            for (int d = 0; d < 8; ++d) cq[d] = par[lq[d] >= 0 ? lq[d] : lk];
+            if (A == X) {
+                A = -1;
+            } else {
+#pragma unroll
+                for (int d = 0; d < 8; ++d) {
+                    if (lq[d] < 0 || cq[d] == A) continue;
+                    keep = true;
+                    const unsigned long long key = edge_key(k, q[d], d, kp, kq[d]);
+                    m = key < m ? key : m;
+                }
+            }
+        }
+        if (in) keep_out[i] = keep;
+        offer_block(A, m, best);
+    }
+}
+
+// Each listed component with an outgoing edge records the component across
+// it (hook targets are read before any component of the round hooks)
+__global__ void k_bv_target(const int* __restrict__ lab, const int* __restrict__ par,
+                            const unsigned long long* __restrict__ best, int nr, int nc, int per_ns, int per_ew,
+                            const int* __restrict__ list, const int* __restrict__ count, int* __restrict__ hp,
+                            const int* __restrict__ go)
+{
+    if (*go == 0) return;
+    const int cnt = *count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const int c = list[i];
+        const unsigned long long b = best[c];
+        if (b == kNoEdge) continue;
+        const unsigned eid = static_cast<unsigned>(b);
+        const int p = static_cast<int>(eid >> 3), d = static_cast<int>(eid & 7);
+        const int q = nbr_flat(p / nc, p % nc, d, nr, nc, per_ns, per_ew);
+        const int A = par[lab[p]], B = par[lab[q]];
+        hp[c] = A == c ? B : A;
+    }
+}
+
+// Each listed component with an outgoing edge hooks onto its target, except
+// the lower id of a pair that chose the same edge
+__global__ void k_bv_hook(const unsigned long long* __restrict__ best, int X, const int* __restrict__ list,
                           const int* __restrict__ count, int* __restrict__ par, int* __restrict__ hp,
                           int* __restrict__ hw, const int* __restrict__ go, int* __restrict__ hooked)
 {
@@ -390,15 +622,12 @@ __global__ void k_bv_hook(const int* __restrict__ lab, const unsigned long long*
             const int c = list[i];
             const unsigned long long b = best[c];
             if (b != kNoEdge) {
-                const unsigned eid = static_cast<unsigned>(b);
-                const int p = static_cast<int>(eid >> 3), d = static_cast<int>(eid & 7);
-                const int q = nbr_flat(p / nc, p % nc, d, nr, nc, per_ns, per_ew);
-                const int A = lab[p], B = lab[q];
-                const int t = A == c ? B : A;
+                const int t = hp[c];
                 const bool stays = t != X && best[t] == b && c < t;
-                if (!stays) {
+                if (stays) {
+                    hp[c] = -1;
+                } else {
                     par[c] = t;
-                    hp[c] = t;
                     hw[c] = static_cast<int>(static_cast<unsigned>(b >> 32) ^ 0x80000000u);
                     h = true;
                 }
@@ -408,125 +637,122 @@ __global__ void k_bv_hook(const int* __restrict__ lab, const unsigned long long*
     }
 }
 
-// flattening of this round's hook trees: each listed component walks up to 16
-// parents (in place: a parent pointer only ever moves towards its tree's root)
+// Flattening of this round's hook trees: each listed component walks to its
+// tree's root, storing every step (in place: a parent pointer only ever
+// moves towards its tree's root, and walks reuse the steps stored by the
+// nodes they pass)
 __global__ void k_bv_flatten(int* __restrict__ par, const int* __restrict__ list, const int* __restrict__ count,
-                             const int* __restrict__ go, const int* __restrict__ prev, int* __restrict__ cur)
-{
-    if (*go == 0 || *prev == 0) return;
-    const int cnt = *count;
-    const int stride = gridDim.x * blockDim.x;
-    for (int i0 = blockIdx.x * blockDim.x; i0 < cnt; i0 += stride) {
-        const int i = i0 + threadIdx.x;
-        bool ch = false;
-        if (i < cnt) {
-            const int c = list[i];
-            int p = __hip_atomic_load(&par[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const int p0 = p;
-            for (int s = 0; s < 16; ++s) {
-                const int q = __hip_atomic_load(&par[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (q == p) break;
-                p = q;
-            }
-            if (p != p0) __hip_atomic_store(&par[c], p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const int top = __hip_atomic_load(&par[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ch = top != p;  // not at a root yet
-        }
-        wave_flag(ch, cur);
-    }
-}
-
-// every cell takes its component's new representative (grid-stride)
-__global__ void k_bv_relabel(int* __restrict__ lab, long long n, const int* __restrict__ par,
-                             const int* __restrict__ go)
+                             const int* __restrict__ go, int* __restrict__ stuck)
 {
     if (*go == 0) return;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += stride) {
-        const int a = lab[k];
-        if (a >= 0) {
-            const int b = par[a];
-            if (b != a) lab[k] = b;
+    const int cnt = *count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const int c = list[i];
+        int p = ld_rel(&par[c]);
+        if (p == c) continue;
+        for (int s = 0;; ++s) {
+            const int q = ld_rel(&par[p]);
+            if (q == p) break;
+            if (s == kWalkSteps) {
+                *stuck = 1;
+                break;
+            }
+            p = q;
+            st_rel(&par[c], p);
         }
     }
 }
 
-// Chain maxima: (up, mx) -> (up2, mx2) with mx2 = max(mx, mx(up)), up2 =
-// up(up), until every chain has reached the marker (or ends unhooked).  A
-// launch after the last one that moved returns at once; the one that ran
-// last records its index j (its output pair is j's parity).
-__global__ void k_bv_level(const int* __restrict__ up, const int* __restrict__ mx, int* __restrict__ up2,
-                           int* __restrict__ mx2, long long n, int X, int j, const int* __restrict__ prev,
-                           int* __restrict__ cur, int* __restrict__ last)
+// Hook chains as packed words, one per basin: hook weight key << 32 | hook
+// parent (-1 when the basin's component never hooked)
+__global__ void k_chain_pack(const int* __restrict__ roots, const int* __restrict__ count,
+                             const int* __restrict__ hp, const int* __restrict__ hw,
+                             unsigned long long* __restrict__ P)
 {
-    if (*prev == 0) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *last = j;
-    bool ch = false;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += stride) {
-        const int u = up[k];
-        int m = mx[k], nu = u;
-        if (u >= 0 && u != X) {
-            m = max(m, mx[u]);
-            nu = up[u];
-            ch |= nu >= 0 && nu != X;
-        }
-        up2[k] = nu;
-        mx2[k] = m;
+    const int cnt = *count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const int b = roots[i];
+        P[b] = (static_cast<unsigned long long>(static_cast<unsigned>(hw[b])) << 32) | static_cast<unsigned>(hp[b]);
     }
-    wave_flag(ch, cur);
 }
 
-// filled = max(z, L(root)) in a pit basin whose chain reached the marker,
+// Chain maxima in place: a basin's word (m, u) becomes (max(m, m_u), u_u)
+// from its parent u's word until u is the marker or hooks nowhere.  Every
+// stored word is (the largest hook weight from the basin up to u, u), so
+// words read while their owners update them still compose correctly.
+__global__ void k_chain_walk(const int* __restrict__ roots, const int* __restrict__ count,
+                             unsigned long long* __restrict__ P, int X, int* __restrict__ stuck)
+{
+    const int cnt = *count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const int b = roots[i];
+        unsigned long long w = __hip_atomic_load(&P[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int s = 0;; ++s) {
+            const int u = static_cast<int>(static_cast<unsigned>(w));
+            if (u < 0 || u == X) break;
+            if (s == kWalkSteps) {
+                *stuck = 1;
+                break;
+            }
+            const unsigned long long wu = __hip_atomic_load(&P[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int m = max(static_cast<int>(w >> 32), static_cast<int>(wu >> 32));
+            w = (static_cast<unsigned long long>(static_cast<unsigned>(m)) << 32) | (wu & 0xffffffffull);
+            __hip_atomic_store(&P[b], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// filled = max(z, L(basin)) in a pit basin whose chain reached the marker,
 // z otherwise; nodata for invalid cells
-__global__ void k_fill_out(const float* __restrict__ z, const uint8_t* __restrict__ mask, const int* __restrict__ zk,
-                           const int* __restrict__ ptr, const int* __restrict__ up_even,
-                           const int* __restrict__ mx_even, const int* __restrict__ up_odd,
-                           const int* __restrict__ mx_odd, const int* __restrict__ last, long long n, int X,
+__global__ void k_fill_out(const float* __restrict__ z, const uint8_t* __restrict__ mask, const int* __restrict__ wk,
+                           const int* __restrict__ lab, const unsigned long long* __restrict__ P, long long n, int X,
                            double nodata, int nan_nodata, float* __restrict__ out)
 {
     const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const bool odd = (*last & 1) != 0;
-    const int* __restrict__ up = odd ? up_odd : up_even;
-    const int* __restrict__ mx = odd ? mx_odd : mx_even;
     const float zv = z[k];
     const bool valid = mask[k] && !is_nodata(zv, nodata, nan_nodata);
     if (!valid) {
         out[k] = static_cast<float>(nodata);
         return;
     }
-    const int r = ptr[k];
+    const int b = lab[k];
     float v = zv;
-    if (zk[r] != kFixed && up[r] == X) v = kfloat(max(zk[k], mx[r]));
+    if (b != X) {
+        const unsigned long long w = P[b];
+        if (static_cast<int>(static_cast<unsigned>(w)) == X) v = kfloat(max(wk[k], static_cast<int>(w >> 32)));
+    }
     out[k] = v;
 }
 
-// the flags whose launches run unconditionally: the first descent jump, the
-// first round, the first flattening launch of every round, the first chain
-// launch
+// the flags whose launches run unconditionally: the first descent jump and
+// the first round
 __global__ void k_fill_start_flags(int* __restrict__ flags)
 {
     const int i = threadIdx.x;
     if (i == 0) flags[kFlagJump] = 1;
     if (i == 1) flags[kFlagRound] = 1;
-    if (i == 2) flags[kFlagLevel] = 1;
-    if (i < kMaxRounds) flags[kFlagPar + i * (kParJumps + 1)] = 1;
 }
+
+// descent jumps the last fill of this process needed, plus a spare: the
+// length of the next fill's first batch (a hint only; more follow when it
+// falls short)
+std::atomic<int> g_jump_hint{12};
 
 }  // namespace
 
 // ints of scratch: the cheapest edges (2 per slot, first, 8-byte aligned),
-// zk, wk, ptr, lab, two component lists, par, hp, hw and the flags
+// zk (then the basin list), wk, the labels, par, hp, hw, two component
+// lists, two boundary-cell lists, a keep byte per cell and the flags
 size_t fill_scratch_ints(int nr, int nc)
 {
     const long long n = (long long)nr * nc;
-    return static_cast<size_t>(2 * (n + 1) + 6 * n + 3 * (n + 1) + kFlagInts + 8);
+    return static_cast<size_t>(2 * (n + 1) + 3 * n + 3 * (n + 1) + 4 * n + (n + 3) / 4 + kFlagInts + 8);
 }
 
 // Enqueues the fill on `s` and returns the number of Boruvka rounds run.
 // `scratch` holds fill_scratch_ints(nr, nc) ints, 8-byte aligned.  Blocks on
-// one read-back per batch of rounds.
+// one read-back per batch of rounds and one at the end.
 int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double nodata, BC bc, float* out,
                 int* scratch, hipStream_t s)
 {
@@ -537,73 +763,117 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
     const int X = static_cast<int>(n);
     unsigned long long* best = reinterpret_cast<unsigned long long*>(scratch);
     int* zk = scratch + 2 * (n + 1);
+    int* roots = zk;  // once the labels are made
     int* wk = zk + n;
-    int* ptr = wk + n;
-    int* lab = ptr + n;
-    int* lists[2] = {lab + n, lab + 2 * n};
-    int* par = lab + 3 * n;
+    int* lab = wk + n;  // descent pointers, then labels
+    int* par = lab + n;
     int* hp = par + (n + 1);
     int* hw = hp + (n + 1);
-    int* flags = hw + (n + 1);
-    int* counts = flags + kFlagCount;
+    int* lists[2] = {hw + (n + 1), hw + (n + 1) + n};
+    int* blist[2] = {lists[1] + n, lists[1] + 2 * n};
+    uint8_t* keep = reinterpret_cast<uint8_t*>(blist[1] + n);
+    int* flags = blist[1] + n + (n + 3) / 4;
+    int* counts = flags + kFlagCount;    // [0], [1] component lists, [2] basins
+    int* bcount = flags + kFlagBoundary;  // boundary lists
     dim3 b2(64, 4), g2(div_up(nc, 64), div_up(nr, 4));
-    const int g1 = div_up(n, kB), g1x = div_up(n + 1, kB);
+    const int g1 = div_up(n, kB);
     const int gs = std::min(g1, kStrideGrid);  // grid-stride launches
     const dim3 bsc(kScanW, kScanH), gsc(div_up(nc, kScanW), div_up(nr, kScanH));
-    LSDTT_HIP(hipMemsetAsync(flags, 0, sizeof(int) * kFlagInts, s));
+    k_zero_ints<<<div_up(kFlagInts, kB), kB, 0, s>>>(flags, kFlagInts);
     k_fill_interior_invalid<<<g2, b2, 0, s>>>(z, mask, nr, nc, nodata, nan_nodata, flags);
     LSDTT_LAUNCH_CHECK();
     k_fill_init<<<g2, b2, 0, s>>>(z, mask, nr, nc, nodata, nan_nodata, per_ns, per_ew, flags, zk, wk);
     LSDTT_LAUNCH_CHECK();
-    k_fill_descent<<<g2, b2, 0, s>>>(zk, wk, nr, nc, per_ns, per_ew, ptr);
+    k_fill_descent<<<g2, b2, 0, s>>>(zk, wk, nr, nc, per_ns, per_ew, lab);
     LSDTT_LAUNCH_CHECK();
-    static_assert(kMaxRounds <= 64, "one thread per round flag");
     k_fill_start_flags<<<1, 64, 0, s>>>(flags);
     LSDTT_LAUNCH_CHECK();
-    for (int r = 0; r < kJumpRounds; ++r) {
-        k_fill_jump<<<gs, kB, 0, s>>>(ptr, n, flags + kFlagJump + r, flags + kFlagJump + r + 1);
-        LSDTT_LAUNCH_CHECK();
+    // descent jumps: a first batch as long as the last fill needed (plus a
+    // spare), the rest only when its last jump still moved a pointer
+    int fh[kFlagChunks];
+    auto read_flags = [&]() {
+        LSDTT_HIP(hipMemcpyAsync(fh, flags, sizeof(fh), hipMemcpyDeviceToHost, s));
+        LSDTT_HIP(hipStreamSynchronize(s));
+    };
+    auto jumps = [&](int r0, int r1) {
+        for (int r = r0; r < r1; ++r) {
+            k_fill_jump<<<gs, kB, 0, s>>>(lab, n, flags + kFlagJump + r, flags + kFlagJump + r + 1);
+            LSDTT_LAUNCH_CHECK();
+        }
+    };
+    const int first_jumps = std::min(kJumpRounds, g_jump_hint.load(std::memory_order_relaxed));
+    jumps(0, first_jumps);
+    read_flags();
+    if (fh[kFlagJump + first_jumps]) {
+        jumps(first_jumps, kJumpRounds);
+        read_flags();
+        if (fh[kFlagJump + kJumpRounds]) throw std::runtime_error("fill: descent pointers did not converge");
+    }
+    {
+        int moved = 0;
+        while (moved < kJumpRounds && fh[kFlagJump + moved + 1]) ++moved;
+        g_jump_hint.store(moved + 2, std::memory_order_relaxed);
     }
     int* chunk_count = flags + kFlagChunks;
     int* chunk_off = chunk_count + kChunkBlocks;
-    k_bv_label<<<g1x, kB, 0, s>>>(zk, wk, ptr, n, lab, par, hp, hw, best);
-    k_list_count<kRoots><<<kChunkBlocks, kB, 0, s>>>(lab, par, nullptr, n, nullptr, chunk_count, nullptr);
-    k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, counts, nullptr);
-    k_list_write<kRoots><<<kChunkBlocks, kB, 0, s>>>(lab, par, nullptr, n, nullptr, chunk_off, lists[0], best,
+    k_bv_label<<<kChunkBlocks, kB, 0, s>>>(zk, wk, lab, n, chunk_count);
+    k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, counts + 2, nullptr);
+    k_list_write<kRoots><<<kChunkBlocks, kB, 0, s>>>(lab, par, nullptr, nullptr, n, nullptr, chunk_off, roots, best,
                                                      nullptr);
+    k_basin_init<<<kScanGrid, kB, 0, s>>>(roots, counts + 2, X, par, hp, hw, best);
     LSDTT_LAUNCH_CHECK();
-    // Boruvka rounds in batches of 4 with one read-back of their flags; a
-    // round after the last hook returns at once
-    int launched = 0, rounds = 0;
-    for (const int batch = 4;;) {
+    // Boruvka rounds in batches with one read-back of their flags; a round
+    // after the last hook returns at once.  Round r reads the basin list
+    // (r = 0) or component list (r + 1) & 1 and writes list r & 1; its scan
+    // reads boundary list (r + 1) & 1 (the grid when r = 0), flags the cells
+    // that stay and compacts them, in order, into list r & 1.  After a batch
+    // the list lengths size the next: each round at least halves the
+    // components that can still hook, so at most log2(live) + 1 more rounds
+    // hook, and the grids follow the lengths.
+    auto grid_for = [](long long items) { return static_cast<int>(std::clamp<long long>(div_up(items, kB), 1, kScanGrid)); };
+    int list_grid = kScanGrid, scan_grid = kScanGrid, basin_grid = kScanGrid;
+    int launched = 0, rounds = 0, batch = 4;
+    for (;;) {
         const int end = std::min(kMaxRounds, launched + batch);
         for (int r = launched; r < end; ++r) {
             const int* go = flags + kFlagRound + r;
             int* hooked = flags + kFlagRound + r + 1;
-            const int cur = r & 1;
-            k_bv_scan<<<gsc, bsc, 0, s>>>(wk, lab, nr, nc, per_ns, per_ew, X, best, go);
-            k_bv_hook<<<kListGrid, kB, 0, s>>>(lab, best, nr, nc, per_ns, per_ew, X, lists[cur], counts + cur, par,
-                                               hp, hw, go, hooked);
-            int* pf = flags + kFlagPar + r * (kParJumps + 1);
-            for (int j = 0; j < kParJumps; ++j)
-                k_bv_flatten<<<kListGrid, kB, 0, s>>>(par, lists[cur], counts + cur, go, pf + j, pf + j + 1);
-            k_bv_relabel<<<gs, kB, 0, s>>>(lab, n, par, go);
-            k_list_count<kSurvivors><<<kChunkBlocks, kB, 0, s>>>(lab, par, lists[cur], 0, counts + cur, chunk_count,
+            const int* in_list = r == 0 ? roots : lists[(r + 1) & 1];
+            const int* in_count = r == 0 ? counts + 2 : counts + ((r + 1) & 1);
+            int* b_out = blist[r & 1];
+            int* b_out_count = bcount + (r & 1);
+            if (r == 0) {
+                k_bv_scan<<<gsc, bsc, 0, s>>>(wk, lab, nr, nc, per_ns, per_ew, X, best, keep);
+                k_list_count<kKeepCells><<<kChunkBlocks, kB, 0, s>>>(nullptr, nullptr, nullptr, keep, n, nullptr,
+                                                                     chunk_count, nullptr);
+                k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, b_out_count, nullptr);
+                k_list_write<kKeepCells><<<kChunkBlocks, kB, 0, s>>>(nullptr, nullptr, nullptr, keep, n, nullptr,
+                                                                     chunk_off, b_out, nullptr, nullptr);
+            } else {
+                const int* b_in = blist[(r + 1) & 1];
+                const int* b_in_count = bcount + ((r + 1) & 1);
+                k_bv_scan_list<<<scan_grid, kB, 0, s>>>(wk, lab, par, nr, nc, per_ns, per_ew, X, b_in, b_in_count,
+                                                         keep, best, go);
+                k_list_count<kKeepList><<<kChunkBlocks, kB, 0, s>>>(nullptr, nullptr, b_in, keep, 0, b_in_count,
+                                                                    chunk_count, go);
+                k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, b_out_count, go);
+                k_list_write<kKeepList><<<kChunkBlocks, kB, 0, s>>>(nullptr, nullptr, b_in, keep, 0, b_in_count,
+                                                                    chunk_off, b_out, nullptr, go);
+            }
+            k_bv_target<<<list_grid, kB, 0, s>>>(lab, par, best, nr, nc, per_ns, per_ew, in_list, in_count, hp, go);
+            k_bv_hook<<<list_grid, kB, 0, s>>>(best, X, in_list, in_count, par, hp, hw, go, hooked);
+            k_bv_flatten<<<list_grid, kB, 0, s>>>(par, in_list, in_count, go, flags + kFlagStuck);
+            k_bv_compress<<<basin_grid, kB, 0, s>>>(par, roots, counts + 2, go, flags + kFlagStuck);
+            k_list_count<kSurvivors><<<kChunkBlocks, kB, 0, s>>>(lab, par, in_list, nullptr, 0, in_count, chunk_count,
                                                                   go);
-            k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, counts + (cur ^ 1), go);
-            k_list_write<kSurvivors><<<kChunkBlocks, kB, 0, s>>>(lab, par, lists[cur], 0, counts + cur, chunk_off,
-                                                                  lists[cur ^ 1], best, go);
+            k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, counts + (r & 1), go);
+            k_list_write<kSurvivors><<<kChunkBlocks, kB, 0, s>>>(lab, par, in_list, nullptr, 0, in_count, chunk_off,
+                                                                  lists[r & 1], best, go);
             LSDTT_LAUNCH_CHECK();
         }
-        int fh[kFlagInts];
-        LSDTT_HIP(hipMemcpyAsync(fh, flags, sizeof(int) * kFlagInts, hipMemcpyDeviceToHost, s));
-        LSDTT_HIP(hipStreamSynchronize(s));
+        read_flags();
+        if (fh[kFlagStuck]) throw std::runtime_error("fill: a hook tree deeper than the walk bound");
         const int* h = fh + kFlagRound;
-        // a round whose last flattening launch still found a pointer short
-        // of its root would have relabelled with unflattened trees
-        for (int r = launched; r < end; ++r)
-            if (h[r] && fh[kFlagPar + r * (kParJumps + 1) + kParJumps])
-                throw std::runtime_error("fill: hook trees deeper than the flattening covers");
         launched = end;
         if (!h[end]) {
             rounds = 0;
@@ -611,22 +881,26 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
             break;
         }
         if (launched >= kMaxRounds) throw std::runtime_error("fill: Boruvka rounds did not finish");
+        const int live = fh[kFlagCount + ((end - 1) & 1)];
+        int lg = 0;
+        while ((1 << (lg + 1)) <= live) ++lg;
+        batch = std::min(4, lg + 2);
+        list_grid = grid_for(live);
+        scan_grid = grid_for(fh[kFlagBoundary + ((end - 1) & 1)]);
+        basin_grid = grid_for(fh[kFlagCount + 2]);
     }
-    // chain maxima, ping-pong between (hp, hw) and (lab, par) (both free
-    // now): launch j reads the pair j - 1 wrote and writes (lab, par) when j
-    // is even, (hp, hw) when odd
-    for (int j = 0; j < kLevelJumps; ++j) {
-        const int* up = (j & 1) ? lab : hp;
-        const int* mx = (j & 1) ? par : hw;
-        int* up2 = (j & 1) ? hp : lab;
-        int* mx2 = (j & 1) ? hw : par;
-        k_bv_level<<<gs, kB, 0, s>>>(up, mx, up2, mx2, n, X, j, flags + kFlagLevel + j, flags + kFlagLevel + j + 1,
-                                     flags + kFlagLast);
-        LSDTT_LAUNCH_CHECK();
-    }
-    k_fill_out<<<g1, kB, 0, s>>>(z, mask, zk, ptr, lab, par, hp, hw, flags + kFlagLast, n, X, nodata, nan_nodata,
-                                 out);
+    // chain maxima over the basins, in the cheapest-edge words (free now)
+    unsigned long long* P = best;
+    const int chain_grid = grid_for(fh[kFlagCount + 2]);
+    k_chain_pack<<<chain_grid, kB, 0, s>>>(roots, counts + 2, hp, hw, P);
+    k_chain_walk<<<chain_grid, kB, 0, s>>>(roots, counts + 2, P, X, flags + kFlagStuck);
     LSDTT_LAUNCH_CHECK();
+    k_fill_out<<<g1, kB, 0, s>>>(z, mask, wk, lab, P, n, X, nodata, nan_nodata, out);
+    LSDTT_LAUNCH_CHECK();
+    int stuck = 0;
+    LSDTT_HIP(hipMemcpyAsync(&stuck, flags + kFlagStuck, sizeof(int), hipMemcpyDeviceToHost, s));
+    LSDTT_HIP(hipStreamSynchronize(s));
+    if (stuck) throw std::runtime_error("fill: a hook chain longer than the walk bound");
     return rounds;
 }
 

@@ -266,30 +266,34 @@ void launch_receivers(const T* z, const uint8_t* msk, const uint8_t* forced, int
     LSDTT_LAUNCH_CHECK();
 }
 
-// donor count = number of distinct valid neighbours whose receiver is f
+// donor count = number of distinct valid neighbours whose receiver is f;
+// dmask bit i set when the i-th of distinct_neighbours(f) is one
 __global__ void k_donors(const int* __restrict__ rcv, const uint8_t* __restrict__ mask, int nr, int nc,
-                         BC bc, uint8_t* __restrict__ donors, int* __restrict__ outlet_flag)
+                         BC bc, uint8_t* __restrict__ donors, uint8_t* __restrict__ dmask,
+                         int* __restrict__ outlet_flag)
 {
     const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (f >= (long long)nr * nc) return;
     if (!mask[f]) {
         donors[f] = 0;
+        dmask[f] = 0;
         outlet_flag[f] = 0;
         return;
     }
     int nb[8];
     const int k = distinct_neighbours(static_cast<int>(f), nr, nc, bc, nb);
-    int c = 0;
+    unsigned bits = 0;
     if (k == 8) {
         int rv[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) rv[i] = rcv[nb[i]];  // masked cells hold -1, never == f
 #pragma unroll
-        for (int i = 0; i < 8; ++i) c += (rv[i] == f) ? 1 : 0;
+        for (int i = 0; i < 8; ++i) bits |= (rv[i] == f) ? (1u << i) : 0u;
     } else {
-        for (int i = 0; i < k; ++i) c += (mask[nb[i]] && rcv[nb[i]] == f) ? 1 : 0;
+        for (int i = 0; i < k; ++i) bits |= (mask[nb[i]] && rcv[nb[i]] == f) ? (1u << i) : 0u;
     }
-    donors[f] = static_cast<uint8_t>(c);
+    donors[f] = static_cast<uint8_t>(__popc(bits));
+    dmask[f] = static_cast<uint8_t>(bits);
     outlet_flag[f] = (rcv[f] == kOutletReceiver) ? 1 : 0;
 }
 
@@ -354,11 +358,16 @@ __global__ void k_pj_round(unsigned long long* __restrict__ P, long long n, cons
 // max of values (depth) and valid-cell count when `stats` is set.
 // grid-stride (a few thousand blocks) so the two global atomics per block
 // stay uncontended
+// gate: when not null, the extraction runs only if *gate == 0 (the last
+// launched round changed nothing)
+// stats (when not null): [0] max value, [1] valid cells, [2] valid cells
+// without donors (sources)
 __global__ void k_pj_extract(const unsigned long long* __restrict__ P, const uint8_t* __restrict__ mask,
-                             long long n, int* __restrict__ value, int* __restrict__ root,
-                             int* __restrict__ stats)
+                             const uint8_t* __restrict__ donors, long long n, int* __restrict__ value,
+                             int* __restrict__ root, int* __restrict__ stats, const int* __restrict__ gate)
 {
-    int v = -1, valid = 0;
+    if (gate && *gate != 0) return;
+    int v = -1, valid = 0, sources = 0;
     for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
          f += (long long)gridDim.x * blockDim.x) {
         int vf = -1;
@@ -369,6 +378,7 @@ __global__ void k_pj_extract(const unsigned long long* __restrict__ P, const uin
             if (nx != f) acc += static_cast<unsigned>(P[nx] >> 32);
             vf = static_cast<int>(acc);
             ++valid;
+            if (stats) sources += donors[f] == 0;
             if (root) root[f] = static_cast<int>(nx);
         } else if (root) {
             root[f] = -1;
@@ -383,11 +393,31 @@ __global__ void k_pj_extract(const unsigned long long* __restrict__ P, const uin
         BR().reduce(m, m, s1, rocprim::maximum<int>());
         __syncthreads();
         BR().reduce(c, c, s1);
+        __syncthreads();
+        int so = sources;
+        BR().reduce(so, so, s1);
         if (threadIdx.x == 0) {
             atomicMax(&stats[0], m);
             atomicAdd(&stats[1], c);
+            atomicAdd(&stats[2], so);
         }
     }
+}
+
+// small device-side initialisations (no copies from pageable host memory):
+// p[0..2] = a, b, c and p[3..count) = rest
+__global__ void k_init_ints(int* __restrict__ p, int count, int a, int b, int c, int rest)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = i == 0 ? a : i == 1 ? b : i == 2 ? c : rest;
+}
+
+// a few single ints gathered into one block (after the round flags), for
+// one read-back
+__global__ void k_collect_ints(FlowEngine::ReadInts extra, int* __restrict__ out)
+{
+    const int i = threadIdx.x;
+    if (i < extra.count) out[i] = *extra.src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -473,36 +503,75 @@ __global__ void __launch_bounds__(1024) k_level_scatter(const int* __restrict__ 
 struct UpArgs {
     const int* rcv;
     const uint8_t* mask;
+    const uint8_t* dmask;  // donor bits per cell, in distinct_neighbours order
     int nr, nc;
     BC bc;
     int* cnt;
     unsigned long long* key;  // null: counts only
     const double* w;          // null: no weighted accumulation
     double* acc;
+    int* off;                 // null: no s-vector sibling offsets
 };
 
 __device__ __forceinline__ void up_node(const UpArgs& a, int v)
 {
+    // the donors (from the routing's donor bits), then every donor's count
+    // and key, issued before any is used
+    const unsigned dm = a.dmask[v];
+    if (dm == 0) {  // a source
+        a.cnt[v] = 1;
+        if (a.key) a.key[v] = static_cast<unsigned long long>(static_cast<unsigned>(v));
+        if (a.acc) a.acc[v] = a.w[v];
+        return;
+    }
     int nb[8];
-    const int k = distinct_neighbours(v, a.nr, a.nc, a.bc, nb);
-    int dn[8];
-    int nd = 0;
-    for (int i = 0; i < k; ++i)
-        if (a.mask[nb[i]] && a.rcv[nb[i]] == v) dn[nd++] = nb[i];
+    distinct_neighbours(v, a.nr, a.nc, a.bc, nb);
+    int cv[8];
+    unsigned long long kv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        cv[i] = 0;
+        kv[i] = 0;
+        if ((dm >> i) & 1u) {
+            cv[i] = a.cnt[nb[i]];
+            if (a.key) kv[i] = a.key[nb[i]];
+        }
+    }
     int c = 1;
     unsigned long long mk = 0;
-    for (int i = 0; i < nd; ++i) {
-        c += a.cnt[dn[i]];
-        if (a.key) mk = max(mk, a.key[dn[i]]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        c += cv[i];
+        mk = max(mk, kv[i]);
     }
     a.cnt[v] = c;
     if (a.key)
-        a.key[v] = nd ? ((((mk >> 32) + 1ull) << 32) | (mk & 0xffffffffull))
+        a.key[v] = dm ? ((((mk >> 32) + 1ull) << 32) | (mk & 0xffffffffull))
                       : static_cast<unsigned long long>(static_cast<unsigned>(v));
+    if (a.off) {
+        // s-vector sibling offsets: donors are visited in ascending flat
+        // order, each after the subtrees of the donors before it
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (!((dm >> i) & 1u)) continue;
+            int o = 1;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (((dm >> j) & 1u) && nb[j] < nb[i]) o += cv[j];
+            a.off[nb[i]] = o;
+        }
+    }
     if (a.acc) {
         // donors in topological order = ascending Kahn key
+        int dn[8];
         unsigned long long dk[8];
-        for (int i = 0; i < nd; ++i) dk[i] = a.key[dn[i]];
+        int nd = 0;
+        for (int i = 0; i < 8; ++i)
+            if ((dm >> i) & 1u) {
+                dn[nd] = nb[i];
+                dk[nd] = kv[i];
+                ++nd;
+            }
         for (int i = 1; i < nd; ++i) {
             const unsigned long long kk = dk[i];
             const int vv = dn[i];
@@ -568,14 +637,134 @@ __global__ void __launch_bounds__(1024) k_cnt_tail(const int* __restrict__ order
     }
 }
 
-// compact Kahn keys: (height << nbits) | last-donor ancestor
-__global__ void k_topo_keys(const int* __restrict__ order, int nv, const unsigned long long* __restrict__ key,
-                            int nbits, unsigned long long* __restrict__ out)
+// compact Kahn keys: (height << nbits) | last-donor ancestor, in 32 bits
+// when they fit
+template <class K>
+__global__ void k_topo_keys(const int* __restrict__ cells, int m, const unsigned long long* __restrict__ key,
+                            int nbits, K* __restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nv) {
-        const unsigned long long k = key[order[i]];
-        out[i] = ((k >> 32) << nbits) | (k & 0xffffffffull);
+    if (i < m) {
+        const unsigned long long k = key[cells[i]];
+        out[i] = static_cast<K>(((k >> 32) << nbits) | (k & 0xffffffffull));
+    }
+}
+
+// Stable split of the valid cells into sources (no donors: Kahn key (0, own
+// index)) and the others, each in flat order, in three launches: per-chunk
+// counts, one scan of the chunk counts, the writes.  Each thread takes four
+// consecutive cells per step, their loads issued together.
+constexpr int kPartChunks = 1024;
+__device__ __forceinline__ long long part_chunk_len(long long n)
+{
+    const long long step = 4ll * kBlock;
+    return ((n + kPartChunks - 1) / kPartChunks + step - 1) / step * step;
+}
+// class of cell f: 0 source, 1 other valid cell, 2 invalid
+__device__ __forceinline__ int part_class(const uint8_t* __restrict__ mask, const uint8_t* __restrict__ donors,
+                                          long long f, long long n)
+{
+    if (f >= n) return 2;
+    const uint8_t mv = mask[f], dv = donors[f];
+    return !mv ? 2 : (dv == 0 ? 0 : 1);
+}
+__global__ void __launch_bounds__(kBlock) k_part_count(const uint8_t* __restrict__ mask,
+                                                       const uint8_t* __restrict__ donors, long long n,
+                                                       int* __restrict__ counts)
+{
+    const long long len = part_chunk_len(n);
+    const long long b0 = blockIdx.x * len, b1 = std::min(n, b0 + len);
+    int c0 = 0, c1 = 0;
+    for (long long f = b0 + 4ll * threadIdx.x; f < b1; f += 4ll * blockDim.x) {
+        int k[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) k[j] = part_class(mask, donors, f + j, b1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c0 += k[j] == 0;
+            c1 += k[j] == 1;
+        }
+    }
+    using BR = rocprim::block_reduce<int, kBlock>;
+    __shared__ typename BR::storage_type sr;
+    BR().reduce(c0, c0, sr);
+    __syncthreads();
+    BR().reduce(c1, c1, sr);
+    if (threadIdx.x == 0) {
+        counts[blockIdx.x] = c0;
+        counts[kPartChunks + blockIdx.x] = c1;
+    }
+}
+// exclusive scans of both count rows (one block of kPartChunks threads)
+__global__ void __launch_bounds__(kPartChunks) k_part_scan(int* __restrict__ counts)
+{
+    __shared__ int sc[2][kPartChunks];
+    const int t = threadIdx.x;
+    const int a0 = counts[t], a1 = counts[kPartChunks + t];
+    sc[0][t] = a0;
+    sc[1][t] = a1;
+    __syncthreads();
+    for (int o = 1; o < kPartChunks; o <<= 1) {
+        const int u0 = t >= o ? sc[0][t - o] : 0, u1 = t >= o ? sc[1][t - o] : 0;
+        __syncthreads();
+        sc[0][t] += u0;
+        sc[1][t] += u1;
+        __syncthreads();
+    }
+    counts[t] = sc[0][t] - a0;
+    counts[kPartChunks + t] = sc[1][t] - a1;
+}
+// exclusive prefix over the block of per-thread values (in thread order);
+// returns the prefix and sets `total`
+__device__ __forceinline__ int block_prefix(int v, int& total)
+{
+    __shared__ int ws[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[wave] = x;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        before += w < wave ? ws[w] : 0;
+        total += ws[w];
+    }
+    __syncthreads();
+    return before + x - v;
+}
+__global__ void __launch_bounds__(kBlock) k_part_write(const uint8_t* __restrict__ mask,
+                                                       const uint8_t* __restrict__ donors, long long n,
+                                                       const int* __restrict__ offs, int* __restrict__ out0,
+                                                       int* __restrict__ out1)
+{
+    const long long len = part_chunk_len(n);
+    const long long b0 = blockIdx.x * len, b1 = std::min(n, b0 + len);
+    int base0 = offs[blockIdx.x], base1 = offs[kPartChunks + blockIdx.x];
+    for (long long f0 = b0; f0 < b1; f0 += 4ll * blockDim.x) {
+        const long long f = f0 + 4ll * threadIdx.x;
+        int k[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) k[j] = part_class(mask, donors, f + j, b1);
+        int n0 = 0, n1 = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            n0 += k[j] == 0;
+            n1 += k[j] == 1;
+        }
+        int t0, t1;
+        int p0 = block_prefix(n0, t0);
+        int p1 = block_prefix(n1, t1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (k[j] == 0) out0[base0 + p0++] = static_cast<int>(f + j);
+            if (k[j] == 1) out1[base1 + p1++] = static_cast<int>(f + j);
+        }
+        base0 += t0;
+        base1 += t1;
     }
 }
 
@@ -588,7 +777,7 @@ __global__ void k_sv_offsets(const int* __restrict__ rcv, const uint8_t* __restr
     const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (f >= (long long)nr * nc) return;
     const int r = rcv[f];
-    if (!mask[f] || r < 0) return;  // roots are written by k_sv_outlet_start
+    if (!mask[f] || r < 0) return;  // roots start at their outlet's prefix
     int nb[8];
     const int k = distinct_neighbours(r, nr, nc, bc, nb);
     int s = 1;
@@ -605,17 +794,45 @@ __global__ void k_gather_outlet_sizes(const int* __restrict__ outlets, int no, c
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < no) out[i] = cnt[outlets[i]];
 }
+// s-vector positions, outlets first (their subtrees in row-major order),
+// then level by level away from them: a cell's subtree starts at its
+// receiver's position plus its sibling offset
 __global__ void k_sv_outlet_start(const int* __restrict__ outlets, int no, const int* __restrict__ pref,
-                                  int* __restrict__ off)
+                                  int* __restrict__ start, int* __restrict__ svec)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < no) off[outlets[i]] = pref[i];
+    if (i < no) {
+        const int v = outlets[i], st = pref[i];
+        start[v] = st;
+        svec[st] = v;
+    }
 }
-__global__ void k_sv_scatter(const int* __restrict__ start, const uint8_t* __restrict__ mask, long long n,
-                             int* __restrict__ svec)
+struct SvArgs {
+    const int* rcv;
+    const int* off;
+    int* start;
+    int* svec;
+};
+__device__ __forceinline__ void sv_node(const SvArgs& a, int v)
 {
-    const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (f < n && mask[f]) svec[start[f]] = static_cast<int>(f);
+    const int st = a.start[a.rcv[v]] + a.off[v];
+    a.start[v] = st;
+    a.svec[st] = v;
+}
+__global__ void k_sv_level(const int* __restrict__ order, int begin, int end, SvArgs a)
+{
+    const int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < end) sv_node(a, order[i]);
+}
+// one workgroup walks levels lo..hi (each reads the level before it)
+__global__ void __launch_bounds__(1024) k_sv_tail(const int* __restrict__ order, const int* __restrict__ lstart,
+                                                  int lo, int hi, SvArgs a)
+{
+    for (int L = lo; L <= hi; ++L) {
+        const int b = lstart[L], e = lstart[L + 1];
+        for (int i = b + threadIdx.x; i < e; i += blockDim.x) sv_node(a, order[i]);
+        __syncthreads();
+    }
 }
 
 inline int bit_length(unsigned long long x)
@@ -638,6 +855,7 @@ FlowEngine::FlowEngine(const FlowOptions& o) : opt(o)
     rcv.reserve(n);
     dir.reserve(n);
     donors.reserve(n);
+    dmask.reserve(n);
     depth.reserve(n);
     root.reserve(n);
     pj.reserve(n);
@@ -645,18 +863,21 @@ FlowEngine::FlowEngine(const FlowOptions& o) : opt(o)
     scan_out.reserve(n);
     outlets.reserve(n);
     misc.reserve(8);
-    pjflags.reserve(40);
+    pjflags.reserve(kReadSlot + 8);
+    hread.reserve(kReadSlot + 8);
 }
 
 template <class T>
 long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* forced, hipStream_t s)
 {
     mask = msk;
+    off_ready = false;
     const int nr = opt.nr, nc = opt.nc;
     const int g1 = div_up(n, kBlock);
-    // misc[0] = first unresolved flat, misc[1] = max depth, misc[2] = n_valid
-    int init[3] = {0x7fffffff, -1, 0};
-    LSDTT_HIP(hipMemcpyAsync(misc.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    // misc[0] = first unresolved flat, misc[1] = max depth, misc[2] = n_valid,
+    // misc[3] = valid cells without donors
+    k_init_ints<<<1, 64, 0, s>>>(misc.p, 4, 0x7fffffff, -1, 0, 0);
+    LSDTT_LAUNCH_CHECK();
     {
         TScope ts_(timer, "receivers", s, (sizeof(T) + 6.0) * n);
         launch_receivers<T>(z, msk, forced, nr, nc, opt.dx, opt.bc, opt.interior_nodata_is_outlet,
@@ -664,7 +885,7 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
     }
     {
         TScope ts_(timer, "donors", s, 9.0 * n);
-        k_donors<<<g1, kBlock, 0, s>>>(rcv, msk, nr, nc, opt.bc, donors, flags);
+        k_donors<<<g1, kBlock, 0, s>>>(rcv, msk, nr, nc, opt.bc, donors, dmask, flags);
     }
     LSDTT_LAUNCH_CHECK();
     size_t tb = 0;
@@ -673,18 +894,22 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
     k_scatter_flagged<<<g1, kBlock, 0, s>>>(flags, scan_out, n, outlets);
     LSDTT_LAUNCH_CHECK();
 
-    pointer_jump(nullptr, depth, root, s);
-
-    int h[3];
-    int last_sum[2];
-    LSDTT_HIP(hipMemcpyAsync(h, misc.p, sizeof(h), hipMemcpyDeviceToHost, s));
-    LSDTT_HIP(hipMemcpyAsync(last_sum, scan_out.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-    LSDTT_HIP(hipMemcpyAsync(last_sum + 1, flags.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-    LSDTT_HIP(hipStreamSynchronize(s));
-    n_outlets = last_sum[0] + last_sum[1];
+    // one read-back with the pointer jumping's: misc and the outlet total
+    ReadInts extra;
+    extra.src[0] = misc.p;
+    extra.src[1] = misc.p + 1;
+    extra.src[2] = misc.p + 2;
+    extra.src[3] = scan_out.p + (n - 1);
+    extra.src[4] = flags.p + (n - 1);
+    extra.src[5] = misc.p + 3;
+    extra.count = 6;
+    pointer_jump(nullptr, depth, root, s, extra);
+    const int* h = hread.p + kReadSlot;
+    n_outlets = h[3] + h[4];
     if (h[0] != 0x7fffffff) return -(static_cast<long long>(h[0]) + 1);
     max_depth = h[1];
     n_valid = h[2];
+    n_sources = h[5];
 
     level_start_h.assign(static_cast<size_t>(std::max(max_depth, 0)) + 2, 0);
     if (n_valid == 0) {
@@ -725,6 +950,7 @@ template <class T>
 void FlowEngine::receivers_only(const T* z, const uint8_t* msk, hipStream_t s, T* zcopy, const TileMark& mark)
 {
     mask = msk;
+    off_ready = false;
     TScope ts_(timer, "receivers", s, (sizeof(T) + 6.0) * n);
     launch_receivers<T>(z, msk, nullptr, opt.nr, opt.nc, opt.dx, opt.bc, opt.interior_nodata_is_outlet,
                         opt.allow_pits_as_outlets, rcv, dir, misc.p, s, zcopy, mark);
@@ -737,40 +963,66 @@ template void FlowEngine::receivers_only<float>(const float*, const uint8_t*, hi
 template long long FlowEngine::route<float>(const float*, const uint8_t*, const uint8_t*, hipStream_t);
 template long long FlowEngine::route<double>(const double*, const uint8_t*, const uint8_t*, hipStream_t);
 
-void FlowEngine::pointer_jump(const int* weight, int* value_out, int* root_out, hipStream_t s)
+void FlowEngine::pointer_jump(const int* weight, int* value_out, int* root_out, hipStream_t s,
+                              const ReadInts& extra)
 {
-    const int g1 = div_up(n, kBlock);
+    const int g1 = div_up(n, kBlock), gs = std::min(g1, 2048);
     k_pj_init<<<g1, kBlock, 0, s>>>(rcv, mask, weight, n, pj);
     LSDTT_LAUNCH_CHECK();
-    // flags[r] == 1 while round r-1 changed something; enough rounds for a
-    // path of 2^31 cells
-    constexpr int kRounds = 34;
+    // chg[r] == 1 while round r-1 changed something; a round after
+    // convergence returns at once
     int* chg = pjflags.p;
-    LSDTT_HIP(hipMemsetAsync(chg, 0, sizeof(int) * (kRounds + 1), s));
-    const int one = 1;
-    LSDTT_HIP(hipMemcpyAsync(chg, &one, sizeof(int), hipMemcpyHostToDevice, s));
-    for (int r = 0; r < kRounds; ++r) {
-        {
-            TScope ts_(timer, "pointer_jump", s, 24.0 * n);
-            k_pj_round<<<std::min(g1, 2048), kBlock, 0, s>>>(pj, n, chg + r, chg + r + 1);
-        }
-        LSDTT_LAUNCH_CHECK();
-    }
-    k_pj_extract<<<std::min(g1, 2048), kBlock, 0, s>>>(pj, mask, n, value_out, root_out,
-                                                       weight ? nullptr : misc.p + 1);
+    k_init_ints<<<1, 64, 0, s>>>(chg, kPjRounds + 1, 1, 0, 0, 0);
     LSDTT_LAUNCH_CHECK();
-    int last = 0;
-    LSDTT_HIP(hipMemcpyAsync(&last, chg + kRounds, sizeof(int), hipMemcpyDeviceToHost, s));
-    LSDTT_HIP(hipStreamSynchronize(s));
-    if (last != 0) throw std::runtime_error("pointer jumping did not converge");
+    int* stats = weight ? nullptr : misc.p + 1;
+    auto rounds = [&](int r0, int r1) {
+        for (int r = r0; r < r1; ++r) {
+            {
+                TScope ts_(timer, "pointer_jump", s, 24.0 * n);
+                k_pj_round<<<gs, kBlock, 0, s>>>(pj, n, chg + r, chg + r + 1);
+            }
+            LSDTT_LAUNCH_CHECK();
+        }
+    };
+    auto read_back = [&]() {
+        static_assert(kPjRounds + 1 <= kReadSlot && kReadSlot + 8 <= 64, "read-back slots");
+        if (extra.count > 0) {
+            k_collect_ints<<<1, 64, 0, s>>>(extra, pjflags.p + kReadSlot);
+            LSDTT_LAUNCH_CHECK();
+        }
+        LSDTT_HIP(hipMemcpyAsync(hread.p, pjflags.p, sizeof(int) * (kReadSlot + 8), hipMemcpyDeviceToHost, s));
+        LSDTT_HIP(hipStreamSynchronize(s));
+    };
+    // a first batch as long as the last call needed (plus a spare): the
+    // extraction runs only if the batch's last round changed nothing, else
+    // the remaining rounds and the extraction follow the read-back
+    const int first = std::min(kPjRounds, pj_hint);
+    rounds(0, first);
+    k_pj_extract<<<gs, kBlock, 0, s>>>(pj, mask, donors, n, value_out, root_out, stats, chg + first);
+    LSDTT_LAUNCH_CHECK();
+    read_back();
+    if (hread.p[first] != 0) {
+        if (first == kPjRounds) throw std::runtime_error("pointer jumping did not converge");
+        rounds(first, kPjRounds);
+        k_pj_extract<<<gs, kBlock, 0, s>>>(pj, mask, donors, n, value_out, root_out, stats, nullptr);
+        LSDTT_LAUNCH_CHECK();
+        read_back();
+        if (hread.p[kPjRounds] != 0) throw std::runtime_error("pointer jumping did not converge");
+    }
+    int changed = 0;
+    while (changed < kPjRounds && hread.p[changed + 1] != 0) ++changed;
+    pj_hint = changed + 2;
 }
 
-void FlowEngine::upstream(bool keys, const double* w, double* acc, hipStream_t s)
+void FlowEngine::upstream(bool keys, const double* w, double* acc, hipStream_t s, bool sv_offsets)
 {
     cnt.reserve(n);
     if (keys || w) key.reserve(n);
+    if (sv_offsets) off.reserve(n);
+    off_ready = sv_offsets;
     if (n_valid == 0) return;
-    UpArgs a{rcv, mask, opt.nr, opt.nc, opt.bc, cnt, (keys || w) ? key.p : nullptr, w, w ? acc : nullptr};
+    UpArgs a{rcv, mask, dmask, opt.nr, opt.nc, opt.bc, cnt, (keys || w) ? key.p : nullptr, w, w ? acc : nullptr,
+             sv_offsets ? off.p : nullptr};
     const int nl = max_depth + 1;
     if (tail_from < nl) {
         k_up_tail<<<1, 1024, 0, s>>>(order, level_start, tail_from, nl - 1, a);
@@ -819,15 +1071,37 @@ void FlowEngine::topological_order(hipStream_t s)
     if (n_valid == 0) return;
     sort_keys_a.reserve(n_valid);
     sort_keys_b.reserve(n_valid);
-    const int nbits = std::max(1, bit_length(static_cast<unsigned long long>(n - 1)));
-    k_topo_keys<<<div_up(n_valid, kBlock), kBlock, 0, s>>>(order, n_valid, key, nbits, sort_keys_a);
+    // Sources sort first (height 0), by their own index: a stable partition
+    // puts them in place; only the other cells go through the radix sort.
+    const int ns = n_sources, m = n_valid - n_sources;
+    topo_inner.reserve(n_valid);
+    int* inner = topo_inner.p;
+    part_counts.reserve(2 * kPartChunks);
+    k_part_count<<<kPartChunks, kBlock, 0, s>>>(mask, donors, n, part_counts);
+    k_part_scan<<<1, kPartChunks, 0, s>>>(part_counts);
+    k_part_write<<<kPartChunks, kBlock, 0, s>>>(mask, donors, n, part_counts, topo.p, inner);
     LSDTT_LAUNCH_CHECK();
-    const int end_bit = std::min(64, nbits + bit_length(static_cast<unsigned long long>(max_depth) + 1));
+    if (m == 0) return;
     size_t tb = 0;
-    LSDTT_HIP(rocprim::radix_sort_pairs(nullptr, tb, sort_keys_a.p, sort_keys_b.p, order.p, topo.p,
-                                        (size_t)n_valid, 0, end_bit, s));
-    LSDTT_HIP(rocprim::radix_sort_pairs(tmp.need(tb), tb, sort_keys_a.p, sort_keys_b.p, order.p, topo.p,
-                                        (size_t)n_valid, 0, end_bit, s));
+    const int nbits = std::max(1, bit_length(static_cast<unsigned long long>(n - 1)));
+    const int end_bit = std::min(64, nbits + bit_length(static_cast<unsigned long long>(max_depth) + 1));
+    int* out = topo.p + ns;
+    if (end_bit <= 32) {
+        // half the key bytes on every radix pass
+        unsigned* ka = reinterpret_cast<unsigned*>(sort_keys_a.p);
+        unsigned* kb = reinterpret_cast<unsigned*>(sort_keys_b.p);
+        k_topo_keys<unsigned><<<div_up(m, kBlock), kBlock, 0, s>>>(inner, m, key, nbits, ka);
+        LSDTT_LAUNCH_CHECK();
+        LSDTT_HIP(rocprim::radix_sort_pairs(nullptr, tb, ka, kb, inner, out, (size_t)m, 0, end_bit, s));
+        LSDTT_HIP(rocprim::radix_sort_pairs(tmp.need(tb), tb, ka, kb, inner, out, (size_t)m, 0, end_bit, s));
+        return;
+    }
+    k_topo_keys<unsigned long long><<<div_up(m, kBlock), kBlock, 0, s>>>(inner, m, key, nbits, sort_keys_a.p);
+    LSDTT_LAUNCH_CHECK();
+    LSDTT_HIP(rocprim::radix_sort_pairs(nullptr, tb, sort_keys_a.p, sort_keys_b.p, inner, out, (size_t)m, 0, end_bit,
+                                        s));
+    LSDTT_HIP(rocprim::radix_sort_pairs(tmp.need(tb), tb, sort_keys_a.p, sort_keys_b.p, inner, out, (size_t)m, 0,
+                                        end_bit, s));
 }
 
 void FlowEngine::s_vector(hipStream_t s)
@@ -837,6 +1111,10 @@ void FlowEngine::s_vector(hipStream_t s)
     off.reserve(n);
     if (n_valid == 0) return;
     const int g1 = div_up(n, kBlock);
+    if (!off_ready) {
+        k_sv_offsets<<<g1, kBlock, 0, s>>>(rcv, mask, cnt, opt.nr, opt.nc, opt.bc, off);
+        LSDTT_LAUNCH_CHECK();
+    }
     // outlet starts: exclusive scan of outlet sizes in row-major order
     k_gather_outlet_sizes<<<div_up(n_outlets, kBlock), kBlock, 0, s>>>(outlets, n_outlets, cnt, flags);
     LSDTT_LAUNCH_CHECK();
@@ -845,13 +1123,24 @@ void FlowEngine::s_vector(hipStream_t s)
                                       rocprim::plus<int>(), s));
     LSDTT_HIP(rocprim::exclusive_scan(tmp.need(tb), tb, flags.p, scan_out.p, 0, (size_t)n_outlets,
                                       rocprim::plus<int>(), s));
-    k_sv_outlet_start<<<div_up(n_outlets, kBlock), kBlock, 0, s>>>(outlets, n_outlets, scan_out, off);
+    k_sv_outlet_start<<<div_up(n_outlets, kBlock), kBlock, 0, s>>>(outlets, n_outlets, scan_out, start, svec);
     LSDTT_LAUNCH_CHECK();
-    k_sv_offsets<<<g1, kBlock, 0, s>>>(rcv, mask, cnt, opt.nr, opt.nc, opt.bc, off);
-    LSDTT_LAUNCH_CHECK();
-    pointer_jump(off, start, nullptr, s);
-    k_sv_scatter<<<g1, kBlock, 0, s>>>(start, mask, n, svec);
-    LSDTT_LAUNCH_CHECK();
+    // the other cells, level by level (levels from tail_from on in one
+    // workgroup)
+    const SvArgs a{rcv, off, start, svec};
+    const int nl = max_depth + 1;
+    const int split = std::max(tail_from, 1);
+    for (int L = 1; L < split && L < nl; ++L) {
+        const int b = level_start_h[L], e = level_start_h[L + 1];
+        if (e > b) {
+            k_sv_level<<<div_up(e - b, kBlock), kBlock, 0, s>>>(order, b, e, a);
+            LSDTT_LAUNCH_CHECK();
+        }
+    }
+    if (split < nl) {
+        k_sv_tail<<<1, 1024, 0, s>>>(order, level_start, split, nl - 1, a);
+        LSDTT_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace lsdtt_gpu

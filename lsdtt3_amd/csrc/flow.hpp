@@ -4,14 +4,19 @@
 // with a data-parallel pipeline whose outputs are byte-identical:
 //   receivers            stencil, strict `>` first-wins over dirs 0..7
 //   donor counts         gather of neighbours whose receiver is the cell
+//                        (and a donor bit per neighbour)
 //   outlets              flag + exclusive scan (row-major order kept)
 //   depth from outlet    pointer jumping on packed (distance, next) words
 //   level buckets        cells grouped by depth (counting sort)
-//   upstream sweep       levels deep -> shallow: counts, Kahn keys, weighted
-//                        accumulation (donors summed in topological order)
-//   topological order    radix sort of (height-from-source, last-donor
-//                        ancestor) keys == the reference's Kahn FIFO order
-//   s-vector             sibling offsets + pointer jumping from outlet starts
+//   upstream sweep       levels deep -> shallow: counts, Kahn keys, s-vector
+//                        sibling offsets, weighted accumulation (donors
+//                        summed in topological order)
+//   topological order    sources (height 0) first in row-major order, then a
+//                        radix sort of the other cells' (height-from-source,
+//                        last-donor ancestor) keys == the reference's Kahn
+//                        FIFO order
+//   s-vector             outlet starts (scan of outlet sizes), then levels
+//                        shallow -> deep: receiver's start + sibling offset
 #pragma once
 
 #include <vector>
@@ -39,6 +44,12 @@ struct TileMark {
     int nc = 0, th = 0, ntx = 0;
 };
 
+// device ints read back with the pointer jumping's round flags
+struct FlowReadInts {
+    const int* src[6] = {};
+    int count = 0;
+};
+
 class FlowEngine {
 public:
     explicit FlowEngine(const FlowOptions& o);
@@ -57,8 +68,9 @@ public:
                         const TileMark& mark = TileMark{});
 
     // Upstream sweep (requires route()).  keys: Kahn keys too; w/acc: optional
-    // weighted double accumulation (w null -> no accumulation).
-    void upstream(bool keys, const double* w, double* acc, hipStream_t s);
+    // weighted double accumulation (w null -> no accumulation); sv_offsets:
+    // the s-vector's sibling offsets too (s_vector() then skips its own pass)
+    void upstream(bool keys, const double* w, double* acc, hipStream_t s, bool sv_offsets = false);
     // counts only, by atomics (order-free): used on the LEM path
     void upstream_counts_fast(hipStream_t s);
     void topological_order(hipStream_t s);  // requires upstream(keys=true)
@@ -68,13 +80,15 @@ public:
     long long n = 0;
     const uint8_t* mask = nullptr;  // device, borrowed
     int n_valid = 0, n_outlets = 0, max_depth = 0;
+    int n_sources = 0;  // valid cells without donors
     std::vector<int> level_start_h;  // host copy, max_depth + 2 entries
 
     dbuf<int> rcv, outlets, depth, root, order, level_count, level_start, cursor;
-    dbuf<int> cnt, topo, start, svec, off;
-    dbuf<uint8_t> dir, donors;
+    dbuf<int> cnt, topo, topo_inner, start, svec, off;
+    dbuf<uint8_t> dir, donors, dmask;  // dmask: donor bits (distinct_neighbours order)
     dbuf<unsigned long long> pj, key, sort_keys_a, sort_keys_b;
-    dbuf<int> flags, scan_out, misc, pjflags;
+    dbuf<int> flags, scan_out, misc, pjflags, part_counts;
+    hbuf<int> hread;  // pinned read-back of the round flags and single ints
     dscratch tmp;
     KernelTimer* timer = nullptr;  // optional, owned by the caller
 
@@ -84,8 +98,17 @@ public:
     int tail_from = 0;
     static constexpr int kTailThreshold = 4096;
 
+    using ReadInts = FlowReadInts;
+    static constexpr int kPjRounds = 34;  // enough for a path of 2^31 cells
+    static constexpr int kReadSlot = 40;  // where the extra ints land in hread
+
 private:
-    void pointer_jump(const int* weight_or_null, int* value_out, int* root_out, hipStream_t s);
+    // rounds the last pointer jumping needed plus a spare (the first batch
+    // of the next one)
+    int pj_hint = 12;
+    bool off_ready = false;  // off holds the sibling offsets of the current routing
+    void pointer_jump(const int* weight_or_null, int* value_out, int* root_out, hipStream_t s,
+                      const ReadInts& extra = ReadInts{});
 };
 
 }  // namespace lsdtt_gpu
