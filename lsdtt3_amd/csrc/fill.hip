@@ -113,75 +113,124 @@ __device__ __forceinline__ void wave_flag(bool on, int* flag)
     const unsigned long long m = __ballot(on);
     if (on && (int)(threadIdx.x & 63) == __ffsll(m) - 1) *flag = 1;
 }
-__global__ void k_fill_interior_invalid(const float* __restrict__ z, const uint8_t* __restrict__ mask, int nr,
-                                        int nc, double nodata, int nan_nodata, int* __restrict__ flag)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y * blockDim.y + threadIdx.y;
-    if (r < 1 || r >= nr - 1 || c < 1 || c >= nc - 1) return;
-    const long long k = (long long)r * nc + c;
-    if (!mask[k] || is_nodata(z[k], nodata, nan_nodata)) *flag = 1;
-}
-
-// wk: key of every valid cell (kFixed when invalid); zk: the same for
-// non-exits, kFixed for exits and invalid cells.
+// wk: key of every valid cell (kFixed when invalid); flags an invalid cell
+// off the raster border (then cells beside invalid cells are exits too)
 __global__ void k_fill_init(const float* __restrict__ z, const uint8_t* __restrict__ mask, int nr, int nc,
-                            double nodata, int nan_nodata, int per_ns, int per_ew,
-                            const int* __restrict__ interior_invalid, int* __restrict__ zk,
-                            int* __restrict__ wk)
+                            double nodata, int nan_nodata, int* __restrict__ interior_invalid, int* __restrict__ wk)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y * blockDim.y + threadIdx.y;
-    if (r >= nr || c >= nc) return;
-    const long long k = (long long)r * nc + c;
-    const bool valid = mask[k] && !is_nodata(z[k], nodata, nan_nodata);
-    if (!valid) {
-        zk[k] = kFixed;
-        wk[k] = kFixed;
-        return;
+    const bool in = r < nr && c < nc;
+    bool inner_invalid = false;
+    if (in) {
+        const long long k = (long long)r * nc + c;
+        const float zv = z[k];
+        const bool valid = mask[k] && !is_nodata(zv, nodata, nan_nodata);
+        wk[k] = valid ? fkey(zv) : kFixed;
+        inner_invalid = !valid && r >= 1 && r < nr - 1 && c >= 1 && c < nc - 1;
     }
-    bool exit_cell = (!per_ns && (r == 0 || r == nr - 1)) || (!per_ew && (c == 0 || c == nc - 1));
-    if (!exit_cell && *interior_invalid) {
-        for (int d = 0; d < 8 && !exit_cell; ++d) {
-            const int rr = r + d8_row[d], cc = c + d8_col[d];
-            if (rr < 0 || rr >= nr || cc < 0 || cc >= nc) continue;
-            const long long q = (long long)rr * nc + cc;
-            if (!mask[q] || is_nodata(z[q], nodata, nan_nodata)) exit_cell = true;
-        }
-    }
-    const int key = fkey(z[k]);
-    zk[k] = exit_cell ? kFixed : key;
-    wk[k] = key;
+    wave_flag(inner_invalid, interior_invalid);
 }
 
-// Descent pointer: the valid 8-neighbour lowest in (key, flat index) order
-// when below the cell's own (key, index), else the cell itself -- so
-// equal-height plateaus chain into one basin, and (key, index) strictly
-// falls along every pointer (no cycles; paths never rise in the key order
-// that every min and max below uses).  Exits and invalid cells point at
-// themselves.
-__global__ void k_fill_descent(const int* __restrict__ zk, const int* __restrict__ wk, int nr, int nc, int per_ns,
-                               int per_ew, int* __restrict__ ptr)
+// Exits and descent pointers over 32 x 32 tiles staged in LDS with a
+// one-cell halo.  zk: the key of non-exits, kFixed for exits and invalid
+// cells.  Descent pointer: the valid 8-neighbour lowest in (key, flat
+// index) order when below the cell's own (key, index), else the cell
+// itself -- so equal-height plateaus chain into one basin, and (key, index)
+// strictly falls along every pointer (no cycles; paths never rise in the
+// key order that every min and max below uses).  Exits and invalid cells
+// point at themselves.  Before leaving the tile, every pointer is followed
+// inside it (pointer doubling in LDS) to the first cell outside the tile or
+// to its root, so the global jumps only cross tiles.
+constexpr int kDT = 32;
+__global__ void __launch_bounds__(kDT * 8) k_fill_descent(const int* __restrict__ wk, int nr, int nc, int per_ns,
+                                                          int per_ew, const int* __restrict__ interior_invalid,
+                                                          int* __restrict__ zk, int* __restrict__ ptr)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y * blockDim.y + threadIdx.y;
-    if (r >= nr || c >= nc) return;
-    const int k = r * nc + c;
-    int best = k;
-    if (zk[k] != kFixed) {
-        int bk = zk[k];
-        for (int d = 0; d < 8; ++d) {
-            const int q = nbr_flat(r, c, d, nr, nc, per_ns, per_ew);
-            if (q < 0) continue;
-            const int kq = wk[q];
-            if (kq == kFixed) continue;  // invalid
-            if (kq < bk || (kq == bk && q < best)) {
-                bk = kq;
-                best = q;
-            }
-        }
+    constexpr int TW = kDT + 2;
+    __shared__ int sk[TW * TW];
+    __shared__ int sq[TW * TW];  // flat index of the staged cell, -1 outside the grid
+    __shared__ int lp[kDT * kDT];
+    const int c0 = blockIdx.x * kDT, r0 = blockIdx.y * kDT;
+    const int tid = threadIdx.y * kDT + threadIdx.x;
+    for (int i = tid; i < TW * TW; i += kDT * 8) {
+        int rr = r0 - 1 + i / TW, cc = c0 - 1 + i % TW;
+        if (per_ns) rr = (rr + nr) % nr;
+        if (per_ew) cc = (cc + nc) % nc;
+        const bool ok = rr >= 0 && rr < nr && cc >= 0 && cc < nc;
+        const int q = ok ? rr * nc + cc : -1;
+        sq[i] = q;
+        sk[i] = ok ? wk[q] : kFixed;
     }
-    ptr[k] = best;
+    __syncthreads();
+    const bool inv_inside = *interior_invalid != 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int ty = threadIdx.y + 8 * j, tx = threadIdx.x;
+        const int r = r0 + ty, c = c0 + tx;
+        const int li = (ty + 1) * TW + tx + 1;
+        const int t = ty * kDT + tx;
+        if (r >= nr || c >= nc) {
+            lp[t] = t;
+            continue;
+        }
+        const int k = r * nc + c;
+        const int key = sk[li];
+        int best = k;
+        if (key != kFixed) {
+            bool exit_cell = (!per_ns && (r == 0 || r == nr - 1)) || (!per_ew && (c == 0 || c == nc - 1));
+            if (!exit_cell && inv_inside) {
+                // an in-bounds invalid neighbour (no wrap for this rule)
+                for (int d = 0; d < 8 && !exit_cell; ++d) {
+                    const int rr = r + d8_row[d], cc = c + d8_col[d];
+                    if (rr < 0 || rr >= nr || cc < 0 || cc >= nc) continue;
+                    exit_cell = sk[li + d8_row[d] * TW + d8_col[d]] == kFixed;
+                }
+            }
+            zk[k] = exit_cell ? kFixed : key;
+            if (!exit_cell) {
+                int bk = key;
+                for (int d = 0; d < 8; ++d) {
+                    const int lq = li + d8_row[d] * TW + d8_col[d];
+                    const int q = sq[lq];
+                    if (q < 0) continue;
+                    const int kq = sk[lq];
+                    if (kq == kFixed) continue;  // invalid
+                    if (kq < bk || (kq == bk && q < best)) {
+                        bk = kq;
+                        best = q;
+                    }
+                }
+            }
+        } else {
+            zk[k] = kFixed;
+        }
+        // local index of the target when it lies in the tile, else -(flat + 1)
+        const int br = best / nc, bc = best - br * nc;
+        const bool inside = br >= r0 && br < r0 + kDT && bc >= c0 && bc < c0 + kDT;
+        lp[t] = inside ? (br - r0) * kDT + (bc - c0) : -(best + 1);
+    }
+    __syncthreads();
+    // pointer doubling inside the tile: 10 rounds cover any chain of its
+    // 1024 cells; a pointer read while its owner updates it is still a cell
+    // of the same path
+    for (int it = 0; it < 10; ++it) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int t = (threadIdx.y + 8 * j) * kDT + threadIdx.x;
+            const int p = lp[t];
+            if (p >= 0 && p != t) lp[t] = lp[p];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int ty = threadIdx.y + 8 * j, tx = threadIdx.x;
+        const int r = r0 + ty, c = c0 + tx;
+        if (r >= nr || c >= nc) continue;
+        const int p = lp[ty * kDT + tx];
+        ptr[r * nc + c] = p >= 0 ? (r0 + p / kDT) * nc + (c0 + p % kDT) : -(p + 1);
+    }
 }
 
 // ptr[k] <- ptr[ptr[k]] until every pointer reaches a root (in place: a
@@ -219,10 +268,15 @@ __global__ void k_zero_ints(int* __restrict__ p, int count)
 
 // Labels, in place over the descent roots: lab[k] = X (= n, the marker) when
 // the cell's root is an exit, the root (its basin) otherwise, -1 when
-// invalid.  Also counts the pit basins (lab[k] == k) of each contiguous
-// chunk, the first launch of their list's compaction.
+// invalid.  Every slot starts as its own component (par[k] = k) with no
+// cheapest edge (the marker's slot is set apart); hook parents and weights
+// are only read where a hook wrote them.  Also counts the pit basins
+// (lab[k] == k) of each contiguous chunk, the first launch of their list's
+// compaction.
 __global__ void __launch_bounds__(kB) k_bv_label(const int* __restrict__ zk, const int* __restrict__ wk,
-                                                 int* __restrict__ lab, long long n, int* __restrict__ chunk_count)
+                                                 int* __restrict__ lab, long long n, int* __restrict__ par,
+                                                 unsigned long long* __restrict__ best,
+                                                 int* __restrict__ chunk_count)
 {
     const long long len = (n + gridDim.x - 1) / gridDim.x;
     const long long b0 = blockIdx.x * len, b1 = std::min(n, b0 + len);
@@ -242,6 +296,8 @@ __global__ void __launch_bounds__(kB) k_bv_label(const int* __restrict__ zk, con
             int v = -1;
             if (w[j] != kFixed) v = zk[r[j]] == kFixed ? static_cast<int>(n) : r[j];
             lab[k] = v;
+            par[k] = static_cast<int>(k);
+            best[k] = kNoEdge;
             c += v == k;
         }
     }
@@ -253,22 +309,6 @@ __global__ void __launch_bounds__(kB) k_bv_label(const int* __restrict__ zk, con
         int t = 0;
         for (int wv = 0; wv < kB / 64; ++wv) t += red[wv];
         chunk_count[blockIdx.x] = t;
-    }
-}
-
-// Every basin slot and the marker's start as their own component with no
-// hook (hp -1, hw INT_MIN) and no cheapest edge; no other slot is read
-__global__ void k_basin_init(const int* __restrict__ roots, const int* __restrict__ count, int X,
-                             int* __restrict__ par, int* __restrict__ hp, int* __restrict__ hw,
-                             unsigned long long* __restrict__ best)
-{
-    const int cnt = *count;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= cnt; i += gridDim.x * blockDim.x) {
-        const int b = i < cnt ? roots[i] : X;
-        par[b] = b;
-        hp[b] = -1;
-        hw[b] = INT_MIN;
-        best[b] = kNoEdge;
     }
 }
 
@@ -447,13 +487,14 @@ __device__ __forceinline__ void offer(int A, unsigned long long m, unsigned long
 // fold, the run heads meet in an LDS table keyed by component, so the block
 // issues one global atomic per component it saw (large components late in
 // the rounds would otherwise take one per run from every wavefront)
-__device__ __forceinline__ void offer_block(int A, unsigned long long m, unsigned long long* __restrict__ best)
+__device__ __forceinline__ void offer_block(int A, unsigned long long m, unsigned long long* __restrict__ best,
+                                            int tid)
 {
     __shared__ int tkey[kB];
     __shared__ unsigned long long tval[kB];
-    const int lane = threadIdx.x & 63;
-    tkey[threadIdx.x] = -1;
-    tval[threadIdx.x] = kNoEdge;
+    const int lane = tid & 63;
+    tkey[tid] = -1;
+    tval[tid] = kNoEdge;
     for (int o = 1; o < 64; o <<= 1) {
         const unsigned long long om = __shfl_down(m, o);
         const int oa = __shfl_down(A, o);
@@ -475,24 +516,28 @@ __device__ __forceinline__ void offer_block(int A, unsigned long long m, unsigne
         }
     }
     __syncthreads();
-    const int key = tkey[threadIdx.x];
-    const unsigned long long v = tval[threadIdx.x];
+    const int key = tkey[tid];
+    const unsigned long long v = tval[tid];
     if (key >= 0 && v < best[key]) atomicMin(&best[key], v);
     __syncthreads();
 }
 
-// First round: each cell of a non-marker component offers its cheapest edge
-// to another component, and the cells with such an edge are flagged for the
-// boundary list.  Components are the basins themselves (no hook yet), so the labels
-// are read directly.  A block covers 64 x 4 cells; their labels and keys,
-// with a one-cell halo (periodic axes wrap; -1 outside the grid), are
-// staged in LDS first.
+// Early rounds, over the whole grid: each cell of a non-marker component
+// offers its cheapest edge to another component, and (keep_out set) the
+// cells with such an edge are flagged for the boundary list.  A block covers
+// 64 x 4 cells; their components and keys, with a one-cell halo (periodic
+// axes wrap; -1 outside the grid), are staged in LDS first.  In the first
+// round the components are the basins themselves (no hook yet), so the
+// labels are read directly; later a cell's component is its basin's parent.
 constexpr int kScanW = 64, kScanH = 4;
+template <bool VIA_PARENT>
 __global__ void __launch_bounds__(kScanW * kScanH) k_bv_scan(const int* __restrict__ wk, const int* __restrict__ lab,
-                                                             int nr, int nc, int per_ns, int per_ew, int X,
-                                                             unsigned long long* __restrict__ best,
-                                                             uint8_t* __restrict__ keep_out)
+                                                             const int* __restrict__ par, int nr, int nc, int per_ns,
+                                                             int per_ew, int X, unsigned long long* __restrict__ best,
+                                                             uint8_t* __restrict__ keep_out,
+                                                             const int* __restrict__ go)
 {
+    if (go && *go == 0) return;
     constexpr int TW = kScanW + 2, TH = kScanH + 2;
     __shared__ int sl[TH * TW], sk[TH * TW];
     const int c0 = blockIdx.x * kScanW, r0 = blockIdx.y * kScanH;
@@ -503,7 +548,9 @@ __global__ void __launch_bounds__(kScanW * kScanH) k_bv_scan(const int* __restri
         if (per_ew) cc = (cc + nc) % nc;
         const bool ok = rr >= 0 && rr < nr && cc >= 0 && cc < nc;
         const int q = ok ? rr * nc + cc : 0;
-        sl[i] = ok ? lab[q] : -1;
+        int l = ok ? lab[q] : -1;
+        if (VIA_PARENT && l >= 0) l = par[l];
+        sl[i] = l;
         sk[i] = ok ? wk[q] : 0;
     }
     __syncthreads();
@@ -527,8 +574,12 @@ __global__ void __launch_bounds__(kScanW * kScanH) k_bv_scan(const int* __restri
             m = key < m ? key : m;
         }
     }
-    if (in) keep_out[k] = keep;
-    offer(A, m, best);
+    if (keep_out && in) keep_out[k] = keep;
+    if (VIA_PARENT) {
+        offer_block(A, m, best, tid);
+    } else {
+        offer(A, m, best);
+    }
 }
 
 // Later rounds: the cells of the boundary list offer their cheapest edges
@@ -581,7 +632,7 @@ __global__ void __launch_bounds__(kB) k_bv_scan_list(const int* __restrict__ wk,
             }
         }
         if (in) keep_out[i] = keep;
-        offer_block(A, m, best);
+        offer_block(A, m, best, threadIdx.x);
     }
 }
 
@@ -664,15 +715,37 @@ __global__ void k_bv_flatten(int* __restrict__ par, const int* __restrict__ list
 }
 
 // Hook chains as packed words, one per basin: hook weight key << 32 | hook
-// parent (-1 when the basin's component never hooked)
+// parent, or parent -1 when the basin's component never hooked (it is
+// still its own parent)
 __global__ void k_chain_pack(const int* __restrict__ roots, const int* __restrict__ count,
-                             const int* __restrict__ hp, const int* __restrict__ hw,
+                             const int* __restrict__ par, const int* __restrict__ hp, const int* __restrict__ hw,
                              unsigned long long* __restrict__ P)
 {
     const int cnt = *count;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const int b = roots[i];
-        P[b] = (static_cast<unsigned long long>(static_cast<unsigned>(hw[b])) << 32) | static_cast<unsigned>(hp[b]);
+        P[b] = par[b] == b ? 0xffffffffull
+                           : (static_cast<unsigned long long>(static_cast<unsigned>(hw[b])) << 32) |
+                                 static_cast<unsigned>(hp[b]);
+    }
+}
+
+// One step of the chain maxima below for every basin (pointer doubling when
+// all basins step together); the walk then finishes the few long chains
+__global__ void k_chain_step(const int* __restrict__ roots, const int* __restrict__ count,
+                             unsigned long long* __restrict__ P, int X)
+{
+    const int cnt = *count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const int b = roots[i];
+        const unsigned long long w = __hip_atomic_load(&P[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int u = static_cast<int>(static_cast<unsigned>(w));
+        if (u < 0 || u == X) continue;
+        const unsigned long long wu = __hip_atomic_load(&P[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int m = max(static_cast<int>(w >> 32), static_cast<int>(wu >> 32));
+        __hip_atomic_store(&P[b], (static_cast<unsigned long long>(static_cast<unsigned>(m)) << 32) |
+                                      (wu & 0xffffffffull),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -703,35 +776,38 @@ __global__ void k_chain_walk(const int* __restrict__ roots, const int* __restric
 }
 
 // filled = max(z, L(basin)) in a pit basin whose chain reached the marker,
-// z otherwise; nodata for invalid cells
-__global__ void k_fill_out(const float* __restrict__ z, const uint8_t* __restrict__ mask, const int* __restrict__ wk,
-                           const int* __restrict__ lab, const unsigned long long* __restrict__ P, long long n, int X,
-                           double nodata, int nan_nodata, float* __restrict__ out)
+// z otherwise (its key read back: the key map is a bijection on float bits);
+// nodata for invalid cells
+__global__ void k_fill_out(const int* __restrict__ wk, const int* __restrict__ lab,
+                           const unsigned long long* __restrict__ P, long long n, int X, float nodata_f,
+                           float* __restrict__ out)
 {
     const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const float zv = z[k];
-    const bool valid = mask[k] && !is_nodata(zv, nodata, nan_nodata);
-    if (!valid) {
-        out[k] = static_cast<float>(nodata);
+    const int key = wk[k];
+    if (key == kFixed) {
+        out[k] = nodata_f;
         return;
     }
     const int b = lab[k];
-    float v = zv;
+    int v = key;
     if (b != X) {
         const unsigned long long w = P[b];
-        if (static_cast<int>(static_cast<unsigned>(w)) == X) v = kfloat(max(wk[k], static_cast<int>(w >> 32)));
+        if (static_cast<int>(static_cast<unsigned>(w)) == X) v = max(key, static_cast<int>(w >> 32));
     }
-    out[k] = v;
+    out[k] = kfloat(v);
 }
 
-// the flags whose launches run unconditionally: the first descent jump and
-// the first round
-__global__ void k_fill_start_flags(int* __restrict__ flags)
+// the flags whose launches run unconditionally (the first descent jump and
+// the first round), and the marker's slot: its own component, no edge
+__global__ void k_fill_start_flags(int* __restrict__ flags, int X, int* __restrict__ par,
+                                   unsigned long long* __restrict__ best)
 {
     const int i = threadIdx.x;
     if (i == 0) flags[kFlagJump] = 1;
     if (i == 1) flags[kFlagRound] = 1;
+    if (i == 2) par[X] = X;
+    if (i == 3) best[X] = kNoEdge;
 }
 
 // descent jumps the last fill of this process needed, plus a spare: the
@@ -780,13 +856,12 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
     const int gs = std::min(g1, kStrideGrid);  // grid-stride launches
     const dim3 bsc(kScanW, kScanH), gsc(div_up(nc, kScanW), div_up(nr, kScanH));
     k_zero_ints<<<div_up(kFlagInts, kB), kB, 0, s>>>(flags, kFlagInts);
-    k_fill_interior_invalid<<<g2, b2, 0, s>>>(z, mask, nr, nc, nodata, nan_nodata, flags);
+    k_fill_init<<<g2, b2, 0, s>>>(z, mask, nr, nc, nodata, nan_nodata, flags, wk);
     LSDTT_LAUNCH_CHECK();
-    k_fill_init<<<g2, b2, 0, s>>>(z, mask, nr, nc, nodata, nan_nodata, per_ns, per_ew, flags, zk, wk);
+    k_fill_descent<<<dim3(div_up(nc, kDT), div_up(nr, kDT)), dim3(kDT, 8), 0, s>>>(wk, nr, nc, per_ns, per_ew, flags,
+                                                                                   zk, lab);
     LSDTT_LAUNCH_CHECK();
-    k_fill_descent<<<g2, b2, 0, s>>>(zk, wk, nr, nc, per_ns, per_ew, lab);
-    LSDTT_LAUNCH_CHECK();
-    k_fill_start_flags<<<1, 64, 0, s>>>(flags);
+    k_fill_start_flags<<<1, 64, 0, s>>>(flags, X, par, best);
     LSDTT_LAUNCH_CHECK();
     // descent jumps: a first batch as long as the last fill needed (plus a
     // spare), the rest only when its last jump still moved a pointer
@@ -816,17 +891,18 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
     }
     int* chunk_count = flags + kFlagChunks;
     int* chunk_off = chunk_count + kChunkBlocks;
-    k_bv_label<<<kChunkBlocks, kB, 0, s>>>(zk, wk, lab, n, chunk_count);
+    k_bv_label<<<kChunkBlocks, kB, 0, s>>>(zk, wk, lab, n, par, best, chunk_count);
     k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, counts + 2, nullptr);
     k_list_write<kRoots><<<kChunkBlocks, kB, 0, s>>>(lab, par, nullptr, nullptr, n, nullptr, chunk_off, roots, best,
                                                      nullptr);
-    k_basin_init<<<kScanGrid, kB, 0, s>>>(roots, counts + 2, X, par, hp, hw, best);
     LSDTT_LAUNCH_CHECK();
     // Boruvka rounds in batches with one read-back of their flags; a round
     // after the last hook returns at once.  Round r reads the basin list
-    // (r = 0) or component list (r + 1) & 1 and writes list r & 1; its scan
-    // reads boundary list (r + 1) & 1 (the grid when r = 0), flags the cells
-    // that stay and compacts them, in order, into list r & 1.  After a batch
+    // (r = 0) or component list (r + 1) & 1 and writes list r & 1.  The scans
+    // of rounds 0 and 1 cover the grid (most cells are still on a boundary);
+    // from round 1 on a scan flags the cells that stay on a boundary and
+    // compacts them, in order, into boundary list r & 1, which the next
+    // round scans instead of the grid.  After a batch
     // the list lengths size the next: each round at least halves the
     // components that can still hook, so at most log2(live) + 1 more rounds
     // hook, and the grids follow the lengths.
@@ -843,12 +919,14 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
             int* b_out = blist[r & 1];
             int* b_out_count = bcount + (r & 1);
             if (r == 0) {
-                k_bv_scan<<<gsc, bsc, 0, s>>>(wk, lab, nr, nc, per_ns, per_ew, X, best, keep);
+                k_bv_scan<false><<<gsc, bsc, 0, s>>>(wk, lab, par, nr, nc, per_ns, per_ew, X, best, nullptr, nullptr);
+            } else if (r == 1) {
+                k_bv_scan<true><<<gsc, bsc, 0, s>>>(wk, lab, par, nr, nc, per_ns, per_ew, X, best, keep, go);
                 k_list_count<kKeepCells><<<kChunkBlocks, kB, 0, s>>>(nullptr, nullptr, nullptr, keep, n, nullptr,
-                                                                     chunk_count, nullptr);
-                k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, b_out_count, nullptr);
+                                                                     chunk_count, go);
+                k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, b_out_count, go);
                 k_list_write<kKeepCells><<<kChunkBlocks, kB, 0, s>>>(nullptr, nullptr, nullptr, keep, n, nullptr,
-                                                                     chunk_off, b_out, nullptr, nullptr);
+                                                                     chunk_off, b_out, nullptr, go);
             } else {
                 const int* b_in = blist[(r + 1) & 1];
                 const int* b_in_count = bcount + ((r + 1) & 1);
@@ -892,10 +970,11 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
     // chain maxima over the basins, in the cheapest-edge words (free now)
     unsigned long long* P = best;
     const int chain_grid = grid_for(fh[kFlagCount + 2]);
-    k_chain_pack<<<chain_grid, kB, 0, s>>>(roots, counts + 2, hp, hw, P);
+    k_chain_pack<<<chain_grid, kB, 0, s>>>(roots, counts + 2, par, hp, hw, P);
+    for (int j = 0; j < 5; ++j) k_chain_step<<<chain_grid, kB, 0, s>>>(roots, counts + 2, P, X);
     k_chain_walk<<<chain_grid, kB, 0, s>>>(roots, counts + 2, P, X, flags + kFlagStuck);
     LSDTT_LAUNCH_CHECK();
-    k_fill_out<<<g1, kB, 0, s>>>(z, mask, wk, lab, P, n, X, nodata, nan_nodata, out);
+    k_fill_out<<<g1, kB, 0, s>>>(wk, lab, P, n, X, static_cast<float>(nodata), out);
     LSDTT_LAUNCH_CHECK();
     int stuck = 0;
     LSDTT_HIP(hipMemcpyAsync(&stuck, flags + kFlagStuck, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -16,6 +16,7 @@
 #include <rocprim/block/block_reduce.hpp>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include <algorithm>
 
@@ -267,42 +268,42 @@ void launch_receivers(const T* z, const uint8_t* msk, const uint8_t* forced, int
 }
 
 // donor count = number of distinct valid neighbours whose receiver is f;
-// dmask bit i set when the i-th of distinct_neighbours(f) is one
-__global__ void k_donors(const int* __restrict__ rcv, const uint8_t* __restrict__ mask, int nr, int nc,
-                         BC bc, uint8_t* __restrict__ donors, uint8_t* __restrict__ dmask,
-                         int* __restrict__ outlet_flag)
+// dmask bit i set when the i-th of distinct_neighbours(f) is one.  Blocks of
+// 32 x 8 cells stage their receivers with a one-cell halo in LDS; interior
+// cells (neighbours in direction order, none wrapped) read them there,
+// raster-border cells take the general path.  Masked cells hold the no-data
+// code, never a cell index.
+constexpr int kDonW = 32, kDonH = 8;
+__global__ void __launch_bounds__(kDonW * kDonH) k_donors(const int* __restrict__ rcv,
+                                                          const uint8_t* __restrict__ mask, int nr, int nc, BC bc,
+                                                          uint8_t* __restrict__ donors, uint8_t* __restrict__ dmask)
 {
-    const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (f >= (long long)nr * nc) return;
-    if (!mask[f]) {
-        donors[f] = 0;
-        dmask[f] = 0;
-        outlet_flag[f] = 0;
-        return;
+    constexpr int TW = kDonW + 2, TH = kDonH + 2;
+    __shared__ int sr[TH * TW];
+    const int c0 = blockIdx.x * kDonW, r0 = blockIdx.y * kDonH;
+    const int tid = threadIdx.y * kDonW + threadIdx.x;
+    for (int i = tid; i < TH * TW; i += kDonW * kDonH) {
+        const int rr = r0 - 1 + i / TW, cc = c0 - 1 + i % TW;
+        sr[i] = (rr >= 0 && rr < nr && cc >= 0 && cc < nc) ? rcv[rr * nc + cc] : kNodataReceiver;
     }
-    int nb[8];
-    const int k = distinct_neighbours(static_cast<int>(f), nr, nc, bc, nb);
+    __syncthreads();
+    const int r = r0 + threadIdx.y, c = c0 + threadIdx.x;
+    if (r >= nr || c >= nc) return;
+    const int f = r * nc + c;
     unsigned bits = 0;
-    if (k == 8) {
-        int rv[8];
+    if (mask[f]) {
+        if (r > 0 && r < nr - 1 && c > 0 && c < nc - 1) {
+            const int li = (threadIdx.y + 1) * TW + threadIdx.x + 1;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) rv[i] = rcv[nb[i]];  // masked cells hold -1, never == f
-#pragma unroll
-        for (int i = 0; i < 8; ++i) bits |= (rv[i] == f) ? (1u << i) : 0u;
-    } else {
-        for (int i = 0; i < k; ++i) bits |= (mask[nb[i]] && rcv[nb[i]] == f) ? (1u << i) : 0u;
+            for (int d = 0; d < 8; ++d) bits |= (sr[li + d8_row[d] * TW + d8_col[d]] == f) ? (1u << d) : 0u;
+        } else {
+            int nb[8];
+            const int k = distinct_neighbours(f, nr, nc, bc, nb);
+            for (int i = 0; i < k; ++i) bits |= (mask[nb[i]] && rcv[nb[i]] == f) ? (1u << i) : 0u;
+        }
     }
     donors[f] = static_cast<uint8_t>(__popc(bits));
     dmask[f] = static_cast<uint8_t>(bits);
-    outlet_flag[f] = (rcv[f] == kOutletReceiver) ? 1 : 0;
-}
-
-__global__ void k_scatter_flagged(const int* __restrict__ flag, const int* __restrict__ pos, long long n,
-                                  int* __restrict__ out)
-{
-    const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (f >= n) return;
-    if (flag[f]) out[pos[f]] = static_cast<int>(f);
 }
 
 // ---------------------------------------------------------------------------
@@ -650,27 +651,34 @@ __global__ void k_topo_keys(const int* __restrict__ cells, int m, const unsigned
     }
 }
 
-// Stable split of the valid cells into sources (no donors: Kahn key (0, own
-// index)) and the others, each in flat order, in three launches: per-chunk
-// counts, one scan of the chunk counts, the writes.  Each thread takes four
-// consecutive cells per step, their loads issued together.
+// Stable three-way split of the cells by a class (0, 1, or 2 = dropped),
+// classes 0 and 1 each written in flat order, in three launches: per-chunk
+// counts, one scan of the chunk counts (which also writes both totals), the
+// writes.  Each thread takes four consecutive cells per step, their loads
+// issued together.  Used for the topological order (0: sources -- no
+// donors, Kahn key (0, own index) -- 1: the other valid cells) and for the
+// outlets (0: cells whose receiver is the outlet code).
 constexpr int kPartChunks = 1024;
 __device__ __forceinline__ long long part_chunk_len(long long n)
 {
     const long long step = 4ll * kBlock;
     return ((n + kPartChunks - 1) / kPartChunks + step - 1) / step * step;
 }
-// class of cell f: 0 source, 1 other valid cell, 2 invalid
-__device__ __forceinline__ int part_class(const uint8_t* __restrict__ mask, const uint8_t* __restrict__ donors,
-                                          long long f, long long n)
-{
-    if (f >= n) return 2;
-    const uint8_t mv = mask[f], dv = donors[f];
-    return !mv ? 2 : (dv == 0 ? 0 : 1);
-}
-__global__ void __launch_bounds__(kBlock) k_part_count(const uint8_t* __restrict__ mask,
-                                                       const uint8_t* __restrict__ donors, long long n,
-                                                       int* __restrict__ counts)
+struct SourceClass {
+    const uint8_t* mask;
+    const uint8_t* donors;
+    __device__ int operator()(long long f) const
+    {
+        const uint8_t mv = mask[f], dv = donors[f];
+        return !mv ? 2 : (dv == 0 ? 0 : 1);
+    }
+};
+struct OutletClass {
+    const int* rcv;  // masked cells hold the no-data code, never the outlet code
+    __device__ int operator()(long long f) const { return rcv[f] == kOutletReceiver ? 0 : 2; }
+};
+template <class C>
+__global__ void __launch_bounds__(kBlock) k_part_count(C cls, long long n, int* __restrict__ counts)
 {
     const long long len = part_chunk_len(n);
     const long long b0 = blockIdx.x * len, b1 = std::min(n, b0 + len);
@@ -678,7 +686,7 @@ __global__ void __launch_bounds__(kBlock) k_part_count(const uint8_t* __restrict
     for (long long f = b0 + 4ll * threadIdx.x; f < b1; f += 4ll * blockDim.x) {
         int k[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) k[j] = part_class(mask, donors, f + j, b1);
+        for (int j = 0; j < 4; ++j) k[j] = f + j < b1 ? cls(f + j) : 2;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             c0 += k[j] == 0;
@@ -695,8 +703,9 @@ __global__ void __launch_bounds__(kBlock) k_part_count(const uint8_t* __restrict
         counts[kPartChunks + blockIdx.x] = c1;
     }
 }
-// exclusive scans of both count rows (one block of kPartChunks threads)
-__global__ void __launch_bounds__(kPartChunks) k_part_scan(int* __restrict__ counts)
+// exclusive scans of both count rows (one block of kPartChunks threads);
+// totals[0..1] = the two class sizes
+__global__ void __launch_bounds__(kPartChunks) k_part_scan(int* __restrict__ counts, int* __restrict__ totals)
 {
     __shared__ int sc[2][kPartChunks];
     const int t = threadIdx.x;
@@ -713,6 +722,10 @@ __global__ void __launch_bounds__(kPartChunks) k_part_scan(int* __restrict__ cou
     }
     counts[t] = sc[0][t] - a0;
     counts[kPartChunks + t] = sc[1][t] - a1;
+    if (t == kPartChunks - 1) {
+        totals[0] = sc[0][t];
+        totals[1] = sc[1][t];
+    }
 }
 // exclusive prefix over the block of per-thread values (in thread order);
 // returns the prefix and sets `total`
@@ -736,10 +749,9 @@ __device__ __forceinline__ int block_prefix(int v, int& total)
     __syncthreads();
     return before + x - v;
 }
-__global__ void __launch_bounds__(kBlock) k_part_write(const uint8_t* __restrict__ mask,
-                                                       const uint8_t* __restrict__ donors, long long n,
-                                                       const int* __restrict__ offs, int* __restrict__ out0,
-                                                       int* __restrict__ out1)
+template <class C>
+__global__ void __launch_bounds__(kBlock) k_part_write(C cls, long long n, const int* __restrict__ offs,
+                                                       int* __restrict__ out0, int* __restrict__ out1)
 {
     const long long len = part_chunk_len(n);
     const long long b0 = blockIdx.x * len, b1 = std::min(n, b0 + len);
@@ -748,7 +760,7 @@ __global__ void __launch_bounds__(kBlock) k_part_write(const uint8_t* __restrict
         const long long f = f0 + 4ll * threadIdx.x;
         int k[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) k[j] = part_class(mask, donors, f + j, b1);
+        for (int j = 0; j < 4; ++j) k[j] = f + j < b1 ? cls(f + j) : 2;
         int n0 = 0, n1 = 0;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -767,6 +779,7 @@ __global__ void __launch_bounds__(kBlock) k_part_write(const uint8_t* __restrict
         base1 += t1;
     }
 }
+
 
 // s-vector sibling offset: 1 + sizes of the receiver's donors with a smaller
 // flat index (donors are visited in ascending flat order); outlets take the
@@ -788,12 +801,11 @@ __global__ void k_sv_offsets(const int* __restrict__ rcv, const uint8_t* __restr
     off[f] = s;
 }
 
-__global__ void k_gather_outlet_sizes(const int* __restrict__ outlets, int no, const int* __restrict__ cnt,
-                                      int* __restrict__ out)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < no) out[i] = cnt[outlets[i]];
-}
+// an outlet's subtree size
+struct CountOf {
+    const int* cnt;
+    __device__ int operator()(int v) const { return cnt[v]; }
+};
 // s-vector positions, outlets first (their subtrees in row-major order),
 // then level by level away from them: a cell's subtree starts at its
 // receiver's position plus its sibling offset
@@ -857,7 +869,6 @@ FlowEngine::FlowEngine(const FlowOptions& o) : opt(o)
     donors.reserve(n);
     dmask.reserve(n);
     depth.reserve(n);
-    root.reserve(n);
     pj.reserve(n);
     flags.reserve(n);
     scan_out.reserve(n);
@@ -873,7 +884,6 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
     mask = msk;
     off_ready = false;
     const int nr = opt.nr, nc = opt.nc;
-    const int g1 = div_up(n, kBlock);
     // misc[0] = first unresolved flat, misc[1] = max depth, misc[2] = n_valid,
     // misc[3] = valid cells without donors
     k_init_ints<<<1, 64, 0, s>>>(misc.p, 4, 0x7fffffff, -1, 0, 0);
@@ -884,14 +894,17 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
                             opt.allow_pits_as_outlets, rcv, dir, misc.p, s);
     }
     {
-        TScope ts_(timer, "donors", s, 9.0 * n);
-        k_donors<<<g1, kBlock, 0, s>>>(rcv, msk, nr, nc, opt.bc, donors, dmask, flags);
+        TScope ts_(timer, "donors", s, 6.0 * n);
+        k_donors<<<dim3(div_up(nc, kDonW), div_up(nr, kDonH)), dim3(kDonW, kDonH), 0, s>>>(rcv, msk, nr, nc, opt.bc,
+                                                                                        donors, dmask);
     }
     LSDTT_LAUNCH_CHECK();
-    size_t tb = 0;
-    LSDTT_HIP(rocprim::exclusive_scan(nullptr, tb, flags.p, scan_out.p, 0, (size_t)n, rocprim::plus<int>(), s));
-    LSDTT_HIP(rocprim::exclusive_scan(tmp.need(tb), tb, flags.p, scan_out.p, 0, (size_t)n, rocprim::plus<int>(), s));
-    k_scatter_flagged<<<g1, kBlock, 0, s>>>(flags, scan_out, n, outlets);
+    // the outlets in row-major order
+    part_counts.reserve(2 * kPartChunks + 2);
+    const OutletClass ocls{rcv};
+    k_part_count<<<kPartChunks, kBlock, 0, s>>>(ocls, n, part_counts.p);
+    k_part_scan<<<1, kPartChunks, 0, s>>>(part_counts.p, part_counts.p + 2 * kPartChunks);
+    k_part_write<<<kPartChunks, kBlock, 0, s>>>(ocls, n, part_counts.p, outlets.p, nullptr);
     LSDTT_LAUNCH_CHECK();
 
     // one read-back with the pointer jumping's: misc and the outlet total
@@ -899,17 +912,16 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
     extra.src[0] = misc.p;
     extra.src[1] = misc.p + 1;
     extra.src[2] = misc.p + 2;
-    extra.src[3] = scan_out.p + (n - 1);
-    extra.src[4] = flags.p + (n - 1);
-    extra.src[5] = misc.p + 3;
-    extra.count = 6;
-    pointer_jump(nullptr, depth, root, s, extra);
+    extra.src[3] = part_counts.p + 2 * kPartChunks;
+    extra.src[4] = misc.p + 3;
+    extra.count = 5;
+    pointer_jump(nullptr, depth, nullptr, s, extra);
     const int* h = hread.p + kReadSlot;
-    n_outlets = h[3] + h[4];
+    n_outlets = h[3];
     if (h[0] != 0x7fffffff) return -(static_cast<long long>(h[0]) + 1);
     max_depth = h[1];
     n_valid = h[2];
-    n_sources = h[5];
+    n_sources = h[4];
 
     level_start_h.assign(static_cast<size_t>(std::max(max_depth, 0)) + 2, 0);
     if (n_valid == 0) {
@@ -929,6 +941,7 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
         k_level_hist<<<g1k, 1024, 0, s>>>(depth, n, level_count);
     }
     LSDTT_LAUNCH_CHECK();
+    size_t tb = 0;
     LSDTT_HIP(rocprim::exclusive_scan(nullptr, tb, level_count.p, level_start.p, 0, (size_t)(nl + 1),
                                       rocprim::plus<int>(), s));
     LSDTT_HIP(rocprim::exclusive_scan(tmp.need(tb), tb, level_count.p, level_start.p, 0, (size_t)(nl + 1),
@@ -1076,10 +1089,11 @@ void FlowEngine::topological_order(hipStream_t s)
     const int ns = n_sources, m = n_valid - n_sources;
     topo_inner.reserve(n_valid);
     int* inner = topo_inner.p;
-    part_counts.reserve(2 * kPartChunks);
-    k_part_count<<<kPartChunks, kBlock, 0, s>>>(mask, donors, n, part_counts);
-    k_part_scan<<<1, kPartChunks, 0, s>>>(part_counts);
-    k_part_write<<<kPartChunks, kBlock, 0, s>>>(mask, donors, n, part_counts, topo.p, inner);
+    part_counts.reserve(2 * kPartChunks + 2);
+    const SourceClass cls{mask, donors.p};
+    k_part_count<<<kPartChunks, kBlock, 0, s>>>(cls, n, part_counts.p);
+    k_part_scan<<<1, kPartChunks, 0, s>>>(part_counts.p, part_counts.p + 2 * kPartChunks);
+    k_part_write<<<kPartChunks, kBlock, 0, s>>>(cls, n, part_counts.p, topo.p, inner);
     LSDTT_LAUNCH_CHECK();
     if (m == 0) return;
     size_t tb = 0;
@@ -1116,12 +1130,11 @@ void FlowEngine::s_vector(hipStream_t s)
         LSDTT_LAUNCH_CHECK();
     }
     // outlet starts: exclusive scan of outlet sizes in row-major order
-    k_gather_outlet_sizes<<<div_up(n_outlets, kBlock), kBlock, 0, s>>>(outlets, n_outlets, cnt, flags);
-    LSDTT_LAUNCH_CHECK();
+    const auto sizes = rocprim::make_transform_iterator(outlets.p, CountOf{cnt.p});
     size_t tb = 0;
-    LSDTT_HIP(rocprim::exclusive_scan(nullptr, tb, flags.p, scan_out.p, 0, (size_t)n_outlets,
-                                      rocprim::plus<int>(), s));
-    LSDTT_HIP(rocprim::exclusive_scan(tmp.need(tb), tb, flags.p, scan_out.p, 0, (size_t)n_outlets,
+    LSDTT_HIP(rocprim::exclusive_scan(nullptr, tb, sizes, scan_out.p, 0, (size_t)n_outlets, rocprim::plus<int>(),
+                                      s));
+    LSDTT_HIP(rocprim::exclusive_scan(tmp.need(tb), tb, sizes, scan_out.p, 0, (size_t)n_outlets,
                                       rocprim::plus<int>(), s));
     k_sv_outlet_start<<<div_up(n_outlets, kBlock), kBlock, 0, s>>>(outlets, n_outlets, scan_out, start, svec);
     LSDTT_LAUNCH_CHECK();
