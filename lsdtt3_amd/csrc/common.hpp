@@ -105,6 +105,33 @@ __device__ __forceinline__ int distinct_neighbours(int flat, int nr, int nc, BC 
 
 inline int div_up(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
 
+// a / b, correctly rounded, from yb = 1 / b (itself correctly rounded): one
+// product and two FMAs instead of a division.  Markstein's correction: q0 =
+// a*yb is within an ulp of a/b and r = a - b*q0 is exact, so q0 + r*yb
+// rounds to the correctly rounded quotient (round to nearest, no underflow
+// or overflow on the way; |a| outside [2^-900, 2^900] takes the division).
+__device__ __forceinline__ double div_by_rcp(double a, double b, double yb)
+{
+    const double m = fabs(a);
+    if (!(m >= 0x1p-900 && m <= 0x1p+900)) return a / b;
+    const double q0 = a * yb;
+    const double r = __builtin_fma(-q0, b, a);
+    return __builtin_fma(r, yb, q0);
+}
+
+// XCD-aware tile numbering for a 1-D launch of `tiles` workgroups.  The
+// hardware deals consecutive workgroups to its 8 XCDs in turn, and each XCD
+// has its own L2; renumbered so, each XCD walks one contiguous band of tiles
+// in order and its L2 holds the halo rows and columns that neighbouring
+// tiles share (a bijection on [0, tiles)).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int tiles)
+{
+    const int per = tiles / kXcds, extra = tiles % kXcds;  // the first `extra` XCDs take one more
+    const int x = b % kXcds, i = b / kXcds;
+    return x < extra ? x * (per + 1) + i : extra * (per + 1) + (x - extra) * per + i;
+}
+
 struct hip_error : std::runtime_error {
     using std::runtime_error::runtime_error;
 };

@@ -145,13 +145,15 @@ __global__ void k_fill_init(const float* __restrict__ z, const uint8_t* __restri
 constexpr int kDT = 32;
 __global__ void __launch_bounds__(kDT * 8) k_fill_descent(const int* __restrict__ wk, int nr, int nc, int per_ns,
                                                           int per_ew, const int* __restrict__ interior_invalid,
-                                                          int* __restrict__ zk, int* __restrict__ ptr)
+                                                          int ntx, int tiles, int* __restrict__ zk,
+                                                          int* __restrict__ ptr)
 {
     constexpr int TW = kDT + 2;
     __shared__ int sk[TW * TW];
     __shared__ int sq[TW * TW];  // flat index of the staged cell, -1 outside the grid
     __shared__ int lp[kDT * kDT];
-    const int c0 = blockIdx.x * kDT, r0 = blockIdx.y * kDT;
+    const int tile = xcd_tile(blockIdx.x, tiles);
+    const int c0 = (tile % ntx) * kDT, r0 = (tile / ntx) * kDT;
     const int tid = threadIdx.y * kDT + threadIdx.x;
     for (int i = tid; i < TW * TW; i += kDT * 8) {
         int rr = r0 - 1 + i / TW, cc = c0 - 1 + i % TW;
@@ -467,26 +469,13 @@ __device__ __forceinline__ unsigned long long edge_key(int k, int q, int d, int 
     return (static_cast<unsigned long long>(uw) << 32) | eid;
 }
 
-// Offers of one wavefront's cells, in list (or row) order: runs of one
-// component fold their minima towards the run's first lane, which alone
-// issues the atomic
-__device__ __forceinline__ void offer(int A, unsigned long long m, unsigned long long* __restrict__ best)
-{
-    const int lane = threadIdx.x & 63;
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned long long om = __shfl_down(m, o);
-        const int oa = __shfl_down(A, o);
-        if (lane + o < 64 && oa == A && om < m) m = om;
-    }
-    const int left = __shfl_up(A, 1);
-    const bool head = lane == 0 || left != A;
-    if (head && A >= 0 && m != kNoEdge && m < best[A]) atomicMin(&best[A], m);
-}
-
-// Offers of a block's cells (every thread calls it): after the wavefront
-// fold, the run heads meet in an LDS table keyed by component, so the block
-// issues one global atomic per component it saw (large components late in
-// the rounds would otherwise take one per run from every wavefront)
+// Offers of a block's cells (every thread calls it with its index in the
+// block; wavefronts are runs of 64 consecutive indices).  Within a
+// wavefront, runs of one component fold their minima towards the run's
+// first lane; the run heads then meet in an LDS table keyed by component,
+// so the block issues one global atomic per component it saw (large
+// components late in the rounds would otherwise take one per run from
+// every wavefront)
 __device__ __forceinline__ void offer_block(int A, unsigned long long m, unsigned long long* __restrict__ best,
                                             int tid)
 {
@@ -534,13 +523,14 @@ template <bool VIA_PARENT>
 __global__ void __launch_bounds__(kScanW * kScanH) k_bv_scan(const int* __restrict__ wk, const int* __restrict__ lab,
                                                              const int* __restrict__ par, int nr, int nc, int per_ns,
                                                              int per_ew, int X, unsigned long long* __restrict__ best,
-                                                             uint8_t* __restrict__ keep_out,
+                                                             uint8_t* __restrict__ keep_out, int ntx, int tiles,
                                                              const int* __restrict__ go)
 {
     if (go && *go == 0) return;
     constexpr int TW = kScanW + 2, TH = kScanH + 2;
     __shared__ int sl[TH * TW], sk[TH * TW];
-    const int c0 = blockIdx.x * kScanW, r0 = blockIdx.y * kScanH;
+    const int tile = xcd_tile(blockIdx.x, tiles);
+    const int c0 = (tile % ntx) * kScanW, r0 = (tile / ntx) * kScanH;
     const int tid = threadIdx.y * kScanW + threadIdx.x;
     for (int i = tid; i < TH * TW; i += kScanW * kScanH) {
         int rr = r0 - 1 + i / TW, cc = c0 - 1 + i % TW;
@@ -575,11 +565,7 @@ __global__ void __launch_bounds__(kScanW * kScanH) k_bv_scan(const int* __restri
         }
     }
     if (keep_out && in) keep_out[k] = keep;
-    if (VIA_PARENT) {
-        offer_block(A, m, best, tid);
-    } else {
-        offer(A, m, best);
-    }
+    offer_block(A, m, best, tid);
 }
 
 // Later rounds: the cells of the boundary list offer their cheapest edges
@@ -854,12 +840,15 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
     dim3 b2(64, 4), g2(div_up(nc, 64), div_up(nr, 4));
     const int g1 = div_up(n, kB);
     const int gs = std::min(g1, kStrideGrid);  // grid-stride launches
-    const dim3 bsc(kScanW, kScanH), gsc(div_up(nc, kScanW), div_up(nr, kScanH));
+    const dim3 bsc(kScanW, kScanH);
+    const int scan_ntx = div_up(nc, kScanW), scan_tiles = scan_ntx * div_up(nr, kScanH);
     k_zero_ints<<<div_up(kFlagInts, kB), kB, 0, s>>>(flags, kFlagInts);
     k_fill_init<<<g2, b2, 0, s>>>(z, mask, nr, nc, nodata, nan_nodata, flags, wk);
     LSDTT_LAUNCH_CHECK();
-    k_fill_descent<<<dim3(div_up(nc, kDT), div_up(nr, kDT)), dim3(kDT, 8), 0, s>>>(wk, nr, nc, per_ns, per_ew, flags,
-                                                                                   zk, lab);
+    {
+        const int ntx = div_up(nc, kDT), tiles = ntx * div_up(nr, kDT);
+        k_fill_descent<<<tiles, dim3(kDT, 8), 0, s>>>(wk, nr, nc, per_ns, per_ew, flags, ntx, tiles, zk, lab);
+    }
     LSDTT_LAUNCH_CHECK();
     k_fill_start_flags<<<1, 64, 0, s>>>(flags, X, par, best);
     LSDTT_LAUNCH_CHECK();
@@ -919,9 +908,11 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
             int* b_out = blist[r & 1];
             int* b_out_count = bcount + (r & 1);
             if (r == 0) {
-                k_bv_scan<false><<<gsc, bsc, 0, s>>>(wk, lab, par, nr, nc, per_ns, per_ew, X, best, nullptr, nullptr);
+                k_bv_scan<false><<<scan_tiles, bsc, 0, s>>>(wk, lab, par, nr, nc, per_ns, per_ew, X, best, nullptr,
+                                                             scan_ntx, scan_tiles, nullptr);
             } else if (r == 1) {
-                k_bv_scan<true><<<gsc, bsc, 0, s>>>(wk, lab, par, nr, nc, per_ns, per_ew, X, best, keep, go);
+                k_bv_scan<true><<<scan_tiles, bsc, 0, s>>>(wk, lab, par, nr, nc, per_ns, per_ew, X, best, keep, scan_ntx,
+                                                            scan_tiles, go);
                 k_list_count<kKeepCells><<<kChunkBlocks, kB, 0, s>>>(nullptr, nullptr, nullptr, keep, n, nullptr,
                                                                      chunk_count, go);
                 k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, b_out_count, go);

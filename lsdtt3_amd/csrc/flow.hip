@@ -90,7 +90,8 @@ __device__ __forceinline__ void write_receiver(int f, int r, int d, int* __restr
 // N, NE, E, SE, S, SW, W, NW), the first of equal slopes winning
 template <class T>
 __device__ __forceinline__ void decide_interior(const uint8_t* __restrict__ forced, int nc, int row, int col,
-                                                double card, double diag, int nodata_outlet, int pits_outlet,
+                                                double card, double diag, double rcard, double rdiag,
+                                                int nodata_outlet, int pits_outlet,
                                                 uint8_t mc, T zcell, const uint8_t (&mk)[8], const T (&zn)[8],
                                                 int* __restrict__ rcv, uint8_t* __restrict__ dir,
                                                 int* __restrict__ fail, const TileMark& mark, int old)
@@ -118,7 +119,7 @@ __device__ __forceinline__ void decide_interior(const uint8_t* __restrict__ forc
         }
         const double dz = zc - static_cast<double>(zn[d]);
         if (dz <= 0.0) continue;
-        const double sl = dz / ((d & 1) ? diag : card);
+        const double sl = (d & 1) ? div_by_rcp(dz, diag, rdiag) : div_by_rcp(dz, card, rcard);
         if (sl > best) {
             best = sl;
             bdir = d;
@@ -135,12 +136,15 @@ __device__ __forceinline__ void decide_interior(const uint8_t* __restrict__ forc
 template <class T>
 __global__ void __launch_bounds__(256) k_receivers(const T* __restrict__ z, const uint8_t* __restrict__ mask,
                                                    const uint8_t* __restrict__ forced, int nr, int nc,
-                                                   double card, double diag, int nodata_outlet, int pits_outlet,
+                                                   double card, double diag, double rcard, double rdiag,
+                                                   int nodata_outlet, int pits_outlet,
                                                    int* __restrict__ rcv, uint8_t* __restrict__ dir,
-                                                   int* __restrict__ fail, T* __restrict__ zcopy, TileMark mark)
+                                                   int* __restrict__ fail, T* __restrict__ zcopy, TileMark mark,
+                                                   int ntx, int tiles)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r0 = (blockIdx.y * blockDim.y + threadIdx.y) * kRecvRows;
+    const int tile = xcd_tile(blockIdx.x, tiles);
+    const int col = (tile % ntx) * blockDim.x + threadIdx.x;
+    const int r0 = ((tile / ntx) * blockDim.y + threadIdx.y) * kRecvRows;
     if (col >= nc || r0 >= nr) return;
     // the window: rows r0 - 1 .. r0 + kRecvRows, clamped into the grid (the
     // clamped values belong to ring cells, which this kernel skips)
@@ -171,8 +175,8 @@ __global__ void __launch_bounds__(256) k_receivers(const T* __restrict__ z, cons
                                mw[k + 1][1], mw[k + 1][0], mw[k][0], mw[k - 1][0]};
         const T zn[8] = {zw[k - 1][1], zw[k - 1][2], zw[k][2], zw[k + 1][2],
                          zw[k + 1][1], zw[k + 1][0], zw[k][0], zw[k - 1][0]};
-        decide_interior<T>(forced, nc, row, col, card, diag, nodata_outlet, pits_outlet, mw[k][1], zw[k][1], mk, zn,
-                           rcv, dir, fail, mark, oldr[k]);
+        decide_interior<T>(forced, nc, row, col, card, diag, rcard, rdiag, nodata_outlet, pits_outlet, mw[k][1],
+                           zw[k][1], mk, zn, rcv, dir, fail, mark, oldr[k]);
     }
 }
 
@@ -181,7 +185,8 @@ __global__ void __launch_bounds__(256) k_receivers(const T* __restrict__ z, cons
 template <class T>
 __global__ void __launch_bounds__(256) k_receivers_ring(const T* __restrict__ z, const uint8_t* __restrict__ mask,
                                                         const uint8_t* __restrict__ forced, int nr, int nc,
-                                                        double card, double diag, uint32_t bcw, int nodata_outlet,
+                                                        double card, double diag, double rcard, double rdiag,
+                                                        uint32_t bcw, int nodata_outlet,
                                                         int pits_outlet, int* __restrict__ rcv,
                                                         uint8_t* __restrict__ dir, int* __restrict__ fail,
                                                         T* __restrict__ zcopy, TileMark mark)
@@ -236,7 +241,7 @@ __global__ void __launch_bounds__(256) k_receivers_ring(const T* __restrict__ z,
         if (g == f) continue;
         const double dz = zc - static_cast<double>(z[g]);
         if (dz <= 0.0) continue;
-        const double sl = dz / ((d & 1) ? diag : card);
+        const double sl = (d & 1) ? div_by_rcp(dz, diag, rdiag) : div_by_rcp(dz, card, rcard);
         if (sl > best) {
             best = sl;
             brcv = g;
@@ -257,13 +262,17 @@ void launch_receivers(const T* z, const uint8_t* msk, const uint8_t* forced, int
                       bool nodata_outlet, bool pits_outlet, int* rcv, uint8_t* dir, int* fail, hipStream_t s,
                       T* zcopy = nullptr, const TileMark& mark = TileMark{})
 {
-    const dim3 b2(64, 4), g2(div_up(nc, 64), div_up(nr, 4 * kRecvRows));
-    k_receivers<T><<<g2, b2, 0, s>>>(z, msk, forced, nr, nc, dx, dx * kSqrt2, nodata_outlet, pits_outlet, rcv, dir,
-                                     fail, zcopy, mark);
+    const dim3 b2(64, 4);
+    const int ntx = div_up(nc, 64), tiles = ntx * div_up(nr, 4 * kRecvRows);
+    // slopes divide by the two step lengths through their reciprocals
+    const double card = dx, diag = dx * kSqrt2;
+    k_receivers<T><<<tiles, b2, 0, s>>>(z, msk, forced, nr, nc, card, diag, 1.0 / card, 1.0 / diag, nodata_outlet,
+                                        pits_outlet, rcv, dir, fail, zcopy, mark, ntx, tiles);
     LSDTT_LAUNCH_CHECK();
     const int nring = nc + (nr > 1 ? nc : 0) + (nr > 2 ? (nc > 1 ? 2 : 1) * (nr - 2) : 0);
-    k_receivers_ring<T><<<div_up(nring, 256), 256, 0, s>>>(z, msk, forced, nr, nc, dx, dx * kSqrt2, bc_pack(bc),
-                                                          nodata_outlet, pits_outlet, rcv, dir, fail, zcopy, mark);
+    k_receivers_ring<T><<<div_up(nring, 256), 256, 0, s>>>(z, msk, forced, nr, nc, card, diag, 1.0 / card,
+                                                          1.0 / diag, bc_pack(bc), nodata_outlet, pits_outlet, rcv,
+                                                          dir, fail, zcopy, mark);
     LSDTT_LAUNCH_CHECK();
 }
 
@@ -276,11 +285,13 @@ void launch_receivers(const T* z, const uint8_t* msk, const uint8_t* forced, int
 constexpr int kDonW = 32, kDonH = 8;
 __global__ void __launch_bounds__(kDonW * kDonH) k_donors(const int* __restrict__ rcv,
                                                           const uint8_t* __restrict__ mask, int nr, int nc, BC bc,
-                                                          uint8_t* __restrict__ donors, uint8_t* __restrict__ dmask)
+                                                          int ntx, int tiles, uint8_t* __restrict__ donors,
+                                                          uint8_t* __restrict__ dmask)
 {
     constexpr int TW = kDonW + 2, TH = kDonH + 2;
     __shared__ int sr[TH * TW];
-    const int c0 = blockIdx.x * kDonW, r0 = blockIdx.y * kDonH;
+    const int t = xcd_tile(blockIdx.x, tiles);
+    const int c0 = (t % ntx) * kDonW, r0 = (t / ntx) * kDonH;
     const int tid = threadIdx.y * kDonW + threadIdx.x;
     for (int i = tid; i < TH * TW; i += kDonW * kDonH) {
         const int rr = r0 - 1 + i / TW, cc = c0 - 1 + i % TW;
@@ -895,8 +906,8 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
     }
     {
         TScope ts_(timer, "donors", s, 6.0 * n);
-        k_donors<<<dim3(div_up(nc, kDonW), div_up(nr, kDonH)), dim3(kDonW, kDonH), 0, s>>>(rcv, msk, nr, nc, opt.bc,
-                                                                                        donors, dmask);
+        const int ntx = div_up(nc, kDonW), tiles = ntx * div_up(nr, kDonH);
+        k_donors<<<tiles, dim3(kDonW, kDonH), 0, s>>>(rcv, msk, nr, nc, opt.bc, ntx, tiles, donors, dmask);
     }
     LSDTT_LAUNCH_CHECK();
     // the outlets in row-major order
