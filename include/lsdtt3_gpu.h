@@ -197,7 +197,7 @@ int lsdtt_gpu_fill_device(const float* d_z, const uint8_t* d_mask, int32_t n_row
  * cannot allocate device memory (code under HIP graph capture, serving
  * loops, harnesses that forbid hipMalloc).  Same arguments and results
  * (bit for bit); the working arrays live in a fixed device array of the
- * library (196 MiB per GPU, held until the process ends) instead of an
+ * library (164 MiB per GPU, held until the process ends) instead of an
  * allocated workspace, so grids are limited to
  * lsdtt_gpu_fill_noalloc_max_cells() = 4194304 cells (2048 x 2048, any
  * shape); larger grids are rejected with LSDTT_GPU_ERR_ARG.  Calls share

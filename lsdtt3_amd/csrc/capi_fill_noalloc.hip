@@ -28,10 +28,10 @@ using namespace lsdtt_gpu;
 namespace {
 
 // room for the fill's working arrays of every grid of up to 2^22 cells
-// (2048 x 2048, any shape: 12 ints per cell -- cheapest edges 2, keys 2,
-// labels, parents, hooks 2, two component lists, two boundary-cell lists --
-// a byte per cell, and 4096 for flags and compaction offsets): 196 MiB
-constexpr size_t kScratchInts = 12ull * (1ull << 22) + (1ull << 20) + 4096ull;
+// (2048 x 2048, any shape: 10 ints per cell -- cheapest edges 2, keys 2,
+// labels, parents, hooks 2, two boundary-cell lists -- a byte per cell, and
+// 4096 for flags and compaction offsets): 164 MiB
+constexpr size_t kScratchInts = 10ull * (1ull << 22) + (1ull << 20) + 4096ull;
 
 static __device__ __align__(256) int scratch[kScratchInts];
 

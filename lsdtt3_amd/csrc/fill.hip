@@ -74,14 +74,15 @@ constexpr int kStrideGrid = 2048;    // blocks of the grid-stride full-grid kern
 
 // flag slots (ints): [0] interior invalid, [1] a pointer walk hit its bound,
 // [2, 2+kJumpRounds] descent jumps, [64, 64+kMaxRounds] round go flags,
-// [600..602] lengths of the two component lists and of the basin list,
-// [603..604] lengths of the two boundary-cell lists, [640, 640+2
+// [128, 128+kMaxRounds) live components after each round, [600] number of
+// basins, [603..604] lengths of the two boundary-cell lists, [640, 640+2
 // kChunkBlocks) per-chunk counts and offsets of a compaction
-constexpr int kFlagStuck = 1, kFlagJump = 2, kFlagRound = 64, kFlagCount = 600, kFlagBoundary = 603;
-constexpr int kFlagChunks = 640;
+constexpr int kFlagStuck = 1, kFlagJump = 2, kFlagRound = 64, kFlagLive = 128, kFlagBasins = 600;
+constexpr int kFlagBoundary = 603, kFlagChunks = 640;
 constexpr int kFlagInts = kFlagChunks + 2 * kChunkBlocks;
 static_assert(kFlagJump + kJumpRounds + 1 <= kFlagRound, "flag slots");
-static_assert(kFlagRound + kMaxRounds + 1 <= kFlagCount, "flag slots");
+static_assert(kFlagRound + kMaxRounds + 1 <= kFlagLive, "flag slots");
+static_assert(kFlagLive + kMaxRounds <= kFlagBasins, "flag slots");
 static_assert(kFlagBoundary + 2 <= kFlagChunks, "flag slots");
 
 __device__ __forceinline__ int fkey(float f)
@@ -268,17 +269,12 @@ __global__ void k_zero_ints(int* __restrict__ p, int count)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
-// Labels, in place over the descent roots: lab[k] = X (= n, the marker) when
-// the cell's root is an exit, the root (its basin) otherwise, -1 when
-// invalid.  Every slot starts as its own component (par[k] = k) with no
-// cheapest edge (the marker's slot is set apart); hook parents and weights
-// are only read where a hook wrote them.  Also counts the pit basins
-// (lab[k] == k) of each contiguous chunk, the first launch of their list's
-// compaction.
+// Labels, in place over the descent roots: the root (a pit basin's cell),
+// X (= n, the marker) when the root is an exit, -1 when invalid.  Also
+// counts the pit basins (lab[k] == k) of each contiguous chunk, the first
+// launch of their numbering.
 __global__ void __launch_bounds__(kB) k_bv_label(const int* __restrict__ zk, const int* __restrict__ wk,
-                                                 int* __restrict__ lab, long long n, int* __restrict__ par,
-                                                 unsigned long long* __restrict__ best,
-                                                 int* __restrict__ chunk_count)
+                                                 int* __restrict__ lab, long long n, int* __restrict__ chunk_count)
 {
     const long long len = (n + gridDim.x - 1) / gridDim.x;
     const long long b0 = blockIdx.x * len, b1 = std::min(n, b0 + len);
@@ -298,8 +294,6 @@ __global__ void __launch_bounds__(kB) k_bv_label(const int* __restrict__ zk, con
             int v = -1;
             if (w[j] != kFixed) v = zk[r[j]] == kFixed ? static_cast<int>(n) : r[j];
             lab[k] = v;
-            par[k] = static_cast<int>(k);
-            best[k] = kNoEdge;
             c += v == k;
         }
     }
@@ -310,87 +304,6 @@ __global__ void __launch_bounds__(kB) k_bv_label(const int* __restrict__ zk, con
     if (threadIdx.x == 0) {
         int t = 0;
         for (int wv = 0; wv < kB / 64; ++wv) t += red[wv];
-        chunk_count[blockIdx.x] = t;
-    }
-}
-
-// Every basin's parent made its component (in place, after the round's
-// flattening): components still in the rounds and the marker are their own
-// parents, every other slot's parent lies on the way to its component, and
-// a parent pointer only ever moves towards it, so walks that overlap stay
-// correct.  The scans and hook targets then read a basin's component as
-// par[basin].
-__global__ void k_bv_compress(int* __restrict__ par, const int* __restrict__ roots, const int* __restrict__ count,
-                              const int* __restrict__ go, int* __restrict__ stuck)
-{
-    if (*go == 0) return;
-    const int cnt = *count;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        const int b = roots[i];
-        const int a = ld_rel(&par[b]);
-        int p = ld_rel(&par[a]);
-        if (p == a) continue;
-        for (int s = 0;; ++s) {
-            const int q = ld_rel(&par[p]);
-            if (q == p) break;
-            if (s == kWalkSteps) {
-                *stuck = 1;
-                break;
-            }
-            p = q;
-        }
-        st_rel(&par[b], p);
-    }
-}
-
-// List compaction in three launches (a count per contiguous chunk, one scan,
-// the writes in order), so no counter is shared by the whole grid.
-// kRoots: over the cells, those that are pit basins' roots (lab[c] == c);
-// kSurvivors: over a component list, the entries that did not hook
-// (par[c] == c), whose cheapest edge is cleared for the next round;
-// kKeepCells / kKeepList: the cells / list entries whose keep byte is set.
-enum { kRoots = 0, kSurvivors = 1, kKeepCells = 2, kKeepList = 3 };
-
-template <int MODE>
-__device__ __forceinline__ int list_item(int i, const int* __restrict__ lab, const int* __restrict__ par,
-                                         const int* __restrict__ list, const uint8_t* __restrict__ keep)
-{
-    if (MODE == kRoots) return lab[i] == i ? i : -1;
-    if (MODE == kKeepCells) return keep[i] ? i : -1;
-    if (MODE == kKeepList) return keep[i] ? list[i] : -1;
-    const int c = list[i];
-    return par[c] == c ? c : -1;
-}
-
-template <int MODE>
-__global__ void __launch_bounds__(kB) k_list_count(const int* __restrict__ lab, const int* __restrict__ par,
-                                                   const int* __restrict__ list, const uint8_t* __restrict__ keep,
-                                                   long long m_fixed, const int* __restrict__ m_dev,
-                                                   int* __restrict__ chunk_count, const int* __restrict__ go)
-{
-    if (go && *go == 0) return;
-    const long long m = m_dev ? *m_dev : m_fixed;
-    const long long len = (m + gridDim.x - 1) / gridDim.x;
-    const long long b0 = blockIdx.x * len, b1 = std::min(m, b0 + len);
-    int c = 0;
-    // four items per thread per step, their loads issued together
-    for (long long i = b0 + threadIdx.x; i < b1; i += 4 * blockDim.x) {
-        int v[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long long ij = i + j * (long long)blockDim.x;
-            v[j] = ij < b1 ? list_item<MODE>((int)ij, lab, par, list, keep) : -1;
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) c += v[j] >= 0;
-    }
-    __shared__ int red[kB / 64];
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int w = 0; w < kB / 64; ++w) t += red[w];
         chunk_count[blockIdx.x] = t;
     }
 }
@@ -416,11 +329,60 @@ __global__ void __launch_bounds__(kChunkBlocks) k_list_scan(const int* __restric
     if (t == kChunkBlocks - 1) *total = sc[t];
 }
 
+// Compactions in three launches (a count per contiguous chunk, one scan,
+// the writes in order), so no counter is shared by the whole grid.
+// kRanks: over the cells, the pit basins' roots (lab[c] == c) numbered in
+// flat order (rank_of[root] = its number); kKeepCells / kKeepList: the
+// cells / list entries whose keep byte is set, into a list.
+enum { kRanks = 0, kKeepCells = 2, kKeepList = 3 };
+
 template <int MODE>
-__global__ void __launch_bounds__(kB) k_list_write(const int* __restrict__ lab, const int* __restrict__ par,
-                                                   const int* __restrict__ list, const uint8_t* __restrict__ keep,
+__device__ __forceinline__ int list_item(int i, const int* __restrict__ lab, const int* __restrict__ list,
+                                         const uint8_t* __restrict__ keep)
+{
+    if (MODE == kRanks) return lab[i] == i ? i : -1;
+    if (MODE == kKeepCells) return keep[i] ? i : -1;
+    return keep[i] ? list[i] : -1;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kB) k_list_count(const int* __restrict__ list, const uint8_t* __restrict__ keep,
                                                    long long m_fixed, const int* __restrict__ m_dev,
-                                                   const int* __restrict__ chunk_off, int* __restrict__ out,
+                                                   int* __restrict__ chunk_count, const int* __restrict__ go)
+{
+    if (go && *go == 0) return;
+    const long long m = m_dev ? *m_dev : m_fixed;
+    const long long len = (m + gridDim.x - 1) / gridDim.x;
+    const long long b0 = blockIdx.x * len, b1 = std::min(m, b0 + len);
+    int c = 0;
+    // four items per thread per step, their loads issued together
+    for (long long i = b0 + threadIdx.x; i < b1; i += 4 * blockDim.x) {
+        int v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long long ij = i + j * (long long)blockDim.x;
+            v[j] = ij < b1 ? (keep[ij] ? 1 : -1) : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c += v[j] >= 0;
+    }
+    __shared__ int red[kB / 64];
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kB / 64; ++w) t += red[w];
+        chunk_count[blockIdx.x] = t;
+    }
+}
+
+// kRanks also starts each basin as its own component with no cheapest edge
+template <int MODE>
+__global__ void __launch_bounds__(kB) k_list_write(const int* __restrict__ lab, const int* __restrict__ list,
+                                                   const uint8_t* __restrict__ keep, long long m_fixed,
+                                                   const int* __restrict__ m_dev, const int* __restrict__ chunk_off,
+                                                   int* __restrict__ out, int* __restrict__ par,
                                                    unsigned long long* __restrict__ best, const int* __restrict__ go)
 {
     if (go && *go == 0) return;
@@ -437,7 +399,7 @@ __global__ void __launch_bounds__(kB) k_list_write(const int* __restrict__ lab, 
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const long long i = i0 + j * (long long)blockDim.x + threadIdx.x;
-            v[j] = i < b1 ? list_item<MODE>((int)i, lab, par, list, keep) : -1;
+            v[j] = i < b1 ? list_item<MODE>((int)i, lab, list, keep) : -1;
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -450,13 +412,29 @@ __global__ void __launch_bounds__(kB) k_list_write(const int* __restrict__ lab, 
                 all += wsum[w];
             }
             if (v[j] >= 0) {
-                out[base + before + __popcll(bal & ((1ull << lane) - 1ull))] = v[j];
-                if (MODE == kSurvivors) best[v[j]] = kNoEdge;
+                const int pos = base + before + __popcll(bal & ((1ull << lane) - 1ull));
+                if (MODE == kRanks) {
+                    out[v[j]] = pos;
+                    par[pos] = pos;
+                    best[pos] = kNoEdge;
+                } else {
+                    out[pos] = v[j];
+                }
             }
             base += all;
             __syncthreads();
         }
     }
+}
+
+// labels become basin numbers: a component is named by the number of the
+// basin it started from, so every per-component array is dense
+__global__ void k_relabel(int* __restrict__ lab, long long n, int X, const int* __restrict__ rank_of)
+{
+    const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int l = lab[k];
+    if (l >= 0 && l != X) lab[k] = rank_of[l];
 }
 
 // the key of the edge from cell k (key kp) to its neighbour q in direction d
@@ -573,8 +551,8 @@ __global__ void __launch_bounds__(kScanW * kScanH) k_bv_scan(const int* __restri
 // the list.  A cell's component is its basin's parent; the loads of all
 // nine cells are issued together, one step at a time.
 __global__ void __launch_bounds__(kB) k_bv_scan_list(const int* __restrict__ wk, const int* __restrict__ lab,
-                                                     const int* __restrict__ par, int nr, int nc, int per_ns, int per_ew,
-                                                     int X, const int* __restrict__ list,
+                                                     const int* __restrict__ par, int nr, int nc, int per_ns,
+                                                     int per_ew, int X, const int* __restrict__ list,
                                                      const int* __restrict__ count, uint8_t* __restrict__ keep_out,
                                                      unsigned long long* __restrict__ best,
                                                      const int* __restrict__ go)
@@ -622,17 +600,17 @@ __global__ void __launch_bounds__(kB) k_bv_scan_list(const int* __restrict__ wk,
     }
 }
 
-// Each listed component with an outgoing edge records the component across
-// it (hook targets are read before any component of the round hooks)
+// Each component still in the rounds (its own parent) with an outgoing edge
+// records the component across it; targets are read before any component of
+// the round hooks
 __global__ void k_bv_target(const int* __restrict__ lab, const int* __restrict__ par,
                             const unsigned long long* __restrict__ best, int nr, int nc, int per_ns, int per_ew,
-                            const int* __restrict__ list, const int* __restrict__ count, int* __restrict__ hp,
-                            const int* __restrict__ go)
+                            const int* __restrict__ nb_dev, int* __restrict__ hp, const int* __restrict__ go)
 {
     if (*go == 0) return;
-    const int cnt = *count;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        const int c = list[i];
+    const int nb = *nb_dev;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nb; c += gridDim.x * blockDim.x) {
+        if (par[c] != c) continue;
         const unsigned long long b = best[c];
         if (b == kNoEdge) continue;
         const unsigned eid = static_cast<unsigned>(b);
@@ -643,27 +621,25 @@ __global__ void k_bv_target(const int* __restrict__ lab, const int* __restrict__
     }
 }
 
-// Each listed component with an outgoing edge hooks onto its target, except
-// the lower id of a pair that chose the same edge
-__global__ void k_bv_hook(const unsigned long long* __restrict__ best, int X, const int* __restrict__ list,
-                          const int* __restrict__ count, int* __restrict__ par, int* __restrict__ hp,
-                          int* __restrict__ hw, const int* __restrict__ go, int* __restrict__ hooked)
+// Each such component hooks onto its target, except the lower number of a
+// pair that chose the same edge (only a component's own thread reads or
+// writes its parent here)
+__global__ void k_bv_hook(const unsigned long long* __restrict__ best, int X, const int* __restrict__ nb_dev,
+                          int* __restrict__ par, const int* __restrict__ hp, int* __restrict__ hw,
+                          const int* __restrict__ go, int* __restrict__ hooked)
 {
     if (*go == 0) return;
-    const int cnt = *count;
+    const int nb = *nb_dev;
     const int stride = gridDim.x * blockDim.x;
-    for (int i0 = blockIdx.x * blockDim.x; i0 < cnt; i0 += stride) {
-        const int i = i0 + threadIdx.x;
+    for (int c0 = blockIdx.x * blockDim.x; c0 < nb; c0 += stride) {
+        const int c = c0 + threadIdx.x;
         bool h = false;
-        if (i < cnt) {
-            const int c = list[i];
+        if (c < nb && par[c] == c) {
             const unsigned long long b = best[c];
             if (b != kNoEdge) {
                 const int t = hp[c];
                 const bool stays = t != X && best[t] == b && c < t;
-                if (stays) {
-                    hp[c] = -1;
-                } else {
+                if (!stays) {
                     par[c] = t;
                     hw[c] = static_cast<int>(static_cast<unsigned>(b >> 32) ^ 0x80000000u);
                     h = true;
@@ -674,19 +650,29 @@ __global__ void k_bv_hook(const unsigned long long* __restrict__ best, int X, co
     }
 }
 
-// Flattening of this round's hook trees: each listed component walks to its
-// tree's root, storing every step (in place: a parent pointer only ever
-// moves towards its tree's root, and walks reuse the steps stored by the
-// nodes they pass)
-__global__ void k_bv_flatten(int* __restrict__ par, const int* __restrict__ list, const int* __restrict__ count,
-                             const int* __restrict__ go, int* __restrict__ stuck)
+// Every basin's parent made its component, in place: components still in
+// the rounds and the marker are their own parents, every other parent lies
+// on the way to its component and only ever moves towards it, so walks that
+// overlap stay correct (this also flattens the round's hook trees).  The
+// scans and hook targets then read a basin's component as par[basin].
+// Components still in the rounds clear their cheapest edge for the next
+// round and are counted.
+__global__ void __launch_bounds__(kB) k_bv_compress(int* __restrict__ par, unsigned long long* __restrict__ best,
+                                                    const int* __restrict__ nb_dev, const int* __restrict__ go,
+                                                    int* __restrict__ live, int* __restrict__ stuck)
 {
     if (*go == 0) return;
-    const int cnt = *count;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        const int c = list[i];
-        int p = ld_rel(&par[c]);
-        if (p == c) continue;
+    const int nb = *nb_dev;
+    int alive = 0;
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+        const int a = ld_rel(&par[b]);
+        if (a == b) {
+            best[b] = kNoEdge;
+            ++alive;
+            continue;
+        }
+        int p = ld_rel(&par[a]);
+        if (p == a) continue;
         for (int s = 0;; ++s) {
             const int q = ld_rel(&par[p]);
             if (q == p) break;
@@ -695,21 +681,29 @@ __global__ void k_bv_flatten(int* __restrict__ par, const int* __restrict__ list
                 break;
             }
             p = q;
-            st_rel(&par[c], p);
         }
+        st_rel(&par[b], p);
+    }
+    __shared__ int red[kB / 64];
+    for (int o = 32; o > 0; o >>= 1) alive += __shfl_xor(alive, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = alive;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kB / 64; ++w) t += red[w];
+        if (t) atomicAdd(live, t);
     }
 }
 
 // Hook chains as packed words, one per basin: hook weight key << 32 | hook
 // parent, or parent -1 when the basin's component never hooked (it is
 // still its own parent)
-__global__ void k_chain_pack(const int* __restrict__ roots, const int* __restrict__ count,
-                             const int* __restrict__ par, const int* __restrict__ hp, const int* __restrict__ hw,
+__global__ void k_chain_pack(const int* __restrict__ nb_dev, const int* __restrict__ par,
+                             const int* __restrict__ hp, const int* __restrict__ hw,
                              unsigned long long* __restrict__ P)
 {
-    const int cnt = *count;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        const int b = roots[i];
+    const int nb = *nb_dev;
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
         P[b] = par[b] == b ? 0xffffffffull
                            : (static_cast<unsigned long long>(static_cast<unsigned>(hw[b])) << 32) |
                                  static_cast<unsigned>(hp[b]);
@@ -718,12 +712,10 @@ __global__ void k_chain_pack(const int* __restrict__ roots, const int* __restric
 
 // One step of the chain maxima below for every basin (pointer doubling when
 // all basins step together); the walk then finishes the few long chains
-__global__ void k_chain_step(const int* __restrict__ roots, const int* __restrict__ count,
-                             unsigned long long* __restrict__ P, int X)
+__global__ void k_chain_step(const int* __restrict__ nb_dev, unsigned long long* __restrict__ P, int X)
 {
-    const int cnt = *count;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        const int b = roots[i];
+    const int nb = *nb_dev;
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
         const unsigned long long w = __hip_atomic_load(&P[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int u = static_cast<int>(static_cast<unsigned>(w));
         if (u < 0 || u == X) continue;
@@ -739,12 +731,11 @@ __global__ void k_chain_step(const int* __restrict__ roots, const int* __restric
 // from its parent u's word until u is the marker or hooks nowhere.  Every
 // stored word is (the largest hook weight from the basin up to u, u), so
 // words read while their owners update them still compose correctly.
-__global__ void k_chain_walk(const int* __restrict__ roots, const int* __restrict__ count,
-                             unsigned long long* __restrict__ P, int X, int* __restrict__ stuck)
+__global__ void k_chain_walk(const int* __restrict__ nb_dev, unsigned long long* __restrict__ P, int X,
+                             int* __restrict__ stuck)
 {
-    const int cnt = *count;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        const int b = roots[i];
+    const int nb = *nb_dev;
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
         unsigned long long w = __hip_atomic_load(&P[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         for (int s = 0;; ++s) {
             const int u = static_cast<int>(static_cast<unsigned>(w));
@@ -804,17 +795,18 @@ std::atomic<int> g_jump_hint{12};
 }  // namespace
 
 // ints of scratch: the cheapest edges (2 per slot, first, 8-byte aligned),
-// zk (then the basin list), wk, the labels, par, hp, hw, two component
-// lists, two boundary-cell lists, a keep byte per cell and the flags
+// zk (then each root's basin number), wk, the labels, par, hp, hw, two
+// boundary-cell lists, a keep byte per cell and the flags
 size_t fill_scratch_ints(int nr, int nc)
 {
     const long long n = (long long)nr * nc;
-    return static_cast<size_t>(2 * (n + 1) + 3 * n + 3 * (n + 1) + 4 * n + (n + 3) / 4 + kFlagInts + 8);
+    return static_cast<size_t>(2 * (n + 1) + 3 * n + 3 * (n + 1) + 2 * n + (n + 3) / 4 + kFlagInts + 8);
 }
 
 // Enqueues the fill on `s` and returns the number of Boruvka rounds run.
 // `scratch` holds fill_scratch_ints(nr, nc) ints, 8-byte aligned.  Blocks on
-// one read-back per batch of rounds and one at the end.
+// one read-back after the descent jumps, one per batch of rounds and one at
+// the end.
 int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double nodata, BC bc, float* out,
                 int* scratch, hipStream_t s)
 {
@@ -823,19 +815,20 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
     const long long n = (long long)nr * nc;
     if (n >= (1ll << 29)) throw std::invalid_argument("the fill supports grids of fewer than 2^29 cells");
     const int X = static_cast<int>(n);
+    // per-component arrays are indexed by basin number (dense, < number of
+    // basins) and by X for the marker
     unsigned long long* best = reinterpret_cast<unsigned long long*>(scratch);
     int* zk = scratch + 2 * (n + 1);
-    int* roots = zk;  // once the labels are made
+    int* rank_of = zk;  // once the labels are made
     int* wk = zk + n;
-    int* lab = wk + n;  // descent pointers, then labels
+    int* lab = wk + n;  // descent pointers, then labels, then basin numbers
     int* par = lab + n;
     int* hp = par + (n + 1);
     int* hw = hp + (n + 1);
-    int* lists[2] = {hw + (n + 1), hw + (n + 1) + n};
-    int* blist[2] = {lists[1] + n, lists[1] + 2 * n};
+    int* blist[2] = {hw + (n + 1), hw + (n + 1) + n};
     uint8_t* keep = reinterpret_cast<uint8_t*>(blist[1] + n);
     int* flags = blist[1] + n + (n + 3) / 4;
-    int* counts = flags + kFlagCount;    // [0], [1] component lists, [2] basins
+    int* nb_dev = flags + kFlagBasins;
     int* bcount = flags + kFlagBoundary;  // boundary lists
     dim3 b2(64, 4), g2(div_up(nc, 64), div_up(nr, 4));
     const int g1 = div_up(n, kB);
@@ -878,33 +871,36 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
         while (moved < kJumpRounds && fh[kFlagJump + moved + 1]) ++moved;
         g_jump_hint.store(moved + 2, std::memory_order_relaxed);
     }
+    // labels, then the pit basins numbered in flat order (each starting as
+    // its own component), then every label as its basin's number
     int* chunk_count = flags + kFlagChunks;
     int* chunk_off = chunk_count + kChunkBlocks;
-    k_bv_label<<<kChunkBlocks, kB, 0, s>>>(zk, wk, lab, n, par, best, chunk_count);
-    k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, counts + 2, nullptr);
-    k_list_write<kRoots><<<kChunkBlocks, kB, 0, s>>>(lab, par, nullptr, nullptr, n, nullptr, chunk_off, roots, best,
+    k_bv_label<<<kChunkBlocks, kB, 0, s>>>(zk, wk, lab, n, chunk_count);
+    k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, nb_dev, nullptr);
+    k_list_write<kRanks><<<kChunkBlocks, kB, 0, s>>>(lab, nullptr, nullptr, n, nullptr, chunk_off, rank_of, par, best,
                                                      nullptr);
+    k_relabel<<<g1, kB, 0, s>>>(lab, n, X, rank_of);
     LSDTT_LAUNCH_CHECK();
     // Boruvka rounds in batches with one read-back of their flags; a round
-    // after the last hook returns at once.  Round r reads the basin list
-    // (r = 0) or component list (r + 1) & 1 and writes list r & 1.  The scans
-    // of rounds 0 and 1 cover the grid (most cells are still on a boundary);
-    // from round 1 on a scan flags the cells that stay on a boundary and
-    // compacts them, in order, into boundary list r & 1, which the next
-    // round scans instead of the grid.  After a batch
-    // the list lengths size the next: each round at least halves the
-    // components that can still hook, so at most log2(live) + 1 more rounds
-    // hook, and the grids follow the lengths.
-    auto grid_for = [](long long items) { return static_cast<int>(std::clamp<long long>(div_up(items, kB), 1, kScanGrid)); };
-    int list_grid = kScanGrid, scan_grid = kScanGrid, basin_grid = kScanGrid;
+    // after the last hook returns at once.  Every round: the scan offers
+    // cheapest edges, then targets, hooks and the compression of the
+    // parents over all basins.  The scans of rounds 0 and 1 cover the grid
+    // (most cells are still on a boundary); from round 1 on a scan flags the
+    // cells that stay on a boundary and compacts them, in order, into
+    // boundary list r & 1, which the next round scans instead of the grid.
+    // After a batch the live components size the next: each round at least
+    // halves the components that can still hook, so at most log2(live) + 1
+    // more rounds hook, and the grids follow the lengths.
+    auto grid_for = [](long long items) {
+        return static_cast<int>(std::clamp<long long>(div_up(items, kB), 1, kScanGrid));
+    };
+    int basin_grid = kScanGrid, scan_grid = kScanGrid;
     int launched = 0, rounds = 0, batch = 4;
     for (;;) {
         const int end = std::min(kMaxRounds, launched + batch);
         for (int r = launched; r < end; ++r) {
             const int* go = flags + kFlagRound + r;
             int* hooked = flags + kFlagRound + r + 1;
-            const int* in_list = r == 0 ? roots : lists[(r + 1) & 1];
-            const int* in_count = r == 0 ? counts + 2 : counts + ((r + 1) & 1);
             int* b_out = blist[r & 1];
             int* b_out_count = bcount + (r & 1);
             if (r == 0) {
@@ -913,31 +909,23 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
             } else if (r == 1) {
                 k_bv_scan<true><<<scan_tiles, bsc, 0, s>>>(wk, lab, par, nr, nc, per_ns, per_ew, X, best, keep, scan_ntx,
                                                             scan_tiles, go);
-                k_list_count<kKeepCells><<<kChunkBlocks, kB, 0, s>>>(nullptr, nullptr, nullptr, keep, n, nullptr,
-                                                                     chunk_count, go);
+                k_list_count<kKeepCells><<<kChunkBlocks, kB, 0, s>>>(nullptr, keep, n, nullptr, chunk_count, go);
                 k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, b_out_count, go);
-                k_list_write<kKeepCells><<<kChunkBlocks, kB, 0, s>>>(nullptr, nullptr, nullptr, keep, n, nullptr,
-                                                                     chunk_off, b_out, nullptr, go);
+                k_list_write<kKeepCells><<<kChunkBlocks, kB, 0, s>>>(nullptr, nullptr, keep, n, nullptr, chunk_off,
+                                                                     b_out, nullptr, nullptr, go);
             } else {
                 const int* b_in = blist[(r + 1) & 1];
                 const int* b_in_count = bcount + ((r + 1) & 1);
                 k_bv_scan_list<<<scan_grid, kB, 0, s>>>(wk, lab, par, nr, nc, per_ns, per_ew, X, b_in, b_in_count,
                                                          keep, best, go);
-                k_list_count<kKeepList><<<kChunkBlocks, kB, 0, s>>>(nullptr, nullptr, b_in, keep, 0, b_in_count,
-                                                                    chunk_count, go);
+                k_list_count<kKeepList><<<kChunkBlocks, kB, 0, s>>>(b_in, keep, 0, b_in_count, chunk_count, go);
                 k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, b_out_count, go);
-                k_list_write<kKeepList><<<kChunkBlocks, kB, 0, s>>>(nullptr, nullptr, b_in, keep, 0, b_in_count,
-                                                                    chunk_off, b_out, nullptr, go);
+                k_list_write<kKeepList><<<kChunkBlocks, kB, 0, s>>>(nullptr, b_in, keep, 0, b_in_count, chunk_off,
+                                                                    b_out, nullptr, nullptr, go);
             }
-            k_bv_target<<<list_grid, kB, 0, s>>>(lab, par, best, nr, nc, per_ns, per_ew, in_list, in_count, hp, go);
-            k_bv_hook<<<list_grid, kB, 0, s>>>(best, X, in_list, in_count, par, hp, hw, go, hooked);
-            k_bv_flatten<<<list_grid, kB, 0, s>>>(par, in_list, in_count, go, flags + kFlagStuck);
-            k_bv_compress<<<basin_grid, kB, 0, s>>>(par, roots, counts + 2, go, flags + kFlagStuck);
-            k_list_count<kSurvivors><<<kChunkBlocks, kB, 0, s>>>(lab, par, in_list, nullptr, 0, in_count, chunk_count,
-                                                                  go);
-            k_list_scan<<<1, kChunkBlocks, 0, s>>>(chunk_count, chunk_off, counts + (r & 1), go);
-            k_list_write<kSurvivors><<<kChunkBlocks, kB, 0, s>>>(lab, par, in_list, nullptr, 0, in_count, chunk_off,
-                                                                  lists[r & 1], best, go);
+            k_bv_target<<<basin_grid, kB, 0, s>>>(lab, par, best, nr, nc, per_ns, per_ew, nb_dev, hp, go);
+            k_bv_hook<<<basin_grid, kB, 0, s>>>(best, X, nb_dev, par, hp, hw, go, hooked);
+            k_bv_compress<<<basin_grid, kB, 0, s>>>(par, best, nb_dev, go, flags + kFlagLive + r, flags + kFlagStuck);
             LSDTT_LAUNCH_CHECK();
         }
         read_flags();
@@ -950,20 +938,19 @@ int fill_device(const float* z, const uint8_t* mask, int nr, int nc, double noda
             break;
         }
         if (launched >= kMaxRounds) throw std::runtime_error("fill: Boruvka rounds did not finish");
-        const int live = fh[kFlagCount + ((end - 1) & 1)];
+        const int live = fh[kFlagLive + end - 1];
         int lg = 0;
         while ((1 << (lg + 1)) <= live) ++lg;
         batch = std::min(4, lg + 2);
-        list_grid = grid_for(live);
+        basin_grid = grid_for(fh[kFlagBasins]);
         scan_grid = grid_for(fh[kFlagBoundary + ((end - 1) & 1)]);
-        basin_grid = grid_for(fh[kFlagCount + 2]);
     }
     // chain maxima over the basins, in the cheapest-edge words (free now)
     unsigned long long* P = best;
-    const int chain_grid = grid_for(fh[kFlagCount + 2]);
-    k_chain_pack<<<chain_grid, kB, 0, s>>>(roots, counts + 2, par, hp, hw, P);
-    for (int j = 0; j < 5; ++j) k_chain_step<<<chain_grid, kB, 0, s>>>(roots, counts + 2, P, X);
-    k_chain_walk<<<chain_grid, kB, 0, s>>>(roots, counts + 2, P, X, flags + kFlagStuck);
+    const int chain_grid = grid_for(fh[kFlagBasins]);
+    k_chain_pack<<<chain_grid, kB, 0, s>>>(nb_dev, par, hp, hw, P);
+    for (int j = 0; j < 5; ++j) k_chain_step<<<chain_grid, kB, 0, s>>>(nb_dev, P, X);
+    k_chain_walk<<<chain_grid, kB, 0, s>>>(nb_dev, P, X, flags + kFlagStuck);
     LSDTT_LAUNCH_CHECK();
     k_fill_out<<<g1, kB, 0, s>>>(wk, lab, P, n, X, static_cast<float>(nodata), out);
     LSDTT_LAUNCH_CHECK();
