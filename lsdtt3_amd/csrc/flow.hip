@@ -140,9 +140,10 @@ __global__ void __launch_bounds__(256) k_receivers(const T* __restrict__ z, cons
                                                    int nodata_outlet, int pits_outlet,
                                                    int* __restrict__ rcv, uint8_t* __restrict__ dir,
                                                    int* __restrict__ fail, T* __restrict__ zcopy, TileMark mark,
-                                                   int ntx, int tiles)
+                                                   int ntx)
 {
-    const int tile = xcd_tile(blockIdx.x, tiles);
+    // row-major tile order (an XCD-banded order measured slower here)
+    const int tile = blockIdx.x;
     const int col = (tile % ntx) * blockDim.x + threadIdx.x;
     const int r0 = ((tile / ntx) * blockDim.y + threadIdx.y) * kRecvRows;
     if (col >= nc || r0 >= nr) return;
@@ -267,7 +268,7 @@ void launch_receivers(const T* z, const uint8_t* msk, const uint8_t* forced, int
     // slopes divide by the two step lengths through their reciprocals
     const double card = dx, diag = dx * kSqrt2;
     k_receivers<T><<<tiles, b2, 0, s>>>(z, msk, forced, nr, nc, card, diag, 1.0 / card, 1.0 / diag, nodata_outlet,
-                                        pits_outlet, rcv, dir, fail, zcopy, mark, ntx, tiles);
+                                        pits_outlet, rcv, dir, fail, zcopy, mark, ntx);
     LSDTT_LAUNCH_CHECK();
     const int nring = nc + (nr > 1 ? nc : 0) + (nr > 2 ? (nc > 1 ? 2 : 1) * (nr - 2) : 0);
     k_receivers_ring<T><<<div_up(nring, 256), 256, 0, s>>>(z, msk, forced, nr, nc, card, diag, 1.0 / card,

@@ -134,8 +134,12 @@ struct TileAt {
 
 __device__ __forceinline__ TileAt grid_tile()
 {
-    return TileAt{static_cast<int>(blockIdx.y * gridDim.x + blockIdx.x), 0, static_cast<int>(blockIdx.x),
-                  static_cast<int>(blockIdx.y), static_cast<int>(gridDim.x)};
+    // XCD by XCD in contiguous bands of tiles (the hardware deals
+    // consecutive workgroups to the XCDs in turn): the halos and cross-tile
+    // neighbours a band reads stay in its XCD's L2
+    const int ntx = static_cast<int>(gridDim.x);
+    const int t = xcd_tile(static_cast<int>(blockIdx.y * gridDim.x + blockIdx.x), ntx * static_cast<int>(gridDim.y));
+    return TileAt{t, 0, t % ntx, t / ntx, ntx};
 }
 
 __device__ __forceinline__ TileAt batch_tile(const BatchTiles& B, BatchGrid& g)

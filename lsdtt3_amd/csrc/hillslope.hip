@@ -521,7 +521,11 @@ __global__ void __launch_bounds__(256) k_bi_pv(BiArgs a)
     float* __restrict__ pout = a.p;
     const PhatView view = phat_view(a);
     double acc[1] = {0.0};
-    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    // tiles in rounds of gridDim.x, each round's tiles dealt XCD by XCD in
+    // contiguous bands (shared halos stay in one L2)
+    for (int base = 0; base < nt; base += gridDim.x) {
+        const int tile = base + xcd_tile(blockIdx.x, gridDim.x);
+        if (tile >= nt) continue;
         const int r0 = (tile / tx_n) * kTH, c0 = (tile % tx_n) * kTW;
         __syncthreads();
         stage_tile_view(a.z, view, nr, nc, r0, c0, zs, xs, ps);
@@ -603,7 +607,11 @@ __global__ void __launch_bounds__(256) k_bi_st(BiArgs a)
     view.dinv = a.dinv;
     view.al = static_cast<float>(alpha_s);
     double acc[2] = {0.0, 0.0};
-    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    // tiles in rounds of gridDim.x, each round's tiles dealt XCD by XCD in
+    // contiguous bands (shared halos stay in one L2)
+    for (int base = 0; base < nt; base += gridDim.x) {
+        const int tile = base + xcd_tile(blockIdx.x, gridDim.x);
+        if (tile >= nt) continue;
         const int r0 = (tile / tx_n) * kTH, c0 = (tile % tx_n) * kTW;
         __syncthreads();
         stage_tile_view(a.z, view, nr, nc, r0, c0, zs, xs, ss);
@@ -760,7 +768,11 @@ __global__ void __launch_bounds__(256) k_trial(HsDom d, const double* __restrict
     const int nr = d.nr, nc = d.nc, tx_n = (nc + kTW - 1) / kTW, nt = tx_n * ((nr + kTH - 1) / kTH);
     const int cl = threadIdx.x & 63;
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    // tiles in rounds of gridDim.x, each round's tiles dealt XCD by XCD in
+    // contiguous bands (shared halos stay in one L2)
+    for (int base = 0; base < nt; base += gridDim.x) {
+        const int tile = base + xcd_tile(blockIdx.x, gridDim.x);
+        if (tile >= nt) continue;
         const int r0 = (tile / tx_n) * kTH, c0 = (tile % tx_n) * kTW;
         __syncthreads();
         if (lam != 0.0) stage_tile<true, false>(z, dz, lam, nullptr, nr, nc, r0, c0, zs, nullptr);
