@@ -145,6 +145,17 @@ def test_gpu_flow_long_chain(gpu):
     assert fi.max_depth == 19999
 
 
+def test_gpu_flow_long_chain_wide_keys(gpu):
+    """a 1-D channel of 100000 cells: the Kahn keys need 17 + 17 > 32 bits,
+    so the topological order sorts 64-bit keys; the depth pointer jumping
+    needs more rounds than its first batch"""
+    print("[gpu flow chain 100000]", flush=True)
+    z = np.arange(100000, dtype=np.float32)[None, :] * 0.5
+    mask = np.ones(z.shape, np.uint8)
+    fi = compare(gpu, z, mask, 10.0, "nnno", pits=False)
+    assert fi.max_depth == 99999
+
+
 def test_gpu_flow_unresolved_raises(gpu):
     """a pit without allow_pits_as_outlets throws, naming the first cell"""
     import lsdtt3_amd._native as N
