@@ -150,6 +150,75 @@ __global__ void k_restore(double* __restrict__ z, const int* __restrict__ idx, c
     if (i < m) z[idx[i]] = bel[i] - bdz;
 }
 
+// exact k-th smallest of the float bit patterns (all >= 0): three radix
+// passes of 11, 11 and 10 bits from the top
+constexpr int kRadixBins = 2048;
+constexpr int kRadixTopShift = 21;  // the first pass's digit: bits 21..31
+
+// adds 1 to h[bin] for every lane with `want`; lanes that share a bin add
+// once through their leader (|dz| values of one magnitude crowd into few
+// bins)
+__device__ __forceinline__ void wave_hist_add(unsigned* h, unsigned bin, bool want)
+{
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(want);
+    while (pending) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const unsigned lb = __shfl(bin, leader);
+        const unsigned long long same = __ballot(want && bin == lb);
+        if (lane == leader) atomicAdd(&h[lb], static_cast<unsigned>(__popcll(same)));
+        if (bin == lb) want = false;
+        pending &= ~same;
+    }
+}
+
+__global__ void k_clear_u32(unsigned* __restrict__ a, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0u;
+}
+
+// one cell of the statistics pass below; returns the cell's |dz| as stored
+// (+inf for an invalid cell)
+__device__ __forceinline__ float stats_cell(const uint8_t* __restrict__ cflag, const double* __restrict__ zb,
+                                            const double* zsrc, double* za, long long f, bool moved, double udz,
+                                            double inv_dt, float* __restrict__ absdz, float* __restrict__ ero_rate,
+                                            double (&acc)[8])
+{
+    // loads first, then the flag tests (no wait between them)
+    const uint8_t m = cflag[f];
+    const double b = zb[f];
+    double a = zsrc[f];
+    if ((m & 2) && udz != 0.0) {
+        a += udz;
+        za[f] = a;
+    } else if (moved) {
+        za[f] = a;
+    }
+    if (!(m & 1)) {
+        const float inf = __int_as_float(0x7f800000);
+        absdz[f] = inf;
+        if (ero_rate) ero_rate[f] = 0.0f;
+        return inf;
+    }
+    acc[0] += b;
+    acc[1] += a;
+    acc[2] = fmax(acc[2], b);
+    acc[3] = fmax(acc[3], a);
+    const double l = fabs(a - b);
+    acc[4] = fmax(acc[4], l);
+    const float lf = static_cast<float>(l);
+    absdz[f] = lf;
+    const double ero = b - (a - udz);
+    if (ero_rate) ero_rate[f] = static_cast<float>(ero * inv_dt);
+    if (ero > 0.0) {
+        acc[5] += ero;
+        acc[6] = fmax(acc[6], ero);
+    }
+    acc[7] += 1.0;
+    return lf;
+}
+
 // uplift (lsdtt_lem_cpu.cpp:617-625: z += U dt on the cells with bit 1
 // of cflag, udz = 0 without uplift) fused with the step statistics
 // (:655-722 sums and maxima; |dz| as float for p99.9, invalid cells +inf,
@@ -160,49 +229,35 @@ __global__ void k_restore(double* __restrict__ z, const int* __restrict__ idx, c
 // solve left it) and every cell is written to za, the context's state
 // array, whose address therefore never changes between steps.  ero_rate
 // (optional): the per-pixel erosion rate (erosion * inv_dt as float, 0 off
-// the mask, lsdtt_lem_cpu.cpp:666-696).
+// the mask, lsdtt_lem_cpu.cpp:666-696).  Also the p99.9 selection's first
+// radix histogram (the top 11 bits of every |dz|) into hist0.
 __global__ void __launch_bounds__(256) k_stats(const uint8_t* __restrict__ cflag, const double* __restrict__ zb,
                                                const double* zsrc, double* za, long long n, double udz,
                                                double inv_dt, float* __restrict__ absdz,
-                                               float* __restrict__ ero_rate, double* partials)
+                                               float* __restrict__ ero_rate, double* partials,
+                                               unsigned* __restrict__ hist0)
 {
+    __shared__ unsigned h[kRadixBins];
+    for (int i = threadIdx.x; i < kRadixBins; i += blockDim.x) h[i] = 0;
+    __syncthreads();
     const bool moved = zsrc != za;
     double acc[8] = {0.0, 0.0, -1e30, -1e30, 0.0, 0.0, 0.0, 0.0};
-    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
-         f += (long long)gridDim.x * blockDim.x) {
-        // loads first, then the flag tests (no wait between them)
-        const uint8_t m = cflag[f];
-        const double b = zb[f];
-        double a = zsrc[f];
-        if ((m & 2) && udz != 0.0) {
-            a += udz;
-            za[f] = a;
-        } else if (moved) {
-            za[f] = a;
-        }
-        if (!(m & 1)) {
-            absdz[f] = __int_as_float(0x7f800000);
-            if (ero_rate) ero_rate[f] = 0.0f;
-            continue;
-        }
-        acc[0] += b;
-        acc[1] += a;
-        acc[2] = fmax(acc[2], b);
-        acc[3] = fmax(acc[3], a);
-        const double l = fabs(a - b);
-        acc[4] = fmax(acc[4], l);
-        absdz[f] = static_cast<float>(l);
-        const double ero = b - (a - udz);
-        if (ero_rate) ero_rate[f] = static_cast<float>(ero * inv_dt);
-        if (ero > 0.0) {
-            acc[5] += ero;
-            acc[6] = fmax(acc[6], ero);
-        }
-        acc[7] += 1.0;
+    // wavefront-uniform trip count (the histogram is added per wavefront)
+    for (long long base = blockIdx.x * (long long)blockDim.x + (threadIdx.x & ~63); base < n;
+         base += (long long)gridDim.x * blockDim.x) {
+        const long long f = base + (threadIdx.x & 63);
+        float adz = 0.0f;
+        const bool in = f < n;
+        if (in) adz = stats_cell(cflag, zb, zsrc, za, f, moved, udz, inv_dt, absdz, ero_rate, acc);
+        wave_hist_add(h, __float_as_uint(adz) >> kRadixTopShift, in);
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kRadixBins; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist0[i], h[i]);
     const bool mx[8] = {false, false, true, true, true, false, true, false};
     block_partial<8>(acc, mx, partials, blockIdx.x);
 }
+
 
 __global__ void __launch_bounds__(256) k_stats_fin(const double* partials, unsigned count, double* out)
 {
@@ -213,10 +268,6 @@ __global__ void __launch_bounds__(256) k_stats_fin(const double* partials, unsig
         for (int i = 0; i < 8; ++i) out[i] = res[i];
 }
 
-
-// exact k-th smallest of the float bit patterns (all >= 0): three radix
-// passes of 11, 11 and 10 bits from the top
-constexpr int kRadixBins = 2048;
 
 __global__ void k_radix_hist(const float* __restrict__ v, long long n, const unsigned* __restrict__ sel, int shift,
                              int bits, unsigned* __restrict__ hist)
@@ -238,15 +289,7 @@ __global__ void k_radix_hist(const float* __restrict__ v, long long n, const uns
             want = (u & pmask) == prefix;
             bin = (u >> shift) & ((1u << bits) - 1u);  // this pass's digit only
         }
-        unsigned long long pending = __ballot(want);
-        while (pending) {
-            const int leader = __ffsll(static_cast<long long>(pending)) - 1;
-            const unsigned lb = __shfl(bin, leader);
-            const unsigned long long same = __ballot(want && bin == lb);
-            if (lane == leader) atomicAdd(&h[lb], static_cast<unsigned>(__popcll(same)));
-            if (bin == lb) want = false;
-            pending &= ~same;
-        }
+        wave_hist_add(h, bin, want);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kRadixBins; i += blockDim.x)
@@ -846,17 +889,22 @@ int LemEngine::hillslope(LemStats* st, hipStream_t s)
 
 // the p99.9 selection of the k-th smallest |dz| (k from the valid count),
 // enqueued with its read-back into sel
-void LemEngine::enqueue_p999(long long nv, unsigned* sel, hipStream_t s)
+void LemEngine::enqueue_p999(long long nv, unsigned* sel, hipStream_t s, bool first_pass_done)
 {
     sel[0] = 0u;
     sel[1] = 0u;
     sel[2] = static_cast<unsigned>(std::floor(0.999 * static_cast<double>(nv - 1)));
     unsigned* dsel = hist.p + kRadixBins;
-    LSDTT_HIP(hipMemsetAsync(hist.p, 0, sizeof(unsigned) * kRadixBins, s));
+    if (!first_pass_done) {
+        k_clear_u32<<<div_up(kRadixBins, kB), kB, 0, s>>>(hist.p, kRadixBins);
+        LSDTT_LAUNCH_CHECK();
+    }
     LSDTT_HIP(hipMemcpyAsync(dsel, sel, sizeof(unsigned) * 3, hipMemcpyHostToDevice, s));
-    const int shifts[3] = {21, 10, 0}, bits[3] = {11, 11, 10};
+    const int shifts[3] = {kRadixTopShift, 10, 0}, bits[3] = {11, 11, 10};
     for (int pass = 0; pass < 3; ++pass) {
-        k_radix_hist<<<grid_for(n, 2048), kB, 0, s>>>(absdz.p, n, dsel, shifts[pass], bits[pass], hist.p);
+        // the first histogram may come from the statistics pass
+        if (pass > 0 || !first_pass_done)
+            k_radix_hist<<<grid_for(n, 2048), kB, 0, s>>>(absdz.p, n, dsel, shifts[pass], bits[pass], hist.p);
         k_radix_pick<<<1, kPickThreads, 0, s>>>(hist.p, dsel, shifts[pass], bits[pass]);
         LSDTT_LAUNCH_CHECK();
     }
@@ -870,8 +918,9 @@ void LemEngine::stats(LemStats* st, const double* zres, hipStream_t s)
     if (p.erosion_rate_field) ero_rate.reserve(n);
     const double udz = p.enable_uplift ? p.U * p.dt : 0.0;
     timer.begin("step_stats", s);
+    k_clear_u32<<<div_up(kRadixBins, kB), kB, 0, s>>>(hist.p, kRadixBins);
     k_stats<<<hs_grid, kB, 0, s>>>(cflag.p, z_before.p, zres, z.p, n, udz, 1.0 / p.dt, absdz.p,
-                                   p.erosion_rate_field ? ero_rate.p : nullptr, partials.p);
+                                   p.erosion_rate_field ? ero_rate.p : nullptr, partials.p, hist.p);
     k_stats_fin<<<1, kB, 0, s>>>(partials.p, (unsigned)hs_grid, scal.p + kStatSlot);
     timer.end(s, 21.0 * n);
     LSDTT_LAUNCH_CHECK();
@@ -880,11 +929,11 @@ void LemEngine::stats(LemStats* st, const double* zres, hipStream_t s)
     // the valid cells are the mask's, fixed for the context: the selection
     // is enqueued behind the sums and both come back with one wait
     unsigned* sel = reinterpret_cast<unsigned*>(hpin.p + 24);
-    if (n_mask > 0) enqueue_p999(n_mask, sel, s);
+    if (n_mask > 0) enqueue_p999(n_mask, sel, s, true);
     LSDTT_HIP(hipStreamSynchronize(s));
     const long long nv = static_cast<long long>(h[7]);
     if (nv != n_mask && nv > 0) {  // not expected: the statistics count the mask's cells
-        enqueue_p999(nv, sel, s);
+        enqueue_p999(nv, sel, s, false);
         LSDTT_HIP(hipStreamSynchronize(s));
     }
     const double inv_dt = 1.0 / p.dt, inv_n = nv > 0 ? 1.0 / static_cast<double>(nv) : 0.0;

@@ -211,7 +211,9 @@ public:
 
 private:
     void stats(LemStats* st, const double* zres, hipStream_t s);
-    void enqueue_p999(long long nv, unsigned* sel, hipStream_t s);
+    // first_pass_done: the first radix histogram is already in hist (the
+    // statistics pass builds it)
+    void enqueue_p999(long long nv, unsigned* sel, hipStream_t s, bool first_pass_done);
 };
 
 }  // namespace lsdtt_gpu
