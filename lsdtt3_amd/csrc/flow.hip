@@ -343,6 +343,68 @@ __global__ void k_pj_init(const int* __restrict__ rcv, const uint8_t* __restrict
     }
 }
 
+// Unit weights (depth): the packed words start already followed inside
+// 32 x 32 tiles.  In LDS every cell holds (hops << 16 | local cell) and
+// points at its receiver while that lies in the tile; pointer doubling (10 rounds cover any in-tile chain of the 1024 cells)
+// takes it to the last cell L of its path in the tile, and the word becomes
+// (hops to L, L) when L is a root, else (hops to L + 1, L's receiver).  A
+// word read while its owner updates it is still a valid (hops, cell) pair
+// of the same path.
+constexpr int kPjT = 32;
+__global__ void __launch_bounds__(kPjT * 8) k_pj_init_tiles(const int* __restrict__ rcv,
+                                                            const uint8_t* __restrict__ mask, int nr, int nc,
+                                                            int ntx, int tiles, unsigned long long* __restrict__ P)
+{
+    __shared__ unsigned st[kPjT * kPjT];
+    __shared__ int rg[kPjT * kPjT];  // receiver (global; negative for roots and masked cells)
+    const int tile = xcd_tile(blockIdx.x, tiles);
+    const int c0 = (tile % ntx) * kPjT, r0 = (tile / ntx) * kPjT;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int ty = threadIdx.y + 8 * j, tx = threadIdx.x, t = ty * kPjT + tx;
+        const int r = r0 + ty, c = c0 + tx;
+        int g = -1;
+        unsigned w = static_cast<unsigned>(t);
+        if (r < nr && c < nc) {
+            const int f = r * nc + c;
+            g = mask[f] ? rcv[f] : -1;
+            if (g >= 0) {
+                const int gr = g / nc, gc = g - gr * nc;
+                if (gr >= r0 && gr < r0 + kPjT && gc >= c0 && gc < c0 + kPjT)
+                    w = (1u << 16) | static_cast<unsigned>((gr - r0) * kPjT + (gc - c0));
+            }
+        }
+        rg[t] = g;
+        st[t] = w;
+    }
+    __syncthreads();
+    for (int it = 0; it < 10; ++it) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int t = (threadIdx.y + 8 * j) * kPjT + threadIdx.x;
+            const unsigned w = st[t];
+            const int p = static_cast<int>(w & 0xffffu);
+            if (p != t) {
+                const unsigned wp = st[p];
+                st[t] = (((w >> 16) + (wp >> 16)) << 16) | (wp & 0xffffu);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int ty = threadIdx.y + 8 * j, tx = threadIdx.x, t = ty * kPjT + tx;
+        const int r = r0 + ty, c = c0 + tx;
+        if (r >= nr || c >= nc) continue;
+        const unsigned w = st[t];
+        const int L = static_cast<int>(w & 0xffffu);
+        const unsigned hops = w >> 16;
+        const int gL = rg[L];
+        const int fL = (r0 + L / kPjT) * nc + (c0 + L % kPjT);
+        P[r * nc + c] = gL >= 0 ? pack(hops + 1u, static_cast<unsigned>(gL)) : pack(hops, static_cast<unsigned>(fL));
+    }
+}
+
 __global__ void k_pj_round(unsigned long long* __restrict__ P, long long n, const int* __restrict__ prev,
                            int* __restrict__ cur)
 {
@@ -461,32 +523,53 @@ __device__ __forceinline__ int wave_agg_add(int* h, int d, bool active)
 
 constexpr int kCellsPerThread = 4;
 
-__global__ void __launch_bounds__(1024) k_level_hist(const int* __restrict__ depth, long long n,
+// Level counters are split by block group for the shallow levels (the
+// first `lg`): block b counts into slot d * kLevelGroups + b % kLevelGroups,
+// so the few crowded levels do not serialise every block's atomics on one
+// address; deeper levels keep one slot each after them.  In slot order the
+// counters are level-major, so their exclusive scan gives every group its
+// range inside its level's.
+constexpr int kLevelGroups = 64;
+__device__ __forceinline__ int level_slot(int d, int g, int lg)
+{
+    return d < lg ? d * kLevelGroups + g : lg * kLevelGroups + (d - lg);
+}
+
+__global__ void __launch_bounds__(1024) k_level_hist(const int* __restrict__ depth, long long n, int lg,
                                                      int* __restrict__ count)
 {
     __shared__ int h[kLdsLevels];
     for (int i = threadIdx.x; i < kLdsLevels; i += blockDim.x) h[i] = 0;
     __syncthreads();
+    const int g = blockIdx.x % kLevelGroups;
     const long long base = blockIdx.x * (long long)blockDim.x * kCellsPerThread;
     for (int j = 0; j < kCellsPerThread; ++j) {
         const long long f = base + j * (long long)blockDim.x + threadIdx.x;
         const int d = f < n ? depth[f] : -1;
         wave_agg_add(h, d < kLdsLevels ? d : 0, d >= 0 && d < kLdsLevels);
-        if (d >= kLdsLevels) atomicAdd(&count[d], 1);
+        if (d >= kLdsLevels) atomicAdd(&count[level_slot(d, g, lg)], 1);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kLdsLevels; i += blockDim.x)
-        if (h[i]) atomicAdd(&count[i], h[i]);
+        if (h[i]) atomicAdd(&count[level_slot(i, g, lg)], h[i]);
 }
 
-__global__ void __launch_bounds__(1024) k_level_scatter(const int* __restrict__ depth, long long n,
-                                                        const int* __restrict__ start, int* __restrict__ cursor,
+// the start of every level (and the total after the last)
+__global__ void k_level_starts(const int* __restrict__ sub, int levels, int lg, int* __restrict__ start)
+{
+    const int L = blockIdx.x * blockDim.x + threadIdx.x;
+    if (L < levels) start[L] = sub[level_slot(L, 0, lg)];
+}
+
+__global__ void __launch_bounds__(1024) k_level_scatter(const int* __restrict__ depth, long long n, int lg,
+                                                        const int* __restrict__ sub, int* __restrict__ cursor,
                                                         int* __restrict__ order)
 {
     __shared__ int h[kLdsLevels];
     __shared__ int base_s[kLdsLevels];
     for (int i = threadIdx.x; i < kLdsLevels; i += blockDim.x) h[i] = 0;
     __syncthreads();
+    const int g = blockIdx.x % kLevelGroups;
     const long long base = blockIdx.x * (long long)blockDim.x * kCellsPerThread;
     int d[kCellsPerThread], rank[kCellsPerThread];
     for (int j = 0; j < kCellsPerThread; ++j) {
@@ -496,15 +579,16 @@ __global__ void __launch_bounds__(1024) k_level_scatter(const int* __restrict__ 
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kLdsLevels; i += blockDim.x)
-        if (h[i]) base_s[i] = atomicAdd(&cursor[i], h[i]);
+        if (h[i]) base_s[i] = atomicAdd(&cursor[level_slot(i, g, lg)], h[i]);
     __syncthreads();
     for (int j = 0; j < kCellsPerThread; ++j) {
         const int dj = d[j];
         if (dj < 0) continue;
         const long long f = base + j * (long long)blockDim.x + threadIdx.x;
+        const int slot = level_slot(dj, g, lg);
         int pos;
-        if (dj < kLdsLevels) pos = start[dj] + base_s[dj] + rank[j];
-        else pos = start[dj] + atomicAdd(&cursor[dj], 1);
+        if (dj < kLdsLevels) pos = sub[slot] + base_s[dj] + rank[j];
+        else pos = sub[slot] + atomicAdd(&cursor[slot], 1);
         order[pos] = static_cast<int>(f);
     }
 }
@@ -941,26 +1025,32 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
         return 0;
     }
     const int nl = max_depth + 1;
-    level_count.reserve(nl + 1);
+    // level counters: shallow levels split by block group (see
+    // level_slot), then one slot per deeper level
+    const int lg = std::min(nl + 1, kLdsLevels);
+    const long long slots = (long long)lg * kLevelGroups + (nl + 1 - lg);
+    level_count.reserve(slots);
+    level_sub.reserve(slots);
     level_start.reserve(nl + 1);
-    cursor.reserve(nl + 1);
+    cursor.reserve(slots);
     order.reserve(n_valid);
-    LSDTT_HIP(hipMemsetAsync(level_count.p, 0, sizeof(int) * (nl + 1), s));
-    LSDTT_HIP(hipMemsetAsync(cursor.p, 0, sizeof(int) * (nl + 1), s));
+    LSDTT_HIP(hipMemsetAsync(level_count.p, 0, sizeof(int) * slots, s));
+    LSDTT_HIP(hipMemsetAsync(cursor.p, 0, sizeof(int) * slots, s));
     const int g1k = div_up(n, 1024 * kCellsPerThread);
     {
         TScope ts_(timer, "level_hist", s, 4.0 * n);
-        k_level_hist<<<g1k, 1024, 0, s>>>(depth, n, level_count);
+        k_level_hist<<<g1k, 1024, 0, s>>>(depth, n, lg, level_count);
     }
     LSDTT_LAUNCH_CHECK();
     size_t tb = 0;
-    LSDTT_HIP(rocprim::exclusive_scan(nullptr, tb, level_count.p, level_start.p, 0, (size_t)(nl + 1),
+    LSDTT_HIP(rocprim::exclusive_scan(nullptr, tb, level_count.p, level_sub.p, 0, (size_t)slots,
                                       rocprim::plus<int>(), s));
-    LSDTT_HIP(rocprim::exclusive_scan(tmp.need(tb), tb, level_count.p, level_start.p, 0, (size_t)(nl + 1),
+    LSDTT_HIP(rocprim::exclusive_scan(tmp.need(tb), tb, level_count.p, level_sub.p, 0, (size_t)slots,
                                       rocprim::plus<int>(), s));
+    k_level_starts<<<div_up(nl + 1, kBlock), kBlock, 0, s>>>(level_sub, nl + 1, lg, level_start);
     {
         TScope ts_(timer, "level_scatter", s, 8.0 * n);
-        k_level_scatter<<<g1k, 1024, 0, s>>>(depth, n, level_start, cursor, order);
+        k_level_scatter<<<g1k, 1024, 0, s>>>(depth, n, lg, level_sub, cursor, order);
     }
     LSDTT_LAUNCH_CHECK();
     LSDTT_HIP(hipMemcpyAsync(level_start_h.data(), level_start.p, sizeof(int) * (nl + 1), hipMemcpyDeviceToHost, s));
@@ -992,7 +1082,12 @@ void FlowEngine::pointer_jump(const int* weight, int* value_out, int* root_out, 
                               const ReadInts& extra)
 {
     const int g1 = div_up(n, kBlock), gs = std::min(g1, 2048);
-    k_pj_init<<<g1, kBlock, 0, s>>>(rcv, mask, weight, n, pj);
+    if (weight) {
+        k_pj_init<<<g1, kBlock, 0, s>>>(rcv, mask, weight, n, pj);
+    } else {
+        const int ntx = div_up(opt.nc, kPjT), tiles = ntx * div_up(opt.nr, kPjT);
+        k_pj_init_tiles<<<tiles, dim3(kPjT, 8), 0, s>>>(rcv, mask, opt.nr, opt.nc, ntx, tiles, pj);
+    }
     LSDTT_LAUNCH_CHECK();
     // chg[r] == 1 while round r-1 changed something; a round after
     // convergence returns at once

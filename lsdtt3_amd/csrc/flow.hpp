@@ -83,7 +83,7 @@ public:
     int n_sources = 0;  // valid cells without donors
     std::vector<int> level_start_h;  // host copy, max_depth + 2 entries
 
-    dbuf<int> rcv, outlets, depth, order, level_count, level_start, cursor;
+    dbuf<int> rcv, outlets, depth, order, level_count, level_sub, level_start, cursor;
     dbuf<int> cnt, topo, topo_inner, start, svec, off;
     dbuf<uint8_t> dir, donors, dmask;  // dmask: donor bits (distinct_neighbours order)
     dbuf<unsigned long long> pj, key, sort_keys_a, sort_keys_b;
