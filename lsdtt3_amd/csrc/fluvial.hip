@@ -800,8 +800,9 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_order(const int* __restrict__ 
     __shared__ int hist[TN + 1];
     __shared__ int cur[TN];
     __shared__ short lrec[2][TN];
-    const int tx = blockIdx.x, ty = blockIdx.y, lane = threadIdx.x;
-    const int tile = ty * gridDim.x + tx;
+    const TileAt T = grid_tile();
+    const int tx = T.tx, ty = T.ty, lane = threadIdx.x;
+    const int tile = T.tile;
     const int r0 = ty * TH, c0 = tx * TW;
     bool valid[PER];
     {
@@ -935,8 +936,9 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_accum(DinfTileArgs A, int nr, 
 {
     constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
     if (P.prev_marked && *P.prev_marked == 0) return;
-    const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x;
-    const int tile = ty * ntx + tx;
+    const TileAt T = grid_tile();
+    const int tx = T.tx, ty = T.ty, ntx = T.ntx;
+    const int tile = T.tile;
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
     if (P.prev_marked && !P.act[tile]) return;  // the first pass runs every tile
@@ -1135,8 +1137,9 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_up_order(const int* __restrict
     __shared__ int cur[TN];
     __shared__ short lrec[2][TN];
     __shared__ uint8_t xd[TN];
-    const int tx = blockIdx.x, ty = blockIdx.y, lane = threadIdx.x;
-    const int tile = ty * gridDim.x + tx;
+    const TileAt T = grid_tile();
+    const int tx = T.tx, ty = T.ty, lane = threadIdx.x;
+    const int tile = T.tile;
     const int r0 = ty * TH, c0 = tx * TW;
     bool valid[PER];
     {
@@ -1295,8 +1298,9 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_sp(DinfSpArgs A, int nr, int n
 {
     constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
     if (P.prev_marked && *P.prev_marked == 0) return;
-    const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x;
-    const int tile = ty * ntx + tx;
+    const TileAt T = grid_tile();
+    const int tx = T.tx, ty = T.ty, ntx = T.ntx;
+    const int tile = T.tile;
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
     if (P.prev_marked && !P.act[tile]) return;  // the first pass runs every tile
@@ -1522,8 +1526,9 @@ __global__ void __launch_bounds__(WL) k_dinf_tile_chi(DinfChiArgs A, int nr, int
 {
     constexpr int TN = TileShape<TH>::TN, PER = TileShape<TH>::PER;
     if (P.prev_marked && *P.prev_marked == 0) return;
-    const int tx = blockIdx.x, ty = blockIdx.y, ntx = gridDim.x;
-    const int tile = ty * ntx + tx;
+    const TileAt T = grid_tile();
+    const int tx = T.tx, ty = T.ty, ntx = T.ntx;
+    const int tile = T.tile;
     const int lane = threadIdx.x;
     if (lane == 0) P.act_clear[tile] = 0;
     if (P.prev_marked && !P.act[tile]) return;  // the first pass runs every tile
