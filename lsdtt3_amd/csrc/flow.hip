@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(kDonW * kDonH) k_donors(const int* __restrict_
 
 // ---------------------------------------------------------------------------
 // pointer jumping on packed words (acc << 32 | next).  Invariant kept by
-// every update: acc(v) = sum of weights on the path from v up to, but not
+// every update: acc(v) = number of hops on the path from v up to, but not
 // including, next(v).  Roots (outlets, masked cells) point at themselves.
 // 64-bit loads/stores are single accesses, so updates in place are safe.
 // ---------------------------------------------------------------------------
@@ -329,23 +329,10 @@ __device__ __forceinline__ unsigned long long pack(unsigned acc, unsigned nxt)
     return (static_cast<unsigned long long>(acc) << 32) | nxt;
 }
 
-__global__ void k_pj_init(const int* __restrict__ rcv, const uint8_t* __restrict__ mask,
-                          const int* __restrict__ weight, long long n, unsigned long long* __restrict__ P)
-{
-    const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (f >= n) return;
-    const int r = rcv[f];
-    if (!mask[f] || r < 0) {
-        // root: acc holds the root's own value (0 for depth, start for s-vector)
-        P[f] = pack(weight ? static_cast<unsigned>(weight[f]) : 0u, static_cast<unsigned>(f));
-    } else {
-        P[f] = pack(weight ? static_cast<unsigned>(weight[f]) : 1u, static_cast<unsigned>(r));
-    }
-}
-
-// Unit weights (depth): the packed words start already followed inside
+// Depth (unit weights): the packed words start already followed inside
 // 32 x 32 tiles.  In LDS every cell holds (hops << 16 | local cell) and
-// points at its receiver while that lies in the tile; pointer doubling (10 rounds cover any in-tile chain of the 1024 cells)
+// points at its receiver while that lies in the tile; pointer doubling
+// (10 rounds cover any in-tile chain of the 1024 cells)
 // takes it to the last cell L of its path in the tile, and the word becomes
 // (hops to L, L) when L is a root, else (hops to L + 1, L's receiver).  A
 // word read while its owner updates it is still a valid (hops, cell) pair
@@ -439,7 +426,7 @@ __global__ void k_pj_round(unsigned long long* __restrict__ P, long long n, cons
 // without donors (sources)
 __global__ void k_pj_extract(const unsigned long long* __restrict__ P, const uint8_t* __restrict__ mask,
                              const uint8_t* __restrict__ donors, long long n, int* __restrict__ value,
-                             int* __restrict__ root, int* __restrict__ stats, const int* __restrict__ gate)
+                             int* __restrict__ stats, const int* __restrict__ gate)
 {
     if (gate && *gate != 0) return;
     int v = -1, valid = 0, sources = 0;
@@ -454,9 +441,6 @@ __global__ void k_pj_extract(const unsigned long long* __restrict__ P, const uin
             vf = static_cast<int>(acc);
             ++valid;
             if (stats) sources += donors[f] == 0;
-            if (root) root[f] = static_cast<int>(nx);
-        } else if (root) {
-            root[f] = -1;
         }
         value[f] = vf;
         v = max(v, vf);
@@ -1011,7 +995,7 @@ long long FlowEngine::route(const T* z, const uint8_t* msk, const uint8_t* force
     extra.src[3] = part_counts.p + 2 * kPartChunks;
     extra.src[4] = misc.p + 3;
     extra.count = 5;
-    pointer_jump(nullptr, depth, nullptr, s, extra);
+    pointer_jump(depth, s, extra);
     const int* h = hread.p + kReadSlot;
     n_outlets = h[3];
     if (h[0] != 0x7fffffff) return -(static_cast<long long>(h[0]) + 1);
@@ -1078,13 +1062,10 @@ template void FlowEngine::receivers_only<float>(const float*, const uint8_t*, hi
 template long long FlowEngine::route<float>(const float*, const uint8_t*, const uint8_t*, hipStream_t);
 template long long FlowEngine::route<double>(const double*, const uint8_t*, const uint8_t*, hipStream_t);
 
-void FlowEngine::pointer_jump(const int* weight, int* value_out, int* root_out, hipStream_t s,
-                              const ReadInts& extra)
+void FlowEngine::pointer_jump(int* value_out, hipStream_t s, const ReadInts& extra)
 {
     const int g1 = div_up(n, kBlock), gs = std::min(g1, 2048);
-    if (weight) {
-        k_pj_init<<<g1, kBlock, 0, s>>>(rcv, mask, weight, n, pj);
-    } else {
+    {
         const int ntx = div_up(opt.nc, kPjT), tiles = ntx * div_up(opt.nr, kPjT);
         k_pj_init_tiles<<<tiles, dim3(kPjT, 8), 0, s>>>(rcv, mask, opt.nr, opt.nc, ntx, tiles, pj);
     }
@@ -1094,7 +1075,7 @@ void FlowEngine::pointer_jump(const int* weight, int* value_out, int* root_out, 
     int* chg = pjflags.p;
     k_init_ints<<<1, 64, 0, s>>>(chg, kPjRounds + 1, 1, 0, 0, 0);
     LSDTT_LAUNCH_CHECK();
-    int* stats = weight ? nullptr : misc.p + 1;
+    int* stats = misc.p + 1;
     auto rounds = [&](int r0, int r1) {
         for (int r = r0; r < r1; ++r) {
             {
@@ -1118,13 +1099,13 @@ void FlowEngine::pointer_jump(const int* weight, int* value_out, int* root_out, 
     // the remaining rounds and the extraction follow the read-back
     const int first = std::min(kPjRounds, pj_hint);
     rounds(0, first);
-    k_pj_extract<<<gs, kBlock, 0, s>>>(pj, mask, donors, n, value_out, root_out, stats, chg + first);
+    k_pj_extract<<<gs, kBlock, 0, s>>>(pj, mask, donors, n, value_out, stats, chg + first);
     LSDTT_LAUNCH_CHECK();
     read_back();
     if (hread.p[first] != 0) {
         if (first == kPjRounds) throw std::runtime_error("pointer jumping did not converge");
         rounds(first, kPjRounds);
-        k_pj_extract<<<gs, kBlock, 0, s>>>(pj, mask, donors, n, value_out, root_out, stats, nullptr);
+        k_pj_extract<<<gs, kBlock, 0, s>>>(pj, mask, donors, n, value_out, stats, nullptr);
         LSDTT_LAUNCH_CHECK();
         read_back();
         if (hread.p[kPjRounds] != 0) throw std::runtime_error("pointer jumping did not converge");

@@ -107,8 +107,9 @@ private:
     // of the next one)
     int pj_hint = 12;
     bool off_ready = false;  // off holds the sibling offsets of the current routing
-    void pointer_jump(const int* weight_or_null, int* value_out, int* root_out, hipStream_t s,
-                      const ReadInts& extra = ReadInts{});
+    // depth from the outlet by pointer jumping, into value_out; the extra
+    // ints come back with the round flags in one read-back
+    void pointer_jump(int* value_out, hipStream_t s, const ReadInts& extra = ReadInts{});
 };
 
 }  // namespace lsdtt_gpu
