@@ -88,6 +88,32 @@ def test_gpu_fill_plateaus_and_nested_pits(gpu, bc):
     assert same(gpu.fill_pits(z, mask, -9999.0, bc), orc.fill(z, mask, -9999.0, bc))
 
 
+@pytest.mark.parametrize("bc", ["oooo", "popo"])
+def test_gpu_fill_white_noise(gpu, bc):
+    """uniform noise: about one cell in nine is a pit, the most basins and
+    merge rounds per cell of any surface"""
+    print(f"[gpu fill] white noise {bc}", flush=True)
+    z = np.random.default_rng(23).random((511, 517)).astype(np.float32)
+    mask = np.ones(z.shape, np.uint8)
+    assert same(gpu.fill_pits(z, mask, -9999.0, bc), orc.fill(z, mask, -9999.0, bc))
+
+
+def test_gpu_fill_pit_staircase(gpu):
+    """10000 pits in a row, each spilling over a lower rim into the next:
+    the basins merge in one chain 10000 deep"""
+    print("[gpu fill] pit staircase", flush=True)
+    nc = 20001
+    z = np.full((3, nc), 100.0, np.float32)
+    col = np.arange(1, nc - 1)
+    z[1, 1:nc - 1] = np.where(col % 2 == 0, -1.0 - 1e-3 * (col % 7), 50.0 - 2e-3 * col).astype(np.float32)
+    z[1, nc - 1] = 0.0
+    mask = np.ones(z.shape, np.uint8)
+    want = orc.fill(z, mask)
+    # every pit fills to the lower of its two rims, the one downstream
+    assert np.array_equal(want[1, 2:nc - 1:2], z[1, 3:nc:2][:want[1, 2:nc - 1:2].size])
+    assert same(gpu.fill_pits(z, mask), want)
+
+
 def test_gpu_fill_1024(gpu):
     print("[gpu fill] 1024^2", flush=True)
     from lsdtt3_amd.synthetic import routing_surface
